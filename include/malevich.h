@@ -1,0 +1,185 @@
+/*
+ * malevich.h -- C ABI of the MI355X (gfx950) draw pipeline.
+ *
+ * Drop-in for the D3D11-style raster path of Malevich. The reference keeps
+ * one global `Pipeline graphics_pipeline` (source/main.c:109-115, global at
+ * main.c:222) that the caller writes field by field (main.c:1276-1294), and
+ * three commands: clear_render_target_view (main.c:1191),
+ * clear_depth_stencil_view (main.c:1204) and draw_indexed (main.c:1219).
+ * Here the same state lives in an mlv_context; each state field has a
+ * setter, each command an entry point. Shaders are chosen by enum because
+ * the reference's shader function pointers (csc.h:10-18) cannot cross to
+ * the device; the enum set is the reference's compiled-in shader set
+ * (main.c:46-52).
+ *
+ * Conventions: every call returns an mlv_status (0 = ok) and never aborts.
+ * Commands are stream-ordered on the context's HIP stream; a readback or
+ * mlv_synchronize waits for them. One host thread per context.
+ */
+#ifndef MALEVICH_H
+#define MALEVICH_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#if defined(__GNUC__)
+#define MLV_API __attribute__((visibility("default")))
+#else
+#define MLV_API
+#endif
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct mlv_context mlv_context;
+typedef struct mlv_buffer mlv_buffer;
+typedef struct mlv_texture2d mlv_texture2d;
+typedef struct mlv_render_target mlv_render_target;
+typedef struct mlv_depth_stencil mlv_depth_stencil;
+
+typedef enum mlv_status {
+    MLV_OK = 0,
+    MLV_ERROR_INVALID_ARGUMENT = -1,
+    MLV_ERROR_OUT_OF_MEMORY = -2,
+    MLV_ERROR_DEVICE = -3,
+    MLV_ERROR_INVALID_STATE = -4,
+    MLV_ERROR_CAPACITY = -5,
+    MLV_ERROR_NO_DEVICE = -6
+} mlv_status;
+
+typedef enum mlv_buffer_kind {
+    MLV_BUFFER_VERTEX = 1,
+    MLV_BUFFER_INDEX_U32 = 2, /* IA.p_index_buffer is u32 (main.c:72) */
+    MLV_BUFFER_CONSTANT = 3
+} mlv_buffer_kind;
+
+typedef enum mlv_format {
+    MLV_FORMAT_RGBA8_UNORM = 1, /* Texture2D u32 texels, R in the low byte (csc.h:30-36, math.h:336-344) */
+    MLV_FORMAT_RGBA32F = 2,     /* Texture2D float4 texels (csc.h:42-53) */
+    MLV_FORMAT_RGBA16F = 3      /* extension: half4 texels, widened to float on fetch */
+} mlv_format;
+
+typedef enum mlv_topology {
+    MLV_TOPOLOGY_UNDEFINED = 0,    /* main.c:67 */
+    MLV_TOPOLOGY_TRIANGLELIST = 1  /* main.c:68 */
+} mlv_topology;
+
+/* The reference's vertex shaders (main.c:46-52): basic_vs.c, vertex_lighting_vs.c,
+ * passthrough_vs.c, fullscreen_vs.c. */
+typedef enum mlv_vertex_shader {
+    MLV_VS_BASIC = 1,
+    MLV_VS_VERTEX_LIGHTING = 2,
+    MLV_VS_PASSTHROUGH = 3,
+    MLV_VS_FULLSCREEN = 4
+} mlv_vertex_shader;
+
+/* The reference's pixel shaders: basic_ps.c, env_lighting_ps.c, passthrough_ps.c. */
+typedef enum mlv_pixel_shader {
+    MLV_PS_BASIC = 1,
+    MLV_PS_ENV_LIGHTING = 2,
+    MLV_PS_PASSTHROUGH = 3
+} mlv_pixel_shader;
+
+typedef enum mlv_depth_func {
+    MLV_DEPTH_GREATER_EQUAL = 0, /* reference: reversed Z, _CMP_GE_OQ (main.c:1166) */
+    MLV_DEPTH_LESS_EQUAL = 1     /* extension: mirrored test and hierarchical-Z */
+} mlv_depth_func;
+
+/* Viewport, field for field (main.c:85-92). */
+typedef struct mlv_viewport {
+    float top_left_x;
+    float top_left_y;
+    float width;
+    float height;
+    float min_depth;
+    float max_depth;
+} mlv_viewport;
+
+/* Stats (main.c:199-206), accumulated per draw like main.c:1228-1246. */
+typedef struct mlv_stats {
+    uint64_t vertex_count;
+    uint64_t input_triangle_count;
+    uint64_t assembled_triangle_count;
+    uint64_t active_bin_count;             /* non-empty 8x8 bins, summed over draws */
+    uint64_t total_triangle_count_in_bins; /* (triangle, 8x8 bin) pairs, summed over draws */
+    uint64_t oob_triangle_count;           /* triangles dropped for an index past the vertex buffer */
+    uint64_t draw_count;
+    uint64_t overflow_count;               /* draws that ran the slow path after a bin-list overflow */
+} mlv_stats;
+
+/* ---- lifecycle (no reference counterpart: the reference state is a global, main.c:222) */
+MLV_API int mlv_context_create(int device, mlv_context **out);
+MLV_API int mlv_context_destroy(mlv_context *ctx);
+MLV_API const char *mlv_get_last_error(const mlv_context *ctx);
+MLV_API int mlv_synchronize(mlv_context *ctx);
+MLV_API void *mlv_context_stream(mlv_context *ctx); /* the hipStream_t commands are queued on */
+MLV_API int mlv_device_count(int *out);
+
+/* ---- resources. Replaces the caller-owned arrays the reference binds by pointer:
+ * Mesh VB/IB (main.c:60-64), PerFrameCB (main.c:169-173), Texture2D (csc.h:20-24),
+ * frame_buffer / depth_buffer (main.c:35-36). */
+MLV_API int mlv_buffer_create(mlv_context *ctx, int kind, size_t bytes, const void *host_init, mlv_buffer **out);
+MLV_API int mlv_buffer_wrap(mlv_context *ctx, int kind, void *device_ptr, size_t bytes, mlv_buffer **out);
+MLV_API int mlv_buffer_update(mlv_context *ctx, mlv_buffer *buf, size_t offset, size_t bytes, const void *host_src);
+MLV_API int mlv_buffer_readback(mlv_context *ctx, const mlv_buffer *buf, size_t offset, size_t bytes, void *host_dst);
+MLV_API int mlv_buffer_destroy(mlv_context *ctx, mlv_buffer *buf);
+MLV_API void *mlv_buffer_device_ptr(const mlv_buffer *buf);
+
+MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t height, int format, uint32_t mip_levels,
+                                 const void *host_data, mlv_texture2d **out);
+MLV_API int mlv_texture2d_destroy(mlv_context *ctx, mlv_texture2d *tex);
+MLV_API void *mlv_texture2d_device_ptr(const mlv_texture2d *tex);
+
+MLV_API int mlv_render_target_create(mlv_context *ctx, uint32_t width, uint32_t height, mlv_render_target **out);
+MLV_API int mlv_render_target_destroy(mlv_context *ctx, mlv_render_target *rt);
+MLV_API void *mlv_render_target_device_ptr(const mlv_render_target *rt);
+MLV_API int mlv_depth_stencil_create(mlv_context *ctx, uint32_t width, uint32_t height, mlv_depth_stencil **out);
+MLV_API int mlv_depth_stencil_destroy(mlv_context *ctx, mlv_depth_stencil *ds);
+MLV_API void *mlv_depth_stencil_device_ptr(const mlv_depth_stencil *ds);
+
+/* ---- IA (main.c:71-76) */
+MLV_API int mlv_ia_set_index_buffer(mlv_context *ctx, mlv_buffer *ib, uint32_t offset_bytes);
+/* stride = IA.input_layout, the per-vertex byte size (main.c:74, main.c:1286) */
+MLV_API int mlv_ia_set_vertex_buffer(mlv_context *ctx, mlv_buffer *vb, uint32_t stride, uint32_t offset_bytes);
+MLV_API int mlv_ia_set_primitive_topology(mlv_context *ctx, int topology);
+
+/* ---- VS (main.c:78-83; shader and register count set per draw at main.c:1287-1288) */
+MLV_API int mlv_vs_set_shader(mlv_context *ctx, int vertex_shader);
+MLV_API int mlv_vs_set_constant_buffer(mlv_context *ctx, uint32_t slot, mlv_buffer *cb);
+MLV_API int mlv_vs_set_shader_resource(mlv_context *ctx, uint32_t slot, mlv_texture2d *tex);
+
+/* ---- RS (main.c:94-96) */
+MLV_API int mlv_rs_set_viewport(mlv_context *ctx, const mlv_viewport *vp);
+
+/* ---- PS (main.c:98-101) */
+MLV_API int mlv_ps_set_shader(mlv_context *ctx, int pixel_shader);
+MLV_API int mlv_ps_set_shader_resource(mlv_context *ctx, uint32_t slot, mlv_texture2d *tex);
+
+/* ---- OM (main.c:103-107) */
+MLV_API int mlv_om_set_render_targets(mlv_context *ctx, mlv_render_target *rt, mlv_depth_stencil *ds);
+MLV_API int mlv_om_set_depth_func(mlv_context *ctx, int depth_func);
+/* Extension: per-pixel visibility (assembled-triangle id and draw tag of the last writer).
+ * Two device arrays of width*height u32 each, caller-owned; NULL disables. */
+MLV_API int mlv_om_set_visibility_buffer(mlv_context *ctx, uint32_t *device_tri_ids, uint32_t *device_draw_tags);
+
+/* ---- commands */
+/* main.c:1191 clear_render_target_view(const f32 *p_clear_color) */
+MLV_API int mlv_clear_render_target_view(mlv_context *ctx, mlv_render_target *rt, const float rgba[4]);
+/* main.c:1204 clear_depth_stencil_view(const f32 depth); also resets the 8x8 tile minima (main.c:1212-1214) */
+MLV_API int mlv_clear_depth_stencil_view(mlv_context *ctx, mlv_depth_stencil *ds, float depth);
+/* main.c:1219 draw_indexed(UINT index_count); start_index/base_vertex are the D3D11 arguments
+ * the reference leaves as a TODO (main.c:1219). (n, 0, 0) is the reference call. */
+MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t start_index, int32_t base_vertex);
+/* main.c:1301 present(): headless form, copies the render target (u32 per pixel, row-major) to the host. */
+MLV_API int mlv_render_target_readback(mlv_context *ctx, mlv_render_target *rt, void *host_dst);
+MLV_API int mlv_depth_stencil_readback(mlv_context *ctx, mlv_depth_stencil *ds, float *host_dst);
+
+/* Stats (main.c:199-206), reset like render() does at main.c:1268-1269. */
+MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out);
+MLV_API int mlv_reset_stats(mlv_context *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,113 @@
+"""ctypes loader for libmalevich_hip.so (the C ABI declared in include/malevich.h).
+
+The library is built in-tree by malevich_amd/build.py. There is no fallback:
+if the shared library is missing, loading raises, so nothing ever runs a
+host-side substitute of the GPU pipeline.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+LIB_PATH = Path(__file__).resolve().parent / "libmalevich_hip.so"
+
+OK = 0
+ERROR_INVALID_ARGUMENT = -1
+ERROR_OUT_OF_MEMORY = -2
+ERROR_DEVICE = -3
+ERROR_INVALID_STATE = -4
+ERROR_CAPACITY = -5
+ERROR_NO_DEVICE = -6
+
+BUFFER_VERTEX, BUFFER_INDEX_U32, BUFFER_CONSTANT = 1, 2, 3
+FORMAT_RGBA8_UNORM, FORMAT_RGBA32F, FORMAT_RGBA16F = 1, 2, 3
+TOPOLOGY_UNDEFINED, TOPOLOGY_TRIANGLELIST = 0, 1
+VS_BASIC, VS_VERTEX_LIGHTING, VS_PASSTHROUGH, VS_FULLSCREEN = 1, 2, 3, 4
+PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH = 1, 2, 3
+DEPTH_GREATER_EQUAL, DEPTH_LESS_EQUAL = 0, 1
+
+
+class Viewport(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("top_left_x", "top_left_y", "width", "height", "min_depth", "max_depth")]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in (
+        "vertex_count", "input_triangle_count", "assembled_triangle_count", "active_bin_count",
+        "total_triangle_count_in_bins", "oob_triangle_count", "draw_count", "overflow_count")]
+
+
+_P = C.c_void_p
+_I = C.c_int
+_U32 = C.c_uint32
+_SZ = C.c_size_t
+_PP = C.POINTER(C.c_void_p)
+
+# name -> (restype, argtypes); every symbol declared in include/malevich.h
+SIGNATURES = {
+    "mlv_context_create": (_I, [_I, _PP]),
+    "mlv_context_destroy": (_I, [_P]),
+    "mlv_get_last_error": (C.c_char_p, [_P]),
+    "mlv_synchronize": (_I, [_P]),
+    "mlv_context_stream": (_P, [_P]),
+    "mlv_device_count": (_I, [C.POINTER(C.c_int)]),
+    "mlv_buffer_create": (_I, [_P, _I, _SZ, _P, _PP]),
+    "mlv_buffer_wrap": (_I, [_P, _I, _P, _SZ, _PP]),
+    "mlv_buffer_update": (_I, [_P, _P, _SZ, _SZ, _P]),
+    "mlv_buffer_readback": (_I, [_P, _P, _SZ, _SZ, _P]),
+    "mlv_buffer_destroy": (_I, [_P, _P]),
+    "mlv_buffer_device_ptr": (_P, [_P]),
+    "mlv_texture2d_create": (_I, [_P, _U32, _U32, _I, _U32, _P, _PP]),
+    "mlv_texture2d_destroy": (_I, [_P, _P]),
+    "mlv_texture2d_device_ptr": (_P, [_P]),
+    "mlv_render_target_create": (_I, [_P, _U32, _U32, _PP]),
+    "mlv_render_target_destroy": (_I, [_P, _P]),
+    "mlv_render_target_device_ptr": (_P, [_P]),
+    "mlv_depth_stencil_create": (_I, [_P, _U32, _U32, _PP]),
+    "mlv_depth_stencil_destroy": (_I, [_P, _P]),
+    "mlv_depth_stencil_device_ptr": (_P, [_P]),
+    "mlv_ia_set_index_buffer": (_I, [_P, _P, _U32]),
+    "mlv_ia_set_vertex_buffer": (_I, [_P, _P, _U32, _U32]),
+    "mlv_ia_set_primitive_topology": (_I, [_P, _I]),
+    "mlv_vs_set_shader": (_I, [_P, _I]),
+    "mlv_vs_set_constant_buffer": (_I, [_P, _U32, _P]),
+    "mlv_vs_set_shader_resource": (_I, [_P, _U32, _P]),
+    "mlv_rs_set_viewport": (_I, [_P, C.POINTER(Viewport)]),
+    "mlv_ps_set_shader": (_I, [_P, _I]),
+    "mlv_ps_set_shader_resource": (_I, [_P, _U32, _P]),
+    "mlv_om_set_render_targets": (_I, [_P, _P, _P]),
+    "mlv_om_set_depth_func": (_I, [_P, _I]),
+    "mlv_om_set_visibility_buffer": (_I, [_P, _P, _P]),
+    "mlv_clear_render_target_view": (_I, [_P, _P, C.POINTER(C.c_float)]),
+    "mlv_clear_depth_stencil_view": (_I, [_P, _P, C.c_float]),
+    "mlv_draw_indexed": (_I, [_P, _U32, _U32, C.c_int32]),
+    "mlv_render_target_readback": (_I, [_P, _P, _P]),
+    "mlv_depth_stencil_readback": (_I, [_P, _P, _P]),
+    "mlv_get_stats": (_I, [_P, C.POINTER(Stats)]),
+    "mlv_reset_stats": (_I, [_P]),
+}
+
+_lib: C.CDLL | None = None
+
+
+def load() -> C.CDLL:
+    """Load the gfx950 library; raises if it has not been built."""
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(
+                f"{LIB_PATH} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
+                "(the pipeline has no CPU fallback)")
+        lib = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = lib
+    return _lib
+
+
+class MalevichError(RuntimeError):
+    def __init__(self, code: int, message: str):
+        super().__init__(f"[{code}] {message}")
+        self.code = code

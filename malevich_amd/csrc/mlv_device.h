@@ -1,0 +1,356 @@
+// mlv_device.h -- per-lane arithmetic of the draw pipeline for gfx950.
+//
+// Every function reproduces the reference's IEEE f32 operation sequence (the
+// library is compiled with -ffp-contract=off, so a*b+c stays a multiply and an
+// add). Integer edge arithmetic wraps mod 2^32 like the reference's AVX2 code.
+// Transcendentals (acosf/expf/powf) and 1/sqrt stand in for SVML and the x86
+// rsqrt approximation; their results agree to a few ulp, inside the 1/255
+// RGBA8 tolerance.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <hip/hip_fp16.h>
+#include <stdint.h>
+
+#include "mlv_internal.h"
+
+namespace mlv {
+
+// ---------------------------------------------------------------- math.h
+
+// v4f32_dot, math.h:137-140: ((x*x' + y*y') + z*z') + w*w'
+__device__ __forceinline__ float dot4_seq(float ax, float ay, float az, float aw, float bx, float by, float bz,
+                                          float bw) {
+    float r = ax * bx;
+    r = r + ay * by;
+    r = r + az * bz;
+    r = r + aw * bw;
+    return r;
+}
+
+// v4f256_dot against a matrix row (m4x4f32_mul_v4f256, math.h:142-146,174-177)
+__device__ __forceinline__ float dot4_pair(const float *row, float x, float y, float z, float w) {
+    float xy = row[0] * x + row[1] * y;
+    float zw = row[2] * z + row[3] * w;
+    return xy + zw;
+}
+
+// v3f256_dot, math.h:153-157
+__device__ __forceinline__ float dot3(float ax, float ay, float az, float bx, float by, float bz) {
+    float r = ax * bx + ay * by;
+    return r + az * bz;
+}
+
+// v3f256_normalize, math.h:277-280 (rsqrt approximated by the correctly rounded 1/sqrt)
+__device__ __forceinline__ void normalize3(float x, float y, float z, float &ox, float &oy, float &oz) {
+    float r = 1.0f / sqrtf(dot3(x, y, z, x, y, z));
+    ox = x * r;
+    oy = y * r;
+    oz = z * r;
+}
+
+// _mm256_cvtps_epi32: nearest-even; NaN or out of i32 range gives 0x80000000
+__device__ __forceinline__ int32_t cvt_rne(float f) {
+    if (!(f >= -2147483648.0f && f < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)rintf(f);
+}
+
+__device__ __forceinline__ float lerp_ref(float a, float b, float f) {  // math.h:381-384
+    float p = a * (1.0f - f);
+    float q = b * f;
+    return p + q;
+}
+
+// f256_srgb_from_linear_approx, math.h:419-422
+__device__ __forceinline__ float srgb_approx(float c) {
+    float p = powf(c, 0.416666667f);
+    float s = 1.055f * p;
+    s = s + -0.055f;
+    return (s > 0.0f) ? s : 0.0f;
+}
+
+// env_lighting_ps.c:20-21, vertex_lighting_vs.c:34-35
+__device__ __forceinline__ float tonemap(float c) {
+    float e = expf(c * -1.0f);
+    return powf(1.0f - e, (float)(1.0 / 2.2));
+}
+
+// Output merger encode, main.c:1176-1178
+__device__ __forceinline__ uint32_t encode_output(float r, float g, float b) {
+    uint32_t ir = (uint32_t)cvt_rne(r * 255.0f);
+    uint32_t ig = (uint32_t)cvt_rne(g * 255.0f);
+    uint32_t ib = (uint32_t)cvt_rne(b * 255.0f);
+    return (ir << 16) + (ig << 8) + ib;
+}
+
+// ----------------------------------------------------- texture sampling
+
+__device__ __forceinline__ uint32_t texel_index(const TexDesc &t, int32_t s, int32_t tt) {
+    int32_t wm1 = (int32_t)(t.width - 1u), hm1 = (int32_t)(t.height - 1u);
+    s = s < wm1 ? s : wm1;
+    s = s > 0 ? s : 0;
+    tt = tt < hm1 ? tt : hm1;
+    tt = tt > 0 ? tt : 0;
+    return (uint32_t)tt * t.width + (uint32_t)s;
+}
+
+// get_texel_u_x8 / get_texel_f_x8 (csc.h:30-53) + decode_u32_as_color_x8 (math.h:336-344)
+__device__ __forceinline__ float4 fetch(const TexDesc &t, int32_t s, int32_t tt) {
+    uint32_t i = texel_index(t, s, tt);
+    if (t.format == FMT_RGBA8) {
+        uint32_t c = ((const uint32_t *)t.data)[i];
+        const float k = (float)(1.0 / 255.0);
+        return make_float4((float)(c & 0xFFu) * k, (float)((c >> 8) & 0xFFu) * k, (float)((c >> 16) & 0xFFu) * k,
+                           (float)((c >> 24) & 0xFFu) * k);
+    } else if (t.format == FMT_RGBA32F) {
+        return ((const float4 *)t.data)[i];
+    } else {
+        const ushort4 h = ((const ushort4 *)t.data)[i];
+        return make_float4(__half2float(__ushort_as_half(h.x)), __half2float(__ushort_as_half(h.y)),
+                           __half2float(__ushort_as_half(h.z)), __half2float(__ushort_as_half(h.w)));
+    }
+}
+
+// bilinear_u_x8 / bilinear_f_x8, csc.h:103-121 and csc.h:143-161
+__device__ __forceinline__ float4 bilinear(const TexDesc &t, float u, float v) {
+    float sf = (float)t.width * u;
+    sf = sf + -0.5f;
+    float tf = (float)t.height * (1.0f - v);
+    tf = tf + -0.5f;
+    int32_t s = cvt_rne(floorf(sf));
+    int32_t tt = cvt_rne(floorf(tf));
+    float fs = sf - (float)s;
+    float ft = tf - (float)tt;
+    int32_t s1 = (int32_t)((uint32_t)s + 1u), t1 = (int32_t)((uint32_t)tt + 1u);
+    float4 t00 = fetch(t, s, tt), t10 = fetch(t, s1, tt);
+    float4 t01 = fetch(t, s, t1), t11 = fetch(t, s1, t1);
+    float4 a = make_float4(lerp_ref(t00.x, t10.x, fs), lerp_ref(t00.y, t10.y, fs), lerp_ref(t00.z, t10.z, fs),
+                           lerp_ref(t00.w, t10.w, fs));
+    float4 b = make_float4(lerp_ref(t01.x, t11.x, fs), lerp_ref(t01.y, t11.y, fs), lerp_ref(t01.z, t11.z, fs),
+                           lerp_ref(t01.w, t11.w, fs));
+    return make_float4(lerp_ref(a.x, b.x, ft), lerp_ref(a.y, b.y, ft), lerp_ref(a.z, b.z, ft),
+                       lerp_ref(a.w, b.w, ft));
+}
+
+// sample_2D_latlon_x8, csc.h:226-244
+__device__ __forceinline__ float4 latlon(const TexDesc &t, float dx, float dy, float dz) {
+    const float TAU_F = 6.283185307f, PI_F = 3.141592654f;  // math.h:33-34
+    float kx0, kx1, kx2, ky0, ky1, ky2, kz0, kz1, kz2, cx, cy, cz;
+    normalize3(0.0f, 0.0f, 1.0f, kz0, kz1, kz2);
+    float cos_theta = dot3(kz0, kz1, kz2, dx, dy, dz);
+    normalize3(dx, dy, 0.0f, cx, cy, cz);
+    normalize3(1.0f, 0.0f, 0.0f, kx0, kx1, kx2);
+    normalize3(0.0f, 1.0f, 0.0f, ky0, ky1, ky2);
+    float cos_x = dot3(kx0, kx1, kx2, cx, cy, cz);
+    float cos_y = dot3(ky0, ky1, ky2, cx, cy, cz);
+    float ax = acosf(cos_x) * (float)(1.0 / (double)TAU_F);
+    float uv_x = (cos_y >= 0.0f) ? ax : (1.0f - ax);
+    float uv_y = acosf(cos_theta) * (float)(1.0 / (double)PI_F);
+    bool pos = cos_theta > 0.999f, neg = cos_theta < -0.999f;
+    if (pos || neg) uv_x = 0.5f;
+    if (pos) uv_y = 0.0f;
+    if (neg) uv_y = 1.0f;
+    uv_y = 1.0f - uv_y;
+    return bilinear(t, uv_x, uv_y);
+}
+
+// ------------------------------------------------------------- shaders
+
+// Vertex shaders (basic_vs.c:22-33, vertex_lighting_vs.c:22-39, passthrough_vs.c:16-25,
+// fullscreen_vs.c:22-39). Writes the three output registers.
+template <int VS>
+__device__ __forceinline__ void vertex_shader(const DrawParams &p, const float *in, float4 &r0, float4 &r1,
+                                              float4 &r2) {
+    if constexpr (VS == VS_BASIC || VS == VS_VERTEX_LIGHTING) {
+        const float *cfw = p.cb;
+        float x = in[0], y = in[1], z = in[2];
+        r0 = make_float4(dot4_pair(cfw, x, y, z, 1.0f), dot4_pair(cfw + 4, x, y, z, 1.0f),
+                         dot4_pair(cfw + 8, x, y, z, 1.0f), dot4_pair(cfw + 12, x, y, z, 1.0f));
+        float nx, ny, nz;
+        normalize3(in[3], in[4], in[5], nx, ny, nz);
+        if constexpr (VS == VS_BASIC) {
+            r1 = make_float4(nx, ny, nz, in[6]);
+        } else {
+            float4 c = latlon(p.vs_tex, nx, ny, nz);
+            r1 = make_float4(tonemap(c.x), tonemap(c.y), tonemap(c.z), in[6]);
+        }
+        r2 = make_float4(in[7], 0.0f, 0.0f, 0.0f);
+    } else if constexpr (VS == VS_PASSTHROUGH) {
+        r0 = make_float4(in[0] * 2.0f - 1.0f, in[1] * 2.0f - 1.0f, in[2], in[3]);
+        r1 = make_float4(in[4], in[5], in[6], 0.0f);
+        r2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    } else {  // VS_FULLSCREEN
+        float px = in[0] * 2.0f - 1.0f, py = in[1] * 2.0f - 1.0f, pz = in[2], pw = in[3];
+        const float *vfc = p.cb + 16, *wfv = p.cb + 32;
+        float d0 = dot4_pair(vfc, px, py, pz, pw), d1 = dot4_pair(vfc + 4, px, py, pz, pw);
+        float d2 = dot4_pair(vfc + 8, px, py, pz, pw);
+        r0 = make_float4(px, py, pz, pw);
+        r1 = make_float4(dot4_pair(wfv, d0, d1, d2, 0.0f), dot4_pair(wfv + 4, d0, d1, d2, 0.0f),
+                         dot4_pair(wfv + 8, d0, d1, d2, 0.0f), 0.0f);
+        r2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+}
+
+// Pixel shaders (basic_ps.c:16-25, env_lighting_ps.c:13-22, passthrough_ps.c:13-19).
+// a1/a2 are interpolated registers 1 and 2.
+template <int PS>
+__device__ __forceinline__ void pixel_shader(const DrawParams &p, float4 a1, float4 a2, float &r, float &g,
+                                             float &b) {
+    if constexpr (PS == PS_BASIC) {
+        float4 c = bilinear(p.ps_tex, a1.w, a2.x);  // UV = (reg1.w, reg2.x)
+        r = srgb_approx(c.x);
+        g = srgb_approx(c.y);
+        b = srgb_approx(c.z);
+    } else if constexpr (PS == PS_ENV_LIGHTING) {
+        float nx, ny, nz;
+        normalize3(a1.x, a1.y, a1.z, nx, ny, nz);
+        float4 c = latlon(p.ps_tex, nx, ny, nz);
+        r = tonemap(c.x);
+        g = tonemap(c.y);
+        b = tonemap(c.z);
+    } else {
+        r = a1.x;
+        g = a1.y;
+        b = a1.z;
+    }
+}
+
+// ---------------------------------------------------- primitive assembly
+
+__device__ __forceinline__ int32_t snap28_4(float v) {  // main.c:842-848: floor(v*16 + 0.5) in double
+    double d = floor((double)(v * 16.0f) + 0.5);
+    if (!(d >= -2147483648.0 && d < 2147483648.0)) return INT32_MIN;
+    return (int32_t)d;
+}
+
+// set_edge_function, main.c:563-575, u32 wrap
+__device__ __forceinline__ void set_edge(int32_t area, int32_t x0, int32_t y0, int32_t x1, int32_t y1, int32_t &ea,
+                                         int32_t &eb, int32_t &ec) {
+    uint32_t a = (uint32_t)y0 - (uint32_t)y1;
+    uint32_t b = (uint32_t)x1 - (uint32_t)x0;
+    if (area < 0) {
+        a = 0u - a;
+        b = 0u - b;
+    }
+    uint32_t c = (0u - a) * (uint32_t)x0 - b * (uint32_t)y0;
+    ea = (int32_t)a;
+    eb = (int32_t)b;
+    ec = (int32_t)c;
+}
+
+__device__ __forceinline__ float max_ref(float x, float y) { return (x > y) ? x : y; }  // math.h:28-30
+__device__ __forceinline__ float min_ref(float x, float y) { return (x < y) ? x : y; }  // math.h:25-27
+__device__ __forceinline__ int32_t imin(int32_t x, int32_t y) { return (x < y) ? x : y; }
+__device__ __forceinline__ int32_t imax(int32_t x, int32_t y) { return (x > y) ? x : y; }
+
+// Trivial frustum reject (main.c:764-772) and the "no clipping needed" test (main.c:775-781).
+// Returns 0 = dropped, 1 = inside, 2 = needs clipping.
+__device__ __forceinline__ int classify(float4 a, float4 b, float4 c) {
+    if (a.w == 0.0f || b.w == 0.0f || c.w == 0.0f) return 0;  // main.c:759
+    if ((a.x < -a.w && b.x < -b.w && c.x < -c.w) || (a.x > +a.w && b.x > +b.w && c.x > +c.w) ||
+        (a.y < -a.w && b.y < -b.w && c.y < -c.w) || (a.y > +a.w && b.y > +b.w && c.y > +c.w) ||
+        (a.z < 0.f && b.z < 0.f && c.z < 0.f) || (a.z > +a.w && b.z > +b.w && c.z > +c.w))
+        return 0;
+    bool inside = (a.x >= -a.w && b.x >= -b.w && c.x >= -c.w) && (a.x <= +a.w && b.x <= +b.w && c.x <= +c.w) &&
+                  (a.y >= -a.w && b.y >= -b.w && c.y >= -c.w) && (a.y <= +a.w && b.y <= +b.w && c.y <= +c.w) &&
+                  (a.z >= 0.f && b.z >= 0.f && c.z >= 0.f) && (a.z <= +a.w && b.z <= +b.w && c.z <= +c.w);
+    return inside ? 1 : 2;
+}
+
+// Projection, viewport transform, snapping, area, cull and setup for one fan triangle
+// (main.c:799-898). Returns false when back-face culled (signed_area > 0).
+__device__ __forceinline__ bool setup_triangle(const DrawParams &p, float4 q0, float4 q1, float4 q2, TriVis &tv,
+                                               float rw[3]) {
+    float4 q[3] = {q0, q1, q2};
+    float sx[3], sy[3], sz[3];
+    int32_t x[3], y[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        rw[k] = 1.0f / q[k].w;  // (f32)(1.0/(f64)w) equals the f32 quotient
+        float X = q[k].x * rw[k], Y = q[k].y * rw[k], Z = q[k].z * rw[k], Wn = q[k].w * rw[k];
+        sx[k] = dot4_seq(p.screen[0][0], p.screen[0][1], p.screen[0][2], p.screen[0][3], X, Y, Z, Wn);
+        sy[k] = dot4_seq(p.screen[1][0], p.screen[1][1], p.screen[1][2], p.screen[1][3], X, Y, Z, Wn);
+        sz[k] = dot4_seq(p.screen[2][0], p.screen[2][1], p.screen[2][2], p.screen[2][3], X, Y, Z, Wn);
+        x[k] = snap28_4(sx[k]);
+        y[k] = snap28_4(sy[k]);
+    }
+    int32_t area = (int32_t)(((uint32_t)x[1] - (uint32_t)x[0]) * ((uint32_t)y[2] - (uint32_t)y[0]) -
+                             ((uint32_t)x[2] - (uint32_t)x[0]) * ((uint32_t)y[1] - (uint32_t)y[0]));
+    if (area > 0) return false;  // main.c:856
+    int32_t ea, eb, ec;
+    set_edge(area, x[1], y[1], x[2], y[2], ea, eb, ec);  // slot 0 = (v1, v2)
+    tv.a16[0] = (int32_t)((uint32_t)ea << 4); tv.b16[0] = (int32_t)((uint32_t)eb << 4); tv.c[0] = ec;
+    set_edge(area, x[2], y[2], x[0], y[0], ea, eb, ec);  // slot 1 = (v2, v0)
+    tv.a16[1] = (int32_t)((uint32_t)ea << 4); tv.b16[1] = (int32_t)((uint32_t)eb << 4); tv.c[1] = ec;
+    set_edge(area, x[0], y[0], x[1], y[1], ea, eb, ec);  // slot 2 = (v0, v1)
+    tv.a16[2] = (int32_t)((uint32_t)ea << 4); tv.b16[2] = (int32_t)((uint32_t)eb << 4); tv.c[2] = ec;
+    float af = (float)(area >> 8);
+    if (af == 0.0f) af = 1.0f;
+    tv.ooa = fabsf(1.f / af);
+    tv.z0 = sz[0];
+    tv.dz1 = sz[1] - sz[0];
+    tv.dz2 = sz[2] - sz[0];
+    float mx = max_ref(sz[0], max_ref(sz[1], sz[2]));
+    float mn = min_ref(sz[0], min_ref(sz[1], sz[2]));
+    float S = fabsf(sz[0]) + fabsf(sz[1]) + fabsf(sz[2]);
+    if (p.depth_func == DEPTH_GE) {
+        tv.hiz_depth = mx;
+        tv.cull_bound = (mx + S * 3.814697265625e-06f) + 1e-30f;  // 2^-18 relative margin
+    } else {
+        tv.hiz_depth = mn;
+        tv.cull_bound = (mn - S * 3.814697265625e-06f) - 1e-30f;
+    }
+    // bounds (main.c:888-898) then the 8x8 bin rect (main.c:930-931)
+    int32_t mnx = imin(x[0], imin(x[1], x[2])) >> 4, mny = imin(y[0], imin(y[1], y[2])) >> 4;
+    mnx = imin(imax(mnx, 0), p.W - 1);
+    mny = imin(imax(mny, 0), p.H - 1);
+    int32_t mxx = imax(x[0], imax(x[1], x[2])) >> 4, mxy = imax(y[0], imax(y[1], y[2])) >> 4;
+    mxx = imin(mxx + 1, p.W - 1);
+    mxy = imin(mxy + 1, p.H - 1);
+    int32_t bx0 = imax(mnx / kBin, 0), by0 = imax(mny / kBin, 0);
+    int32_t bx1 = imin(mxx / kBin, p.bins_x - 1), by1 = imin(mxy / kBin, p.bins_y - 1);
+    tv.rect0 = (uint32_t)(uint16_t)bx0 | ((uint32_t)(uint16_t)by0 << 16);
+    tv.rect1 = (uint32_t)(uint16_t)(int16_t)bx1 | ((uint32_t)(uint16_t)(int16_t)by1 << 16);
+    return true;
+}
+
+__device__ __forceinline__ void rect_of(const TriVis &tv, int32_t &bx0, int32_t &by0, int32_t &bx1, int32_t &by1) {
+    bx0 = (int32_t)(tv.rect0 & 0xFFFFu);
+    by0 = (int32_t)(tv.rect0 >> 16);
+    bx1 = (int32_t)(int16_t)(tv.rect1 & 0xFFFFu);
+    by1 = (int32_t)(int16_t)(tv.rect1 >> 16);
+}
+
+// clip_by_plane, main.c:609-647 (plane_d = 0). v holds n vertices of 12 floats.
+__device__ inline int clip_by_plane(float (*v)[12], int n, const float *pl, float (*out)[12]) {
+    int k = 0;
+    if (n <= 0) return 0;
+    float cur = dot4_seq(pl[0], pl[1], pl[2], pl[3], v[0][0], v[0][1], v[0][2], v[0][3]);
+    const float plane_d = 0.0f;
+    bool cur_in = cur > -plane_d;
+    for (int i = 0; i < n; ++i) {
+        if (k >= kMaxClip) return -1;
+        int nx = (i + 1) % n;
+        if (cur_in) {
+            for (int c = 0; c < 12; ++c) out[k][c] = v[i][c];
+            k++;
+        }
+        float nd = dot4_seq(pl[0], pl[1], pl[2], pl[3], v[nx][0], v[nx][1], v[nx][2], v[nx][3]);
+        bool nx_in = nd > -plane_d;
+        if (cur_in != nx_in) {
+            if (k >= kMaxClip) return -1;
+            float t = (plane_d + cur) / (cur - nd);
+            float omt = 1.0f - t;
+            for (int c = 0; c < 12; ++c) {
+                float a = v[i][c] * omt;
+                float b = v[nx][c] * t;
+                out[k][c] = a + b;
+            }
+            k++;
+        }
+        cur = nd;
+        cur_in = nx_in;
+    }
+    return k;
+}
+
+}  // namespace mlv

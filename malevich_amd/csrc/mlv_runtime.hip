@@ -1,0 +1,734 @@
+// mlv_runtime.hip -- host side of the C ABI (include/malevich.h).
+//
+// The context owns the pipeline state that the reference keeps in its global
+// `Pipeline graphics_pipeline` (main.c:109-115, 222), one HIP stream, and the
+// transient arenas that draw_indexed mallocs and frees on every call in the
+// reference (main.c:675-746, 947, 965, 986, 1253-1259). Arenas here grow
+// geometrically and are reused, so a steady-state draw allocates nothing.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdarg>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <cstring>
+#include <new>
+#include <string>
+
+#include "malevich.h"
+#include "mlv_internal.h"
+
+using namespace mlv;
+
+struct mlv_buffer {
+    void *ptr;
+    size_t bytes;
+    int kind;
+    bool owned;
+};
+
+struct mlv_texture2d {
+    void *ptr;
+    uint32_t width, height, format, mip_levels;
+    size_t bytes;
+};
+
+struct mlv_render_target {
+    uint32_t *ptr;
+    uint32_t width, height;
+};
+
+struct mlv_depth_stencil {
+    float *ptr;
+    float *hiz;
+    uint32_t width, height, bins_x, bins_y;
+    int hiz_mode;
+};
+
+namespace {
+
+struct Arena {
+    void *p = nullptr;
+    size_t cap = 0;
+};
+
+}  // namespace
+
+struct mlv_context {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // Pipeline state (main.c:66-115)
+    mlv_buffer *ib = nullptr;
+    uint32_t ib_off = 0;
+    mlv_buffer *vb = nullptr;
+    uint32_t vb_stride = 0, vb_off = 0;
+    int topology = MLV_TOPOLOGY_UNDEFINED;
+    int vs = 0, ps = 0;
+    mlv_buffer *vs_cb[16] = {};
+    mlv_texture2d *vs_srv[16] = {};
+    mlv_texture2d *ps_srv[16] = {};
+    mlv_viewport vp{};
+    bool vp_set = false;
+    mlv_render_target *rt = nullptr;
+    mlv_depth_stencil *ds = nullptr;
+    int depth_func = MLV_DEPTH_GREATER_EQUAL;
+    uint32_t *vis_tri = nullptr, *vis_draw = nullptr;
+    // arenas
+    Arena vpos, vattr, cnt, blocksum, blockoff_tri, blockoff_vtx, clip_list, clip_off, tvis, tshade;
+    Arena tile_cnt, tile_start, tile_fill, tile_list;
+    DrawCounters *ctr = nullptr;
+    DevStats *dstats = nullptr;
+    DevStats seen{};
+    // host-side stats (main.c:1228-1246)
+    uint64_t vertex_count = 0, input_tris = 0, draws = 0;
+    uint32_t draw_counter = 0;
+    uint64_t capacity_errors = 0;
+    size_t clip_hint = 4096, pair_hint = 0;
+    size_t pair_limit = 0;  // MLV_PAIR_CAPACITY: caps the bin-list arena (exercises the overflow path)
+    // last draw, for inspection
+    uint32_t last_tri_cap = 0;
+};
+
+static int fail(mlv_context *ctx, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
+static int fail(mlv_context *ctx, int code, const char *fmt, ...) {
+    if (ctx) {
+        char buf[512];
+        va_list ap;
+        va_start(ap, fmt);
+        vsnprintf(buf, sizeof buf, fmt, ap);
+        va_end(ap);
+        ctx->err = buf;
+    }
+    return code;
+}
+
+#define HIP_OR_FAIL(ctx, call)                                                                                \
+    do {                                                                                                      \
+        hipError_t e_ = (call);                                                                               \
+        if (e_ != hipSuccess)                                                                                 \
+            return fail((ctx), e_ == hipErrorOutOfMemory ? MLV_ERROR_OUT_OF_MEMORY : MLV_ERROR_DEVICE,        \
+                        "%s failed: %s", #call, hipGetErrorString(e_));                                       \
+    } while (0)
+
+static int ensure(mlv_context *ctx, Arena &a, size_t bytes) {
+    if (bytes <= a.cap) return MLV_OK;
+    size_t want = std::max(bytes, a.cap + a.cap / 2);
+    want = (want + 255) & ~(size_t)255;
+    if (a.p) {
+        HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+        HIP_OR_FAIL(ctx, hipFree(a.p));
+        a.p = nullptr;
+        a.cap = 0;
+    }
+    HIP_OR_FAIL(ctx, hipMalloc(&a.p, want));
+    a.cap = want;
+    return MLV_OK;
+}
+
+static void release(Arena &a) {
+    if (a.p) (void)hipFree(a.p);
+    a.p = nullptr;
+    a.cap = 0;
+}
+
+static int set_device(mlv_context *ctx) {
+    HIP_OR_FAIL(ctx, hipSetDevice(ctx->device));
+    return MLV_OK;
+}
+
+extern "C" {
+
+MLV_API int mlv_device_count(int *out) {
+    if (!out) return MLV_ERROR_INVALID_ARGUMENT;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) n = 0;
+    *out = n;
+    return MLV_OK;
+}
+
+MLV_API int mlv_context_create(int device, mlv_context **out) {
+    if (!out) return MLV_ERROR_INVALID_ARGUMENT;
+    *out = nullptr;
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess || n <= 0) return MLV_ERROR_NO_DEVICE;
+    if (device < 0 || device >= n) return MLV_ERROR_INVALID_ARGUMENT;
+    mlv_context *ctx = new (std::nothrow) mlv_context();
+    if (!ctx) return MLV_ERROR_OUT_OF_MEMORY;
+    ctx->device = device;
+    if (const char *lim = getenv("MLV_PAIR_CAPACITY")) ctx->pair_limit = (size_t)strtoull(lim, nullptr, 10);
+    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&ctx->ctr, sizeof(DrawCounters)) != hipSuccess ||
+        hipMalloc(&ctx->dstats, sizeof(DevStats)) != hipSuccess ||
+        hipMemset(ctx->dstats, 0, sizeof(DevStats)) != hipSuccess ||
+        hipMemset(ctx->ctr, 0, sizeof(DrawCounters)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+        if (ctx->ctr) (void)hipFree(ctx->ctr);
+        if (ctx->dstats) (void)hipFree(ctx->dstats);
+        delete ctx;
+        return MLV_ERROR_DEVICE;
+    }
+    *out = ctx;
+    return MLV_OK;
+}
+
+MLV_API int mlv_context_destroy(mlv_context *ctx) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    Arena *all[] = {&ctx->vpos,     &ctx->vattr,     &ctx->cnt,       &ctx->blocksum,  &ctx->blockoff_tri,
+                    &ctx->blockoff_vtx, &ctx->clip_list, &ctx->clip_off, &ctx->tvis,    &ctx->tshade,
+                    &ctx->tile_cnt, &ctx->tile_start, &ctx->tile_fill, &ctx->tile_list};
+    for (Arena *a : all) release(*a);
+    (void)hipFree(ctx->ctr);
+    (void)hipFree(ctx->dstats);
+    (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return MLV_OK;
+}
+
+MLV_API const char *mlv_get_last_error(const mlv_context *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+MLV_API void *mlv_context_stream(mlv_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
+
+// Reads the device-side counters and turns capacity overflows into an error and a
+// larger arena for the next draws.
+static int check_device_stats(mlv_context *ctx) {
+    DevStats s{};
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(&s, ctx->dstats, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->seen = s;
+    // size the next draws' arenas from what the device saw
+    ctx->pair_hint = std::max<size_t>(ctx->pair_hint, (size_t)(s.max_pairs + s.max_pairs / 4));
+    ctx->clip_hint = std::max<size_t>(ctx->clip_hint, (size_t)(s.max_clip_list + s.max_clip_list / 4));
+    if (s.capacity_errors > ctx->capacity_errors) {
+        ctx->capacity_errors = s.capacity_errors;
+        ctx->clip_hint *= 4;
+        return fail(ctx, MLV_ERROR_CAPACITY, "a draw exceeded the clipper arena; capacity was raised, redraw");
+    }
+    return MLV_OK;
+}
+
+MLV_API int mlv_synchronize(mlv_context *ctx) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (int rc = set_device(ctx)) return rc;
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    return check_device_stats(ctx);
+}
+
+// ------------------------------------------------------------------ resources
+
+MLV_API int mlv_buffer_create(mlv_context *ctx, int kind, size_t bytes, const void *host_init, mlv_buffer **out) {
+    if (!ctx || !out || bytes == 0 || kind < MLV_BUFFER_VERTEX || kind > MLV_BUFFER_CONSTANT)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_buffer_create: bad argument");
+    if (int rc = set_device(ctx)) return rc;
+    void *p = nullptr;
+    HIP_OR_FAIL(ctx, hipMalloc(&p, (bytes + 15) & ~(size_t)15));
+    if (host_init) {
+        hipError_t e = hipMemcpyAsync(p, host_init, bytes, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return fail(ctx, MLV_ERROR_DEVICE, "mlv_buffer_create upload: %s", hipGetErrorString(e));
+        }
+    }
+    mlv_buffer *b = new (std::nothrow) mlv_buffer{p, bytes, kind, true};
+    if (!b) {
+        (void)hipFree(p);
+        return MLV_ERROR_OUT_OF_MEMORY;
+    }
+    *out = b;
+    return MLV_OK;
+}
+
+MLV_API int mlv_buffer_wrap(mlv_context *ctx, int kind, void *device_ptr, size_t bytes, mlv_buffer **out) {
+    if (!ctx || !out || !device_ptr || bytes == 0 || kind < MLV_BUFFER_VERTEX || kind > MLV_BUFFER_CONSTANT)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_buffer_wrap: bad argument");
+    mlv_buffer *b = new (std::nothrow) mlv_buffer{device_ptr, bytes, kind, false};
+    if (!b) return MLV_ERROR_OUT_OF_MEMORY;
+    *out = b;
+    return MLV_OK;
+}
+
+MLV_API int mlv_buffer_update(mlv_context *ctx, mlv_buffer *buf, size_t offset, size_t bytes, const void *host_src) {
+    if (!ctx || !buf || !host_src || offset + bytes > buf->bytes)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_buffer_update: bad argument");
+    if (int rc = set_device(ctx)) return rc;
+    HIP_OR_FAIL(ctx, hipMemcpyAsync((uint8_t *)buf->ptr + offset, host_src, bytes, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    return MLV_OK;
+}
+
+MLV_API int mlv_buffer_readback(mlv_context *ctx, const mlv_buffer *buf, size_t offset, size_t bytes, void *host_dst) {
+    if (!ctx || !buf || !host_dst || offset + bytes > buf->bytes)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_buffer_readback: bad argument");
+    if (int rc = set_device(ctx)) return rc;
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, (const uint8_t *)buf->ptr + offset, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    return check_device_stats(ctx);
+}
+
+MLV_API int mlv_buffer_destroy(mlv_context *ctx, mlv_buffer *buf) {
+    if (!ctx || !buf) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)hipStreamSynchronize(ctx->stream);
+    if (buf->owned) (void)hipFree(buf->ptr);
+    for (auto &c : ctx->vs_cb)
+        if (c == buf) c = nullptr;
+    if (ctx->ib == buf) ctx->ib = nullptr;
+    if (ctx->vb == buf) ctx->vb = nullptr;
+    delete buf;
+    return MLV_OK;
+}
+
+MLV_API void *mlv_buffer_device_ptr(const mlv_buffer *buf) { return buf ? buf->ptr : nullptr; }
+
+static size_t texel_bytes(int fmt) {
+    switch (fmt) {
+    case MLV_FORMAT_RGBA8_UNORM: return 4;
+    case MLV_FORMAT_RGBA32F: return 16;
+    case MLV_FORMAT_RGBA16F: return 8;
+    default: return 0;
+    }
+}
+
+MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t height, int format, uint32_t mip_levels,
+                                 const void *host_data, mlv_texture2d **out) {
+    const size_t tb = texel_bytes(format);
+    if (!ctx || !out || !width || !height || !tb || width > 65536u || height > 65536u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_texture2d_create: bad argument");
+    if (mip_levels == 0) mip_levels = 1;
+    if (int rc = set_device(ctx)) return rc;
+    const size_t bytes = (size_t)width * height * tb;
+    void *p = nullptr;
+    HIP_OR_FAIL(ctx, hipMalloc(&p, bytes));
+    if (host_data) {
+        hipError_t e = hipMemcpyAsync(p, host_data, bytes, hipMemcpyHostToDevice, ctx->stream);
+        if (e == hipSuccess) e = hipStreamSynchronize(ctx->stream);
+        if (e != hipSuccess) {
+            (void)hipFree(p);
+            return fail(ctx, MLV_ERROR_DEVICE, "texture upload: %s", hipGetErrorString(e));
+        }
+    }
+    mlv_texture2d *t = new (std::nothrow) mlv_texture2d{p, width, height, (uint32_t)format, mip_levels, bytes};
+    if (!t) {
+        (void)hipFree(p);
+        return MLV_ERROR_OUT_OF_MEMORY;
+    }
+    *out = t;
+    return MLV_OK;
+}
+
+MLV_API int mlv_texture2d_destroy(mlv_context *ctx, mlv_texture2d *tex) {
+    if (!ctx || !tex) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(tex->ptr);
+    for (auto &s : ctx->vs_srv)
+        if (s == tex) s = nullptr;
+    for (auto &s : ctx->ps_srv)
+        if (s == tex) s = nullptr;
+    delete tex;
+    return MLV_OK;
+}
+
+MLV_API void *mlv_texture2d_device_ptr(const mlv_texture2d *tex) { return tex ? tex->ptr : nullptr; }
+
+MLV_API int mlv_render_target_create(mlv_context *ctx, uint32_t width, uint32_t height, mlv_render_target **out) {
+    if (!ctx || !out || !width || !height || width > 16384u || height > 16384u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_render_target_create: bad size");
+    if (int rc = set_device(ctx)) return rc;
+    uint32_t *p = nullptr;
+    HIP_OR_FAIL(ctx, hipMalloc(&p, (size_t)width * height * 4));
+    mlv_render_target *rt = new (std::nothrow) mlv_render_target{p, width, height};
+    if (!rt) {
+        (void)hipFree(p);
+        return MLV_ERROR_OUT_OF_MEMORY;
+    }
+    *out = rt;
+    return MLV_OK;
+}
+
+MLV_API int mlv_render_target_destroy(mlv_context *ctx, mlv_render_target *rt) {
+    if (!ctx || !rt) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(rt->ptr);
+    if (ctx->rt == rt) ctx->rt = nullptr;
+    delete rt;
+    return MLV_OK;
+}
+
+MLV_API void *mlv_render_target_device_ptr(const mlv_render_target *rt) { return rt ? rt->ptr : nullptr; }
+
+MLV_API int mlv_depth_stencil_create(mlv_context *ctx, uint32_t width, uint32_t height, mlv_depth_stencil **out) {
+    if (!ctx || !out || !width || !height || width > 16384u || height > 16384u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_depth_stencil_create: bad size");
+    if (int rc = set_device(ctx)) return rc;
+    const uint32_t bx = (width + kBin - 1) / kBin, by = (height + kBin - 1) / kBin;
+    float *p = nullptr, *h = nullptr;
+    HIP_OR_FAIL(ctx, hipMalloc(&p, (size_t)width * height * 4));
+    if (hipMalloc(&h, (size_t)bx * by * 4) != hipSuccess) {
+        (void)hipFree(p);
+        return fail(ctx, MLV_ERROR_OUT_OF_MEMORY, "depth hi-z allocation failed");
+    }
+    mlv_depth_stencil *ds = new (std::nothrow) mlv_depth_stencil{p, h, width, height, bx, by, MLV_DEPTH_GREATER_EQUAL};
+    if (!ds) {
+        (void)hipFree(p);
+        (void)hipFree(h);
+        return MLV_ERROR_OUT_OF_MEMORY;
+    }
+    // a fresh depth target behaves as if cleared: hierarchical-Z starts at the reset value
+    if (launch_fill_f32(h, 0.0f, (size_t)bx * by, ctx->stream) != hipSuccess) {
+        (void)hipFree(p);
+        (void)hipFree(h);
+        delete ds;
+        return fail(ctx, MLV_ERROR_DEVICE, "hi-z init failed");
+    }
+    *out = ds;
+    return MLV_OK;
+}
+
+MLV_API int mlv_depth_stencil_destroy(mlv_context *ctx, mlv_depth_stencil *ds) {
+    if (!ctx || !ds) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipFree(ds->ptr);
+    (void)hipFree(ds->hiz);
+    if (ctx->ds == ds) ctx->ds = nullptr;
+    delete ds;
+    return MLV_OK;
+}
+
+MLV_API void *mlv_depth_stencil_device_ptr(const mlv_depth_stencil *ds) { return ds ? ds->ptr : nullptr; }
+
+// ------------------------------------------------------------------ state
+
+MLV_API int mlv_ia_set_index_buffer(mlv_context *ctx, mlv_buffer *ib, uint32_t offset_bytes) {
+    if (!ctx || (ib && (ib->kind != MLV_BUFFER_INDEX_U32 || offset_bytes % 4 || offset_bytes > ib->bytes)))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_ia_set_index_buffer: bad argument");
+    ctx->ib = ib;
+    ctx->ib_off = offset_bytes;
+    return MLV_OK;
+}
+
+MLV_API int mlv_ia_set_vertex_buffer(mlv_context *ctx, mlv_buffer *vb, uint32_t stride, uint32_t offset_bytes) {
+    if (!ctx || (vb && (vb->kind != MLV_BUFFER_VERTEX || stride < 32 || stride % 4 || offset_bytes % 4 ||
+                        offset_bytes > vb->bytes)))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT,
+                    "mlv_ia_set_vertex_buffer: stride must be >= 32 and a multiple of 4 (the reference layouts read 8 floats)");
+    ctx->vb = vb;
+    ctx->vb_stride = stride;
+    ctx->vb_off = offset_bytes;
+    return MLV_OK;
+}
+
+MLV_API int mlv_ia_set_primitive_topology(mlv_context *ctx, int topology) {
+    if (!ctx || (topology != MLV_TOPOLOGY_UNDEFINED && topology != MLV_TOPOLOGY_TRIANGLELIST))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown topology");
+    ctx->topology = topology;
+    return MLV_OK;
+}
+
+MLV_API int mlv_vs_set_shader(mlv_context *ctx, int vertex_shader) {
+    if (!ctx || vertex_shader < MLV_VS_BASIC || vertex_shader > MLV_VS_FULLSCREEN)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown vertex shader");
+    ctx->vs = vertex_shader;
+    return MLV_OK;
+}
+
+MLV_API int mlv_vs_set_constant_buffer(mlv_context *ctx, uint32_t slot, mlv_buffer *cb) {
+    if (!ctx || slot >= 16 || (cb && cb->kind != MLV_BUFFER_CONSTANT))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_vs_set_constant_buffer: bad argument");
+    ctx->vs_cb[slot] = cb;
+    return MLV_OK;
+}
+
+MLV_API int mlv_vs_set_shader_resource(mlv_context *ctx, uint32_t slot, mlv_texture2d *tex) {
+    if (!ctx || slot >= 16) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "bad shader resource slot");
+    ctx->vs_srv[slot] = tex;
+    return MLV_OK;
+}
+
+MLV_API int mlv_rs_set_viewport(mlv_context *ctx, const mlv_viewport *vp) {
+    if (!ctx || !vp || !(vp->width > 0.0f) || !(vp->height > 0.0f))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "bad viewport");
+    ctx->vp = *vp;
+    ctx->vp_set = true;
+    return MLV_OK;
+}
+
+MLV_API int mlv_ps_set_shader(mlv_context *ctx, int pixel_shader) {
+    if (!ctx || pixel_shader < MLV_PS_BASIC || pixel_shader > MLV_PS_PASSTHROUGH)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown pixel shader");
+    ctx->ps = pixel_shader;
+    return MLV_OK;
+}
+
+MLV_API int mlv_ps_set_shader_resource(mlv_context *ctx, uint32_t slot, mlv_texture2d *tex) {
+    if (!ctx || slot >= 16) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "bad shader resource slot");
+    ctx->ps_srv[slot] = tex;
+    return MLV_OK;
+}
+
+MLV_API int mlv_om_set_render_targets(mlv_context *ctx, mlv_render_target *rt, mlv_depth_stencil *ds) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (rt && ds && (rt->width != ds->width || rt->height != ds->height))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "render target and depth target sizes differ");
+    ctx->rt = rt;
+    ctx->ds = ds;
+    return MLV_OK;
+}
+
+MLV_API int mlv_om_set_depth_func(mlv_context *ctx, int depth_func) {
+    if (!ctx || (depth_func != MLV_DEPTH_GREATER_EQUAL && depth_func != MLV_DEPTH_LESS_EQUAL))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown depth func");
+    ctx->depth_func = depth_func;
+    return MLV_OK;
+}
+
+MLV_API int mlv_om_set_visibility_buffer(mlv_context *ctx, uint32_t *device_tri_ids, uint32_t *device_draw_tags) {
+    if (!ctx || ((device_tri_ids == nullptr) != (device_draw_tags == nullptr)))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "visibility buffer needs both arrays or neither");
+    ctx->vis_tri = device_tri_ids;
+    ctx->vis_draw = device_draw_tags;
+    return MLV_OK;
+}
+
+// ------------------------------------------------------------------ commands
+
+// encode_color_as_u32 (math.h:322-324): truncating conversion per channel
+static uint32_t trunc_u32(float f) {
+    if (!(f > -9.2e18f && f < 9.2e18f)) return 0u;
+    return (uint32_t)(int64_t)f;
+}
+
+MLV_API int mlv_clear_render_target_view(mlv_context *ctx, mlv_render_target *rt, const float rgba[4]) {
+    if (!ctx || !rt || !rgba) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "clear_render_target_view: bad argument");
+    if (int rc = set_device(ctx)) return rc;
+    uint32_t v = (trunc_u32(rgba[3] * 255.f) << 24) + (trunc_u32(rgba[2] * 255.f) << 16) +
+                 (trunc_u32(rgba[1] * 255.f) << 8) + trunc_u32(rgba[0] * 255.f);
+    HIP_OR_FAIL(ctx, launch_fill_u32(rt->ptr, v, (size_t)rt->width * rt->height, ctx->stream));
+    if (ctx->vis_tri && ctx->rt == rt) {
+        HIP_OR_FAIL(ctx, launch_fill_u32(ctx->vis_tri, 0xFFFFFFFFu, (size_t)rt->width * rt->height, ctx->stream));
+        HIP_OR_FAIL(ctx, launch_fill_u32(ctx->vis_draw, 0xFFFFFFFFu, (size_t)rt->width * rt->height, ctx->stream));
+    }
+    return MLV_OK;
+}
+
+MLV_API int mlv_clear_depth_stencil_view(mlv_context *ctx, mlv_depth_stencil *ds, float depth) {
+    if (!ctx || !ds) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "clear_depth_stencil_view: bad argument");
+    if (int rc = set_device(ctx)) return rc;
+    HIP_OR_FAIL(ctx, launch_fill_f32(ds->ptr, depth, (size_t)ds->width * ds->height, ctx->stream));
+    // main.c:1212-1214: every tile minimum back to 0.0 (1.0 for the LESS_EQUAL mirror)
+    const float h = ctx->depth_func == MLV_DEPTH_LESS_EQUAL ? 1.0f : 0.0f;
+    HIP_OR_FAIL(ctx, launch_fill_f32(ds->hiz, h, (size_t)ds->bins_x * ds->bins_y, ctx->stream));
+    ds->hiz_mode = ctx->depth_func;
+    return MLV_OK;
+}
+
+static bool is_float_tex(const mlv_texture2d *t) {
+    return t && (t->format == MLV_FORMAT_RGBA32F || t->format == MLV_FORMAT_RGBA16F);
+}
+
+static TexDesc tex_desc(const mlv_texture2d *t) {
+    TexDesc d{};
+    if (t) {
+        d.data = t->ptr;
+        d.width = t->width;
+        d.height = t->height;
+        d.format = t->format;
+        d.mip_levels = t->mip_levels;
+    }
+    return d;
+}
+
+MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t start_index, int32_t base_vertex) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (ctx->topology != MLV_TOPOLOGY_TRIANGLELIST)
+        return fail(ctx, MLV_ERROR_INVALID_STATE, "draw_indexed: topology must be TRIANGLELIST (main.c:666)");
+    if (index_count % 3u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "draw_indexed: index_count must be a multiple of 3 (main.c:1230)");
+    if (!ctx->ib || !ctx->vb) return fail(ctx, MLV_ERROR_INVALID_STATE, "draw_indexed: index/vertex buffer not bound");
+    if (!ctx->vs || !ctx->ps) return fail(ctx, MLV_ERROR_INVALID_STATE, "draw_indexed: shaders not set");
+    if (!ctx->rt || !ctx->ds) return fail(ctx, MLV_ERROR_INVALID_STATE, "draw_indexed: render targets not set");
+    if (!ctx->vp_set) return fail(ctx, MLV_ERROR_INVALID_STATE, "draw_indexed: viewport not set");
+    const uint32_t W = ctx->rt->width, H = ctx->rt->height;
+    // read_tile/write_tile use viewport.width as the row pitch (main.c:582,596): the viewport
+    // must cover the render target exactly.
+    if (ctx->vp.width != (float)W || ctx->vp.height != (float)H)
+        return fail(ctx, MLV_ERROR_INVALID_STATE, "draw_indexed: viewport must equal the render target size");
+    if ((uint64_t)ctx->ib_off + 4ull * ((uint64_t)start_index + index_count) > ctx->ib->bytes)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "draw_indexed: index range past the index buffer");
+    const bool needs_cb = ctx->vs != MLV_VS_PASSTHROUGH;
+    if (needs_cb && (!ctx->vs_cb[0] || ctx->vs_cb[0]->bytes < 48 * sizeof(float)))
+        return fail(ctx, MLV_ERROR_INVALID_STATE, "draw_indexed: constant buffer 0 (3 float4x4) not bound");
+    if (ctx->vs == MLV_VS_VERTEX_LIGHTING && !is_float_tex(ctx->vs_srv[0]))
+        return fail(ctx, MLV_ERROR_INVALID_STATE, "vertex_lighting_vs needs a float RGBA texture in slot 0");
+    if (ctx->ps == MLV_PS_ENV_LIGHTING && !is_float_tex(ctx->ps_srv[0]))
+        return fail(ctx, MLV_ERROR_INVALID_STATE, "env_lighting_ps needs a float RGBA texture in slot 0");
+    if (ctx->ps == MLV_PS_BASIC && !ctx->ps_srv[0])
+        return fail(ctx, MLV_ERROR_INVALID_STATE, "basic_ps needs a texture in slot 0");
+    if (int rc = set_device(ctx)) return rc;
+
+    const uint32_t T = index_count / 3u;
+    const uint64_t vbytes = ctx->vb->bytes - ctx->vb_off;
+    const uint32_t V = (uint32_t)std::min<uint64_t>(vbytes / ctx->vb_stride, 0xFFFFFFFFull);
+    ctx->vertex_count += index_count;
+    ctx->input_tris += T;
+    ctx->draws++;
+    const uint32_t draw_tag = ctx->draw_counter++;
+    if (T == 0) return MLV_OK;
+
+    mlv_depth_stencil *ds = ctx->ds;
+    if (ds->hiz_mode != ctx->depth_func) {
+        // hierarchical-Z built for the other comparison: reset to a value that rejects nothing
+        const float h = ctx->depth_func == MLV_DEPTH_LESS_EQUAL ? HUGE_VALF : -HUGE_VALF;
+        HIP_OR_FAIL(ctx, launch_fill_f32(ds->hiz, h, (size_t)ds->bins_x * ds->bins_y, ctx->stream));
+        ds->hiz_mode = ctx->depth_func;
+    }
+
+    const uint32_t tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kTile - 1) / kTile;
+    const uint32_t ntiles = tiles_x * tiles_y;
+    const uint32_t nblocks = (T + kSetupBlock - 1) / kSetupBlock;
+    const size_t clip_tris = std::max<size_t>(ctx->clip_hint, 4096);
+    const uint64_t tri_cap64 = (uint64_t)T + 6ull * clip_tris;
+    const uint32_t tri_cap = (uint32_t)std::min<uint64_t>(tri_cap64, 0xFFFFFF00ull);
+    const uint64_t rows64 = (uint64_t)V + 9ull * clip_tris;
+    const uint32_t vattr_rows = (uint32_t)std::min<uint64_t>(rows64, 0xFFFFFF00ull);
+    size_t pair_cap = std::max<size_t>(ctx->pair_hint, 3ull * T + 4ull * ntiles);
+    if (ctx->pair_limit) pair_cap = std::min(pair_cap, ctx->pair_limit);
+
+    int rc;
+    if ((rc = ensure(ctx, ctx->vpos, (size_t)std::max<uint32_t>(V, 1) * 16)) ||
+        (rc = ensure(ctx, ctx->vattr, (size_t)vattr_rows * 32)) || (rc = ensure(ctx, ctx->cnt, (size_t)T * 4)) ||
+        (rc = ensure(ctx, ctx->blocksum, (size_t)nblocks * 4)) ||
+        (rc = ensure(ctx, ctx->blockoff_tri, (size_t)nblocks * 4)) ||
+        (rc = ensure(ctx, ctx->blockoff_vtx, (size_t)nblocks * 4)) ||
+        (rc = ensure(ctx, ctx->clip_list, (size_t)T * 4)) || (rc = ensure(ctx, ctx->clip_off, (size_t)T * 8)) ||
+        (rc = ensure(ctx, ctx->tvis, (size_t)tri_cap * sizeof(TriVis))) ||
+        (rc = ensure(ctx, ctx->tshade, (size_t)tri_cap * sizeof(TriShade))) ||
+        (rc = ensure(ctx, ctx->tile_cnt, (size_t)ntiles * 4)) ||
+        (rc = ensure(ctx, ctx->tile_start, (size_t)ntiles * 4)) ||
+        (rc = ensure(ctx, ctx->tile_fill, (size_t)ntiles * 4)) ||
+        (rc = ensure(ctx, ctx->tile_list, pair_cap * 4)))
+        return rc;
+
+    DrawParams p{};
+    p.ib = (const uint32_t *)((const uint8_t *)ctx->ib->ptr + ctx->ib_off);
+    p.index_count = index_count;
+    p.start_index = start_index;
+    p.base_vertex = base_vertex;
+    p.tri_count = T;
+    p.vb = (const uint8_t *)ctx->vb->ptr + ctx->vb_off;
+    p.stride = ctx->vb_stride;
+    p.vertex_count = V;
+    p.cb = ctx->vs_cb[0] ? (const float *)ctx->vs_cb[0]->ptr : nullptr;
+    p.vs_tex = tex_desc(ctx->vs_srv[0]);
+    p.ps_tex = tex_desc(ctx->ps_srv[0]);
+    // screen_from_ndc, main.c:826-831: entries computed in double and stored as f32
+    const mlv_viewport &vp = ctx->vp;
+    float S[4][4] = {{(float)((double)vp.width * 0.5), 0.0f, 0.0f, (float)((double)vp.width * 0.5 + (double)vp.top_left_x)},
+                     {0.0f, (float)(-(double)vp.height * 0.5), 0.0f, (float)((double)vp.height * 0.5 + (double)vp.top_left_y)},
+                     {0.0f, 0.0f, vp.max_depth - vp.min_depth, vp.min_depth},
+                     {0.0f, 0.0f, 0.0f, 1.0f}};
+    memcpy(p.screen, S, sizeof S);
+    // clipper planes, main.c:652-657 via v4f32_normalize (math.h:267-270)
+    static const float raw[6][4] = {{1, 0, 0, 1}, {-1, 0, 0, 1}, {0, 1, 0, 1}, {0, -1, 0, 1}, {0, 0, 1, 1}, {0, 0, -1, 1}};
+    for (int i = 0; i < 6; ++i) {
+        float d = raw[i][0] * raw[i][0];
+        d = d + raw[i][1] * raw[i][1];
+        d = d + raw[i][2] * raw[i][2];
+        d = d + raw[i][3] * raw[i][3];
+        float len = (float)std::sqrt((double)d);
+        float ool = (float)(1.0 / (double)len);
+        for (int k = 0; k < 4; ++k) p.planes[i][k] = raw[i][k] * ool;
+    }
+    p.W = (int32_t)W;
+    p.H = (int32_t)H;
+    p.tiles_x = (int32_t)tiles_x;
+    p.tiles_y = (int32_t)tiles_y;
+    p.bins_x = (int32_t)ds->bins_x;
+    p.bins_y = (int32_t)ds->bins_y;
+    p.depth_func = ctx->depth_func;
+    // The in-draw depth cull relies on exact (unwrapped) edge values: |E| < 2^31 at every pixel.
+    p.depth_cull = (vp.top_left_x == 0.0f && vp.top_left_y == 0.0f &&
+                    2.0 * (16.0 * W + 2.0) * (16.0 * H + 2.0) < 2147483648.0 * 0.99)
+                       ? 1
+                       : 0;
+    p.draw_tag = draw_tag;
+    p.color = ctx->rt->ptr;
+    p.depth = ds->ptr;
+    p.hiz = ds->hiz;
+    p.vis_tri = ctx->vis_tri;
+    p.vis_draw = ctx->vis_draw;
+    p.vpos = (float4 *)ctx->vpos.p;
+    p.vattr = (float4 *)ctx->vattr.p;
+    p.vattr_rows = vattr_rows;
+    p.cnt = (uint32_t *)ctx->cnt.p;
+    p.blocksum = (uint32_t *)ctx->blocksum.p;
+    p.blockoff_tri = (uint32_t *)ctx->blockoff_tri.p;
+    p.blockoff_vtx = (uint32_t *)ctx->blockoff_vtx.p;
+    p.nblocks = nblocks;
+    p.clip_list = (uint32_t *)ctx->clip_list.p;
+    p.clip_off = (uint2 *)ctx->clip_off.p;
+    p.tvis = (TriVis *)ctx->tvis.p;
+    p.tshade = (TriShade *)ctx->tshade.p;
+    p.tri_cap = tri_cap;
+    p.tile_cnt = (uint32_t *)ctx->tile_cnt.p;
+    p.tile_start = (uint32_t *)ctx->tile_start.p;
+    p.tile_fill = (uint32_t *)ctx->tile_fill.p;
+    p.tile_list = (uint32_t *)ctx->tile_list.p;
+    p.pair_cap = (uint32_t)std::min<size_t>(pair_cap, 0xFFFFFFFFull);
+    p.ctr = ctx->ctr;
+    p.stats = ctx->dstats;
+    ctx->last_tri_cap = tri_cap;
+
+    HIP_OR_FAIL(ctx, launch_draw(p, ctx->vs, ctx->ps, ctx->stream));
+    return MLV_OK;
+}
+
+MLV_API int mlv_render_target_readback(mlv_context *ctx, mlv_render_target *rt, void *host_dst) {
+    if (!ctx || !rt || !host_dst) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "readback: bad argument");
+    if (int rc = set_device(ctx)) return rc;
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, rt->ptr, (size_t)rt->width * rt->height * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    return check_device_stats(ctx);
+}
+
+MLV_API int mlv_depth_stencil_readback(mlv_context *ctx, mlv_depth_stencil *ds, float *host_dst) {
+    if (!ctx || !ds || !host_dst) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "readback: bad argument");
+    if (int rc = set_device(ctx)) return rc;
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, ds->ptr, (size_t)ds->width * ds->height * 4, hipMemcpyDeviceToHost,
+                                    ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    return check_device_stats(ctx);
+}
+
+MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out) {
+    if (!ctx || !out) return MLV_ERROR_INVALID_ARGUMENT;
+    if (int rc = set_device(ctx)) return rc;
+    DevStats s{};
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(&s, ctx->dstats, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    out->vertex_count = ctx->vertex_count;
+    out->input_triangle_count = ctx->input_tris;
+    out->assembled_triangle_count = s.assembled;
+    out->active_bin_count = s.active8;
+    out->total_triangle_count_in_bins = s.pairs8;
+    out->oob_triangle_count = s.oob;
+    out->draw_count = ctx->draws;
+    out->overflow_count = s.overflow_draws;
+    return MLV_OK;
+}
+
+MLV_API int mlv_reset_stats(mlv_context *ctx) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (int rc = set_device(ctx)) return rc;
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    HIP_OR_FAIL(ctx, hipMemset(ctx->dstats, 0, sizeof(DevStats)));
+    HIP_OR_FAIL(ctx, hipDeviceSynchronize());
+    ctx->vertex_count = ctx->input_tris = ctx->draws = 0;
+    ctx->capacity_errors = 0;
+    return MLV_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,239 @@
+"""D3D11-style host API over the gfx950 C ABI.
+
+Mirrors the reference's pipeline interface (source/main.c): the `Pipeline`
+state struct with IA / VS / RS / PS / OM parts (main.c:71-115) and the three
+commands clear_render_target_view (main.c:1191), clear_depth_stencil_view
+(main.c:1204) and draw_indexed (main.c:1219). Each call goes straight to the
+C ABI of libmalevich_hip.so; errors come back as MalevichError with the
+library's status code and message.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from ._lib import MalevichError
+
+
+def _check(ctx, rc: int) -> None:
+    if rc != L.OK:
+        msg = L.load().mlv_get_last_error(ctx).decode(errors="replace") if ctx else ""
+        raise MalevichError(rc, msg)
+
+
+def device_count() -> int:
+    n = C.c_int(0)
+    L.load().mlv_device_count(C.byref(n))
+    return n.value
+
+
+class Buffer:
+    def __init__(self, dev: "Device", handle: int, nbytes: int, kind: int):
+        self.dev, self.handle, self.nbytes, self.kind = dev, handle, nbytes, kind
+
+    @property
+    def device_ptr(self) -> int:
+        return L.load().mlv_buffer_device_ptr(self.handle)
+
+    def update(self, data, offset: int = 0) -> None:
+        a = np.ascontiguousarray(data)
+        _check(self.dev.ctx, L.load().mlv_buffer_update(self.dev.ctx, self.handle, offset, a.nbytes, a.ctypes.data))
+
+    def read(self, dtype=np.uint8, count: int | None = None, offset: int = 0) -> np.ndarray:
+        dt = np.dtype(dtype)
+        n = (self.nbytes - offset) // dt.itemsize if count is None else count
+        out = np.empty(n, dt)
+        _check(self.dev.ctx, L.load().mlv_buffer_readback(self.dev.ctx, self.handle, offset, out.nbytes,
+                                                          out.ctypes.data))
+        return out
+
+    def release(self) -> None:
+        if self.handle:
+            L.load().mlv_buffer_destroy(self.dev.ctx, self.handle)
+            self.handle = None
+
+
+class Texture2D:
+    def __init__(self, dev, handle, width, height, fmt):
+        self.dev, self.handle, self.width, self.height, self.format = dev, handle, width, height, fmt
+
+    def release(self) -> None:
+        if self.handle:
+            L.load().mlv_texture2d_destroy(self.dev.ctx, self.handle)
+            self.handle = None
+
+
+class RenderTarget:
+    def __init__(self, dev, handle, width, height):
+        self.dev, self.handle, self.width, self.height = dev, handle, width, height
+
+    @property
+    def device_ptr(self) -> int:
+        return L.load().mlv_render_target_device_ptr(self.handle)
+
+    def read(self) -> np.ndarray:
+        out = np.empty((self.height, self.width), np.uint32)
+        _check(self.dev.ctx, L.load().mlv_render_target_readback(self.dev.ctx, self.handle, out.ctypes.data))
+        return out
+
+    def release(self) -> None:
+        if self.handle:
+            L.load().mlv_render_target_destroy(self.dev.ctx, self.handle)
+            self.handle = None
+
+
+class DepthStencil:
+    def __init__(self, dev, handle, width, height):
+        self.dev, self.handle, self.width, self.height = dev, handle, width, height
+
+    @property
+    def device_ptr(self) -> int:
+        return L.load().mlv_depth_stencil_device_ptr(self.handle)
+
+    def read(self) -> np.ndarray:
+        out = np.empty((self.height, self.width), np.float32)
+        _check(self.dev.ctx, L.load().mlv_depth_stencil_readback(self.dev.ctx, self.handle, out.ctypes.data))
+        return out
+
+    def release(self) -> None:
+        if self.handle:
+            L.load().mlv_depth_stencil_destroy(self.dev.ctx, self.handle)
+            self.handle = None
+
+
+class Device:
+    """One mlv_context: the pipeline state plus one HIP stream on one GPU."""
+
+    def __init__(self, device: int = 0):
+        self.lib = L.load()
+        h = C.c_void_p()
+        rc = self.lib.mlv_context_create(device, C.byref(h))
+        if rc != L.OK:
+            raise MalevichError(rc, "mlv_context_create failed (no usable GPU?)")
+        self.ctx = h.value
+
+    def close(self) -> None:
+        if self.ctx:
+            self.lib.mlv_context_destroy(self.ctx)
+            self.ctx = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # ---------------------------------------------------------- resources
+    def create_buffer(self, kind: int, data=None, nbytes: int | None = None) -> Buffer:
+        a = None if data is None else np.ascontiguousarray(data)
+        n = a.nbytes if a is not None else int(nbytes)
+        h = C.c_void_p()
+        _check(self.ctx, self.lib.mlv_buffer_create(self.ctx, kind, n, None if a is None else a.ctypes.data,
+                                                    C.byref(h)))
+        return Buffer(self, h.value, n, kind)
+
+    def wrap_buffer(self, kind: int, device_ptr: int, nbytes: int) -> Buffer:
+        h = C.c_void_p()
+        _check(self.ctx, self.lib.mlv_buffer_wrap(self.ctx, kind, device_ptr, nbytes, C.byref(h)))
+        return Buffer(self, h.value, nbytes, kind)
+
+    def create_texture2d(self, data: np.ndarray, fmt: int | None = None, mip_levels: int = 1) -> Texture2D:
+        """data: (h, w) uint32 RGBA8 (R in the low byte), (h, w, 4) float32 or float16."""
+        a = np.ascontiguousarray(data)
+        if fmt is None:
+            fmt = {np.dtype(np.uint32): L.FORMAT_RGBA8_UNORM, np.dtype(np.float32): L.FORMAT_RGBA32F,
+                   np.dtype(np.float16): L.FORMAT_RGBA16F}[a.dtype]
+        h_, w_ = a.shape[:2]
+        h = C.c_void_p()
+        _check(self.ctx, self.lib.mlv_texture2d_create(self.ctx, w_, h_, fmt, mip_levels, a.ctypes.data,
+                                                       C.byref(h)))
+        return Texture2D(self, h.value, w_, h_, fmt)
+
+    def create_render_target(self, width: int, height: int) -> RenderTarget:
+        h = C.c_void_p()
+        _check(self.ctx, self.lib.mlv_render_target_create(self.ctx, width, height, C.byref(h)))
+        return RenderTarget(self, h.value, width, height)
+
+    def create_depth_stencil(self, width: int, height: int) -> DepthStencil:
+        h = C.c_void_p()
+        _check(self.ctx, self.lib.mlv_depth_stencil_create(self.ctx, width, height, C.byref(h)))
+        return DepthStencil(self, h.value, width, height)
+
+    # ---------------------------------------------------------------- IA
+    def ia_set_index_buffer(self, buf: Buffer | None, offset: int = 0) -> None:
+        _check(self.ctx, self.lib.mlv_ia_set_index_buffer(self.ctx, buf.handle if buf else None, offset))
+
+    def ia_set_vertex_buffer(self, buf: Buffer | None, stride: int, offset: int = 0) -> None:
+        _check(self.ctx, self.lib.mlv_ia_set_vertex_buffer(self.ctx, buf.handle if buf else None, stride, offset))
+
+    def ia_set_primitive_topology(self, topology: int = L.TOPOLOGY_TRIANGLELIST) -> None:
+        _check(self.ctx, self.lib.mlv_ia_set_primitive_topology(self.ctx, topology))
+
+    # ---------------------------------------------------------------- VS
+    def vs_set_shader(self, vs: int) -> None:
+        _check(self.ctx, self.lib.mlv_vs_set_shader(self.ctx, vs))
+
+    def vs_set_constant_buffer(self, slot: int, buf: Buffer | None) -> None:
+        _check(self.ctx, self.lib.mlv_vs_set_constant_buffer(self.ctx, slot, buf.handle if buf else None))
+
+    def vs_set_shader_resource(self, slot: int, tex: Texture2D | None) -> None:
+        _check(self.ctx, self.lib.mlv_vs_set_shader_resource(self.ctx, slot, tex.handle if tex else None))
+
+    # ---------------------------------------------------------------- RS
+    def rs_set_viewport(self, top_left_x=0.0, top_left_y=0.0, width=0.0, height=0.0, min_depth=0.0,
+                        max_depth=1.0) -> None:
+        vp = L.Viewport(top_left_x, top_left_y, width, height, min_depth, max_depth)
+        _check(self.ctx, self.lib.mlv_rs_set_viewport(self.ctx, C.byref(vp)))
+
+    # ---------------------------------------------------------------- PS
+    def ps_set_shader(self, ps: int) -> None:
+        _check(self.ctx, self.lib.mlv_ps_set_shader(self.ctx, ps))
+
+    def ps_set_shader_resource(self, slot: int, tex: Texture2D | None) -> None:
+        _check(self.ctx, self.lib.mlv_ps_set_shader_resource(self.ctx, slot, tex.handle if tex else None))
+
+    # ---------------------------------------------------------------- OM
+    def om_set_render_targets(self, rt: RenderTarget | None, ds: DepthStencil | None) -> None:
+        _check(self.ctx, self.lib.mlv_om_set_render_targets(self.ctx, rt.handle if rt else None,
+                                                            ds.handle if ds else None))
+
+    def om_set_depth_func(self, func: int) -> None:
+        _check(self.ctx, self.lib.mlv_om_set_depth_func(self.ctx, func))
+
+    def om_set_visibility_buffer(self, tri_ids: Buffer | None, draw_tags: Buffer | None) -> None:
+        _check(self.ctx, self.lib.mlv_om_set_visibility_buffer(
+            self.ctx, tri_ids.device_ptr if tri_ids else None, draw_tags.device_ptr if draw_tags else None))
+
+    # ---------------------------------------------------------- commands
+    def clear_render_target_view(self, rt: RenderTarget, rgba) -> None:
+        c = (C.c_float * 4)(*[float(x) for x in rgba])
+        _check(self.ctx, self.lib.mlv_clear_render_target_view(self.ctx, rt.handle, c))
+
+    def clear_depth_stencil_view(self, ds: DepthStencil, depth: float) -> None:
+        _check(self.ctx, self.lib.mlv_clear_depth_stencil_view(self.ctx, ds.handle, float(depth)))
+
+    def draw_indexed(self, index_count: int, start_index: int = 0, base_vertex: int = 0) -> None:
+        _check(self.ctx, self.lib.mlv_draw_indexed(self.ctx, index_count, start_index, base_vertex))
+
+    def synchronize(self) -> None:
+        _check(self.ctx, self.lib.mlv_synchronize(self.ctx))
+
+    @property
+    def stream(self) -> int:
+        return self.lib.mlv_context_stream(self.ctx)
+
+    def stats(self) -> dict:
+        s = L.Stats()
+        _check(self.ctx, self.lib.mlv_get_stats(self.ctx, C.byref(s)))
+        return {k: getattr(s, k) for k, _ in L.Stats._fields_}
+
+    def reset_stats(self) -> None:
+        _check(self.ctx, self.lib.mlv_reset_stats(self.ctx))

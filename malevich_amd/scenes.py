@@ -1,0 +1,312 @@
+"""Seeded synthetic scenes for the benchmark configurations and parity tests.
+
+Camera conventions follow the reference (source/main.c:1422-1477): view space
+is left-handed with +y up and +z forward, world space is right-handed with
++z up, and the projection is reversed-Z and infinite (main.c:1431-1441), so
+z_ndc = near / z_view. The constant buffer is PerFrameCB (main.c:169-173):
+clip_from_world, view_from_clip, world_from_view, row-major float32.
+
+Vertex layouts are the reference's input layouts:
+  basic_vs / vertex_lighting_vs: pos[3], normal[3], uv[2]  (basic_vs.c:3-7), 32 B
+  passthrough_vs / fullscreen_vs: pos[4], color[3], pad    (main.c:208-212), 32 B
+Triangles are wound so that the reference's back-face rule (cull when the
+screen-space signed area is > 0, main.c:856) keeps the faces seen from outside.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+# ----------------------------------------------------------------- camera
+
+
+def perspective_reversed_z(fov_y_deg: float, aspect: float, near: float) -> np.ndarray:
+    """clip_from_view of main.c:1431-1441 (left-handed, reversed-Z, infinite)."""
+    sy = 1.0 / math.tan(math.radians(fov_y_deg) / 2.0)
+    sx = sy / aspect
+    return np.array([[sx, 0, 0, 0], [0, sy, 0, 0], [0, 0, 0, near], [0, 0, 1, 0]], np.float64)
+
+
+def look_at_world_from_view(eye, target, up=(0.0, 0.0, 1.0)) -> np.ndarray:
+    """world_from_view whose columns are (right, up, forward, eye) in world space.
+
+    With eye looking down -x and z up this equals the reference's
+    change_of_basis (main.c:1459-1464): view +z -> world -x, view +x -> world +y,
+    view +y -> world +z.
+    """
+    eye = np.asarray(eye, np.float64)
+    f = np.asarray(target, np.float64) - eye
+    f /= np.linalg.norm(f)
+    r = np.cross(f, np.asarray(up, np.float64))
+    if np.linalg.norm(r) < 1e-9:
+        r = np.cross(f, np.array([0.0, 1.0, 0.0]))
+    r /= np.linalg.norm(r)
+    u = np.cross(r, f)
+    m = np.eye(4)
+    m[:3, 0], m[:3, 1], m[:3, 2], m[:3, 3] = r, u, f, eye
+    return m
+
+
+def per_frame_cb(world_from_view: np.ndarray, clip_from_view: np.ndarray) -> np.ndarray:
+    """PerFrameCB (main.c:169-173) as 48 float32: clip_from_world, view_from_clip, world_from_view."""
+    view_from_world = np.linalg.inv(world_from_view)
+    clip_from_world = clip_from_view @ view_from_world
+    view_from_clip = np.linalg.inv(clip_from_view)
+    return np.concatenate([clip_from_world.ravel(), view_from_clip.ravel(), world_from_view.ravel()]).astype(
+        np.float32)
+
+
+def camera_cb(eye, target, fov_y_deg: float, width: int, height: int, near: float = 0.01) -> np.ndarray:
+    return per_frame_cb(look_at_world_from_view(eye, target), perspective_reversed_z(fov_y_deg, width / height, near))
+
+
+# ------------------------------------------------------------------ meshes
+
+
+def geodesic_sphere(freq: int) -> tuple[np.ndarray, np.ndarray]:
+    """Class-I geodesic sphere: each icosahedron face split into freq^2 triangles.
+
+    Returns unit positions (V,3) float64 and faces (F,3) int64 wound
+    counter-clockwise about the outward normal. V = 10*freq^2 + 2, F = 20*freq^2
+    (freq = 32 gives 10,242 vertices and 20,480 triangles).
+    """
+    t = (1.0 + 5 ** 0.5) / 2.0
+    base = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t],
+                     [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
+    base /= np.linalg.norm(base, axis=1, keepdims=True)
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
+             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
+             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    n = int(freq)
+    key_to_idx: dict = {}
+    verts: list = []
+
+    def vid(a: int, b: int, c: int, i: int, j: int) -> int:
+        # barycentric lattice point (n-i-j, i, j) of face (a,b,c); keyed by the sorted
+        # corner weights so points on shared edges and corners get one index
+        w = {a: n - i - j, b: i, c: j}
+        key = tuple(sorted((k, v) for k, v in w.items() if v))
+        idx = key_to_idx.get(key)
+        if idx is None:
+            p = (base[a] * (n - i - j) + base[b] * i + base[c] * j) / n
+            idx = len(verts)
+            verts.append(p / np.linalg.norm(p))
+            key_to_idx[key] = idx
+        return idx
+
+    out = []
+    for a, b, c in faces:
+        for i in range(n):
+            for j in range(n - i):
+                v0, v1, v2 = vid(a, b, c, i, j), vid(a, b, c, i + 1, j), vid(a, b, c, i, j + 1)
+                out.append((v0, v1, v2))
+                if i + j < n - 1:
+                    v3 = vid(a, b, c, i + 1, j + 1)
+                    out.append((v1, v3, v2))
+    pos = np.array(verts)
+    f = np.array(out, np.int64)
+    # orient every face counter-clockwise about the outward normal
+    nrm = np.cross(pos[f[:, 1]] - pos[f[:, 0]], pos[f[:, 2]] - pos[f[:, 0]])
+    flip = (nrm * pos[f].mean(axis=1)).sum(1) < 0
+    f[flip] = f[flip][:, [0, 2, 1]]
+    return pos, f
+
+
+def value_noise3(p: np.ndarray, seed: int, octaves: int = 3, base_freq: float = 2.0) -> np.ndarray:
+    """Smooth seeded value noise in [-1, 1] at points p (N,3)."""
+    rng = np.random.default_rng(seed)
+    table = rng.uniform(-1.0, 1.0, 4096)
+    perm = rng.permutation(4096)
+    total = np.zeros(len(p))
+    amp, freq, norm = 1.0, base_freq, 0.0
+    for _ in range(octaves):
+        q = p * freq
+        i0 = np.floor(q).astype(np.int64)
+        f = q - i0
+        f = f * f * (3 - 2 * f)
+        acc = np.zeros(len(p))
+        for dx in (0, 1):
+            for dy in (0, 1):
+                for dz in (0, 1):
+                    h = perm[(i0[:, 0] + dx) % 4096]
+                    h = perm[(h + i0[:, 1] + dy) % 4096]
+                    h = perm[(h + i0[:, 2] + dz) % 4096]
+                    wgt = (f[:, 0] if dx else 1 - f[:, 0]) * (f[:, 1] if dy else 1 - f[:, 1]) * (
+                        f[:, 2] if dz else 1 - f[:, 2])
+                    acc += wgt * table[h]
+        total += amp * acc
+        norm += amp
+        amp *= 0.5
+        freq *= 2.0
+    return total / norm
+
+
+def sphere_uv(p: np.ndarray) -> np.ndarray:
+    u = np.arctan2(p[:, 1], p[:, 0]) / (2 * np.pi) + 0.5
+    v = np.arccos(np.clip(p[:, 2], -1, 1)) / np.pi
+    return np.stack([u, 1.0 - v], 1)
+
+
+def vertex_normals(pos: np.ndarray, faces: np.ndarray) -> np.ndarray:
+    fn = np.cross(pos[faces[:, 1]] - pos[faces[:, 0]], pos[faces[:, 2]] - pos[faces[:, 0]])
+    n = np.zeros_like(pos)
+    for k in range(3):
+        np.add.at(n, faces[:, k], fn)
+    return n / np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
+
+
+@dataclass
+class Mesh:
+    vertices: np.ndarray   # (V, 8) float32: pos3, normal3, uv2
+    indices: np.ndarray    # (F*3,) uint32
+
+    @property
+    def triangle_count(self) -> int:
+        return self.indices.size // 3
+
+
+def sphere_mesh(freq: int, displace: float = 0.0, seed: int = 0, transform: np.ndarray | None = None) -> Mesh:
+    pos, faces = geodesic_sphere(freq)
+    uv = sphere_uv(pos)
+    if displace:
+        pos = pos * (1.0 + displace * value_noise3(pos, seed))[:, None]
+    nrm = vertex_normals(pos, faces) if displace else pos.copy()
+    if transform is not None:
+        pos = pos @ transform[:3, :3].T + transform[:3, 3]
+        nrm = nrm @ np.linalg.inv(transform[:3, :3])
+        nrm /= np.linalg.norm(nrm, axis=1, keepdims=True)
+    v = np.concatenate([pos, nrm, uv], 1).astype(np.float32)
+    return Mesh(v, faces.astype(np.uint32).ravel())
+
+
+def reverse_winding(indices: np.ndarray) -> np.ndarray:
+    return indices.reshape(-1, 3)[:, [0, 2, 1]].ravel().copy()
+
+
+# ----------------------------------------------------------------- textures
+
+
+def random_rgba8(size: int, seed: int) -> np.ndarray:
+    """(size, size) uint32 RGBA8 with uniform random bytes (R in the low byte)."""
+    return np.random.default_rng(seed).integers(0, 2 ** 32, (size, size), dtype=np.uint64).astype(np.uint32)
+
+
+def smooth_env_map(width: int, height: int, seed: int, lo: float = 0.05, hi: float = 2.0) -> np.ndarray:
+    """(h, w, 4) float32 equirect map of seeded smooth noise, alpha 1 (range like the
+    reference's irradiance asset, Appendix of the asset notes: about [0.085, 2.11])."""
+    yy, xx = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
+    theta = (yy + 0.5) / height * np.pi
+    phi = (xx + 0.5) / width * 2 * np.pi
+    d = np.stack([np.sin(theta) * np.cos(phi), np.sin(theta) * np.sin(phi), np.cos(theta)], -1).reshape(-1, 3)
+    out = np.empty((height * width, 4), np.float64)
+    for c in range(3):
+        out[:, c] = lo + (hi - lo) * 0.5 * (1.0 + value_noise3(d, seed * 7 + c, octaves=4, base_freq=1.5))
+    out[:, 3] = 1.0
+    return out.reshape(height, width, 4).astype(np.float32)
+
+
+# ------------------------------------------------------------------ scenes
+
+
+# Suprematism scene, the reference's built-in asset-free draw (main.c:232-254):
+# pos4 + colour3 + pad per vertex, passthrough_vs / passthrough_ps, 24 indices.
+SUPREMATISM_VERTICES = np.array([
+    [0.34107, 0.12215, 0.5, 1.0, 0.07500, 0.08200, 0.06300, 0.0],
+    [0.95357, 0.12500, 0.5, 1.0, 0.07500, 0.08200, 0.06300, 0.0],
+    [0.96250, 0.86931, 0.5, 1.0, 0.07500, 0.08200, 0.06300, 0.0],
+    [0.33928, 0.86505, 0.5, 1.0, 0.07500, 0.08200, 0.06300, 0.0],
+    [0.09464, 0.12500, 0.75, 1.0, 0.14100, 0.29000, 0.60800, 0.0],
+    [0.69285, 0.39772, 0.75, 1.0, 0.14100, 0.29000, 0.60800, 0.0],
+    [0.09107, 0.60937, 0.75, 1.0, 0.14100, 0.29000, 0.60800, 0.0],
+    [0.00000, 0.00000, 0.25, 1.0, 0.96100, 0.96100, 0.92900, 0.0],
+    [1.00000, 0.00000, 0.25, 1.0, 0.96100, 0.96100, 0.92900, 0.0],
+    [1.00000, 1.00000, 0.25, 1.0, 0.96100, 0.96100, 0.92900, 0.0],
+    [0.00000, 1.00000, 0.25, 1.0, 0.96100, 0.96100, 0.92900, 0.0],
+], np.float32)
+SUPREMATISM_INDICES = np.array([0, 1, 2, 2, 3, 0, 4, 5, 6, 7, 8, 9, 9, 10, 7, 0, 0, 0, 0, 0, 0, 0, 0, 0], np.uint32)
+
+# Fullscreen quad (main.c:255-270): drawn with fullscreen_vs / env_lighting_ps.
+FULLSCREEN_VERTICES = np.array([
+    [0.0, 0.0, 0.0, 1.0, 0, 0, 0, 0], [1.0, 0.0, 0.0, 1.0, 0, 0, 0, 0],
+    [1.0, 1.0, 0.0, 1.0, 0, 0, 0, 0], [0.0, 1.0, 0.0, 1.0, 0, 0, 0, 0]], np.float32)
+FULLSCREEN_INDICES = np.array([0, 1, 2, 2, 3, 0] + [0] * 18, np.uint32)
+
+# render()'s clear colour (main.c:1271)
+CLEAR_COLOR = (227 / 255, 223 / 255, 216 / 255, 0.0)
+
+
+@dataclass
+class ScreenTriangles:
+    """Passthrough-layout triangle soup (de-indexed): vertices (3N, 8), indices arange."""
+    vertices: np.ndarray
+    indices: np.ndarray
+    big_mask: np.ndarray = field(default_factory=lambda: np.zeros(0, bool))
+
+
+def _ndc_from_pixels(px: np.ndarray, py: np.ndarray, width: int, height: int) -> tuple[np.ndarray, np.ndarray]:
+    # passthrough_vs maps input x to 2x-1; the viewport maps NDC to pixels with y flipped (main.c:826-831)
+    return px / width, 1.0 - py / height
+
+
+def binning_stress(n_tiny: int, n_big: int, width: int, height: int, seed: int,
+                   big_every: int = 10000) -> ScreenTriangles:
+    """Tiny random triangles (area U[1,4] px^2, uniform centroids and depths) with
+    screen-covering triangles interleaved every `big_every` in submission order."""
+    rng = np.random.default_rng(seed)
+    total = n_tiny + n_big
+    big = np.zeros(total, bool)
+    if n_big:
+        pos_big = np.linspace(big_every // 2, total - 1, n_big).astype(np.int64) if n_tiny else np.arange(n_big)
+        pos_big = np.minimum(pos_big, total - 1)
+        big[np.unique(pos_big)] = True
+        while big.sum() < n_big:  # unique() may merge positions in tiny cases
+            big[rng.integers(0, total)] = True
+    nt = int((~big).sum())
+    # tiny: centroid + random orientation; equilateral-ish with area A
+    cx = rng.uniform(0, width, nt)
+    cy = rng.uniform(0, height, nt)
+    area = rng.uniform(1.0, 4.0, nt)
+    r = np.sqrt(4 * area / (3 * np.sqrt(3)))  # circumradius of an equilateral triangle of that area
+    ang = rng.uniform(0, 2 * np.pi, nt)
+    k = np.arange(3)[None, :] * (2 * np.pi / 3)
+    tx = cx[:, None] + r[:, None] * np.cos(ang[:, None] + k)
+    ty = cy[:, None] + r[:, None] * np.sin(ang[:, None] + k)
+    # screen y is down: reference keeps signed_area <= 0 (main.c:851-856)
+    sa = (tx[:, 1] - tx[:, 0]) * (ty[:, 2] - ty[:, 0]) - (tx[:, 2] - tx[:, 0]) * (ty[:, 1] - ty[:, 0])
+    sw = sa > 0
+    tx[sw] = tx[sw][:, [0, 2, 1]]
+    ty[sw] = ty[sw][:, [0, 2, 1]]
+    zt = 0.01 / rng.uniform(0.1, 100.0, nt)
+    # big: a triangle covering the whole screen with a margin
+    nb = int(big.sum())
+    bx = np.array([-0.5 * width, 2.5 * width, -0.5 * width])[None].repeat(nb, 0)
+    by = np.array([-0.5 * height, -0.5 * height, 2.5 * height])[None].repeat(nb, 0)
+    sa = (bx[:, 1] - bx[:, 0]) * (by[:, 2] - by[:, 0]) - (bx[:, 2] - bx[:, 0]) * (by[:, 1] - by[:, 0])
+    if (sa > 0).any():
+        bx, by = bx[:, [0, 2, 1]], by[:, [0, 2, 1]]
+    zb = 0.01 / rng.uniform(0.1, 100.0, nb)
+    X = np.empty((total, 3))
+    Y = np.empty((total, 3))
+    Z = np.empty(total)
+    X[~big], Y[~big], Z[~big] = tx, ty, zt
+    X[big], Y[big], Z[big] = bx, by, zb
+    ix, iy = _ndc_from_pixels(X, Y, width, height)
+    col = rng.uniform(0.0, 1.0, (total, 3))
+    v = np.zeros((total, 3, 8), np.float32)
+    v[:, :, 0] = ix
+    v[:, :, 1] = iy
+    v[:, :, 2] = Z[:, None]
+    v[:, :, 3] = 1.0
+    v[:, :, 4:7] = col[:, None, :]
+    return ScreenTriangles(v.reshape(-1, 8), np.arange(total * 3, dtype=np.uint32), big)
+
+
+def c1_scene(width: int = 640, height: int = 360, seed: int = 1):
+    """sphere of 10,242 vertices / 20,480 triangles, camera at distance 3,
+    60 deg vertical FOV, 512x512 RGBA8 random texture, basic_vs + basic_ps."""
+    mesh = sphere_mesh(32)
+    cb = camera_cb(eye=(3.0, 0.0, 0.0), target=(0.0, 0.0, 0.0), fov_y_deg=60.0, width=width, height=height)
+    tex = random_rgba8(512, seed)
+    return mesh, cb, tex

@@ -1,0 +1,240 @@
+"""ctypes binding of the CPU oracle -- TEST INFRASTRUCTURE ONLY.
+
+Loads oracle/liboracle.so (built by oracle/Makefile). Only tests/,
+__graft_entry__.smoke() and bench.py's cpu_baseline leg may import this
+module; the product path never does.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIB_PATH = HERE / "liboracle.so"
+
+VS_BASIC, VS_VERTEX_LIGHTING, VS_PASSTHROUGH, VS_FULLSCREEN = 0, 1, 2, 3
+PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH = 0, 1, 2
+FMT_RGBA8, FMT_RGBA32F = 0, 1
+DEPTH_GE, DEPTH_LE = 0, 1
+
+
+class Texture(C.Structure):
+    _fields_ = [("data", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32)]
+
+
+class Viewport(C.Structure):
+    _fields_ = [(n, C.c_float) for n in ("top_left_x", "top_left_y", "width", "height", "min_depth", "max_depth")]
+
+
+class DrawDesc(C.Structure):
+    _fields_ = [
+        ("index_buffer", C.c_void_p),
+        ("index_count", C.c_uint32),
+        ("start_index", C.c_uint32),
+        ("base_vertex", C.c_int32),
+        ("vertex_buffer", C.c_void_p),
+        ("vertex_stride", C.c_uint32),
+        ("vertex_count", C.c_uint32),
+        ("vs", C.c_int32),
+        ("ps", C.c_int32),
+        ("cb", C.c_void_p),
+        ("vs_srv0", Texture),
+        ("ps_srv0", Texture),
+        ("viewport", Viewport),
+        ("depth_func", C.c_int32),
+    ]
+
+
+class Target(C.Structure):
+    _fields_ = [
+        ("colors", C.c_void_p),
+        ("depth", C.c_void_p),
+        ("hiz", C.c_void_p),
+        ("vis_tri", C.c_void_p),
+        ("vis_draw", C.c_void_p),
+        ("width", C.c_uint32),
+        ("height", C.c_uint32),
+        ("draw_counter", C.c_uint32),
+    ]
+
+
+class Stats(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("vertex_count", "input_triangle_count", "assembled_triangle_count",
+                                           "active_bin_count", "total_triangle_count_in_bins",
+                                           "oob_triangle_count")]
+
+
+class Triangle(C.Structure):
+    _fields_ = [
+        ("edge", (C.c_int32 * 3) * 3),
+        ("rw", C.c_float * 3),
+        ("one_over_area", C.c_float),
+        ("max_depth", C.c_float),
+        ("min_bounds", C.c_int32 * 2),
+        ("max_bounds", C.c_int32 * 2),
+        ("prim", C.c_uint32),
+        ("attr", (C.c_float * 12) * 3),
+    ]
+
+
+_lib = None
+
+
+def lib() -> C.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIB_PATH.exists():
+            raise RuntimeError(f"{LIB_PATH} is missing; run `make -C oracle` (or __graft_entry__.build())")
+        L = C.CDLL(str(LIB_PATH))
+        P = C.POINTER
+        L.orc_clear_render_target_view.argtypes = [P(Target), P(C.c_float)]
+        L.orc_clear_depth_stencil_view.argtypes = [P(Target), C.c_float, C.c_int32]
+        L.orc_draw_indexed.argtypes = [P(Target), P(DrawDesc), P(Stats)]
+        L.orc_vertex_shader.argtypes = [P(DrawDesc), C.c_uint32, P(C.c_float)]
+        L.orc_primitive_assembly.argtypes = [P(DrawDesc), P(Triangle), C.c_uint32, P(C.c_uint32), P(C.c_uint64)]
+        L.orc_coverage_mask.argtypes = [P(C.c_int32), C.c_int32, C.c_int32]
+        L.orc_coverage_mask.restype = C.c_uint64
+        L.orc_bilinear.argtypes = [P(Texture), C.c_float, C.c_float, P(C.c_float)]
+        L.orc_latlon.argtypes = [P(Texture), P(C.c_float), P(C.c_float)]
+        L.orc_encode_output.argtypes = [P(C.c_float)]
+        L.orc_encode_output.restype = C.c_uint32
+        L.orc_encode_clear.argtypes = [P(C.c_float)]
+        L.orc_encode_clear.restype = C.c_uint32
+        L.orc_srgb_from_linear_approx.argtypes = [C.c_float]
+        L.orc_srgb_from_linear_approx.restype = C.c_float
+        L.orc_tonemap.argtypes = [C.c_float]
+        L.orc_tonemap.restype = C.c_float
+        L.orc_normalize3.argtypes = [P(C.c_float), P(C.c_float)]
+        L.orc_cvt_rne.argtypes = [C.c_float]
+        L.orc_cvt_rne.restype = C.c_int32
+        L.orc_clip_planes.argtypes = [P(C.c_float)]
+        L.orc_clip_polygon.argtypes = [P(C.c_float), C.c_int, P(C.c_float)]
+        L.orc_clip_polygon.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def _ptr(a: np.ndarray | None) -> int | None:
+    return None if a is None else a.ctypes.data
+
+
+def make_texture(arr: np.ndarray | None) -> Texture:
+    """arr: (h, w) uint32 RGBA8 or (h, w, 4) float32."""
+    if arr is None:
+        return Texture(None, 0, 0, 0)
+    arr = np.ascontiguousarray(arr)
+    if arr.dtype == np.uint32:
+        h, w = arr.shape
+        return Texture(arr.ctypes.data, w, h, FMT_RGBA8)
+    assert arr.dtype == np.float32 and arr.ndim == 3 and arr.shape[2] == 4
+    h, w, _ = arr.shape
+    return Texture(arr.ctypes.data, w, h, FMT_RGBA32F)
+
+
+class RenderTarget:
+    """Colour + depth + 8x8 tile minima (+ optional visibility) in host memory."""
+
+    def __init__(self, width: int, height: int, with_vis: bool = True):
+        self.width, self.height = width, height
+        self.colors = np.zeros((height, width), np.uint32)
+        self.depth = np.zeros((height, width), np.float32)
+        self.hiz = np.zeros(((height // 8) * (width // 8),), np.float32)
+        self.vis_tri = np.full((height, width), 0xFFFFFFFF, np.uint32) if with_vis else None
+        self.vis_draw = np.full((height, width), 0xFFFFFFFF, np.uint32) if with_vis else None
+        self.t = Target(self.colors.ctypes.data, self.depth.ctypes.data, self.hiz.ctypes.data,
+                        _ptr(self.vis_tri), _ptr(self.vis_draw), width, height, 0)
+        self.stats = Stats()
+
+    def clear_render_target_view(self, rgba) -> None:
+        c = (C.c_float * 4)(*rgba)
+        rc = lib().orc_clear_render_target_view(C.byref(self.t), c)
+        assert rc == 0, rc
+
+    def clear_depth_stencil_view(self, depth: float, depth_func: int = DEPTH_GE) -> None:
+        rc = lib().orc_clear_depth_stencil_view(C.byref(self.t), C.c_float(depth), depth_func)
+        assert rc == 0, rc
+
+
+class Draw:
+    """Keeps the numpy inputs alive for the lifetime of the descriptor."""
+
+    def __init__(self, indices, vertices, vs, ps, cb=None, vs_tex=None, ps_tex=None, viewport=None,
+                 start_index=0, base_vertex=0, index_count=None, depth_func=DEPTH_GE):
+        self.indices = np.ascontiguousarray(indices, np.uint32).ravel()
+        self.vertices = np.ascontiguousarray(vertices, np.float32)
+        assert self.vertices.ndim == 2
+        self.cb = np.ascontiguousarray(cb if cb is not None else np.zeros(48), np.float32).ravel()
+        assert self.cb.size == 48
+        self.vs_tex_arr = None if vs_tex is None else np.ascontiguousarray(vs_tex)
+        self.ps_tex_arr = None if ps_tex is None else np.ascontiguousarray(ps_tex)
+        n = self.indices.size if index_count is None else index_count
+        self.desc = DrawDesc(
+            self.indices.ctypes.data, n, start_index, base_vertex,
+            self.vertices.ctypes.data, self.vertices.shape[1] * 4, self.vertices.shape[0],
+            vs, ps, self.cb.ctypes.data, make_texture(self.vs_tex_arr), make_texture(self.ps_tex_arr),
+            Viewport(*(viewport if viewport is not None else (0, 0, 0, 0, 0, 1))), depth_func)
+
+
+def draw_indexed(rt: RenderTarget, draw: Draw) -> None:
+    rc = lib().orc_draw_indexed(C.byref(rt.t), C.byref(draw.desc), C.byref(rt.stats))
+    if rc != 0:
+        raise RuntimeError(f"orc_draw_indexed failed: {rc}")
+
+
+def primitive_assembly(draw: Draw, capacity: int | None = None) -> list:
+    """Runs VS + primitive assembly; returns a structured array of Triangle records."""
+    T = draw.desc.index_count // 3
+    cap = capacity or (T + max(2 * T, 512))
+    buf = (Triangle * cap)()
+    n = C.c_uint32(0)
+    oob = C.c_uint64(0)
+    rc = lib().orc_primitive_assembly(C.byref(draw.desc), buf, cap, C.byref(n), C.byref(oob))
+    if rc != 0:
+        raise RuntimeError(f"orc_primitive_assembly failed: {rc}")
+    return list(buf[: n.value])
+
+
+def coverage_mask(edges, ox: int, oy: int) -> int:
+    e = (C.c_int32 * 9)(*np.asarray(edges, np.int64).astype(np.int32).ravel().tolist())
+    return int(lib().orc_coverage_mask(e, ox, oy))
+
+
+def bilinear(tex: np.ndarray, u: float, v: float) -> np.ndarray:
+    t = make_texture(tex)
+    out = (C.c_float * 4)()
+    lib().orc_bilinear(C.byref(t), C.c_float(u), C.c_float(v), out)
+    return np.array(out[:], np.float32)
+
+
+def latlon(tex: np.ndarray, d) -> np.ndarray:
+    t = make_texture(tex)
+    dd = (C.c_float * 3)(*[float(x) for x in d])
+    out = (C.c_float * 4)()
+    lib().orc_latlon(C.byref(t), dd, out)
+    return np.array(out[:], np.float32)
+
+
+def encode_output(rgb) -> int:
+    return int(lib().orc_encode_output((C.c_float * 3)(*[float(x) for x in rgb])))
+
+
+def encode_clear(rgba) -> int:
+    return int(lib().orc_encode_clear((C.c_float * 4)(*[float(x) for x in rgba])))
+
+
+def clip_polygon(vertices: np.ndarray) -> np.ndarray:
+    v = np.ascontiguousarray(vertices, np.float32)
+    n = v.shape[0]
+    out = np.zeros((16, 12), np.float32)
+    k = lib().orc_clip_polygon(v.ctypes.data_as(C.POINTER(C.c_float)), n, out.ctypes.data_as(C.POINTER(C.c_float)))
+    if k < 0:
+        raise RuntimeError("clip polygon overflow")
+    return out[:k].copy()
+
+
+def clip_planes() -> np.ndarray:
+    out = np.zeros((6, 4), np.float32)
+    lib().orc_clip_planes(out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
