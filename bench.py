@@ -1,0 +1,242 @@
+"""Benchmark of the gfx950 draw pipeline on the 1080p forward workload.
+
+One step = one frame: clear colour + clear depth + one draw_indexed per mesh
+instance, all inputs resident in HBM. Prints one JSON line (the contract the
+driver reads) with the roofline of the dominant kernel and a CPU baseline
+(the oracle, a scalar restatement of the reference's raster path, timed on
+this host's cores on a bounded sample of the same workload).
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream (the library's own stream)."""
+
+    def __init__(self):
+        self.hip = C.CDLL("libamdhip64.so")
+        self.hip.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.hip.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.hip.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.hip.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.hip.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self):
+        e = C.c_void_p()
+        assert self.hip.hipEventCreate(C.byref(e)) == 0
+        return e
+
+    def record(self, e, stream):
+        assert self.hip.hipEventRecord(e, C.c_void_p(stream)) == 0
+
+    def elapsed_ms(self, a, b) -> float:
+        assert self.hip.hipEventSynchronize(b) == 0
+        ms = C.c_float()
+        assert self.hip.hipEventElapsedTime(C.byref(ms), a, b) == 0
+        return float(ms.value)
+
+
+def build_c2(args):
+    from malevich_amd import scenes as S
+    sc = S.c2_scene(args.width, args.height, n_instances=args.instances, seed=args.seed, texture_size=0)
+    env = S.smooth_env_map(2048, 1024, seed=args.seed + 17, lo=0.05, hi=2.0)
+    return sc, env
+
+
+def upload(dev, sc, env):
+    from malevich_amd import _lib as L
+    res = {"vb": [], "ib": [], "counts": []}
+    for m in sc.instances:
+        res["vb"].append(dev.create_buffer(L.BUFFER_VERTEX, m.vertices))
+        res["ib"].append(dev.create_buffer(L.BUFFER_INDEX_U32, m.indices))
+        res["counts"].append(m.indices.size)
+    res["cb"] = dev.create_buffer(L.BUFFER_CONSTANT, sc.cb)
+    res["env"] = dev.create_texture2d(env)
+    res["rt"] = dev.create_render_target(sc.width, sc.height)
+    res["ds"] = dev.create_depth_stencil(sc.width, sc.height)
+    return res
+
+
+def frame(dev, res, sc):
+    from malevich_amd import _lib as L
+    from malevich_amd import scenes as S
+    dev.clear_render_target_view(res["rt"], S.CLEAR_COLOR)
+    dev.clear_depth_stencil_view(res["ds"], 0.0)
+    for vb, ib, n in zip(res["vb"], res["ib"], res["counts"]):
+        dev.ia_set_vertex_buffer(vb, 32)
+        dev.ia_set_index_buffer(ib)
+        dev.draw_indexed(n)
+
+
+def bind_state(dev, res, sc):
+    from malevich_amd import _lib as L
+    dev.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
+    dev.vs_set_shader(L.VS_BASIC)
+    dev.ps_set_shader(L.PS_ENV_LIGHTING)
+    dev.vs_set_constant_buffer(0, res["cb"])
+    dev.ps_set_shader_resource(0, res["env"])
+    dev.rs_set_viewport(0, 0, sc.width, sc.height, 0.0, 1.0)
+    dev.om_set_render_targets(res["rt"], res["ds"])
+    dev.om_set_depth_func(L.DEPTH_GREATER_EQUAL)
+
+
+def cpu_baseline(sc, env, budget_s: float):
+    """The oracle (scalar C restatement, 1 thread) on the first instances of the same frame
+    until about `budget_s` of CPU work; reported as input triangles per second."""
+    from oracle import binding as ob
+    rt = ob.RenderTarget(sc.width, sc.height, with_vis=False)
+    rt.clear_render_target_view((227 / 255, 223 / 255, 216 / 255, 0.0))
+    rt.clear_depth_stencil_view(0.0)
+    tris = 0
+    t0 = time.perf_counter()
+    n = 0
+    for m in sc.instances:
+        d = ob.Draw(m.indices, m.vertices, ob.VS_BASIC, ob.PS_ENV_LIGHTING, cb=sc.cb, ps_tex=env,
+                    viewport=(0, 0, sc.width, sc.height, 0, 1))
+        ob.draw_indexed(rt, d)
+        tris += m.triangle_count
+        n += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    dt = time.perf_counter() - t0
+    return {"value": tris / dt / 1e6, "unit": "Mtri/s", "cores": 1, "kind": "port",
+            "sample": f"{n} of {len(sc.instances)} instance draws of the same 1080p frame "
+                      f"({tris} triangles, {dt:.1f} s, oracle built with gcc -O2, 1 thread)"}
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--width", type=int, default=1920)
+    ap.add_argument("--height", type=int, default=1080)
+    ap.add_argument("--instances", type=int, default=64)
+    ap.add_argument("--seed", type=int, default=2)
+    ap.add_argument("--cpu-budget", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args(argv)
+    if args.gpus != 1:
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
+              "design is described in DESIGN.md and is not launched here.", file=sys.stderr)
+        sys.exit(2)
+
+    from malevich_amd import pipeline as P
+
+    t_build = time.perf_counter()
+    sc, env = build_c2(args)
+    print(f"[bench] scene: {len(sc.instances)} instances, {sc.triangle_count} triangles, "
+          f"depth complexity {sc.depth_complexity:.2f} ({time.perf_counter() - t_build:.1f} s)", file=sys.stderr,
+          flush=True)
+    dev = P.Device(0)
+    res = upload(dev, sc, env)
+    bind_state(dev, res, sc)
+    ev = HipEvents()
+    for _ in range(args.warmup):
+        frame(dev, res, sc)
+    dev.synchronize()
+    dev.reset_stats()
+    dev.profile()  # drop anything accumulated
+    dev.set_profiling(True)
+    e0, e1 = ev.create(), ev.create()
+    dev.synchronize()
+    ev.record(e0, dev.stream)
+    for _ in range(args.steps):
+        frame(dev, res, sc)
+    ev.record(e1, dev.stream)
+    ms_total = ev.elapsed_ms(e0, e1)
+    dev.synchronize()
+    prof = dev.profile()
+    dev.set_profiling(False)
+    stats = dev.stats()
+    # a second timed pass without per-stage events gives the headline number
+    e2, e3 = ev.create(), ev.create()
+    ev.record(e2, dev.stream)
+    for _ in range(args.steps):
+        frame(dev, res, sc)
+    ev.record(e3, dev.stream)
+    ms_plain = ev.elapsed_ms(e2, e3)
+    dev.synchronize()
+    print(f"[bench] timed {args.steps} frames: {ms_plain / args.steps:.3f} ms/frame "
+          f"(with stage events {ms_total / args.steps:.3f})", file=sys.stderr, flush=True)
+
+    ms_step = ms_plain / args.steps
+    tris = sc.triangle_count
+    P_px = sc.width * sc.height
+    mtri_s = tris / (ms_step * 1e-3) / 1e6
+    gpix_s = P_px / (ms_step * 1e-3) / 1e9
+    # dominant kernel: the stage with the most time
+    stage_ms = {k: prof[k] for k in ("vertex", "setup", "binning", "tile")}
+    launches = max(1, int(prof["flushes"]))  # one launch of each stage kernel per flush
+    dom = max(stage_ms, key=stage_ms.get)
+    per_launch_ms = stage_ms[dom] / launches
+    flushes_per_frame = launches / args.steps
+    # Algorithmic (compulsory) bytes per launch, texels excluded:
+    #  tile:    TriVis 80 B + TriShade 32 B + three 32 B attribute rows per assembled triangle,
+    #           plus one depth read and one colour + one depth write per frame pixel (12 B)
+    #  setup:   12 B of indices and three 16 B positions per input triangle, 112 B written per
+    #           assembled triangle
+    #  vertex:  32 B read and 48 B written per vertex
+    #  binning: a 4 B key + 4 B value written and re-read per (triangle, tile) pair per radix pass
+    asm_frame = stats["assembled_triangle_count"] / args.steps
+    verts_frame = sum(m.vertices.shape[0] for m in sc.instances)
+    algo_frame = {
+        "tile": asm_frame * (80 + 32 + 3 * 32) + 12.0 * P_px,
+        "setup": tris * (12 + 3 * 16) + asm_frame * (80 + 32),
+        "vertex": verts_frame * (32 + 48),
+        "binning": asm_frame * 16 + (stats["total_triangle_count_in_bins"] / args.steps) * 8 * 2,
+    }
+    algo = {k: v / flushes_per_frame for k, v in algo_frame.items()}
+    achieved = algo[dom] / (per_launch_ms * 1e-3) / 1e9
+    roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "avg_launch_us": round(per_launch_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(algo[dom])}
+    out = {
+        "metric": "fwd Mtri/s + Gpix/s @1080p; fwd+bwd ms/frame; batched vs per-draw views/s",
+        "value": round(mtri_s, 2),
+        "unit": "Mtri/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"c2-1080p-forward: {len(sc.instances)} displaced geodesic spheres, {tris} "
+                               f"triangles, depth complexity {sc.depth_complexity:.1f}, basic_vs + env_lighting_ps "
+                               f"(2048x1024 RGBA32F equirect), reversed-Z GE, {sc.width}x{sc.height}",
+                   "width": sc.width, "height": sc.height, "triangles": tris, "draws_per_frame": len(sc.instances)},
+        "gpix_per_s": round(gpix_s, 4),
+        "stage_ms_per_frame": {k: round(v / args.steps, 4) for k, v in stage_ms.items()},
+        "flushes_per_frame": flushes_per_frame,
+        "assembled_triangles_per_frame": int(stats["assembled_triangle_count"] / args.steps),
+        "fwd_bwd_ms": None,
+        "batched_views_per_s": None,
+        "per_draw_views_per_s": None,
+        "roofline": roofline,
+    }
+    if not args.no_cpu:
+        out["cpu_baseline"] = cpu_baseline(sc, env, args.cpu_budget)
+        out["cpu_baseline"]["gpu_over_cpu"] = round(mtri_s / out["cpu_baseline"]["value"], 1)
+    dev.close()
+    print(json.dumps(out))
+
+
+if __name__ == "__main__":
+    main()

@@ -175,7 +175,27 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
 MLV_API int mlv_render_target_readback(mlv_context *ctx, mlv_render_target *rt, void *host_dst);
 MLV_API int mlv_depth_stencil_readback(mlv_context *ctx, mlv_depth_stencil *ds, float *host_dst);
 
-/* Stats (main.c:199-206), reset like render() does at main.c:1268-1269. */
+/* Recorded draws run together when their results are needed (readback, synchronize,
+ * a clear of a target they draw into, a buffer/texture/target update or destroy).
+ * mlv_flush enqueues them now without waiting. No reference counterpart: the
+ * reference executes each draw_indexed to completion (main.c:1219-1261). */
+MLV_API int mlv_flush(mlv_context *ctx);
+
+/* Extension: per-stage GPU time measured with HIP events on the context stream.
+ * Stages: 0 vertex, 1 setup (cull/clip/ordered compaction), 2 binning, 3 tile
+ * (raster + early-Z + shading + output merger). Timing adds two event records per
+ * stage per flush; mlv_get_profile synchronizes, returns totals since the last call
+ * and restarts them. */
+typedef struct mlv_profile {
+    double stage_ms[4];
+    uint64_t draws;
+    uint64_t flushes;
+} mlv_profile;
+MLV_API int mlv_context_set_profiling(mlv_context *ctx, int enable);
+MLV_API int mlv_get_profile(mlv_context *ctx, mlv_profile *out);
+
+/* Stats (main.c:199-206), reset like render() does at main.c:1268-1269. The reset also
+ * restarts the draw tags written to the visibility buffer at 0. */
 MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out);
 MLV_API int mlv_reset_stats(mlv_context *ctx);
 

@@ -37,6 +37,12 @@ class Stats(C.Structure):
         "total_triangle_count_in_bins", "oob_triangle_count", "draw_count", "overflow_count")]
 
 
+class Profile(C.Structure):
+    _fields_ = [("stage_ms", C.c_double * 4), ("draws", C.c_uint64), ("flushes", C.c_uint64)]
+
+
+STAGES = ("vertex", "setup", "binning", "tile")
+
 _P = C.c_void_p
 _I = C.c_int
 _U32 = C.c_uint32
@@ -85,6 +91,9 @@ SIGNATURES = {
     "mlv_depth_stencil_readback": (_I, [_P, _P, _P]),
     "mlv_get_stats": (_I, [_P, C.POINTER(Stats)]),
     "mlv_reset_stats": (_I, [_P]),
+    "mlv_context_set_profiling": (_I, [_P, _I]),
+    "mlv_flush": (_I, [_P]),
+    "mlv_get_profile": (_I, [_P, C.POINTER(Profile)]),
 }
 
 _lib: C.CDLL | None = None

@@ -158,7 +158,7 @@ __device__ __forceinline__ float4 latlon(const TexDesc &t, float dx, float dy, f
 // Vertex shaders (basic_vs.c:22-33, vertex_lighting_vs.c:22-39, passthrough_vs.c:16-25,
 // fullscreen_vs.c:22-39). Writes the three output registers.
 template <int VS>
-__device__ __forceinline__ void vertex_shader(const DrawParams &p, const float *in, float4 &r0, float4 &r1,
+__device__ __forceinline__ void vertex_shader(const DrawRec &p, const float *in, float4 &r0, float4 &r1,
                                               float4 &r2) {
     if constexpr (VS == VS_BASIC || VS == VS_VERTEX_LIGHTING) {
         const float *cfw = p.cb;
@@ -192,18 +192,17 @@ __device__ __forceinline__ void vertex_shader(const DrawParams &p, const float *
 
 // Pixel shaders (basic_ps.c:16-25, env_lighting_ps.c:13-22, passthrough_ps.c:13-19).
 // a1/a2 are interpolated registers 1 and 2.
-template <int PS>
-__device__ __forceinline__ void pixel_shader(const DrawParams &p, float4 a1, float4 a2, float &r, float &g,
+__device__ __forceinline__ void pixel_shader(int ps, const TexDesc &tex, float4 a1, float4 a2, float &r, float &g,
                                              float &b) {
-    if constexpr (PS == PS_BASIC) {
-        float4 c = bilinear(p.ps_tex, a1.w, a2.x);  // UV = (reg1.w, reg2.x)
+    if (ps == PS_BASIC) {
+        float4 c = bilinear(tex, a1.w, a2.x);  // UV = (reg1.w, reg2.x)
         r = srgb_approx(c.x);
         g = srgb_approx(c.y);
         b = srgb_approx(c.z);
-    } else if constexpr (PS == PS_ENV_LIGHTING) {
+    } else if (ps == PS_ENV_LIGHTING) {
         float nx, ny, nz;
         normalize3(a1.x, a1.y, a1.z, nx, ny, nz);
-        float4 c = latlon(p.ps_tex, nx, ny, nz);
+        float4 c = latlon(tex, nx, ny, nz);
         r = tonemap(c.x);
         g = tonemap(c.y);
         b = tonemap(c.z);
@@ -258,8 +257,8 @@ __device__ __forceinline__ int classify(float4 a, float4 b, float4 c) {
 
 // Projection, viewport transform, snapping, area, cull and setup for one fan triangle
 // (main.c:799-898). Returns false when back-face culled (signed_area > 0).
-__device__ __forceinline__ bool setup_triangle(const DrawParams &p, float4 q0, float4 q1, float4 q2, TriVis &tv,
-                                               float rw[3]) {
+__device__ __forceinline__ bool setup_triangle(const DrawRec &p, const ViewRec &v, float4 q0, float4 q1, float4 q2,
+                                               TriVis &tv, float rw[3]) {
     float4 q[3] = {q0, q1, q2};
     float sx[3], sy[3], sz[3];
     int32_t x[3], y[3];
@@ -301,13 +300,13 @@ __device__ __forceinline__ bool setup_triangle(const DrawParams &p, float4 q0, f
     }
     // bounds (main.c:888-898) then the 8x8 bin rect (main.c:930-931)
     int32_t mnx = imin(x[0], imin(x[1], x[2])) >> 4, mny = imin(y[0], imin(y[1], y[2])) >> 4;
-    mnx = imin(imax(mnx, 0), p.W - 1);
-    mny = imin(imax(mny, 0), p.H - 1);
+    mnx = imin(imax(mnx, 0), v.W - 1);
+    mny = imin(imax(mny, 0), v.H - 1);
     int32_t mxx = imax(x[0], imax(x[1], x[2])) >> 4, mxy = imax(y[0], imax(y[1], y[2])) >> 4;
-    mxx = imin(mxx + 1, p.W - 1);
-    mxy = imin(mxy + 1, p.H - 1);
+    mxx = imin(mxx + 1, v.W - 1);
+    mxy = imin(mxy + 1, v.H - 1);
     int32_t bx0 = imax(mnx / kBin, 0), by0 = imax(mny / kBin, 0);
-    int32_t bx1 = imin(mxx / kBin, p.bins_x - 1), by1 = imin(mxy / kBin, p.bins_y - 1);
+    int32_t bx1 = imin(mxx / kBin, v.bins_x - 1), by1 = imin(mxy / kBin, v.bins_y - 1);
     tv.rect0 = (uint32_t)(uint16_t)bx0 | ((uint32_t)(uint16_t)by0 << 16);
     tv.rect1 = (uint32_t)(uint16_t)(int16_t)bx1 | ((uint32_t)(uint16_t)(int16_t)by1 << 16);
     return true;

@@ -1,26 +1,30 @@
-// mlv_kernels.hip -- the draw pipeline as gfx950 kernels.
+// mlv_kernels.hip -- the draw pipeline as gfx950 kernels, one pass per stage per flush.
 //
-// Per draw (launch_draw):
-//   k_reset            zero the per-draw counters and the tile histogram
-//   k_vertex<VS>       one lane per vertex: VS registers 0..2 -> vpos / vattr   (main.c:698-734)
-//   k_setup_count      one lane per input triangle: cull/classify, output count (main.c:751-908)
-//   k_clip_count       one lane per triangle that needs the clipper           (main.c:609-660)
-//   k_scan_blocks      exclusive scan of per-block counts (order-preserving compaction)
-//   k_setup_emit       one lane per input triangle: write TriVis/TriShade at its ordered slot
-//   k_clip_emit        clipped triangles: write fan triangles and clipper vertices
-//   k_bin_count        (triangle, 16x16 tile) histogram; large triangles binned block-wide
-//   k_scan_tiles       tile offsets
-//   k_bin_scatter      tile lists
-//   k_tile<PS>         one block per 16x16 tile, one wave per 8x8 bin: hierarchical-Z,
-//                      corner-sampled coverage, early-Z visibility in registers, then the
-//                      pixel shader for the surviving fragment and the output merger
-//                      (main.c:983-1189).
+// Stage 0, vertex:   kb_vertex           one lane per vertex slot of every draw (main.c:662-734)
+// Stage 1, setup:    kb_setup_count      one lane per input triangle: cull/classify, output count
+//                    kb_clip_count       triangles that need the clipper (main.c:609-660)
+//                    scan (u64)          ordered compaction offsets over all setup blocks
+//                    kb_setup_totals     totals, capacity checks, first assembled index per draw
+//                    kb_setup_emit       TriVis/TriShade at the ordered slot (main.c:751-908)
+//                    kb_clip_emit        clipped fan triangles + clipper vertices
+// Stage 2, binning:  kb_pairs<false>     (triangle, 16x16 tile) counts and the tile histogram
+//                    scans               pair offsets (triangle order) and tile starts
+//                    kb_pairs<true>      pairs in triangle order
+//                    radix passes        stable LSD sort by tile: lists stay in submission order
+// Stage 3, tile:     kb_tile             one block per 16x16 tile of every view, one wave per
+//                                        8x8 bin: hierarchical-Z, corner-sampled coverage,
+//                                        early-Z in registers, the pixel shader of the surviving
+//                                        fragment and the output merger (main.c:983-1189)
 //
-// Visibility is resolved per pixel as the maximum of (depth, submission order) under the
-// reference's GREATER_EQUAL test (main.c:1166): sequential "z >= d then write" over a bin
-// list in submission order leaves the fragment of maximal depth, and among equal depths the
-// latest one. The tile lists therefore need no ordering for the result to equal the
-// reference's ordered loop, which lets binning use atomics.
+// Visibility per pixel is the maximum of (depth, submission order) under the reference's
+// GREATER_EQUAL test (main.c:1166): running "z >= d then write" over a list in submission
+// order leaves the fragment of maximal depth and, among equal depths, the latest. The tile
+// lists are also kept in submission order, so each wave meets the draws of its 8x8 bin in
+// order and can refresh that bin's hierarchical-Z minimum between draws exactly as the
+// reference's write_tile does at the end of every draw (main.c:589-603).
+#include <algorithm>
+#include <utility>
+
 #include "mlv_device.h"
 
 namespace mlv {
@@ -31,43 +35,39 @@ __device__ __forceinline__ uint32_t mbcnt(uint64_t m) {
     return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
 }
 
-__device__ __forceinline__ uint32_t wave_inclusive_scan(uint32_t x) {
+template <class T>
+__device__ __forceinline__ T wave_inclusive_scan(T x) {
     const uint32_t lane = threadIdx.x & 63u;
 #pragma unroll
     for (int d = 1; d < 64; d <<= 1) {
-        uint32_t y = __shfl_up(x, d, 64);
+        T y = __shfl_up(x, d, 64);
         if (lane >= (uint32_t)d) x += y;
     }
     return x;
 }
 
-__device__ __forceinline__ uint32_t wave_sum(uint32_t x) {
+template <class T>
+__device__ __forceinline__ T wave_sum(T x) {
 #pragma unroll
     for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
     return x;
 }
 
-__device__ __forceinline__ unsigned long long wave_sum64(unsigned long long x) {
-#pragma unroll
-    for (int d = 32; d >= 1; d >>= 1) x += __shfl_xor(x, d, 64);
-    return x;
-}
-
-// Block-wide exclusive scan for NT threads (NT a multiple of 64, <= 1024).
-template <int NT>
-__device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t *s_wave, uint32_t &total) {
+// Block-wide exclusive scan for NT threads (NT a multiple of 64, <= 1024). s_wave holds NT/64+1.
+template <int NT, class T>
+__device__ __forceinline__ T block_exclusive_scan(T v, T *s_wave, T &total) {
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
-    uint32_t inc = wave_inclusive_scan(v);
+    T inc = wave_inclusive_scan(v);
     if (lane == 63u) s_wave[w] = inc;
     __syncthreads();
     if (threadIdx.x < 64u) {
-        uint32_t s = threadIdx.x < (uint32_t)(NT / 64) ? s_wave[threadIdx.x] : 0u;
-        uint32_t si = wave_inclusive_scan(s);
+        T s = threadIdx.x < (uint32_t)(NT / 64) ? s_wave[threadIdx.x] : T(0);
+        T si = wave_inclusive_scan(s);
         if (threadIdx.x < (uint32_t)(NT / 64)) s_wave[threadIdx.x] = si - s;
         if (threadIdx.x == 63u) s_wave[NT / 64] = si;
     }
     __syncthreads();
-    uint32_t r = s_wave[w] + inc - v;
+    T r = s_wave[w] + inc - v;
     total = s_wave[NT / 64];
     __syncthreads();
     return r;
@@ -82,23 +82,39 @@ __device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
     return x * per + (x < rem ? x : rem) + loc;
 }
 
-__device__ __forceinline__ bool load_tri_indices(const DrawParams &p, uint32_t t, uint32_t vi[3]) {
-    const uint32_t *ib = p.ib + p.start_index + 3u * t;
+// Largest index lo in [0, n) with key(lo) <= x, for ascending keys with key(0) <= x.
+template <class F>
+__device__ __forceinline__ uint32_t upper_index(uint32_t n, uint32_t x, F key) {
+    uint32_t lo = 0, hi = n;
+    while (hi - lo > 1) {
+        uint32_t mid = (lo + hi) >> 1;
+        if (key(mid) <= x)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ bool load_tri_indices(const DrawRec &d, uint32_t t, uint32_t vi[3]) {
+    const uint32_t *ib = d.ib + 3u * t;
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        long long v = (long long)ib[k] + (long long)p.base_vertex;
-        if (v < 0 || v >= (long long)p.vertex_count) return false;
+        long long v = (long long)ib[k] + (long long)d.base_vertex;
+        if (v < 0 || v >= (long long)d.vertex_count) return false;
         vi[k] = (uint32_t)v;
     }
     return true;
 }
 
-// Polygon buffer for the clipper: the three vertices' registers, then the six planes.
-__device__ __noinline__ int clip_triangle(const DrawParams &p, uint32_t t, float (*poly)[12], float (*tmp)[12],
-                                          uint32_t vi[3]) {
-    if (!load_tri_indices(p, t, vi)) return 0;
+// The clipper's polygon: the three vertices' registers, then the six planes (main.c:649-660).
+__device__ __noinline__ int clip_triangle(const BatchParams &p, const DrawRec &d, uint32_t t, float (*poly)[12],
+                                          float (*tmp)[12]) {
+    uint32_t vi[3];
+    if (!load_tri_indices(d, t, vi)) return 0;
     for (int k = 0; k < 3; ++k) {
-        float4 a = p.vpos[vi[k]], b = p.vattr[2u * vi[k]], c = p.vattr[2u * vi[k] + 1u];
+        const uint32_t slot = d.vbase + vi[k];
+        float4 a = p.vpos[slot], b = p.vattr[2u * slot], c = p.vattr[2u * slot + 1u];
         poly[k][0] = a.x; poly[k][1] = a.y; poly[k][2] = a.z; poly[k][3] = a.w;
         poly[k][4] = b.x; poly[k][5] = b.y; poly[k][6] = b.z; poly[k][7] = b.w;
         poly[k][8] = c.x; poly[k][9] = c.y; poly[k][10] = c.z; poly[k][11] = c.w;
@@ -115,7 +131,11 @@ __device__ __noinline__ int clip_triangle(const DrawParams &p, uint32_t t, float
 
 __device__ __forceinline__ float4 poly_pos(const float *v) { return make_float4(v[0], v[1], v[2], v[3]); }
 
-// ------------------------------------------------------------ kernels
+__device__ __forceinline__ uint32_t draw_of_slot(const BatchParams &p, uint32_t slot) {
+    return upper_index(p.n_draws, slot, [&](uint32_t i) { return p.draws[i].tbase; });
+}
+
+// ------------------------------------------------------------ fills
 
 __global__ void k_fill_u32(uint32_t *d, uint32_t v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
@@ -127,128 +147,212 @@ __global__ void k_fill_f32(float *d, float v, size_t n) {
         d[i] = v;
 }
 
-__global__ void k_reset(DrawParams p) {
-    const uint32_t ntiles = (uint32_t)(p.tiles_x * p.tiles_y);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        DrawCounters z = {};
-        *p.ctr = z;
+// ------------------------------------------------------------ scans
+// Exclusive scan over n elements (n read from the device when n_dev is given) in three
+// kernels: chunk sums, a one-block scan of the sums, then the chunk scans.
+
+constexpr uint32_t kScanItems = 16;
+constexpr uint32_t kScanChunk = 256 * kScanItems;
+
+template <class T>
+__global__ __launch_bounds__(256) void k_scan_reduce(const T *in, size_t n_host, const uint32_t *n_dev, T *part) {
+    __shared__ T s_w[4];
+    const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
+    const size_t base = (size_t)blockIdx.x * kScanChunk;
+    T s = T(0);
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        size_t i = base + (size_t)k * 256u + threadIdx.x;
+        if (i < n) s += in[i];
     }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < ntiles; i += gridDim.x * blockDim.x)
-        p.tile_cnt[i] = 0u;
+    s = wave_sum(s);
+    if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) part[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_top(T *part, size_t nparts, T *total) {
+    __shared__ T s_w[17];
+    const size_t per = (nparts + 1023) / 1024;
+    const size_t b0 = threadIdx.x * per;
+    T s = T(0);
+    for (size_t i = 0; i < per; ++i)
+        if (b0 + i < nparts) s += part[b0 + i];
+    T tot;
+    T e = block_exclusive_scan<1024, T>(s, s_w, tot);
+    for (size_t i = 0; i < per; ++i)
+        if (b0 + i < nparts) {
+            T v = part[b0 + i];
+            part[b0 + i] = e;
+            e += v;
+        }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
+template <class T>
+__global__ __launch_bounds__(256) void k_scan_down(const T *in, T *out, size_t n_host, const uint32_t *n_dev,
+                                                   const T *part) {
+    __shared__ T s_w[5];
+    const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
+    const size_t base = (size_t)blockIdx.x * kScanChunk;
+    if (base >= n) return;
+    const size_t i0 = base + (size_t)threadIdx.x * kScanItems;
+    T v[kScanItems];
+    T s = T(0);
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        v[k] = (i0 + k < n) ? in[i0 + k] : T(0);
+        s += v[k];
+    }
+    T tot;
+    T e = block_exclusive_scan<256, T>(s, s_w, tot) + part[blockIdx.x];
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) {
+        if (i0 + k < n) out[i0 + k] = e;
+        e += v[k];
+    }
+}
+
+template <class T>
+static hipError_t launch_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, void *part_raw, T *total,
+                              hipStream_t s) {
+    if (n_host == 0) return hipSuccess;
+    T *part = (T *)part_raw;
+    const size_t nparts = (n_host + kScanChunk - 1) / kScanChunk;
+    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nparts), dim3(256), 0, s, in, n_host, n_dev, part);
+    hipLaunchKernelGGL(k_scan_top<T>, dim3(1), dim3(1024), 0, s, part, nparts, total);
+    hipLaunchKernelGGL(k_scan_down<T>, dim3((unsigned)nparts), dim3(256), 0, s, in, out, n_host, n_dev,
+                       (const T *)part);
+    return hipGetLastError();
+}
+
+size_t scan_part_elems(size_t n_max) { return (n_max + kScanChunk - 1) / kScanChunk + 1; }
+
+// ------------------------------------------------------------ stage 0: vertex
+
 template <int VS>
-__global__ __launch_bounds__(256) void k_vertex(DrawParams p) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= p.vertex_count) return;
+__device__ __forceinline__ void run_vs(const BatchParams &p, const DrawRec &d, uint32_t i) {
     float in[8];
-    const uint8_t *src = p.vb + (size_t)i * p.stride;
-    if ((((uintptr_t)p.vb) & 15u) == 0u && (p.stride & 15u) == 0u) {
+    const uint8_t *src = d.vb + (size_t)i * d.stride;
+    if ((((uintptr_t)d.vb) & 15u) == 0u && (d.stride & 15u) == 0u) {
         float4 a = ((const float4 *)src)[0], b = ((const float4 *)src)[1];
         in[0] = a.x; in[1] = a.y; in[2] = a.z; in[3] = a.w; in[4] = b.x; in[5] = b.y; in[6] = b.z; in[7] = b.w;
     } else {
         for (int k = 0; k < 8; ++k) in[k] = ((const float *)src)[k];
     }
     float4 r0, r1, r2;
-    vertex_shader<VS>(p, in, r0, r1, r2);
-    p.vpos[i] = r0;
-    p.vattr[2u * i] = r1;
-    p.vattr[2u * i + 1u] = r2;
+    vertex_shader<VS>(d, in, r0, r1, r2);
+    const uint32_t slot = d.vbase + i;
+    p.vpos[slot] = r0;
+    p.vattr[2u * slot] = r1;
+    p.vattr[2u * slot + 1u] = r2;
 }
 
-__global__ __launch_bounds__(256) void k_setup_count(DrawParams p) {
+__global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
+    const uint32_t b = blockIdx.x;
+    const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_v0; });
+    const DrawRec &d = p.draws[di];
+    const uint32_t i = (b - d.blk_v0) * 256u + threadIdx.x;
+    if (i >= d.vertex_count) return;
+    switch (d.vs) {
+    case VS_BASIC: run_vs<VS_BASIC>(p, d, i); break;
+    case VS_VERTEX_LIGHTING: run_vs<VS_VERTEX_LIGHTING>(p, d, i); break;
+    case VS_PASSTHROUGH: run_vs<VS_PASSTHROUGH>(p, d, i); break;
+    default: run_vs<VS_FULLSCREEN>(p, d, i); break;
+    }
+}
+
+// ------------------------------------------------------------ stage 1: setup
+
+__global__ __launch_bounds__(256) void kb_reset(BatchParams p) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        BatchCounters z = {};
+        *p.ctr = z;
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.total_tiles; i += gridDim.x * blockDim.x)
+        p.tile_cnt[i] = 0u;
+}
+
+__global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
     __shared__ uint32_t s_w[4];
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t b = blockIdx.x;
+    const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
+    const DrawRec &d = p.draws[di];
+    const ViewRec &v = p.views[d.view];
+    const uint32_t t = (b - d.blk_t0) * 256u + threadIdx.x;
+    const uint32_t slot = b * 256u + threadIdx.x;
     uint32_t packed = 0u;
-    if (t < p.tri_count) {
+    if (t < d.tri_count) {
         uint32_t vi[3];
-        if (!load_tri_indices(p, t, vi)) {
-            atomicAdd(&p.ctr->oob, 1u);
+        if (!load_tri_indices(d, t, vi)) {
             atomicAdd(&p.stats->oob, 1ull);
         } else {
-            float4 a = p.vpos[vi[0]], b = p.vpos[vi[1]], c = p.vpos[vi[2]];
-            int cls = classify(a, b, c);
+            float4 a = p.vpos[d.vbase + vi[0]], bb = p.vpos[d.vbase + vi[1]], c = p.vpos[d.vbase + vi[2]];
+            int cls = classify(a, bb, c);
             if (cls == 1) {
                 TriVis tv;
                 float rw[3];
-                packed = setup_triangle(p, a, b, c, tv, rw) ? 1u : 0u;
+                packed = setup_triangle(d, v, a, bb, c, tv, rw) ? 1u : 0u;
             } else if (cls == 2) {
-                uint32_t slot = atomicAdd(&p.ctr->n_clip_list, 1u);
-                p.clip_list[slot] = t;
+                uint32_t k = atomicAdd(&p.ctr->n_clip_list, 1u);
+                p.clip_list[k] = slot;
             }
         }
-        p.cnt[t] = packed;
     }
+    p.cnt[slot] = packed;
     uint32_t s = wave_sum(packed);
     if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) p.blocksum[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+    if (threadIdx.x == 0) p.blocksum[b] = (unsigned long long)(s_w[0] + s_w[1] + s_w[2] + s_w[3]);
 }
 
-__global__ __launch_bounds__(64) void k_clip_count(DrawParams p) {
+__global__ __launch_bounds__(64) void kb_clip_count(BatchParams p) {
     const uint32_t n = p.ctr->n_clip_list;
     float poly[kMaxClip][12], tmp[kMaxClip][12];
     for (uint32_t s = blockIdx.x * 64u + threadIdx.x; s < n; s += gridDim.x * 64u) {
-        const uint32_t t = p.clip_list[s];
-        uint32_t vi[3];
-        int nv = clip_triangle(p, t, poly, tmp, vi);
+        const uint32_t slot = p.clip_list[s];
+        const DrawRec &d = p.draws[draw_of_slot(p, slot)];
+        const ViewRec &v = p.views[d.view];
+        int nv = clip_triangle(p, d, slot - d.tbase, poly, tmp);
         uint32_t ntri = 0u;
         for (int ci = 1; ci < nv - 1; ++ci) {
             TriVis tv;
             float rw[3];
-            if (setup_triangle(p, poly_pos(poly[0]), poly_pos(poly[ci]), poly_pos(poly[ci + 1]), tv, rw)) ntri++;
+            if (setup_triangle(d, v, poly_pos(poly[0]), poly_pos(poly[ci]), poly_pos(poly[ci + 1]), tv, rw)) ntri++;
         }
         uint32_t packed = ntri ? (ntri | ((uint32_t)nv << 16)) : 0u;
-        p.cnt[t] = packed;
-        if (packed) atomicAdd(&p.blocksum[t / 256u], packed);
+        p.cnt[slot] = packed;
+        if (packed) atomicAdd(&p.blocksum[slot >> 8], (unsigned long long)ntri | ((unsigned long long)nv << 32));
     }
 }
 
-// Exclusive scan of the packed block counts, one block of 1024 threads.
-__global__ __launch_bounds__(1024) void k_scan_blocks(DrawParams p) {
-    __shared__ uint32_t s_w[17];
-    const uint32_t n = p.nblocks;
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b0 = threadIdx.x * per;
-    uint32_t st = 0u, sv = 0u;
-    for (uint32_t i = 0; i < per; ++i)
-        if (b0 + i < n) {
-            uint32_t v = p.blocksum[b0 + i];
-            st += v & 0xFFFFu;
-            sv += v >> 16;
-        }
-    uint32_t tt, tv_;
-    uint32_t et = block_exclusive_scan<1024>(st, s_w, tt);
-    uint32_t ev = block_exclusive_scan<1024>(sv, s_w, tv_);
-    for (uint32_t i = 0; i < per; ++i)
-        if (b0 + i < n) {
-            uint32_t v = p.blocksum[b0 + i];
-            p.blockoff_tri[b0 + i] = et;
-            p.blockoff_vtx[b0 + i] = ev;
-            et += v & 0xFFFFu;
-            ev += v >> 16;
-        }
-    if (threadIdx.x == 0) {
-        p.ctr->n_assembled = tt;
-        p.ctr->n_clipverts = tv_;
+__global__ void kb_setup_totals(BatchParams p) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned long long tot = p.ctr->n_assembled_clip;
+    const uint32_t n_asm = (uint32_t)(tot & 0xFFFFFFFFull), n_cv = (uint32_t)(tot >> 32);
+    if (i < p.n_draws) p.asm_base[i] = (uint32_t)(p.blockoff[p.draws[i].blk_t0] & 0xFFFFFFFFull);
+    if (i == 0) {
+        p.ctr->n_assembled = n_asm;
+        p.ctr->n_clipverts = n_cv;
         uint32_t ovf = 0u;
-        if (tt > p.tri_cap) ovf |= OVF_TRIANGLES;
-        if ((unsigned long long)p.vertex_count + tv_ > (unsigned long long)p.vattr_rows) ovf |= OVF_CLIPVERTS;
+        if (n_asm > p.tri_cap) ovf |= OVF_TRIANGLES;
+        if ((unsigned long long)p.vslots + n_cv > (unsigned long long)p.vattr_rows) ovf |= OVF_CLIPVERTS;
         if (ovf) {
             atomicOr(&p.ctr->overflow, ovf);
             atomicAdd(&p.stats->capacity_errors, 1ull);
         }
-        atomicAdd(&p.stats->assembled, (unsigned long long)tt);
+        atomicAdd(&p.stats->assembled, (unsigned long long)n_asm);
         atomicMax(&p.stats->max_clip_list, (unsigned long long)p.ctr->n_clip_list);
     }
 }
 
-__device__ __forceinline__ void write_triangle(const DrawParams &p, uint32_t idx, TriVis &tv, const float rw[3],
-                                               uint32_t r0, uint32_t r1, uint32_t r2, uint32_t prim, uint32_t sub) {
+__device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t idx, uint32_t draw, TriVis &tv,
+                                               const float rw[3], uint32_t r0, uint32_t r1, uint32_t r2,
+                                               uint32_t prim, uint32_t sub) {
     if (idx >= p.tri_cap) return;
     tv.id = idx;
-    tv.pad[0] = 0u;
-    tv.pad[1] = 0u;
+    tv.draw = draw;
+    tv.local = idx - p.asm_base[draw];
     p.tvis[idx] = tv;
     TriShade sh;
     sh.rw[0] = rw[0]; sh.rw[1] = rw[1]; sh.rw[2] = rw[2];
@@ -258,38 +362,47 @@ __device__ __forceinline__ void write_triangle(const DrawParams &p, uint32_t idx
     p.tshade[idx] = sh;
 }
 
-__global__ __launch_bounds__(256) void k_setup_emit(DrawParams p) {
+__global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     __shared__ uint32_t s_w[5];
-    const uint32_t t = blockIdx.x * 256u + threadIdx.x;
-    uint32_t packed = t < p.tri_count ? p.cnt[t] : 0u;
+    const uint32_t b = blockIdx.x;
+    const uint32_t slot = b * 256u + threadIdx.x;
+    const uint32_t packed = p.cnt[slot];
     uint32_t total;
-    uint32_t ex = block_exclusive_scan<256>(packed, s_w, total);
+    // within one block the packed counts (<= 14 triangles, <= 16 vertices each) cannot carry
+    const uint32_t ex = block_exclusive_scan<256, uint32_t>(packed, s_w, total);
     if (packed == 0u) return;
-    const uint32_t off_tri = p.blockoff_tri[blockIdx.x] + (ex & 0xFFFFu);
-    const uint32_t off_vtx = p.blockoff_vtx[blockIdx.x] + (ex >> 16);
+    const unsigned long long bo = p.blockoff[b];
+    const uint32_t off_tri = (uint32_t)(bo & 0xFFFFFFFFull) + (ex & 0xFFFFu);
+    const uint32_t off_vtx = (uint32_t)(bo >> 32) + (ex >> 16);
     if (packed == 1u) {
+        const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
+        const DrawRec &d = p.draws[di];
+        const uint32_t t = slot - d.tbase;
         uint32_t vi[3];
-        load_tri_indices(p, t, vi);
-        float4 a = p.vpos[vi[0]], b = p.vpos[vi[1]], c = p.vpos[vi[2]];
+        load_tri_indices(d, t, vi);
+        const uint32_t s0 = d.vbase + vi[0], s1 = d.vbase + vi[1], s2 = d.vbase + vi[2];
         TriVis tv;
         float rw[3];
-        setup_triangle(p, a, b, c, tv, rw);
-        write_triangle(p, off_tri, tv, rw, vi[0], vi[1], vi[2], t, 0u);
+        setup_triangle(d, p.views[d.view], p.vpos[s0], p.vpos[s1], p.vpos[s2], tv, rw);
+        write_triangle(p, off_tri, di, tv, rw, s0, s1, s2, t, 0u);
     } else {
-        p.clip_off[t] = make_uint2(off_tri, off_vtx);
+        p.clip_off[slot] = make_uint2(off_tri, off_vtx);
     }
 }
 
-__global__ __launch_bounds__(64) void k_clip_emit(DrawParams p) {
+__global__ __launch_bounds__(64) void kb_clip_emit(BatchParams p) {
     const uint32_t n = p.ctr->n_clip_list;
     float poly[kMaxClip][12], tmp[kMaxClip][12];
     for (uint32_t s = blockIdx.x * 64u + threadIdx.x; s < n; s += gridDim.x * 64u) {
-        const uint32_t t = p.clip_list[s];
-        if (p.cnt[t] == 0u) continue;
-        const uint2 off = p.clip_off[t];
-        uint32_t vi[3];
-        int nv = clip_triangle(p, t, poly, tmp, vi);
-        const uint32_t row0 = p.vertex_count + off.y;
+        const uint32_t slot = p.clip_list[s];
+        if (p.cnt[slot] == 0u) continue;
+        const uint32_t di = draw_of_slot(p, slot);
+        const DrawRec &d = p.draws[di];
+        const ViewRec &v = p.views[d.view];
+        const uint2 off = p.clip_off[slot];
+        const uint32_t t = slot - d.tbase;
+        int nv = clip_triangle(p, d, t, poly, tmp);
+        const uint32_t row0 = p.vslots + off.y;
         for (int k = 0; k < nv; ++k) {
             uint32_t row = row0 + (uint32_t)k;
             if (row >= p.vattr_rows) break;
@@ -300,8 +413,8 @@ __global__ __launch_bounds__(64) void k_clip_emit(DrawParams p) {
         for (int ci = 1; ci < nv - 1; ++ci) {
             TriVis tv;
             float rw[3];
-            if (setup_triangle(p, poly_pos(poly[0]), poly_pos(poly[ci]), poly_pos(poly[ci + 1]), tv, rw)) {
-                write_triangle(p, off.x + j, tv, rw, row0, row0 + (uint32_t)ci, row0 + (uint32_t)ci + 1u, t,
+            if (setup_triangle(d, v, poly_pos(poly[0]), poly_pos(poly[ci]), poly_pos(poly[ci + 1]), tv, rw)) {
+                write_triangle(p, off.x + j, di, tv, rw, row0, row0 + (uint32_t)ci, row0 + (uint32_t)ci + 1u, t,
                                (uint32_t)(ci - 1));
                 j++;
             }
@@ -309,28 +422,34 @@ __global__ __launch_bounds__(64) void k_clip_emit(DrawParams p) {
     }
 }
 
-// Tile rect (16x16 tiles) from a TriVis 8x8-bin rect. Returns the tile count (0 if empty).
-__device__ __forceinline__ uint32_t tile_rect(const DrawParams &p, uint32_t rect0, uint32_t rect1, int32_t &tx0,
-                                              int32_t &ty0, int32_t &tw, uint32_t &pairs8) {
+// ------------------------------------------------------------ stage 2: binning
+
+// Tile rect (16x16 tiles) of a TriVis 8x8-bin rect; returns the tile count (0 if empty).
+__device__ __forceinline__ uint32_t tile_rect(uint32_t rect0, uint32_t rect1, int32_t &tx0, int32_t &ty0, int32_t &tw,
+                                              uint32_t &pairs8) {
     int32_t bx0 = (int32_t)(rect0 & 0xFFFFu), by0 = (int32_t)(rect0 >> 16);
     int32_t bx1 = (int32_t)(int16_t)(rect1 & 0xFFFFu), by1 = (int32_t)(int16_t)(rect1 >> 16);
     if (bx1 < bx0 || by1 < by0) {
         pairs8 = 0u;
+        tx0 = ty0 = 0;
+        tw = 1;
         return 0u;
     }
     pairs8 = (uint32_t)(bx1 - bx0 + 1) * (uint32_t)(by1 - by0 + 1);
     tx0 = bx0 >> 1;
     ty0 = by0 >> 1;
-    int32_t tx1 = bx1 >> 1, ty1 = by1 >> 1;
-    tw = tx1 - tx0 + 1;
-    return (uint32_t)tw * (uint32_t)(ty1 - ty0 + 1);
+    tw = (bx1 >> 1) - tx0 + 1;
+    return (uint32_t)tw * (uint32_t)((by1 >> 1) - ty0 + 1);
 }
 
-template <bool SCATTER>
-__global__ __launch_bounds__(256) void k_bin(DrawParams p) {
+// EMIT = false: per-triangle pair counts and the tile histogram. EMIT = true: the pairs in
+// triangle order at the scanned offsets. Triangles spanning more than kBigRect tiles are
+// expanded by the whole block.
+template <bool EMIT>
+__global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
     __shared__ uint32_t s_big[256];
     __shared__ uint32_t s_nbig;
-    if (SCATTER && (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES))) return;
+    if (EMIT && (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES))) return;
     const uint32_t n = min(p.ctr->n_assembled, p.tri_cap);
     unsigned long long pairs8_acc = 0ull;
     for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
@@ -338,41 +457,48 @@ __global__ __launch_bounds__(256) void k_bin(DrawParams p) {
         __syncthreads();
         const uint32_t i = base + threadIdx.x;
         if (i < n) {
-            const uint2 r = make_uint2(p.tvis[i].rect0, p.tvis[i].rect1);
+            const TriVis &tv = p.tvis[i];
+            const ViewRec &v = p.views[p.draws[tv.draw].view];
             int32_t tx0, ty0, tw;
             uint32_t p8;
-            uint32_t nt = tile_rect(p, r.x, r.y, tx0, ty0, tw, p8);
-            pairs8_acc += p8;
+            const uint32_t nt = tile_rect(tv.rect0, tv.rect1, tx0, ty0, tw, p8);
+            if (!EMIT) {
+                p.pair_cnt[i] = nt;
+                pairs8_acc += p8;
+            }
             if (nt <= (uint32_t)kBigRect) {
+                const uint32_t off = EMIT ? p.pair_off[i] : 0u;
                 for (uint32_t k = 0; k < nt; ++k) {
-                    uint32_t tile = (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * (uint32_t)p.tiles_x +
-                                    (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
-                    if (SCATTER) {
-                        uint32_t slot = atomicAdd(&p.tile_fill[tile], 1u);
-                        p.tile_list[p.tile_start[tile] + slot] = i;
+                    const uint32_t tile = v.tile_base +
+                                          (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * (uint32_t)v.tiles_x +
+                                          (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
+                    if (EMIT) {
+                        p.keys0[off + k] = tile;
+                        p.vals0[off + k] = i;
                     } else {
                         atomicAdd(&p.tile_cnt[tile], 1u);
                     }
                 }
             } else {
-                uint32_t k = atomicAdd(&s_nbig, 1u);
-                s_big[k] = i;
+                s_big[atomicAdd(&s_nbig, 1u)] = i;
             }
         }
         __syncthreads();
         const uint32_t nbig = s_nbig;
-        for (uint32_t b = 0; b < nbig; ++b) {
-            const uint32_t ti = s_big[b];
-            const uint2 r = make_uint2(p.tvis[ti].rect0, p.tvis[ti].rect1);
+        for (uint32_t bi = 0; bi < nbig; ++bi) {
+            const uint32_t ti = s_big[bi];
+            const TriVis &tv = p.tvis[ti];
+            const ViewRec &v = p.views[p.draws[tv.draw].view];
             int32_t tx0, ty0, tw;
             uint32_t p8;
-            uint32_t nt = tile_rect(p, r.x, r.y, tx0, ty0, tw, p8);
+            const uint32_t nt = tile_rect(tv.rect0, tv.rect1, tx0, ty0, tw, p8);
+            const uint32_t off = EMIT ? p.pair_off[ti] : 0u;
             for (uint32_t k = threadIdx.x; k < nt; k += 256u) {
-                uint32_t tile = (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * (uint32_t)p.tiles_x +
-                                (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
-                if (SCATTER) {
-                    uint32_t slot = atomicAdd(&p.tile_fill[tile], 1u);
-                    p.tile_list[p.tile_start[tile] + slot] = ti;
+                const uint32_t tile = v.tile_base + (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * (uint32_t)v.tiles_x +
+                                      (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
+                if (EMIT) {
+                    p.keys0[off + k] = tile;
+                    p.vals0[off + k] = ti;
                 } else {
                     atomicAdd(&p.tile_cnt[tile], 1u);
                 }
@@ -380,33 +506,15 @@ __global__ __launch_bounds__(256) void k_bin(DrawParams p) {
         }
         __syncthreads();
     }
-    if (!SCATTER) {
-        unsigned long long s = wave_sum64(pairs8_acc);
-        if ((threadIdx.x & 63u) == 0u && s) {
-            atomicAdd(&p.ctr->pairs8, s);
-            atomicAdd(&p.stats->pairs8, s);
-        }
+    if (!EMIT) {
+        unsigned long long s = wave_sum(pairs8_acc);
+        if ((threadIdx.x & 63u) == 0u && s) atomicAdd(&p.stats->pairs8, s);
     }
 }
 
-__global__ __launch_bounds__(1024) void k_scan_tiles(DrawParams p) {
-    __shared__ uint32_t s_w[17];
-    const uint32_t n = (uint32_t)(p.tiles_x * p.tiles_y);
-    const uint32_t per = (n + 1023u) / 1024u;
-    const uint32_t b0 = threadIdx.x * per;
-    uint32_t s = 0u;
-    for (uint32_t i = 0; i < per; ++i)
-        if (b0 + i < n) s += p.tile_cnt[b0 + i];
-    uint32_t total;
-    uint32_t e = block_exclusive_scan<1024>(s, s_w, total);
-    for (uint32_t i = 0; i < per; ++i)
-        if (b0 + i < n) {
-            p.tile_start[b0 + i] = e;
-            p.tile_fill[b0 + i] = 0u;
-            e += p.tile_cnt[b0 + i];
-        }
-    if (threadIdx.x == 0) {
-        p.ctr->n_pairs = total;
+__global__ void kb_pair_check(BatchParams p) {
+    if (threadIdx.x == 0 && blockIdx.x == 0) {
+        const uint32_t total = p.ctr->n_pairs;
         atomicMax(&p.stats->max_pairs, (unsigned long long)total);
         if (total > p.pair_cap) {
             atomicOr(&p.ctr->overflow, OVF_PAIRS);
@@ -414,6 +522,83 @@ __global__ __launch_bounds__(1024) void k_scan_tiles(DrawParams p) {
         }
     }
 }
+
+// Stable LSD radix sort of (tile, triangle) pairs by tile, 8 bits per pass. A block owns
+// kRadixItems consecutive pairs; within a round of 256, a pair's rank among equal digits
+// comes from eight ballots (the lanes sharing its digit) and mbcnt; per-wave digit counts
+// are combined across waves and rounds in LDS, so equal keys keep their input order.
+constexpr uint32_t kRadixRounds = 8;
+constexpr uint32_t kRadixItems = 256 * kRadixRounds;
+
+size_t radix_hist_elems(size_t n_max) { return 256 * ((n_max + kRadixItems - 1) / kRadixItems + 1); }
+
+__global__ __launch_bounds__(256) void kb_radix_hist(BatchParams p, const uint32_t *keys, uint32_t shift,
+                                                     uint32_t *hist, uint32_t nb) {
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0u;
+    __syncthreads();
+    const bool skip = (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) != 0u;
+    const uint32_t n = skip ? 0u : p.ctr->n_pairs;
+    const uint32_t base = blockIdx.x * kRadixItems;
+    for (uint32_t r = 0; r < kRadixRounds; ++r) {
+        const uint32_t i = base + r * 256u + threadIdx.x;
+        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uint32_t *kin, const uint32_t *vin,
+                                                        uint32_t *kout, uint32_t *vout, uint32_t shift,
+                                                        const uint32_t *hist, uint32_t nb) {
+    __shared__ uint32_t run[256];
+    __shared__ uint32_t wc[4][256];
+    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
+    const uint32_t n = p.ctr->n_pairs;
+    const uint32_t base = blockIdx.x * kRadixItems;
+    if (base >= n) return;
+    const uint32_t tid = threadIdx.x, w = tid >> 6;
+    run[tid] = 0u;
+    for (uint32_t r = 0; r < kRadixRounds; ++r) {
+        wc[0][tid] = 0u;
+        wc[1][tid] = 0u;
+        wc[2][tid] = 0u;
+        wc[3][tid] = 0u;
+        __syncthreads();
+        const uint32_t i = base + r * 256u + tid;
+        const bool valid = i < n;
+        const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t dg = (key >> shift) & 255u;
+        uint64_t m = __ballot(valid);
+#pragma unroll
+        for (int bit = 0; bit < 8; ++bit) {
+            const bool on = (dg >> bit) & 1u;
+            const uint64_t bb = __ballot(on);
+            m &= on ? bb : ~bb;
+        }
+        const uint32_t rank = mbcnt(m);
+        if (valid && rank == 0u) wc[w][dg] = (uint32_t)__popcll(m);
+        __syncthreads();
+        {
+            const uint32_t c0 = wc[0][tid], c1 = wc[1][tid], c2 = wc[2][tid], c3 = wc[3][tid];
+            const uint32_t e0 = run[tid];
+            wc[0][tid] = e0;
+            wc[1][tid] = e0 + c0;
+            wc[2][tid] = e0 + c0 + c1;
+            wc[3][tid] = e0 + c0 + c1 + c2;
+            run[tid] = e0 + c0 + c1 + c2 + c3;
+        }
+        __syncthreads();
+        if (valid) {
+            const uint32_t pos = hist[dg * nb + blockIdx.x] + wc[w][dg] + rank;
+            kout[pos] = key;
+            vout[pos] = vin[i];
+        }
+        __syncthreads();
+    }
+}
+
+// ------------------------------------------------------------ stage 3: tile
 
 // Reduce over the 64 lanes of a wave: minimum (GE) or maximum (LE) of the in-range depths,
 // ignoring NaN. Equal on every lane.
@@ -427,52 +612,63 @@ __device__ __forceinline__ float wave_depth_extreme(bool inside, float d, bool g
     return v;
 }
 
-template <int PS>
-__global__ __launch_bounds__(256) void k_tile(DrawParams p) {
+// write_tile's tile minimum (main.c:589-603), folded from 1.0 (0.0 for the LESS_EQUAL mirror)
+__device__ __forceinline__ float tile_hiz(bool inside, float d, bool ge) {
+    float m = wave_depth_extreme(inside, d, ge);
+    return ge ? min_ref(1.0f, m) : max_ref(0.0f, m);
+}
+
+__global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
     __shared__ TriVis s_tri[256];
     __shared__ uint32_t s_mask[256];
     __shared__ uint16_t s_list[4][256];
 
+    if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    const uint32_t ntiles = (uint32_t)(p.tiles_x * p.tiles_y);
-    const uint32_t tile = xcd_remap(blockIdx.x, ntiles);
-    const int32_t tx = (int32_t)(tile % (uint32_t)p.tiles_x), ty = (int32_t)(tile / (uint32_t)p.tiles_x);
+    const uint32_t g = xcd_remap(blockIdx.x, p.total_tiles);
+    const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
+    const ViewRec &V = p.views[vi];
+    const uint32_t tile = g - V.tile_base;
+    const int32_t tx = (int32_t)(tile % (uint32_t)V.tiles_x), ty = (int32_t)(tile / (uint32_t)V.tiles_x);
     const int32_t sbx = 2 * tx + (int32_t)(w & 1u), sby = 2 * ty + (int32_t)(w >> 1);
     const int32_t px = sbx * kBin + (int32_t)(lane & 7u), py = sby * kBin + (int32_t)(lane >> 3);
-    const bool sub_valid = sbx < p.bins_x && sby < p.bins_y;
-    const bool inside = px < p.W && py < p.H;
-    const size_t pix = (size_t)py * (size_t)p.W + (size_t)px;
-    const bool ge = p.depth_func == DEPTH_GE;
-
-    if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // draw failed; reported on the host
-
-    float d = inside ? p.depth[pix] : 0.0f;
-    int32_t best = -1;
-    const float hz = sub_valid ? p.hiz[sby * p.bins_x + sbx] : 0.0f;
-    bool touched = false;
-    float wext = wave_depth_extreme(inside, d, ge);
+    const bool sub_valid = sbx < V.bins_x && sby < V.bins_y;
+    const bool inside = px < V.W && py < V.H;
+    const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
+    const bool ge = V.depth_func == DEPTH_GE;
 
     const uint32_t n_asm = min(p.ctr->n_assembled, p.tri_cap);
     const bool ovf = (p.ctr->overflow & OVF_PAIRS) != 0u;
     uint32_t lstart = 0u, lcount = n_asm;
     if (!ovf) {
-        lstart = p.tile_start[tile];
-        lcount = p.tile_cnt[tile];
+        lstart = p.tile_start[g];
+        lcount = p.tile_cnt[g];
     }
+    if (lcount == 0u) return;
+
+    float d = inside ? V.depth[pix] : 0.0f;
+    int32_t best = -1;
+    float hz = sub_valid ? V.hiz[sby * V.bins_x + sbx] : 0.0f;
+    int32_t cur_draw = -1;
+    uint32_t n_draws_here = 0u;
+    float wext = wave_depth_extreme(inside, d, ge);
+    int32_t depth_cull = 0;
 
     for (uint32_t base = 0; base < lcount; base += 256u) {
         const uint32_t n = min(256u, lcount - base);
         if (tid < n) {
-            const uint32_t t = ovf ? base + tid : p.tile_list[lstart + base + tid];
+            const uint32_t t = ovf ? base + tid : sorted_vals[lstart + base + tid];
             const TriVis tv = p.tvis[t];
             s_tri[tid] = tv;
             int32_t bx0, by0, bx1, by1;
             rect_of(tv, bx0, by0, bx1, by1);
             uint32_t m = 0u;
+            if (!ovf || p.draws[tv.draw].view == vi) {
 #pragma unroll
-            for (int k = 0; k < 4; ++k) {
-                int32_t gx = 2 * tx + (k & 1), gy = 2 * ty + (k >> 1);
-                if (gx >= bx0 && gx <= bx1 && gy >= by0 && gy <= by1) m |= 1u << k;
+                for (int k = 0; k < 4; ++k) {
+                    int32_t gx = 2 * tx + (k & 1), gy = 2 * ty + (k >> 1);
+                    if (gx >= bx0 && gx <= bx1 && gy >= by0 && gy <= by1) m |= 1u << k;
+                }
             }
             s_mask[tid] = m;
         }
@@ -487,14 +683,21 @@ __global__ __launch_bounds__(256) void k_tile(DrawParams p) {
             cnt += (uint32_t)__popcll(bal);
         }
         if (sub_valid && cnt) {
-            touched = true;
             for (uint32_t k = 0; k < cnt; ++k) {
                 const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_list[w][k]);
                 const TriVis T = s_tri[i];
+                const int32_t tdraw = __builtin_amdgcn_readfirstlane((int)T.draw);
+                if (tdraw != cur_draw) {
+                    // the bin's tile minimum after the previous draw (write_tile, main.c:589-603)
+                    if (cur_draw >= 0) hz = tile_hiz(inside, d, ge);
+                    cur_draw = tdraw;
+                    n_draws_here++;
+                    depth_cull = p.draws[tdraw].depth_cull;
+                }
                 // reference hierarchical-Z (main.c:1005-1010), per (triangle, 8x8 bin)
                 bool rej = ge ? (T.hiz_depth < hz) : (T.hiz_depth > hz);
-                // conservative in-draw cull: no fragment of T can pass the current depths
-                if (p.depth_cull) rej = rej || (ge ? (T.cull_bound < wext) : (T.cull_bound > wext));
+                // conservative cull: no fragment of T can pass the bin's current depths
+                if (depth_cull) rej = rej || (ge ? (T.cull_bound < wext) : (T.cull_bound > wext));
                 if (__builtin_amdgcn_readfirstlane((int)rej)) continue;
                 const uint32_t e0 = (uint32_t)T.c[0] + (uint32_t)__mul24(T.a16[0], px) + (uint32_t)__mul24(T.b16[0], py);
                 const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
@@ -502,7 +705,7 @@ __global__ __launch_bounds__(256) void k_tile(DrawParams p) {
                 const bool cov = inside && (int32_t)(e0 | e1 | e2) > 0;  // main.c:1028
                 const float b1 = (float)((int32_t)e1 >> 8) * T.ooa;     // main.c:1089-1102
                 const float b2 = (float)((int32_t)e2 >> 8) * T.ooa;
-                float z = T.z0 + T.dz1 * b1;                             // main.c:1118-1154, register 0 (affine)
+                float z = T.z0 + T.dz1 * b1;  // register 0 is affine (main.c:1118-1154)
                 z = z + T.dz2 * b2;
                 const int32_t id = (int32_t)T.id;
                 const bool pass = ge ? (z > d || (z == d && id > best)) : (z < d || (z == d && id > best));
@@ -511,7 +714,7 @@ __global__ __launch_bounds__(256) void k_tile(DrawParams p) {
                     d = z;
                     best = id;
                 }
-                if (p.depth_cull && __ballot(upd)) wext = wave_depth_extreme(inside, d, ge);
+                if (depth_cull && __ballot(upd)) wext = wave_depth_extreme(inside, d, ge);
             }
         }
         __syncthreads();
@@ -520,6 +723,7 @@ __global__ __launch_bounds__(256) void k_tile(DrawParams p) {
     if (best >= 0) {
         const TriVis T = p.tvis[best];
         const TriShade S = p.tshade[best];
+        const DrawRec &D = p.draws[T.draw];
         const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
         const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
         const float bx = (float)((int32_t)e1 >> 8) * T.ooa;
@@ -542,22 +746,20 @@ __global__ __launch_bounds__(256) void k_tile(DrawParams p) {
                                       MLV_INTERP(b0.z, b1.z, b2.z), MLV_INTERP(b0.w, b1.w, b2.w));
 #undef MLV_INTERP
         float cr, cg, cb;
-        pixel_shader<PS>(p, r1, r2, cr, cg, cb);
-        p.color[pix] = encode_output(cr, cg, cb);
-        p.depth[pix] = d;
-        if (p.vis_tri) {
-            p.vis_tri[pix] = (uint32_t)best;
-            p.vis_draw[pix] = p.draw_tag;
+        pixel_shader(D.ps, D.ps_tex, r1, r2, cr, cg, cb);
+        V.color[pix] = encode_output(cr, cg, cb);
+        V.depth[pix] = d;
+        if (V.vis_tri) {
+            V.vis_tri[pix] = T.local;
+            V.vis_draw[pix] = D.draw_tag;
         }
     }
 
-    // write_tile's tile minimum (main.c:589-603) for every 8x8 bin the draw touched
-    if (sub_valid && touched) {
-        float m = wave_depth_extreme(inside, d, ge);
-        m = ge ? min_ref(1.0f, m) : max_ref(0.0f, m);
+    if (sub_valid && cur_draw >= 0) {
+        const float m = tile_hiz(inside, d, ge);
         if (lane == 0u) {
-            p.hiz[sby * p.bins_x + sbx] = m;
-            atomicAdd(&p.stats->active8, 1ull);
+            V.hiz[sby * V.bins_x + sbx] = m;
+            atomicAdd(&p.stats->active8, (unsigned long long)n_draws_here);
         }
     }
 }
@@ -566,49 +768,72 @@ __global__ __launch_bounds__(256) void k_tile(DrawParams p) {
 
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s) {
     if (!n) return hipSuccess;
-    unsigned grid = (unsigned)((n + 255) / 256);
-    if (grid > 65535u * 8u) grid = 65535u * 8u;
+    unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 65535u * 8u);
     hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, s, dst, value, n);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s) {
     if (!n) return hipSuccess;
-    unsigned grid = (unsigned)((n + 255) / 256);
-    if (grid > 65535u * 8u) grid = 65535u * 8u;
+    unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 65535u * 8u);
     hipLaunchKernelGGL(k_fill_f32, dim3(grid), dim3(256), 0, s, dst, value, n);
     return hipGetLastError();
 }
 
-hipError_t launch_draw(const DrawParams &p, int vs, int ps, hipStream_t s) {
-    const uint32_t ntiles = (uint32_t)(p.tiles_x * p.tiles_y);
-    hipLaunchKernelGGL(k_reset, dim3((ntiles + 255) / 256), dim3(256), 0, s, p);
-    if (p.vertex_count) {
-        dim3 g((p.vertex_count + 255u) / 256u);
-        switch (vs) {
-        case VS_BASIC: hipLaunchKernelGGL(k_vertex<VS_BASIC>, g, dim3(256), 0, s, p); break;
-        case VS_VERTEX_LIGHTING: hipLaunchKernelGGL(k_vertex<VS_VERTEX_LIGHTING>, g, dim3(256), 0, s, p); break;
-        case VS_PASSTHROUGH: hipLaunchKernelGGL(k_vertex<VS_PASSTHROUGH>, g, dim3(256), 0, s, p); break;
-        case VS_FULLSCREEN: hipLaunchKernelGGL(k_vertex<VS_FULLSCREEN>, g, dim3(256), 0, s, p); break;
-        default: return hipErrorInvalidValue;
+hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, uint32_t radix_passes,
+                        hipStream_t s) {
+    hipError_t e;
+    switch (stage) {
+    case 0:
+        hipLaunchKernelGGL(kb_reset, dim3((p.total_tiles + 255) / 256 + 1), dim3(256), 0, s, p);
+        if (p.n_vblocks) hipLaunchKernelGGL(kb_vertex, dim3(p.n_vblocks), dim3(256), 0, s, p);
+        break;
+    case 1:
+        if (!p.n_tblocks) break;
+        hipLaunchKernelGGL(kb_setup_count, dim3(p.n_tblocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(kb_clip_count, dim3(256), dim3(64), 0, s, p);
+        if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc.part,
+                                                 &p.ctr->n_assembled_clip, s)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(kb_setup_totals, dim3((p.n_draws + 255) / 256), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(kb_setup_emit, dim3(p.n_tblocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(kb_clip_emit, dim3(256), dim3(64), 0, s, p);
+        break;
+    case 2: {
+        if (!p.n_tblocks) break;
+        const uint32_t grid = std::min<uint32_t>(p.n_tblocks, 8192u);
+        hipLaunchKernelGGL(kb_pairs<false>, dim3(grid), dim3(256), 0, s, p);
+        if ((e = launch_scan<uint32_t>(p.pair_cnt, p.pair_off, p.tri_cap, &p.ctr->n_assembled, sc.part,
+                                       &p.ctr->n_pairs, s)) != hipSuccess)
+            return e;
+        hipLaunchKernelGGL(kb_pair_check, dim3(1), dim3(64), 0, s, p);
+        if ((e = launch_scan<uint32_t>(p.tile_cnt, p.tile_start, p.total_tiles, nullptr, sc.part, nullptr, s)) !=
+            hipSuccess)
+            return e;
+        hipLaunchKernelGGL(kb_pairs<true>, dim3(grid), dim3(256), 0, s, p);
+        const uint32_t nb = (p.pair_cap + kRadixItems - 1) / kRadixItems;
+        uint32_t *kin = p.keys0, *vin = p.vals0, *kout = p.keys1, *vout = p.vals1;
+        for (uint32_t pass = 0; pass < radix_passes; ++pass) {
+            const uint32_t shift = 8u * pass;
+            hipLaunchKernelGGL(kb_radix_hist, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin, shift, sc.hist, nb);
+            if ((e = launch_scan<uint32_t>(sc.hist, sc.hist, (size_t)nb * 256u, nullptr, sc.part, nullptr, s)) !=
+                hipSuccess)
+                return e;
+            hipLaunchKernelGGL(kb_radix_scatter, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin,
+                               (const uint32_t *)vin, kout, vout, shift, (const uint32_t *)sc.hist, nb);
+            std::swap(kin, kout);
+            std::swap(vin, vout);
         }
+        break;
     }
-    if (p.tri_count) {
-        hipLaunchKernelGGL(k_setup_count, dim3(p.nblocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(k_clip_count, dim3(512), dim3(64), 0, s, p);
-        hipLaunchKernelGGL(k_scan_blocks, dim3(1), dim3(1024), 0, s, p);
-        hipLaunchKernelGGL(k_setup_emit, dim3(p.nblocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(k_clip_emit, dim3(512), dim3(64), 0, s, p);
-        const uint32_t bgrid = p.nblocks < 8192u ? p.nblocks : 8192u;
-        hipLaunchKernelGGL(k_bin<false>, dim3(bgrid), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(k_scan_tiles, dim3(1), dim3(1024), 0, s, p);
-        hipLaunchKernelGGL(k_bin<true>, dim3(bgrid), dim3(256), 0, s, p);
-        switch (ps) {
-        case PS_BASIC: hipLaunchKernelGGL(k_tile<PS_BASIC>, dim3(ntiles), dim3(256), 0, s, p); break;
-        case PS_ENV_LIGHTING: hipLaunchKernelGGL(k_tile<PS_ENV_LIGHTING>, dim3(ntiles), dim3(256), 0, s, p); break;
-        case PS_PASSTHROUGH: hipLaunchKernelGGL(k_tile<PS_PASSTHROUGH>, dim3(ntiles), dim3(256), 0, s, p); break;
-        default: return hipErrorInvalidValue;
-        }
+    case 3: {
+        if (!p.n_tblocks || !p.total_tiles) break;
+        const uint32_t *sorted = (radix_passes & 1u) ? p.vals1 : p.vals0;
+        hipLaunchKernelGGL(kb_tile, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
+        break;
+    }
+    default:
+        return hipErrorInvalidValue;
     }
     return hipGetLastError();
 }

@@ -4,17 +4,25 @@
 // `Pipeline graphics_pipeline` (main.c:109-115, 222), one HIP stream, and the
 // transient arenas that draw_indexed mallocs and frees on every call in the
 // reference (main.c:675-746, 947, 965, 986, 1253-1259). Arenas here grow
-// geometrically and are reused, so a steady-state draw allocates nothing.
+// geometrically and are reused, so a steady-state frame allocates nothing.
+//
+// mlv_draw_indexed validates the state and records the draw; recorded draws run
+// together in one pass of each stage (a flush) when their results are needed: a
+// readback, mlv_synchronize, a clear of a target they draw into, a change to a
+// buffer, texture or target they read, or when the batch reaches its size limit.
+// Stream order then gives the same results as running each draw on its own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
-#include <cstdarg>
 #include <cmath>
+#include <cstdarg>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
+#include <utility>
+#include <vector>
 
 #include "malevich.h"
 #include "mlv_internal.h"
@@ -53,6 +61,27 @@ struct Arena {
     size_t cap = 0;
 };
 
+struct ViewKey {
+    mlv_render_target *rt;
+    mlv_depth_stencil *ds;
+    uint32_t *vis_tri, *vis_draw;
+    int depth_func;
+    bool operator==(const ViewKey &o) const {
+        return rt == o.rt && ds == o.ds && vis_tri == o.vis_tri && vis_draw == o.vis_draw && depth_func == o.depth_func;
+    }
+};
+
+struct PendingDraw {
+    DrawRec rec;
+    ViewKey key;
+};
+
+// One flush never exceeds these (indices are 32-bit; the arenas stay bounded).
+constexpr uint64_t kMaxBatchTriangles = 48ull << 20;
+constexpr uint64_t kMaxBatchVertices = 48ull << 20;
+constexpr size_t kMaxBatchDraws = 1u << 16;
+constexpr int kStagingSlots = 3;
+
 }  // namespace
 
 struct mlv_context {
@@ -75,20 +104,31 @@ struct mlv_context {
     mlv_depth_stencil *ds = nullptr;
     int depth_func = MLV_DEPTH_GREATER_EQUAL;
     uint32_t *vis_tri = nullptr, *vis_draw = nullptr;
+    // recorded draws
+    std::vector<PendingDraw> pending;
+    uint64_t pending_tris = 0, pending_verts = 0;
     // arenas
-    Arena vpos, vattr, cnt, blocksum, blockoff_tri, blockoff_vtx, clip_list, clip_off, tvis, tshade;
-    Arena tile_cnt, tile_start, tile_fill, tile_list;
-    DrawCounters *ctr = nullptr;
+    Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, tvis, tshade;
+    Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist;
+    BatchCounters *ctr = nullptr;
     DevStats *dstats = nullptr;
-    DevStats seen{};
+    // pinned staging for the draw/view tables, one slot per flush in flight
+    void *staging[kStagingSlots] = {};
+    size_t staging_cap[kStagingSlots] = {};
+    hipEvent_t staging_ev[kStagingSlots] = {};
+    int staging_next = 0;
     // host-side stats (main.c:1228-1246)
     uint64_t vertex_count = 0, input_tris = 0, draws = 0;
     uint32_t draw_counter = 0;
     uint64_t capacity_errors = 0;
     size_t clip_hint = 4096, pair_hint = 0;
     size_t pair_limit = 0;  // MLV_PAIR_CAPACITY: caps the bin-list arena (exercises the overflow path)
-    // last draw, for inspection
-    uint32_t last_tri_cap = 0;
+    // stage profiling: event pairs recorded per stage, resolved at mlv_get_profile
+    bool profiling = false;
+    std::vector<hipEvent_t> ev_pool;
+    std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
+    double stage_ms[kStages] = {};
+    uint64_t prof_draws = 0, prof_flushes = 0;
 };
 
 static int fail(mlv_context *ctx, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
@@ -110,6 +150,12 @@ static int fail(mlv_context *ctx, int code, const char *fmt, ...) {
         if (e_ != hipSuccess)                                                                                 \
             return fail((ctx), e_ == hipErrorOutOfMemory ? MLV_ERROR_OUT_OF_MEMORY : MLV_ERROR_DEVICE,        \
                         "%s failed: %s", #call, hipGetErrorString(e_));                                       \
+    } while (0)
+
+#define RC_OR_RETURN(expr)             \
+    do {                               \
+        int rc_ = (expr);              \
+        if (rc_ != MLV_OK) return rc_; \
     } while (0)
 
 static int ensure(mlv_context *ctx, Arena &a, size_t bytes) {
@@ -138,6 +184,244 @@ static int set_device(mlv_context *ctx) {
     return MLV_OK;
 }
 
+static hipEvent_t take_event(mlv_context *ctx) {
+    if (!ctx->ev_pool.empty()) {
+        hipEvent_t e = ctx->ev_pool.back();
+        ctx->ev_pool.pop_back();
+        return e;
+    }
+    hipEvent_t e = nullptr;
+    if (hipEventCreate(&e) != hipSuccess) return nullptr;
+    return e;
+}
+
+static void resolve_events(mlv_context *ctx) {
+    for (auto &pe : ctx->ev_pending) {
+        float ms = 0.0f;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) ctx->stage_ms[pe.first] += ms;
+        ctx->ev_pool.push_back(pe.second.first);
+        ctx->ev_pool.push_back(pe.second.second);
+    }
+    ctx->ev_pending.clear();
+}
+
+// ------------------------------------------------------------------ flush
+
+static uint32_t align256(uint64_t v) { return (uint32_t)((v + 255u) & ~255ull); }
+
+// screen_from_ndc, main.c:826-831: entries computed in double and stored as f32
+static void screen_matrix(const mlv_viewport &vp, float S[4][4]) {
+    const float M[4][4] = {
+        {(float)((double)vp.width * 0.5), 0.0f, 0.0f, (float)((double)vp.width * 0.5 + (double)vp.top_left_x)},
+        {0.0f, (float)(-(double)vp.height * 0.5), 0.0f, (float)((double)vp.height * 0.5 + (double)vp.top_left_y)},
+        {0.0f, 0.0f, vp.max_depth - vp.min_depth, vp.min_depth},
+        {0.0f, 0.0f, 0.0f, 1.0f}};
+    memcpy(S, M, sizeof M);
+}
+
+static int flush(mlv_context *ctx) {
+    if (ctx->pending.empty()) return MLV_OK;
+    RC_OR_RETURN(set_device(ctx));
+    std::vector<PendingDraw> draws;
+    draws.swap(ctx->pending);
+    ctx->pending_tris = ctx->pending_verts = 0;
+
+    // view table (first appearance order) and slot offsets
+    std::vector<ViewKey> keys;
+    std::vector<ViewRec> views;
+    uint64_t total_tiles = 0, vslots = 0, tslots = 0;
+    for (auto &pd : draws) {
+        size_t vi = 0;
+        while (vi < keys.size() && !(keys[vi] == pd.key)) ++vi;
+        if (vi == keys.size()) {
+            keys.push_back(pd.key);
+            ViewRec v{};
+            v.color = pd.key.rt->ptr;
+            v.depth = pd.key.ds->ptr;
+            v.hiz = pd.key.ds->hiz;
+            v.vis_tri = pd.key.vis_tri;
+            v.vis_draw = pd.key.vis_draw;
+            v.W = (int32_t)pd.key.rt->width;
+            v.H = (int32_t)pd.key.rt->height;
+            v.tiles_x = (v.W + kTile - 1) / kTile;
+            v.tiles_y = (v.H + kTile - 1) / kTile;
+            v.bins_x = (int32_t)pd.key.ds->bins_x;
+            v.bins_y = (int32_t)pd.key.ds->bins_y;
+            v.tile_base = (uint32_t)total_tiles;
+            v.depth_func = pd.key.depth_func;
+            total_tiles += (uint64_t)v.tiles_x * v.tiles_y;
+            views.push_back(v);
+        }
+        DrawRec &r = pd.rec;
+        r.view = (uint32_t)vi;
+        r.vbase = (uint32_t)vslots;
+        r.blk_v0 = (uint32_t)(vslots / 256);
+        vslots += align256(r.vertex_count);
+        r.tbase = (uint32_t)tslots;
+        r.blk_t0 = (uint32_t)(tslots / 256);
+        tslots += align256(r.tri_count);
+    }
+    const uint32_t n_draws = (uint32_t)draws.size();
+    const uint32_t n_views = (uint32_t)views.size();
+    const uint64_t T = tslots, Vs = vslots;
+    const size_t clip_tris = std::max<size_t>(ctx->clip_hint, 4096);
+    const uint64_t tri_cap64 = T + 6ull * clip_tris;
+    const uint64_t rows64 = Vs + 9ull * clip_tris;
+    if (tri_cap64 > 0xF0000000ull || rows64 > 0xF0000000ull || total_tiles > 0xF0000000ull)
+        return fail(ctx, MLV_ERROR_CAPACITY, "flush too large for 32-bit indexing");
+    const uint32_t tri_cap = (uint32_t)tri_cap64;
+    const uint32_t vattr_rows = (uint32_t)rows64;
+    size_t pair_cap = std::max<size_t>(ctx->pair_hint, 3ull * T + 4ull * total_tiles + 1024);
+    pair_cap = std::min<size_t>(pair_cap, 0xF0000000ull);
+    if (ctx->pair_limit) pair_cap = std::min(pair_cap, ctx->pair_limit);
+    const uint32_t n_tblocks = (uint32_t)(T / 256), n_vblocks = (uint32_t)(Vs / 256);
+    const size_t scan_n =
+        std::max<size_t>({(size_t)n_tblocks, (size_t)tri_cap, (size_t)total_tiles, radix_hist_elems(pair_cap)});
+
+    RC_OR_RETURN(ensure(ctx, ctx->draws_dev, sizeof(DrawRec) * n_draws));
+    RC_OR_RETURN(ensure(ctx, ctx->views_dev, sizeof(ViewRec) * n_views));
+    RC_OR_RETURN(ensure(ctx, ctx->vpos, std::max<size_t>(Vs, 1) * 16));
+    RC_OR_RETURN(ensure(ctx, ctx->vattr, (size_t)vattr_rows * 32));
+    RC_OR_RETURN(ensure(ctx, ctx->cnt, std::max<size_t>(T, 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->blocksum, (size_t)(n_tblocks + 1) * 8));
+    RC_OR_RETURN(ensure(ctx, ctx->blockoff, (size_t)(n_tblocks + 1) * 8));
+    RC_OR_RETURN(ensure(ctx, ctx->asm_base, (size_t)n_draws * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->clip_list, std::max<size_t>(T, 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->clip_off, std::max<size_t>(T, 1) * 8));
+    RC_OR_RETURN(ensure(ctx, ctx->tvis, (size_t)tri_cap * sizeof(TriVis)));
+    RC_OR_RETURN(ensure(ctx, ctx->tshade, (size_t)tri_cap * sizeof(TriShade)));
+    RC_OR_RETURN(ensure(ctx, ctx->pair_cnt, (size_t)tri_cap * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->pair_off, (size_t)tri_cap * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->keys0, pair_cap * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->vals0, pair_cap * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->keys1, pair_cap * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->vals1, pair_cap * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->tile_cnt, (size_t)(total_tiles + 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->tile_start, (size_t)(total_tiles + 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_part_elems(scan_n) * 8));
+    RC_OR_RETURN(ensure(ctx, ctx->radix_hist, radix_hist_elems(pair_cap) * 4));
+
+    // tables through pinned staging; a slot is reused only after its copy completed
+    const size_t bytes_d = sizeof(DrawRec) * n_draws, bytes_v = sizeof(ViewRec) * n_views;
+    const int slot = ctx->staging_next;
+    ctx->staging_next = (slot + 1) % kStagingSlots;
+    if (ctx->staging_ev[slot]) HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->staging_ev[slot]));
+    if (ctx->staging_cap[slot] < bytes_d + bytes_v) {
+        if (ctx->staging[slot]) (void)hipHostFree(ctx->staging[slot]);
+        ctx->staging[slot] = nullptr;
+        ctx->staging_cap[slot] = 0;
+        size_t want = std::max<size_t>(bytes_d + bytes_v, 64 * 1024);
+        HIP_OR_FAIL(ctx, hipHostMalloc(&ctx->staging[slot], want, hipHostMallocDefault));
+        ctx->staging_cap[slot] = want;
+    }
+    if (!ctx->staging_ev[slot])
+        HIP_OR_FAIL(ctx, hipEventCreateWithFlags(&ctx->staging_ev[slot], hipEventDisableTiming));
+    DrawRec *stage_d = (DrawRec *)ctx->staging[slot];
+    for (uint32_t i = 0; i < n_draws; ++i) stage_d[i] = draws[i].rec;
+    memcpy((uint8_t *)ctx->staging[slot] + bytes_d, views.data(), bytes_v);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->draws_dev.p, ctx->staging[slot], bytes_d, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->views_dev.p, (uint8_t *)ctx->staging[slot] + bytes_d, bytes_v,
+                                    hipMemcpyHostToDevice, ctx->stream));
+    HIP_OR_FAIL(ctx, hipEventRecord(ctx->staging_ev[slot], ctx->stream));
+
+    BatchParams p{};
+    p.draws = (const DrawRec *)ctx->draws_dev.p;
+    p.views = (const ViewRec *)ctx->views_dev.p;
+    p.n_draws = n_draws;
+    p.n_views = n_views;
+    p.n_vblocks = n_vblocks;
+    p.n_tblocks = n_tblocks;
+    p.total_tiles = (uint32_t)total_tiles;
+    // clipper planes, main.c:652-657 via v4f32_normalize (math.h:267-270)
+    static const float raw[6][4] = {{1, 0, 0, 1}, {-1, 0, 0, 1}, {0, 1, 0, 1}, {0, -1, 0, 1}, {0, 0, 1, 1}, {0, 0, -1, 1}};
+    for (int i = 0; i < 6; ++i) {
+        float d = raw[i][0] * raw[i][0];
+        d = d + raw[i][1] * raw[i][1];
+        d = d + raw[i][2] * raw[i][2];
+        d = d + raw[i][3] * raw[i][3];
+        float len = (float)std::sqrt((double)d);
+        float ool = (float)(1.0 / (double)len);
+        for (int k = 0; k < 4; ++k) p.planes[i][k] = raw[i][k] * ool;
+    }
+    p.vpos = (float4 *)ctx->vpos.p;
+    p.vattr = (float4 *)ctx->vattr.p;
+    p.vslots = (uint32_t)Vs;
+    p.vattr_rows = vattr_rows;
+    p.cnt = (uint32_t *)ctx->cnt.p;
+    p.blocksum = (unsigned long long *)ctx->blocksum.p;
+    p.blockoff = (unsigned long long *)ctx->blockoff.p;
+    p.asm_base = (uint32_t *)ctx->asm_base.p;
+    p.clip_list = (uint32_t *)ctx->clip_list.p;
+    p.clip_off = (uint2 *)ctx->clip_off.p;
+    p.tvis = (TriVis *)ctx->tvis.p;
+    p.tshade = (TriShade *)ctx->tshade.p;
+    p.tri_cap = tri_cap;
+    p.pair_cnt = (uint32_t *)ctx->pair_cnt.p;
+    p.pair_off = (uint32_t *)ctx->pair_off.p;
+    p.keys0 = (uint32_t *)ctx->keys0.p;
+    p.vals0 = (uint32_t *)ctx->vals0.p;
+    p.keys1 = (uint32_t *)ctx->keys1.p;
+    p.vals1 = (uint32_t *)ctx->vals1.p;
+    p.pair_cap = (uint32_t)pair_cap;
+    p.tile_cnt = (uint32_t *)ctx->tile_cnt.p;
+    p.tile_start = (uint32_t *)ctx->tile_start.p;
+    p.ctr = ctx->ctr;
+    p.stats = ctx->dstats;
+    ScanScratch sc{ctx->scan_part.p, ctx->scan_part.cap / 8, (uint32_t *)ctx->radix_hist.p, ctx->radix_hist.cap / 4};
+
+    // keys are global tile indices < total_tiles: one 8-bit pass per byte they use
+    uint32_t bits = 0;
+    while (bits < 32 && (1ull << bits) < total_tiles) ++bits;
+    const uint32_t passes = std::max<uint32_t>(1u, (bits + 7u) / 8u);
+
+    for (int st = 0; st < kStages; ++st) {
+        hipEvent_t a = nullptr, b = nullptr;
+        if (ctx->profiling) {
+            a = take_event(ctx);
+            b = take_event(ctx);
+            if (!a || !b) return fail(ctx, MLV_ERROR_DEVICE, "hipEventCreate failed");
+            HIP_OR_FAIL(ctx, hipEventRecord(a, ctx->stream));
+        }
+        HIP_OR_FAIL(ctx, launch_stage(st, p, sc, passes, ctx->stream));
+        if (ctx->profiling) {
+            HIP_OR_FAIL(ctx, hipEventRecord(b, ctx->stream));
+            ctx->ev_pending.push_back({st, {a, b}});
+        }
+    }
+    if (ctx->profiling) {
+        ctx->prof_draws += n_draws;
+        ctx->prof_flushes++;
+        if (ctx->ev_pending.size() > 4096) {
+            HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+            resolve_events(ctx);
+        }
+    }
+    return MLV_OK;
+}
+
+// Flush when a recorded draw targets the render target or depth target given.
+static int flush_if_targets(mlv_context *ctx, const mlv_render_target *rt, const mlv_depth_stencil *ds) {
+    for (auto &pd : ctx->pending)
+        if ((rt && pd.key.rt == rt) || (ds && pd.key.ds == ds)) return flush(ctx);
+    return MLV_OK;
+}
+
+// Reads the device-side counters and turns capacity overflows into an error and a
+// larger arena for the next flushes.
+static int check_device_stats(mlv_context *ctx) {
+    DevStats s{};
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(&s, ctx->dstats, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    ctx->pair_hint = std::max<size_t>(ctx->pair_hint, (size_t)(s.max_pairs + s.max_pairs / 4));
+    ctx->clip_hint = std::max<size_t>(ctx->clip_hint, (size_t)(s.max_clip_list + s.max_clip_list / 4));
+    if (s.capacity_errors > ctx->capacity_errors) {
+        ctx->capacity_errors = s.capacity_errors;
+        ctx->clip_hint *= 4;
+        return fail(ctx, MLV_ERROR_CAPACITY, "a flush exceeded the clipper arena; capacity was raised, redraw");
+    }
+    return MLV_OK;
+}
+
 extern "C" {
 
 MLV_API int mlv_device_count(int *out) {
@@ -158,11 +442,12 @@ MLV_API int mlv_context_create(int device, mlv_context **out) {
     if (!ctx) return MLV_ERROR_OUT_OF_MEMORY;
     ctx->device = device;
     if (const char *lim = getenv("MLV_PAIR_CAPACITY")) ctx->pair_limit = (size_t)strtoull(lim, nullptr, 10);
-    if (hipSetDevice(device) != hipSuccess || hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipMalloc(&ctx->ctr, sizeof(DrawCounters)) != hipSuccess ||
+    if (hipSetDevice(device) != hipSuccess ||
+        hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipMalloc(&ctx->ctr, sizeof(BatchCounters)) != hipSuccess ||
         hipMalloc(&ctx->dstats, sizeof(DevStats)) != hipSuccess ||
         hipMemset(ctx->dstats, 0, sizeof(DevStats)) != hipSuccess ||
-        hipMemset(ctx->ctr, 0, sizeof(DrawCounters)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
+        hipMemset(ctx->ctr, 0, sizeof(BatchCounters)) != hipSuccess || hipDeviceSynchronize() != hipSuccess) {
         if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
         if (ctx->ctr) (void)hipFree(ctx->ctr);
         if (ctx->dstats) (void)hipFree(ctx->dstats);
@@ -176,11 +461,20 @@ MLV_API int mlv_context_create(int device, mlv_context **out) {
 MLV_API int mlv_context_destroy(mlv_context *ctx) {
     if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
     (void)hipSetDevice(ctx->device);
+    (void)flush(ctx);
     (void)hipStreamSynchronize(ctx->stream);
-    Arena *all[] = {&ctx->vpos,     &ctx->vattr,     &ctx->cnt,       &ctx->blocksum,  &ctx->blockoff_tri,
-                    &ctx->blockoff_vtx, &ctx->clip_list, &ctx->clip_off, &ctx->tvis,    &ctx->tshade,
-                    &ctx->tile_cnt, &ctx->tile_start, &ctx->tile_fill, &ctx->tile_list};
+    Arena *all[] = {&ctx->draws_dev, &ctx->views_dev, &ctx->vpos,     &ctx->vattr,     &ctx->cnt,
+                    &ctx->blocksum,  &ctx->blockoff,  &ctx->asm_base, &ctx->clip_list, &ctx->clip_off,
+                    &ctx->tvis,      &ctx->tshade,    &ctx->pair_cnt, &ctx->pair_off,  &ctx->keys0,
+                    &ctx->vals0,     &ctx->keys1,     &ctx->vals1,    &ctx->tile_cnt,  &ctx->tile_start,
+                    &ctx->scan_part, &ctx->radix_hist};
     for (Arena *a : all) release(*a);
+    resolve_events(ctx);
+    for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
+    for (int i = 0; i < kStagingSlots; ++i) {
+        if (ctx->staging[i]) (void)hipHostFree(ctx->staging[i]);
+        if (ctx->staging_ev[i]) (void)hipEventDestroy(ctx->staging_ev[i]);
+    }
     (void)hipFree(ctx->ctr);
     (void)hipFree(ctx->dstats);
     (void)hipStreamDestroy(ctx->stream);
@@ -192,27 +486,10 @@ MLV_API const char *mlv_get_last_error(const mlv_context *ctx) { return ctx ? ct
 
 MLV_API void *mlv_context_stream(mlv_context *ctx) { return ctx ? (void *)ctx->stream : nullptr; }
 
-// Reads the device-side counters and turns capacity overflows into an error and a
-// larger arena for the next draws.
-static int check_device_stats(mlv_context *ctx) {
-    DevStats s{};
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(&s, ctx->dstats, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
-    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-    ctx->seen = s;
-    // size the next draws' arenas from what the device saw
-    ctx->pair_hint = std::max<size_t>(ctx->pair_hint, (size_t)(s.max_pairs + s.max_pairs / 4));
-    ctx->clip_hint = std::max<size_t>(ctx->clip_hint, (size_t)(s.max_clip_list + s.max_clip_list / 4));
-    if (s.capacity_errors > ctx->capacity_errors) {
-        ctx->capacity_errors = s.capacity_errors;
-        ctx->clip_hint *= 4;
-        return fail(ctx, MLV_ERROR_CAPACITY, "a draw exceeded the clipper arena; capacity was raised, redraw");
-    }
-    return MLV_OK;
-}
-
 MLV_API int mlv_synchronize(mlv_context *ctx) {
     if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     return check_device_stats(ctx);
 }
@@ -222,7 +499,7 @@ MLV_API int mlv_synchronize(mlv_context *ctx) {
 MLV_API int mlv_buffer_create(mlv_context *ctx, int kind, size_t bytes, const void *host_init, mlv_buffer **out) {
     if (!ctx || !out || bytes == 0 || kind < MLV_BUFFER_VERTEX || kind > MLV_BUFFER_CONSTANT)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_buffer_create: bad argument");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
     void *p = nullptr;
     HIP_OR_FAIL(ctx, hipMalloc(&p, (bytes + 15) & ~(size_t)15));
     if (host_init) {
@@ -254,7 +531,8 @@ MLV_API int mlv_buffer_wrap(mlv_context *ctx, int kind, void *device_ptr, size_t
 MLV_API int mlv_buffer_update(mlv_context *ctx, mlv_buffer *buf, size_t offset, size_t bytes, const void *host_src) {
     if (!ctx || !buf || !host_src || offset + bytes > buf->bytes)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_buffer_update: bad argument");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // recorded draws read the old contents
     HIP_OR_FAIL(ctx, hipMemcpyAsync((uint8_t *)buf->ptr + offset, host_src, bytes, hipMemcpyHostToDevice, ctx->stream));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     return MLV_OK;
@@ -263,14 +541,17 @@ MLV_API int mlv_buffer_update(mlv_context *ctx, mlv_buffer *buf, size_t offset, 
 MLV_API int mlv_buffer_readback(mlv_context *ctx, const mlv_buffer *buf, size_t offset, size_t bytes, void *host_dst) {
     if (!ctx || !buf || !host_dst || offset + bytes > buf->bytes)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_buffer_readback: bad argument");
-    if (int rc = set_device(ctx)) return rc;
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, (const uint8_t *)buf->ptr + offset, bytes, hipMemcpyDeviceToHost, ctx->stream));
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, (const uint8_t *)buf->ptr + offset, bytes, hipMemcpyDeviceToHost,
+                                    ctx->stream));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     return check_device_stats(ctx);
 }
 
 MLV_API int mlv_buffer_destroy(mlv_context *ctx, mlv_buffer *buf) {
     if (!ctx || !buf) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)flush(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     if (buf->owned) (void)hipFree(buf->ptr);
     for (auto &c : ctx->vs_cb)
@@ -298,7 +579,7 @@ MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t heig
     if (!ctx || !out || !width || !height || !tb || width > 65536u || height > 65536u)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_texture2d_create: bad argument");
     if (mip_levels == 0) mip_levels = 1;
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
     const size_t bytes = (size_t)width * height * tb;
     void *p = nullptr;
     HIP_OR_FAIL(ctx, hipMalloc(&p, bytes));
@@ -321,6 +602,7 @@ MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t heig
 
 MLV_API int mlv_texture2d_destroy(mlv_context *ctx, mlv_texture2d *tex) {
     if (!ctx || !tex) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)flush(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(tex->ptr);
     for (auto &s : ctx->vs_srv)
@@ -336,7 +618,7 @@ MLV_API void *mlv_texture2d_device_ptr(const mlv_texture2d *tex) { return tex ? 
 MLV_API int mlv_render_target_create(mlv_context *ctx, uint32_t width, uint32_t height, mlv_render_target **out) {
     if (!ctx || !out || !width || !height || width > 16384u || height > 16384u)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_render_target_create: bad size");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
     uint32_t *p = nullptr;
     HIP_OR_FAIL(ctx, hipMalloc(&p, (size_t)width * height * 4));
     mlv_render_target *rt = new (std::nothrow) mlv_render_target{p, width, height};
@@ -350,6 +632,7 @@ MLV_API int mlv_render_target_create(mlv_context *ctx, uint32_t width, uint32_t 
 
 MLV_API int mlv_render_target_destroy(mlv_context *ctx, mlv_render_target *rt) {
     if (!ctx || !rt) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)flush(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(rt->ptr);
     if (ctx->rt == rt) ctx->rt = nullptr;
@@ -362,7 +645,7 @@ MLV_API void *mlv_render_target_device_ptr(const mlv_render_target *rt) { return
 MLV_API int mlv_depth_stencil_create(mlv_context *ctx, uint32_t width, uint32_t height, mlv_depth_stencil **out) {
     if (!ctx || !out || !width || !height || width > 16384u || height > 16384u)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_depth_stencil_create: bad size");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
     const uint32_t bx = (width + kBin - 1) / kBin, by = (height + kBin - 1) / kBin;
     float *p = nullptr, *h = nullptr;
     HIP_OR_FAIL(ctx, hipMalloc(&p, (size_t)width * height * 4));
@@ -389,6 +672,7 @@ MLV_API int mlv_depth_stencil_create(mlv_context *ctx, uint32_t width, uint32_t 
 
 MLV_API int mlv_depth_stencil_destroy(mlv_context *ctx, mlv_depth_stencil *ds) {
     if (!ctx || !ds) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)flush(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     (void)hipFree(ds->ptr);
     (void)hipFree(ds->hiz);
@@ -502,7 +786,8 @@ static uint32_t trunc_u32(float f) {
 
 MLV_API int mlv_clear_render_target_view(mlv_context *ctx, mlv_render_target *rt, const float rgba[4]) {
     if (!ctx || !rt || !rgba) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "clear_render_target_view: bad argument");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush_if_targets(ctx, rt, nullptr));
     uint32_t v = (trunc_u32(rgba[3] * 255.f) << 24) + (trunc_u32(rgba[2] * 255.f) << 16) +
                  (trunc_u32(rgba[1] * 255.f) << 8) + trunc_u32(rgba[0] * 255.f);
     HIP_OR_FAIL(ctx, launch_fill_u32(rt->ptr, v, (size_t)rt->width * rt->height, ctx->stream));
@@ -515,7 +800,8 @@ MLV_API int mlv_clear_render_target_view(mlv_context *ctx, mlv_render_target *rt
 
 MLV_API int mlv_clear_depth_stencil_view(mlv_context *ctx, mlv_depth_stencil *ds, float depth) {
     if (!ctx || !ds) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "clear_depth_stencil_view: bad argument");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush_if_targets(ctx, nullptr, ds));
     HIP_OR_FAIL(ctx, launch_fill_f32(ds->ptr, depth, (size_t)ds->width * ds->height, ctx->stream));
     // main.c:1212-1214: every tile minimum back to 0.0 (1.0 for the LESS_EQUAL mirror)
     const float h = ctx->depth_func == MLV_DEPTH_LESS_EQUAL ? 1.0f : 0.0f;
@@ -566,11 +852,14 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
         return fail(ctx, MLV_ERROR_INVALID_STATE, "env_lighting_ps needs a float RGBA texture in slot 0");
     if (ctx->ps == MLV_PS_BASIC && !ctx->ps_srv[0])
         return fail(ctx, MLV_ERROR_INVALID_STATE, "basic_ps needs a texture in slot 0");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
 
     const uint32_t T = index_count / 3u;
     const uint64_t vbytes = ctx->vb->bytes - ctx->vb_off;
-    const uint32_t V = (uint32_t)std::min<uint64_t>(vbytes / ctx->vb_stride, 0xFFFFFFFFull);
+    const uint32_t V = (uint32_t)std::min<uint64_t>(vbytes / ctx->vb_stride, 0x7FFFFFFFull);
+    if (V > kMaxBatchVertices || T > kMaxBatchTriangles)
+        return fail(ctx, MLV_ERROR_CAPACITY, "draw_indexed: at most %llu triangles and vertices per draw",
+                    (unsigned long long)kMaxBatchTriangles);
     ctx->vertex_count += index_count;
     ctx->input_tris += T;
     ctx->draws++;
@@ -578,116 +867,83 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
     if (T == 0) return MLV_OK;
 
     mlv_depth_stencil *ds = ctx->ds;
+    ViewKey key{ctx->rt, ds, ctx->vis_tri, ctx->vis_draw, ctx->depth_func};
+    // one depth function per target within a flush, and hierarchical-Z built for it
+    for (auto &pd : ctx->pending)
+        if ((pd.key.rt == key.rt || pd.key.ds == key.ds) && !(pd.key == key)) {
+            RC_OR_RETURN(flush(ctx));
+            break;
+        }
     if (ds->hiz_mode != ctx->depth_func) {
-        // hierarchical-Z built for the other comparison: reset to a value that rejects nothing
-        const float h = ctx->depth_func == MLV_DEPTH_LESS_EQUAL ? HUGE_VALF : -HUGE_VALF;
+        RC_OR_RETURN(flush_if_targets(ctx, nullptr, ds));
+        const float h = ctx->depth_func == MLV_DEPTH_LESS_EQUAL ? HUGE_VALF : -HUGE_VALF;  // rejects nothing
         HIP_OR_FAIL(ctx, launch_fill_f32(ds->hiz, h, (size_t)ds->bins_x * ds->bins_y, ctx->stream));
         ds->hiz_mode = ctx->depth_func;
     }
+    if (ctx->pending.size() >= kMaxBatchDraws || ctx->pending_tris + align256(T) > kMaxBatchTriangles ||
+        ctx->pending_verts + align256(V) > kMaxBatchVertices)
+        RC_OR_RETURN(flush(ctx));
 
-    const uint32_t tiles_x = (W + kTile - 1) / kTile, tiles_y = (H + kTile - 1) / kTile;
-    const uint32_t ntiles = tiles_x * tiles_y;
-    const uint32_t nblocks = (T + kSetupBlock - 1) / kSetupBlock;
-    const size_t clip_tris = std::max<size_t>(ctx->clip_hint, 4096);
-    const uint64_t tri_cap64 = (uint64_t)T + 6ull * clip_tris;
-    const uint32_t tri_cap = (uint32_t)std::min<uint64_t>(tri_cap64, 0xFFFFFF00ull);
-    const uint64_t rows64 = (uint64_t)V + 9ull * clip_tris;
-    const uint32_t vattr_rows = (uint32_t)std::min<uint64_t>(rows64, 0xFFFFFF00ull);
-    size_t pair_cap = std::max<size_t>(ctx->pair_hint, 3ull * T + 4ull * ntiles);
-    if (ctx->pair_limit) pair_cap = std::min(pair_cap, ctx->pair_limit);
-
-    int rc;
-    if ((rc = ensure(ctx, ctx->vpos, (size_t)std::max<uint32_t>(V, 1) * 16)) ||
-        (rc = ensure(ctx, ctx->vattr, (size_t)vattr_rows * 32)) || (rc = ensure(ctx, ctx->cnt, (size_t)T * 4)) ||
-        (rc = ensure(ctx, ctx->blocksum, (size_t)nblocks * 4)) ||
-        (rc = ensure(ctx, ctx->blockoff_tri, (size_t)nblocks * 4)) ||
-        (rc = ensure(ctx, ctx->blockoff_vtx, (size_t)nblocks * 4)) ||
-        (rc = ensure(ctx, ctx->clip_list, (size_t)T * 4)) || (rc = ensure(ctx, ctx->clip_off, (size_t)T * 8)) ||
-        (rc = ensure(ctx, ctx->tvis, (size_t)tri_cap * sizeof(TriVis))) ||
-        (rc = ensure(ctx, ctx->tshade, (size_t)tri_cap * sizeof(TriShade))) ||
-        (rc = ensure(ctx, ctx->tile_cnt, (size_t)ntiles * 4)) ||
-        (rc = ensure(ctx, ctx->tile_start, (size_t)ntiles * 4)) ||
-        (rc = ensure(ctx, ctx->tile_fill, (size_t)ntiles * 4)) ||
-        (rc = ensure(ctx, ctx->tile_list, pair_cap * 4)))
-        return rc;
-
-    DrawParams p{};
-    p.ib = (const uint32_t *)((const uint8_t *)ctx->ib->ptr + ctx->ib_off);
-    p.index_count = index_count;
-    p.start_index = start_index;
-    p.base_vertex = base_vertex;
-    p.tri_count = T;
-    p.vb = (const uint8_t *)ctx->vb->ptr + ctx->vb_off;
-    p.stride = ctx->vb_stride;
-    p.vertex_count = V;
-    p.cb = ctx->vs_cb[0] ? (const float *)ctx->vs_cb[0]->ptr : nullptr;
-    p.vs_tex = tex_desc(ctx->vs_srv[0]);
-    p.ps_tex = tex_desc(ctx->ps_srv[0]);
-    // screen_from_ndc, main.c:826-831: entries computed in double and stored as f32
-    const mlv_viewport &vp = ctx->vp;
-    float S[4][4] = {{(float)((double)vp.width * 0.5), 0.0f, 0.0f, (float)((double)vp.width * 0.5 + (double)vp.top_left_x)},
-                     {0.0f, (float)(-(double)vp.height * 0.5), 0.0f, (float)((double)vp.height * 0.5 + (double)vp.top_left_y)},
-                     {0.0f, 0.0f, vp.max_depth - vp.min_depth, vp.min_depth},
-                     {0.0f, 0.0f, 0.0f, 1.0f}};
-    memcpy(p.screen, S, sizeof S);
-    // clipper planes, main.c:652-657 via v4f32_normalize (math.h:267-270)
-    static const float raw[6][4] = {{1, 0, 0, 1}, {-1, 0, 0, 1}, {0, 1, 0, 1}, {0, -1, 0, 1}, {0, 0, 1, 1}, {0, 0, -1, 1}};
-    for (int i = 0; i < 6; ++i) {
-        float d = raw[i][0] * raw[i][0];
-        d = d + raw[i][1] * raw[i][1];
-        d = d + raw[i][2] * raw[i][2];
-        d = d + raw[i][3] * raw[i][3];
-        float len = (float)std::sqrt((double)d);
-        float ool = (float)(1.0 / (double)len);
-        for (int k = 0; k < 4; ++k) p.planes[i][k] = raw[i][k] * ool;
-    }
-    p.W = (int32_t)W;
-    p.H = (int32_t)H;
-    p.tiles_x = (int32_t)tiles_x;
-    p.tiles_y = (int32_t)tiles_y;
-    p.bins_x = (int32_t)ds->bins_x;
-    p.bins_y = (int32_t)ds->bins_y;
-    p.depth_func = ctx->depth_func;
-    // The in-draw depth cull relies on exact (unwrapped) edge values: |E| < 2^31 at every pixel.
-    p.depth_cull = (vp.top_left_x == 0.0f && vp.top_left_y == 0.0f &&
+    PendingDraw pd{};
+    DrawRec &r = pd.rec;
+    r.ib = (const uint32_t *)((const uint8_t *)ctx->ib->ptr + ctx->ib_off) + start_index;
+    r.vb = (const uint8_t *)ctx->vb->ptr + ctx->vb_off;
+    r.cb = ctx->vs_cb[0] ? (const float *)ctx->vs_cb[0]->ptr : nullptr;
+    r.vs_tex = tex_desc(ctx->vs_srv[0]);
+    r.ps_tex = tex_desc(ctx->ps_srv[0]);
+    screen_matrix(ctx->vp, r.screen);
+    r.stride = ctx->vb_stride;
+    r.vertex_count = V;
+    r.tri_count = T;
+    r.base_vertex = base_vertex;
+    r.vs = ctx->vs;
+    r.ps = ctx->ps;
+    r.depth_func = ctx->depth_func;
+    // the conservative in-draw depth cull relies on exact edge values: |E| < 2^31 at every pixel
+    r.depth_cull = (ctx->vp.top_left_x == 0.0f && ctx->vp.top_left_y == 0.0f &&
                     2.0 * (16.0 * W + 2.0) * (16.0 * H + 2.0) < 2147483648.0 * 0.99)
                        ? 1
                        : 0;
-    p.draw_tag = draw_tag;
-    p.color = ctx->rt->ptr;
-    p.depth = ds->ptr;
-    p.hiz = ds->hiz;
-    p.vis_tri = ctx->vis_tri;
-    p.vis_draw = ctx->vis_draw;
-    p.vpos = (float4 *)ctx->vpos.p;
-    p.vattr = (float4 *)ctx->vattr.p;
-    p.vattr_rows = vattr_rows;
-    p.cnt = (uint32_t *)ctx->cnt.p;
-    p.blocksum = (uint32_t *)ctx->blocksum.p;
-    p.blockoff_tri = (uint32_t *)ctx->blockoff_tri.p;
-    p.blockoff_vtx = (uint32_t *)ctx->blockoff_vtx.p;
-    p.nblocks = nblocks;
-    p.clip_list = (uint32_t *)ctx->clip_list.p;
-    p.clip_off = (uint2 *)ctx->clip_off.p;
-    p.tvis = (TriVis *)ctx->tvis.p;
-    p.tshade = (TriShade *)ctx->tshade.p;
-    p.tri_cap = tri_cap;
-    p.tile_cnt = (uint32_t *)ctx->tile_cnt.p;
-    p.tile_start = (uint32_t *)ctx->tile_start.p;
-    p.tile_fill = (uint32_t *)ctx->tile_fill.p;
-    p.tile_list = (uint32_t *)ctx->tile_list.p;
-    p.pair_cap = (uint32_t)std::min<size_t>(pair_cap, 0xFFFFFFFFull);
-    p.ctr = ctx->ctr;
-    p.stats = ctx->dstats;
-    ctx->last_tri_cap = tri_cap;
-
-    HIP_OR_FAIL(ctx, launch_draw(p, ctx->vs, ctx->ps, ctx->stream));
+    r.draw_tag = draw_tag;
+    pd.key = key;
+    ctx->pending.push_back(pd);
+    ctx->pending_tris += align256(T);
+    ctx->pending_verts += align256(V);
     return MLV_OK;
+}
+
+MLV_API int mlv_context_set_profiling(mlv_context *ctx, int enable) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    RC_OR_RETURN(flush(ctx));
+    ctx->profiling = enable != 0;
+    return MLV_OK;
+}
+
+MLV_API int mlv_get_profile(mlv_context *ctx, mlv_profile *out) {
+    if (!ctx || !out) return MLV_ERROR_INVALID_ARGUMENT;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    resolve_events(ctx);
+    for (int i = 0; i < kStages; ++i) {
+        out->stage_ms[i] = ctx->stage_ms[i];
+        ctx->stage_ms[i] = 0.0;
+    }
+    out->draws = ctx->prof_draws;
+    out->flushes = ctx->prof_flushes;
+    ctx->prof_draws = ctx->prof_flushes = 0;
+    return MLV_OK;
+}
+
+MLV_API int mlv_flush(mlv_context *ctx) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    return flush(ctx);
 }
 
 MLV_API int mlv_render_target_readback(mlv_context *ctx, mlv_render_target *rt, void *host_dst) {
     if (!ctx || !rt || !host_dst) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "readback: bad argument");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
     HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, rt->ptr, (size_t)rt->width * rt->height * 4, hipMemcpyDeviceToHost,
                                     ctx->stream));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
@@ -696,7 +952,8 @@ MLV_API int mlv_render_target_readback(mlv_context *ctx, mlv_render_target *rt, 
 
 MLV_API int mlv_depth_stencil_readback(mlv_context *ctx, mlv_depth_stencil *ds, float *host_dst) {
     if (!ctx || !ds || !host_dst) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "readback: bad argument");
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
     HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, ds->ptr, (size_t)ds->width * ds->height * 4, hipMemcpyDeviceToHost,
                                     ctx->stream));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
@@ -705,7 +962,8 @@ MLV_API int mlv_depth_stencil_readback(mlv_context *ctx, mlv_depth_stencil *ds, 
 
 MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out) {
     if (!ctx || !out) return MLV_ERROR_INVALID_ARGUMENT;
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
     DevStats s{};
     HIP_OR_FAIL(ctx, hipMemcpyAsync(&s, ctx->dstats, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
@@ -722,12 +980,14 @@ MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out) {
 
 MLV_API int mlv_reset_stats(mlv_context *ctx) {
     if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
-    if (int rc = set_device(ctx)) return rc;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     HIP_OR_FAIL(ctx, hipMemset(ctx->dstats, 0, sizeof(DevStats)));
     HIP_OR_FAIL(ctx, hipDeviceSynchronize());
     ctx->vertex_count = ctx->input_tris = ctx->draws = 0;
     ctx->capacity_errors = 0;
+    ctx->draw_counter = 0;
     return MLV_OK;
 }
 

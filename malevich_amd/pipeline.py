@@ -235,5 +235,21 @@ class Device:
         _check(self.ctx, self.lib.mlv_get_stats(self.ctx, C.byref(s)))
         return {k: getattr(s, k) for k, _ in L.Stats._fields_}
 
+    def set_profiling(self, enable: bool) -> None:
+        _check(self.ctx, self.lib.mlv_context_set_profiling(self.ctx, int(bool(enable))))
+
+    def profile(self) -> dict:
+        """Per-stage GPU milliseconds since the last call (HIP events on the context stream)."""
+        pr = L.Profile()
+        _check(self.ctx, self.lib.mlv_get_profile(self.ctx, C.byref(pr)))
+        out = {name: pr.stage_ms[i] for i, name in enumerate(L.STAGES)}
+        out["draws"] = pr.draws
+        out["flushes"] = pr.flushes
+        return out
+
+    def flush(self) -> None:
+        """Enqueue the recorded draws now (they otherwise run when their results are needed)."""
+        _check(self.ctx, self.lib.mlv_flush(self.ctx))
+
     def reset_stats(self) -> None:
         _check(self.ctx, self.lib.mlv_reset_stats(self.ctx))

@@ -65,6 +65,12 @@ def camera_cb(eye, target, fov_y_deg: float, width: int, height: int, near: floa
 # ------------------------------------------------------------------ meshes
 
 
+_ICO_FACES = np.array([(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4),
+                       (11, 10, 2), (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+                       (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)], np.int64)
+_GEODESIC_CACHE: dict = {}
+
+
 def geodesic_sphere(freq: int) -> tuple[np.ndarray, np.ndarray]:
     """Class-I geodesic sphere: each icosahedron face split into freq^2 triangles.
 
@@ -72,46 +78,49 @@ def geodesic_sphere(freq: int) -> tuple[np.ndarray, np.ndarray]:
     counter-clockwise about the outward normal. V = 10*freq^2 + 2, F = 20*freq^2
     (freq = 32 gives 10,242 vertices and 20,480 triangles).
     """
+    n = int(freq)
+    if n in _GEODESIC_CACHE:
+        p, f = _GEODESIC_CACHE[n]
+        return p.copy(), f.copy()
     t = (1.0 + 5 ** 0.5) / 2.0
     base = np.array([[-1, t, 0], [1, t, 0], [-1, -t, 0], [1, -t, 0], [0, -1, t], [0, 1, t], [0, -1, -t],
                      [0, 1, -t], [t, 0, -1], [t, 0, 1], [-t, 0, -1], [-t, 0, 1]], np.float64)
     base /= np.linalg.norm(base, axis=1, keepdims=True)
-    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11), (1, 5, 9), (5, 11, 4), (11, 10, 2),
-             (10, 7, 6), (7, 1, 8), (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9), (4, 9, 5), (2, 4, 11),
-             (6, 2, 10), (8, 6, 7), (9, 8, 1)]
-    n = int(freq)
-    key_to_idx: dict = {}
-    verts: list = []
-
-    def vid(a: int, b: int, c: int, i: int, j: int) -> int:
-        # barycentric lattice point (n-i-j, i, j) of face (a,b,c); keyed by the sorted
-        # corner weights so points on shared edges and corners get one index
-        w = {a: n - i - j, b: i, c: j}
-        key = tuple(sorted((k, v) for k, v in w.items() if v))
-        idx = key_to_idx.get(key)
-        if idx is None:
-            p = (base[a] * (n - i - j) + base[b] * i + base[c] * j) / n
-            idx = len(verts)
-            verts.append(p / np.linalg.norm(p))
-            key_to_idx[key] = idx
-        return idx
-
-    out = []
-    for a, b, c in faces:
-        for i in range(n):
-            for j in range(n - i):
-                v0, v1, v2 = vid(a, b, c, i, j), vid(a, b, c, i + 1, j), vid(a, b, c, i, j + 1)
-                out.append((v0, v1, v2))
-                if i + j < n - 1:
-                    v3 = vid(a, b, c, i + 1, j + 1)
-                    out.append((v1, v3, v2))
-    pos = np.array(verts)
-    f = np.array(out, np.int64)
-    # orient every face counter-clockwise about the outward normal
+    ii, jj = np.meshgrid(np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    m = (ii + jj) <= n
+    li, lj = ii[m], jj[m]                      # lattice points of one face
+    lut = -np.ones((n + 1, n + 1), np.int64)
+    lut[li, lj] = np.arange(li.size)
+    nf = len(_ICO_FACES)
+    # weights on the face corners (a, b, c) = (n-i-j, i, j); a canonical key per point:
+    # (corner id, weight) pairs sorted by corner id, so shared edges/corners coincide
+    corners = np.repeat(_ICO_FACES[:, None, :], li.size, 1)                  # (F, L, 3)
+    w = np.stack([n - li - lj, li, lj], 1)[None].repeat(nf, 0)               # (F, L, 3)
+    ckey = np.where(w > 0, corners, 12)           # zero-weight corners sort last
+    order = np.argsort(ckey, axis=2)
+    cs = np.take_along_axis(ckey, order, 2)
+    ws = np.take_along_axis(w, order, 2)
+    key = ((cs[..., 0] * (n + 1) + ws[..., 0]) * 13 * (n + 1) + cs[..., 1] * (n + 1) + ws[..., 1]) \
+        * 13 * (n + 1) + cs[..., 2] * (n + 1) + ws[..., 2]
+    uk, first, inv = np.unique(key.ravel(), return_index=True, return_inverse=True)
+    csf, wsf = np.minimum(cs.reshape(-1, 3)[first], 11), ws.reshape(-1, 3)[first]
+    pos = (base[csf[:, 0]] * wsf[:, 0:1] + base[csf[:, 1]] * wsf[:, 1:2] + base[csf[:, 2]] * wsf[:, 2:3]) / n
+    pos /= np.linalg.norm(pos, axis=1, keepdims=True)
+    gid = inv.reshape(nf, li.size)
+    # triangles of one face in lattice coordinates
+    tri = []
+    for i in range(n):
+        for j in range(n - i):
+            tri.append((lut[i, j], lut[i + 1, j], lut[i, j + 1]))
+            if i + j < n - 1:
+                tri.append((lut[i + 1, j], lut[i + 1, j + 1], lut[i, j + 1]))
+    tri = np.array(tri, np.int64)
+    f = gid[:, tri].reshape(-1, 3)
     nrm = np.cross(pos[f[:, 1]] - pos[f[:, 0]], pos[f[:, 2]] - pos[f[:, 0]])
     flip = (nrm * pos[f].mean(axis=1)).sum(1) < 0
     f[flip] = f[flip][:, [0, 2, 1]]
-    return pos, f
+    _GEODESIC_CACHE[n] = (pos, f)
+    return pos.copy(), f.copy()
 
 
 def value_noise3(p: np.ndarray, seed: int, octaves: int = 3, base_freq: float = 2.0) -> np.ndarray:
@@ -153,7 +162,8 @@ def vertex_normals(pos: np.ndarray, faces: np.ndarray) -> np.ndarray:
     fn = np.cross(pos[faces[:, 1]] - pos[faces[:, 0]], pos[faces[:, 2]] - pos[faces[:, 0]])
     n = np.zeros_like(pos)
     for k in range(3):
-        np.add.at(n, faces[:, k], fn)
+        for c in range(3):
+            n[:, c] += np.bincount(faces[:, k], weights=fn[:, c], minlength=len(pos))
     return n / np.maximum(np.linalg.norm(n, axis=1, keepdims=True), 1e-30)
 
 
@@ -194,8 +204,13 @@ def random_rgba8(size: int, seed: int) -> np.ndarray:
 
 
 def smooth_env_map(width: int, height: int, seed: int, lo: float = 0.05, hi: float = 2.0) -> np.ndarray:
-    """(h, w, 4) float32 equirect map of seeded smooth noise, alpha 1 (range like the
-    reference's irradiance asset, Appendix of the asset notes: about [0.085, 2.11])."""
+    """(h, w, 4) float32 equirect map of seeded smooth noise, alpha 1, values in [lo, hi]
+    (the reference's irradiance asset spans about [0.085, 2.11]). Maps wider than 256
+    texels are the bilinear upsampling of a 256-wide noise map (the noise is smooth at
+    that scale), which keeps generation fast."""
+    if width > 256:
+        small = smooth_env_map(256, max(1, 256 * height // width), seed, lo, hi)
+        return _resize_bilinear(small, height, width)
     yy, xx = np.meshgrid(np.arange(height), np.arange(width), indexing="ij")
     theta = (yy + 0.5) / height * np.pi
     phi = (xx + 0.5) / width * 2 * np.pi
@@ -205,6 +220,18 @@ def smooth_env_map(width: int, height: int, seed: int, lo: float = 0.05, hi: flo
         out[:, c] = lo + (hi - lo) * 0.5 * (1.0 + value_noise3(d, seed * 7 + c, octaves=4, base_freq=1.5))
     out[:, 3] = 1.0
     return out.reshape(height, width, 4).astype(np.float32)
+
+
+def _resize_bilinear(img: np.ndarray, h: int, w: int) -> np.ndarray:
+    sh, sw = img.shape[:2]
+    ys = np.clip((np.arange(h) + 0.5) * sh / h - 0.5, 0, sh - 1)
+    xs = np.clip((np.arange(w) + 0.5) * sw / w - 0.5, 0, sw - 1)
+    y0, x0 = np.floor(ys).astype(int), np.floor(xs).astype(int)
+    y1, x1 = np.minimum(y0 + 1, sh - 1), np.minimum(x0 + 1, sw - 1)
+    fy, fx = (ys - y0)[:, None, None], (xs - x0)[None, :, None]
+    top = img[y0][:, x0] * (1 - fx) + img[y0][:, x1] * fx
+    bot = img[y1][:, x0] * (1 - fx) + img[y1][:, x1] * fx
+    return (top * (1 - fy) + bot * fy).astype(np.float32)
 
 
 # ------------------------------------------------------------------ scenes
@@ -310,3 +337,53 @@ def c1_scene(width: int = 640, height: int = 360, seed: int = 1):
     cb = camera_cb(eye=(3.0, 0.0, 0.0), target=(0.0, 0.0, 0.0), fov_y_deg=60.0, width=width, height=height)
     tex = random_rgba8(512, seed)
     return mesh, cb, tex
+
+
+def random_rotation(rng) -> np.ndarray:
+    q = rng.normal(size=4)
+    q /= np.linalg.norm(q)
+    a, b, c, d = q
+    return np.array([[a * a + b * b - c * c - d * d, 2 * (b * c - a * d), 2 * (b * d + a * c)],
+                     [2 * (b * c + a * d), a * a - b * b + c * c - d * d, 2 * (c * d - a * b)],
+                     [2 * (b * d - a * c), 2 * (c * d + a * b), a * a - b * b - c * c + d * d]])
+
+
+@dataclass
+class C2Scene:
+    instances: list            # Mesh per instance, transform baked into the vertices
+    textures: list             # (1024, 1024) uint32 RGBA8 per instance
+    cb: np.ndarray
+    width: int
+    height: int
+    depth_complexity: float
+
+    @property
+    def triangle_count(self) -> int:
+        return sum(m.triangle_count for m in self.instances)
+
+
+def c2_scene(width: int = 1920, height: int = 1080, n_instances: int = 64, seed: int = 2,
+             texture_size: int = 1024, target_complexity: float = 4.0) -> C2Scene:
+    """seeded displaced spheres of 16k-64k triangles (geodesic
+    frequency 29..56, about 2M triangles for 64 instances) at random positions in a
+    20-unit cube, random rotations, scale 0.5-2 times a common factor chosen so the
+    summed projected sphere area is about 4 screens (mean depth complexity 4)."""
+    rng = np.random.default_rng(seed)
+    freqs = rng.integers(29, 57, n_instances)
+    centers = rng.uniform(-10.0, 10.0, (n_instances, 3))
+    scales = rng.uniform(0.5, 2.0, n_instances)
+    eye = np.array([30.0, 0.0, 0.0])
+    fov = 60.0
+    f_px = (height / 2.0) / math.tan(math.radians(fov) / 2.0)
+    z = np.abs(eye[0] - centers[:, 0])
+    proj = np.pi * (f_px * scales / z) ** 2
+    k = math.sqrt(target_complexity * width * height / proj.sum())
+    cb = camera_cb(eye, (0.0, 0.0, 0.0), fov, width, height)
+    inst, texs = [], []
+    for i in range(n_instances):
+        T = np.eye(4)
+        T[:3, :3] = random_rotation(rng) * (scales[i] * k)
+        T[:3, 3] = centers[i]
+        inst.append(sphere_mesh(int(freqs[i]), displace=0.15, seed=seed * 1000 + i, transform=T))
+        texs.append(random_rgba8(texture_size, seed * 1000 + i) if texture_size else None)
+    return C2Scene(inst, texs, cb, width, height, float((proj * k * k).sum() / (width * height)))

@@ -146,7 +146,7 @@ def test_out_of_range_indices_are_dropped_and_counted(device, oracle):
 
 def test_empty_and_degenerate_draws(device, oracle):
     W, H = 64, 48
-    d0 = DrawSpec(np.zeros(0, np.uint32), S.SUPREMATISM_VERTICES, "passthrough", "passthrough")
+    d0 = DrawSpec(np.zeros(3, np.uint32), S.SUPREMATISM_VERTICES, "passthrough", "passthrough", index_count=0)
     d1 = DrawSpec(np.zeros(24, np.uint32), S.SUPREMATISM_VERTICES, "passthrough", "passthrough")
     g, o = both(device, oracle, W, H, [d0, d1])
     assert_frames_match(g, o, color_tol=0)
