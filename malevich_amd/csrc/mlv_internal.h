@@ -12,7 +12,7 @@
 //   tvis[N]          TriVis    80 B per assembled triangle (edge functions, depth plane, bin rect)
 //   tshade[N]        TriShade  32 B per assembled triangle (1/w, attribute rows)
 //   pair keys/vals   u32       (global tile, triangle) pairs, stably sorted by tile
-//   tile_cnt/start   u32       per global tile (tiles of all views, view-major)
+//   tile_start/end   u32       per global tile (tiles of all views, view-major): its pair range
 // Slots of draw d: vertices [vbase, vbase+V_d), input triangles [tbase, tbase+T_d); both
 // start on a 256 boundary so every block of the vertex and setup stages belongs to one draw.
 // The render target is the reference's row-major u32 frame_buffer and f32 depth_buffer
@@ -132,7 +132,7 @@ struct BatchParams {
     uint32_t *pair_cnt, *pair_off;        // per assembled triangle
     uint32_t *keys0, *vals0, *keys1, *vals1;
     uint32_t pair_cap;
-    uint32_t *tile_cnt, *tile_start;
+    uint32_t *tile_cnt, *tile_start;      // per global tile: end and start of its range in the sorted pairs
     BatchCounters *ctr;
     DevStats *stats;
 };

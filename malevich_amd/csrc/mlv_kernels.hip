@@ -7,10 +7,11 @@
 //                    kb_setup_totals     totals, capacity checks, first assembled index per draw
 //                    kb_setup_emit       TriVis/TriShade at the ordered slot (main.c:751-908)
 //                    kb_clip_emit        clipped fan triangles + clipper vertices
-// Stage 2, binning:  kb_pairs<false>     (triangle, 16x16 tile) counts and the tile histogram
-//                    scans               pair offsets (triangle order) and tile starts
+// Stage 2, binning:  kb_pairs<false>     (triangle, 16x16 tile) pair count per triangle
+//                    scan                pair offsets (triangle order)
 //                    kb_pairs<true>      pairs in triangle order
 //                    radix passes        stable LSD sort by tile: lists stay in submission order
+//                    kb_tile_ranges      each tile's [start, end) read off the sorted keys
 // Stage 3, tile:     kb_tile             one block per 16x16 tile of every view, one wave per
 //                                        8x8 bin: hierarchical-Z, corner-sampled coverage,
 //                                        early-Z in registers, the pixel shader of the surviving
@@ -108,28 +109,42 @@ __device__ __forceinline__ bool load_tri_indices(const DrawRec &d, uint32_t t, u
 }
 
 // The clipper's polygon: the three vertices' registers, then the six planes (main.c:649-660).
-__device__ __noinline__ int clip_triangle(const BatchParams &p, const DrawRec &d, uint32_t t, float (*poly)[12],
-                                          float (*tmp)[12]) {
+// Two LDS polygons per lane (ping-pong); the result is left in `a` and its size returned.
+constexpr int kClipLanes = 64;
+constexpr size_t kClipLdsFloats = (size_t)2 * kMaxClip * 12 * kClipLanes;
+
+__device__ __noinline__ int clip_triangle(const BatchParams &p, const DrawRec &d, uint32_t t, LdsPoly &a,
+                                          LdsPoly &b) {
     uint32_t vi[3];
     if (!load_tri_indices(d, t, vi)) return 0;
     for (int k = 0; k < 3; ++k) {
         const uint32_t slot = d.vbase + vi[k];
-        float4 a = p.vpos[slot], b = p.vattr[2u * slot], c = p.vattr[2u * slot + 1u];
-        poly[k][0] = a.x; poly[k][1] = a.y; poly[k][2] = a.z; poly[k][3] = a.w;
-        poly[k][4] = b.x; poly[k][5] = b.y; poly[k][6] = b.z; poly[k][7] = b.w;
-        poly[k][8] = c.x; poly[k][9] = c.y; poly[k][10] = c.z; poly[k][11] = c.w;
+        const float4 x = p.vpos[slot], y = p.vattr[2u * slot], z = p.vattr[2u * slot + 1u];
+        a.at(k, 0) = x.x; a.at(k, 1) = x.y; a.at(k, 2) = x.z; a.at(k, 3) = x.w;
+        a.at(k, 4) = y.x; a.at(k, 5) = y.y; a.at(k, 6) = y.z; a.at(k, 7) = y.w;
+        a.at(k, 8) = z.x; a.at(k, 9) = z.y; a.at(k, 10) = z.z; a.at(k, 11) = z.w;
     }
     int n = 3;
-    for (int pl = 0; pl < 6; pl += 2) {
-        n = clip_by_plane(poly, n, p.planes[pl], tmp);
+    for (int pl = 0; pl < 6; ++pl) {
+        // Sutherland-Hodgman returns the polygon unchanged, in order, when every vertex is
+        // inside; skipping that plane is exact and saves the copy
+        bool all_in = true;
+        for (int k = 0; k < n; ++k)
+            all_in = all_in && dot4_seq(p.planes[pl][0], p.planes[pl][1], p.planes[pl][2], p.planes[pl][3], a.at(k, 0),
+                                        a.at(k, 1), a.at(k, 2), a.at(k, 3)) > -0.0f;
+        if (all_in) continue;
+        n = clip_by_plane_lds(a, n, p.planes[pl], b);
         if (n < 0) return 0;
-        n = clip_by_plane(tmp, n, p.planes[pl + 1], poly);
-        if (n < 0) return 0;
+        LdsPoly t2 = a;
+        a = b;
+        b = t2;
     }
     return n;
 }
 
-__device__ __forceinline__ float4 poly_pos(const float *v) { return make_float4(v[0], v[1], v[2], v[3]); }
+__device__ __forceinline__ float4 poly_pos(const LdsPoly &q, int v) {
+    return make_float4(q.at(v, 0), q.at(v, 1), q.at(v, 2), q.at(v, 3));
+}
 
 __device__ __forceinline__ uint32_t draw_of_slot(const BatchParams &p, uint32_t slot) {
     return upper_index(p.n_draws, slot, [&](uint32_t i) { return p.draws[i].tbase; });
@@ -269,8 +284,10 @@ __global__ __launch_bounds__(256) void kb_reset(BatchParams p) {
         BatchCounters z = {};
         *p.ctr = z;
     }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.total_tiles; i += gridDim.x * blockDim.x)
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.total_tiles; i += gridDim.x * blockDim.x) {
+        p.tile_start[i] = 0u;
         p.tile_cnt[i] = 0u;
+    }
 }
 
 __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
@@ -307,18 +324,19 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
 }
 
 __global__ __launch_bounds__(64) void kb_clip_count(BatchParams p) {
+    __shared__ float s_poly[kClipLdsFloats];
     const uint32_t n = p.ctr->n_clip_list;
-    float poly[kMaxClip][12], tmp[kMaxClip][12];
     for (uint32_t s = blockIdx.x * 64u + threadIdx.x; s < n; s += gridDim.x * 64u) {
         const uint32_t slot = p.clip_list[s];
         const DrawRec &d = p.draws[draw_of_slot(p, slot)];
         const ViewRec &v = p.views[d.view];
-        int nv = clip_triangle(p, d, slot - d.tbase, poly, tmp);
+        LdsPoly pa{s_poly + threadIdx.x}, pb{s_poly + (size_t)kMaxClip * 12 * kClipLanes + threadIdx.x};
+        int nv = clip_triangle(p, d, slot - d.tbase, pa, pb);
         uint32_t ntri = 0u;
         for (int ci = 1; ci < nv - 1; ++ci) {
             TriVis tv;
             float rw[3];
-            if (setup_triangle(d, v, poly_pos(poly[0]), poly_pos(poly[ci]), poly_pos(poly[ci + 1]), tv, rw)) ntri++;
+            if (setup_triangle(d, v, poly_pos(pa, 0), poly_pos(pa, ci), poly_pos(pa, ci + 1), tv, rw)) ntri++;
         }
         uint32_t packed = ntri ? (ntri | ((uint32_t)nv << 16)) : 0u;
         p.cnt[slot] = packed;
@@ -391,8 +409,8 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
 }
 
 __global__ __launch_bounds__(64) void kb_clip_emit(BatchParams p) {
+    __shared__ float s_poly[kClipLdsFloats];
     const uint32_t n = p.ctr->n_clip_list;
-    float poly[kMaxClip][12], tmp[kMaxClip][12];
     for (uint32_t s = blockIdx.x * 64u + threadIdx.x; s < n; s += gridDim.x * 64u) {
         const uint32_t slot = p.clip_list[s];
         if (p.cnt[slot] == 0u) continue;
@@ -401,19 +419,20 @@ __global__ __launch_bounds__(64) void kb_clip_emit(BatchParams p) {
         const ViewRec &v = p.views[d.view];
         const uint2 off = p.clip_off[slot];
         const uint32_t t = slot - d.tbase;
-        int nv = clip_triangle(p, d, t, poly, tmp);
+        LdsPoly pa{s_poly + threadIdx.x}, pb{s_poly + (size_t)kMaxClip * 12 * kClipLanes + threadIdx.x};
+        int nv = clip_triangle(p, d, t, pa, pb);
         const uint32_t row0 = p.vslots + off.y;
         for (int k = 0; k < nv; ++k) {
             uint32_t row = row0 + (uint32_t)k;
             if (row >= p.vattr_rows) break;
-            p.vattr[2u * row] = make_float4(poly[k][4], poly[k][5], poly[k][6], poly[k][7]);
-            p.vattr[2u * row + 1u] = make_float4(poly[k][8], poly[k][9], poly[k][10], poly[k][11]);
+            p.vattr[2u * row] = make_float4(pa.at(k, 4), pa.at(k, 5), pa.at(k, 6), pa.at(k, 7));
+            p.vattr[2u * row + 1u] = make_float4(pa.at(k, 8), pa.at(k, 9), pa.at(k, 10), pa.at(k, 11));
         }
         uint32_t j = 0u;
         for (int ci = 1; ci < nv - 1; ++ci) {
             TriVis tv;
             float rw[3];
-            if (setup_triangle(d, v, poly_pos(poly[0]), poly_pos(poly[ci]), poly_pos(poly[ci + 1]), tv, rw)) {
+            if (setup_triangle(d, v, poly_pos(pa, 0), poly_pos(pa, ci), poly_pos(pa, ci + 1), tv, rw)) {
                 write_triangle(p, off.x + j, di, tv, rw, row0, row0 + (uint32_t)ci, row0 + (uint32_t)ci + 1u, t,
                                (uint32_t)(ci - 1));
                 j++;
@@ -442,8 +461,8 @@ __device__ __forceinline__ uint32_t tile_rect(uint32_t rect0, uint32_t rect1, in
     return (uint32_t)tw * (uint32_t)((by1 >> 1) - ty0 + 1);
 }
 
-// EMIT = false: per-triangle pair counts and the tile histogram. EMIT = true: the pairs in
-// triangle order at the scanned offsets. Triangles spanning more than kBigRect tiles are
+// EMIT = false: per-triangle pair counts. EMIT = true: the pairs in triangle order at the
+// scanned offsets. Triangles spanning more than kBigRect tiles are
 // expanded by the whole block.
 template <bool EMIT>
 __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
@@ -466,18 +485,16 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
                 p.pair_cnt[i] = nt;
                 pairs8_acc += p8;
             }
-            if (nt <= (uint32_t)kBigRect) {
-                const uint32_t off = EMIT ? p.pair_off[i] : 0u;
+            if (!EMIT) {
+                // counting pass: nothing more per triangle
+            } else if (nt <= (uint32_t)kBigRect) {
+                const uint32_t off = p.pair_off[i];
                 for (uint32_t k = 0; k < nt; ++k) {
                     const uint32_t tile = v.tile_base +
                                           (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * (uint32_t)v.tiles_x +
                                           (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
-                    if (EMIT) {
-                        p.keys0[off + k] = tile;
-                        p.vals0[off + k] = i;
-                    } else {
-                        atomicAdd(&p.tile_cnt[tile], 1u);
-                    }
+                    p.keys0[off + k] = tile;
+                    p.vals0[off + k] = i;
                 }
             } else {
                 s_big[atomicAdd(&s_nbig, 1u)] = i;
@@ -492,28 +509,36 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
             int32_t tx0, ty0, tw;
             uint32_t p8;
             const uint32_t nt = tile_rect(tv.rect0, tv.rect1, tx0, ty0, tw, p8);
-            const uint32_t off = EMIT ? p.pair_off[ti] : 0u;
+            const uint32_t off = p.pair_off[ti];
             for (uint32_t k = threadIdx.x; k < nt; k += 256u) {
                 const uint32_t tile = v.tile_base + (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * (uint32_t)v.tiles_x +
                                       (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
-                if (EMIT) {
-                    p.keys0[off + k] = tile;
-                    p.vals0[off + k] = ti;
-                } else {
-                    atomicAdd(&p.tile_cnt[tile], 1u);
-                }
+                p.keys0[off + k] = tile;
+                p.vals0[off + k] = ti;
             }
         }
         __syncthreads();
     }
     if (!EMIT) {
+        // one partial per block (blocksum is free once setup has emitted): a single shared
+        // counter hit by every wave serialises in L2
+        __shared__ unsigned long long s_p8[4];
         unsigned long long s = wave_sum(pairs8_acc);
-        if ((threadIdx.x & 63u) == 0u && s) atomicAdd(&p.stats->pairs8, s);
+        if ((threadIdx.x & 63u) == 0u) s_p8[threadIdx.x >> 6] = s;
+        __syncthreads();
+        if (threadIdx.x == 0) p.blocksum[blockIdx.x] = s_p8[0] + s_p8[1] + s_p8[2] + s_p8[3];
     }
 }
 
-__global__ void kb_pair_check(BatchParams p) {
-    if (threadIdx.x == 0 && blockIdx.x == 0) {
+__global__ __launch_bounds__(256) void kb_pair_check(BatchParams p, uint32_t n_partials) {
+    __shared__ unsigned long long s_w[4];
+    unsigned long long acc = 0ull;
+    for (uint32_t i = threadIdx.x; i < n_partials; i += 256u) acc += p.blocksum[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        p.stats->pairs8 += s_w[0] + s_w[1] + s_w[2] + s_w[3];
         const uint32_t total = p.ctr->n_pairs;
         atomicMax(&p.stats->max_pairs, (unsigned long long)total);
         if (total > p.pair_cap) {
@@ -598,6 +623,18 @@ __global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uin
     }
 }
 
+// Tile ranges from the sorted keys: the first pair of a tile writes its start, the last its
+// end (tile_cnt holds the end). Tiles without pairs keep start = end = 0.
+__global__ __launch_bounds__(256) void kb_tile_ranges(BatchParams p, const uint32_t *keys) {
+    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
+    const uint32_t n = p.ctr->n_pairs;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        const uint32_t k = keys[i];
+        if (i == 0u || keys[i - 1u] != k) p.tile_start[k] = i;
+        if (i + 1u == n || keys[i + 1u] != k) p.tile_cnt[k] = i + 1u;
+    }
+}
+
 // ------------------------------------------------------------ stage 3: tile
 
 // Reduce over the 64 lanes of a wave: minimum (GE) or maximum (LE) of the in-range depths,
@@ -621,7 +658,7 @@ __device__ __forceinline__ float tile_hiz(bool inside, float d, bool ge) {
 __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
     __shared__ TriVis s_tri[256];
     __shared__ uint32_t s_mask[256];
-    __shared__ uint16_t s_list[4][256];
+    __shared__ __attribute__((aligned(4))) uint16_t s_list[4][256];
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -642,7 +679,7 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
     uint32_t lstart = 0u, lcount = n_asm;
     if (!ovf) {
         lstart = p.tile_start[g];
-        lcount = p.tile_cnt[g];
+        lcount = p.tile_cnt[g] - lstart;
     }
     if (lcount == 0u) return;
 
@@ -683,9 +720,8 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
             cnt += (uint32_t)__popcll(bal);
         }
         if (sub_valid && cnt) {
-            for (uint32_t k = 0; k < cnt; ++k) {
-                const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_list[w][k]);
-                const TriVis T = s_tri[i];
+            // one (triangle, 8x8 bin) pair: hierarchical-Z, coverage, early-Z
+            auto fragment = [&](const TriVis &T) {
                 const int32_t tdraw = __builtin_amdgcn_readfirstlane((int)T.draw);
                 if (tdraw != cur_draw) {
                     // the bin's tile minimum after the previous draw (write_tile, main.c:589-603)
@@ -698,12 +734,13 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
                 bool rej = ge ? (T.hiz_depth < hz) : (T.hiz_depth > hz);
                 // conservative cull: no fragment of T can pass the bin's current depths
                 if (depth_cull) rej = rej || (ge ? (T.cull_bound < wext) : (T.cull_bound > wext));
-                if (__builtin_amdgcn_readfirstlane((int)rej)) continue;
+                if (__builtin_amdgcn_readfirstlane((int)rej)) return;
                 const uint32_t e0 = (uint32_t)T.c[0] + (uint32_t)__mul24(T.a16[0], px) + (uint32_t)__mul24(T.b16[0], py);
                 const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
                 const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
                 const bool cov = inside && (int32_t)(e0 | e1 | e2) > 0;  // main.c:1028
-                const float b1 = (float)((int32_t)e1 >> 8) * T.ooa;     // main.c:1089-1102
+                if (__ballot(cov) == 0ull) return;
+                const float b1 = (float)((int32_t)e1 >> 8) * T.ooa;  // main.c:1089-1102
                 const float b2 = (float)((int32_t)e2 >> 8) * T.ooa;
                 float z = T.z0 + T.dz1 * b1;  // register 0 is affine (main.c:1118-1154)
                 z = z + T.dz2 * b2;
@@ -715,6 +752,20 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
                     best = id;
                 }
                 if (depth_cull && __ballot(upd)) wext = wave_depth_extreme(inside, d, ge);
+            };
+            // two records per iteration: the second LDS read overlaps the first evaluation
+            uint32_t k = 0;
+            for (; k + 1u < cnt; k += 2u) {
+                const uint32_t pair = (uint32_t)__builtin_amdgcn_readfirstlane(
+                    (int)*reinterpret_cast<const uint32_t *>(&s_list[w][k]));
+                const TriVis T0 = s_tri[pair & 0xFFFFu];
+                const TriVis T1 = s_tri[pair >> 16];
+                fragment(T0);
+                fragment(T1);
+            }
+            if (k < cnt) {
+                const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_list[w][k]);
+                fragment(s_tri[i]);
             }
         }
         __syncthreads();
@@ -755,13 +806,32 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
         }
     }
 
+    __shared__ uint32_t s_active[4];
+    if (lane == 0u) s_active[w] = 0u;
     if (sub_valid && cur_draw >= 0) {
         const float m = tile_hiz(inside, d, ge);
         if (lane == 0u) {
             V.hiz[sby * V.bins_x + sbx] = m;
-            atomicAdd(&p.stats->active8, (unsigned long long)n_draws_here);
+            s_active[w] = n_draws_here;
         }
     }
+    __syncthreads();
+    // (bin, draw) activity of this tile for the stats; this tile's start is no longer needed
+    if (tid == 0u) p.tile_start[g] = s_active[0] + s_active[1] + s_active[2] + s_active[3];
+}
+
+// Sum of the per-tile activity counts into the stats (one atomic per block).
+__global__ __launch_bounds__(256) void kb_active_sum(BatchParams p) {
+    __shared__ unsigned long long s_w[4];
+    unsigned long long acc = 0ull;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < p.total_tiles; i += gridDim.x * 256u) {
+        // tiles without pairs kept start = 0 and returned early, which counts nothing
+        acc += p.tile_start[i];
+    }
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(&p.stats->active8, s_w[0] + s_w[1] + s_w[2] + s_w[3]);
 }
 
 // ------------------------------------------------------------ launchers
@@ -791,13 +861,13 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
     case 1:
         if (!p.n_tblocks) break;
         hipLaunchKernelGGL(kb_setup_count, dim3(p.n_tblocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(kb_clip_count, dim3(256), dim3(64), 0, s, p);
+        hipLaunchKernelGGL(kb_clip_count, dim3(1024), dim3(64), 0, s, p);
         if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc.part,
                                                  &p.ctr->n_assembled_clip, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(kb_setup_totals, dim3((p.n_draws + 255) / 256), dim3(256), 0, s, p);
         hipLaunchKernelGGL(kb_setup_emit, dim3(p.n_tblocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(kb_clip_emit, dim3(256), dim3(64), 0, s, p);
+        hipLaunchKernelGGL(kb_clip_emit, dim3(1024), dim3(64), 0, s, p);
         break;
     case 2: {
         if (!p.n_tblocks) break;
@@ -806,10 +876,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         if ((e = launch_scan<uint32_t>(p.pair_cnt, p.pair_off, p.tri_cap, &p.ctr->n_assembled, sc.part,
                                        &p.ctr->n_pairs, s)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(kb_pair_check, dim3(1), dim3(64), 0, s, p);
-        if ((e = launch_scan<uint32_t>(p.tile_cnt, p.tile_start, p.total_tiles, nullptr, sc.part, nullptr, s)) !=
-            hipSuccess)
-            return e;
+        hipLaunchKernelGGL(kb_pair_check, dim3(1), dim3(256), 0, s, p, grid);
         hipLaunchKernelGGL(kb_pairs<true>, dim3(grid), dim3(256), 0, s, p);
         const uint32_t nb = (p.pair_cap + kRadixItems - 1) / kRadixItems;
         uint32_t *kin = p.keys0, *vin = p.vals0, *kout = p.keys1, *vout = p.vals1;
@@ -824,12 +891,16 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
             std::swap(kin, kout);
             std::swap(vin, vout);
         }
+        hipLaunchKernelGGL(kb_tile_ranges, dim3(std::min<uint32_t>(nb * 8u, 65535u)), dim3(256), 0, s, p,
+                           (const uint32_t *)kin);
         break;
     }
     case 3: {
         if (!p.n_tblocks || !p.total_tiles) break;
         const uint32_t *sorted = (radix_passes & 1u) ? p.vals1 : p.vals0;
         hipLaunchKernelGGL(kb_tile, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
+        hipLaunchKernelGGL(kb_active_sum, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 64u)), dim3(256), 0,
+                           s, p);
         break;
     }
     default:
