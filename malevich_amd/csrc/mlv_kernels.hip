@@ -74,15 +74,6 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T *s_wave, T &total) {
     return r;
 }
 
-// Blocks are dispatched round-robin over the 8 XCDs; give each XCD a contiguous range of
-// tiles so neighbouring tiles (which share triangles and texels) meet in the same L2.
-__device__ __forceinline__ uint32_t xcd_remap(uint32_t b, uint32_t n) {
-    const uint32_t X = 8u;
-    if (n < X) return b;
-    uint32_t per = n / X, rem = n % X, x = b % X, loc = b / X;
-    return x * per + (x < rem ? x : rem) + loc;
-}
-
 // Largest index lo in [0, n) with key(lo) <= x, for ascending keys with key(0) <= x.
 template <class F>
 __device__ __forceinline__ uint32_t upper_index(uint32_t n, uint32_t x, F key) {
@@ -637,16 +628,33 @@ __global__ __launch_bounds__(256) void kb_tile_ranges(BatchParams p, const uint3
 
 // ------------------------------------------------------------ stage 3: tile
 
+// DPP lane moves within a row of 16 lanes (GFX9 encodings): quad_perm [1,0,3,2] and
+// [2,3,0,1] swap neighbours at distance 1 and 2; row_ror 4 and 8 rotate the row.
+template <int CTRL>
+__device__ __forceinline__ float dpp_move(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, 0xF, 0xF, false));
+}
+
 // Reduce over the 64 lanes of a wave: minimum (GE) or maximum (LE) of the in-range depths,
-// ignoring NaN. Equal on every lane.
+// ignoring NaN. Register-only (DPP within rows, then the four row results); equal on every lane.
 __device__ __forceinline__ float wave_depth_extreme(bool inside, float d, bool ge) {
     float v = inside ? d : (ge ? __builtin_inff() : -__builtin_inff());
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) {
-        float u = __shfl_xor(v, o, 64);
-        v = ge ? fminf(v, u) : fmaxf(v, u);
+    if (ge) {
+        v = fminf(v, dpp_move<0xB1>(v));
+        v = fminf(v, dpp_move<0x4E>(v));
+        v = fminf(v, dpp_move<0x124>(v));
+        v = fminf(v, dpp_move<0x128>(v));
+    } else {
+        v = fmaxf(v, dpp_move<0xB1>(v));
+        v = fmaxf(v, dpp_move<0x4E>(v));
+        v = fmaxf(v, dpp_move<0x124>(v));
+        v = fmaxf(v, dpp_move<0x128>(v));
     }
-    return v;
+    const float r0 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 0));
+    const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
+    const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
+    const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
+    return ge ? fminf(fminf(r0, r1), fminf(r2, r3)) : fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
 }
 
 // write_tile's tile minimum (main.c:589-603), folded from 1.0 (0.0 for the LESS_EQUAL mirror)
@@ -662,7 +670,9 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    const uint32_t g = xcd_remap(blockIdx.x, p.total_tiles);
+    // The dispatcher deals consecutive blocks to the 8 XCDs in turn, so neighbouring tiles
+    // (whose triangle counts are alike) spread over all XCDs and their load evens out.
+    const uint32_t g = blockIdx.x;
     const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
     const ViewRec &V = p.views[vi];
     const uint32_t tile = g - V.tile_base;
