@@ -118,6 +118,95 @@ def cpu_baseline(sc, env, budget_s: float):
                       f"({tris} triangles, {dt:.1f} s, oracle built with gcc -O2, 1 thread)"}
 
 
+def warm(dev, step, n):
+    """Untimed warmup. A flush whose clipper or triangle arena was too small reports
+    MLV_ERROR_CAPACITY after raising the capacity; the frame is then drawn again."""
+    from malevich_amd import _lib as L
+    from malevich_amd._lib import MalevichError
+    for attempt in range(4):
+        try:
+            for _ in range(max(1, n)):
+                step()
+            dev.synchronize()
+            return
+        except MalevichError as e:
+            if e.code != L.ERROR_CAPACITY or attempt == 3:
+                raise
+            print(f"[bench] warmup: {e}; drawing again", file=sys.stderr, flush=True)
+
+
+def bench_c3(dev, args):
+    """Binning stress at 1920x1080: 10M tiny triangles (1-4 px^2) plus 1k screen-covering ones
+    interleaved every 10,000 in submission order, passthrough_vs/ps, one draw per frame."""
+    from malevich_amd import _lib as L
+    from malevich_amd import scenes as S
+    W, H = args.width, args.height
+    t0 = time.perf_counter()
+    st = S.binning_stress(args.c3_tiny, args.c3_big, W, H, seed=args.seed + 1)
+    gen_s = time.perf_counter() - t0
+    vb = dev.create_buffer(L.BUFFER_VERTEX, st.vertices)
+    ib = dev.create_buffer(L.BUFFER_INDEX_U32, st.indices)
+    rt = dev.create_render_target(W, H)
+    ds = dev.create_depth_stencil(W, H)
+    dev.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
+    dev.vs_set_shader(L.VS_PASSTHROUGH)
+    dev.ps_set_shader(L.PS_PASSTHROUGH)
+    dev.rs_set_viewport(0, 0, W, H, 0.0, 1.0)
+    dev.om_set_render_targets(rt, ds)
+    dev.om_set_depth_func(L.DEPTH_GREATER_EQUAL)
+    dev.ia_set_vertex_buffer(vb, 32)
+    dev.ia_set_index_buffer(ib)
+    n_idx = int(st.indices.size)
+
+    def one():
+        dev.clear_render_target_view(rt, (0.0, 0.0, 0.0, 0.0))
+        dev.clear_depth_stencil_view(ds, 0.0)
+        dev.draw_indexed(n_idx)
+
+    warm(dev, one, args.warmup)
+    dev.reset_stats()
+    ev = HipEvents()
+    a, b = ev.create(), ev.create()
+    ev.record(a, dev.stream)
+    for _ in range(args.c3_steps):
+        one()
+    ev.record(b, dev.stream)
+    ms = ev.elapsed_ms(a, b) / args.c3_steps
+    dev.synchronize()
+    stats = dev.stats()
+    tris = n_idx // 3
+    for h in (vb, ib, rt, ds):
+        h.release()
+    return {"workload": f"c3-1080p-binning-stress: {args.c3_tiny} tiny (1-4 px^2) + {args.c3_big} screen-covering "
+                        f"triangles, passthrough_vs/ps, one draw, reversed-Z GE, {W}x{H}",
+            "ms_per_frame": round(ms, 4), "mtri_per_s": round(tris / (ms * 1e-3) / 1e6, 2),
+            "gpix_per_s": round(W * H / (ms * 1e-3) / 1e9, 4), "steps": args.c3_steps,
+            "assembled_per_frame": int(stats["assembled_triangle_count"] / args.c3_steps),
+            "pairs_per_frame": int(stats["total_triangle_count_in_bins"] / args.c3_steps),
+            "scene_gen_s": round(gen_s, 1)}
+
+
+def copy_calibration(mib: int):
+    """One device-to-device hipMemcpy of `mib` MiB (the runtime runs it as a copy kernel that
+    reads and writes exactly that many bytes), so that a counter run can relate FETCH_SIZE /
+    WRITE_SIZE to bytes moved."""
+    hip = C.CDLL("libamdhip64.so")
+    hip.hipMalloc.argtypes = [C.POINTER(C.c_void_p), C.c_size_t]
+    hip.hipMemset.argtypes = [C.c_void_p, C.c_int, C.c_size_t]
+    hip.hipMemcpy.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t, C.c_int]
+    hip.hipFree.argtypes = [C.c_void_p]
+    n = mib << 20
+    a, b = C.c_void_p(), C.c_void_p()
+    assert hip.hipMalloc(C.byref(a), n) == 0 and hip.hipMalloc(C.byref(b), n) == 0
+    assert hip.hipMemset(a, 1, n) == 0
+    assert hip.hipDeviceSynchronize() == 0
+    assert hip.hipMemcpy(b, a, n, 3) == 0  # hipMemcpyDeviceToDevice
+    assert hip.hipDeviceSynchronize() == 0
+    hip.hipFree(a)
+    hip.hipFree(b)
+    print(f"[bench] copy calibration: {mib} MiB read + {mib} MiB written", file=sys.stderr, flush=True)
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +218,13 @@ def main(argv=None):
     ap.add_argument("--seed", type=int, default=2)
     ap.add_argument("--cpu-budget", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--c3-tiny", type=int, default=10_000_000)
+    ap.add_argument("--c3-big", type=int, default=1000)
+    ap.add_argument("--c3-steps", type=int, default=5)
+    ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--copy-calibration-mib", type=int, default=0,
+                    help="also run one plain device-to-device copy of this many MiB (a known-size kernel "
+                         "against which HBM byte counters are scaled)")
     args = ap.parse_args(argv)
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
@@ -143,12 +239,12 @@ def main(argv=None):
           f"depth complexity {sc.depth_complexity:.2f} ({time.perf_counter() - t_build:.1f} s)", file=sys.stderr,
           flush=True)
     dev = P.Device(0)
+    if args.copy_calibration_mib > 0:
+        copy_calibration(args.copy_calibration_mib)
     res = upload(dev, sc, env)
     bind_state(dev, res, sc)
     ev = HipEvents()
-    for _ in range(args.warmup):
-        frame(dev, res, sc)
-    dev.synchronize()
+    warm(dev, lambda: frame(dev, res, sc), args.warmup)
     dev.reset_stats()
     dev.profile()  # drop anything accumulated
     dev.set_profiling(True)
@@ -231,6 +327,9 @@ def main(argv=None):
         "per_draw_views_per_s": None,
         "roofline": roofline,
     }
+    if not args.no_c3:
+        out["c3_binning_stress"] = bench_c3(dev, args)
+        print(f"[bench] c3: {out['c3_binning_stress']}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, env, args.cpu_budget)
         out["cpu_baseline"]["gpu_over_cpu"] = round(mtri_s / out["cpu_baseline"]["value"], 1)

@@ -264,7 +264,9 @@ static int flush(mlv_context *ctx) {
     const uint32_t n_draws = (uint32_t)draws.size();
     const uint32_t n_views = (uint32_t)views.size();
     const uint64_t T = tslots, Vs = vslots;
-    const size_t clip_tris = std::max<size_t>(ctx->clip_hint, 4096);
+    // Clipped triangles are unknown until setup runs: start from 1/32 of the batch (triangles
+    // straddling the screen edges of a dense scene) and grow from the measured maximum.
+    const size_t clip_tris = std::max<size_t>({ctx->clip_hint, (size_t)4096, (size_t)(T / 32)});
     const uint64_t tri_cap64 = T + 6ull * clip_tris;
     const uint64_t rows64 = Vs + 9ull * clip_tris;
     if (tri_cap64 > 0xF0000000ull || rows64 > 0xF0000000ull || total_tiles > 0xF0000000ull)

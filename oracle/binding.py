@@ -196,6 +196,26 @@ def primitive_assembly(draw: Draw, capacity: int | None = None) -> list:
     return list(buf[: n.value])
 
 
+TRIANGLE_DTYPE = np.dtype([("edge", np.int32, (3, 3)), ("rw", np.float32, 3), ("one_over_area", np.float32),
+                           ("max_depth", np.float32), ("min_bounds", np.int32, 2), ("max_bounds", np.int32, 2),
+                           ("prim", np.uint32), ("attr", np.float32, (3, 12))])
+assert TRIANGLE_DTYPE.itemsize == C.sizeof(Triangle)
+
+
+def primitive_assembly_array(draw: Draw, capacity: int | None = None) -> np.ndarray:
+    """As primitive_assembly, as a structured numpy array (for millions of triangles)."""
+    T = draw.desc.index_count // 3
+    cap = capacity or (T + max(2 * T, 512))
+    out = np.zeros(cap, TRIANGLE_DTYPE)
+    n = C.c_uint32(0)
+    oob = C.c_uint64(0)
+    rc = lib().orc_primitive_assembly(C.byref(draw.desc), out.ctypes.data_as(C.POINTER(Triangle)), cap,
+                                      C.byref(n), C.byref(oob))
+    if rc != 0:
+        raise RuntimeError(f"orc_primitive_assembly failed: {rc}")
+    return out[: n.value].copy()
+
+
 def coverage_mask(edges, ox: int, oy: int) -> int:
     e = (C.c_int32 * 9)(*np.asarray(edges, np.int64).astype(np.int32).ravel().tolist())
     return int(lib().orc_coverage_mask(e, ox, oy))
