@@ -222,6 +222,7 @@ def main(argv=None):
     ap.add_argument("--c3-big", type=int, default=1000)
     ap.add_argument("--c3-steps", type=int, default=5)
     ap.add_argument("--no-c3", action="store_true")
+    ap.add_argument("--c3-only", action="store_true", help="run only the binning-stress leg (for profiling)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (a known-size kernel "
                          "against which HBM byte counters are scaled)")
@@ -232,6 +233,12 @@ def main(argv=None):
         sys.exit(2)
 
     from malevich_amd import pipeline as P
+
+    if args.c3_only:
+        dev = P.Device(0)
+        print(json.dumps({"c3_binning_stress": bench_c3(dev, args)}))
+        dev.close()
+        return
 
     t_build = time.perf_counter()
     sc, env = build_c2(args)

@@ -11,6 +11,7 @@
 //   cnt[Tt]          u32       per input-triangle slot: (#assembled) | (#clip vertices << 16)
 //   tvis[N]          TriVis    80 B per assembled triangle (edge functions, depth plane, bin rect)
 //   tshade[N]        TriShade  32 B per assembled triangle (1/w, attribute rows)
+//   tbin[N]          uint4     16 B per assembled triangle (bin rect and its view's tile grid)
 //   pair keys/vals   u32       (global tile, triangle) pairs, stably sorted by tile
 //   tile_start/end   u32       per global tile (tiles of all views, view-major): its pair range
 // Slots of draw d: vertices [vbase, vbase+V_d), input triangles [tbase, tbase+T_d); both
@@ -103,6 +104,8 @@ struct BatchCounters {
     uint32_t n_clip_list;
     uint32_t n_pairs;
     uint32_t overflow;
+    uint32_t radix_blocks;                 // blocks of the radix passes over n_pairs
+    uint32_t hist_n;                       // radix_blocks * 256 digit counters
     uint32_t pad;
 };
 
@@ -128,6 +131,7 @@ struct BatchParams {
     uint2 *clip_off;
     TriVis *tvis;
     TriShade *tshade;
+    uint4 *tbin;                          // per assembled triangle: rect0, rect1, tile_base, tiles_x of its view
     uint32_t tri_cap;
     uint32_t *pair_cnt, *pair_off;        // per assembled triangle
     uint32_t *keys0, *vals0, *keys1, *vals1;

@@ -369,6 +369,8 @@ __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t id
     sh.prim = prim;
     sh.sub = sub;
     p.tshade[idx] = sh;
+    const ViewRec &v = p.views[p.draws[draw].view];
+    p.tbin[idx] = make_uint4(tv.rect0, tv.rect1, v.tile_base, (uint32_t)v.tiles_x);
 }
 
 __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
@@ -452,6 +454,9 @@ __device__ __forceinline__ uint32_t tile_rect(uint32_t rect0, uint32_t rect1, in
     return (uint32_t)tw * (uint32_t)((by1 >> 1) - ty0 + 1);
 }
 
+constexpr uint32_t kRadixRounds = 8;
+constexpr uint32_t kRadixItems = 256 * kRadixRounds;
+
 // EMIT = false: per-triangle pair counts. EMIT = true: the pairs in triangle order at the
 // scanned offsets. Triangles spanning more than kBigRect tiles are
 // expanded by the whole block.
@@ -467,11 +472,10 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
         __syncthreads();
         const uint32_t i = base + threadIdx.x;
         if (i < n) {
-            const TriVis &tv = p.tvis[i];
-            const ViewRec &v = p.views[p.draws[tv.draw].view];
+            const uint4 tb = p.tbin[i];
             int32_t tx0, ty0, tw;
             uint32_t p8;
-            const uint32_t nt = tile_rect(tv.rect0, tv.rect1, tx0, ty0, tw, p8);
+            const uint32_t nt = tile_rect(tb.x, tb.y, tx0, ty0, tw, p8);
             if (!EMIT) {
                 p.pair_cnt[i] = nt;
                 pairs8_acc += p8;
@@ -481,8 +485,7 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
             } else if (nt <= (uint32_t)kBigRect) {
                 const uint32_t off = p.pair_off[i];
                 for (uint32_t k = 0; k < nt; ++k) {
-                    const uint32_t tile = v.tile_base +
-                                          (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * (uint32_t)v.tiles_x +
+                    const uint32_t tile = tb.z + (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * tb.w +
                                           (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
                     p.keys0[off + k] = tile;
                     p.vals0[off + k] = i;
@@ -495,14 +498,13 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
         const uint32_t nbig = s_nbig;
         for (uint32_t bi = 0; bi < nbig; ++bi) {
             const uint32_t ti = s_big[bi];
-            const TriVis &tv = p.tvis[ti];
-            const ViewRec &v = p.views[p.draws[tv.draw].view];
+            const uint4 tb = p.tbin[ti];
             int32_t tx0, ty0, tw;
             uint32_t p8;
-            const uint32_t nt = tile_rect(tv.rect0, tv.rect1, tx0, ty0, tw, p8);
+            const uint32_t nt = tile_rect(tb.x, tb.y, tx0, ty0, tw, p8);
             const uint32_t off = p.pair_off[ti];
             for (uint32_t k = threadIdx.x; k < nt; k += 256u) {
-                const uint32_t tile = v.tile_base + (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * (uint32_t)v.tiles_x +
+                const uint32_t tile = tb.z + (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * tb.w +
                                       (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
                 p.keys0[off + k] = tile;
                 p.vals0[off + k] = ti;
@@ -532,10 +534,14 @@ __global__ __launch_bounds__(256) void kb_pair_check(BatchParams p, uint32_t n_p
         p.stats->pairs8 += s_w[0] + s_w[1] + s_w[2] + s_w[3];
         const uint32_t total = p.ctr->n_pairs;
         atomicMax(&p.stats->max_pairs, (unsigned long long)total);
+        uint32_t nb = (total + kRadixItems - 1u) / kRadixItems;
         if (total > p.pair_cap) {
             atomicOr(&p.ctr->overflow, OVF_PAIRS);
             atomicAdd(&p.stats->overflow_draws, 1ull);
+            nb = 0u;
         }
+        p.ctr->radix_blocks = nb;
+        p.ctr->hist_n = nb * 256u;
     }
 }
 
@@ -543,18 +549,19 @@ __global__ __launch_bounds__(256) void kb_pair_check(BatchParams p, uint32_t n_p
 // kRadixItems consecutive pairs; within a round of 256, a pair's rank among equal digits
 // comes from eight ballots (the lanes sharing its digit) and mbcnt; per-wave digit counts
 // are combined across waves and rounds in LDS, so equal keys keep their input order.
-constexpr uint32_t kRadixRounds = 8;
-constexpr uint32_t kRadixItems = 256 * kRadixRounds;
+// The grids are sized for the arena; blocks past the device's pair count return at once.
 
 size_t radix_hist_elems(size_t n_max) { return 256 * ((n_max + kRadixItems - 1) / kRadixItems + 1); }
 
 __global__ __launch_bounds__(256) void kb_radix_hist(BatchParams p, const uint32_t *keys, uint32_t shift,
-                                                     uint32_t *hist, uint32_t nb) {
+                                                     uint32_t *hist) {
     __shared__ uint32_t h[256];
+    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
+    const uint32_t nb = p.ctr->radix_blocks;
+    if (blockIdx.x >= nb) return;
     h[threadIdx.x] = 0u;
     __syncthreads();
-    const bool skip = (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) != 0u;
-    const uint32_t n = skip ? 0u : p.ctr->n_pairs;
+    const uint32_t n = p.ctr->n_pairs;
     const uint32_t base = blockIdx.x * kRadixItems;
     for (uint32_t r = 0; r < kRadixRounds; ++r) {
         const uint32_t i = base + r * 256u + threadIdx.x;
@@ -566,11 +573,12 @@ __global__ __launch_bounds__(256) void kb_radix_hist(BatchParams p, const uint32
 
 __global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uint32_t *kin, const uint32_t *vin,
                                                         uint32_t *kout, uint32_t *vout, uint32_t shift,
-                                                        const uint32_t *hist, uint32_t nb) {
+                                                        const uint32_t *hist) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wc[4][256];
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
     const uint32_t n = p.ctr->n_pairs;
+    const uint32_t nb = p.ctr->radix_blocks;
     const uint32_t base = blockIdx.x * kRadixItems;
     if (base >= n) return;
     const uint32_t tid = threadIdx.x, w = tid >> 6;
@@ -892,16 +900,16 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         uint32_t *kin = p.keys0, *vin = p.vals0, *kout = p.keys1, *vout = p.vals1;
         for (uint32_t pass = 0; pass < radix_passes; ++pass) {
             const uint32_t shift = 8u * pass;
-            hipLaunchKernelGGL(kb_radix_hist, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin, shift, sc.hist, nb);
-            if ((e = launch_scan<uint32_t>(sc.hist, sc.hist, (size_t)nb * 256u, nullptr, sc.part, nullptr, s)) !=
-                hipSuccess)
+            hipLaunchKernelGGL(kb_radix_hist, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin, shift, sc.hist);
+            if ((e = launch_scan<uint32_t>(sc.hist, sc.hist, (size_t)nb * 256u, &p.ctr->hist_n, sc.part, nullptr,
+                                           s)) != hipSuccess)
                 return e;
             hipLaunchKernelGGL(kb_radix_scatter, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin,
-                               (const uint32_t *)vin, kout, vout, shift, (const uint32_t *)sc.hist, nb);
+                               (const uint32_t *)vin, kout, vout, shift, (const uint32_t *)sc.hist);
             std::swap(kin, kout);
             std::swap(vin, vout);
         }
-        hipLaunchKernelGGL(kb_tile_ranges, dim3(std::min<uint32_t>(nb * 8u, 65535u)), dim3(256), 0, s, p,
+        hipLaunchKernelGGL(kb_tile_ranges, dim3(std::min<uint32_t>(nb * 8u, 2048u)), dim3(256), 0, s, p,
                            (const uint32_t *)kin);
         break;
     }

@@ -108,7 +108,7 @@ struct mlv_context {
     std::vector<PendingDraw> pending;
     uint64_t pending_tris = 0, pending_verts = 0;
     // arenas
-    Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, tvis, tshade;
+    Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, tvis, tshade, tbin;
     Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist;
     BatchCounters *ctr = nullptr;
     DevStats *dstats = nullptr;
@@ -273,7 +273,10 @@ static int flush(mlv_context *ctx) {
         return fail(ctx, MLV_ERROR_CAPACITY, "flush too large for 32-bit indexing");
     const uint32_t tri_cap = (uint32_t)tri_cap64;
     const uint32_t vattr_rows = (uint32_t)rows64;
-    size_t pair_cap = std::max<size_t>(ctx->pair_hint, 3ull * T + 4ull * total_tiles + 1024);
+    // (triangle, tile) pairs cost 16 B each; a generous first guess avoids the exact but slow
+    // overflow path (every tile walking every triangle), and later flushes size from the
+    // measured maximum.
+    size_t pair_cap = std::max<size_t>(ctx->pair_hint, 8ull * T + 16ull * total_tiles + 1024);
     pair_cap = std::min<size_t>(pair_cap, 0xF0000000ull);
     if (ctx->pair_limit) pair_cap = std::min(pair_cap, ctx->pair_limit);
     const uint32_t n_tblocks = (uint32_t)(T / 256), n_vblocks = (uint32_t)(Vs / 256);
@@ -292,6 +295,7 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->clip_off, std::max<size_t>(T, 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->tvis, (size_t)tri_cap * sizeof(TriVis)));
     RC_OR_RETURN(ensure(ctx, ctx->tshade, (size_t)tri_cap * sizeof(TriShade)));
+    RC_OR_RETURN(ensure(ctx, ctx->tbin, (size_t)tri_cap * 16));
     RC_OR_RETURN(ensure(ctx, ctx->pair_cnt, (size_t)tri_cap * 4));
     RC_OR_RETURN(ensure(ctx, ctx->pair_off, (size_t)tri_cap * 4));
     RC_OR_RETURN(ensure(ctx, ctx->keys0, pair_cap * 4));
@@ -357,6 +361,7 @@ static int flush(mlv_context *ctx) {
     p.clip_off = (uint2 *)ctx->clip_off.p;
     p.tvis = (TriVis *)ctx->tvis.p;
     p.tshade = (TriShade *)ctx->tshade.p;
+    p.tbin = (uint4 *)ctx->tbin.p;
     p.tri_cap = tri_cap;
     p.pair_cnt = (uint32_t *)ctx->pair_cnt.p;
     p.pair_off = (uint32_t *)ctx->pair_off.p;
@@ -467,7 +472,7 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
     (void)hipStreamSynchronize(ctx->stream);
     Arena *all[] = {&ctx->draws_dev, &ctx->views_dev, &ctx->vpos,     &ctx->vattr,     &ctx->cnt,
                     &ctx->blocksum,  &ctx->blockoff,  &ctx->asm_base, &ctx->clip_list, &ctx->clip_off,
-                    &ctx->tvis,      &ctx->tshade,    &ctx->pair_cnt, &ctx->pair_off,  &ctx->keys0,
+                    &ctx->tvis,      &ctx->tshade,    &ctx->tbin,    &ctx->pair_cnt, &ctx->pair_off,  &ctx->keys0,
                     &ctx->vals0,     &ctx->keys1,     &ctx->vals1,    &ctx->tile_cnt,  &ctx->tile_start,
                     &ctx->scan_part, &ctx->radix_hist};
     for (Arena *a : all) release(*a);
