@@ -118,6 +118,23 @@ def cpu_baseline(sc, env, budget_s: float):
                       f"({tris} triangles, {dt:.1f} s, oracle built with gcc -O2, 1 thread)"}
 
 
+STAGE_KERNEL = {"vertex": "kb_vertex", "setup": "kb_setup_emit", "binning": "kb_radix_scatter", "tile": "kb_tile"}
+
+
+def measured_traffic(kernel: str, sc):
+    """HBM bytes per launch of `kernel` on this workload from the committed counter summary
+    (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
+    path = ROOT / "profiles" / "traffic.json"
+    try:
+        rec = json.loads(path.read_text())["c2-1080p-forward"]
+    except (OSError, KeyError, ValueError):
+        return None
+    if (rec.get("triangles"), rec.get("width"), rec.get("height")) != (sc.triangle_count, sc.width, sc.height):
+        return None
+    k = rec["kernels"].get(kernel)
+    return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
+
+
 def warm(dev, step, n):
     """Untimed warmup. A flush whose clipper or triangle arena was too small reports
     MLV_ERROR_CAPACITY after raising the capacity; the frame is then drawn again."""
@@ -305,8 +322,9 @@ def main(argv=None):
     }
     algo = {k: v / flushes_per_frame for k, v in algo_frame.items()}
     achieved = algo[dom] / (per_launch_ms * 1e-3) / 1e9
+    traffic = measured_traffic(STAGE_KERNEL[dom], sc)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": dom,
+                "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": STAGE_KERNEL[dom],
                 "avg_launch_us": round(per_launch_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(algo[dom])}
     out = {
         "metric": "fwd Mtri/s + Gpix/s @1080p; fwd+bwd ms/frame; batched vs per-draw views/s",

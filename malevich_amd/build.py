@@ -70,6 +70,16 @@ def build_library(verbose: bool = False) -> Path:
     return LIB
 
 
+HOST_PROGRAM = PKG / "mlv_render"
+
+
+def build_host_program() -> Path:
+    """Compile the C host program (malevich_amd/host/mlv_render.c) with gcc against the C ABI."""
+    _run(["gcc", "-std=c11", "-O2", "-Wall", "-Wextra", f"-I{ROOT / 'include'}", str(PKG / "host" / "mlv_render.c"),
+          f"-L{PKG}", "-lmalevich_hip", "-Wl,-rpath,$ORIGIN", "-o", str(HOST_PROGRAM)])
+    return HOST_PROGRAM
+
+
 def build_oracle() -> Path:
     """Compile the CPU oracle (test infrastructure) with its Makefile."""
     _run(["make", "-s", "-C", str(ROOT / "oracle")])
@@ -78,4 +88,5 @@ def build_oracle() -> Path:
 
 if __name__ == "__main__":
     print(build_library(verbose="-v" in sys.argv))
+    print(build_host_program())
     print(build_oracle())
