@@ -162,6 +162,11 @@ MLV_API int mlv_om_set_depth_func(mlv_context *ctx, int depth_func);
 /* Extension: per-pixel visibility (assembled-triangle id and draw tag of the last writer).
  * Two device arrays of width*height u32 each, caller-owned; NULL disables. */
 MLV_API int mlv_om_set_visibility_buffer(mlv_context *ctx, uint32_t *device_tri_ids, uint32_t *device_draw_tags);
+/* Extension: per-pixel primitive id (the input triangle index within its draw, counted from
+ * start_index, like SV_PrimitiveID) of the visible triangle; 0xFFFFFFFF where nothing was drawn
+ * since the last clear of the bound render target. One device array of width*height u32,
+ * caller-owned; NULL disables. The differentiable shading entry points below read it. */
+MLV_API int mlv_om_set_primitive_id_buffer(mlv_context *ctx, uint32_t *device_prim_ids);
 
 /* ---- commands */
 /* main.c:1191 clear_render_target_view(const f32 *p_clear_color) */
@@ -198,6 +203,52 @@ MLV_API int mlv_get_profile(mlv_context *ctx, mlv_profile *out);
  * restarts the draw tags written to the visibility buffer at 0. */
 MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out);
 MLV_API int mlv_reset_stats(mlv_context *ctx);
+
+/* ---- differentiable shading (extension; the reference renders forward only)
+ *
+ * Shading of a mesh with per-vertex colours, UVs and normals, an RGBA32F mipmapped texture and
+ * a 9-coefficient RGB spherical-harmonics light, on top of the visibility the raster pipeline
+ * produced (mlv_om_set_primitive_id_buffer). Per covered pixel (x, y), sampled at the integer
+ * corner the coverage test uses:
+ *   lambda   perspective-correct barycentrics of the pixel in the visible input triangle,
+ *            from the clip-space positions clip = clip_from_world * (position, 1)
+ *   albedo   bilinear fetch (basic_ps's filter, csc.h:143-161) of the texture at uv on the mip
+ *            level nearest the pixel's uv footprint
+ *   E        sum_k sh[k] * Y_k(normalize(n)), real SH basis of bands 0..2
+ *   out.rgb  srgb(albedo * colour * E) with basic_ps's curve max(1.055 c^(1/2.4) - 0.055, 0)
+ *            (math.h:419-422) of max(c, 0); out.a = 1. Uncovered pixels are (0, 0, 0, 0).
+ * The backward pass recomputes this from the primitive ids (nothing else is kept from the
+ * forward pass) and returns the gradient of sum(out * grad_out) with respect to positions,
+ * colours, UVs, normals, texels (every mip level) and SH coefficients, at fixed visibility and
+ * fixed mip choice. Position gradients flow through the barycentrics only. */
+typedef struct mlv_diff_scene {
+    const float *positions;    /* device, [vertex_count][3], world space */
+    const float *colors;       /* device, [vertex_count][3] */
+    const float *uvs;          /* device, [vertex_count][2] */
+    const float *normals;      /* device, [vertex_count][3] */
+    const uint32_t *indices;   /* device, [triangle_count][3] */
+    uint32_t vertex_count, triangle_count;
+    const float *texture;      /* device RGBA32F mip chain, level 0 first, level l is max(w>>l,1) x max(h>>l,1) */
+    uint32_t tex_width, tex_height, tex_levels;
+    const float *sh;           /* device, [9][3] */
+    float clip_from_world[16]; /* row-major, the first matrix of basic_vs's constant buffer (basic_vs.c:16-20) */
+    uint32_t width, height;    /* image size; the viewport covers it */
+} mlv_diff_scene;
+
+typedef struct mlv_diff_grads {
+    float *positions, *colors, *uvs, *normals; /* device, shapes as in mlv_diff_scene; NULL skips */
+    float *texture;                            /* device, the whole mip chain */
+    float *sh;                                 /* device, [9][3] */
+} mlv_diff_grads;
+
+/* device_rgba_out: width*height float4. */
+MLV_API int mlv_diff_forward(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
+                             float *device_rgba_out);
+/* Overwrites every non-NULL gradient array. deterministic = 0 accumulates with FP32 atomics;
+ * deterministic = 1 sorts the per-pixel contributions by texel / vertex and sums each segment
+ * in pixel order, so repeated calls give bit-identical gradients. */
+MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
+                              const float *device_grad_rgba, const mlv_diff_grads *grads, int deterministic);
 
 #ifdef __cplusplus
 }

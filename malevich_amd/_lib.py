@@ -50,6 +50,21 @@ _SZ = C.c_size_t
 _PP = C.POINTER(C.c_void_p)
 
 # name -> (restype, argtypes); every symbol declared in include/malevich.h
+class DiffScene(C.Structure):
+    """mlv_diff_scene"""
+    _fields_ = [
+        ("positions", C.c_void_p), ("colors", C.c_void_p), ("uvs", C.c_void_p), ("normals", C.c_void_p),
+        ("indices", C.c_void_p), ("vertex_count", C.c_uint32), ("triangle_count", C.c_uint32),
+        ("texture", C.c_void_p), ("tex_width", C.c_uint32), ("tex_height", C.c_uint32), ("tex_levels", C.c_uint32),
+        ("sh", C.c_void_p), ("clip_from_world", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32),
+    ]
+
+
+class DiffGrads(C.Structure):
+    """mlv_diff_grads"""
+    _fields_ = [(n, C.c_void_p) for n in ("positions", "colors", "uvs", "normals", "texture", "sh")]
+
+
 SIGNATURES = {
     "mlv_context_create": (_I, [_I, _PP]),
     "mlv_context_destroy": (_I, [_P]),
@@ -84,6 +99,7 @@ SIGNATURES = {
     "mlv_om_set_render_targets": (_I, [_P, _P, _P]),
     "mlv_om_set_depth_func": (_I, [_P, _I]),
     "mlv_om_set_visibility_buffer": (_I, [_P, _P, _P]),
+    "mlv_om_set_primitive_id_buffer": (_I, [_P, _P]),
     "mlv_clear_render_target_view": (_I, [_P, _P, C.POINTER(C.c_float)]),
     "mlv_clear_depth_stencil_view": (_I, [_P, _P, C.c_float]),
     "mlv_draw_indexed": (_I, [_P, _U32, _U32, C.c_int32]),
@@ -94,6 +110,8 @@ SIGNATURES = {
     "mlv_context_set_profiling": (_I, [_P, _I]),
     "mlv_flush": (_I, [_P]),
     "mlv_get_profile": (_I, [_P, C.POINTER(Profile)]),
+    "mlv_diff_forward": (_I, [_P, C.POINTER(DiffScene), _P, _P]),
+    "mlv_diff_backward": (_I, [_P, C.POINTER(DiffScene), _P, _P, C.POINTER(DiffGrads), _I]),
 }
 
 _lib: C.CDLL | None = None

@@ -1,0 +1,580 @@
+// mlv_diff.hip -- differentiable shading over the raster's primitive ids (gfx950).
+//
+// Forward: one lane per pixel. The visible input triangle (from the primitive-id buffer the
+// tile kernel writes) is re-projected with the first constant-buffer matrix, exactly as
+// basic_vs computes clip positions (basic_vs.c:22-35, pairwise dot math.h:142-146).
+// Perspective-correct barycentrics come from the homogeneous 2D form
+//   a_k = X_k - u W_k,  b_k = Y_k - v W_k,  l = a x b,  lambda = l / sum(l)
+// at the pixel's NDC (u, v): the point of the triangle's plane seen through that pixel. For
+// the reference's own interpolants this is the continuous limit of its quantised
+// perspective correction (main.c:1106-1154); unlike those, it is differentiable in the vertex
+// positions.
+//
+// Backward: the same lane recomputes the pixel (only the primitive ids survive the forward
+// pass) and applies the chain rule in reverse: sRGB curve, SH light, bilinear filter,
+// normalisation, interpolation, barycentrics, projection. The recompute and the adjoint run
+// in FP64: position gradients of small triangles are sums of large, nearly cancelling
+// per-pixel terms, and f32 terms would lose about three digits there. The mip level is the
+// forward pass's f32 choice. Per-vertex and per-texel sums either use atomics (FP64 for
+// positions, FP32 for the rest) or, in the deterministic mode, are emitted as records, sorted
+// by target (stable radix sort) and summed per segment in pixel order.
+#include <algorithm>
+
+#include "mlv_device.h"
+
+namespace mlv {
+
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+// real spherical harmonics of bands 0..2 (f32 constants; the FP64 path widens the same values)
+constexpr float kY0 = 0.282094792f, kY1 = 0.488602512f, kY2 = 1.092548431f, kY3 = 0.315391565f,
+                kY4 = 0.546274215f;
+constexpr float kSrgbP = 0.416666667f;
+
+__device__ __forceinline__ float r_pow(float x, float y) { return powf(x, y); }
+__device__ __forceinline__ double r_pow(double x, double y) { return pow(x, y); }
+__device__ __forceinline__ float r_sqrt(float x) { return sqrtf(x); }
+__device__ __forceinline__ double r_sqrt(double x) { return sqrt(x); }
+__device__ __forceinline__ float r_floor(float x) { return floorf(x); }
+__device__ __forceinline__ double r_floor(double x) { return floor(x); }
+
+// basic_ps's curve (math.h:419-422) on max(c, 0): f32 in the reference's operation order,
+// FP64 for the adjoint's recompute
+__device__ __forceinline__ float srgb_of(float c) { return c > 0.0f ? srgb_approx(c) : 0.0f; }
+__device__ __forceinline__ double srgb_of(double c) {
+    if (!(c > 0.0)) return 0.0;
+    const double s = 1.055 * pow(c, (double)kSrgbP) - 0.055;
+    return s > 0.0 ? s : 0.0;
+}
+
+template <class R>
+__device__ __forceinline__ void sh_basis(R x, R y, R z, R Y[9]) {
+    Y[0] = (R)kY0;
+    Y[1] = (R)kY1 * y;
+    Y[2] = (R)kY1 * z;
+    Y[3] = (R)kY1 * x;
+    Y[4] = (R)kY2 * (x * y);
+    Y[5] = (R)kY2 * (y * z);
+    Y[6] = (R)kY3 * ((R)3 * (z * z) - (R)1);
+    Y[7] = (R)kY2 * (x * z);
+    Y[8] = (R)kY4 * (x * x - y * y);
+}
+
+template <class R>
+struct Clip {
+    R X[3], Y[3], W[3];
+};
+
+__device__ __forceinline__ void clip_of(const DiffParams &p, const uint32_t vi[3], Clip<float> &c) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float px = p.pos[3u * vi[k]], py = p.pos[3u * vi[k] + 1u], pz = p.pos[3u * vi[k] + 2u];
+        c.X[k] = dot4_pair(p.M + 0, px, py, pz, 1.0f);
+        c.Y[k] = dot4_pair(p.M + 4, px, py, pz, 1.0f);
+        c.W[k] = dot4_pair(p.M + 12, px, py, pz, 1.0f);
+    }
+}
+
+__device__ __forceinline__ void clip_of(const DiffParams &p, const uint32_t vi[3], Clip<double> &c) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double px = p.pos[3u * vi[k]], py = p.pos[3u * vi[k] + 1u], pz = p.pos[3u * vi[k] + 2u];
+        const float *M = p.M;
+        c.X[k] = (double)M[0] * px + (double)M[1] * py + (double)M[2] * pz + (double)M[3];
+        c.Y[k] = (double)M[4] * px + (double)M[5] * py + (double)M[6] * pz + (double)M[7];
+        c.W[k] = (double)M[12] * px + (double)M[13] * py + (double)M[14] * pz + (double)M[15];
+    }
+}
+
+// pixel corner (x, y) in NDC, the inverse of screen_from_ndc (main.c:826-831)
+__device__ __forceinline__ float ndc_u(float x, uint32_t W) { return (2.0f * x) / (float)W - 1.0f; }
+__device__ __forceinline__ float ndc_v(float y, uint32_t H) { return 1.0f - (2.0f * y) / (float)H; }
+
+template <class R>
+struct Bary {
+    R a[3], b[3], l[3], S, lam[3];
+};
+
+template <class R>
+__device__ __forceinline__ void bary(const Clip<R> &c, R u, R v, Bary<R> &B) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        B.a[k] = c.X[k] - u * c.W[k];
+        B.b[k] = c.Y[k] - v * c.W[k];
+    }
+    B.l[0] = B.a[1] * B.b[2] - B.a[2] * B.b[1];
+    B.l[1] = B.a[2] * B.b[0] - B.a[0] * B.b[2];
+    B.l[2] = B.a[0] * B.b[1] - B.a[1] * B.b[0];
+    B.S = (B.l[0] + B.l[1]) + B.l[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) B.lam[k] = B.l[k] / B.S;
+}
+
+__device__ __forceinline__ void uv_at(const DiffParams &p, const uint32_t vi[3], const float lam[3], float &u,
+                                      float &v) {
+    u = (lam[0] * p.uv[2u * vi[0]] + lam[1] * p.uv[2u * vi[1]]) + lam[2] * p.uv[2u * vi[2]];
+    v = (lam[0] * p.uv[2u * vi[0] + 1u] + lam[1] * p.uv[2u * vi[1] + 1u]) + lam[2] * p.uv[2u * vi[2] + 1u];
+}
+
+// Mip level nearest the pixel's footprint, in f32: the uv differences to the next pixel right
+// and below, in level-0 texels; level k is taken when rho >= 2^(k - 1/2), i.e.
+// rho^2 >= 2^(2k-1), which needs no logarithm.
+__device__ __forceinline__ uint32_t mip_level(const DiffParams &p, const uint32_t vi[3], const Clip<float> &c,
+                                              float x, float y) {
+    if (p.levels <= 1u) return 0u;
+    Bary<float> b0, bx, by;
+    bary(c, ndc_u(x, p.W), ndc_v(y, p.H), b0);
+    bary(c, ndc_u(x + 1.0f, p.W), ndc_v(y, p.H), bx);
+    bary(c, ndc_u(x, p.W), ndc_v(y + 1.0f, p.H), by);
+    float u0, v0, ux, vx, uy, vy;
+    uv_at(p, vi, b0.lam, u0, v0);
+    uv_at(p, vi, bx.lam, ux, vx);
+    uv_at(p, vi, by.lam, uy, vy);
+    const float tw = (float)p.lw[0], th = (float)p.lh[0];
+    const float dux = (ux - u0) * tw, dvx = (vx - v0) * th, duy = (uy - u0) * tw, dvy = (vy - v0) * th;
+    const float rx = dux * dux + dvx * dvx, ry = duy * duy + dvy * dvy;
+    const float r2 = rx > ry ? rx : ry;
+    uint32_t lvl = 0u;
+    float thr = 2.0f;  // 2^(2k-1) for k = 1
+    for (uint32_t k = 1; k < p.levels; ++k) {
+        if (r2 >= thr) lvl = k;
+        thr *= 4.0f;
+    }
+    return lvl;
+}
+
+// Everything one pixel computes, kept for the backward pass.
+template <class R>
+struct Eval {
+    uint32_t vi[3];
+    Bary<R> B;
+    R u, v;                    // NDC of the pixel corner
+    R col[3], uvs[2], n[3];
+    uint32_t lvl;
+    int32_t x0, y0, x1, y1;    // clamped taps
+    R fs, ft;
+    R t00[3], t10[3], t01[3], t11[3];
+    R alb[3];
+    R nh[3], inv_len;
+    R Y[9], E[3], lin[3], out[3];
+};
+
+template <class R>
+__device__ __forceinline__ R lerp_r(R a, R b, R f) {
+    return a * ((R)1 - f) + b * f;
+}
+
+template <class R>
+__device__ __forceinline__ bool evaluate(const DiffParams &p, uint32_t x, uint32_t y, Eval<R> &e) {
+    const uint32_t t = p.prim[(size_t)y * p.W + x];
+    if (t == kNone || t >= p.T) return false;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) e.vi[k] = p.idx[3u * t + (uint32_t)k];
+    if (e.vi[0] >= p.V || e.vi[1] >= p.V || e.vi[2] >= p.V) return false;
+    Clip<float> cf;
+    clip_of(p, e.vi, cf);
+    e.lvl = mip_level(p, e.vi, cf, (float)x, (float)y);
+    Clip<R> c;
+    clip_of(p, e.vi, c);
+    e.u = (R)ndc_u((float)x, p.W);
+    e.v = (R)ndc_v((float)y, p.H);
+    bary(c, e.u, e.v, e.B);
+    const R *L = e.B.lam;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        e.col[j] = (L[0] * (R)p.col[3u * e.vi[0] + j] + L[1] * (R)p.col[3u * e.vi[1] + j]) +
+                   L[2] * (R)p.col[3u * e.vi[2] + j];
+        e.n[j] = (L[0] * (R)p.nrm[3u * e.vi[0] + j] + L[1] * (R)p.nrm[3u * e.vi[1] + j]) +
+                 L[2] * (R)p.nrm[3u * e.vi[2] + j];
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        e.uvs[j] = (L[0] * (R)p.uv[2u * e.vi[0] + j] + L[1] * (R)p.uv[2u * e.vi[1] + j]) +
+                   L[2] * (R)p.uv[2u * e.vi[2] + j];
+    // bilinear_f_x8 (csc.h:143-161) on the chosen level
+    const uint32_t lw = p.lw[e.lvl], lh = p.lh[e.lvl];
+    const R s = (R)lw * e.uvs[0] - (R)0.5;
+    const R tt = (R)lh * ((R)1 - e.uvs[1]) - (R)0.5;
+    const R fx = r_floor(s), fy = r_floor(tt);
+    e.fs = s - fx;
+    e.ft = tt - fy;
+    const int32_t ix = (fx > (R)-1e9 && fx < (R)1e9) ? (int32_t)fx : 0;
+    const int32_t iy = (fy > (R)-1e9 && fy < (R)1e9) ? (int32_t)fy : 0;
+    const int32_t wm = (int32_t)lw - 1, hm = (int32_t)lh - 1;
+    e.x0 = min(max(ix, 0), wm);
+    e.x1 = min(max(ix + 1, 0), wm);
+    e.y0 = min(max(iy, 0), hm);
+    e.y1 = min(max(iy + 1, 0), hm);
+    const float4 *T = p.tex + p.loff[e.lvl];
+    const float4 q00 = T[(uint32_t)e.y0 * lw + (uint32_t)e.x0], q10 = T[(uint32_t)e.y0 * lw + (uint32_t)e.x1],
+                 q01 = T[(uint32_t)e.y1 * lw + (uint32_t)e.x0], q11 = T[(uint32_t)e.y1 * lw + (uint32_t)e.x1];
+    e.t00[0] = q00.x; e.t00[1] = q00.y; e.t00[2] = q00.z;
+    e.t10[0] = q10.x; e.t10[1] = q10.y; e.t10[2] = q10.z;
+    e.t01[0] = q01.x; e.t01[1] = q01.y; e.t01[2] = q01.z;
+    e.t11[0] = q11.x; e.t11[1] = q11.y; e.t11[2] = q11.z;
+#pragma unroll
+    for (int j = 0; j < 3; ++j)
+        e.alb[j] = lerp_r(lerp_r(e.t00[j], e.t10[j], e.fs), lerp_r(e.t01[j], e.t11[j], e.fs), e.ft);
+    const R len = r_sqrt((e.n[0] * e.n[0] + e.n[1] * e.n[1]) + e.n[2] * e.n[2]);
+    e.inv_len = (R)1 / len;
+    e.nh[0] = e.n[0] * e.inv_len;
+    e.nh[1] = e.n[1] * e.inv_len;
+    e.nh[2] = e.n[2] * e.inv_len;
+    sh_basis(e.nh[0], e.nh[1], e.nh[2], e.Y);
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        R acc = (R)0;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) acc = acc + (R)p.sh[3 * k + j] * e.Y[k];
+        e.E[j] = acc;
+        e.lin[j] = (e.alb[j] * e.col[j]) * e.E[j];
+        e.out[j] = srgb_of(e.lin[j]);
+    }
+    return true;
+}
+
+__global__ __launch_bounds__(256) void kd_forward(DiffParams p, float4 *out) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= p.W * p.H) return;
+    Eval<float> e;
+    const uint32_t x = i % p.W, y = i / p.W;
+    out[i] = evaluate(p, x, y, e) ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// Per-pixel adjoint (FP64). Outputs: per-corner vertex gradients (dpos, dcol, duv, dn),
+// per-tap texel gradients (rgb) and the pixel's SH gradient.
+struct Adj {
+    double dpos[3][3], dcol[3][3], duv[3][2], dn[3][3];
+    double dtex[4][3];
+    uint32_t tap[4];
+    double dsh[27];
+};
+
+__device__ __forceinline__ void adjoint(const DiffParams &p, const Eval<double> &e, float4 g, Adj &a) {
+    const double P = (double)kSrgbP;
+    double dlin[3];
+    const double gg[3] = {g.x, g.y, g.z};
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        const bool on = e.lin[j] > 0.0 && e.out[j] > 0.0;
+        dlin[j] = on ? gg[j] * (1.055 * P) * pow(e.lin[j], P - 1.0) : 0.0;
+    }
+    double dalb[3], dci[3], dE[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        dalb[j] = dlin[j] * e.col[j] * e.E[j];
+        dci[j] = dlin[j] * e.alb[j] * e.E[j];
+        dE[j] = dlin[j] * e.alb[j] * e.col[j];
+    }
+    double dY[9];
+#pragma unroll
+    for (int k = 0; k < 9; ++k) {
+        a.dsh[3 * k + 0] = dE[0] * e.Y[k];
+        a.dsh[3 * k + 1] = dE[1] * e.Y[k];
+        a.dsh[3 * k + 2] = dE[2] * e.Y[k];
+        dY[k] = dE[0] * (double)p.sh[3 * k] + dE[1] * (double)p.sh[3 * k + 1] + dE[2] * (double)p.sh[3 * k + 2];
+    }
+    const double x = e.nh[0], y = e.nh[1], z = e.nh[2];
+    const double c1 = kY1, c2 = kY2, c3 = kY3, c4 = kY4;
+    double dnh[3];
+    dnh[0] = c1 * dY[3] + c2 * (y * dY[4] + z * dY[7]) + 2.0 * c4 * x * dY[8];
+    dnh[1] = c1 * dY[1] + c2 * (x * dY[4] + z * dY[5]) - 2.0 * c4 * y * dY[8];
+    dnh[2] = c1 * dY[2] + c2 * (y * dY[5] + x * dY[7]) + 6.0 * c3 * z * dY[6];
+    const double pr = x * dnh[0] + y * dnh[1] + z * dnh[2];
+    const double dni[3] = {(dnh[0] - x * pr) * e.inv_len, (dnh[1] - y * pr) * e.inv_len,
+                           (dnh[2] - z * pr) * e.inv_len};
+    // bilinear taps and the uv derivative (s = w u - 0.5, t = h (1 - v) - 0.5)
+    const double w00 = (1.0 - e.fs) * (1.0 - e.ft), w10 = e.fs * (1.0 - e.ft), w01 = (1.0 - e.fs) * e.ft,
+                 w11 = e.fs * e.ft;
+    const uint32_t lw = p.lw[e.lvl], base = p.loff[e.lvl];
+    a.tap[0] = base + (uint32_t)e.y0 * lw + (uint32_t)e.x0;
+    a.tap[1] = base + (uint32_t)e.y0 * lw + (uint32_t)e.x1;
+    a.tap[2] = base + (uint32_t)e.y1 * lw + (uint32_t)e.x0;
+    a.tap[3] = base + (uint32_t)e.y1 * lw + (uint32_t)e.x1;
+    double ds = 0.0, dt = 0.0;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        a.dtex[0][j] = w00 * dalb[j];
+        a.dtex[1][j] = w10 * dalb[j];
+        a.dtex[2][j] = w01 * dalb[j];
+        a.dtex[3][j] = w11 * dalb[j];
+        const double dAs = (1.0 - e.ft) * (e.t10[j] - e.t00[j]) + e.ft * (e.t11[j] - e.t01[j]);
+        const double dAt = lerp_r(e.t01[j], e.t11[j], e.fs) - lerp_r(e.t00[j], e.t10[j], e.fs);
+        ds += dalb[j] * dAs;
+        dt += dalb[j] * dAt;
+    }
+    const double duvi[2] = {ds * (double)lw, -dt * (double)p.lh[e.lvl]};
+    // interpolation: attributes to corners and barycentric adjoints
+    const double *L = e.B.lam;
+    double dlam[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t v = e.vi[k];
+        double acc = 0.0;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            a.dcol[k][j] = L[k] * dci[j];
+            a.dn[k][j] = L[k] * dni[j];
+            acc += dci[j] * (double)p.col[3u * v + j] + dni[j] * (double)p.nrm[3u * v + j];
+        }
+        a.duv[k][0] = L[k] * duvi[0];
+        a.duv[k][1] = L[k] * duvi[1];
+        acc += duvi[0] * (double)p.uv[2u * v] + duvi[1] * (double)p.uv[2u * v + 1u];
+        dlam[k] = acc;
+    }
+    // lambda = l / S
+    const double mean = L[0] * dlam[0] + L[1] * dlam[1] + L[2] * dlam[2];
+    double dl[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dl[k] = (dlam[k] - mean) / e.B.S;
+    // l = a x b
+    const double *A = e.B.a, *Bb = e.B.b;
+    double da[3], db[3];
+    da[0] = dl[2] * Bb[1] - dl[1] * Bb[2];
+    da[1] = dl[0] * Bb[2] - dl[2] * Bb[0];
+    da[2] = dl[1] * Bb[0] - dl[0] * Bb[1];
+    db[0] = dl[1] * A[2] - dl[2] * A[1];
+    db[1] = dl[2] * A[0] - dl[0] * A[2];
+    db[2] = dl[0] * A[1] - dl[1] * A[0];
+    // a = X - u W, b = Y - v W; clip = M (p, 1)
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const double dX = da[k], dYc = db[k], dW = -e.u * da[k] - e.v * db[k];
+#pragma unroll
+        for (int j = 0; j < 3; ++j)
+            a.dpos[k][j] = (double)p.M[j] * dX + (double)p.M[4 + j] * dYc + (double)p.M[12 + j] * dW;
+    }
+}
+
+__device__ __forceinline__ void atomic_add(float *dst, double v) {
+    if (v != 0.0) unsafeAtomicAdd(dst, (float)v);
+}
+
+__device__ __forceinline__ void atomic_add(double *dst, double v) {
+    if (v != 0.0) unsafeAtomicAdd(dst, v);
+}
+
+__device__ __forceinline__ double wave_sum_d(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block sum of the 27 SH gradients into sh_part[block][27] (fixed order: deterministic).
+__device__ __forceinline__ void sh_block_sum(const double dsh[27], double *sh_part) {
+    __shared__ double s_w[4][27];
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int k = 0; k < 27; ++k) {
+        const double v = wave_sum_d(dsh[k]);
+        if (lane == 0u) s_w[w][k] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < 27u)
+        sh_part[blockIdx.x * 27u + threadIdx.x] =
+            ((s_w[0][threadIdx.x] + s_w[1][threadIdx.x]) + s_w[2][threadIdx.x]) + s_w[3][threadIdx.x];
+}
+
+__global__ __launch_bounds__(256) void kd_backward_atomic(DiffParams p, const float4 *gout, DiffGrads g,
+                                                          double *posacc, double *sh_part) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    Eval<double> e;
+    Adj a;
+    bool live = false;
+    if (i < p.W * p.H) {
+        live = evaluate(p, i % p.W, i / p.W, e);
+        if (live) adjoint(p, e, gout[i], a);
+    }
+    if (!live)
+        for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
+    if (live) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = e.vi[k];
+            for (int j = 0; j < 3; ++j) {
+                if (posacc) atomic_add(&posacc[3u * v + j], a.dpos[k][j]);
+                if (g.col) atomic_add(&g.col[3u * v + j], a.dcol[k][j]);
+                if (g.nrm) atomic_add(&g.nrm[3u * v + j], a.dn[k][j]);
+            }
+            if (g.uv) {
+                atomic_add(&g.uv[2u * v], a.duv[k][0]);
+                atomic_add(&g.uv[2u * v + 1u], a.duv[k][1]);
+            }
+        }
+        if (g.tex)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                for (int j = 0; j < 3; ++j) atomic_add(&g.tex[4u * a.tap[t] + j], a.dtex[t][j]);
+    }
+    sh_block_sum(a.dsh, sh_part);
+}
+
+__global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const float4 *gout, DiffScratch sc) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    const uint32_t P = p.W * p.H;
+    Eval<double> e;
+    Adj a;
+    bool live = false;
+    if (i < P) {
+        live = evaluate(p, i % p.W, i / p.W, e);
+        if (live) adjoint(p, e, gout[i], a);
+    }
+    if (!live)
+        for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
+    if (i < P) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+            const uint32_t r = 4u * i + (uint32_t)t;
+            sc.tkey[r] = live ? a.tap[t] : p.tex_total;
+            sc.tval[r] = r;
+            sc.tpay[r] = live ? make_float4((float)a.dtex[t][0], (float)a.dtex[t][1], (float)a.dtex[t][2], 0.0f)
+                              : make_float4(0.f, 0.f, 0.f, 0.f);
+        }
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t r = 3u * i + (uint32_t)k;
+            sc.vkey[r] = live ? e.vi[k] : p.V;
+            sc.vval[r] = r;
+            if (live) {
+                sc.vposd[4u * r] = a.dpos[k][0];
+                sc.vposd[4u * r + 1u] = a.dpos[k][1];
+                sc.vposd[4u * r + 2u] = a.dpos[k][2];
+                sc.vpay[2u * r] = make_float4((float)a.dcol[k][0], (float)a.dcol[k][1], (float)a.dcol[k][2],
+                                              (float)a.duv[k][0]);
+                sc.vpay[2u * r + 1u] = make_float4((float)a.duv[k][1], (float)a.dn[k][0], (float)a.dn[k][1],
+                                                   (float)a.dn[k][2]);
+            }
+        }
+    }
+    sh_block_sum(a.dsh, sc.sh_part);
+}
+
+// Sum of the per-block SH partials in block order.
+__global__ __launch_bounds__(64) void kd_sh_sum(const double *sh_part, uint32_t nblocks, float *dsh) {
+    if (threadIdx.x >= 27u) return;
+    double acc = 0.0;
+    for (uint32_t b = 0; b < nblocks; ++b) acc += sh_part[b * 27u + threadIdx.x];
+    dsh[threadIdx.x] = (float)acc;
+}
+
+__global__ __launch_bounds__(256) void kd_pos_finalize(const double *acc, float *out, uint32_t n) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i < n) out[i] = (float)acc[i];
+}
+
+// Segments of the sorted texel records: the first record of each texel sums its segment in
+// pixel order (the sort is stable) and writes the texel's gradient.
+__global__ __launch_bounds__(256) void kd_segment_texel(const uint32_t *key, const uint32_t *val, const float4 *pay,
+                                                        uint32_t n, uint32_t none, float *dtex) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = key[i];
+    if (k >= none || (i > 0u && key[i - 1u] == k)) return;
+    double r = 0.0, g = 0.0, b = 0.0;
+    for (uint32_t j = i; j < n && key[j] == k; ++j) {
+        const float4 v = pay[val[j]];
+        r += v.x;
+        g += v.y;
+        b += v.z;
+    }
+    dtex[4u * k] = (float)r;
+    dtex[4u * k + 1u] = (float)g;
+    dtex[4u * k + 2u] = (float)b;
+}
+
+__global__ __launch_bounds__(256) void kd_segment_vertex(const uint32_t *key, const uint32_t *val, const float4 *pay,
+                                                         const double *posd, uint32_t n, uint32_t none, DiffGrads g) {
+    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t k = key[i];
+    if (k >= none || (i > 0u && key[i - 1u] == k)) return;
+    double s[11] = {};
+    for (uint32_t j = i; j < n && key[j] == k; ++j) {
+        const uint32_t r = val[j];
+        s[0] += posd[4u * r];
+        s[1] += posd[4u * r + 1u];
+        s[2] += posd[4u * r + 2u];
+        const float4 a = pay[2u * r], b = pay[2u * r + 1u];
+        s[3] += a.x; s[4] += a.y; s[5] += a.z; s[6] += a.w;
+        s[7] += b.x; s[8] += b.y; s[9] += b.z; s[10] += b.w;
+    }
+    if (g.pos) { g.pos[3u * k] = (float)s[0]; g.pos[3u * k + 1u] = (float)s[1]; g.pos[3u * k + 2u] = (float)s[2]; }
+    if (g.col) { g.col[3u * k] = (float)s[3]; g.col[3u * k + 1u] = (float)s[4]; g.col[3u * k + 2u] = (float)s[5]; }
+    if (g.uv) { g.uv[2u * k] = (float)s[6]; g.uv[2u * k + 1u] = (float)s[7]; }
+    if (g.nrm) { g.nrm[3u * k] = (float)s[8]; g.nrm[3u * k + 1u] = (float)s[9]; g.nrm[3u * k + 2u] = (float)s[10]; }
+}
+
+template <class T>
+__global__ void kd_zero(T *d, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
+        d[i] = T(0);
+}
+
+template <class T>
+hipError_t zero(T *d, size_t n, hipStream_t s) {
+    if (!d || !n) return hipSuccess;
+    const unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 8192);
+    hipLaunchKernelGGL(kd_zero<T>, dim3(grid), dim3(256), 0, s, d, n);
+    return hipGetLastError();
+}
+
+uint32_t bits_for(uint32_t n) {
+    uint32_t b = 0;
+    while (b < 32 && (1ull << b) <= n) ++b;
+    return b;
+}
+
+}  // namespace
+
+uint32_t diff_blocks(uint32_t W, uint32_t H) { return (W * H + 255u) / 256u; }
+
+hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s) {
+    const uint32_t nb = diff_blocks(p.W, p.H);
+    if (nb == 0) return hipSuccess;
+    hipLaunchKernelGGL(kd_forward, dim3(nb), dim3(256), 0, s, p, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const DiffGrads &g, const DiffScratch &sc,
+                                bool deterministic, hipStream_t s) {
+    const uint32_t nb = diff_blocks(p.W, p.H);
+    hipError_t e;
+    if ((e = zero(g.pos, 3ull * p.V, s)) != hipSuccess || (e = zero(g.col, 3ull * p.V, s)) != hipSuccess ||
+        (e = zero(g.uv, 2ull * p.V, s)) != hipSuccess || (e = zero(g.nrm, 3ull * p.V, s)) != hipSuccess ||
+        (e = zero(g.tex, 4ull * p.tex_total, s)) != hipSuccess)
+        return e;
+    if (nb == 0) return zero(g.sh, 27, s);
+    if (!deterministic) {
+        double *posacc = g.pos ? sc.posacc : nullptr;
+        if ((e = zero(posacc, 3ull * p.V, s)) != hipSuccess) return e;
+        hipLaunchKernelGGL(kd_backward_atomic, dim3(nb), dim3(256), 0, s, p, gout, g, posacc, sc.sh_part);
+        if (g.pos)
+            hipLaunchKernelGGL(kd_pos_finalize, dim3((3u * p.V + 255u) / 256u), dim3(256), 0, s,
+                               (const double *)posacc, g.pos, 3u * p.V);
+    } else {
+        hipLaunchKernelGGL(kd_backward_records, dim3(nb), dim3(256), 0, s, p, gout, sc);
+        const uint32_t P = p.W * p.H;
+        bool alt = false;
+        if (g.tex) {
+            if ((e = sort_pairs(sc.tkey, sc.tval, sc.tkey2, sc.tval2, 4u * P, bits_for(p.tex_total), sc.hist, sc.part,
+                                &alt, s)) != hipSuccess)
+                return e;
+            hipLaunchKernelGGL(kd_segment_texel, dim3((4u * P + 255u) / 256u), dim3(256), 0, s,
+                               (const uint32_t *)(alt ? sc.tkey2 : sc.tkey), (const uint32_t *)(alt ? sc.tval2 : sc.tval),
+                               (const float4 *)sc.tpay, 4u * P, p.tex_total, g.tex);
+        }
+        if (g.pos || g.col || g.uv || g.nrm) {
+            if ((e = sort_pairs(sc.vkey, sc.vval, sc.vkey2, sc.vval2, 3u * P, bits_for(p.V), sc.hist, sc.part, &alt,
+                                s)) != hipSuccess)
+                return e;
+            hipLaunchKernelGGL(kd_segment_vertex, dim3((3u * P + 255u) / 256u), dim3(256), 0, s,
+                               (const uint32_t *)(alt ? sc.vkey2 : sc.vkey), (const uint32_t *)(alt ? sc.vval2 : sc.vval),
+                               (const float4 *)sc.vpay, (const double *)sc.vposd, 3u * P, p.V, g);
+        }
+    }
+    if (g.sh) hipLaunchKernelGGL(kd_sh_sum, dim3(1), dim3(64), 0, s, (const double *)sc.sh_part, nb, g.sh);
+    return hipGetLastError();
+}
+
+}  // namespace mlv

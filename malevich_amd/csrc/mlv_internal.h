@@ -68,6 +68,7 @@ struct ViewRec {
     float *depth;
     float *hiz;
     uint32_t *vis_tri, *vis_draw;
+    uint32_t *vis_prim;           // input triangle index of the visible triangle (optional)
     int32_t W, H, tiles_x, tiles_y, bins_x, bins_y;
     uint32_t tile_base;           // first global tile of this view
     int32_t depth_func;
@@ -140,6 +141,45 @@ struct BatchParams {
     BatchCounters *ctr;
     DevStats *stats;
 };
+
+// Differentiable shading of one mesh over the raster's primitive ids (mlv_diff.hip).
+constexpr uint32_t kMaxMips = 16;
+struct DiffParams {
+    const float *pos, *col, *uv, *nrm;
+    const uint32_t *idx;
+    uint32_t V, T;
+    const float4 *tex;
+    uint32_t tex_total;                   // texels of the whole chain
+    uint32_t levels;
+    uint32_t lw[kMaxMips], lh[kMaxMips], loff[kMaxMips];
+    const float *sh;
+    float M[16];
+    uint32_t W, H;
+    const uint32_t *prim;
+};
+struct DiffGrads {
+    float *pos, *col, *uv, *nrm, *tex, *sh;
+};
+// Scratch of the deterministic backward: per-pixel contributions, their sort keys and order.
+struct DiffScratch {
+    uint32_t *tkey, *tval, *tkey2, *tval2;   // 4 per pixel (bilinear taps)
+    float4 *tpay;
+    uint32_t *vkey, *vval, *vkey2, *vval2;   // 3 per pixel (triangle corners)
+    float4 *vpay;                            // 2 float4 per record: dcol.xyz duv.x | duv.y dn.xyz
+    double *vposd;                           // 4 doubles per record: dpos.xyz -
+    double *posacc;                          // [V][3] position gradient accumulator (atomic mode)
+    double *sh_part;                         // 27 per block
+    void *part;                              // scan partials
+    uint32_t *hist;                          // radix digit counts
+};
+hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s);
+hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const DiffGrads &g, const DiffScratch &sc,
+                                bool deterministic, hipStream_t s);
+uint32_t diff_blocks(uint32_t W, uint32_t H);
+// Stable LSD sort of n (key, value) pairs by the low `bits` of the key (8 bits per pass).
+// Returns in `*in_alt` whether the sorted data ended in k1/v1 (else k0/v0).
+hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, uint32_t bits,
+                      uint32_t *hist, void *part, bool *in_alt, hipStream_t s);
 
 // Scratch of the multi-block exclusive scans and of the radix sort histograms.
 struct ScanScratch {

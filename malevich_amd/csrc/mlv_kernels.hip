@@ -553,15 +553,13 @@ __global__ __launch_bounds__(256) void kb_pair_check(BatchParams p, uint32_t n_p
 
 size_t radix_hist_elems(size_t n_max) { return 256 * ((n_max + kRadixItems - 1) / kRadixItems + 1); }
 
-__global__ __launch_bounds__(256) void kb_radix_hist(BatchParams p, const uint32_t *keys, uint32_t shift,
-                                                     uint32_t *hist) {
+// Digit counts of one block's kRadixItems keys, stored digit-major (hist[digit * nb + block])
+// so that one exclusive scan over hist gives every (digit, block) its output offset.
+__device__ __forceinline__ void radix_hist_block(const uint32_t *keys, uint32_t n, uint32_t nb, uint32_t shift,
+                                                 uint32_t *hist) {
     __shared__ uint32_t h[256];
-    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
-    const uint32_t nb = p.ctr->radix_blocks;
-    if (blockIdx.x >= nb) return;
     h[threadIdx.x] = 0u;
     __syncthreads();
-    const uint32_t n = p.ctr->n_pairs;
     const uint32_t base = blockIdx.x * kRadixItems;
     for (uint32_t r = 0; r < kRadixRounds; ++r) {
         const uint32_t i = base + r * 256u + threadIdx.x;
@@ -571,16 +569,13 @@ __global__ __launch_bounds__(256) void kb_radix_hist(BatchParams p, const uint32
     hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
 }
 
-__global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uint32_t *kin, const uint32_t *vin,
-                                                        uint32_t *kout, uint32_t *vout, uint32_t shift,
-                                                        const uint32_t *hist) {
+// Stable scatter of one block's keys and values to the scanned offsets.
+__device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                                                    uint32_t *vout, uint32_t n, uint32_t nb, uint32_t shift,
+                                                    const uint32_t *hist) {
     __shared__ uint32_t run[256];
     __shared__ uint32_t wc[4][256];
-    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
-    const uint32_t n = p.ctr->n_pairs;
-    const uint32_t nb = p.ctr->radix_blocks;
     const uint32_t base = blockIdx.x * kRadixItems;
-    if (base >= n) return;
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     run[tid] = 0u;
     for (uint32_t r = 0; r < kRadixRounds; ++r) {
@@ -620,6 +615,56 @@ __global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uin
         }
         __syncthreads();
     }
+}
+
+// Binning passes: the pair count and the block count are read from the device counters.
+__global__ __launch_bounds__(256) void kb_radix_hist(BatchParams p, const uint32_t *keys, uint32_t shift,
+                                                     uint32_t *hist) {
+    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
+    const uint32_t nb = p.ctr->radix_blocks;
+    if (blockIdx.x >= nb) return;
+    radix_hist_block(keys, p.ctr->n_pairs, nb, shift, hist);
+}
+
+__global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uint32_t *kin, const uint32_t *vin,
+                                                        uint32_t *kout, uint32_t *vout, uint32_t shift,
+                                                        const uint32_t *hist) {
+    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
+    const uint32_t nb = p.ctr->radix_blocks;
+    if (blockIdx.x >= nb) return;
+    radix_scatter_block(kin, vin, kout, vout, p.ctr->n_pairs, nb, shift, hist);
+}
+
+// Host-sized passes (n known when launching): one block per kRadixItems keys.
+__global__ __launch_bounds__(256) void k_sort_hist(const uint32_t *keys, uint32_t n, uint32_t shift, uint32_t *hist) {
+    radix_hist_block(keys, n, gridDim.x, shift, hist);
+}
+
+__global__ __launch_bounds__(256) void k_sort_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
+                                                      uint32_t *vout, uint32_t n, uint32_t shift,
+                                                      const uint32_t *hist) {
+    radix_scatter_block(kin, vin, kout, vout, n, gridDim.x, shift, hist);
+}
+
+hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, uint32_t bits,
+                      uint32_t *hist, void *part, bool *in_alt, hipStream_t s) {
+    *in_alt = false;
+    if (n == 0) return hipSuccess;
+    const uint32_t nb = (n + kRadixItems - 1) / kRadixItems;
+    const uint32_t passes = std::max<uint32_t>(1u, (bits + 7u) / 8u);
+    uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
+    for (uint32_t pass = 0; pass < passes; ++pass) {
+        const uint32_t shift = 8u * pass;
+        hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(256), 0, s, (const uint32_t *)kin, n, shift, hist);
+        hipError_t e = launch_scan<uint32_t>(hist, hist, (size_t)nb * 256u, nullptr, part, nullptr, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(256), 0, s, (const uint32_t *)kin, (const uint32_t *)vin,
+                           kout, vout, n, shift, (const uint32_t *)hist);
+        std::swap(kin, kout);
+        std::swap(vin, vout);
+        *in_alt = !*in_alt;
+    }
+    return hipGetLastError();
 }
 
 // Tile ranges from the sorted keys: the first pair of a tile writes its start, the last its
@@ -822,6 +867,7 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
             V.vis_tri[pix] = T.local;
             V.vis_draw[pix] = D.draw_tag;
         }
+        if (V.vis_prim) V.vis_prim[pix] = S.prim;
     }
 
     __shared__ uint32_t s_active[4];

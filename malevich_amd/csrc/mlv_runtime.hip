@@ -64,10 +64,11 @@ struct Arena {
 struct ViewKey {
     mlv_render_target *rt;
     mlv_depth_stencil *ds;
-    uint32_t *vis_tri, *vis_draw;
+    uint32_t *vis_tri, *vis_draw, *vis_prim;
     int depth_func;
     bool operator==(const ViewKey &o) const {
-        return rt == o.rt && ds == o.ds && vis_tri == o.vis_tri && vis_draw == o.vis_draw && depth_func == o.depth_func;
+        return rt == o.rt && ds == o.ds && vis_tri == o.vis_tri && vis_draw == o.vis_draw && vis_prim == o.vis_prim &&
+               depth_func == o.depth_func;
     }
 };
 
@@ -103,13 +104,16 @@ struct mlv_context {
     mlv_render_target *rt = nullptr;
     mlv_depth_stencil *ds = nullptr;
     int depth_func = MLV_DEPTH_GREATER_EQUAL;
-    uint32_t *vis_tri = nullptr, *vis_draw = nullptr;
+    uint32_t *vis_tri = nullptr, *vis_draw = nullptr, *vis_prim = nullptr;
     // recorded draws
     std::vector<PendingDraw> pending;
     uint64_t pending_tris = 0, pending_verts = 0;
     // arenas
     Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, tvis, tshade, tbin;
     Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist;
+    // differentiable shading scratch
+    Arena d_shpart, d_tkey, d_tval, d_tkey2, d_tval2, d_tpay, d_vkey, d_vval, d_vkey2, d_vval2, d_vpay, d_part, d_hist;
+    Arena d_vposd, d_posacc;
     BatchCounters *ctr = nullptr;
     DevStats *dstats = nullptr;
     // pinned staging for the draw/view tables, one slot per flush in flight
@@ -241,6 +245,7 @@ static int flush(mlv_context *ctx) {
             v.hiz = pd.key.ds->hiz;
             v.vis_tri = pd.key.vis_tri;
             v.vis_draw = pd.key.vis_draw;
+            v.vis_prim = pd.key.vis_prim;
             v.W = (int32_t)pd.key.rt->width;
             v.H = (int32_t)pd.key.rt->height;
             v.tiles_x = (v.W + kTile - 1) / kTile;
@@ -474,7 +479,10 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->blocksum,  &ctx->blockoff,  &ctx->asm_base, &ctx->clip_list, &ctx->clip_off,
                     &ctx->tvis,      &ctx->tshade,    &ctx->tbin,    &ctx->pair_cnt, &ctx->pair_off,  &ctx->keys0,
                     &ctx->vals0,     &ctx->keys1,     &ctx->vals1,    &ctx->tile_cnt,  &ctx->tile_start,
-                    &ctx->scan_part, &ctx->radix_hist};
+                    &ctx->scan_part, &ctx->radix_hist, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
+                    &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
+                    &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
+                    &ctx->d_vposd,   &ctx->d_posacc};
     for (Arena *a : all) release(*a);
     resolve_events(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -783,6 +791,12 @@ MLV_API int mlv_om_set_visibility_buffer(mlv_context *ctx, uint32_t *device_tri_
     return MLV_OK;
 }
 
+MLV_API int mlv_om_set_primitive_id_buffer(mlv_context *ctx, uint32_t *device_prim_ids) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    ctx->vis_prim = device_prim_ids;
+    return MLV_OK;
+}
+
 // ------------------------------------------------------------------ commands
 
 // encode_color_as_u32 (math.h:322-324): truncating conversion per channel
@@ -802,6 +816,8 @@ MLV_API int mlv_clear_render_target_view(mlv_context *ctx, mlv_render_target *rt
         HIP_OR_FAIL(ctx, launch_fill_u32(ctx->vis_tri, 0xFFFFFFFFu, (size_t)rt->width * rt->height, ctx->stream));
         HIP_OR_FAIL(ctx, launch_fill_u32(ctx->vis_draw, 0xFFFFFFFFu, (size_t)rt->width * rt->height, ctx->stream));
     }
+    if (ctx->vis_prim && ctx->rt == rt)
+        HIP_OR_FAIL(ctx, launch_fill_u32(ctx->vis_prim, 0xFFFFFFFFu, (size_t)rt->width * rt->height, ctx->stream));
     return MLV_OK;
 }
 
@@ -874,7 +890,7 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
     if (T == 0) return MLV_OK;
 
     mlv_depth_stencil *ds = ctx->ds;
-    ViewKey key{ctx->rt, ds, ctx->vis_tri, ctx->vis_draw, ctx->depth_func};
+    ViewKey key{ctx->rt, ds, ctx->vis_tri, ctx->vis_draw, ctx->vis_prim, ctx->depth_func};
     // one depth function per target within a flush, and hierarchical-Z built for it
     for (auto &pd : ctx->pending)
         if ((pd.key.rt == key.rt || pd.key.ds == key.ds) && !(pd.key == key)) {
@@ -995,6 +1011,105 @@ MLV_API int mlv_reset_stats(mlv_context *ctx) {
     ctx->vertex_count = ctx->input_tris = ctx->draws = 0;
     ctx->capacity_errors = 0;
     ctx->draw_counter = 0;
+    return MLV_OK;
+}
+
+// ------------------------------------------------------------------ differentiable shading
+
+static int diff_params(mlv_context *ctx, const mlv_diff_scene *sc, const uint32_t *prim, DiffParams &p) {
+    if (!sc || !prim) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: scene and primitive ids are required");
+    if (!sc->positions || !sc->colors || !sc->uvs || !sc->normals || !sc->indices || !sc->texture || !sc->sh)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: a scene array is NULL");
+    if (!sc->vertex_count || !sc->triangle_count || !sc->width || !sc->height || sc->width > 16384u ||
+        sc->height > 16384u || !sc->tex_width || !sc->tex_height || sc->tex_width > 32768u || sc->tex_height > 32768u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: bad sizes");
+    uint32_t max_levels = 1;
+    while (max_levels < kMaxMips && ((sc->tex_width >> max_levels) || (sc->tex_height >> max_levels))) ++max_levels;
+    if (sc->tex_levels < 1 || sc->tex_levels > max_levels)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: tex_levels must be 1..%u for a %ux%u texture", max_levels,
+                    sc->tex_width, sc->tex_height);
+    p = DiffParams{};
+    p.pos = sc->positions;
+    p.col = sc->colors;
+    p.uv = sc->uvs;
+    p.nrm = sc->normals;
+    p.idx = sc->indices;
+    p.V = sc->vertex_count;
+    p.T = sc->triangle_count;
+    p.tex = (const float4 *)sc->texture;
+    p.levels = sc->tex_levels;
+    uint64_t off = 0;
+    for (uint32_t l = 0; l < p.levels; ++l) {
+        p.lw[l] = std::max(sc->tex_width >> l, 1u);
+        p.lh[l] = std::max(sc->tex_height >> l, 1u);
+        p.loff[l] = (uint32_t)off;
+        off += (uint64_t)p.lw[l] * p.lh[l];
+    }
+    p.tex_total = (uint32_t)off;
+    p.sh = sc->sh;
+    memcpy(p.M, sc->clip_from_world, sizeof p.M);
+    p.W = sc->width;
+    p.H = sc->height;
+    p.prim = prim;
+    return MLV_OK;
+}
+
+MLV_API int mlv_diff_forward(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
+                             float *device_rgba_out) {
+    if (!ctx || !device_rgba_out) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff_forward: bad argument");
+    DiffParams p;
+    RC_OR_RETURN(diff_params(ctx, scene, device_prim_ids, p));
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // the primitive ids come from recorded draws
+    HIP_OR_FAIL(ctx, launch_diff_forward(p, (float4 *)device_rgba_out, ctx->stream));
+    return MLV_OK;
+}
+
+MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
+                              const float *device_grad_rgba, const mlv_diff_grads *grads, int deterministic) {
+    if (!ctx || !device_grad_rgba || !grads) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff_backward: bad argument");
+    DiffParams p;
+    RC_OR_RETURN(diff_params(ctx, scene, device_prim_ids, p));
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    const uint32_t nb = diff_blocks(p.W, p.H);
+    const size_t P = (size_t)p.W * p.H;
+    DiffScratch sc{};
+    RC_OR_RETURN(ensure(ctx, ctx->d_shpart, (size_t)std::max<uint32_t>(nb, 1u) * 27 * 8));
+    sc.sh_part = (double *)ctx->d_shpart.p;
+    RC_OR_RETURN(ensure(ctx, ctx->d_posacc, (size_t)p.V * 3 * 8));
+    sc.posacc = (double *)ctx->d_posacc.p;
+    if (deterministic) {
+        const size_t nt = 4 * P, nv = 3 * P, nmax = std::max(nt, nv);
+        RC_OR_RETURN(ensure(ctx, ctx->d_tkey, nt * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_tval, nt * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_tkey2, nt * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_tval2, nt * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_tpay, nt * 16));
+        RC_OR_RETURN(ensure(ctx, ctx->d_vkey, nv * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_vval, nv * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_vkey2, nv * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_vval2, nv * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_vpay, nv * 32));
+        RC_OR_RETURN(ensure(ctx, ctx->d_vposd, nv * 32));
+        RC_OR_RETURN(ensure(ctx, ctx->d_hist, radix_hist_elems(nmax) * 4));
+        RC_OR_RETURN(ensure(ctx, ctx->d_part, scan_part_elems(radix_hist_elems(nmax)) * 8));
+        sc.tkey = (uint32_t *)ctx->d_tkey.p;
+        sc.tval = (uint32_t *)ctx->d_tval.p;
+        sc.tkey2 = (uint32_t *)ctx->d_tkey2.p;
+        sc.tval2 = (uint32_t *)ctx->d_tval2.p;
+        sc.tpay = (float4 *)ctx->d_tpay.p;
+        sc.vkey = (uint32_t *)ctx->d_vkey.p;
+        sc.vval = (uint32_t *)ctx->d_vval.p;
+        sc.vkey2 = (uint32_t *)ctx->d_vkey2.p;
+        sc.vval2 = (uint32_t *)ctx->d_vval2.p;
+        sc.vpay = (float4 *)ctx->d_vpay.p;
+        sc.vposd = (double *)ctx->d_vposd.p;
+        sc.hist = (uint32_t *)ctx->d_hist.p;
+        sc.part = ctx->d_part.p;
+    }
+    DiffGrads g{grads->positions, grads->colors, grads->uvs, grads->normals, grads->texture, grads->sh};
+    HIP_OR_FAIL(ctx, launch_diff_backward(p, (const float4 *)device_grad_rgba, g, sc, deterministic != 0, ctx->stream));
     return MLV_OK;
 }
 

@@ -212,6 +212,10 @@ class Device:
         _check(self.ctx, self.lib.mlv_om_set_visibility_buffer(
             self.ctx, tri_ids.device_ptr if tri_ids else None, draw_tags.device_ptr if draw_tags else None))
 
+    def om_set_primitive_id_buffer(self, device_ptr: int | None) -> None:
+        """Per-pixel input-triangle index of the visible triangle (u32 device array, W*H)."""
+        _check(self.ctx, self.lib.mlv_om_set_primitive_id_buffer(self.ctx, device_ptr))
+
     # ---------------------------------------------------------- commands
     def clear_render_target_view(self, rt: RenderTarget, rgba) -> None:
         c = (C.c_float * 4)(*[float(x) for x in rgba])

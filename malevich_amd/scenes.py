@@ -387,3 +387,108 @@ def c2_scene(width: int = 1920, height: int = 1080, n_instances: int = 64, seed:
         inst.append(sphere_mesh(int(freqs[i]), displace=0.15, seed=seed * 1000 + i, transform=T))
         texs.append(random_rgba8(texture_size, seed * 1000 + i) if texture_size else None)
     return C2Scene(inst, texs, cb, width, height, float((proj * k * k).sum() / (width * height)))
+
+
+# ------------------------------------------------------------------ differentiable render scene
+
+@dataclass
+class DiffMesh:
+    positions: np.ndarray  # (V, 3) float32
+    colors: np.ndarray     # (V, 3) float32
+    uvs: np.ndarray        # (V, 2) float32
+    normals: np.ndarray    # (V, 3) float32
+    indices: np.ndarray    # (T, 3) uint32
+
+    @property
+    def triangle_count(self) -> int:
+        return len(self.indices)
+
+    def basic_vertices(self) -> np.ndarray:
+        """(V, 8) float32 in basic_vs's input layout (pos3, normal3, uv2; basic_vs.c:3-8)."""
+        return np.concatenate([self.positions, self.normals, self.uvs], 1).astype(np.float32)
+
+
+def uv_sphere(rings: int, segments: int, displace: float = 0.0, seed: int = 0) -> DiffMesh:
+    """Latitude/longitude sphere with 2*rings*segments triangles (the pole rows are degenerate
+    slivers), wound counter-clockwise about the outward normal, seeded low-frequency radial
+    displacement, smooth per-vertex colours in [0.4, 1], uv = (longitude, latitude)."""
+    th = np.linspace(0.0, np.pi, rings + 1)
+    ph = np.linspace(0.0, 2 * np.pi, segments + 1)
+    T, PH = np.meshgrid(th, ph, indexing="ij")
+    unit = np.stack([np.sin(T) * np.cos(PH), np.sin(T) * np.sin(PH), np.cos(T)], -1).reshape(-1, 3)
+    uv = np.stack([PH / (2 * np.pi), 1.0 - T / np.pi], -1).reshape(-1, 2)
+    r = np.arange(rings)[:, None]
+    s = np.arange(segments)[None, :]
+    a = r * (segments + 1) + s
+    b = (r + 1) * (segments + 1) + s
+    c = a + 1
+    d = b + 1
+    faces = np.stack([np.stack([a, b, c], -1), np.stack([c, b, d], -1)], 2).reshape(-1, 3)
+    pos = unit * (1.0 + displace * value_noise3(unit, seed))[:, None] if displace else unit.copy()
+    nrm = vertex_normals(pos, faces)
+    bad = ~np.isfinite(nrm).all(1) | (np.linalg.norm(nrm, axis=1) < 0.5)
+    nrm[bad] = unit[bad]
+    col = 0.7 + 0.3 * np.stack([value_noise3(unit, seed + 11 + k, octaves=2) for k in range(3)], 1)
+    return DiffMesh(pos.astype(np.float32), col.astype(np.float32), uv.astype(np.float32), nrm.astype(np.float32),
+                    faces.astype(np.uint32))
+
+
+def smooth_texture(size: int, seed: int, lo: float = 0.25, hi: float = 1.0) -> np.ndarray:
+    """(size, size, 4) float32 RGBA of seeded smooth noise in [lo, hi], alpha 1."""
+    t = smooth_env_map(size, size, seed, lo, hi)
+    return np.ascontiguousarray(t, np.float32)
+
+
+def mip_chain(tex: np.ndarray) -> tuple[np.ndarray, int]:
+    """Full mip chain of an (h, w, 4) float texture by 2x2 box filtering (odd edges clamp);
+    returns the levels concatenated as (N, 4), level 0 first, and the level count."""
+    levels = [np.asarray(tex, np.float64)]
+    while levels[-1].shape[0] > 1 or levels[-1].shape[1] > 1:
+        t = levels[-1]
+        h, w = t.shape[:2]
+        h2, w2 = max(h // 2, 1), max(w // 2, 1)
+        t = t[: h2 * 2 if h > 1 else 1, : w2 * 2 if w > 1 else 1]
+        if h > 1:
+            t = 0.5 * (t[0::2] + t[1::2])
+        if w > 1:
+            t = 0.5 * (t[:, 0::2] + t[:, 1::2])
+        levels.append(t)
+    chain = np.concatenate([l.reshape(-1, 4) for l in levels]).astype(np.float32)
+    return chain, len(levels)
+
+
+def random_sh(seed: int) -> np.ndarray:
+    """(9, 3) float32 RGB SH light: a strong ambient band-0 term and small higher bands, so the
+    irradiance stays in about [0.5, 2] over the sphere."""
+    rng = np.random.default_rng(seed)
+    sh = rng.uniform(-0.25, 0.25, (9, 3))
+    sh[0] = rng.uniform(3.2, 3.8, 3)
+    return sh.astype(np.float32)
+
+
+@dataclass
+class DiffScene:
+    mesh: DiffMesh
+    texture: np.ndarray      # (h, w, 4) float32, level 0
+    chain: np.ndarray        # (N, 4) float32 mip chain
+    levels: int
+    sh: np.ndarray           # (9, 3) float32
+    cb: np.ndarray           # (48,) float32 PerFrameCB
+    width: int
+    height: int
+
+    @property
+    def clip_from_world(self) -> np.ndarray:
+        return self.cb[:16].reshape(4, 4)
+
+
+def c4_scene(width: int = 512, height: int = 512, rings: int = 250, segments: int = 400, tex_size: int = 1024,
+             seed: int = 4) -> DiffScene:
+    """Differentiable-render workload: a displaced sphere of 2*rings*segments triangles (200k at
+    the defaults) with per-vertex colours, UVs and normals, a tex_size^2 RGBA32F texture with its
+    full mip chain and a 9-coefficient RGB SH light, seen from distance 2.8 with a 50 deg FOV."""
+    mesh = uv_sphere(rings, segments, displace=0.08, seed=seed)
+    tex = smooth_texture(tex_size, seed + 1)
+    chain, levels = mip_chain(tex)
+    cb = camera_cb(eye=(2.2, -1.4, 0.9), target=(0.0, 0.0, 0.0), fov_y_deg=50.0, width=width, height=height)
+    return DiffScene(mesh, tex, chain, levels, random_sh(seed + 2), cb, width, height)

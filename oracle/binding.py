@@ -79,6 +79,21 @@ class Triangle(C.Structure):
     ]
 
 
+class DiffScene(C.Structure):
+    _fields_ = [
+        ("positions", C.c_void_p), ("colors", C.c_void_p), ("uvs", C.c_void_p), ("normals", C.c_void_p),
+        ("positions_f", C.c_void_p), ("uvs_f", C.c_void_p), ("indices", C.c_void_p),
+        ("vertex_count", C.c_uint32), ("triangle_count", C.c_uint32),
+        ("texture", C.c_void_p), ("tex_width", C.c_uint32), ("tex_height", C.c_uint32), ("tex_levels", C.c_uint32),
+        ("sh", C.c_void_p), ("clip_from_world", C.c_double * 16), ("clip_from_world_f", C.c_float * 16),
+        ("width", C.c_uint32), ("height", C.c_uint32),
+    ]
+
+
+class DiffGrads(C.Structure):
+    _fields_ = [(n, C.c_void_p) for n in ("positions", "colors", "uvs", "normals", "texture", "sh")]
+
+
 _lib = None
 
 
@@ -112,6 +127,8 @@ def lib() -> C.CDLL:
         L.orc_clip_planes.argtypes = [P(C.c_float)]
         L.orc_clip_polygon.argtypes = [P(C.c_float), C.c_int, P(C.c_float)]
         L.orc_clip_polygon.restype = C.c_int
+        L.orc_diff_forward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p]
+        L.orc_diff_backward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p, P(DiffGrads)]
         _lib = L
     return _lib
 
@@ -258,3 +275,61 @@ def clip_planes() -> np.ndarray:
     out = np.zeros((6, 4), np.float32)
     lib().orc_clip_planes(out.ctypes.data_as(C.POINTER(C.c_float)))
     return out
+
+
+class DiffInputs:
+    """FP64 copies of a differentiable-shading scene (the f32 originals kept for the mip choice).
+
+    positions/colors/normals (V,3), uvs (V,2), indices (T,3), texture_chain (N,4) RGBA mip chain,
+    sh (9,3), clip_from_world (4,4), image width/height."""
+
+    def __init__(self, positions, colors, uvs, normals, indices, texture_chain, tex_width, tex_height, tex_levels,
+                 sh, clip_from_world, width, height):
+        self.positions_f = np.ascontiguousarray(positions, np.float32)
+        self.uvs_f = np.ascontiguousarray(uvs, np.float32)
+        self.positions = self.positions_f.astype(np.float64)
+        self.colors = np.ascontiguousarray(colors, np.float32).astype(np.float64)
+        self.uvs = self.uvs_f.astype(np.float64)
+        self.normals = np.ascontiguousarray(normals, np.float32).astype(np.float64)
+        self.indices = np.ascontiguousarray(indices, np.uint32)
+        self.texture = np.ascontiguousarray(texture_chain, np.float32).astype(np.float64)
+        self.tex_width, self.tex_height, self.tex_levels = tex_width, tex_height, tex_levels
+        self.sh = np.ascontiguousarray(sh, np.float32).astype(np.float64)
+        self.M_f = np.ascontiguousarray(clip_from_world, np.float32).reshape(16)
+        self.M = self.M_f.astype(np.float64)
+        self.width, self.height = width, height
+
+    def struct(self) -> DiffScene:
+        for name in ("positions", "colors", "uvs", "normals", "texture", "sh"):
+            setattr(self, name, np.ascontiguousarray(getattr(self, name), np.float64))
+        return DiffScene(self.positions.ctypes.data, self.colors.ctypes.data, self.uvs.ctypes.data,
+                         self.normals.ctypes.data, self.positions_f.ctypes.data, self.uvs_f.ctypes.data,
+                         self.indices.ctypes.data, len(self.positions), len(self.indices),
+                         self.texture.ctypes.data, self.tex_width, self.tex_height, self.tex_levels,
+                         self.sh.ctypes.data, (C.c_double * 16)(*self.M.tolist()),
+                         (C.c_float * 16)(*self.M_f.tolist()), self.width, self.height)
+
+
+def diff_forward(d: DiffInputs, prim: np.ndarray) -> np.ndarray:
+    prim = np.ascontiguousarray(prim, np.uint32)
+    out = np.zeros((d.height, d.width, 4), np.float64)
+    sc = d.struct()
+    rc = lib().orc_diff_forward(C.byref(sc), prim.ctypes.data, out.ctypes.data)
+    if rc != 0:
+        raise RuntimeError(f"orc_diff_forward failed: {rc}")
+    return out
+
+
+def diff_backward(d: DiffInputs, prim: np.ndarray, grad_rgba: np.ndarray) -> dict:
+    prim = np.ascontiguousarray(prim, np.uint32)
+    g_out = np.ascontiguousarray(grad_rgba, np.float64)
+    grads = {"positions": np.zeros_like(d.positions), "colors": np.zeros_like(d.colors),
+             "uvs": np.zeros_like(d.uvs), "normals": np.zeros_like(d.normals),
+             "texture": np.zeros_like(d.texture), "sh": np.zeros_like(d.sh)}
+    gs = DiffGrads(*[grads[k].ctypes.data for k in ("positions", "colors", "uvs", "normals", "texture", "sh")])
+    sc = d.struct()
+    rc = lib().orc_diff_backward(C.byref(sc), prim.ctypes.data, g_out.ctypes.data, C.byref(gs))
+    if rc != 0:
+        raise RuntimeError(f"orc_diff_backward failed: {rc}")
+    return grads
+

@@ -110,6 +110,28 @@ int32_t orc_cvt_rne(float f);
 void orc_clip_planes(float out[6][4]);
 int orc_clip_polygon(const float *in_vertices /* n x 12 */, int n, float *out_vertices /* 16 x 12 */);
 
+/* ---- differentiable shading, FP64 (malevich_diff.c). The f32 copies of positions, uvs and
+ * the matrix feed only the discrete mip choice, which is computed bit-identically to the GPU. */
+typedef struct orc_diff_scene {
+    const double *positions, *colors, *uvs, *normals; /* [V][3], [V][3], [V][2], [V][3] */
+    const float *positions_f, *uvs_f;                 /* f32 originals of positions and uvs */
+    const uint32_t *indices;                          /* [T][3] */
+    uint32_t vertex_count, triangle_count;
+    const double *texture;                            /* RGBA mip chain, level 0 first */
+    uint32_t tex_width, tex_height, tex_levels;
+    const double *sh;                                 /* [9][3] */
+    double clip_from_world[16];
+    float clip_from_world_f[16];
+    uint32_t width, height;
+} orc_diff_scene;
+
+typedef struct orc_diff_grads {
+    double *positions, *colors, *uvs, *normals, *texture, *sh;
+} orc_diff_grads;
+
+int orc_diff_forward(const orc_diff_scene *s, const uint32_t *prim, double *rgba);
+int orc_diff_backward(const orc_diff_scene *s, const uint32_t *prim, const double *grad_rgba, orc_diff_grads *g);
+
 #ifdef __cplusplus
 }
 #endif
