@@ -203,6 +203,61 @@ def bench_c3(dev, args):
             "scene_gen_s": round(gen_s, 1)}
 
 
+def bench_c4(dev, args):
+    """Differentiable render at 512x512: a 200k-triangle displaced sphere with per-vertex colours,
+    UVs and normals, a 1024x1024 RGBA32F texture with its mip chain and a 9-coefficient SH light.
+    One step = raster (visibility) + shading forward + L2 loss against a target rendered with
+    perturbed parameters (texels +-0.1, vertices +-1e-3) + the backward pass to every parameter."""
+    import torch
+
+    from malevich_amd import diff as D
+    from malevich_amd import scenes as S
+    sc = S.c4_scene()
+    m = sc.mesh
+    cuda = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    base = dict(positions=cuda(m.positions), colors=cuda(m.colors), uvs=cuda(m.uvs), normals=cuda(m.normals),
+                texture=cuda(sc.texture), sh=cuda(sc.sh))
+    idx = cuda(m.indices.astype(np.int32))
+    M = torch.from_numpy(sc.clip_from_world.copy())
+    rng = np.random.default_rng(args.seed + 4)
+    out = {}
+    for det in (False, True):
+        r = D.DiffRenderer(dev, sc.width, sc.height, deterministic=det)
+        with torch.no_grad():
+            tgt = D.render(r, base["positions"] + cuda(rng.uniform(-1e-3, 1e-3, m.positions.shape).astype(np.float32)),
+                           base["colors"], base["uvs"], base["normals"], idx,
+                           base["texture"] + cuda(rng.uniform(-0.1, 0.1, sc.texture.shape).astype(np.float32)),
+                           base["sh"], M)
+        params = {k: v.clone().requires_grad_(True) for k, v in base.items()}
+
+        def step():
+            for v in params.values():
+                v.grad = None
+            img = D.render(r, params["positions"], params["colors"], params["uvs"], params["normals"], idx,
+                           params["texture"], params["sh"], M)
+            loss = ((img - tgt) ** 2).sum()
+            loss.backward()
+            return loss
+
+        for _ in range(max(1, args.warmup)):
+            step()
+        torch.cuda.synchronize()
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record()
+        for _ in range(args.c4_steps):
+            step()
+        e1.record()
+        torch.cuda.synchronize()
+        ms = e0.elapsed_time(e1) / args.c4_steps
+        out["deterministic" if det else "atomic"] = round(ms, 4)
+        r.release()
+    covered = int((tgt[..., 3] > 0).sum().item())
+    return {"workload": f"c4-512x512-differentiable: {m.triangle_count} triangles, 1024x1024 RGBA32F texture with "
+                        f"{sc.levels} mips, SH9 light, L2 loss, gradients to positions/colours/uvs/normals/texels/SH",
+            "fwd_bwd_ms": out["atomic"], "fwd_bwd_ms_deterministic": out["deterministic"],
+            "covered_pixels": covered, "steps": args.c4_steps}
+
+
 def copy_calibration(mib: int):
     """One device-to-device hipMemcpy of `mib` MiB (the runtime runs it as a copy kernel that
     reads and writes exactly that many bytes), so that a counter run can relate FETCH_SIZE /
@@ -240,6 +295,9 @@ def main(argv=None):
     ap.add_argument("--c3-steps", type=int, default=5)
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-only", action="store_true", help="run only the binning-stress leg (for profiling)")
+    ap.add_argument("--c4-steps", type=int, default=20)
+    ap.add_argument("--no-c4", action="store_true")
+    ap.add_argument("--c4-only", action="store_true", help="run only the differentiable-render leg")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (a known-size kernel "
                          "against which HBM byte counters are scaled)")
@@ -251,6 +309,11 @@ def main(argv=None):
 
     from malevich_amd import pipeline as P
 
+    if args.c4_only:
+        dev = P.Device(0)
+        print(json.dumps({"c4_differentiable": bench_c4(dev, args)}))
+        dev.close()
+        return
     if args.c3_only:
         dev = P.Device(0)
         print(json.dumps({"c3_binning_stress": bench_c3(dev, args)}))
@@ -355,6 +418,11 @@ def main(argv=None):
     if not args.no_c3:
         out["c3_binning_stress"] = bench_c3(dev, args)
         print(f"[bench] c3: {out['c3_binning_stress']}", file=sys.stderr, flush=True)
+    if not args.no_c4:
+        c4 = bench_c4(dev, args)
+        out["c4_differentiable"] = c4
+        out["fwd_bwd_ms"] = c4["fwd_bwd_ms"]
+        print(f"[bench] c4: {c4}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, env, args.cpu_budget)
         out["cpu_baseline"]["gpu_over_cpu"] = round(mtri_s / out["cpu_baseline"]["value"], 1)

@@ -125,6 +125,13 @@ def load() -> C.CDLL:
             raise RuntimeError(
                 f"{LIB_PATH} not found: build it with `python -c 'import __graft_entry__ as g; g.build()'` "
                 "(the pipeline has no CPU fallback)")
+        # PyTorch-ROCm ships its own HIP runtime under the same soname (libamdhip64.so.7). A
+        # process that loads this library first and torch afterwards would hold two HIP
+        # runtimes; loading torch first makes the library bind to torch's runtime instead.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         lib = C.CDLL(str(LIB_PATH))
         for name, (res, args) in SIGNATURES.items():
             fn = getattr(lib, name)

@@ -450,12 +450,20 @@ __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const f
     sh_block_sum(a.dsh, sc.sh_part);
 }
 
-// Sum of the per-block SH partials in block order.
-__global__ __launch_bounds__(64) void kd_sh_sum(const double *sh_part, uint32_t nblocks, float *dsh) {
-    if (threadIdx.x >= 27u) return;
+// Sum of the per-block SH partials, one block per coefficient: thread t sums blocks t, t+256, ...
+// in order, then a fixed tree over the 256 partial sums (the same order on every call).
+__global__ __launch_bounds__(256) void kd_sh_sum(const double *sh_part, uint32_t nblocks, float *dsh) {
+    __shared__ double s[256];
+    const uint32_t k = blockIdx.x, t = threadIdx.x;
     double acc = 0.0;
-    for (uint32_t b = 0; b < nblocks; ++b) acc += sh_part[b * 27u + threadIdx.x];
-    dsh[threadIdx.x] = (float)acc;
+    for (uint32_t b = t; b < nblocks; b += 256u) acc += sh_part[b * 27u + k];
+    s[t] = acc;
+    __syncthreads();
+    for (uint32_t h = 128u; h >= 1u; h >>= 1) {
+        if (t < h) s[t] = s[t] + s[t + h];
+        __syncthreads();
+    }
+    if (t == 0u) dsh[k] = (float)s[0];
 }
 
 __global__ __launch_bounds__(256) void kd_pos_finalize(const double *acc, float *out, uint32_t n) {
@@ -573,7 +581,7 @@ hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const D
                                (const float4 *)sc.vpay, (const double *)sc.vposd, 3u * P, p.V, g);
         }
     }
-    if (g.sh) hipLaunchKernelGGL(kd_sh_sum, dim3(1), dim3(64), 0, s, (const double *)sc.sh_part, nb, g.sh);
+    if (g.sh) hipLaunchKernelGGL(kd_sh_sum, dim3(27), dim3(256), 0, s, (const double *)sc.sh_part, nb, g.sh);
     return hipGetLastError();
 }
 

@@ -719,7 +719,7 @@ __device__ __forceinline__ float tile_hiz(bool inside, float d, bool ge) {
 __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
     __shared__ TriVis s_tri[256];
     __shared__ uint32_t s_mask[256];
-    __shared__ __attribute__((aligned(4))) uint16_t s_list[4][256];
+    __shared__ __attribute__((aligned(8))) uint16_t s_list[4][256];
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
@@ -816,8 +816,22 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
                 }
                 if (depth_cull && __ballot(upd)) wext = wave_depth_extreme(inside, d, ge);
             };
-            // two records per iteration: the second LDS read overlaps the first evaluation
+            // four records per iteration: their LDS reads are issued together, so a wave that
+            // walks a long list alone (a heavy bin) waits for LDS once per four records
             uint32_t k = 0;
+            for (; k + 3u < cnt; k += 4u) {
+                const uint2 q = *reinterpret_cast<const uint2 *>(&s_list[w][k]);
+                const uint32_t qa = (uint32_t)__builtin_amdgcn_readfirstlane((int)q.x);
+                const uint32_t qb = (uint32_t)__builtin_amdgcn_readfirstlane((int)q.y);
+                const TriVis T0 = s_tri[qa & 0xFFFFu];
+                const TriVis T1 = s_tri[qa >> 16];
+                const TriVis T2 = s_tri[qb & 0xFFFFu];
+                const TriVis T3 = s_tri[qb >> 16];
+                fragment(T0);
+                fragment(T1);
+                fragment(T2);
+                fragment(T3);
+            }
             for (; k + 1u < cnt; k += 2u) {
                 const uint32_t pair = (uint32_t)__builtin_amdgcn_readfirstlane(
                     (int)*reinterpret_cast<const uint32_t *>(&s_list[w][k]));
