@@ -689,10 +689,11 @@ __device__ __forceinline__ float dpp_move(float v) {
 }
 
 // Reduce over the 64 lanes of a wave: minimum (GE) or maximum (LE) of the in-range depths,
-// ignoring NaN. Register-only (DPP within rows, then the four row results); equal on every lane.
-__device__ __forceinline__ float wave_depth_extreme(bool inside, float d, bool ge) {
-    float v = inside ? d : (ge ? __builtin_inff() : -__builtin_inff());
-    if (ge) {
+// ignoring NaN. Register-only (DPP within rows, then the four row results); uniform result.
+template <bool GE>
+__device__ __forceinline__ float wave_depth_extreme(bool inside, float d) {
+    float v = inside ? d : (GE ? __builtin_inff() : -__builtin_inff());
+    if (GE) {
         v = fminf(v, dpp_move<0xB1>(v));
         v = fminf(v, dpp_move<0x4E>(v));
         v = fminf(v, dpp_move<0x124>(v));
@@ -707,26 +708,134 @@ __device__ __forceinline__ float wave_depth_extreme(bool inside, float d, bool g
     const float r1 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 16));
     const float r2 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 32));
     const float r3 = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 48));
-    return ge ? fminf(fminf(r0, r1), fminf(r2, r3)) : fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
+    return GE ? fminf(fminf(r0, r1), fminf(r2, r3)) : fmaxf(fmaxf(r0, r1), fmaxf(r2, r3));
 }
 
 // write_tile's tile minimum (main.c:589-603), folded from 1.0 (0.0 for the LESS_EQUAL mirror)
-__device__ __forceinline__ float tile_hiz(bool inside, float d, bool ge) {
-    float m = wave_depth_extreme(inside, d, ge);
-    return ge ? min_ref(1.0f, m) : max_ref(0.0f, m);
+template <bool GE>
+__device__ __forceinline__ float tile_hiz(bool inside, float d) {
+    float m = wave_depth_extreme<GE>(inside, d);
+    return GE ? min_ref(1.0f, m) : max_ref(0.0f, m);
 }
 
-__global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
-    __shared__ TriVis s_tri[256];
-    __shared__ uint32_t s_mask[256];
-    __shared__ __attribute__((aligned(8))) uint16_t s_list[4][256];
+__device__ __forceinline__ float uniform_f(float v) {
+    return __int_as_float(__builtin_amdgcn_readfirstlane(__float_as_int(v)));
+}
+__device__ __forceinline__ uint32_t uniform_u(uint32_t v) { return (uint32_t)__builtin_amdgcn_readfirstlane((int)v); }
 
-    if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
-    // The dispatcher deals consecutive blocks to the 8 XCDs in turn, so neighbouring tiles
-    // (whose triangle counts are alike) spread over all XCDs and their load evens out.
-    const uint32_t g = blockIdx.x;
-    const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
+// Per-wave state of one 8x8 bin while it walks its triangles in submission order.
+struct BinState {
+    float d;            // this lane's depth
+    int32_t best;       // this lane's winning assembled triangle (-1: none yet)
+    float hz;           // the bin's hierarchical-Z value after the previous draw (uniform)
+    float wext;         // a bound on the bin's current depths for the conservative cull (uniform)
+    int32_t cur_draw;   // draw of the last triangle seen (uniform)
+    uint32_t n_draws;   // draws seen (uniform)
+    int32_t depth_cull; // the current draw allows the conservative cull (uniform)
+    uint32_t stale;     // updates since wext was last recomputed (uniform)
+};
+
+// Entering a new draw: the bin's hierarchical-Z value becomes the tile minimum of its depths
+// after the previous draw (write_tile, main.c:589-603).
+template <bool GE>
+__device__ __forceinline__ void bin_draw_change(const BatchParams &p, int32_t draw, BinState &st, bool inside) {
+    if (st.cur_draw >= 0) st.hz = uniform_f(tile_hiz<GE>(inside, st.d));
+    st.cur_draw = draw;
+    st.n_draws++;
+    st.depth_cull = (int32_t)uniform_u((uint32_t)p.draws[draw].depth_cull);
+}
+
+// Corner-sampled coverage of one triangle over the bin's 64 pixels and the early-Z key update
+// (main.c:1015-1028, 1083-1168); the triangle has passed hierarchical-Z.
+template <bool GE>
+__device__ __forceinline__ void bin_cover(const TriVis &T, BinState &st, bool inside, int32_t px, int32_t py) {
+    const uint32_t e0 = (uint32_t)T.c[0] + (uint32_t)__mul24(T.a16[0], px) + (uint32_t)__mul24(T.b16[0], py);
+    const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
+    const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
+    const bool cov = inside && (int32_t)(e0 | e1 | e2) > 0;  // main.c:1028
+    if (__ballot(cov) == 0ull) return;
+    const float b1 = (float)((int32_t)e1 >> 8) * T.ooa;  // main.c:1089-1102
+    const float b2 = (float)((int32_t)e2 >> 8) * T.ooa;
+    float z = T.z0 + T.dz1 * b1;  // register 0 is affine (main.c:1118-1154)
+    z = z + T.dz2 * b2;
+    const int32_t id = (int32_t)T.id;
+    const bool pass = GE ? (z > st.d || (z == st.d && id > st.best)) : (z < st.d || (z == st.d && id > st.best));
+    const bool upd = cov && pass;
+    if (upd) {
+        st.d = z;
+        st.best = id;
+    }
+    if (__ballot(upd)) st.stale = 1u;
+}
+
+// Largest value an edge function takes over the bin's pixels (x0..x0+7, y0..y0+7), exact when
+// the draw guarantees no i32 wrap at any pixel (DrawRec::depth_cull).
+__device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c, int32_t x0, int32_t y0) {
+    const int32_t x = a16 > 0 ? x0 + 7 : x0, y = b16 > 0 ? y0 + 7 : y0;
+    return (int32_t)((uint32_t)c + (uint32_t)__mul24(a16, x) + (uint32_t)__mul24(b16, y));
+}
+
+// Walk entries list[first + i * stride], i < n, of this wave's bin in order. Triangle-parallel
+// screening first: lane j takes entry i0 + j and tests, at once for up to 64 entries of the
+// current draw, the reference's hierarchical-Z reject (main.c:1005-1010), the conservative depth
+// cull and an edge that is negative over the whole bin (then no pixel of the bin is covered).
+// The survivors are then rasterised pixel-parallel, in order. Every test only removes triangles
+// that could not have written a pixel, so the result equals the one-by-one walk.
+template <bool GE>
+__device__ __forceinline__ void walk_bin(const BatchParams &p, const TriVis *s_tri, const uint16_t *list,
+                                         uint32_t first, uint32_t stride, uint32_t n, BinState &st, bool inside,
+                                         int32_t px, int32_t py, int32_t x0, int32_t y0, uint32_t lane) {
+    for (uint32_t i0 = 0; i0 < n;) {
+        const uint32_t i = i0 + lane;
+        const bool in = i < n;
+        const TriVis &T = s_tri[list[first + (in ? i : i0) * stride]];
+        const int32_t draw = (int32_t)T.draw;
+        const int32_t d0 = (int32_t)uniform_u((uint32_t)draw);
+        if (d0 != st.cur_draw) bin_draw_change<GE>(p, d0, st, inside);
+        const uint64_t same = __ballot(in && draw == d0);
+        const uint32_t run = (~same == 0ull) ? 64u : (uint32_t)__builtin_ctzll(~same);
+        if (st.depth_cull && st.stale) {
+            st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
+            st.stale = 0u;
+        }
+        bool live = lane < run;
+        const float hzd = T.hiz_depth;
+        live = live && !(GE ? (hzd < st.hz) : (hzd > st.hz));
+        if (st.depth_cull) {
+            const float cb = T.cull_bound;
+            live = live && !(GE ? (cb < st.wext) : (cb > st.wext));
+            live = live && edge_max(T.a16[0], T.b16[0], T.c[0], x0, y0) >= 0 &&
+                   edge_max(T.a16[1], T.b16[1], T.c[1], x0, y0) >= 0 &&
+                   edge_max(T.a16[2], T.b16[2], T.c[2], x0, y0) >= 0;
+        }
+        uint64_t S = __ballot(live);
+        while (S) {
+            const uint32_t jj = (uint32_t)__builtin_ctzll(S);
+            S &= S - 1ull;
+            const TriVis Tj = s_tri[uniform_u(list[first + (i0 + jj) * stride])];
+            bin_cover<GE>(Tj, st, inside, px, py);
+        }
+        i0 += run;
+    }
+}
+
+constexpr uint32_t kHeavyDraws = 2;
+
+__device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g) {
+    const uint32_t a = p.tile_start[g], e = p.tile_cnt[g];
+    if (e - a <= kHeavyTile) return false;
+    const uint32_t d0 = upper_index(p.n_draws, sorted_vals[a], [&](uint32_t i) { return p.asm_base[i]; });
+    const uint32_t d1 = upper_index(p.n_draws, sorted_vals[e - 1u], [&](uint32_t i) { return p.asm_base[i]; });
+    return d1 - d0 < kHeavyDraws;
+}
+
+template <bool GE>
+__device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t vi,
+                                          TriVis *s_tri, uint32_t *s_mask, uint16_t (*s_list)[256],
+                                          uint32_t *s_active) {
+    // the wave index is uniform per wave; saying so keeps everything derived from it (the bin,
+    // its Hi-Z value, the walk's branches) in scalar registers
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
     const ViewRec &V = p.views[vi];
     const uint32_t tile = g - V.tile_base;
     const int32_t tx = (int32_t)(tile % (uint32_t)V.tiles_x), ty = (int32_t)(tile / (uint32_t)V.tiles_x);
@@ -735,7 +844,6 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
     const bool sub_valid = sbx < V.bins_x && sby < V.bins_y;
     const bool inside = px < V.W && py < V.H;
     const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
-    const bool ge = V.depth_func == DEPTH_GE;
 
     const uint32_t n_asm = min(p.ctr->n_assembled, p.tri_cap);
     const bool ovf = (p.ctr->overflow & OVF_PAIRS) != 0u;
@@ -743,16 +851,19 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
     if (!ovf) {
         lstart = p.tile_start[g];
         lcount = p.tile_cnt[g] - lstart;
+        if (is_heavy(p, sorted_vals, g)) return;  // kb_tile_heavy has drawn it
     }
     if (lcount == 0u) return;
 
-    float d = inside ? V.depth[pix] : 0.0f;
-    int32_t best = -1;
-    float hz = sub_valid ? V.hiz[sby * V.bins_x + sbx] : 0.0f;
-    int32_t cur_draw = -1;
-    uint32_t n_draws_here = 0u;
-    float wext = wave_depth_extreme(inside, d, ge);
-    int32_t depth_cull = 0;
+    BinState st;
+    st.d = inside ? V.depth[pix] : 0.0f;
+    st.best = -1;
+    st.hz = uniform_f(sub_valid ? V.hiz[sby * V.bins_x + sbx] : 0.0f);
+    st.cur_draw = -1;
+    st.n_draws = 0u;
+    st.depth_cull = 0;
+    st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
+    st.stale = 0u;
 
     for (uint32_t base = 0; base < lcount; base += 256u) {
         const uint32_t n = min(256u, lcount - base);
@@ -782,75 +893,13 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
             if (b) s_list[w][cnt + mbcnt(bal)] = (uint16_t)i;
             cnt += (uint32_t)__popcll(bal);
         }
-        if (sub_valid && cnt) {
-            // one (triangle, 8x8 bin) pair: hierarchical-Z, coverage, early-Z
-            auto fragment = [&](const TriVis &T) {
-                const int32_t tdraw = __builtin_amdgcn_readfirstlane((int)T.draw);
-                if (tdraw != cur_draw) {
-                    // the bin's tile minimum after the previous draw (write_tile, main.c:589-603)
-                    if (cur_draw >= 0) hz = tile_hiz(inside, d, ge);
-                    cur_draw = tdraw;
-                    n_draws_here++;
-                    depth_cull = p.draws[tdraw].depth_cull;
-                }
-                // reference hierarchical-Z (main.c:1005-1010), per (triangle, 8x8 bin)
-                bool rej = ge ? (T.hiz_depth < hz) : (T.hiz_depth > hz);
-                // conservative cull: no fragment of T can pass the bin's current depths
-                if (depth_cull) rej = rej || (ge ? (T.cull_bound < wext) : (T.cull_bound > wext));
-                if (__builtin_amdgcn_readfirstlane((int)rej)) return;
-                const uint32_t e0 = (uint32_t)T.c[0] + (uint32_t)__mul24(T.a16[0], px) + (uint32_t)__mul24(T.b16[0], py);
-                const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
-                const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
-                const bool cov = inside && (int32_t)(e0 | e1 | e2) > 0;  // main.c:1028
-                if (__ballot(cov) == 0ull) return;
-                const float b1 = (float)((int32_t)e1 >> 8) * T.ooa;  // main.c:1089-1102
-                const float b2 = (float)((int32_t)e2 >> 8) * T.ooa;
-                float z = T.z0 + T.dz1 * b1;  // register 0 is affine (main.c:1118-1154)
-                z = z + T.dz2 * b2;
-                const int32_t id = (int32_t)T.id;
-                const bool pass = ge ? (z > d || (z == d && id > best)) : (z < d || (z == d && id > best));
-                const bool upd = cov && pass;
-                if (upd) {
-                    d = z;
-                    best = id;
-                }
-                if (depth_cull && __ballot(upd)) wext = wave_depth_extreme(inside, d, ge);
-            };
-            // four records per iteration: their LDS reads are issued together, so a wave that
-            // walks a long list alone (a heavy bin) waits for LDS once per four records
-            uint32_t k = 0;
-            for (; k + 3u < cnt; k += 4u) {
-                const uint2 q = *reinterpret_cast<const uint2 *>(&s_list[w][k]);
-                const uint32_t qa = (uint32_t)__builtin_amdgcn_readfirstlane((int)q.x);
-                const uint32_t qb = (uint32_t)__builtin_amdgcn_readfirstlane((int)q.y);
-                const TriVis T0 = s_tri[qa & 0xFFFFu];
-                const TriVis T1 = s_tri[qa >> 16];
-                const TriVis T2 = s_tri[qb & 0xFFFFu];
-                const TriVis T3 = s_tri[qb >> 16];
-                fragment(T0);
-                fragment(T1);
-                fragment(T2);
-                fragment(T3);
-            }
-            for (; k + 1u < cnt; k += 2u) {
-                const uint32_t pair = (uint32_t)__builtin_amdgcn_readfirstlane(
-                    (int)*reinterpret_cast<const uint32_t *>(&s_list[w][k]));
-                const TriVis T0 = s_tri[pair & 0xFFFFu];
-                const TriVis T1 = s_tri[pair >> 16];
-                fragment(T0);
-                fragment(T1);
-            }
-            if (k < cnt) {
-                const uint32_t i = (uint32_t)__builtin_amdgcn_readfirstlane((int)s_list[w][k]);
-                fragment(s_tri[i]);
-            }
-        }
+        if (sub_valid && cnt) walk_bin<GE>(p, s_tri, s_list[w], 0u, 1u, cnt, st, inside, px, py, sbx * kBin, sby * kBin, lane);
         __syncthreads();
     }
 
-    if (best >= 0) {
-        const TriVis T = p.tvis[best];
-        const TriShade S = p.tshade[best];
+    if (st.best >= 0) {
+        const TriVis T = p.tvis[st.best];
+        const TriShade S = p.tshade[st.best];
         const DrawRec &D = p.draws[T.draw];
         const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
         const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
@@ -876,7 +925,7 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
         float cr, cg, cb;
         pixel_shader(D.ps, D.ps_tex, r1, r2, cr, cg, cb);
         V.color[pix] = encode_output(cr, cg, cb);
-        V.depth[pix] = d;
+        V.depth[pix] = st.d;
         if (V.vis_tri) {
             V.vis_tri[pix] = T.local;
             V.vis_draw[pix] = D.draw_tag;
@@ -884,18 +933,223 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
         if (V.vis_prim) V.vis_prim[pix] = S.prim;
     }
 
-    __shared__ uint32_t s_active[4];
     if (lane == 0u) s_active[w] = 0u;
-    if (sub_valid && cur_draw >= 0) {
-        const float m = tile_hiz(inside, d, ge);
+    if (sub_valid && st.cur_draw >= 0) {
+        const float m = tile_hiz<GE>(inside, st.d);
         if (lane == 0u) {
             V.hiz[sby * V.bins_x + sbx] = m;
-            s_active[w] = n_draws_here;
+            s_active[w] = st.n_draws;
         }
     }
+}
+
+// Tiles whose sorted list is longer than kHeavyTile and spans at most kHeavyDraws draws (long
+// single-draw runs a single wave would walk for long, e.g. where many small triangles of one
+// mesh converge), appended in any order. Lists of many short runs stay with kb_tile: there the
+// 16-wave merges at every draw boundary would cost more than they save.
+__global__ __launch_bounds__(256) void kb_heavy_tiles(BatchParams p, const uint32_t *sorted_vals) {
+    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES | OVF_CLIPVERTS)) return;
+    for (uint32_t g = blockIdx.x * 256u + threadIdx.x; g < p.total_tiles; g += gridDim.x * 256u)
+        if (is_heavy(p, sorted_vals, g)) p.heavy[atomicAdd(&p.ctr->n_heavy, 1u)] = g;
+}
+
+// One 8x8 bin of a heavy tile per block iteration, walked by 16 waves. Within a run of
+// triangles of one draw the per-pixel result is the maximum of (depth, submission order) keys,
+// which does not depend on the order the run is visited in: wave q takes the run's entries
+// q, q+16, ..., and at the end of the run the 16 waves' keys are merged in LDS. Every wave
+// therefore enters the next run with the exact depths the reference would have after the
+// previous draw, refreshes the bin's Hi-Z value from them when the draw changes (main.c:589-603)
+// and rejects exactly what the single-wave walk rejects.
+constexpr uint32_t kHeavyWaves = 16;
+
+template <bool GE>
+__device__ __forceinline__ void heavy_bin(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t bin,
+                                          TriVis *s_tri, uint16_t *s_list, uint32_t *s_runs, uint32_t *s_cnt,
+                                          float (*s_md)[64], int32_t (*s_mb)[64]) {
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
+    const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
+    const ViewRec &V = p.views[vi];
+    const uint32_t tile = g - V.tile_base;
+    const int32_t tx = (int32_t)(tile % (uint32_t)V.tiles_x), ty = (int32_t)(tile / (uint32_t)V.tiles_x);
+    const int32_t sbx = 2 * tx + (int32_t)(bin & 1u), sby = 2 * ty + (int32_t)(bin >> 1);
+    const int32_t px = sbx * kBin + (int32_t)(lane & 7u), py = sby * kBin + (int32_t)(lane >> 3);
+    const bool sub_valid = sbx < V.bins_x && sby < V.bins_y;
+    const bool inside = px < V.W && py < V.H;
+    const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
+    if (!sub_valid) return;  // uniform over the block
+    const uint32_t lstart = p.tile_start[g], lcount = p.tile_cnt[g] - lstart;
+
+    BinState st;
+    st.d = inside ? V.depth[pix] : 0.0f;
+    st.best = -1;
+    st.hz = uniform_f(V.hiz[sby * V.bins_x + sbx]);
+    st.cur_draw = -1;
+    st.n_draws = 0u;
+    st.depth_cull = 0;
+    st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
+    st.stale = 0u;
+    int32_t run_draw = -1;
+    uint32_t n_draws = 0u;
+
+    for (uint32_t base = 0; base < lcount; base += 256u) {
+        const uint32_t n = min(256u, lcount - base);
+        // stage the chunk and compact the entries touching this bin, in order (waves 0..3)
+        bool member = false;
+        if (tid < 256u) {
+            if (tid < n) {
+                const TriVis tv = p.tvis[sorted_vals[lstart + base + tid]];
+                s_tri[tid] = tv;
+                int32_t bx0, by0, bx1, by1;
+                rect_of(tv, bx0, by0, bx1, by1);
+                member = sbx >= bx0 && sbx <= bx1 && sby >= by0 && sby <= by1;
+            }
+            const uint64_t bal = __ballot(member);
+            if (lane == 0u) s_cnt[w] = (uint32_t)__popcll(bal);
+        }
+        __syncthreads();
+        if (tid < 256u) {
+            uint32_t off = 0u;
+            for (uint32_t q = 0; q < w; ++q) off += s_cnt[q];
+            const uint64_t bal = __ballot(member);
+            if (member) s_list[off + mbcnt(bal)] = (uint16_t)tid;
+        }
+        __syncthreads();
+        const uint32_t cnt = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
+        // run starts: entries whose draw differs from the previous entry's (waves 0..3)
+        bool start = false;
+        if (tid < 256u) {
+            if (tid < cnt) start = tid == 0u || s_tri[s_list[tid]].draw != s_tri[s_list[tid - 1u]].draw;
+            const uint64_t bal = __ballot(start);
+            if (lane == 0u) s_cnt[4u + w] = (uint32_t)__popcll(bal);
+        }
+        __syncthreads();
+        if (tid < 256u) {
+            uint32_t off = 0u;
+            for (uint32_t q = 0; q < w; ++q) off += s_cnt[4u + q];
+            const uint64_t bal = __ballot(start);
+            if (start) s_runs[off + mbcnt(bal)] = tid;
+        }
+        __syncthreads();
+        const uint32_t nruns = s_cnt[4] + s_cnt[5] + s_cnt[6] + s_cnt[7];
+        for (uint32_t r = 0; r < nruns; ++r) {
+            const uint32_t a = s_runs[r], e = (r + 1u < nruns) ? s_runs[r + 1u] : cnt;
+            const int32_t dr = (int32_t)s_tri[s_list[a]].draw;
+            if (dr != run_draw) {
+                run_draw = dr;
+                n_draws++;
+            }
+            if (a + w < e)
+                walk_bin<GE>(p, s_tri, s_list, a + w, kHeavyWaves, (e - a - w + kHeavyWaves - 1u) / kHeavyWaves, st,
+                             inside, px, py, sbx * kBin, sby * kBin, lane);
+            // merge the waves' keys: the greater (depth, order) key wins, as in bin_fragment
+            s_md[w][lane] = st.d;
+            s_mb[w][lane] = st.best;
+            __syncthreads();
+            float d = s_md[0][lane];
+            int32_t bb = s_mb[0][lane];
+            for (uint32_t q = 1; q < kHeavyWaves; ++q) {
+                const float dq = s_md[q][lane];
+                const int32_t bq = s_mb[q][lane];
+                const bool better = GE ? (dq > d || (dq == d && bq > bb)) : (dq < d || (dq == d && bq > bb));
+                if (better) {
+                    d = dq;
+                    bb = bq;
+                }
+            }
+            st.d = d;
+            st.best = bb;
+            st.stale = 1u;
+            __syncthreads();
+        }
+    }
+
+    if (w != 0u) return;  // wave 0 writes the bin
+    if (st.best >= 0) {
+        const TriVis T = p.tvis[st.best];
+        const TriShade S = p.tshade[st.best];
+        const DrawRec &D = p.draws[T.draw];
+        const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
+        const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
+        const float bx = (float)((int32_t)e1 >> 8) * T.ooa;
+        const float by = (float)((int32_t)e2 >> 8) * T.ooa;
+        float den = 1.0f - (bx + by);  // main.c:1106-1115
+        den = den * S.rw[0];
+        den = den + bx * S.rw[1];
+        den = den + by * S.rw[2];
+        den = 1.0f / den;
+        const float pbx = (bx * S.rw[1]) * den;
+        const float pby = (by * S.rw[2]) * den;
+        const float4 a0 = p.vattr[2u * S.row[0]], b0 = p.vattr[2u * S.row[0] + 1u];
+        const float4 a1 = p.vattr[2u * S.row[1]], b1 = p.vattr[2u * S.row[1] + 1u];
+        const float4 a2 = p.vattr[2u * S.row[2]], b2 = p.vattr[2u * S.row[2] + 1u];
+#define MLV_INTERP(c0, c1, c2) (((c0) + ((c1) - (c0)) * pbx) + ((c2) - (c0)) * pby)
+        const float4 r1 = make_float4(MLV_INTERP(a0.x, a1.x, a2.x), MLV_INTERP(a0.y, a1.y, a2.y),
+                                      MLV_INTERP(a0.z, a1.z, a2.z), MLV_INTERP(a0.w, a1.w, a2.w));
+        const float4 r2 = make_float4(MLV_INTERP(b0.x, b1.x, b2.x), MLV_INTERP(b0.y, b1.y, b2.y),
+                                      MLV_INTERP(b0.z, b1.z, b2.z), MLV_INTERP(b0.w, b1.w, b2.w));
+#undef MLV_INTERP
+        float cr, cg, cb;
+        pixel_shader(D.ps, D.ps_tex, r1, r2, cr, cg, cb);
+        V.color[pix] = encode_output(cr, cg, cb);
+        V.depth[pix] = st.d;
+        if (V.vis_tri) {
+            V.vis_tri[pix] = T.local;
+            V.vis_draw[pix] = D.draw_tag;
+        }
+        if (V.vis_prim) V.vis_prim[pix] = S.prim;
+    }
+    if (run_draw >= 0) {
+        const float m = tile_hiz<GE>(inside, st.d);
+        if (lane == 0u) {
+            V.hiz[sby * V.bins_x + sbx] = m;
+            atomicAdd(&p.stats->active8, (unsigned long long)n_draws);
+        }
+    }
+}
+
+__global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint32_t *sorted_vals) {
+    __shared__ TriVis s_tri[256];
+    __shared__ uint16_t s_list[256];
+    __shared__ uint32_t s_runs[256];
+    __shared__ uint32_t s_cnt[8];
+    __shared__ float s_md[kHeavyWaves][64];
+    __shared__ int32_t s_mb[kHeavyWaves][64];
+    if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES | OVF_CLIPVERTS)) return;
+    const uint32_t items = 4u * p.ctr->n_heavy;
+    for (uint32_t h = blockIdx.x; h < items; h += gridDim.x) {
+        const uint32_t g = p.heavy[h >> 2], bin = h & 3u;
+        const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
+        if (p.views[vi].depth_func == DEPTH_GE)
+            heavy_bin<true>(p, sorted_vals, g, bin, s_tri, s_list, s_runs, s_cnt, s_md, s_mb);
+        else
+            heavy_bin<false>(p, sorted_vals, g, bin, s_tri, s_list, s_runs, s_cnt, s_md, s_mb);
+        __syncthreads();  // LDS is reused by the next item
+    }
+}
+
+// One block per 16x16 tile of every view, one wave per 8x8 bin. The block stages up to 256
+// records of the tile's sorted list in LDS; each wave compacts the ones touching its bin and
+// walks them in submission order with its pixels' depths in registers; afterwards each pixel's
+// winner is shaded once and colour, depth, visibility and the bin's Hi-Z are written.
+__global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
+    __shared__ TriVis s_tri[256];
+    __shared__ uint32_t s_mask[256];
+    __shared__ __attribute__((aligned(8))) uint16_t s_list[4][256];
+    __shared__ uint32_t s_active[4];
+
+    if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
+    // The dispatcher deals consecutive blocks to the 8 XCDs in turn, so neighbouring tiles
+    // (whose triangle counts are alike) spread over all XCDs and their load evens out.
+    const uint32_t g = blockIdx.x;
+    const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
+    if (threadIdx.x < 4u) s_active[threadIdx.x] = 0u;
+    if (p.views[vi].depth_func == DEPTH_GE)
+        tile_body<true>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
+    else
+        tile_body<false>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
     __syncthreads();
     // (bin, draw) activity of this tile for the stats; this tile's start is no longer needed
-    if (tid == 0u) p.tile_start[g] = s_active[0] + s_active[1] + s_active[2] + s_active[3];
+    if (threadIdx.x == 0u) p.tile_start[g] = s_active[0] + s_active[1] + s_active[2] + s_active[3];
 }
 
 // Sum of the per-tile activity counts into the stats (one atomic per block).
@@ -976,6 +1230,10 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
     case 3: {
         if (!p.n_tblocks || !p.total_tiles) break;
         const uint32_t *sorted = (radix_passes & 1u) ? p.vals1 : p.vals0;
+        hipLaunchKernelGGL(kb_heavy_tiles, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 256u)), dim3(256), 0,
+                           s, p, sorted);
+        hipLaunchKernelGGL(kb_tile_heavy, dim3(std::min<uint32_t>(4u * p.total_tiles, 1024u)), dim3(1024), 0, s, p,
+                           sorted);
         hipLaunchKernelGGL(kb_tile, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
         hipLaunchKernelGGL(kb_active_sum, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 64u)), dim3(256), 0,
                            s, p);

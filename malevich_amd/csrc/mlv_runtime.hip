@@ -110,7 +110,7 @@ struct mlv_context {
     uint64_t pending_tris = 0, pending_verts = 0;
     // arenas
     Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, tvis, tshade, tbin;
-    Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist;
+    Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist, heavy;
     // differentiable shading scratch
     Arena d_shpart, d_tkey, d_tval, d_tkey2, d_tval2, d_tpay, d_vkey, d_vval, d_vkey2, d_vval2, d_vpay, d_part, d_hist;
     Arena d_vposd, d_posacc;
@@ -309,6 +309,7 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->vals1, pair_cap * 4));
     RC_OR_RETURN(ensure(ctx, ctx->tile_cnt, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->tile_start, (size_t)(total_tiles + 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->heavy, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_part_elems(scan_n) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->radix_hist, radix_hist_elems(pair_cap) * 4));
 
@@ -377,6 +378,7 @@ static int flush(mlv_context *ctx) {
     p.pair_cap = (uint32_t)pair_cap;
     p.tile_cnt = (uint32_t *)ctx->tile_cnt.p;
     p.tile_start = (uint32_t *)ctx->tile_start.p;
+    p.heavy = (uint32_t *)ctx->heavy.p;
     p.ctr = ctx->ctr;
     p.stats = ctx->dstats;
     ScanScratch sc{ctx->scan_part.p, ctx->scan_part.cap / 8, (uint32_t *)ctx->radix_hist.p, ctx->radix_hist.cap / 4};
@@ -479,7 +481,7 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->blocksum,  &ctx->blockoff,  &ctx->asm_base, &ctx->clip_list, &ctx->clip_off,
                     &ctx->tvis,      &ctx->tshade,    &ctx->tbin,    &ctx->pair_cnt, &ctx->pair_off,  &ctx->keys0,
                     &ctx->vals0,     &ctx->keys1,     &ctx->vals1,    &ctx->tile_cnt,  &ctx->tile_start,
-                    &ctx->scan_part, &ctx->radix_hist, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
+                    &ctx->scan_part, &ctx->radix_hist, &ctx->heavy, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
                     &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
                     &ctx->d_vposd,   &ctx->d_posacc};
@@ -922,9 +924,11 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
     r.vs = ctx->vs;
     r.ps = ctx->ps;
     r.depth_func = ctx->depth_func;
-    // the conservative in-draw depth cull relies on exact edge values: |E| < 2^31 at every pixel
+    // the conservative in-draw depth cull and the per-bin edge screening rely on exact edge
+    // values: |E| < 2^31 at every pixel of every bin
+    // (bins reach up to 7 pixels past the right and bottom edges)
     r.depth_cull = (ctx->vp.top_left_x == 0.0f && ctx->vp.top_left_y == 0.0f &&
-                    2.0 * (16.0 * W + 2.0) * (16.0 * H + 2.0) < 2147483648.0 * 0.99)
+                    2.0 * (16.0 * (W + 8) + 2.0) * (16.0 * (H + 8) + 2.0) < 2147483648.0 * 0.99)
                        ? 1
                        : 0;
     r.draw_tag = draw_tag;

@@ -191,3 +191,18 @@ def test_bin_list_overflow_slow_path_is_exact():
     r = subprocess.run([sys.executable, "-c", _OVERFLOW_CHILD, root], env=env, capture_output=True, text=True,
                        timeout=180)
     assert r.returncode == 0 and "OVERFLOW_PATH_OK" in r.stdout, r.stdout + r.stderr
+
+
+@pytest.mark.parametrize("depth_func", [0, 1])
+def test_heavy_tiles_multi_draw(device, oracle, depth_func):
+    """Tiles with thousands of entries from several draws (walked by 16 waves per bin, merged
+    at draw boundaries) equal the single-threaded reference order, Hi-Z refreshes included."""
+    W, H = 64, 48
+    draws = []
+    for k in range(8):
+        st = S.binning_stress(3000, 3, W, H, seed=100 + k, big_every=900)
+        draws.append(DrawSpec(st.indices, st.vertices, "passthrough", "passthrough"))
+    clear_depth = 0.0 if depth_func == 0 else 1.0
+    g, o = both(device, oracle, W, H, draws, depth=clear_depth, depth_func=depth_func)
+    assert_frames_match(g, o, color_tol=0)
+    assert len(np.unique(g.vis_draw)) >= 6
