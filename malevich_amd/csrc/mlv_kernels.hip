@@ -819,7 +819,7 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const TriVis *s_t
     }
 }
 
-constexpr uint32_t kHeavyDraws = 2;
+constexpr uint32_t kHeavyDraws = 1;
 
 __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g) {
     const uint32_t a = p.tile_start[g], e = p.tile_cnt[g];
@@ -943,31 +943,31 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
     }
 }
 
-// Tiles whose sorted list is longer than kHeavyTile and spans at most kHeavyDraws draws (long
-// single-draw runs a single wave would walk for long, e.g. where many small triangles of one
-// mesh converge), appended in any order. Lists of many short runs stay with kb_tile: there the
-// 16-wave merges at every draw boundary would cost more than they save.
+// A heavy tile: a sorted list longer than kHeavyTile whose triangles all belong to one draw
+// (kHeavyDraws), e.g. where many small triangles of one mesh converge or a stress draw covers
+// the screen. Lists that span draws stay with kb_tile, whose single wave per bin keeps the
+// Hi-Z refresh between draws in order.
 __global__ __launch_bounds__(256) void kb_heavy_tiles(BatchParams p, const uint32_t *sorted_vals) {
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES | OVF_CLIPVERTS)) return;
     for (uint32_t g = blockIdx.x * 256u + threadIdx.x; g < p.total_tiles; g += gridDim.x * 256u)
         if (is_heavy(p, sorted_vals, g)) p.heavy[atomicAdd(&p.ctr->n_heavy, 1u)] = g;
 }
 
-// One 8x8 bin of a heavy tile per block iteration, walked by 16 waves. Within a run of
-// triangles of one draw the per-pixel result is the maximum of (depth, submission order) keys,
-// which does not depend on the order the run is visited in: wave q takes the run's entries
-// q, q+16, ..., and at the end of the run the 16 waves' keys are merged in LDS. Every wave
-// therefore enters the next run with the exact depths the reference would have after the
-// previous draw, refreshes the bin's Hi-Z value from them when the draw changes (main.c:589-603)
-// and rejects exactly what the single-wave walk rejects.
+// A heavy tile per block iteration, its four 8x8 bins walked by four waves each. Heavy tiles
+// hold one draw, so within a bin the result is the maximum of (depth, submission order) keys
+// over the triangles that pass, which does not depend on the order they are visited in, and the
+// hierarchical-Z value stays the one from before the flush (it changes only between draws,
+// main.c:589-603). Wave q of a bin takes the bin's entries q, q+4, ...; at the end the four
+// waves' keys are merged in LDS and the first wave of the bin writes it.
 constexpr uint32_t kHeavyWaves = 16;
+constexpr uint32_t kWavesPerBin = kHeavyWaves / 4;
 
 template <bool GE>
-__device__ __forceinline__ void heavy_bin(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t bin,
-                                          TriVis *s_tri, uint16_t *s_list, uint32_t *s_runs, uint32_t *s_cnt,
-                                          float (*s_md)[64], int32_t (*s_mb)[64]) {
+__device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g,
+                                           uint32_t vi, TriVis *s_tri, uint32_t *s_mask, uint16_t (*s_list)[256],
+                                           uint32_t *s_cnt, float (*s_md)[64], int32_t (*s_mb)[64]) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
-    const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
+    const uint32_t bin = w / kWavesPerBin, q = w % kWavesPerBin;
     const ViewRec &V = p.views[vi];
     const uint32_t tile = g - V.tile_base;
     const int32_t tx = (int32_t)(tile % (uint32_t)V.tiles_x), ty = (int32_t)(tile / (uint32_t)V.tiles_x);
@@ -976,94 +976,69 @@ __device__ __forceinline__ void heavy_bin(const BatchParams &p, const uint32_t *
     const bool sub_valid = sbx < V.bins_x && sby < V.bins_y;
     const bool inside = px < V.W && py < V.H;
     const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
-    if (!sub_valid) return;  // uniform over the block
     const uint32_t lstart = p.tile_start[g], lcount = p.tile_cnt[g] - lstart;
 
     BinState st;
     st.d = inside ? V.depth[pix] : 0.0f;
     st.best = -1;
-    st.hz = uniform_f(V.hiz[sby * V.bins_x + sbx]);
+    st.hz = uniform_f(sub_valid ? V.hiz[sby * V.bins_x + sbx] : 0.0f);
     st.cur_draw = -1;
     st.n_draws = 0u;
     st.depth_cull = 0;
     st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
     st.stale = 0u;
-    int32_t run_draw = -1;
-    uint32_t n_draws = 0u;
+    bool seen = false;
 
     for (uint32_t base = 0; base < lcount; base += 256u) {
         const uint32_t n = min(256u, lcount - base);
-        // stage the chunk and compact the entries touching this bin, in order (waves 0..3)
-        bool member = false;
-        if (tid < 256u) {
-            if (tid < n) {
-                const TriVis tv = p.tvis[sorted_vals[lstart + base + tid]];
-                s_tri[tid] = tv;
-                int32_t bx0, by0, bx1, by1;
-                rect_of(tv, bx0, by0, bx1, by1);
-                member = sbx >= bx0 && sbx <= bx1 && sby >= by0 && sby <= by1;
+        if (tid < n) {
+            const TriVis tv = p.tvis[sorted_vals[lstart + base + tid]];
+            s_tri[tid] = tv;
+            int32_t bx0, by0, bx1, by1;
+            rect_of(tv, bx0, by0, bx1, by1);
+            uint32_t m = 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) {
+                int32_t gx = 2 * tx + (k & 1), gy = 2 * ty + (k >> 1);
+                if (gx >= bx0 && gx <= bx1 && gy >= by0 && gy <= by1) m |= 1u << k;
             }
-            const uint64_t bal = __ballot(member);
-            if (lane == 0u) s_cnt[w] = (uint32_t)__popcll(bal);
+            s_mask[tid] = m;
         }
         __syncthreads();
-        if (tid < 256u) {
-            uint32_t off = 0u;
-            for (uint32_t q = 0; q < w; ++q) off += s_cnt[q];
-            const uint64_t bal = __ballot(member);
-            if (member) s_list[off + mbcnt(bal)] = (uint16_t)tid;
-        }
-        __syncthreads();
-        const uint32_t cnt = s_cnt[0] + s_cnt[1] + s_cnt[2] + s_cnt[3];
-        // run starts: entries whose draw differs from the previous entry's (waves 0..3)
-        bool start = false;
-        if (tid < 256u) {
-            if (tid < cnt) start = tid == 0u || s_tri[s_list[tid]].draw != s_tri[s_list[tid - 1u]].draw;
-            const uint64_t bal = __ballot(start);
-            if (lane == 0u) s_cnt[4u + w] = (uint32_t)__popcll(bal);
-        }
-        __syncthreads();
-        if (tid < 256u) {
-            uint32_t off = 0u;
-            for (uint32_t q = 0; q < w; ++q) off += s_cnt[4u + q];
-            const uint64_t bal = __ballot(start);
-            if (start) s_runs[off + mbcnt(bal)] = tid;
-        }
-        __syncthreads();
-        const uint32_t nruns = s_cnt[4] + s_cnt[5] + s_cnt[6] + s_cnt[7];
-        for (uint32_t r = 0; r < nruns; ++r) {
-            const uint32_t a = s_runs[r], e = (r + 1u < nruns) ? s_runs[r + 1u] : cnt;
-            const int32_t dr = (int32_t)s_tri[s_list[a]].draw;
-            if (dr != run_draw) {
-                run_draw = dr;
-                n_draws++;
+        if (w < 4u) {  // wave b builds bin b's list, in order
+            uint32_t cnt = 0u;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const uint32_t i = (uint32_t)c * 64u + lane;
+                const bool b = i < n && ((s_mask[i] >> w) & 1u);
+                const uint64_t bal = __ballot(b);
+                if (b) s_list[w][cnt + mbcnt(bal)] = (uint16_t)i;
+                cnt += (uint32_t)__popcll(bal);
             }
-            if (a + w < e)
-                walk_bin<GE>(p, s_tri, s_list, a + w, kHeavyWaves, (e - a - w + kHeavyWaves - 1u) / kHeavyWaves, st,
-                             inside, px, py, sbx * kBin, sby * kBin, lane);
-            // merge the waves' keys: the greater (depth, order) key wins, as in bin_fragment
-            s_md[w][lane] = st.d;
-            s_mb[w][lane] = st.best;
-            __syncthreads();
-            float d = s_md[0][lane];
-            int32_t bb = s_mb[0][lane];
-            for (uint32_t q = 1; q < kHeavyWaves; ++q) {
-                const float dq = s_md[q][lane];
-                const int32_t bq = s_mb[q][lane];
-                const bool better = GE ? (dq > d || (dq == d && bq > bb)) : (dq < d || (dq == d && bq > bb));
-                if (better) {
-                    d = dq;
-                    bb = bq;
-                }
-            }
-            st.d = d;
-            st.best = bb;
-            st.stale = 1u;
-            __syncthreads();
+            if (lane == 0u) s_cnt[w] = cnt;
+        }
+        __syncthreads();
+        const uint32_t cnt = s_cnt[bin];
+        if (cnt) seen = true;
+        if (sub_valid && q < cnt)
+            walk_bin<GE>(p, s_tri, s_list[bin], q, kWavesPerBin, (cnt - q + kWavesPerBin - 1u) / kWavesPerBin, st,
+                         inside, px, py, sbx * kBin, sby * kBin, lane);
+        __syncthreads();
+    }
+    // merge the bin's four keys: the greater (depth, order) key wins, as in bin_cover
+    s_md[w][lane] = st.d;
+    s_mb[w][lane] = st.best;
+    __syncthreads();
+    if (q != 0u || !sub_valid) return;
+    for (uint32_t k = 1; k < kWavesPerBin; ++k) {
+        const float dq = s_md[w + k][lane];
+        const int32_t bq = s_mb[w + k][lane];
+        const bool better = GE ? (dq > st.d || (dq == st.d && bq > st.best)) : (dq < st.d || (dq == st.d && bq > st.best));
+        if (better) {
+            st.d = dq;
+            st.best = bq;
         }
     }
-
-    if (w != 0u) return;  // wave 0 writes the bin
     if (st.best >= 0) {
         const TriVis T = p.tvis[st.best];
         const TriShade S = p.tshade[st.best];
@@ -1098,32 +1073,32 @@ __device__ __forceinline__ void heavy_bin(const BatchParams &p, const uint32_t *
         }
         if (V.vis_prim) V.vis_prim[pix] = S.prim;
     }
-    if (run_draw >= 0) {
+    if (seen) {
         const float m = tile_hiz<GE>(inside, st.d);
         if (lane == 0u) {
             V.hiz[sby * V.bins_x + sbx] = m;
-            atomicAdd(&p.stats->active8, (unsigned long long)n_draws);
+            atomicAdd(&p.stats->active8, 1ull);  // one draw met this bin
         }
     }
 }
 
 __global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint32_t *sorted_vals) {
     __shared__ TriVis s_tri[256];
-    __shared__ uint16_t s_list[256];
-    __shared__ uint32_t s_runs[256];
-    __shared__ uint32_t s_cnt[8];
+    __shared__ uint32_t s_mask[256];
+    __shared__ __attribute__((aligned(8))) uint16_t s_list[4][256];
+    __shared__ uint32_t s_cnt[4];
     __shared__ float s_md[kHeavyWaves][64];
     __shared__ int32_t s_mb[kHeavyWaves][64];
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES | OVF_CLIPVERTS)) return;
-    const uint32_t items = 4u * p.ctr->n_heavy;
+    const uint32_t items = p.ctr->n_heavy;
     for (uint32_t h = blockIdx.x; h < items; h += gridDim.x) {
-        const uint32_t g = p.heavy[h >> 2], bin = h & 3u;
+        const uint32_t g = p.heavy[h];
         const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
         if (p.views[vi].depth_func == DEPTH_GE)
-            heavy_bin<true>(p, sorted_vals, g, bin, s_tri, s_list, s_runs, s_cnt, s_md, s_mb);
+            heavy_tile<true>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_cnt, s_md, s_mb);
         else
-            heavy_bin<false>(p, sorted_vals, g, bin, s_tri, s_list, s_runs, s_cnt, s_md, s_mb);
-        __syncthreads();  // LDS is reused by the next item
+            heavy_tile<false>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_cnt, s_md, s_mb);
+        __syncthreads();  // LDS is reused by the next tile
     }
 }
 
@@ -1232,7 +1207,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         const uint32_t *sorted = (radix_passes & 1u) ? p.vals1 : p.vals0;
         hipLaunchKernelGGL(kb_heavy_tiles, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 256u)), dim3(256), 0,
                            s, p, sorted);
-        hipLaunchKernelGGL(kb_tile_heavy, dim3(std::min<uint32_t>(4u * p.total_tiles, 1024u)), dim3(1024), 0, s, p,
+        hipLaunchKernelGGL(kb_tile_heavy, dim3(std::min<uint32_t>(p.total_tiles, 1024u)), dim3(1024), 0, s, p,
                            sorted);
         hipLaunchKernelGGL(kb_tile, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
         hipLaunchKernelGGL(kb_active_sum, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 64u)), dim3(256), 0,
