@@ -258,6 +258,76 @@ def bench_c4(dev, args):
             "covered_pixels": covered, "steps": args.c4_steps}
 
 
+def bench_c5(dev, args):
+    """Batched views: n_views cameras (square, resolutions from {256, 512, 1024}) each drawing the
+    same n_meshes meshes (triangle counts log-uniform in [1k, 500k], RGBA32F textures 256^2-2048^2),
+    basic_vs + basic_ps, reversed-Z GE. Batched: every draw of every view is recorded and the
+    library runs them in flushes that each cover many views (one launch per stage over all their
+    (view, tile) pairs). Per-draw: the same frame with a flush after every draw."""
+    from malevich_amd import _lib as L
+    from malevich_amd import scenes as S
+    t0 = time.perf_counter()
+    sc = S.batched_scene(n_views=args.c5_views, n_meshes=args.c5_meshes, seed=args.seed + 5)
+    gen_s = time.perf_counter() - t0
+    vbs = [dev.create_buffer(L.BUFFER_VERTEX, m.vertices) for m in sc.meshes]
+    ibs = [dev.create_buffer(L.BUFFER_INDEX_U32, m.indices) for m in sc.meshes]
+    texs = [dev.create_texture2d(t) for t in sc.textures]
+    cbs = [dev.create_buffer(L.BUFFER_CONSTANT, v.cb) for v in sc.views]
+    tg = [(dev.create_render_target(v.width, v.height), dev.create_depth_stencil(v.width, v.height)) for v in sc.views]
+    dev.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
+    dev.vs_set_shader(L.VS_BASIC)
+    dev.ps_set_shader(L.PS_BASIC)
+    dev.om_set_depth_func(L.DEPTH_GREATER_EQUAL)
+
+    def frame(per_draw: bool):
+        for rt, ds in tg:
+            dev.clear_render_target_view(rt, S.CLEAR_COLOR)
+            dev.clear_depth_stencil_view(ds, 0.0)
+        for v, cb, (rt, ds) in zip(sc.views, cbs, tg):
+            dev.rs_set_viewport(0, 0, v.width, v.height, 0, 1)
+            dev.om_set_render_targets(rt, ds)
+            dev.vs_set_constant_buffer(0, cb)
+            for vb, ib, tex, m in zip(vbs, ibs, texs, sc.meshes):
+                dev.ia_set_vertex_buffer(vb, 32)
+                dev.ia_set_index_buffer(ib)
+                dev.ps_set_shader_resource(0, tex)
+                dev.draw_indexed(m.indices.size)
+                if per_draw:
+                    dev.flush()
+
+    ev = HipEvents()
+
+    def timed(per_draw: bool, steps: int) -> float:
+        a, b = ev.create(), ev.create()
+        dev.synchronize()
+        ev.record(a, dev.stream)
+        for _ in range(steps):
+            frame(per_draw)
+        dev.flush()
+        ev.record(b, dev.stream)
+        ms = ev.elapsed_ms(a, b) / steps
+        dev.synchronize()
+        return ms
+
+    warm(dev, lambda: frame(False), 1)
+    ms_b = timed(False, args.c5_steps)
+    warm(dev, lambda: frame(True), 1)
+    ms_p = timed(True, 1)
+    n = len(sc.views)
+    for h in vbs + ibs + texs + cbs:
+        h.release()
+    for rt, ds in tg:
+        rt.release()
+        ds.release()
+    return {"workload": f"c5-batched-views: {n} views (256^2/512^2/1024^2) x {len(sc.meshes)} meshes "
+                        f"({sc.triangles_per_view} triangles per view, 1k-500k per mesh), basic_vs + basic_ps, "
+                        f"RGBA32F textures 256^2-2048^2",
+            "batched_ms": round(ms_b, 3), "per_draw_ms": round(ms_p, 3),
+            "batched_views_per_s": round(n / (ms_b * 1e-3), 2), "per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
+            "batched_over_per_draw": round(ms_p / ms_b, 2), "draws": n * len(sc.meshes),
+            "triangle_draws": n * sc.triangles_per_view, "scene_gen_s": round(gen_s, 1)}
+
+
 def copy_calibration(mib: int):
     """One device-to-device hipMemcpy of `mib` MiB (the runtime runs it as a copy kernel that
     reads and writes exactly that many bytes), so that a counter run can relate FETCH_SIZE /
@@ -295,6 +365,11 @@ def main(argv=None):
     ap.add_argument("--c3-steps", type=int, default=5)
     ap.add_argument("--no-c3", action="store_true")
     ap.add_argument("--c3-only", action="store_true", help="run only the binning-stress leg (for profiling)")
+    ap.add_argument("--c5-views", type=int, default=256)
+    ap.add_argument("--c5-meshes", type=int, default=64)
+    ap.add_argument("--c5-steps", type=int, default=2)
+    ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--c5-only", action="store_true", help="run only the batched-views leg")
     ap.add_argument("--c4-steps", type=int, default=20)
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-only", action="store_true", help="run only the differentiable-render leg")
@@ -309,6 +384,11 @@ def main(argv=None):
 
     from malevich_amd import pipeline as P
 
+    if args.c5_only:
+        dev = P.Device(0)
+        print(json.dumps({"c5_batched_views": bench_c5(dev, args)}))
+        dev.close()
+        return
     if args.c4_only:
         dev = P.Device(0)
         print(json.dumps({"c4_differentiable": bench_c4(dev, args)}))
@@ -369,17 +449,17 @@ def main(argv=None):
     per_launch_ms = stage_ms[dom] / launches
     flushes_per_frame = launches / args.steps
     # Algorithmic (compulsory) bytes per launch, texels excluded:
-    #  tile:    TriVis 80 B + TriShade 32 B + three 32 B attribute rows per assembled triangle,
+    #  tile:    TriVis 96 B + TriShade 32 B + three 32 B attribute rows per assembled triangle,
     #           plus one depth read and one colour + one depth write per frame pixel (12 B)
-    #  setup:   12 B of indices and three 16 B positions per input triangle, 112 B written per
-    #           assembled triangle
+    #  setup:   12 B of indices and three 16 B positions per input triangle, 144 B written per
+    #           assembled triangle (TriVis, TriShade, bin record)
     #  vertex:  32 B read and 48 B written per vertex
     #  binning: a 4 B key + 4 B value written and re-read per (triangle, tile) pair per radix pass
     asm_frame = stats["assembled_triangle_count"] / args.steps
     verts_frame = sum(m.vertices.shape[0] for m in sc.instances)
     algo_frame = {
-        "tile": asm_frame * (80 + 32 + 3 * 32) + 12.0 * P_px,
-        "setup": tris * (12 + 3 * 16) + asm_frame * (80 + 32),
+        "tile": asm_frame * (96 + 32 + 3 * 32) + 12.0 * P_px,
+        "setup": tris * (12 + 3 * 16) + asm_frame * (96 + 32 + 16),
         "vertex": verts_frame * (32 + 48),
         "binning": asm_frame * 16 + (stats["total_triangle_count_in_bins"] / args.steps) * 8 * 2,
     }
@@ -423,6 +503,12 @@ def main(argv=None):
         out["c4_differentiable"] = c4
         out["fwd_bwd_ms"] = c4["fwd_bwd_ms"]
         print(f"[bench] c4: {c4}", file=sys.stderr, flush=True)
+    if not args.no_c5:
+        c5 = bench_c5(dev, args)
+        out["c5_batched_views"] = c5
+        out["batched_views_per_s"] = c5["batched_views_per_s"]
+        out["per_draw_views_per_s"] = c5["per_draw_views_per_s"]
+        print(f"[bench] c5: {c5}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, env, args.cpu_budget)
         out["cpu_baseline"]["gpu_over_cpu"] = round(mtri_s / out["cpu_baseline"]["value"], 1)

@@ -309,6 +309,14 @@ __device__ __forceinline__ bool setup_triangle(const DrawRec &p, const ViewRec &
     int32_t bx1 = imin(mxx / kBin, v.bins_x - 1), by1 = imin(mxy / kBin, v.bins_y - 1);
     tv.rect0 = (uint32_t)(uint16_t)bx0 | ((uint32_t)(uint16_t)by0 << 16);
     tv.rect1 = (uint32_t)(uint16_t)(int16_t)bx1 | ((uint32_t)(uint16_t)(int16_t)by1 << 16);
+    // pixel corners in the snapped bounding box: x from ceil(min/16) to floor(max/16)
+    const int32_t lx0 = imax((imin(x[0], imin(x[1], x[2])) + 15) >> 4, 0);
+    const int32_t lx1 = imin(imax(x[0], imax(x[1], x[2])) >> 4, v.W - 1);
+    const int32_t ly0 = imax((imin(y[0], imin(y[1], y[2])) + 15) >> 4, 0);
+    const int32_t ly1 = imin(imax(y[0], imax(y[1], y[2])) >> 4, v.H - 1);
+    tv.lat_x = (uint32_t)(uint16_t)(int16_t)imin(lx0, 32767) | ((uint32_t)(uint16_t)(int16_t)imax(lx1, -1) << 16);
+    tv.lat_y = (uint32_t)(uint16_t)(int16_t)imin(ly0, 32767) | ((uint32_t)(uint16_t)(int16_t)imax(ly1, -1) << 16);
+    tv.pad[0] = tv.pad[1] = 0u;
     return true;
 }
 

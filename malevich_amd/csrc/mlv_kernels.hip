@@ -799,6 +799,12 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const TriVis *s_t
             st.stale = 0u;
         }
         bool live = lane < run;
+        // a pixel corner of the bin inside the triangle's bounding box
+        {
+            const int32_t lx0 = (int16_t)(T.lat_x & 0xFFFFu), lx1 = (int16_t)(T.lat_x >> 16);
+            const int32_t ly0 = (int16_t)(T.lat_y & 0xFFFFu), ly1 = (int16_t)(T.lat_y >> 16);
+            live = live && imax(lx0, x0) <= imin(lx1, x0 + 7) && imax(ly0, y0) <= imin(ly1, y0 + 7);
+        }
         const float hzd = T.hiz_depth;
         live = live && !(GE ? (hzd < st.hz) : (hzd > st.hz));
         if (st.depth_cull) {

@@ -492,3 +492,62 @@ def c4_scene(width: int = 512, height: int = 512, rings: int = 250, segments: in
     chain, levels = mip_chain(tex)
     cb = camera_cb(eye=(2.2, -1.4, 0.9), target=(0.0, 0.0, 0.0), fov_y_deg=50.0, width=width, height=height)
     return DiffScene(mesh, tex, chain, levels, random_sh(seed + 2), cb, width, height)
+
+
+# ------------------------------------------------------------------ batched views
+
+@dataclass
+class ViewSpec:
+    width: int
+    height: int
+    cb: np.ndarray   # (48,) float32 PerFrameCB of this view's camera
+
+
+@dataclass
+class BatchedScene:
+    meshes: list          # Mesh (basic_vs layout, transform baked in)
+    textures: list        # (s, s, 4) float32 per mesh
+    views: list           # ViewSpec
+
+    @property
+    def triangles_per_view(self) -> int:
+        return sum(m.triangle_count for m in self.meshes)
+
+
+def sphere_of_about(triangles: int, displace: float, seed: int, transform: np.ndarray) -> Mesh:
+    """A displaced lat/long sphere with about `triangles` triangles in basic_vs's layout."""
+    rings = max(3, int(round(np.sqrt(triangles / 2.0 / 1.6))))
+    segments = max(4, int(round(triangles / (2.0 * rings))))
+    dm = uv_sphere(rings, segments, displace=displace, seed=seed)
+    pos = np.c_[dm.positions.astype(np.float64), np.ones(len(dm.positions))] @ transform.T
+    nrm = dm.normals.astype(np.float64) @ np.linalg.inv(transform[:3, :3]).T
+    nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
+    v = np.concatenate([pos[:, :3], nrm, dm.uvs], 1).astype(np.float32)
+    return Mesh(v, dm.indices.reshape(-1).astype(np.uint32))
+
+
+def batched_scene(n_views: int = 256, n_meshes: int = 64, min_tris: int = 1000, max_tris: int = 500_000,
+                  resolutions=(256, 512, 1024), tex_sizes=(256, 512, 1024, 2048), seed: int = 5) -> BatchedScene:
+    """Many views of one set of meshes: triangle counts log-uniform in [min_tris, max_tris],
+    seeded random rotations/scales/positions in a 20-unit cube, an RGBA32F texture per mesh of a
+    size drawn from tex_sizes, and n_views cameras at random directions and distances 24-34 looking
+    at the cube's centre, each with a square resolution drawn from `resolutions`."""
+    rng = np.random.default_rng(seed)
+    meshes, textures = [], []
+    for k in range(n_meshes):
+        t = int(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris))))
+        T = np.eye(4)
+        T[:3, :3] = random_rotation(rng) * rng.uniform(0.8, 2.5)
+        T[:3, 3] = rng.uniform(-10.0, 10.0, 3)
+        meshes.append(sphere_of_about(t, 0.1, seed * 1000 + k, T))
+        s = int(rng.choice(tex_sizes))
+        textures.append(smooth_texture(s, seed * 1000 + 500 + k))
+    views = []
+    for _ in range(n_views):
+        r = int(rng.choice(resolutions))
+        d = rng.normal(size=3)
+        d /= np.linalg.norm(d)
+        eye = d * rng.uniform(24.0, 34.0)
+        views.append(ViewSpec(r, r, camera_cb(eye=tuple(eye), target=(0.0, 0.0, 0.0), fov_y_deg=55.0, width=r,
+                                              height=r)))
+    return BatchedScene(meshes, textures, views)
