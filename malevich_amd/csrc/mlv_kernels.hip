@@ -454,7 +454,7 @@ __device__ __forceinline__ uint32_t tile_rect(uint32_t rect0, uint32_t rect1, in
     return (uint32_t)tw * (uint32_t)((by1 >> 1) - ty0 + 1);
 }
 
-constexpr uint32_t kRadixRounds = 8;
+constexpr uint32_t kRadixRounds = 16;
 constexpr uint32_t kRadixItems = 256 * kRadixRounds;
 
 // EMIT = false: per-triangle pair counts. EMIT = true: the pairs in triangle order at the
@@ -573,11 +573,26 @@ __device__ __forceinline__ void radix_hist_block(const uint32_t *keys, uint32_t 
 __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                     uint32_t *vout, uint32_t n, uint32_t nb, uint32_t shift,
                                                     const uint32_t *hist) {
-    __shared__ uint32_t run[256];
     __shared__ uint32_t wc[4][256];
+    __shared__ uint32_t loff[256];  // per digit: next free slot of the block's items in LDS
+    __shared__ uint32_t lbeg[256];  // per digit: first slot of the block's items in LDS
+    __shared__ uint32_t goff[256];  // per digit: first slot of the block's items in the output
+    __shared__ uint32_t s_w[5];
+    __shared__ uint32_t s_key[kRadixItems], s_val[kRadixItems];
     const uint32_t base = blockIdx.x * kRadixItems;
     const uint32_t tid = threadIdx.x, w = tid >> 6;
-    run[tid] = 0u;
+    // the block's count per digit is the step of the scanned digit-major histogram
+    {
+        const uint32_t idx = tid * nb + blockIdx.x;
+        const uint32_t here = hist[idx];
+        const uint32_t next = (idx + 1u < 256u * nb) ? hist[idx + 1u] : n;
+        uint32_t total;
+        const uint32_t ex = block_exclusive_scan<256, uint32_t>(next - here, s_w, total);
+        goff[tid] = here;
+        loff[tid] = ex;
+        lbeg[tid] = ex;
+    }
+    // rank every item among the block's items of its digit (stable) and place it in LDS
     for (uint32_t r = 0; r < kRadixRounds; ++r) {
         wc[0][tid] = 0u;
         wc[1][tid] = 0u;
@@ -587,6 +602,7 @@ __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const u
         const uint32_t i = base + r * 256u + tid;
         const bool valid = i < n;
         const uint32_t key = valid ? kin[i] : 0u;
+        const uint32_t val = valid ? vin[i] : 0u;
         const uint32_t dg = (key >> shift) & 255u;
         uint64_t m = __ballot(valid);
 #pragma unroll
@@ -600,20 +616,29 @@ __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const u
         __syncthreads();
         {
             const uint32_t c0 = wc[0][tid], c1 = wc[1][tid], c2 = wc[2][tid], c3 = wc[3][tid];
-            const uint32_t e0 = run[tid];
+            const uint32_t e0 = loff[tid];
             wc[0][tid] = e0;
             wc[1][tid] = e0 + c0;
             wc[2][tid] = e0 + c0 + c1;
             wc[3][tid] = e0 + c0 + c1 + c2;
-            run[tid] = e0 + c0 + c1 + c2 + c3;
+            loff[tid] = e0 + c0 + c1 + c2 + c3;
         }
         __syncthreads();
         if (valid) {
-            const uint32_t pos = hist[dg * nb + blockIdx.x] + wc[w][dg] + rank;
-            kout[pos] = key;
-            vout[pos] = vin[i];
+            const uint32_t lp = wc[w][dg] + rank;
+            s_key[lp] = key;
+            s_val[lp] = val;
         }
-        __syncthreads();
+        __syncthreads();  // every wave has read wc before the next round clears it
+    }
+    // write the block's items digit run by digit run, so consecutive lanes write consecutive slots
+    const uint32_t cnt = n > base ? min(kRadixItems, n - base) : 0u;
+    for (uint32_t j = tid; j < cnt; j += 256u) {
+        const uint32_t key = s_key[j];
+        const uint32_t dg = (key >> shift) & 255u;
+        const uint32_t pos = goff[dg] + (j - lbeg[dg]);
+        kout[pos] = key;
+        vout[pos] = s_val[j];
     }
 }
 
