@@ -760,6 +760,35 @@ struct BinState {
     uint32_t stale;     // updates since wext was last recomputed (uniform)
 };
 
+// A chunk of up to 256 records of a tile's list in LDS, one array per field: the
+// triangle-parallel screening reads a different record per lane, and with one array per field
+// consecutive records sit in consecutive banks (a 96-byte record stride would put 64 lanes on
+// a handful of banks). The rasteriser reads one record at a time as broadcasts.
+struct TileLds {
+    int32_t a16[3][256], b16[3][256], c[3][256];
+    float ooa[256], z0[256], dz1[256], dz2[256], hiz[256], cull[256];
+    uint32_t lat_x[256], lat_y[256], id[256], draw[256];
+};
+
+__device__ __forceinline__ void stage_record(TileLds &L, uint32_t k, const TriVis &t) {
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        L.a16[e][k] = t.a16[e];
+        L.b16[e][k] = t.b16[e];
+        L.c[e][k] = t.c[e];
+    }
+    L.ooa[k] = t.ooa;
+    L.z0[k] = t.z0;
+    L.dz1[k] = t.dz1;
+    L.dz2[k] = t.dz2;
+    L.hiz[k] = t.hiz_depth;
+    L.cull[k] = t.cull_bound;
+    L.lat_x[k] = t.lat_x;
+    L.lat_y[k] = t.lat_y;
+    L.id[k] = t.id;
+    L.draw[k] = t.draw;
+}
+
 // Entering a new draw: the bin's hierarchical-Z value becomes the tile minimum of its depths
 // after the previous draw (write_tile, main.c:589-603).
 template <bool GE>
@@ -773,17 +802,19 @@ __device__ __forceinline__ void bin_draw_change(const BatchParams &p, int32_t dr
 // Corner-sampled coverage of one triangle over the bin's 64 pixels and the early-Z key update
 // (main.c:1015-1028, 1083-1168); the triangle has passed hierarchical-Z.
 template <bool GE>
-__device__ __forceinline__ void bin_cover(const TriVis &T, BinState &st, bool inside, int32_t px, int32_t py) {
-    const uint32_t e0 = (uint32_t)T.c[0] + (uint32_t)__mul24(T.a16[0], px) + (uint32_t)__mul24(T.b16[0], py);
-    const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
-    const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
+__device__ __forceinline__ void bin_cover(const TileLds &L, uint32_t k, BinState &st, bool inside, int32_t px,
+                                          int32_t py) {
+    const uint32_t e0 = (uint32_t)L.c[0][k] + (uint32_t)__mul24(L.a16[0][k], px) + (uint32_t)__mul24(L.b16[0][k], py);
+    const uint32_t e1 = (uint32_t)L.c[1][k] + (uint32_t)__mul24(L.a16[1][k], px) + (uint32_t)__mul24(L.b16[1][k], py);
+    const uint32_t e2 = (uint32_t)L.c[2][k] + (uint32_t)__mul24(L.a16[2][k], px) + (uint32_t)__mul24(L.b16[2][k], py);
     const bool cov = inside && (int32_t)(e0 | e1 | e2) > 0;  // main.c:1028
     if (__ballot(cov) == 0ull) return;
-    const float b1 = (float)((int32_t)e1 >> 8) * T.ooa;  // main.c:1089-1102
-    const float b2 = (float)((int32_t)e2 >> 8) * T.ooa;
-    float z = T.z0 + T.dz1 * b1;  // register 0 is affine (main.c:1118-1154)
-    z = z + T.dz2 * b2;
-    const int32_t id = (int32_t)T.id;
+    const float ooa = L.ooa[k];
+    const float b1 = (float)((int32_t)e1 >> 8) * ooa;  // main.c:1089-1102
+    const float b2 = (float)((int32_t)e2 >> 8) * ooa;
+    float z = L.z0[k] + L.dz1[k] * b1;  // register 0 is affine (main.c:1118-1154)
+    z = z + L.dz2[k] * b2;
+    const int32_t id = (int32_t)L.id[k];
     const bool pass = GE ? (z > st.d || (z == st.d && id > st.best)) : (z < st.d || (z == st.d && id > st.best));
     const bool upd = cov && pass;
     if (upd) {
@@ -807,14 +838,14 @@ __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c,
 // The survivors are then rasterised pixel-parallel, in order. Every test only removes triangles
 // that could not have written a pixel, so the result equals the one-by-one walk.
 template <bool GE>
-__device__ __forceinline__ void walk_bin(const BatchParams &p, const TriVis *s_tri, const uint16_t *list,
-                                         uint32_t first, uint32_t stride, uint32_t n, BinState &st, bool inside,
-                                         int32_t px, int32_t py, int32_t x0, int32_t y0, uint32_t lane) {
+__device__ __forceinline__ void walk_bin(const BatchParams &p, const TileLds &L, const uint16_t *list, uint32_t first,
+                                         uint32_t stride, uint32_t n, BinState &st, bool inside, int32_t px,
+                                         int32_t py, int32_t x0, int32_t y0, uint32_t lane) {
     for (uint32_t i0 = 0; i0 < n;) {
         const uint32_t i = i0 + lane;
         const bool in = i < n;
-        const TriVis &T = s_tri[list[first + (in ? i : i0) * stride]];
-        const int32_t draw = (int32_t)T.draw;
+        const uint32_t k = list[first + (in ? i : i0) * stride];
+        const int32_t draw = (int32_t)L.draw[k];
         const int32_t d0 = (int32_t)uniform_u((uint32_t)draw);
         if (d0 != st.cur_draw) bin_draw_change<GE>(p, d0, st, inside);
         const uint64_t same = __ballot(in && draw == d0);
@@ -826,25 +857,25 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const TriVis *s_t
         bool live = lane < run;
         // a pixel corner of the bin inside the triangle's bounding box
         {
-            const int32_t lx0 = (int16_t)(T.lat_x & 0xFFFFu), lx1 = (int16_t)(T.lat_x >> 16);
-            const int32_t ly0 = (int16_t)(T.lat_y & 0xFFFFu), ly1 = (int16_t)(T.lat_y >> 16);
+            const uint32_t lx = L.lat_x[k], ly = L.lat_y[k];
+            const int32_t lx0 = (int16_t)(lx & 0xFFFFu), lx1 = (int16_t)(lx >> 16);
+            const int32_t ly0 = (int16_t)(ly & 0xFFFFu), ly1 = (int16_t)(ly >> 16);
             live = live && imax(lx0, x0) <= imin(lx1, x0 + 7) && imax(ly0, y0) <= imin(ly1, y0 + 7);
         }
-        const float hzd = T.hiz_depth;
+        const float hzd = L.hiz[k];
         live = live && !(GE ? (hzd < st.hz) : (hzd > st.hz));
         if (st.depth_cull) {
-            const float cb = T.cull_bound;
+            const float cb = L.cull[k];
             live = live && !(GE ? (cb < st.wext) : (cb > st.wext));
-            live = live && edge_max(T.a16[0], T.b16[0], T.c[0], x0, y0) >= 0 &&
-                   edge_max(T.a16[1], T.b16[1], T.c[1], x0, y0) >= 0 &&
-                   edge_max(T.a16[2], T.b16[2], T.c[2], x0, y0) >= 0;
+            live = live && edge_max(L.a16[0][k], L.b16[0][k], L.c[0][k], x0, y0) >= 0 &&
+                   edge_max(L.a16[1][k], L.b16[1][k], L.c[1][k], x0, y0) >= 0 &&
+                   edge_max(L.a16[2][k], L.b16[2][k], L.c[2][k], x0, y0) >= 0;
         }
         uint64_t S = __ballot(live);
         while (S) {
             const uint32_t jj = (uint32_t)__builtin_ctzll(S);
             S &= S - 1ull;
-            const TriVis Tj = s_tri[uniform_u(list[first + (i0 + jj) * stride])];
-            bin_cover<GE>(Tj, st, inside, px, py);
+            bin_cover<GE>(L, uniform_u(list[first + (i0 + jj) * stride]), st, inside, px, py);
         }
         i0 += run;
     }
@@ -862,7 +893,7 @@ __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *s
 
 template <bool GE>
 __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t vi,
-                                          TriVis *s_tri, uint32_t *s_mask, uint16_t (*s_list)[256],
+                                          TileLds &s_tri, uint32_t *s_mask, uint16_t (*s_list)[256],
                                           uint32_t *s_active) {
     // the wave index is uniform per wave; saying so keeps everything derived from it (the bin,
     // its Hi-Z value, the walk's branches) in scalar registers
@@ -901,7 +932,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
         if (tid < n) {
             const uint32_t t = ovf ? base + tid : sorted_vals[lstart + base + tid];
             const TriVis tv = p.tvis[t];
-            s_tri[tid] = tv;
+            stage_record(s_tri, tid, tv);
             int32_t bx0, by0, bx1, by1;
             rect_of(tv, bx0, by0, bx1, by1);
             uint32_t m = 0u;
@@ -995,7 +1026,7 @@ constexpr uint32_t kWavesPerBin = kHeavyWaves / 4;
 
 template <bool GE>
 __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g,
-                                           uint32_t vi, TriVis *s_tri, uint32_t *s_mask, uint16_t (*s_list)[256],
+                                           uint32_t vi, TileLds &s_tri, uint32_t *s_mask, uint16_t (*s_list)[256],
                                            uint32_t *s_cnt, float (*s_md)[64], int32_t (*s_mb)[64]) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
     const uint32_t bin = w / kWavesPerBin, q = w % kWavesPerBin;
@@ -1024,7 +1055,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
         const uint32_t n = min(256u, lcount - base);
         if (tid < n) {
             const TriVis tv = p.tvis[sorted_vals[lstart + base + tid]];
-            s_tri[tid] = tv;
+            stage_record(s_tri, tid, tv);
             int32_t bx0, by0, bx1, by1;
             rect_of(tv, bx0, by0, bx1, by1);
             uint32_t m = 0u;
@@ -1114,7 +1145,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
 }
 
 __global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint32_t *sorted_vals) {
-    __shared__ TriVis s_tri[256];
+    __shared__ TileLds s_tri;
     __shared__ uint32_t s_mask[256];
     __shared__ __attribute__((aligned(8))) uint16_t s_list[4][256];
     __shared__ uint32_t s_cnt[4];
@@ -1138,7 +1169,7 @@ __global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint3
 // walks them in submission order with its pixels' depths in registers; afterwards each pixel's
 // winner is shaded once and colour, depth, visibility and the bin's Hi-Z are written.
 __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
-    __shared__ TriVis s_tri[256];
+    __shared__ TileLds s_tri;
     __shared__ uint32_t s_mask[256];
     __shared__ __attribute__((aligned(8))) uint16_t s_list[4][256];
     __shared__ uint32_t s_active[4];
