@@ -334,7 +334,10 @@ struct LdsPoly {
     __device__ __forceinline__ float &at(int v, int c) const { return base[(v * 12 + c) * 64]; }
 };
 
-// clip_by_plane, main.c:609-647 (plane_d = 0), on LDS polygons.
+// clip_by_plane, main.c:609-647 (plane_d = 0), on LDS polygons, for the first NC floats of each
+// vertex: the clip-space position is components 0..3, so NC = 4 gives the exact polygon shape
+// (what counting needs) and NC = 12 the attributes as well.
+template <int NC>
 __device__ inline int clip_by_plane_lds(const LdsPoly &v, int n, const float *pl, const LdsPoly &out) {
     int k = 0;
     if (n <= 0) return 0;
@@ -345,7 +348,7 @@ __device__ inline int clip_by_plane_lds(const LdsPoly &v, int n, const float *pl
         if (k >= kMaxClip) return -1;
         int nx = (i + 1) % n;
         if (cur_in) {
-            for (int c = 0; c < 12; ++c) out.at(k, c) = v.at(i, c);
+            for (int c = 0; c < NC; ++c) out.at(k, c) = v.at(i, c);
             k++;
         }
         float nd = dot4_seq(pl[0], pl[1], pl[2], pl[3], v.at(nx, 0), v.at(nx, 1), v.at(nx, 2), v.at(nx, 3));
@@ -354,7 +357,7 @@ __device__ inline int clip_by_plane_lds(const LdsPoly &v, int n, const float *pl
             if (k >= kMaxClip) return -1;
             float t = (plane_d + cur) / (cur - nd);
             float omt = 1.0f - t;
-            for (int c = 0; c < 12; ++c) {
+            for (int c = 0; c < NC; ++c) {
                 float a = v.at(i, c) * omt;
                 float b = v.at(nx, c) * t;
                 out.at(k, c) = a + b;

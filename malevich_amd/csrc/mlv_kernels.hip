@@ -104,16 +104,20 @@ __device__ __forceinline__ bool load_tri_indices(const DrawRec &d, uint32_t t, u
 constexpr int kClipLanes = 64;
 constexpr size_t kClipLdsFloats = (size_t)2 * kMaxClip * 12 * kClipLanes;
 
+template <int NC>
 __device__ __noinline__ int clip_triangle(const BatchParams &p, const DrawRec &d, uint32_t t, LdsPoly &a,
                                           LdsPoly &b) {
     uint32_t vi[3];
     if (!load_tri_indices(d, t, vi)) return 0;
     for (int k = 0; k < 3; ++k) {
         const uint32_t slot = d.vbase + vi[k];
-        const float4 x = p.vpos[slot], y = p.vattr[2u * slot], z = p.vattr[2u * slot + 1u];
+        const float4 x = p.vpos[slot];
         a.at(k, 0) = x.x; a.at(k, 1) = x.y; a.at(k, 2) = x.z; a.at(k, 3) = x.w;
-        a.at(k, 4) = y.x; a.at(k, 5) = y.y; a.at(k, 6) = y.z; a.at(k, 7) = y.w;
-        a.at(k, 8) = z.x; a.at(k, 9) = z.y; a.at(k, 10) = z.z; a.at(k, 11) = z.w;
+        if (NC > 4) {
+            const float4 y = p.vattr[2u * slot], z = p.vattr[2u * slot + 1u];
+            a.at(k, 4) = y.x; a.at(k, 5) = y.y; a.at(k, 6) = y.z; a.at(k, 7) = y.w;
+            a.at(k, 8) = z.x; a.at(k, 9) = z.y; a.at(k, 10) = z.z; a.at(k, 11) = z.w;
+        }
     }
     int n = 3;
     for (int pl = 0; pl < 6; ++pl) {
@@ -124,7 +128,7 @@ __device__ __noinline__ int clip_triangle(const BatchParams &p, const DrawRec &d
             all_in = all_in && dot4_seq(p.planes[pl][0], p.planes[pl][1], p.planes[pl][2], p.planes[pl][3], a.at(k, 0),
                                         a.at(k, 1), a.at(k, 2), a.at(k, 3)) > -0.0f;
         if (all_in) continue;
-        n = clip_by_plane_lds(a, n, p.planes[pl], b);
+        n = clip_by_plane_lds<NC>(a, n, p.planes[pl], b);
         if (n < 0) return 0;
         LdsPoly t2 = a;
         a = b;
@@ -219,10 +223,40 @@ __global__ __launch_bounds__(256) void k_scan_down(const T *in, T *out, size_t n
     }
 }
 
+// The whole exclusive scan in one block of 1024 threads, for n up to kScanSingleMax: thread t
+// owns elements [t*per, (t+1)*per), sums them, the block scans the sums, and each thread writes
+// its run. One launch instead of three.
+constexpr size_t kScanSingleMax = 131072;
+
+template <class T>
+__global__ __launch_bounds__(1024) void k_scan_single(const T *in, T *out, size_t n_host, const uint32_t *n_dev,
+                                                      T *total) {
+    __shared__ T s_w[17];
+    const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
+    const size_t per = (n + 1023) / 1024;
+    const size_t b0 = threadIdx.x * per;
+    T sum = T(0);
+    for (size_t i = 0; i < per; ++i)
+        if (b0 + i < n) sum += in[b0 + i];
+    T tot;
+    T e = block_exclusive_scan<1024, T>(sum, s_w, tot);
+    for (size_t i = 0; i < per; ++i)
+        if (b0 + i < n) {
+            const T v = in[b0 + i];
+            out[b0 + i] = e;
+            e += v;
+        }
+    if (threadIdx.x == 0 && total) *total = tot;
+}
+
 template <class T>
 static hipError_t launch_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, void *part_raw, T *total,
                               hipStream_t s) {
     if (n_host == 0) return hipSuccess;
+    if (n_host <= kScanSingleMax) {
+        hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(1024), 0, s, in, out, n_host, n_dev, total);
+        return hipGetLastError();
+    }
     T *part = (T *)part_raw;
     const size_t nparts = (n_host + kScanChunk - 1) / kScanChunk;
     hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nparts), dim3(256), 0, s, in, n_host, n_dev, part);
@@ -322,7 +356,7 @@ __global__ __launch_bounds__(64) void kb_clip_count(BatchParams p) {
         const DrawRec &d = p.draws[draw_of_slot(p, slot)];
         const ViewRec &v = p.views[d.view];
         LdsPoly pa{s_poly + threadIdx.x}, pb{s_poly + (size_t)kMaxClip * 12 * kClipLanes + threadIdx.x};
-        int nv = clip_triangle(p, d, slot - d.tbase, pa, pb);
+        int nv = clip_triangle<4>(p, d, slot - d.tbase, pa, pb);  // positions decide count and cull
         uint32_t ntri = 0u;
         for (int ci = 1; ci < nv - 1; ++ci) {
             TriVis tv;
@@ -413,7 +447,7 @@ __global__ __launch_bounds__(64) void kb_clip_emit(BatchParams p) {
         const uint2 off = p.clip_off[slot];
         const uint32_t t = slot - d.tbase;
         LdsPoly pa{s_poly + threadIdx.x}, pb{s_poly + (size_t)kMaxClip * 12 * kClipLanes + threadIdx.x};
-        int nv = clip_triangle(p, d, t, pa, pb);
+        int nv = clip_triangle<12>(p, d, t, pa, pb);
         const uint32_t row0 = p.vslots + off.y;
         for (int k = 0; k < nv; ++k) {
             uint32_t row = row0 + (uint32_t)k;
@@ -1240,7 +1274,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         break;
     case 2: {
         if (!p.n_tblocks) break;
-        const uint32_t grid = std::min<uint32_t>(p.n_tblocks, 8192u);
+        const uint32_t grid = std::min<uint32_t>(p.n_tblocks, 2048u);
         hipLaunchKernelGGL(kb_pairs<false>, dim3(grid), dim3(256), 0, s, p);
         if ((e = launch_scan<uint32_t>(p.pair_cnt, p.pair_off, p.tri_cap, &p.ctr->n_assembled, sc.part,
                                        &p.ctr->n_pairs, s)) != hipSuccess)
