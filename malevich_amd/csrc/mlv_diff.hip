@@ -564,17 +564,18 @@ hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const D
         hipLaunchKernelGGL(kd_backward_records, dim3(nb), dim3(256), 0, s, p, gout, sc);
         const uint32_t P = p.W * p.H;
         bool alt = false;
+        const ScanScratch ss{sc.part, 0, sc.hist, 0, sc.epoch, sc.err};
         if (g.tex) {
-            if ((e = sort_pairs(sc.tkey, sc.tval, sc.tkey2, sc.tval2, 4u * P, bits_for(p.tex_total), sc.hist, sc.part,
-                                &alt, s)) != hipSuccess)
+            if ((e = sort_pairs(sc.tkey, sc.tval, sc.tkey2, sc.tval2, 4u * P, bits_for(p.tex_total), ss, &alt, s)) !=
+                hipSuccess)
                 return e;
             hipLaunchKernelGGL(kd_segment_texel, dim3((4u * P + 255u) / 256u), dim3(256), 0, s,
                                (const uint32_t *)(alt ? sc.tkey2 : sc.tkey), (const uint32_t *)(alt ? sc.tval2 : sc.tval),
                                (const float4 *)sc.tpay, 4u * P, p.tex_total, g.tex);
         }
         if (g.pos || g.col || g.uv || g.nrm) {
-            if ((e = sort_pairs(sc.vkey, sc.vval, sc.vkey2, sc.vval2, 3u * P, bits_for(p.V), sc.hist, sc.part, &alt,
-                                s)) != hipSuccess)
+            if ((e = sort_pairs(sc.vkey, sc.vval, sc.vkey2, sc.vval2, 3u * P, bits_for(p.V), ss, &alt, s)) !=
+                hipSuccess)
                 return e;
             hipLaunchKernelGGL(kd_segment_vertex, dim3((3u * P + 255u) / 256u), dim3(256), 0, s,
                                (const uint32_t *)(alt ? sc.vkey2 : sc.vkey), (const uint32_t *)(alt ? sc.vval2 : sc.vval),

@@ -118,6 +118,8 @@ struct BatchCounters {
 
 struct DevStats {
     unsigned long long assembled, active8, pairs8, oob, overflow_draws, capacity_errors, max_pairs, max_clip_list;
+    uint32_t scan_errors;  // scans whose look-back gave up waiting (never expected)
+    uint32_t pad;
 };
 
 struct BatchParams {
@@ -176,25 +178,30 @@ struct DiffScratch {
     double *vposd;                           // 4 doubles per record: dpos.xyz -
     double *posacc;                          // [V][3] position gradient accumulator (atomic mode)
     double *sh_part;                         // 27 per block
-    void *part;                              // scan partials
+    void *part;                              // scan scratch (scan_scratch_bytes)
     uint32_t *hist;                          // radix digit counts
+    uint32_t *epoch;                         // host: scan epoch counter
+    uint32_t *err;                           // device: scan error counter
 };
 hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s);
 hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const DiffGrads &g, const DiffScratch &sc,
                                 bool deterministic, hipStream_t s);
 uint32_t diff_blocks(uint32_t W, uint32_t H);
+// Scratch of the one-launch exclusive scans and of the radix sort histograms.
+struct ScanScratch {
+    void *part;          // scan_scratch_bytes(n_max): a ticket counter, then one status per partition;
+                         // zeroed when allocated
+    size_t part_bytes;
+    uint32_t *hist;      // radix digit histograms: 256 x blocks
+    size_t hist_elems;
+    uint32_t *epoch;     // host-side counter, advanced by every scan launch
+    uint32_t *err;       // device counter of scans whose look-back gave up waiting
+};
+
 // Stable LSD sort of n (key, value) pairs by the low `bits` of the key (8 bits per pass).
 // Returns in `*in_alt` whether the sorted data ended in k1/v1 (else k0/v0).
 hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, uint32_t bits,
-                      uint32_t *hist, void *part, bool *in_alt, hipStream_t s);
-
-// Scratch of the multi-block exclusive scans and of the radix sort histograms.
-struct ScanScratch {
-    void *part;          // per-chunk partial sums (8 B each)
-    size_t part_elems;
-    uint32_t *hist;      // radix digit histograms: 256 x blocks
-    size_t hist_elems;
-};
+                      const ScanScratch &sc, bool *in_alt, hipStream_t s);
 
 // Launchers (mlv_kernels.hip). All enqueue on `s`.
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
@@ -203,8 +210,8 @@ hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s);
 // binning (ordered tile lists), tile (raster + early-Z + shading + output merger).
 hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, uint32_t radix_passes,
                         hipStream_t s);
-// Element counts the scratch must hold for scans over n elements and a radix sort of n pairs.
-size_t scan_part_elems(size_t n_max);
+// Bytes of scan scratch for scans over up to n_max elements; elements of the radix histograms.
+size_t scan_scratch_bytes(size_t n_max);
 size_t radix_hist_elems(size_t n_max);
 
 }  // namespace mlv

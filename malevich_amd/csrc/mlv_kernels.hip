@@ -158,64 +158,113 @@ __global__ void k_fill_f32(float *d, float v, size_t n) {
 }
 
 // ------------------------------------------------------------ scans
-// Exclusive scan over n elements (n read from the device when n_dev is given) in three
-// kernels: chunk sums, a one-block scan of the sums, then the chunk scans.
+// Exclusive scan over n elements (n read from the device when n_dev is given) in one launch:
+// chained partitions with decoupled look-back. Block b scans its 4096 elements, publishes its
+// aggregate, reads its predecessors' published aggregates 64 at a time (one per lane of its
+// first wave) back to the nearest inclusive prefix, publishes its own inclusive prefix and
+// writes its outputs. A block waits only for blocks with smaller indices, which the dispatcher
+// starts first and which publish their aggregate without waiting, so every wait ends. Status
+// words carry the epoch of the scan they belong to, so the status array never needs clearing.
+// The wait is still bounded: a predecessor that never published would be recorded in the
+// error word and counted as zero rather than hold the wave. (A shared ticket counter would
+// make the order explicit, but hundreds of blocks hitting one address serialise in L2.)
 
 constexpr uint32_t kScanItems = 16;
 constexpr uint32_t kScanChunk = 256 * kScanItems;
+constexpr uint32_t kScanHeader = 256;            // bytes before the status array: the ticket
+constexpr uint32_t kFlagAgg = 1u, kFlagPrefix = 2u;
+constexpr uint32_t kSpinLimit = 1u << 20;
 
 template <class T>
-__global__ __launch_bounds__(256) void k_scan_reduce(const T *in, size_t n_host, const uint32_t *n_dev, T *part) {
-    __shared__ T s_w[4];
-    const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
-    const size_t base = (size_t)blockIdx.x * kScanChunk;
-    T s = T(0);
-    for (uint32_t k = 0; k < kScanItems; ++k) {
-        size_t i = base + (size_t)k * 256u + threadIdx.x;
-        if (i < n) s += in[i];
-    }
-    s = wave_sum(s);
-    if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = s;
-    __syncthreads();
-    if (threadIdx.x == 0) part[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+struct ScanStatus {
+    uint32_t flag;  // epoch << 2 | kFlag*
+    uint32_t pad;
+    T agg, prefix;
+};
+
+__device__ __forceinline__ uint32_t load_acquire(const uint32_t *p) {
+    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_release(uint32_t *p, uint32_t v) {
+    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 template <class T>
-__global__ __launch_bounds__(1024) void k_scan_top(T *part, size_t nparts, T *total) {
-    __shared__ T s_w[17];
-    const size_t per = (nparts + 1023) / 1024;
-    const size_t b0 = threadIdx.x * per;
-    T s = T(0);
-    for (size_t i = 0; i < per; ++i)
-        if (b0 + i < nparts) s += part[b0 + i];
-    T tot;
-    T e = block_exclusive_scan<1024, T>(s, s_w, tot);
-    for (size_t i = 0; i < per; ++i)
-        if (b0 + i < nparts) {
-            T v = part[b0 + i];
-            part[b0 + i] = e;
-            e += v;
-        }
-    if (threadIdx.x == 0 && total) *total = tot;
-}
-
-template <class T>
-__global__ __launch_bounds__(256) void k_scan_down(const T *in, T *out, size_t n_host, const uint32_t *n_dev,
-                                                   const T *part) {
+__global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, T *total,
+                                              uint8_t *scratch, uint32_t epoch, uint32_t nparts, uint32_t *err) {
+    __shared__ T s_prefix;
     __shared__ T s_w[5];
+    ScanStatus<T> *st = (ScanStatus<T> *)(scratch + kScanHeader);
+    const uint32_t part = blockIdx.x;
     const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
-    const size_t base = (size_t)blockIdx.x * kScanChunk;
-    if (base >= n) return;
-    const size_t i0 = base + (size_t)threadIdx.x * kScanItems;
+    const uint32_t nd = (uint32_t)((n + kScanChunk - 1) / kScanChunk);  // partitions holding data
+    if (part >= nd) {
+        // nothing to scan here, and no partition with data looks back this far
+        if (n == 0 && part == 0u && threadIdx.x == 0 && total) *total = T(0);
+        return;
+    }
+    const size_t i0 = (size_t)part * kScanChunk + (size_t)threadIdx.x * kScanItems;
     T v[kScanItems];
-    T s = T(0);
+    T sum = T(0);
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
         v[k] = (i0 + k < n) ? in[i0 + k] : T(0);
-        s += v[k];
+        sum += v[k];
     }
-    T tot;
-    T e = block_exclusive_scan<256, T>(s, s_w, tot) + part[blockIdx.x];
+    T agg;
+    const T ex = block_exclusive_scan<256, T>(sum, s_w, agg);
+    if (threadIdx.x < 64u) {  // wave 0 publishes and looks back, 64 predecessors at a time
+        const uint32_t lane = threadIdx.x;
+        const uint32_t tag = epoch << 2;
+        T prefix = T(0);
+        if (part == 0u) {
+            if (lane == 0u) {
+                st[0].prefix = agg;
+                store_release(&st[0].flag, tag | kFlagPrefix);
+            }
+        } else {
+            if (lane == 0u) {
+                st[part].agg = agg;
+                store_release(&st[part].flag, tag | kFlagAgg);
+            }
+            int64_t j = (int64_t)part - 1;
+            for (;;) {
+                const int64_t idx = j - (int64_t)lane;  // lane 0 is the nearest predecessor
+                uint32_t f = 0u;
+                if (idx >= 0) {
+                    uint32_t spins = 0u;
+                    for (;;) {
+                        f = load_acquire(&st[idx].flag);
+                        if ((f & ~3u) == tag && (f & 3u) != 0u) break;
+                        if (++spins > kSpinLimit) {
+                            atomicAdd(err, 1u);
+                            f = tag | kFlagPrefix;  // give up: count nothing from here on
+                            break;
+                        }
+                        __builtin_amdgcn_s_sleep(1);
+                    }
+                }
+                // partition 0 always holds a prefix, so a window reaching it finds one
+                const uint64_t pre = __ballot(idx >= 0 && (f & 3u) == kFlagPrefix);
+                const uint32_t first = pre ? (uint32_t)__builtin_ctzll(pre) : 64u;
+                T val = T(0);
+                if (idx >= 0 && lane <= first) val = (lane == first) ? st[idx].prefix : st[idx].agg;
+                prefix += wave_sum(val);
+                if (pre) break;
+                j -= 64;
+            }
+            if (lane == 0u) {
+                st[part].prefix = prefix + agg;
+                store_release(&st[part].flag, tag | kFlagPrefix);
+            }
+        }
+        if (lane == 0u) {
+            s_prefix = prefix;
+            if (part == nd - 1u && total) *total = prefix + agg;
+        }
+    }
+    __syncthreads();
+    T e = s_prefix + ex;
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) {
         if (i0 + k < n) out[i0 + k] = e;
@@ -223,50 +272,22 @@ __global__ __launch_bounds__(256) void k_scan_down(const T *in, T *out, size_t n
     }
 }
 
-// The whole exclusive scan in one block of 1024 threads, for n up to kScanSingleMax: thread t
-// owns elements [t*per, (t+1)*per), sums them, the block scans the sums, and each thread writes
-// its run. One launch instead of three.
-constexpr size_t kScanSingleMax = 131072;
-
 template <class T>
-__global__ __launch_bounds__(1024) void k_scan_single(const T *in, T *out, size_t n_host, const uint32_t *n_dev,
-                                                      T *total) {
-    __shared__ T s_w[17];
-    const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
-    const size_t per = (n + 1023) / 1024;
-    const size_t b0 = threadIdx.x * per;
-    T sum = T(0);
-    for (size_t i = 0; i < per; ++i)
-        if (b0 + i < n) sum += in[b0 + i];
-    T tot;
-    T e = block_exclusive_scan<1024, T>(sum, s_w, tot);
-    for (size_t i = 0; i < per; ++i)
-        if (b0 + i < n) {
-            const T v = in[b0 + i];
-            out[b0 + i] = e;
-            e += v;
-        }
-    if (threadIdx.x == 0 && total) *total = tot;
-}
-
-template <class T>
-static hipError_t launch_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, void *part_raw, T *total,
-                              hipStream_t s) {
+static hipError_t launch_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, const ScanScratch &sc,
+                              T *total, hipStream_t s) {
     if (n_host == 0) return hipSuccess;
-    if (n_host <= kScanSingleMax) {
-        hipLaunchKernelGGL(k_scan_single<T>, dim3(1), dim3(1024), 0, s, in, out, n_host, n_dev, total);
-        return hipGetLastError();
-    }
-    T *part = (T *)part_raw;
-    const size_t nparts = (n_host + kScanChunk - 1) / kScanChunk;
-    hipLaunchKernelGGL(k_scan_reduce<T>, dim3((unsigned)nparts), dim3(256), 0, s, in, n_host, n_dev, part);
-    hipLaunchKernelGGL(k_scan_top<T>, dim3(1), dim3(1024), 0, s, part, nparts, total);
-    hipLaunchKernelGGL(k_scan_down<T>, dim3((unsigned)nparts), dim3(256), 0, s, in, out, n_host, n_dev,
-                       (const T *)part);
+    const uint32_t nparts = (uint32_t)((n_host + kScanChunk - 1) / kScanChunk);
+    const uint32_t epoch = (++*sc.epoch) & 0x3FFFFFFFu;
+    hipLaunchKernelGGL(k_scan<T>, dim3(nparts), dim3(256), 0, s, in, out, n_host, n_dev, total, (uint8_t *)sc.part,
+                       epoch, nparts, sc.err);
     return hipGetLastError();
 }
 
-size_t scan_part_elems(size_t n_max) { return (n_max + kScanChunk - 1) / kScanChunk + 1; }
+// Bytes of scan scratch for scans over up to n_max elements (header + one status per partition).
+size_t scan_scratch_bytes(size_t n_max) {
+    return kScanHeader + ((n_max + kScanChunk - 1) / kScanChunk + 1) * sizeof(ScanStatus<unsigned long long>);
+}
+
 
 // ------------------------------------------------------------ stage 0: vertex
 
@@ -706,7 +727,7 @@ __global__ __launch_bounds__(256) void k_sort_scatter(const uint32_t *kin, const
 }
 
 hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, uint32_t bits,
-                      uint32_t *hist, void *part, bool *in_alt, hipStream_t s) {
+                      const ScanScratch &sc, bool *in_alt, hipStream_t s) {
     *in_alt = false;
     if (n == 0) return hipSuccess;
     const uint32_t nb = (n + kRadixItems - 1) / kRadixItems;
@@ -714,11 +735,11 @@ hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, ui
     uint32_t *kin = k0, *vin = v0, *kout = k1, *vout = v1;
     for (uint32_t pass = 0; pass < passes; ++pass) {
         const uint32_t shift = 8u * pass;
-        hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(256), 0, s, (const uint32_t *)kin, n, shift, hist);
-        hipError_t e = launch_scan<uint32_t>(hist, hist, (size_t)nb * 256u, nullptr, part, nullptr, s);
+        hipLaunchKernelGGL(k_sort_hist, dim3(nb), dim3(256), 0, s, (const uint32_t *)kin, n, shift, sc.hist);
+        hipError_t e = launch_scan<uint32_t>(sc.hist, sc.hist, (size_t)nb * 256u, nullptr, sc, nullptr, s);
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(k_sort_scatter, dim3(nb), dim3(256), 0, s, (const uint32_t *)kin, (const uint32_t *)vin,
-                           kout, vout, n, shift, (const uint32_t *)hist);
+                           kout, vout, n, shift, (const uint32_t *)sc.hist);
         std::swap(kin, kout);
         std::swap(vin, vout);
         *in_alt = !*in_alt;
@@ -1265,7 +1286,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         if (!p.n_tblocks) break;
         hipLaunchKernelGGL(kb_setup_count, dim3(p.n_tblocks), dim3(256), 0, s, p);
         hipLaunchKernelGGL(kb_clip_count, dim3(1024), dim3(64), 0, s, p);
-        if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc.part,
+        if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc,
                                                  &p.ctr->n_assembled_clip, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(kb_setup_totals, dim3((p.n_draws + 255) / 256), dim3(256), 0, s, p);
@@ -1276,7 +1297,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         if (!p.n_tblocks) break;
         const uint32_t grid = std::min<uint32_t>(p.n_tblocks, 2048u);
         hipLaunchKernelGGL(kb_pairs<false>, dim3(grid), dim3(256), 0, s, p);
-        if ((e = launch_scan<uint32_t>(p.pair_cnt, p.pair_off, p.tri_cap, &p.ctr->n_assembled, sc.part,
+        if ((e = launch_scan<uint32_t>(p.pair_cnt, p.pair_off, p.tri_cap, &p.ctr->n_assembled, sc,
                                        &p.ctr->n_pairs, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(kb_pair_check, dim3(1), dim3(256), 0, s, p, grid);
@@ -1286,7 +1307,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         for (uint32_t pass = 0; pass < radix_passes; ++pass) {
             const uint32_t shift = 8u * pass;
             hipLaunchKernelGGL(kb_radix_hist, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin, shift, sc.hist);
-            if ((e = launch_scan<uint32_t>(sc.hist, sc.hist, (size_t)nb * 256u, &p.ctr->hist_n, sc.part, nullptr,
+            if ((e = launch_scan<uint32_t>(sc.hist, sc.hist, (size_t)nb * 256u, &p.ctr->hist_n, sc, nullptr,
                                            s)) != hipSuccess)
                 return e;
             hipLaunchKernelGGL(kb_radix_scatter, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin,

@@ -116,6 +116,7 @@ struct mlv_context {
     Arena d_vposd, d_posacc;
     BatchCounters *ctr = nullptr;
     DevStats *dstats = nullptr;
+    uint32_t scan_epoch = 0;  // advanced by every scan launch (tags the scans' status words)
     // pinned staging for the draw/view tables, one slot per flush in flight
     void *staging[kStagingSlots] = {};
     size_t staging_cap[kStagingSlots] = {};
@@ -310,7 +311,12 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->tile_cnt, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->tile_start, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->heavy, (size_t)(total_tiles + 1) * 4));
-    RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_part_elems(scan_n) * 8));
+    {
+        const void *before = ctx->scan_part.p;
+        RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_scratch_bytes(scan_n)));
+        if (ctx->scan_part.p != before)  // the scans' ticket and status words start at zero
+            HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->scan_part.p, 0, ctx->scan_part.cap, ctx->stream));
+    }
     RC_OR_RETURN(ensure(ctx, ctx->radix_hist, radix_hist_elems(pair_cap) * 4));
 
     // tables through pinned staging; a slot is reused only after its copy completed
@@ -381,7 +387,8 @@ static int flush(mlv_context *ctx) {
     p.heavy = (uint32_t *)ctx->heavy.p;
     p.ctr = ctx->ctr;
     p.stats = ctx->dstats;
-    ScanScratch sc{ctx->scan_part.p, ctx->scan_part.cap / 8, (uint32_t *)ctx->radix_hist.p, ctx->radix_hist.cap / 4};
+    ScanScratch sc{ctx->scan_part.p, ctx->scan_part.cap, (uint32_t *)ctx->radix_hist.p, ctx->radix_hist.cap / 4,
+                   &ctx->scan_epoch, &ctx->dstats->scan_errors};
 
     // keys are global tile indices < total_tiles: one 8-bit pass per byte they use
     uint32_t bits = 0;
@@ -428,6 +435,10 @@ static int check_device_stats(mlv_context *ctx) {
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     ctx->pair_hint = std::max<size_t>(ctx->pair_hint, (size_t)(s.max_pairs + s.max_pairs / 4));
     ctx->clip_hint = std::max<size_t>(ctx->clip_hint, (size_t)(s.max_clip_list + s.max_clip_list / 4));
+    if (s.scan_errors) {
+        (void)hipMemsetAsync(&ctx->dstats->scan_errors, 0, sizeof(uint32_t), ctx->stream);
+        return fail(ctx, MLV_ERROR_DEVICE, "a scan's look-back gave up waiting; the flush's results are not valid");
+    }
     if (s.capacity_errors > ctx->capacity_errors) {
         ctx->capacity_errors = s.capacity_errors;
         ctx->clip_hint *= 4;
@@ -1097,7 +1108,12 @@ MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, con
         RC_OR_RETURN(ensure(ctx, ctx->d_vpay, nv * 32));
         RC_OR_RETURN(ensure(ctx, ctx->d_vposd, nv * 32));
         RC_OR_RETURN(ensure(ctx, ctx->d_hist, radix_hist_elems(nmax) * 4));
-        RC_OR_RETURN(ensure(ctx, ctx->d_part, scan_part_elems(radix_hist_elems(nmax)) * 8));
+        {
+            const void *before = ctx->d_part.p;
+            RC_OR_RETURN(ensure(ctx, ctx->d_part, scan_scratch_bytes(radix_hist_elems(nmax))));
+            if (ctx->d_part.p != before)
+                HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->d_part.p, 0, ctx->d_part.cap, ctx->stream));
+        }
         sc.tkey = (uint32_t *)ctx->d_tkey.p;
         sc.tval = (uint32_t *)ctx->d_tval.p;
         sc.tkey2 = (uint32_t *)ctx->d_tkey2.p;
@@ -1111,6 +1127,8 @@ MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, con
         sc.vposd = (double *)ctx->d_vposd.p;
         sc.hist = (uint32_t *)ctx->d_hist.p;
         sc.part = ctx->d_part.p;
+        sc.epoch = &ctx->scan_epoch;
+        sc.err = &ctx->dstats->scan_errors;
     }
     DiffGrads g{grads->positions, grads->colors, grads->uvs, grads->normals, grads->texture, grads->sh};
     HIP_OR_FAIL(ctx, launch_diff_backward(p, (const float4 *)device_grad_rgba, g, sc, deterministic != 0, ctx->stream));
