@@ -127,6 +127,7 @@ def lib() -> C.CDLL:
         L.orc_clip_planes.argtypes = [P(C.c_float)]
         L.orc_clip_polygon.argtypes = [P(C.c_float), C.c_int, P(C.c_float)]
         L.orc_clip_polygon.restype = C.c_int
+        L.orc_threads.restype = C.c_int
         L.orc_diff_forward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p]
         L.orc_diff_backward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p, P(DiffGrads)]
         _lib = L
@@ -332,4 +333,9 @@ def diff_backward(d: DiffInputs, prim: np.ndarray, grad_rgba: np.ndarray) -> dic
     if rc != 0:
         raise RuntimeError(f"orc_diff_backward failed: {rc}")
     return grads
+
+
+def threads() -> int:
+    """OpenMP threads the oracle's assembly and bin loops run on."""
+    return int(lib().orc_threads())
 

@@ -28,6 +28,9 @@
 #include "malevich_oracle.h"
 
 #include <math.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
 #include <stdlib.h>
 #include <string.h>
 
@@ -352,24 +355,11 @@ static int is_valid_vs_ps(const orc_draw_desc *d) {
 }
 
 /* main.c:662-734 (IA + VS, de-indexed) and main.c:736-913 (primitive assembly) in input order. */
-int orc_primitive_assembly(const orc_draw_desc *d, orc_triangle *out, uint32_t capacity,
-                           uint32_t *out_count, uint64_t *oob_count) {
-    if (!d || !out_count || d->index_count % 3u != 0u || !is_valid_vs_ps(d)) return ORC_ERR_INVALID_ARG;
-    const uint32_t T = d->index_count / 3u;
-    const orc_viewport vp = d->viewport;
-    float planes[6][4];
-    orc_clip_planes(planes);
-    /* main.c:826-831 screen_from_ndc, rows stored as f32 */
-    const float M[4][4] = {
-        {(float)((double)vp.width * 0.5), 0.0f, 0.0f, (float)((double)vp.width * 0.5 + (double)vp.top_left_x)},
-        {0.0f, (float)(-(double)vp.height * 0.5), 0.0f, (float)((double)vp.height * 0.5 + (double)vp.top_left_y)},
-        {0.0f, 0.0f, vp.max_depth - vp.min_depth, vp.min_depth},
-        {0.0f, 0.0f, 0.0f, 1.0f}};
-    const int32_t W = (int32_t)vp.width, H = (int32_t)vp.height;
-    uint32_t n_out = 0;
-    uint64_t oob = 0;
-
-    for (uint32_t t = 0; t < T; ++t) {
+/* Primitive assembly of input triangles [t0, t1) in input order (main.c:736-913): up to
+ * `capacity` triangles go to out[0..] (out may be NULL to only count them); *n counts them. */
+static int pa_range(const orc_draw_desc *d, const float planes[6][4], const float M[4][4], int32_t W, int32_t H,
+                    uint32_t t0, uint32_t t1, orc_triangle *out, uint32_t capacity, uint32_t *n, uint64_t *oob) {
+    for (uint32_t t = t0; t < t1; ++t) {
         float cv[MAX_CLIP_VERTICES][12];
         int bad = 0;
         for (int k = 0; k < 3; ++k) {
@@ -377,7 +367,7 @@ int orc_primitive_assembly(const orc_draw_desc *d, orc_triangle *out, uint32_t c
             if (vi < 0 || vi >= (int64_t)d->vertex_count) { bad = 1; break; }
             orc_vertex_shader(d, (uint32_t)vi, cv[k]);
         }
-        if (bad) { oob++; continue; }
+        if (bad) { (*oob)++; continue; }
         v4 p[3];
         for (int k = 0; k < 3; ++k) { p[k].x = cv[k][0]; p[k].y = cv[k][1]; p[k].z = cv[k][2]; p[k].w = cv[k][3]; }
 
@@ -433,8 +423,10 @@ int orc_primitive_assembly(const orc_draw_desc *d, orc_triangle *out, uint32_t c
             int32_t area = (int32_t)(((uint32_t)x[1] - (uint32_t)x[0]) * ((uint32_t)y[2] - (uint32_t)y[0]) -
                                      ((uint32_t)x[2] - (uint32_t)x[0]) * ((uint32_t)y[1] - (uint32_t)y[0]));
             if (area > 0) continue; /* main.c:856 back-face cull */
-            if (n_out >= capacity) return ORC_ERR_CAPACITY;
-            orc_triangle *tr = &out[n_out++];
+            if (*n >= capacity) return ORC_ERR_CAPACITY;
+            orc_triangle scratch_tr;
+            orc_triangle *tr = out ? &out[*n] : &scratch_tr;
+            (*n)++;
             /* main.c:859-861 edge slots */
             set_edge(tr->edge[2], area, x[0], y[0], x[1], y[1]);
             set_edge(tr->edge[0], area, x[1], y[1], x[2], y[2]);
@@ -466,9 +458,73 @@ int orc_primitive_assembly(const orc_draw_desc *d, orc_triangle *out, uint32_t c
             tr->max_bounds[0] = mxx; tr->max_bounds[1] = mxy;
         }
     }
-    *out_count = n_out;
+    return ORC_OK;
+}
+
+int orc_primitive_assembly(const orc_draw_desc *d, orc_triangle *out, uint32_t capacity,
+                           uint32_t *out_count, uint64_t *oob_count) {
+    if (!d || !out_count || d->index_count % 3u != 0u || !is_valid_vs_ps(d)) return ORC_ERR_INVALID_ARG;
+    const uint32_t T = d->index_count / 3u;
+    const orc_viewport vp = d->viewport;
+    float planes[6][4];
+    orc_clip_planes(planes);
+    /* main.c:826-831 screen_from_ndc, rows stored as f32 */
+    const float M[4][4] = {
+        {(float)((double)vp.width * 0.5), 0.0f, 0.0f, (float)((double)vp.width * 0.5 + (double)vp.top_left_x)},
+        {0.0f, (float)(-(double)vp.height * 0.5), 0.0f, (float)((double)vp.height * 0.5 + (double)vp.top_left_y)},
+        {0.0f, 0.0f, vp.max_depth - vp.min_depth, vp.min_depth},
+        {0.0f, 0.0f, 0.0f, 1.0f}};
+    const int32_t W = (int32_t)vp.width, H = (int32_t)vp.height;
+    /* Chunks of input triangles are assembled in parallel twice: once to count each chunk's
+     * output, then again into its place after an exclusive scan, so the output is in input
+     * order (the single-thread order of main.c:874-875) for any thread count. */
+    const uint32_t CH = 4096u, nch = (T + CH - 1u) / CH;
+    uint32_t *cnt = (uint32_t *)calloc((size_t)nch + 1u, sizeof(uint32_t));
+    if (!cnt) return ORC_ERR_NOMEM;
+    uint64_t oob = 0;
+    int err = ORC_OK;
+#pragma omp parallel for schedule(dynamic) reduction(+ : oob)
+    for (int64_t c = 0; c < (int64_t)nch; ++c) {
+        const uint32_t t0 = (uint32_t)c * CH, t1 = t0 + CH < T ? t0 + CH : T;
+        uint32_t n = 0;
+        uint64_t o = 0;
+        const int rc = pa_range(d, planes, M, W, H, t0, t1, NULL, UINT32_MAX, &n, &o);
+        if (rc) {
+#pragma omp critical
+            err = rc;
+        }
+        cnt[c] = n;
+        oob += o;
+    }
+    uint64_t total = 0;
+    for (uint32_t c = 0; c < nch; ++c) {
+        const uint32_t k = cnt[c];
+        cnt[c] = (uint32_t)total;
+        total += k;
+    }
+    if (!err && total > capacity) err = ORC_ERR_CAPACITY;
+    if (!err && out) {
+#pragma omp parallel for schedule(dynamic)
+        for (int64_t c = 0; c < (int64_t)nch; ++c) {
+            const uint32_t t0 = (uint32_t)c * CH, t1 = t0 + CH < T ? t0 + CH : T;
+            uint32_t n = 0;
+            uint64_t o = 0;
+            pa_range(d, planes, M, W, H, t0, t1, out + cnt[c], capacity - cnt[c], &n, &o);
+        }
+    }
+    free(cnt);
+    if (err) return err;
+    *out_count = (uint32_t)total;
     if (oob_count) *oob_count = oob;
     return ORC_OK;
+}
+
+int orc_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
 }
 
 /* ------------------------------------------------------- rasterisation */
@@ -568,7 +624,10 @@ int orc_draw_indexed(orc_target *rt, const orc_draw_desc *d, orc_stats *st) {
     }
 
     const uint32_t W = rt->width;
-    for (size_t b = 0; b < NB; ++b) {
+    /* bins of one draw are independent (each reads and writes only its own 64 pixels) */
+#pragma omp parallel for schedule(dynamic, 16)
+    for (int64_t bi = 0; bi < (int64_t)NB; ++bi) {
+        const size_t b = (size_t)bi;
         if (!cnt[b]) continue;
         const int32_t ox = BIN * (int32_t)(b % WT), oy = BIN * (int32_t)(b / WT);
         uint32_t tc[64], tv[64], td[64];

@@ -108,6 +108,7 @@ float orc_tonemap(float c);
 void orc_normalize3(const float in[3], float out[3]);
 int32_t orc_cvt_rne(float f);
 void orc_clip_planes(float out[6][4]);
+int orc_threads(void); /* OpenMP threads the assembly and bin loops use */
 int orc_clip_polygon(const float *in_vertices /* n x 12 */, int n, float *out_vertices /* 16 x 12 */);
 
 /* ---- differentiable shading, FP64 (malevich_diff.c). The f32 copies of positions, uvs and
