@@ -338,13 +338,97 @@ def bench_c5(dev, args):
     for rt, ds in tg:
         rt.release()
         ds.release()
-    return {"workload": f"c5-batched-views: {n} views (256^2/512^2/1024^2) x {len(sc.meshes)} meshes "
-                        f"({sc.triangles_per_view} triangles per view, 1k-500k per mesh), basic_vs + basic_ps, "
-                        f"RGBA32F textures 256^2-2048^2",
-            "batched_ms": round(ms_b, 3), "per_draw_ms": round(ms_p, 3),
-            "batched_views_per_s": round(n / (ms_b * 1e-3), 2), "per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
-            "batched_over_per_draw": round(ms_p / ms_b, 2), "draws": n * len(sc.meshes),
-            "triangle_draws": n * sc.triangles_per_view, "scene_gen_s": round(gen_s, 1)}
+    res = {"workload": f"c5-batched-views: {n} views (256^2/512^2/1024^2) x {len(sc.meshes)} meshes "
+                       f"({sc.triangles_per_view} triangles per view, 1k-500k per mesh), RGBA32F textures "
+                       f"256^2-2048^2; forward: basic_vs + basic_ps; fwd+bwd: raster (basic_vs) + differentiable "
+                       f"shading (texture mip chain x vertex colour x SH9, sRGB) + L2 loss + adjoint to every "
+                       f"mesh's positions/colours/uvs/normals/texels and the SH light",
+           "fwd_batched_ms": round(ms_b, 3), "fwd_per_draw_ms": round(ms_p, 3),
+           "fwd_batched_views_per_s": round(n / (ms_b * 1e-3), 2),
+           "fwd_per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
+           "fwd_batched_over_per_draw": round(ms_p / ms_b, 2), "draws": n * len(sc.meshes),
+           "triangle_draws": n * sc.triangles_per_view, "scene_gen_s": round(gen_s, 1)}
+    if not args.no_c5_diff:
+        res.update(bench_c5_diff(dev, sc, args, ev))
+    return res
+
+
+def bench_c5_diff(dev, sc, args, ev):
+    """Batched fwd+bwd over the same views and meshes: one raster flush over every draw of every
+    view, one shading launch over every view's pixels, the L2 loss gradient, one adjoint launch
+    summing per-mesh gradients with atomics. Per-draw: a raster flush after every draw, then one
+    shading and one adjoint launch per (view, mesh) pair."""
+    import torch
+    from malevich_amd import batched as B
+    from malevich_amd import diff as D
+    cuda = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    meshes = []
+    for k in range(len(sc.meshes)):
+        dm = sc.diff_mesh(k)
+        chain, levels = D.mip_chain(cuda(sc.textures[k]))
+        meshes.append(B.MeshTensors(cuda(dm.positions), cuda(dm.colors), cuda(dm.uvs), cuda(dm.normals),
+                                    cuda(dm.indices.astype(np.int32)), chain.contiguous(), sc.textures[k].shape[1],
+                                    sc.textures[k].shape[0], levels))
+    views = [B.ViewTensors(v.cb, v.width, v.height) for v in sc.views]
+    r = B.BatchedDiffRenderer(dev, meshes, views)
+    sh = cuda(sc.sh)
+    out = r.new_image()
+    warm(dev, lambda: (r.rasterize(), r.forward(sh, out)), 1)
+    gen = torch.Generator(device="cuda").manual_seed(args.seed + 55)
+    target = (out + 0.05 * torch.randn(out.shape, device="cuda", generator=gen)).clamp_(0, 1)
+    scale = 2.0 / out.numel()
+    grads = r.new_grads()
+    grad_sh = torch.empty((9, 3), dtype=torch.float32, device="cuda")
+    gout = torch.empty_like(out)
+    fwd = r.plan_forward(sh, out)
+    bwd = r.plan_backward(sh, gout, grads, grad_sh)
+    fwd_pd = r.plan_forward_per_draw(sh, out)
+    accum = r.new_accum()
+    bwd_pd = r.plan_backward_per_draw(sh, gout, grads, grad_sh, accum)
+
+    def step(per_draw: bool, marks=None):
+        r.rasterize(per_draw)
+        if marks is not None:
+            ev.record(marks[0], dev.stream)
+        (fwd_pd if per_draw else fwd)()
+        if marks is not None:
+            ev.record(marks[1], dev.stream)
+        torch.sub(out, target, out=gout).mul_(scale)  # d(mean squared error)/d(out)
+        (bwd_pd if per_draw else bwd)()
+
+    def timed(per_draw: bool, steps: int):
+        a, b = ev.create(), ev.create()
+        m = [ev.create(), ev.create()]
+        torch.cuda.synchronize()
+        dev.synchronize()
+        ev.record(a, dev.stream)
+        for i in range(steps):
+            step(per_draw, m if i == steps - 1 else None)
+        ev.record(b, dev.stream)
+        total = ev.elapsed_ms(a, b) / steps
+        phases = {"raster_ms": None, "shade_ms": None, "loss_and_adjoint_ms": None}
+        if steps == 1:
+            phases = {"raster_ms": round(ev.elapsed_ms(a, m[0]), 3), "shade_ms": round(ev.elapsed_ms(m[0], m[1]), 3),
+                      "loss_and_adjoint_ms": round(ev.elapsed_ms(m[1], b), 3)}
+        dev.synchronize()
+        torch.cuda.synchronize()
+        return total, phases
+
+    step(False)
+    ms_b, _ = timed(False, args.c5_steps)
+    _, ph_b = timed(False, 1)
+    step(True)
+    ms_p, ph_p = timed(True, 1)
+    n = len(sc.views)
+    r.release()
+    del grads, accum, out, target, gout
+    torch.cuda.empty_cache()
+    return {"fwd_bwd_batched_ms": round(ms_b, 3), "fwd_bwd_per_draw_ms": round(ms_p, 3),
+            "fwd_bwd_batched_views_per_s": round(n / (ms_b * 1e-3), 2),
+            "fwd_bwd_per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
+            "fwd_bwd_batched_over_per_draw": round(ms_p / ms_b, 2),
+            "fwd_bwd_batched_phases": ph_b, "fwd_bwd_per_draw_phases": ph_p,
+            "shaded_pixels": int(r.offsets[-1])}
 
 
 def copy_calibration(mib: int):
@@ -388,6 +472,7 @@ def main(argv=None):
     ap.add_argument("--c5-meshes", type=int, default=64)
     ap.add_argument("--c5-steps", type=int, default=2)
     ap.add_argument("--no-c5", action="store_true")
+    ap.add_argument("--no-c5-diff", action="store_true", help="C5 forward legs only (no batched fwd+bwd)")
     ap.add_argument("--c5-only", action="store_true", help="run only the batched-views leg")
     ap.add_argument("--c4-steps", type=int, default=20)
     ap.add_argument("--no-c4", action="store_true")
@@ -525,8 +610,9 @@ def main(argv=None):
     if not args.no_c5:
         c5 = bench_c5(dev, args)
         out["c5_batched_views"] = c5
-        out["batched_views_per_s"] = c5["batched_views_per_s"]
-        out["per_draw_views_per_s"] = c5["per_draw_views_per_s"]
+        key = "fwd_bwd" if "fwd_bwd_batched_views_per_s" in c5 else "fwd"
+        out["batched_views_per_s"] = c5[f"{key}_batched_views_per_s"]
+        out["per_draw_views_per_s"] = c5[f"{key}_per_draw_views_per_s"]
         print(f"[bench] c5: {c5}", file=sys.stderr, flush=True)
     if not args.no_cpu:
         out["cpu_baseline"] = cpu_baseline(sc, env, args.cpu_budget)

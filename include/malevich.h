@@ -250,6 +250,56 @@ MLV_API int mlv_diff_forward(mlv_context *ctx, const mlv_diff_scene *scene, cons
 MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
                               const float *device_grad_rgba, const mlv_diff_grads *grads, int deterministic);
 
+/* ---- batched differentiable shading (extension): many views of many meshes in one launch
+ *
+ * The shading above for every covered pixel of every view, with each pixel's mesh found from
+ * the draw that wrote it. Record the views' draws with mlv_om_set_primitive_id_buffer and
+ * mlv_om_set_visibility_buffer bound per view; draw tags count draws from the last
+ * mlv_reset_stats. Pixel (x, y) of a view whose primitive id is p and draw tag is t is shaded
+ * with mesh draw_mesh[t - first_tag] when t - first_tag < draw_count, triangle p of it, the
+ * view's clip_from_world and one SH light shared by all views. */
+typedef struct mlv_diff_mesh {
+    const float *positions;    /* device, [vertex_count][3] */
+    const float *colors;       /* device, [vertex_count][3] */
+    const float *uvs;          /* device, [vertex_count][2] */
+    const float *normals;      /* device, [vertex_count][3] */
+    const uint32_t *indices;   /* device, [triangle_count][3] */
+    uint32_t vertex_count, triangle_count;
+    const float *texture;      /* device RGBA32F mip chain as in mlv_diff_scene */
+    uint32_t tex_width, tex_height, tex_levels;
+    /* backward outputs, device, shapes as the inputs above; NULL skips */
+    float *grad_positions, *grad_colors, *grad_uvs, *grad_normals, *grad_texture;
+    /* optional device scratch of [vertex_count][3] doubles carrying the FP64 position sums;
+       needed (with grad_positions) by accumulating backward calls, NULL uses library scratch */
+    double *pos_accum;
+} mlv_diff_mesh;
+
+typedef struct mlv_diff_view {
+    float clip_from_world[16]; /* row-major */
+    uint32_t width, height;
+    const uint32_t *prim_ids;  /* device, width*height: the view's primitive id buffer */
+    const uint32_t *draw_tags; /* device, width*height: the view's visibility-buffer draw tags */
+    uint32_t first_tag;        /* draw tag of draw_mesh[0] */
+    uint32_t draw_count;
+    const uint32_t *draw_mesh; /* host, draw_count mesh indices; 0xFFFFFFFF (or any index past
+                                  the mesh count) leaves that draw's pixels out */
+    float *rgba;               /* device, width*height float4: forward output */
+    const float *grad_rgba;    /* device, width*height float4: backward input */
+} mlv_diff_view;
+
+/* Writes every view's rgba: (0, 0, 0, 0) where no primitive was drawn, the shaded colour where
+ * the pixel's draw maps to a mesh; pixels of draws mapped to no mesh are left unchanged. */
+MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                                   const mlv_diff_view *views, uint32_t view_count, const float *device_sh);
+/* Gradients of sum over views of sum(rgba * grad_rgba). accumulate = 0 overwrites every
+ * non-NULL gradient array, pos_accum and device_grad_sh ([9][3], if not NULL); accumulate = 1
+ * adds this call's gradients to their contents (positions: to the FP64 sums in pos_accum, then
+ * grad_positions is rewritten from them). Texel, colour, UV and normal gradients accumulate with
+ * FP32 atomics, positions with FP64 atomics. */
+MLV_API int mlv_diff_batch_backward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                                    const mlv_diff_view *views, uint32_t view_count, const float *device_sh,
+                                    float *device_grad_sh, int accumulate);
+
 #ifdef __cplusplus
 }
 #endif

@@ -65,6 +65,26 @@ class DiffGrads(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("positions", "colors", "uvs", "normals", "texture", "sh")]
 
 
+class DiffMesh(C.Structure):
+    """mlv_diff_mesh"""
+    _fields_ = [
+        ("positions", C.c_void_p), ("colors", C.c_void_p), ("uvs", C.c_void_p), ("normals", C.c_void_p),
+        ("indices", C.c_void_p), ("vertex_count", C.c_uint32), ("triangle_count", C.c_uint32),
+        ("texture", C.c_void_p), ("tex_width", C.c_uint32), ("tex_height", C.c_uint32), ("tex_levels", C.c_uint32),
+        ("grad_positions", C.c_void_p), ("grad_colors", C.c_void_p), ("grad_uvs", C.c_void_p),
+        ("grad_normals", C.c_void_p), ("grad_texture", C.c_void_p), ("pos_accum", C.c_void_p),
+    ]
+
+
+class DiffView(C.Structure):
+    """mlv_diff_view"""
+    _fields_ = [
+        ("clip_from_world", C.c_float * 16), ("width", C.c_uint32), ("height", C.c_uint32),
+        ("prim_ids", C.c_void_p), ("draw_tags", C.c_void_p), ("first_tag", C.c_uint32), ("draw_count", C.c_uint32),
+        ("draw_mesh", C.POINTER(C.c_uint32)), ("rgba", C.c_void_p), ("grad_rgba", C.c_void_p),
+    ]
+
+
 SIGNATURES = {
     "mlv_context_create": (_I, [_I, _PP]),
     "mlv_context_destroy": (_I, [_P]),
@@ -112,6 +132,8 @@ SIGNATURES = {
     "mlv_get_profile": (_I, [_P, C.POINTER(Profile)]),
     "mlv_diff_forward": (_I, [_P, C.POINTER(DiffScene), _P, _P]),
     "mlv_diff_backward": (_I, [_P, C.POINTER(DiffScene), _P, _P, C.POINTER(DiffGrads), _I]),
+    "mlv_diff_batch_forward": (_I, [_P, C.POINTER(DiffMesh), _U32, C.POINTER(DiffView), _U32, _P]),
+    "mlv_diff_batch_backward": (_I, [_P, C.POINTER(DiffMesh), _U32, C.POINTER(DiffView), _U32, _P, _P, _I]),
 }
 
 _lib: C.CDLL | None = None

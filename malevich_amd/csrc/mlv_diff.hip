@@ -66,21 +66,21 @@ struct Clip {
     R X[3], Y[3], W[3];
 };
 
-__device__ __forceinline__ void clip_of(const DiffParams &p, const uint32_t vi[3], Clip<float> &c) {
+__device__ __forceinline__ void clip_of(const DiffMesh &m, const DiffView &v, const uint32_t vi[3], Clip<float> &c) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float px = p.pos[3u * vi[k]], py = p.pos[3u * vi[k] + 1u], pz = p.pos[3u * vi[k] + 2u];
-        c.X[k] = dot4_pair(p.M + 0, px, py, pz, 1.0f);
-        c.Y[k] = dot4_pair(p.M + 4, px, py, pz, 1.0f);
-        c.W[k] = dot4_pair(p.M + 12, px, py, pz, 1.0f);
+        const float px = m.pos[3u * vi[k]], py = m.pos[3u * vi[k] + 1u], pz = m.pos[3u * vi[k] + 2u];
+        c.X[k] = dot4_pair(v.M + 0, px, py, pz, 1.0f);
+        c.Y[k] = dot4_pair(v.M + 4, px, py, pz, 1.0f);
+        c.W[k] = dot4_pair(v.M + 12, px, py, pz, 1.0f);
     }
 }
 
-__device__ __forceinline__ void clip_of(const DiffParams &p, const uint32_t vi[3], Clip<double> &c) {
+__device__ __forceinline__ void clip_of(const DiffMesh &m, const DiffView &v, const uint32_t vi[3], Clip<double> &c) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double px = p.pos[3u * vi[k]], py = p.pos[3u * vi[k] + 1u], pz = p.pos[3u * vi[k] + 2u];
-        const float *M = p.M;
+        const double px = m.pos[3u * vi[k]], py = m.pos[3u * vi[k] + 1u], pz = m.pos[3u * vi[k] + 2u];
+        const float *M = v.M;
         c.X[k] = (double)M[0] * px + (double)M[1] * py + (double)M[2] * pz + (double)M[3];
         c.Y[k] = (double)M[4] * px + (double)M[5] * py + (double)M[6] * pz + (double)M[7];
         c.W[k] = (double)M[12] * px + (double)M[13] * py + (double)M[14] * pz + (double)M[15];
@@ -111,33 +111,33 @@ __device__ __forceinline__ void bary(const Clip<R> &c, R u, R v, Bary<R> &B) {
     for (int k = 0; k < 3; ++k) B.lam[k] = B.l[k] / B.S;
 }
 
-__device__ __forceinline__ void uv_at(const DiffParams &p, const uint32_t vi[3], const float lam[3], float &u,
+__device__ __forceinline__ void uv_at(const DiffMesh &m, const uint32_t vi[3], const float lam[3], float &u,
                                       float &v) {
-    u = (lam[0] * p.uv[2u * vi[0]] + lam[1] * p.uv[2u * vi[1]]) + lam[2] * p.uv[2u * vi[2]];
-    v = (lam[0] * p.uv[2u * vi[0] + 1u] + lam[1] * p.uv[2u * vi[1] + 1u]) + lam[2] * p.uv[2u * vi[2] + 1u];
+    u = (lam[0] * m.uv[2u * vi[0]] + lam[1] * m.uv[2u * vi[1]]) + lam[2] * m.uv[2u * vi[2]];
+    v = (lam[0] * m.uv[2u * vi[0] + 1u] + lam[1] * m.uv[2u * vi[1] + 1u]) + lam[2] * m.uv[2u * vi[2] + 1u];
 }
 
 // Mip level nearest the pixel's footprint, in f32: the uv differences to the next pixel right
 // and below, in level-0 texels; level k is taken when rho >= 2^(k - 1/2), i.e.
 // rho^2 >= 2^(2k-1), which needs no logarithm.
-__device__ __forceinline__ uint32_t mip_level(const DiffParams &p, const uint32_t vi[3], const Clip<float> &c,
+__device__ __forceinline__ uint32_t mip_level(const DiffMesh &m, const DiffView &v, const uint32_t vi[3], const Clip<float> &c,
                                               float x, float y) {
-    if (p.levels <= 1u) return 0u;
+    if (m.levels <= 1u) return 0u;
     Bary<float> b0, bx, by;
-    bary(c, ndc_u(x, p.W), ndc_v(y, p.H), b0);
-    bary(c, ndc_u(x + 1.0f, p.W), ndc_v(y, p.H), bx);
-    bary(c, ndc_u(x, p.W), ndc_v(y + 1.0f, p.H), by);
+    bary(c, ndc_u(x, v.W), ndc_v(y, v.H), b0);
+    bary(c, ndc_u(x + 1.0f, v.W), ndc_v(y, v.H), bx);
+    bary(c, ndc_u(x, v.W), ndc_v(y + 1.0f, v.H), by);
     float u0, v0, ux, vx, uy, vy;
-    uv_at(p, vi, b0.lam, u0, v0);
-    uv_at(p, vi, bx.lam, ux, vx);
-    uv_at(p, vi, by.lam, uy, vy);
-    const float tw = (float)p.lw[0], th = (float)p.lh[0];
+    uv_at(m, vi, b0.lam, u0, v0);
+    uv_at(m, vi, bx.lam, ux, vx);
+    uv_at(m, vi, by.lam, uy, vy);
+    const float tw = (float)m.lw[0], th = (float)m.lh[0];
     const float dux = (ux - u0) * tw, dvx = (vx - v0) * th, duy = (uy - u0) * tw, dvy = (vy - v0) * th;
     const float rx = dux * dux + dvx * dvx, ry = duy * duy + dvy * dvy;
     const float r2 = rx > ry ? rx : ry;
     uint32_t lvl = 0u;
     float thr = 2.0f;  // 2^(2k-1) for k = 1
-    for (uint32_t k = 1; k < p.levels; ++k) {
+    for (uint32_t k = 1; k < m.levels; ++k) {
         if (r2 >= thr) lvl = k;
         thr *= 4.0f;
     }
@@ -166,34 +166,34 @@ __device__ __forceinline__ R lerp_r(R a, R b, R f) {
 }
 
 template <class R>
-__device__ __forceinline__ bool evaluate(const DiffParams &p, uint32_t x, uint32_t y, Eval<R> &e) {
-    const uint32_t t = p.prim[(size_t)y * p.W + x];
-    if (t == kNone || t >= p.T) return false;
+__device__ __forceinline__ bool evaluate(const DiffMesh &m, const DiffView &v, const float *sh, uint32_t t, uint32_t x,
+                                         uint32_t y, Eval<R> &e) {
+    if (t >= m.T) return false;
 #pragma unroll
-    for (int k = 0; k < 3; ++k) e.vi[k] = p.idx[3u * t + (uint32_t)k];
-    if (e.vi[0] >= p.V || e.vi[1] >= p.V || e.vi[2] >= p.V) return false;
+    for (int k = 0; k < 3; ++k) e.vi[k] = m.idx[3u * t + (uint32_t)k];
+    if (e.vi[0] >= m.V || e.vi[1] >= m.V || e.vi[2] >= m.V) return false;
     Clip<float> cf;
-    clip_of(p, e.vi, cf);
-    e.lvl = mip_level(p, e.vi, cf, (float)x, (float)y);
+    clip_of(m, v, e.vi, cf);
+    e.lvl = mip_level(m, v, e.vi, cf, (float)x, (float)y);
     Clip<R> c;
-    clip_of(p, e.vi, c);
-    e.u = (R)ndc_u((float)x, p.W);
-    e.v = (R)ndc_v((float)y, p.H);
+    clip_of(m, v, e.vi, c);
+    e.u = (R)ndc_u((float)x, v.W);
+    e.v = (R)ndc_v((float)y, v.H);
     bary(c, e.u, e.v, e.B);
     const R *L = e.B.lam;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        e.col[j] = (L[0] * (R)p.col[3u * e.vi[0] + j] + L[1] * (R)p.col[3u * e.vi[1] + j]) +
-                   L[2] * (R)p.col[3u * e.vi[2] + j];
-        e.n[j] = (L[0] * (R)p.nrm[3u * e.vi[0] + j] + L[1] * (R)p.nrm[3u * e.vi[1] + j]) +
-                 L[2] * (R)p.nrm[3u * e.vi[2] + j];
+        e.col[j] = (L[0] * (R)m.col[3u * e.vi[0] + j] + L[1] * (R)m.col[3u * e.vi[1] + j]) +
+                   L[2] * (R)m.col[3u * e.vi[2] + j];
+        e.n[j] = (L[0] * (R)m.nrm[3u * e.vi[0] + j] + L[1] * (R)m.nrm[3u * e.vi[1] + j]) +
+                 L[2] * (R)m.nrm[3u * e.vi[2] + j];
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j)
-        e.uvs[j] = (L[0] * (R)p.uv[2u * e.vi[0] + j] + L[1] * (R)p.uv[2u * e.vi[1] + j]) +
-                   L[2] * (R)p.uv[2u * e.vi[2] + j];
+        e.uvs[j] = (L[0] * (R)m.uv[2u * e.vi[0] + j] + L[1] * (R)m.uv[2u * e.vi[1] + j]) +
+                   L[2] * (R)m.uv[2u * e.vi[2] + j];
     // bilinear_f_x8 (csc.h:143-161) on the chosen level
-    const uint32_t lw = p.lw[e.lvl], lh = p.lh[e.lvl];
+    const uint32_t lw = m.lw[e.lvl], lh = m.lh[e.lvl];
     const R s = (R)lw * e.uvs[0] - (R)0.5;
     const R tt = (R)lh * ((R)1 - e.uvs[1]) - (R)0.5;
     const R fx = r_floor(s), fy = r_floor(tt);
@@ -206,7 +206,7 @@ __device__ __forceinline__ bool evaluate(const DiffParams &p, uint32_t x, uint32
     e.x1 = min(max(ix + 1, 0), wm);
     e.y0 = min(max(iy, 0), hm);
     e.y1 = min(max(iy + 1, 0), hm);
-    const float4 *T = p.tex + p.loff[e.lvl];
+    const float4 *T = m.tex + m.loff[e.lvl];
     const float4 q00 = T[(uint32_t)e.y0 * lw + (uint32_t)e.x0], q10 = T[(uint32_t)e.y0 * lw + (uint32_t)e.x1],
                  q01 = T[(uint32_t)e.y1 * lw + (uint32_t)e.x0], q11 = T[(uint32_t)e.y1 * lw + (uint32_t)e.x1];
     e.t00[0] = q00.x; e.t00[1] = q00.y; e.t00[2] = q00.z;
@@ -226,7 +226,7 @@ __device__ __forceinline__ bool evaluate(const DiffParams &p, uint32_t x, uint32
     for (int j = 0; j < 3; ++j) {
         R acc = (R)0;
 #pragma unroll
-        for (int k = 0; k < 9; ++k) acc = acc + (R)p.sh[3 * k + j] * e.Y[k];
+        for (int k = 0; k < 9; ++k) acc = acc + (R)sh[3 * k + j] * e.Y[k];
         e.E[j] = acc;
         e.lin[j] = (e.alb[j] * e.col[j]) * e.E[j];
         e.out[j] = srgb_of(e.lin[j]);
@@ -236,10 +236,11 @@ __device__ __forceinline__ bool evaluate(const DiffParams &p, uint32_t x, uint32
 
 __global__ __launch_bounds__(256) void kd_forward(DiffParams p, float4 *out) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    if (i >= p.W * p.H) return;
+    if (i >= p.v.W * p.v.H) return;
     Eval<float> e;
-    const uint32_t x = i % p.W, y = i / p.W;
-    out[i] = evaluate(p, x, y, e) ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+    const uint32_t x = i % p.v.W, y = i / p.v.W;
+    out[i] = evaluate(p.m, p.v, p.sh, p.prim[i], x, y, e) ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f)
+                                                          : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
 // Per-pixel adjoint (FP64). Outputs: per-corner vertex gradients (dpos, dcol, duv, dn),
@@ -251,7 +252,8 @@ struct Adj {
     double dsh[27];
 };
 
-__device__ __forceinline__ void adjoint(const DiffParams &p, const Eval<double> &e, float4 g, Adj &a) {
+__device__ __forceinline__ void adjoint(const DiffMesh &m, const DiffView &v, const float *sh, const Eval<double> &e, float4 g,
+                                        Adj &a) {
     const double P = (double)kSrgbP;
     double dlin[3];
     const double gg[3] = {g.x, g.y, g.z};
@@ -273,7 +275,7 @@ __device__ __forceinline__ void adjoint(const DiffParams &p, const Eval<double> 
         a.dsh[3 * k + 0] = dE[0] * e.Y[k];
         a.dsh[3 * k + 1] = dE[1] * e.Y[k];
         a.dsh[3 * k + 2] = dE[2] * e.Y[k];
-        dY[k] = dE[0] * (double)p.sh[3 * k] + dE[1] * (double)p.sh[3 * k + 1] + dE[2] * (double)p.sh[3 * k + 2];
+        dY[k] = dE[0] * (double)sh[3 * k] + dE[1] * (double)sh[3 * k + 1] + dE[2] * (double)sh[3 * k + 2];
     }
     const double x = e.nh[0], y = e.nh[1], z = e.nh[2];
     const double c1 = kY1, c2 = kY2, c3 = kY3, c4 = kY4;
@@ -287,7 +289,7 @@ __device__ __forceinline__ void adjoint(const DiffParams &p, const Eval<double> 
     // bilinear taps and the uv derivative (s = w u - 0.5, t = h (1 - v) - 0.5)
     const double w00 = (1.0 - e.fs) * (1.0 - e.ft), w10 = e.fs * (1.0 - e.ft), w01 = (1.0 - e.fs) * e.ft,
                  w11 = e.fs * e.ft;
-    const uint32_t lw = p.lw[e.lvl], base = p.loff[e.lvl];
+    const uint32_t lw = m.lw[e.lvl], base = m.loff[e.lvl];
     a.tap[0] = base + (uint32_t)e.y0 * lw + (uint32_t)e.x0;
     a.tap[1] = base + (uint32_t)e.y0 * lw + (uint32_t)e.x1;
     a.tap[2] = base + (uint32_t)e.y1 * lw + (uint32_t)e.x0;
@@ -304,7 +306,7 @@ __device__ __forceinline__ void adjoint(const DiffParams &p, const Eval<double> 
         ds += dalb[j] * dAs;
         dt += dalb[j] * dAt;
     }
-    const double duvi[2] = {ds * (double)lw, -dt * (double)p.lh[e.lvl]};
+    const double duvi[2] = {ds * (double)lw, -dt * (double)m.lh[e.lvl]};
     // interpolation: attributes to corners and barycentric adjoints
     const double *L = e.B.lam;
     double dlam[3];
@@ -316,11 +318,11 @@ __device__ __forceinline__ void adjoint(const DiffParams &p, const Eval<double> 
         for (int j = 0; j < 3; ++j) {
             a.dcol[k][j] = L[k] * dci[j];
             a.dn[k][j] = L[k] * dni[j];
-            acc += dci[j] * (double)p.col[3u * v + j] + dni[j] * (double)p.nrm[3u * v + j];
+            acc += dci[j] * (double)m.col[3u * v + j] + dni[j] * (double)m.nrm[3u * v + j];
         }
         a.duv[k][0] = L[k] * duvi[0];
         a.duv[k][1] = L[k] * duvi[1];
-        acc += duvi[0] * (double)p.uv[2u * v] + duvi[1] * (double)p.uv[2u * v + 1u];
+        acc += duvi[0] * (double)m.uv[2u * v] + duvi[1] * (double)m.uv[2u * v + 1u];
         dlam[k] = acc;
     }
     // lambda = l / S
@@ -343,7 +345,7 @@ __device__ __forceinline__ void adjoint(const DiffParams &p, const Eval<double> 
         const double dX = da[k], dYc = db[k], dW = -e.u * da[k] - e.v * db[k];
 #pragma unroll
         for (int j = 0; j < 3; ++j)
-            a.dpos[k][j] = (double)p.M[j] * dX + (double)p.M[4 + j] * dYc + (double)p.M[12 + j] * dW;
+            a.dpos[k][j] = (double)v.M[j] * dX + (double)v.M[4 + j] * dYc + (double)v.M[12 + j] * dW;
     }
 }
 
@@ -361,7 +363,8 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
-// Block sum of the 27 SH gradients into sh_part[block][27] (fixed order: deterministic).
+// Block sum of the 27 SH gradients into sh_part[coefficient][block] (fixed order: deterministic;
+// coefficient-major so that the final sum reads each coefficient's partials contiguously).
 __device__ __forceinline__ void sh_block_sum(const double dsh[27], double *sh_part) {
     __shared__ double s_w[4][27];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
@@ -372,7 +375,7 @@ __device__ __forceinline__ void sh_block_sum(const double dsh[27], double *sh_pa
     }
     __syncthreads();
     if (threadIdx.x < 27u)
-        sh_part[blockIdx.x * 27u + threadIdx.x] =
+        sh_part[(size_t)threadIdx.x * gridDim.x + blockIdx.x] =
             ((s_w[0][threadIdx.x] + s_w[1][threadIdx.x]) + s_w[2][threadIdx.x]) + s_w[3][threadIdx.x];
 }
 
@@ -382,9 +385,9 @@ __global__ __launch_bounds__(256) void kd_backward_atomic(DiffParams p, const fl
     Eval<double> e;
     Adj a;
     bool live = false;
-    if (i < p.W * p.H) {
-        live = evaluate(p, i % p.W, i / p.W, e);
-        if (live) adjoint(p, e, gout[i], a);
+    if (i < p.v.W * p.v.H) {
+        live = evaluate(p.m, p.v, p.sh, p.prim[i], i % p.v.W, i / p.v.W, e);
+        if (live) adjoint(p.m, p.v, p.sh, e, gout[i], a);
     }
     if (!live)
         for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
@@ -412,13 +415,13 @@ __global__ __launch_bounds__(256) void kd_backward_atomic(DiffParams p, const fl
 
 __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const float4 *gout, DiffScratch sc) {
     const uint32_t i = blockIdx.x * 256u + threadIdx.x;
-    const uint32_t P = p.W * p.H;
+    const uint32_t P = p.v.W * p.v.H;
     Eval<double> e;
     Adj a;
     bool live = false;
     if (i < P) {
-        live = evaluate(p, i % p.W, i / p.W, e);
-        if (live) adjoint(p, e, gout[i], a);
+        live = evaluate(p.m, p.v, p.sh, p.prim[i], i % p.v.W, i / p.v.W, e);
+        if (live) adjoint(p.m, p.v, p.sh, e, gout[i], a);
     }
     if (!live)
         for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
@@ -426,7 +429,7 @@ __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const f
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const uint32_t r = 4u * i + (uint32_t)t;
-            sc.tkey[r] = live ? a.tap[t] : p.tex_total;
+            sc.tkey[r] = live ? a.tap[t] : p.m.tex_total;
             sc.tval[r] = r;
             sc.tpay[r] = live ? make_float4((float)a.dtex[t][0], (float)a.dtex[t][1], (float)a.dtex[t][2], 0.0f)
                               : make_float4(0.f, 0.f, 0.f, 0.f);
@@ -434,7 +437,7 @@ __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const f
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             const uint32_t r = 3u * i + (uint32_t)k;
-            sc.vkey[r] = live ? e.vi[k] : p.V;
+            sc.vkey[r] = live ? e.vi[k] : p.m.V;
             sc.vval[r] = r;
             if (live) {
                 sc.vposd[4u * r] = a.dpos[k][0];
@@ -452,18 +455,19 @@ __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const f
 
 // Sum of the per-block SH partials, one block per coefficient: thread t sums blocks t, t+256, ...
 // in order, then a fixed tree over the 256 partial sums (the same order on every call).
-__global__ __launch_bounds__(256) void kd_sh_sum(const double *sh_part, uint32_t nblocks, float *dsh) {
+__global__ __launch_bounds__(256) void kd_sh_sum(const double *sh_part, uint32_t nblocks, float *dsh,
+                                                 uint32_t accumulate) {
     __shared__ double s[256];
     const uint32_t k = blockIdx.x, t = threadIdx.x;
     double acc = 0.0;
-    for (uint32_t b = t; b < nblocks; b += 256u) acc += sh_part[b * 27u + k];
+    for (uint32_t b = t; b < nblocks; b += 256u) acc += sh_part[(size_t)k * nblocks + b];
     s[t] = acc;
     __syncthreads();
     for (uint32_t h = 128u; h >= 1u; h >>= 1) {
         if (t < h) s[t] = s[t] + s[t + h];
         __syncthreads();
     }
-    if (t == 0u) dsh[k] = (float)s[0];
+    if (t == 0u) dsh[k] = accumulate ? dsh[k] + (float)s[0] : (float)s[0];
 }
 
 __global__ __launch_bounds__(256) void kd_pos_finalize(const double *acc, float *out, uint32_t n) {
@@ -533,12 +537,114 @@ uint32_t bits_for(uint32_t n) {
     return b;
 }
 
+// ---- batched views of many meshes
+
+// Largest i < n with start[i] <= b (start[0] == 0, ascending): the owner of block b. The block
+// index is wave-uniform, so the search runs on scalar loads.
+__device__ __forceinline__ uint32_t block_owner(const uint32_t *start, uint32_t n, uint32_t b) {
+    uint32_t lo = 0u, hi = n;
+    while (hi - lo > 1u) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (start[mid] <= b) lo = mid;
+        else hi = mid;
+    }
+    return lo;
+}
+
+// Mesh shading pixel i of view V, or kNone when its draw is not one this call shades.
+__device__ __forceinline__ uint32_t pixel_mesh(const DiffBatchParams &P, const DiffBatchView &V, uint32_t i) {
+    const uint32_t d = V.tags[i] - V.first_tag;
+    return d < V.n_draws ? P.draw_mesh[V.draw_off + d] : kNone;
+}
+
+// Pixels with no primitive get (0, 0, 0, 0); pixels of draws mapped to no mesh are left as they are.
+__global__ __launch_bounds__(256) void kd_batch_forward(DiffBatchParams P) {
+    const uint32_t vi = block_owner(P.view_block0, P.n_views, blockIdx.x);
+    const DiffBatchView &V = P.views[vi];
+    const uint32_t i = (blockIdx.x - P.view_block0[vi]) * 256u + threadIdx.x;
+    if (i >= V.v.W * V.v.H) return;
+    const uint32_t t = V.prim[i];
+    if (t == kNone) {
+        V.out[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+        return;
+    }
+    const uint32_t mi = pixel_mesh(P, V, i);
+    if (mi >= P.n_meshes) return;
+    Eval<float> e;
+    V.out[i] = evaluate(P.meshes[mi].m, V.v, P.sh, t, i % V.v.W, i / V.v.W, e)
+                   ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f)
+                   : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void kd_batch_backward(DiffBatchParams P, double *sh_part) {
+    const uint32_t vi = block_owner(P.view_block0, P.n_views, blockIdx.x);
+    const DiffBatchView &V = P.views[vi];
+    const uint32_t i = (blockIdx.x - P.view_block0[vi]) * 256u + threadIdx.x;
+    Eval<double> e;
+    Adj a;
+    bool live = false;
+    uint32_t mi = kNone;
+    if (i < V.v.W * V.v.H) {
+        const uint32_t t = V.prim[i];
+        if (t != kNone) {
+            mi = pixel_mesh(P, V, i);
+            if (mi < P.n_meshes) {
+                live = evaluate(P.meshes[mi].m, V.v, P.sh, t, i % V.v.W, i / V.v.W, e);
+                if (live) adjoint(P.meshes[mi].m, V.v, P.sh, e, V.gout[i], a);
+            }
+        }
+    }
+    if (!live)
+        for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
+    if (live) {
+        const DiffBatchMesh &B = P.meshes[mi];
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t v = e.vi[k];
+            for (int j = 0; j < 3; ++j) {
+                if (B.g.pos) atomic_add(&B.posacc[3u * v + j], a.dpos[k][j]);
+                if (B.g.col) atomic_add(&B.g.col[3u * v + j], a.dcol[k][j]);
+                if (B.g.nrm) atomic_add(&B.g.nrm[3u * v + j], a.dn[k][j]);
+            }
+            if (B.g.uv) {
+                atomic_add(&B.g.uv[2u * v], a.duv[k][0]);
+                atomic_add(&B.g.uv[2u * v + 1u], a.duv[k][1]);
+            }
+        }
+        if (B.g.tex)
+#pragma unroll
+            for (int t = 0; t < 4; ++t)
+                for (int j = 0; j < 3; ++j) atomic_add(&B.g.tex[4u * a.tap[t] + j], a.dtex[t][j]);
+    }
+    sh_block_sum(a.dsh, sh_part);
+}
+
+// Clears the listed word ranges, 1024 words a block.
+__global__ __launch_bounds__(256) void kd_batch_zero(DiffBatchParams P) {
+    const uint32_t si = block_owner(P.zero_block0, P.n_zero, blockIdx.x);
+    const DiffZeroSeg &z = P.zero[si];
+    const uint32_t base = (blockIdx.x - P.zero_block0[si]) * 1024u;
+#pragma unroll
+    for (uint32_t k = 0; k < 4u; ++k) {
+        const uint32_t w = base + k * 256u + threadIdx.x;
+        if (w < z.words) z.p[w] = 0u;
+    }
+}
+
+// Position gradients of every mesh from the FP64 sums.
+__global__ __launch_bounds__(256) void kd_batch_pos_finalize(DiffBatchParams P) {
+    const uint32_t mi = block_owner(P.mesh_block0, P.n_meshes, blockIdx.x);
+    const DiffBatchMesh &B = P.meshes[mi];
+    const uint32_t i = (blockIdx.x - P.mesh_block0[mi]) * 256u + threadIdx.x;
+    if (B.g.pos && i < 3u * B.m.V) B.g.pos[i] = (float)B.posacc[i];
+}
+
 }  // namespace
 
 uint32_t diff_blocks(uint32_t W, uint32_t H) { return (W * H + 255u) / 256u; }
 
 hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s) {
-    const uint32_t nb = diff_blocks(p.W, p.H);
+    const uint32_t nb = diff_blocks(p.v.W, p.v.H);
     if (nb == 0) return hipSuccess;
     hipLaunchKernelGGL(kd_forward, dim3(nb), dim3(256), 0, s, p, out);
     return hipGetLastError();
@@ -546,43 +652,65 @@ hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s) 
 
 hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const DiffGrads &g, const DiffScratch &sc,
                                 bool deterministic, hipStream_t s) {
-    const uint32_t nb = diff_blocks(p.W, p.H);
+    const uint32_t nb = diff_blocks(p.v.W, p.v.H);
     hipError_t e;
-    if ((e = zero(g.pos, 3ull * p.V, s)) != hipSuccess || (e = zero(g.col, 3ull * p.V, s)) != hipSuccess ||
-        (e = zero(g.uv, 2ull * p.V, s)) != hipSuccess || (e = zero(g.nrm, 3ull * p.V, s)) != hipSuccess ||
-        (e = zero(g.tex, 4ull * p.tex_total, s)) != hipSuccess)
+    if ((e = zero(g.pos, 3ull * p.m.V, s)) != hipSuccess || (e = zero(g.col, 3ull * p.m.V, s)) != hipSuccess ||
+        (e = zero(g.uv, 2ull * p.m.V, s)) != hipSuccess || (e = zero(g.nrm, 3ull * p.m.V, s)) != hipSuccess ||
+        (e = zero(g.tex, 4ull * p.m.tex_total, s)) != hipSuccess)
         return e;
     if (nb == 0) return zero(g.sh, 27, s);
     if (!deterministic) {
         double *posacc = g.pos ? sc.posacc : nullptr;
-        if ((e = zero(posacc, 3ull * p.V, s)) != hipSuccess) return e;
+        if ((e = zero(posacc, 3ull * p.m.V, s)) != hipSuccess) return e;
         hipLaunchKernelGGL(kd_backward_atomic, dim3(nb), dim3(256), 0, s, p, gout, g, posacc, sc.sh_part);
         if (g.pos)
-            hipLaunchKernelGGL(kd_pos_finalize, dim3((3u * p.V + 255u) / 256u), dim3(256), 0, s,
-                               (const double *)posacc, g.pos, 3u * p.V);
+            hipLaunchKernelGGL(kd_pos_finalize, dim3((3u * p.m.V + 255u) / 256u), dim3(256), 0, s,
+                               (const double *)posacc, g.pos, 3u * p.m.V);
     } else {
         hipLaunchKernelGGL(kd_backward_records, dim3(nb), dim3(256), 0, s, p, gout, sc);
-        const uint32_t P = p.W * p.H;
+        const uint32_t P = p.v.W * p.v.H;
         bool alt = false;
         const ScanScratch ss{sc.part, 0, sc.hist, 0, sc.epoch, sc.err};
         if (g.tex) {
-            if ((e = sort_pairs(sc.tkey, sc.tval, sc.tkey2, sc.tval2, 4u * P, bits_for(p.tex_total), ss, &alt, s)) !=
+            if ((e = sort_pairs(sc.tkey, sc.tval, sc.tkey2, sc.tval2, 4u * P, bits_for(p.m.tex_total), ss, &alt, s)) !=
                 hipSuccess)
                 return e;
             hipLaunchKernelGGL(kd_segment_texel, dim3((4u * P + 255u) / 256u), dim3(256), 0, s,
                                (const uint32_t *)(alt ? sc.tkey2 : sc.tkey), (const uint32_t *)(alt ? sc.tval2 : sc.tval),
-                               (const float4 *)sc.tpay, 4u * P, p.tex_total, g.tex);
+                               (const float4 *)sc.tpay, 4u * P, p.m.tex_total, g.tex);
         }
         if (g.pos || g.col || g.uv || g.nrm) {
-            if ((e = sort_pairs(sc.vkey, sc.vval, sc.vkey2, sc.vval2, 3u * P, bits_for(p.V), ss, &alt, s)) !=
+            if ((e = sort_pairs(sc.vkey, sc.vval, sc.vkey2, sc.vval2, 3u * P, bits_for(p.m.V), ss, &alt, s)) !=
                 hipSuccess)
                 return e;
             hipLaunchKernelGGL(kd_segment_vertex, dim3((3u * P + 255u) / 256u), dim3(256), 0, s,
                                (const uint32_t *)(alt ? sc.vkey2 : sc.vkey), (const uint32_t *)(alt ? sc.vval2 : sc.vval),
-                               (const float4 *)sc.vpay, (const double *)sc.vposd, 3u * P, p.V, g);
+                               (const float4 *)sc.vpay, (const double *)sc.vposd, 3u * P, p.m.V, g);
         }
     }
-    if (g.sh) hipLaunchKernelGGL(kd_sh_sum, dim3(27), dim3(256), 0, s, (const double *)sc.sh_part, nb, g.sh);
+    if (g.sh) hipLaunchKernelGGL(kd_sh_sum, dim3(27), dim3(256), 0, s, (const double *)sc.sh_part, nb, g.sh, 0u);
+    return hipGetLastError();
+}
+
+hipError_t launch_diff_batch_forward(const DiffBatchParams &p, hipStream_t s) {
+    if (p.n_blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(kd_batch_forward, dim3(p.n_blocks), dim3(256), 0, s, p);
+    return hipGetLastError();
+}
+
+hipError_t launch_diff_batch_backward(const DiffBatchParams &p, double *sh_part, float *grad_sh, bool accumulate,
+                                      hipStream_t s) {
+    if (!accumulate && p.n_zero_blocks)
+        hipLaunchKernelGGL(kd_batch_zero, dim3(p.n_zero_blocks), dim3(256), 0, s, p);
+    if (p.n_blocks) hipLaunchKernelGGL(kd_batch_backward, dim3(p.n_blocks), dim3(256), 0, s, p, sh_part);
+    if (p.n_pos_blocks) hipLaunchKernelGGL(kd_batch_pos_finalize, dim3(p.n_pos_blocks), dim3(256), 0, s, p);
+    if (grad_sh) {
+        if (p.n_blocks)
+            hipLaunchKernelGGL(kd_sh_sum, dim3(27), dim3(256), 0, s, (const double *)sh_part, p.n_blocks, grad_sh,
+                               accumulate ? 1u : 0u);
+        else if (!accumulate)
+            return zero(grad_sh, 27, s);
+    }
     return hipGetLastError();
 }
 

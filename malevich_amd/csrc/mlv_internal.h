@@ -151,9 +151,10 @@ struct BatchParams {
     DevStats *stats;
 };
 
-// Differentiable shading of one mesh over the raster's primitive ids (mlv_diff.hip).
+// Differentiable shading over the raster's primitive ids (mlv_diff.hip).
 constexpr uint32_t kMaxMips = 16;
-struct DiffParams {
+// One mesh: per-vertex attributes, triangles and its RGBA32F mip chain.
+struct DiffMesh {
     const float *pos, *col, *uv, *nrm;
     const uint32_t *idx;
     uint32_t V, T;
@@ -161,13 +162,51 @@ struct DiffParams {
     uint32_t tex_total;                   // texels of the whole chain
     uint32_t levels;
     uint32_t lw[kMaxMips], lh[kMaxMips], loff[kMaxMips];
-    const float *sh;
+};
+// One view: the clip_from_world matrix and the image size (the viewport covers the image).
+struct DiffView {
     float M[16];
     uint32_t W, H;
+};
+struct DiffParams {
+    DiffMesh m;
+    DiffView v;
+    const float *sh;
     const uint32_t *prim;
 };
 struct DiffGrads {
     float *pos, *col, *uv, *nrm, *tex, *sh;
+};
+// Batched shading of many views of many meshes in one launch. The grid covers every view's
+// pixels, 256 per block, each view starting on a block of its own (view_block0). A pixel's mesh
+// is draw_mesh[draw_off + tag - first_tag] for the draw tag the visibility buffer holds there.
+struct DiffBatchMesh {
+    DiffMesh m;
+    DiffGrads g;              // per-mesh gradient outputs (g.sh unused); NULL skips
+    double *posacc;           // [V][3] FP64 position accumulator
+};
+struct DiffBatchView {
+    DiffView v;
+    const uint32_t *prim, *tags;
+    uint32_t first_tag, n_draws, draw_off, pad;
+    float4 *out;              // forward output
+    const float4 *gout;       // backward input
+};
+struct DiffZeroSeg {
+    uint32_t *p;              // words to clear
+    uint32_t words;
+    uint32_t pad;
+};
+struct DiffBatchParams {
+    const DiffBatchMesh *meshes;
+    const DiffBatchView *views;
+    const uint32_t *view_block0;          // [n_views] first pixel block of each view
+    const uint32_t *draw_mesh;            // mesh of each draw, per view from draw_off
+    const uint32_t *mesh_block0;          // [n_meshes] first block of each mesh's 3V position values
+    const DiffZeroSeg *zero;              // arrays cleared before the backward pass
+    const uint32_t *zero_block0;          // [n_zero] first block of each segment (1024 words a block)
+    uint32_t n_meshes, n_views, n_blocks, n_pos_blocks, n_zero, n_zero_blocks;
+    const float *sh;
 };
 // Scratch of the deterministic backward: per-pixel contributions, their sort keys and order.
 struct DiffScratch {
@@ -187,6 +226,10 @@ hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s);
 hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const DiffGrads &g, const DiffScratch &sc,
                                 bool deterministic, hipStream_t s);
 uint32_t diff_blocks(uint32_t W, uint32_t H);
+hipError_t launch_diff_batch_forward(const DiffBatchParams &p, hipStream_t s);
+// accumulate: keep the gradients and position sums of the previous call and add to them.
+hipError_t launch_diff_batch_backward(const DiffBatchParams &p, double *sh_part, float *grad_sh, bool accumulate,
+                                      hipStream_t s);
 // Scratch of the one-launch exclusive scans and of the radix sort histograms.
 struct ScanScratch {
     void *part;          // scan_scratch_bytes(n_max): a ticket counter, then one status per partition;

@@ -175,26 +175,58 @@ constexpr uint32_t kScanHeader = 256;            // bytes before the status arra
 constexpr uint32_t kFlagAgg = 1u, kFlagPrefix = 2u;
 constexpr uint32_t kSpinLimit = 1u << 20;
 
+// Status of one partition: its aggregate (flag kFlagAgg) or its inclusive prefix (kFlagPrefix),
+// tagged with the scan's epoch. A 32-bit scan packs epoch, flag and value into one 64-bit word
+// that is published and polled with single relaxed atomics at device scope: no cache-wide
+// release/acquire fences, which would write back or invalidate the whole L2 at every step.
+// A 64-bit scan keeps value and flag apart and orders them with release/acquire (its arrays are
+// short: one value per 256 triangles).
 template <class T>
-struct ScanStatus {
-    uint32_t flag;  // epoch << 2 | kFlag*
-    uint32_t pad;
-    T agg, prefix;
+struct ScanPub;
+
+template <>
+struct ScanPub<uint32_t> {
+    static constexpr size_t kBytes = 8;
+    static __device__ __forceinline__ void publish(uint8_t *st, uint32_t i, uint32_t tagflag, uint32_t v) {
+        __hip_atomic_store((unsigned long long *)st + i, ((unsigned long long)tagflag << 32) | v, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static __device__ __forceinline__ uint32_t poll(uint8_t *st, uint32_t i, uint32_t &v) {
+        const unsigned long long w =
+            __hip_atomic_load((unsigned long long *)st + i, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        v = (uint32_t)w;
+        return (uint32_t)(w >> 32);
+    }
 };
 
-__device__ __forceinline__ uint32_t load_acquire(const uint32_t *p) {
-    return __hip_atomic_load(p, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_release(uint32_t *p, uint32_t v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-}
+template <>
+struct ScanPub<unsigned long long> {
+    struct Rec {
+        uint32_t flag;
+        uint32_t pad;
+        unsigned long long agg, prefix;  // apart: a reader that saw kFlagAgg may still be reading agg
+    };
+    static constexpr size_t kBytes = sizeof(Rec);
+    static __device__ __forceinline__ void publish(uint8_t *st, uint32_t i, uint32_t tagflag, unsigned long long v) {
+        Rec *r = (Rec *)st + i;
+        if ((tagflag & 3u) == kFlagPrefix) r->prefix = v;
+        else r->agg = v;
+        __hip_atomic_store(&r->flag, tagflag, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    static __device__ __forceinline__ uint32_t poll(uint8_t *st, uint32_t i, unsigned long long &v) {
+        Rec *r = (Rec *)st + i;
+        const uint32_t f = __hip_atomic_load(&r->flag, __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        v = ((f & 3u) == kFlagPrefix) ? r->prefix : r->agg;
+        return f;
+    }
+};
 
 template <class T>
 __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, T *total,
                                               uint8_t *scratch, uint32_t epoch, uint32_t nparts, uint32_t *err) {
     __shared__ T s_prefix;
     __shared__ T s_w[5];
-    ScanStatus<T> *st = (ScanStatus<T> *)(scratch + kScanHeader);
+    uint8_t *st = scratch + kScanHeader;
     const uint32_t part = blockIdx.x;
     const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
     const uint32_t nd = (uint32_t)((n + kScanChunk - 1) / kScanChunk);  // partitions holding data
@@ -218,27 +250,23 @@ __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host
         const uint32_t tag = epoch << 2;
         T prefix = T(0);
         if (part == 0u) {
-            if (lane == 0u) {
-                st[0].prefix = agg;
-                store_release(&st[0].flag, tag | kFlagPrefix);
-            }
+            if (lane == 0u) ScanPub<T>::publish(st, 0u, tag | kFlagPrefix, agg);
         } else {
-            if (lane == 0u) {
-                st[part].agg = agg;
-                store_release(&st[part].flag, tag | kFlagAgg);
-            }
+            if (lane == 0u) ScanPub<T>::publish(st, part, tag | kFlagAgg, agg);
             int64_t j = (int64_t)part - 1;
             for (;;) {
                 const int64_t idx = j - (int64_t)lane;  // lane 0 is the nearest predecessor
                 uint32_t f = 0u;
+                T val = T(0);
                 if (idx >= 0) {
                     uint32_t spins = 0u;
                     for (;;) {
-                        f = load_acquire(&st[idx].flag);
+                        f = ScanPub<T>::poll(st, (uint32_t)idx, val);
                         if ((f & ~3u) == tag && (f & 3u) != 0u) break;
                         if (++spins > kSpinLimit) {
                             atomicAdd(err, 1u);
                             f = tag | kFlagPrefix;  // give up: count nothing from here on
+                            val = T(0);
                             break;
                         }
                         __builtin_amdgcn_s_sleep(1);
@@ -247,16 +275,11 @@ __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host
                 // partition 0 always holds a prefix, so a window reaching it finds one
                 const uint64_t pre = __ballot(idx >= 0 && (f & 3u) == kFlagPrefix);
                 const uint32_t first = pre ? (uint32_t)__builtin_ctzll(pre) : 64u;
-                T val = T(0);
-                if (idx >= 0 && lane <= first) val = (lane == first) ? st[idx].prefix : st[idx].agg;
-                prefix += wave_sum(val);
+                prefix += wave_sum((idx >= 0 && lane <= first) ? val : T(0));
                 if (pre) break;
                 j -= 64;
             }
-            if (lane == 0u) {
-                st[part].prefix = prefix + agg;
-                store_release(&st[part].flag, tag | kFlagPrefix);
-            }
+            if (lane == 0u) ScanPub<T>::publish(st, part, tag | kFlagPrefix, prefix + agg);
         }
         if (lane == 0u) {
             s_prefix = prefix;
@@ -285,7 +308,7 @@ static hipError_t launch_scan(const T *in, T *out, size_t n_host, const uint32_t
 
 // Bytes of scan scratch for scans over up to n_max elements (header + one status per partition).
 size_t scan_scratch_bytes(size_t n_max) {
-    return kScanHeader + ((n_max + kScanChunk - 1) / kScanChunk + 1) * sizeof(ScanStatus<unsigned long long>);
+    return kScanHeader + ((n_max + kScanChunk - 1) / kScanChunk + 1) * ScanPub<unsigned long long>::kBytes;
 }
 
 

@@ -81,7 +81,7 @@ struct PendingDraw {
 constexpr uint64_t kMaxBatchTriangles = 48ull << 20;
 constexpr uint64_t kMaxBatchVertices = 48ull << 20;
 constexpr size_t kMaxBatchDraws = 1u << 16;
-constexpr int kStagingSlots = 3;
+constexpr int kStagingSlots = 16;
 
 }  // namespace
 
@@ -114,6 +114,8 @@ struct mlv_context {
     // differentiable shading scratch
     Arena d_shpart, d_tkey, d_tval, d_tkey2, d_tval2, d_tpay, d_vkey, d_vval, d_vkey2, d_vval2, d_vpay, d_part, d_hist;
     Arena d_vposd, d_posacc;
+    // batched differentiable shading: device tables, FP64 position sums, SH partials
+    Arena b_tables, b_posacc, b_shpart;
     BatchCounters *ctr = nullptr;
     DevStats *dstats = nullptr;
     uint32_t scan_epoch = 0;  // advanced by every scan launch (tags the scans' status words)
@@ -224,6 +226,32 @@ static void screen_matrix(const mlv_viewport &vp, float S[4][4]) {
     memcpy(S, M, sizeof M);
 }
 
+// Pinned staging for tables copied to the device: a ring of slots, each reused only after the
+// copies queued from it have completed (staging_commit records that point on the stream).
+static int staging_acquire(mlv_context *ctx, size_t bytes, void **host, int *slot_out) {
+    const int slot = ctx->staging_next;
+    ctx->staging_next = (slot + 1) % kStagingSlots;
+    if (ctx->staging_ev[slot]) HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->staging_ev[slot]));
+    if (ctx->staging_cap[slot] < bytes) {
+        if (ctx->staging[slot]) (void)hipHostFree(ctx->staging[slot]);
+        ctx->staging[slot] = nullptr;
+        ctx->staging_cap[slot] = 0;
+        size_t want = std::max<size_t>(bytes, 64 * 1024);
+        HIP_OR_FAIL(ctx, hipHostMalloc(&ctx->staging[slot], want, hipHostMallocDefault));
+        ctx->staging_cap[slot] = want;
+    }
+    if (!ctx->staging_ev[slot])
+        HIP_OR_FAIL(ctx, hipEventCreateWithFlags(&ctx->staging_ev[slot], hipEventDisableTiming));
+    *host = ctx->staging[slot];
+    *slot_out = slot;
+    return MLV_OK;
+}
+
+static int staging_commit(mlv_context *ctx, int slot) {
+    HIP_OR_FAIL(ctx, hipEventRecord(ctx->staging_ev[slot], ctx->stream));
+    return MLV_OK;
+}
+
 static int flush(mlv_context *ctx) {
     if (ctx->pending.empty()) return MLV_OK;
     RC_OR_RETURN(set_device(ctx));
@@ -319,28 +347,18 @@ static int flush(mlv_context *ctx) {
     }
     RC_OR_RETURN(ensure(ctx, ctx->radix_hist, radix_hist_elems(pair_cap) * 4));
 
-    // tables through pinned staging; a slot is reused only after its copy completed
+    // tables through pinned staging
     const size_t bytes_d = sizeof(DrawRec) * n_draws, bytes_v = sizeof(ViewRec) * n_views;
-    const int slot = ctx->staging_next;
-    ctx->staging_next = (slot + 1) % kStagingSlots;
-    if (ctx->staging_ev[slot]) HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->staging_ev[slot]));
-    if (ctx->staging_cap[slot] < bytes_d + bytes_v) {
-        if (ctx->staging[slot]) (void)hipHostFree(ctx->staging[slot]);
-        ctx->staging[slot] = nullptr;
-        ctx->staging_cap[slot] = 0;
-        size_t want = std::max<size_t>(bytes_d + bytes_v, 64 * 1024);
-        HIP_OR_FAIL(ctx, hipHostMalloc(&ctx->staging[slot], want, hipHostMallocDefault));
-        ctx->staging_cap[slot] = want;
-    }
-    if (!ctx->staging_ev[slot])
-        HIP_OR_FAIL(ctx, hipEventCreateWithFlags(&ctx->staging_ev[slot], hipEventDisableTiming));
-    DrawRec *stage_d = (DrawRec *)ctx->staging[slot];
+    void *stage = nullptr;
+    int slot = 0;
+    RC_OR_RETURN(staging_acquire(ctx, bytes_d + bytes_v, &stage, &slot));
+    DrawRec *stage_d = (DrawRec *)stage;
     for (uint32_t i = 0; i < n_draws; ++i) stage_d[i] = draws[i].rec;
-    memcpy((uint8_t *)ctx->staging[slot] + bytes_d, views.data(), bytes_v);
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->draws_dev.p, ctx->staging[slot], bytes_d, hipMemcpyHostToDevice, ctx->stream));
-    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->views_dev.p, (uint8_t *)ctx->staging[slot] + bytes_d, bytes_v,
-                                    hipMemcpyHostToDevice, ctx->stream));
-    HIP_OR_FAIL(ctx, hipEventRecord(ctx->staging_ev[slot], ctx->stream));
+    memcpy((uint8_t *)stage + bytes_d, views.data(), bytes_v);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->draws_dev.p, stage, bytes_d, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->views_dev.p, (uint8_t *)stage + bytes_d, bytes_v, hipMemcpyHostToDevice,
+                                    ctx->stream));
+    RC_OR_RETURN(staging_commit(ctx, slot));
 
     BatchParams p{};
     p.draws = (const DrawRec *)ctx->draws_dev.p;
@@ -495,7 +513,7 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->scan_part, &ctx->radix_hist, &ctx->heavy, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
                     &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
-                    &ctx->d_vposd,   &ctx->d_posacc};
+                    &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart};
     for (Arena *a : all) release(*a);
     resolve_events(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
@@ -1031,40 +1049,55 @@ MLV_API int mlv_reset_stats(mlv_context *ctx) {
 
 // ------------------------------------------------------------------ differentiable shading
 
+static int diff_mesh(mlv_context *ctx, const float *pos, const float *col, const float *uv, const float *nrm,
+                     const uint32_t *idx, uint32_t V, uint32_t T, const float *tex, uint32_t tw, uint32_t th,
+                     uint32_t levels, DiffMesh &m) {
+    if (!pos || !col || !uv || !nrm || !idx || !tex)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: a mesh array is NULL");
+    if (!V || !T || V > 0x3FFFFFFFu || T > 0x3FFFFFFFu || !tw || !th || tw > 16384u || th > 16384u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: bad mesh or texture sizes");
+    uint32_t max_levels = 1;
+    while (max_levels < kMaxMips && ((tw >> max_levels) || (th >> max_levels))) ++max_levels;
+    if (levels < 1 || levels > max_levels)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: tex_levels must be 1..%u for a %ux%u texture", max_levels,
+                    tw, th);
+    m = DiffMesh{};
+    m.pos = pos;
+    m.col = col;
+    m.uv = uv;
+    m.nrm = nrm;
+    m.idx = idx;
+    m.V = V;
+    m.T = T;
+    m.tex = (const float4 *)tex;
+    m.levels = levels;
+    uint64_t off = 0;
+    for (uint32_t l = 0; l < levels; ++l) {
+        m.lw[l] = std::max(tw >> l, 1u);
+        m.lh[l] = std::max(th >> l, 1u);
+        m.loff[l] = (uint32_t)off;
+        off += (uint64_t)m.lw[l] * m.lh[l];
+    }
+    m.tex_total = (uint32_t)off;  // <= 4/3 * 2^28 texels: 4 * texel index fits 32 bits
+    return MLV_OK;
+}
+
+static int diff_view(mlv_context *ctx, const float *clip_from_world, uint32_t W, uint32_t H, DiffView &v) {
+    if (!W || !H || W > 16384u || H > 16384u) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: bad image size");
+    memcpy(v.M, clip_from_world, sizeof v.M);
+    v.W = W;
+    v.H = H;
+    return MLV_OK;
+}
+
 static int diff_params(mlv_context *ctx, const mlv_diff_scene *sc, const uint32_t *prim, DiffParams &p) {
     if (!sc || !prim) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: scene and primitive ids are required");
-    if (!sc->positions || !sc->colors || !sc->uvs || !sc->normals || !sc->indices || !sc->texture || !sc->sh)
-        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: a scene array is NULL");
-    if (!sc->vertex_count || !sc->triangle_count || !sc->width || !sc->height || sc->width > 16384u ||
-        sc->height > 16384u || !sc->tex_width || !sc->tex_height || sc->tex_width > 32768u || sc->tex_height > 32768u)
-        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: bad sizes");
-    uint32_t max_levels = 1;
-    while (max_levels < kMaxMips && ((sc->tex_width >> max_levels) || (sc->tex_height >> max_levels))) ++max_levels;
-    if (sc->tex_levels < 1 || sc->tex_levels > max_levels)
-        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: tex_levels must be 1..%u for a %ux%u texture", max_levels,
-                    sc->tex_width, sc->tex_height);
+    if (!sc->sh) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff: sh is NULL");
     p = DiffParams{};
-    p.pos = sc->positions;
-    p.col = sc->colors;
-    p.uv = sc->uvs;
-    p.nrm = sc->normals;
-    p.idx = sc->indices;
-    p.V = sc->vertex_count;
-    p.T = sc->triangle_count;
-    p.tex = (const float4 *)sc->texture;
-    p.levels = sc->tex_levels;
-    uint64_t off = 0;
-    for (uint32_t l = 0; l < p.levels; ++l) {
-        p.lw[l] = std::max(sc->tex_width >> l, 1u);
-        p.lh[l] = std::max(sc->tex_height >> l, 1u);
-        p.loff[l] = (uint32_t)off;
-        off += (uint64_t)p.lw[l] * p.lh[l];
-    }
-    p.tex_total = (uint32_t)off;
+    RC_OR_RETURN(diff_mesh(ctx, sc->positions, sc->colors, sc->uvs, sc->normals, sc->indices, sc->vertex_count,
+                           sc->triangle_count, sc->texture, sc->tex_width, sc->tex_height, sc->tex_levels, p.m));
+    RC_OR_RETURN(diff_view(ctx, sc->clip_from_world, sc->width, sc->height, p.v));
     p.sh = sc->sh;
-    memcpy(p.M, sc->clip_from_world, sizeof p.M);
-    p.W = sc->width;
-    p.H = sc->height;
     p.prim = prim;
     return MLV_OK;
 }
@@ -1087,12 +1120,12 @@ MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, con
     RC_OR_RETURN(diff_params(ctx, scene, device_prim_ids, p));
     RC_OR_RETURN(set_device(ctx));
     RC_OR_RETURN(flush(ctx));
-    const uint32_t nb = diff_blocks(p.W, p.H);
-    const size_t P = (size_t)p.W * p.H;
+    const uint32_t nb = diff_blocks(p.v.W, p.v.H);
+    const size_t P = (size_t)p.v.W * p.v.H;
     DiffScratch sc{};
     RC_OR_RETURN(ensure(ctx, ctx->d_shpart, (size_t)std::max<uint32_t>(nb, 1u) * 27 * 8));
     sc.sh_part = (double *)ctx->d_shpart.p;
-    RC_OR_RETURN(ensure(ctx, ctx->d_posacc, (size_t)p.V * 3 * 8));
+    RC_OR_RETURN(ensure(ctx, ctx->d_posacc, (size_t)p.m.V * 3 * 8));
     sc.posacc = (double *)ctx->d_posacc.p;
     if (deterministic) {
         const size_t nt = 4 * P, nv = 3 * P, nmax = std::max(nt, nv);
@@ -1132,6 +1165,167 @@ MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, con
     }
     DiffGrads g{grads->positions, grads->colors, grads->uvs, grads->normals, grads->texture, grads->sh};
     HIP_OR_FAIL(ctx, launch_diff_backward(p, (const float4 *)device_grad_rgba, g, sc, deterministic != 0, ctx->stream));
+    return MLV_OK;
+}
+
+
+// Batched shading: validates the meshes and views, lays out the device tables (meshes, views,
+// per-view first blocks, draw -> mesh map, per-mesh first position blocks, the ranges the
+// backward pass clears) and copies them through pinned staging.
+static int diff_batch_setup(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t n_meshes,
+                            const mlv_diff_view *views, uint32_t n_views, const float *sh, bool backward,
+                            bool accumulate, DiffBatchParams &P) {
+    if ((n_meshes && !meshes) || (n_views && !views) || !sh)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: meshes, views and sh are required");
+    std::vector<DiffBatchMesh> bm(n_meshes);
+    uint64_t pos_total = 0;  // library-held position sums
+    for (uint32_t k = 0; k < n_meshes; ++k) {
+        const mlv_diff_mesh &m = meshes[k];
+        RC_OR_RETURN(diff_mesh(ctx, m.positions, m.colors, m.uvs, m.normals, m.indices, m.vertex_count,
+                               m.triangle_count, m.texture, m.tex_width, m.tex_height, m.tex_levels, bm[k].m));
+        bm[k].g = DiffGrads{m.grad_positions, m.grad_colors, m.grad_uvs, m.grad_normals, m.grad_texture, nullptr};
+        if (backward && accumulate && m.grad_positions && !m.pos_accum)
+            return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: accumulating positions needs pos_accum (mesh %u)",
+                        k);
+        if (!m.pos_accum) pos_total += 3ull * m.vertex_count;
+    }
+    std::vector<DiffBatchView> bv(n_views);
+    std::vector<uint32_t> view_block0(std::max<uint32_t>(n_views, 1u), 0u), draw_mesh;
+    uint64_t blocks = 0;
+    for (uint32_t v = 0; v < n_views; ++v) {
+        const mlv_diff_view &w = views[v];
+        DiffBatchView &o = bv[v];
+        o = DiffBatchView{};
+        RC_OR_RETURN(diff_view(ctx, w.clip_from_world, w.width, w.height, o.v));
+        if (!w.prim_ids || !w.draw_tags || (w.draw_count && !w.draw_mesh))
+            return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: view %u lacks its id buffers or draw map", v);
+        if (backward ? !w.grad_rgba : !w.rgba)
+            return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: view %u lacks its %s image", v,
+                        backward ? "gradient" : "output");
+        o.prim = w.prim_ids;
+        o.tags = w.draw_tags;
+        o.first_tag = w.first_tag;
+        o.n_draws = w.draw_count;
+        o.draw_off = (uint32_t)draw_mesh.size();
+        for (uint32_t d = 0; d < w.draw_count; ++d) draw_mesh.push_back(w.draw_mesh[d]);
+        o.out = (float4 *)w.rgba;
+        o.gout = (const float4 *)w.grad_rgba;
+        view_block0[v] = (uint32_t)blocks;
+        blocks += diff_blocks(w.width, w.height);
+        if (blocks > 0x7FFFFFFFull || draw_mesh.size() > 0x7FFFFFFFull)
+            return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: too many pixels or draws");
+    }
+    if (draw_mesh.empty()) draw_mesh.push_back(0xFFFFFFFFu);
+
+    // FP64 position sums: the caller's pos_accum, else one [V][3] range of library scratch
+    if (backward) RC_OR_RETURN(ensure(ctx, ctx->b_posacc, std::max<uint64_t>(pos_total, 1) * 8));
+    std::vector<uint32_t> mesh_block0(std::max<uint32_t>(n_meshes, 1u), 0u);
+    std::vector<DiffZeroSeg> zero;
+    std::vector<uint32_t> zero_block0;
+    uint64_t pos_blocks = 0, zero_blocks = 0, pos_off = 0;
+    auto clear = [&](void *p, uint64_t words) {
+        for (uint64_t w0 = 0; p && w0 < words; w0 += (1ull << 30)) {
+            const uint32_t n = (uint32_t)std::min<uint64_t>(words - w0, 1ull << 30);
+            zero.push_back(DiffZeroSeg{(uint32_t *)p + w0, n, 0});
+            zero_block0.push_back((uint32_t)zero_blocks);
+            zero_blocks += (n + 1023u) / 1024u;
+        }
+    };
+    for (uint32_t k = 0; k < n_meshes; ++k) {
+        DiffBatchMesh &b = bm[k];
+        const uint64_t V = b.m.V;
+        b.posacc = nullptr;
+        if (backward) {
+            if (meshes[k].pos_accum) {
+                b.posacc = meshes[k].pos_accum;
+            } else {
+                b.posacc = (double *)ctx->b_posacc.p + pos_off;
+                pos_off += 3 * V;
+            }
+        }
+        mesh_block0[k] = (uint32_t)pos_blocks;
+        pos_blocks += (3 * V + 255) / 256;
+        if (backward && !accumulate) {
+            clear(b.g.pos ? b.posacc : nullptr, 6 * V);
+            clear(b.g.pos, 3 * V);
+            clear(b.g.col, 3 * V);
+            clear(b.g.uv, 2 * V);
+            clear(b.g.nrm, 3 * V);
+            clear(b.g.tex, 4ull * b.m.tex_total);
+        }
+    }
+    if (pos_blocks > 0x7FFFFFFFull || zero_blocks > 0x7FFFFFFFull)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: meshes too large");
+    if (zero.empty()) {
+        zero.push_back(DiffZeroSeg{nullptr, 0, 0});
+        zero_block0.push_back(0);
+    }
+
+    // one blob: [meshes][views][view_block0][draw_mesh][mesh_block0][zero][zero_block0]
+    auto a256 = [](size_t b) { return (b + 255) & ~(size_t)255; };
+    const size_t o_mesh = 0, s_mesh = sizeof(DiffBatchMesh) * std::max<uint32_t>(n_meshes, 1u);
+    const size_t o_view = a256(o_mesh + s_mesh), s_view = sizeof(DiffBatchView) * std::max<uint32_t>(n_views, 1u);
+    const size_t o_vb = a256(o_view + s_view), s_vb = 4 * view_block0.size();
+    const size_t o_dm = a256(o_vb + s_vb), s_dm = 4 * draw_mesh.size();
+    const size_t o_mb = a256(o_dm + s_dm), s_mb = 4 * mesh_block0.size();
+    const size_t o_z = a256(o_mb + s_mb), s_z = sizeof(DiffZeroSeg) * zero.size();
+    const size_t o_zb = a256(o_z + s_z), s_zb = 4 * zero_block0.size();
+    const size_t bytes = a256(o_zb + s_zb);
+    RC_OR_RETURN(ensure(ctx, ctx->b_tables, bytes));
+    void *stage = nullptr;
+    int slot = 0;
+    RC_OR_RETURN(staging_acquire(ctx, bytes, &stage, &slot));
+    uint8_t *h = (uint8_t *)stage;
+    if (n_meshes) memcpy(h + o_mesh, bm.data(), sizeof(DiffBatchMesh) * n_meshes);
+    if (n_views) memcpy(h + o_view, bv.data(), sizeof(DiffBatchView) * n_views);
+    memcpy(h + o_vb, view_block0.data(), s_vb);
+    memcpy(h + o_dm, draw_mesh.data(), s_dm);
+    memcpy(h + o_mb, mesh_block0.data(), s_mb);
+    memcpy(h + o_z, zero.data(), s_z);
+    memcpy(h + o_zb, zero_block0.data(), s_zb);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->b_tables.p, stage, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RC_OR_RETURN(staging_commit(ctx, slot));
+    uint8_t *d = (uint8_t *)ctx->b_tables.p;
+    P = DiffBatchParams{};
+    P.meshes = (const DiffBatchMesh *)(d + o_mesh);
+    P.views = (const DiffBatchView *)(d + o_view);
+    P.view_block0 = (const uint32_t *)(d + o_vb);
+    P.draw_mesh = (const uint32_t *)(d + o_dm);
+    P.mesh_block0 = (const uint32_t *)(d + o_mb);
+    P.zero = (const DiffZeroSeg *)(d + o_z);
+    P.zero_block0 = (const uint32_t *)(d + o_zb);
+    P.n_meshes = n_meshes;
+    P.n_views = n_views;
+    P.n_blocks = n_views ? (uint32_t)blocks : 0u;
+    P.n_pos_blocks = (backward && n_meshes) ? (uint32_t)pos_blocks : 0u;
+    P.n_zero = (uint32_t)zero.size();
+    P.n_zero_blocks = (backward && !accumulate) ? (uint32_t)zero_blocks : 0u;
+    P.sh = sh;
+    return MLV_OK;
+}
+
+MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                                   const mlv_diff_view *views, uint32_t view_count, const float *device_sh) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // the id buffers come from recorded draws
+    DiffBatchParams P;
+    RC_OR_RETURN(diff_batch_setup(ctx, meshes, mesh_count, views, view_count, device_sh, false, false, P));
+    HIP_OR_FAIL(ctx, launch_diff_batch_forward(P, ctx->stream));
+    return MLV_OK;
+}
+
+MLV_API int mlv_diff_batch_backward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                                    const mlv_diff_view *views, uint32_t view_count, const float *device_sh,
+                                    float *device_grad_sh, int accumulate) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    DiffBatchParams P;
+    RC_OR_RETURN(diff_batch_setup(ctx, meshes, mesh_count, views, view_count, device_sh, true, accumulate != 0, P));
+    RC_OR_RETURN(ensure(ctx, ctx->b_shpart, (size_t)std::max<uint32_t>(P.n_blocks, 1u) * 27 * 8));
+    HIP_OR_FAIL(ctx, launch_diff_batch_backward(P, (double *)ctx->b_shpart.p, device_grad_sh, accumulate != 0,
+                                                ctx->stream));
     return MLV_OK;
 }
 

@@ -508,14 +508,23 @@ class BatchedScene:
     meshes: list          # Mesh (basic_vs layout, transform baked in)
     textures: list        # (s, s, 4) float32 per mesh
     views: list           # ViewSpec
+    colors: list          # (V, 3) float32 per-vertex colours per mesh (the differentiable shading's)
+    sh: np.ndarray        # (9, 3) float32 SH light shared by all views
 
     @property
     def triangles_per_view(self) -> int:
         return sum(m.triangle_count for m in self.meshes)
 
+    def diff_mesh(self, k: int) -> DiffMesh:
+        """Mesh k as the differentiable shading takes it (positions, colours, uvs, normals, (T, 3) indices)."""
+        v = self.meshes[k].vertices
+        return DiffMesh(np.ascontiguousarray(v[:, 0:3]), self.colors[k], np.ascontiguousarray(v[:, 6:8]),
+                        np.ascontiguousarray(v[:, 3:6]), self.meshes[k].indices.reshape(-1, 3))
 
-def sphere_of_about(triangles: int, displace: float, seed: int, transform: np.ndarray) -> Mesh:
-    """A displaced lat/long sphere with about `triangles` triangles in basic_vs's layout."""
+
+def sphere_of_about(triangles: int, displace: float, seed: int, transform: np.ndarray) -> tuple[Mesh, np.ndarray]:
+    """A displaced lat/long sphere with about `triangles` triangles in basic_vs's layout, and its
+    per-vertex colours."""
     rings = max(3, int(round(np.sqrt(triangles / 2.0 / 1.6))))
     segments = max(4, int(round(triangles / (2.0 * rings))))
     dm = uv_sphere(rings, segments, displace=displace, seed=seed)
@@ -523,7 +532,7 @@ def sphere_of_about(triangles: int, displace: float, seed: int, transform: np.nd
     nrm = dm.normals.astype(np.float64) @ np.linalg.inv(transform[:3, :3]).T
     nrm /= np.maximum(np.linalg.norm(nrm, axis=1, keepdims=True), 1e-30)
     v = np.concatenate([pos[:, :3], nrm, dm.uvs], 1).astype(np.float32)
-    return Mesh(v, dm.indices.reshape(-1).astype(np.uint32))
+    return Mesh(v, dm.indices.reshape(-1).astype(np.uint32)), dm.colors
 
 
 def batched_scene(n_views: int = 256, n_meshes: int = 64, min_tris: int = 1000, max_tris: int = 500_000,
@@ -533,13 +542,15 @@ def batched_scene(n_views: int = 256, n_meshes: int = 64, min_tris: int = 1000, 
     size drawn from tex_sizes, and n_views cameras at random directions and distances 24-34 looking
     at the cube's centre, each with a square resolution drawn from `resolutions`."""
     rng = np.random.default_rng(seed)
-    meshes, textures = [], []
+    meshes, textures, colors = [], [], []
     for k in range(n_meshes):
         t = int(np.exp(rng.uniform(np.log(min_tris), np.log(max_tris))))
         T = np.eye(4)
         T[:3, :3] = random_rotation(rng) * rng.uniform(0.8, 2.5)
         T[:3, 3] = rng.uniform(-10.0, 10.0, 3)
-        meshes.append(sphere_of_about(t, 0.1, seed * 1000 + k, T))
+        mesh, col = sphere_of_about(t, 0.1, seed * 1000 + k, T)
+        meshes.append(mesh)
+        colors.append(col)
         s = int(rng.choice(tex_sizes))
         textures.append(smooth_texture(s, seed * 1000 + 500 + k))
     views = []
@@ -550,4 +561,4 @@ def batched_scene(n_views: int = 256, n_meshes: int = 64, min_tris: int = 1000, 
         eye = d * rng.uniform(24.0, 34.0)
         views.append(ViewSpec(r, r, camera_cb(eye=tuple(eye), target=(0.0, 0.0, 0.0), fov_y_deg=55.0, width=r,
                                               height=r)))
-    return BatchedScene(meshes, textures, views)
+    return BatchedScene(meshes, textures, views, colors, random_sh(seed * 1000 + 999))
