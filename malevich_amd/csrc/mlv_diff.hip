@@ -379,37 +379,152 @@ __device__ __forceinline__ void sh_block_sum(const double dsh[27], double *sh_pa
             ((s_w[0][threadIdx.x] + s_w[1][threadIdx.x]) + s_w[2][threadIdx.x]) + s_w[3][threadIdx.x];
 }
 
+// ---- per-block aggregation of the atomic backward
+//
+// A block shades a 16x16 pixel tile. Neighbouring pixels share vertices and texels, so the
+// block first sums its contributions per (mesh, vertex) and per (mesh, texel) in LDS hash
+// tables (open addressing, keys inserted with compare-and-swap) and then adds each distinct
+// entry to global memory once. Device-wide atomics cost far more than LDS ones; a contribution
+// that finds no free slot within kProbe probes goes straight to global memory.
+constexpr uint32_t kTileSide = 16;
+constexpr uint32_t kVtxSlots = 512, kTexSlots = 1024, kProbe = 16;
+constexpr unsigned long long kEmptyKey = ~0ull;
+
+struct VtxTable {
+    unsigned long long key[kVtxSlots];
+    double pos[3][kVtxSlots];
+    float att[8][kVtxSlots];  // colour xyz, normal xyz, uv xy
+};
+struct TexTable {
+    unsigned long long key[kTexSlots];
+    float rgb[3][kTexSlots];
+};
+
+// Global destinations of one mesh's gradients.
+struct GradSink {
+    double *posacc;
+    float *col, *nrm, *uv, *tex;
+};
+
+__device__ __forceinline__ uint32_t slot_hash(unsigned long long k) {
+    return (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 40);
+}
+
+template <uint32_t N>
+__device__ __forceinline__ int lds_slot(unsigned long long *keys, unsigned long long k) {
+    uint32_t s = slot_hash(k) & (N - 1u);
+    for (uint32_t i = 0; i < kProbe; ++i) {
+        const unsigned long long old = atomicCAS(&keys[s], kEmptyKey, k);
+        if (old == kEmptyKey || old == k) return (int)s;
+        s = (s + 1u) & (N - 1u);
+    }
+    return -1;
+}
+
+__device__ __forceinline__ void tables_clear(VtxTable &vt, TexTable &tt) {
+    for (uint32_t s = threadIdx.x; s < kVtxSlots; s += 256u) {
+        vt.key[s] = kEmptyKey;
+        for (int c = 0; c < 3; ++c) vt.pos[c][s] = 0.0;
+        for (int c = 0; c < 8; ++c) vt.att[c][s] = 0.0f;
+    }
+    for (uint32_t s = threadIdx.x; s < kTexSlots; s += 256u) {
+        tt.key[s] = kEmptyKey;
+        for (int c = 0; c < 3; ++c) tt.rgb[c][s] = 0.0f;
+    }
+}
+
+__device__ __forceinline__ void vertex_to_global(const GradSink &g, uint32_t v, const double dpos[3],
+                                                 const float att[8]) {
+    for (int j = 0; j < 3; ++j) {
+        if (g.posacc) atomic_add(&g.posacc[3u * v + j], dpos[j]);
+        if (g.col) atomic_add(&g.col[3u * v + j], (double)att[j]);
+        if (g.nrm) atomic_add(&g.nrm[3u * v + j], (double)att[3 + j]);
+    }
+    if (g.uv) {
+        atomic_add(&g.uv[2u * v], (double)att[6]);
+        atomic_add(&g.uv[2u * v + 1u], (double)att[7]);
+    }
+}
+
+// One pixel's contributions into the block's tables (mesh index m in the keys' high word).
+__device__ __forceinline__ void gather_pixel(VtxTable &vt, TexTable &tt, const GradSink &g, uint32_t m,
+                                             const Eval<double> &e, const Adj &a) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const uint32_t v = e.vi[k];
+        const float att[8] = {(float)a.dcol[k][0], (float)a.dcol[k][1], (float)a.dcol[k][2], (float)a.dn[k][0],
+                              (float)a.dn[k][1],   (float)a.dn[k][2],   (float)a.duv[k][0], (float)a.duv[k][1]};
+        const int s = lds_slot<kVtxSlots>(vt.key, ((unsigned long long)m << 32) | v);
+        if (s < 0) {
+            vertex_to_global(g, v, a.dpos[k], att);
+            continue;
+        }
+        for (int j = 0; j < 3; ++j)
+            if (a.dpos[k][j] != 0.0) unsafeAtomicAdd(&vt.pos[j][s], a.dpos[k][j]);
+        for (int j = 0; j < 8; ++j)
+            if (att[j] != 0.0f) unsafeAtomicAdd(&vt.att[j][s], att[j]);
+    }
+    if (!g.tex) return;
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+        const int s = lds_slot<kTexSlots>(tt.key, ((unsigned long long)m << 32) | a.tap[t]);
+        for (int j = 0; j < 3; ++j) {
+            if (a.dtex[t][j] == 0.0) continue;
+            if (s < 0) atomic_add(&g.tex[4u * a.tap[t] + j], a.dtex[t][j]);
+            else unsafeAtomicAdd(&tt.rgb[j][s], (float)a.dtex[t][j]);
+        }
+    }
+}
+
+// After a barrier: every filled slot adds its sums to global memory once.
+template <class SinkOf>
+__device__ __forceinline__ void tables_flush(const VtxTable &vt, const TexTable &tt, SinkOf sink_of) {
+    for (uint32_t s = threadIdx.x; s < kVtxSlots; s += 256u) {
+        const unsigned long long k = vt.key[s];
+        if (k == kEmptyKey) continue;
+        const GradSink g = sink_of((uint32_t)(k >> 32));
+        const double dpos[3] = {vt.pos[0][s], vt.pos[1][s], vt.pos[2][s]};
+        float att[8];
+        for (int c = 0; c < 8; ++c) att[c] = vt.att[c][s];
+        vertex_to_global(g, (uint32_t)k, dpos, att);
+    }
+    for (uint32_t s = threadIdx.x; s < kTexSlots; s += 256u) {
+        const unsigned long long k = tt.key[s];
+        if (k == kEmptyKey) continue;
+        const GradSink g = sink_of((uint32_t)(k >> 32));
+        for (int j = 0; j < 3; ++j) atomic_add(&g.tex[4u * (uint32_t)k + j], (double)tt.rgb[j][s]);
+    }
+}
+
+// Pixel of thread t in tile `tile` of a W-wide image (16x16 tiles, row-major).
+__device__ __forceinline__ void tile_pixel(uint32_t tile, uint32_t W, uint32_t &x, uint32_t &y) {
+    const uint32_t tx = (W + kTileSide - 1u) / kTileSide;
+    x = (tile % tx) * kTileSide + (threadIdx.x & (kTileSide - 1u));
+    y = (tile / tx) * kTileSide + threadIdx.x / kTileSide;
+}
+
 __global__ __launch_bounds__(256) void kd_backward_atomic(DiffParams p, const float4 *gout, DiffGrads g,
                                                           double *posacc, double *sh_part) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+    __shared__ VtxTable vt;
+    __shared__ TexTable tt;
+    tables_clear(vt, tt);
+    __syncthreads();
+    uint32_t x, y;
+    tile_pixel(blockIdx.x, p.v.W, x, y);
+    const GradSink sink{posacc, g.col, g.nrm, g.uv, g.tex};
     Eval<double> e;
     Adj a;
     bool live = false;
-    if (i < p.v.W * p.v.H) {
-        live = evaluate(p.m, p.v, p.sh, p.prim[i], i % p.v.W, i / p.v.W, e);
+    if (x < p.v.W && y < p.v.H) {
+        const uint32_t i = y * p.v.W + x;
+        live = evaluate(p.m, p.v, p.sh, p.prim[i], x, y, e);
         if (live) adjoint(p.m, p.v, p.sh, e, gout[i], a);
     }
     if (!live)
         for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
-    if (live) {
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const uint32_t v = e.vi[k];
-            for (int j = 0; j < 3; ++j) {
-                if (posacc) atomic_add(&posacc[3u * v + j], a.dpos[k][j]);
-                if (g.col) atomic_add(&g.col[3u * v + j], a.dcol[k][j]);
-                if (g.nrm) atomic_add(&g.nrm[3u * v + j], a.dn[k][j]);
-            }
-            if (g.uv) {
-                atomic_add(&g.uv[2u * v], a.duv[k][0]);
-                atomic_add(&g.uv[2u * v + 1u], a.duv[k][1]);
-            }
-        }
-        if (g.tex)
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                for (int j = 0; j < 3; ++j) atomic_add(&g.tex[4u * a.tap[t] + j], a.dtex[t][j]);
-    }
+    if (live) gather_pixel(vt, tt, sink, 0u, e, a);
+    __syncthreads();
+    tables_flush(vt, tt, [&](uint32_t) { return sink; });
     sh_block_sum(a.dsh, sh_part);
 }
 
@@ -558,11 +673,14 @@ __device__ __forceinline__ uint32_t pixel_mesh(const DiffBatchParams &P, const D
 }
 
 // Pixels with no primitive get (0, 0, 0, 0); pixels of draws mapped to no mesh are left as they are.
+// Each block covers one 16x16 tile of one view.
 __global__ __launch_bounds__(256) void kd_batch_forward(DiffBatchParams P) {
     const uint32_t vi = block_owner(P.view_block0, P.n_views, blockIdx.x);
     const DiffBatchView &V = P.views[vi];
-    const uint32_t i = (blockIdx.x - P.view_block0[vi]) * 256u + threadIdx.x;
-    if (i >= V.v.W * V.v.H) return;
+    uint32_t x, y;
+    tile_pixel(blockIdx.x - P.view_block0[vi], V.v.W, x, y);
+    if (x >= V.v.W || y >= V.v.H) return;
+    const uint32_t i = y * V.v.W + x;
     const uint32_t t = V.prim[i];
     if (t == kNone) {
         V.out[i] = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -571,51 +689,43 @@ __global__ __launch_bounds__(256) void kd_batch_forward(DiffBatchParams P) {
     const uint32_t mi = pixel_mesh(P, V, i);
     if (mi >= P.n_meshes) return;
     Eval<float> e;
-    V.out[i] = evaluate(P.meshes[mi].m, V.v, P.sh, t, i % V.v.W, i / V.v.W, e)
-                   ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f)
-                   : make_float4(0.f, 0.f, 0.f, 0.f);
+    V.out[i] = evaluate(P.meshes[mi].m, V.v, P.sh, t, x, y, e) ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f)
+                                                                : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__device__ __forceinline__ GradSink mesh_sink(const DiffBatchMesh &B) {
+    return GradSink{B.g.pos ? B.posacc : nullptr, B.g.col, B.g.nrm, B.g.uv, B.g.tex};
 }
 
 __global__ __launch_bounds__(256) void kd_batch_backward(DiffBatchParams P, double *sh_part) {
+    __shared__ VtxTable vt;
+    __shared__ TexTable tt;
+    tables_clear(vt, tt);
+    __syncthreads();
     const uint32_t vi = block_owner(P.view_block0, P.n_views, blockIdx.x);
     const DiffBatchView &V = P.views[vi];
-    const uint32_t i = (blockIdx.x - P.view_block0[vi]) * 256u + threadIdx.x;
+    uint32_t x, y;
+    tile_pixel(blockIdx.x - P.view_block0[vi], V.v.W, x, y);
     Eval<double> e;
     Adj a;
     bool live = false;
     uint32_t mi = kNone;
-    if (i < V.v.W * V.v.H) {
+    if (x < V.v.W && y < V.v.H) {
+        const uint32_t i = y * V.v.W + x;
         const uint32_t t = V.prim[i];
         if (t != kNone) {
             mi = pixel_mesh(P, V, i);
             if (mi < P.n_meshes) {
-                live = evaluate(P.meshes[mi].m, V.v, P.sh, t, i % V.v.W, i / V.v.W, e);
+                live = evaluate(P.meshes[mi].m, V.v, P.sh, t, x, y, e);
                 if (live) adjoint(P.meshes[mi].m, V.v, P.sh, e, V.gout[i], a);
             }
         }
     }
     if (!live)
         for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
-    if (live) {
-        const DiffBatchMesh &B = P.meshes[mi];
-#pragma unroll
-        for (int k = 0; k < 3; ++k) {
-            const uint32_t v = e.vi[k];
-            for (int j = 0; j < 3; ++j) {
-                if (B.g.pos) atomic_add(&B.posacc[3u * v + j], a.dpos[k][j]);
-                if (B.g.col) atomic_add(&B.g.col[3u * v + j], a.dcol[k][j]);
-                if (B.g.nrm) atomic_add(&B.g.nrm[3u * v + j], a.dn[k][j]);
-            }
-            if (B.g.uv) {
-                atomic_add(&B.g.uv[2u * v], a.duv[k][0]);
-                atomic_add(&B.g.uv[2u * v + 1u], a.duv[k][1]);
-            }
-        }
-        if (B.g.tex)
-#pragma unroll
-            for (int t = 0; t < 4; ++t)
-                for (int j = 0; j < 3; ++j) atomic_add(&B.g.tex[4u * a.tap[t] + j], a.dtex[t][j]);
-    }
+    if (live) gather_pixel(vt, tt, mesh_sink(P.meshes[mi]), mi, e, a);
+    __syncthreads();
+    tables_flush(vt, tt, [&](uint32_t m) { return mesh_sink(P.meshes[m]); });
     sh_block_sum(a.dsh, sh_part);
 }
 
@@ -642,6 +752,9 @@ __global__ __launch_bounds__(256) void kd_batch_pos_finalize(DiffBatchParams P) 
 }  // namespace
 
 uint32_t diff_blocks(uint32_t W, uint32_t H) { return (W * H + 255u) / 256u; }
+uint32_t diff_tile_blocks(uint32_t W, uint32_t H) {
+    return ((W + kTileSide - 1u) / kTileSide) * ((H + kTileSide - 1u) / kTileSide);
+}
 
 hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s) {
     const uint32_t nb = diff_blocks(p.v.W, p.v.H);
@@ -652,7 +765,7 @@ hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s) 
 
 hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const DiffGrads &g, const DiffScratch &sc,
                                 bool deterministic, hipStream_t s) {
-    const uint32_t nb = diff_blocks(p.v.W, p.v.H);
+    const uint32_t nb = deterministic ? diff_blocks(p.v.W, p.v.H) : diff_tile_blocks(p.v.W, p.v.H);
     hipError_t e;
     if ((e = zero(g.pos, 3ull * p.m.V, s)) != hipSuccess || (e = zero(g.col, 3ull * p.m.V, s)) != hipSuccess ||
         (e = zero(g.uv, 2ull * p.m.V, s)) != hipSuccess || (e = zero(g.nrm, 3ull * p.m.V, s)) != hipSuccess ||

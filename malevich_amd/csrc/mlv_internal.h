@@ -178,7 +178,7 @@ struct DiffGrads {
     float *pos, *col, *uv, *nrm, *tex, *sh;
 };
 // Batched shading of many views of many meshes in one launch. The grid covers every view's
-// pixels, 256 per block, each view starting on a block of its own (view_block0). A pixel's mesh
+// pixels, one 16x16 tile per block, each view starting on a block of its own (view_block0). A pixel's mesh
 // is draw_mesh[draw_off + tag - first_tag] for the draw tag the visibility buffer holds there.
 struct DiffBatchMesh {
     DiffMesh m;
@@ -225,7 +225,10 @@ struct DiffScratch {
 hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s);
 hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const DiffGrads &g, const DiffScratch &sc,
                                 bool deterministic, hipStream_t s);
+// Blocks of the per-pixel launches: 256 pixels in row order, or 16x16 tiles (the atomic
+// backward passes, which aggregate per tile, and the batched launches).
 uint32_t diff_blocks(uint32_t W, uint32_t H);
+uint32_t diff_tile_blocks(uint32_t W, uint32_t H);
 hipError_t launch_diff_batch_forward(const DiffBatchParams &p, hipStream_t s);
 // accumulate: keep the gradients and position sums of the previous call and add to them.
 hipError_t launch_diff_batch_backward(const DiffBatchParams &p, double *sh_part, float *grad_sh, bool accumulate,

@@ -1120,7 +1120,7 @@ MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, con
     RC_OR_RETURN(diff_params(ctx, scene, device_prim_ids, p));
     RC_OR_RETURN(set_device(ctx));
     RC_OR_RETURN(flush(ctx));
-    const uint32_t nb = diff_blocks(p.v.W, p.v.H);
+    const uint32_t nb = std::max(diff_blocks(p.v.W, p.v.H), diff_tile_blocks(p.v.W, p.v.H));
     const size_t P = (size_t)p.v.W * p.v.H;
     DiffScratch sc{};
     RC_OR_RETURN(ensure(ctx, ctx->d_shpart, (size_t)std::max<uint32_t>(nb, 1u) * 27 * 8));
@@ -1211,7 +1211,7 @@ static int diff_batch_setup(mlv_context *ctx, const mlv_diff_mesh *meshes, uint3
         o.out = (float4 *)w.rgba;
         o.gout = (const float4 *)w.grad_rgba;
         view_block0[v] = (uint32_t)blocks;
-        blocks += diff_blocks(w.width, w.height);
+        blocks += diff_tile_blocks(w.width, w.height);
         if (blocks > 0x7FFFFFFFull || draw_mesh.size() > 0x7FFFFFFFull)
             return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: too many pixels or draws");
     }
