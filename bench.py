@@ -541,6 +541,22 @@ def main(argv=None):
     print(f"[bench] timed {args.steps} frames: {ms_plain / args.steps:.3f} ms/frame "
           f"(with stage events {ms_total / args.steps:.3f})", file=sys.stderr, flush=True)
 
+    # the same frame captured once as a HIP graph and replayed
+    dev.graph_capture_begin()
+    frame(dev, res, sc)
+    graph = dev.graph_capture_end()
+    graph.launch()
+    dev.synchronize()
+    e4, e5 = ev.create(), ev.create()
+    ev.record(e4, dev.stream)
+    for _ in range(args.steps):
+        graph.launch()
+    ev.record(e5, dev.stream)
+    ms_graph = ev.elapsed_ms(e4, e5)
+    dev.synchronize()
+    graph.release()
+    print(f"[bench] graph replay: {ms_graph / args.steps:.3f} ms/frame", file=sys.stderr, flush=True)
+
     ms_step = ms_plain / args.steps
     tris = sc.triangle_count
     P_px = sc.width * sc.height
@@ -593,6 +609,7 @@ def main(argv=None):
         "gpix_per_s": round(gpix_s, 4),
         "stage_ms_per_frame": {k: round(v / args.steps, 4) for k, v in stage_ms.items()},
         "flushes_per_frame": flushes_per_frame,
+        "graph_replay_ms_per_frame": round(ms_graph / args.steps, 4),
         "assembled_triangles_per_frame": int(stats["assembled_triangle_count"] / args.steps),
         "fwd_bwd_ms": None,
         "batched_views_per_s": None,

@@ -204,6 +204,26 @@ MLV_API int mlv_get_profile(mlv_context *ctx, mlv_profile *out);
 MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out);
 MLV_API int mlv_reset_stats(mlv_context *ctx);
 
+/* ---- graphs (extension; no reference counterpart: the reference runs every draw_indexed to
+ * completion, main.c:1219-1261)
+ *
+ * Between mlv_graph_capture_begin and mlv_graph_capture_end, clears and draws are recorded
+ * instead of run, with the state they read resolved as they are issued; only state setters,
+ * clears and draws may be called meanwhile (anything that flushes returns
+ * MLV_ERROR_INVALID_STATE). capture_end runs the recorded commands once (sizing every scratch
+ * arena; this is the frame's first execution) and then captures the same commands into a HIP
+ * graph. mlv_graph_launch replays the whole frame as one graph launch on the context stream:
+ * the same targets, buffers and textures, whose contents the caller may change between
+ * launches (a new camera through mlv_buffer_update, new vertex data). Resources a graph names
+ * must outlive it; destroy graphs before their context. A replay whose geometry outgrows what
+ * was sized at capture reports MLV_ERROR_CAPACITY at the next synchronising call: capture
+ * again. */
+typedef struct mlv_graph mlv_graph;
+MLV_API int mlv_graph_capture_begin(mlv_context *ctx);
+MLV_API int mlv_graph_capture_end(mlv_context *ctx, mlv_graph **out);
+MLV_API int mlv_graph_launch(mlv_context *ctx, mlv_graph *graph);
+MLV_API int mlv_graph_destroy(mlv_context *ctx, mlv_graph *graph);
+
 /* ---- differentiable shading (extension; the reference renders forward only)
  *
  * Shading of a mesh with per-vertex colours, UVs and normals, an RGBA32F mipmapped texture and

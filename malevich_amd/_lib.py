@@ -130,6 +130,10 @@ SIGNATURES = {
     "mlv_context_set_profiling": (_I, [_P, _I]),
     "mlv_flush": (_I, [_P]),
     "mlv_get_profile": (_I, [_P, C.POINTER(Profile)]),
+    "mlv_graph_capture_begin": (_I, [_P]),
+    "mlv_graph_capture_end": (_I, [_P, _PP]),
+    "mlv_graph_launch": (_I, [_P, _P]),
+    "mlv_graph_destroy": (_I, [_P, _P]),
     "mlv_diff_forward": (_I, [_P, C.POINTER(DiffScene), _P, _P]),
     "mlv_diff_backward": (_I, [_P, C.POINTER(DiffScene), _P, _P, C.POINTER(DiffGrads), _I]),
     "mlv_diff_batch_forward": (_I, [_P, C.POINTER(DiffMesh), _U32, C.POINTER(DiffView), _U32, _P]),

@@ -147,6 +147,8 @@ struct BatchParams {
     uint32_t pair_cap;
     uint32_t *tile_cnt, *tile_start;      // per global tile: end and start of its range in the sorted pairs
     uint32_t *heavy;                      // tiles with more than kHeavyTile pairs (n_heavy of them)
+    uint32_t *scan_status;                // the scans' status words, cleared by the first stage
+    uint32_t scan_status_words;
     BatchCounters *ctr;
     DevStats *stats;
 };

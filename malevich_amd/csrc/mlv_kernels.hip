@@ -357,6 +357,8 @@ __global__ __launch_bounds__(256) void kb_reset(BatchParams p) {
         p.tile_start[i] = 0u;
         p.tile_cnt[i] = 0u;
     }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.scan_status_words; i += gridDim.x * blockDim.x)
+        p.scan_status[i] = 0u;
 }
 
 __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {

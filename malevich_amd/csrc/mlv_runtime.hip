@@ -77,6 +77,21 @@ struct PendingDraw {
     ViewKey key;
 };
 
+// A command recorded for a graph: a colour clear, a depth clear or a draw, with every piece of
+// state it reads resolved when it was issued.
+enum CmdKind { kCmdClearColor = 0, kCmdClearDepth = 1, kCmdDraw = 2 };
+struct Cmd {
+    int kind = 0;
+    mlv_render_target *rt = nullptr;
+    mlv_depth_stencil *ds = nullptr;
+    uint32_t value = 0;       // encoded clear colour
+    float depth = 0.0f;
+    int depth_func = 0;
+    uint32_t *vis_tri = nullptr, *vis_draw = nullptr, *vis_prim = nullptr;  // cleared with the colour
+    PendingDraw pd{};
+    uint32_t T = 0, V = 0;
+};
+
 // One flush never exceeds these (indices are 32-bit; the arenas stay bounded).
 constexpr uint64_t kMaxBatchTriangles = 48ull << 20;
 constexpr uint64_t kMaxBatchVertices = 48ull << 20;
@@ -136,6 +151,24 @@ struct mlv_context {
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double stage_ms[kStages] = {};
     uint64_t prof_draws = 0, prof_flushes = 0;
+    // graphs: commands recorded between capture_begin and capture_end, replayed once to size
+    // every arena and then under stream capture
+    bool recording = false;          // clears and draws are recorded, not run
+    bool capturing = false;          // the stream is being captured: no allocation, no host waits
+    bool sizing = false;             // the pass before capture: note the staging sizes
+    std::vector<Cmd> rec;
+    std::vector<size_t> cap_sizes;   // staging bytes of each table upload of the sizing pass
+    std::vector<void *> *cap_bufs = nullptr;  // pinned tables owned by the graph being captured
+    size_t cap_next = 0;
+    int graphs_alive = 0;
+    std::vector<void *> retired;     // arenas replaced while a graph may still read them
+};
+
+struct mlv_graph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    std::vector<void *> pinned;      // table uploads the graph's copy nodes read
+    uint64_t vertex_count = 0, input_tris = 0, draws = 0;
 };
 
 static int fail(mlv_context *ctx, int code, const char *fmt, ...) __attribute__((format(printf, 3, 4)));
@@ -167,11 +200,14 @@ static int fail(mlv_context *ctx, int code, const char *fmt, ...) {
 
 static int ensure(mlv_context *ctx, Arena &a, size_t bytes) {
     if (bytes <= a.cap) return MLV_OK;
+    if (ctx->capturing)
+        return fail(ctx, MLV_ERROR_CAPACITY, "graph capture needed more scratch than its sizing pass; capture again");
     size_t want = std::max(bytes, a.cap + a.cap / 2);
     want = (want + 255) & ~(size_t)255;
     if (a.p) {
         HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
-        HIP_OR_FAIL(ctx, hipFree(a.p));
+        if (ctx->graphs_alive) ctx->retired.push_back(a.p);  // a graph's kernels may still name it
+        else HIP_OR_FAIL(ctx, hipFree(a.p));
         a.p = nullptr;
         a.cap = 0;
     }
@@ -184,6 +220,11 @@ static void release(Arena &a) {
     if (a.p) (void)hipFree(a.p);
     a.p = nullptr;
     a.cap = 0;
+}
+
+static void free_retired(mlv_context *ctx) {
+    for (void *p : ctx->retired) (void)hipFree(p);
+    ctx->retired.clear();
 }
 
 static int set_device(mlv_context *ctx) {
@@ -229,6 +270,14 @@ static void screen_matrix(const mlv_viewport &vp, float S[4][4]) {
 // Pinned staging for tables copied to the device: a ring of slots, each reused only after the
 // copies queued from it have completed (staging_commit records that point on the stream).
 static int staging_acquire(mlv_context *ctx, size_t bytes, void **host, int *slot_out) {
+    if (ctx->capturing) {  // the graph's own buffer, filled once and read by every replay
+        if (!ctx->cap_bufs || ctx->cap_next >= ctx->cap_bufs->size() || ctx->cap_sizes[ctx->cap_next] < bytes)
+            return fail(ctx, MLV_ERROR_CAPACITY, "graph capture uploads differ from its sizing pass; capture again");
+        *host = (*ctx->cap_bufs)[ctx->cap_next++];
+        *slot_out = -1;
+        return MLV_OK;
+    }
+    if (ctx->sizing) ctx->cap_sizes.push_back(bytes);
     const int slot = ctx->staging_next;
     ctx->staging_next = (slot + 1) % kStagingSlots;
     if (ctx->staging_ev[slot]) HIP_OR_FAIL(ctx, hipEventSynchronize(ctx->staging_ev[slot]));
@@ -248,11 +297,13 @@ static int staging_acquire(mlv_context *ctx, size_t bytes, void **host, int *slo
 }
 
 static int staging_commit(mlv_context *ctx, int slot) {
-    HIP_OR_FAIL(ctx, hipEventRecord(ctx->staging_ev[slot], ctx->stream));
+    if (slot >= 0) HIP_OR_FAIL(ctx, hipEventRecord(ctx->staging_ev[slot], ctx->stream));
     return MLV_OK;
 }
 
 static int flush(mlv_context *ctx) {
+    if (ctx->recording)
+        return fail(ctx, MLV_ERROR_INVALID_STATE, "only state, clears and draws may be issued while recording a graph");
     if (ctx->pending.empty()) return MLV_OK;
     RC_OR_RETURN(set_device(ctx));
     std::vector<PendingDraw> draws;
@@ -339,12 +390,7 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->tile_cnt, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->tile_start, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->heavy, (size_t)(total_tiles + 1) * 4));
-    {
-        const void *before = ctx->scan_part.p;
-        RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_scratch_bytes(scan_n)));
-        if (ctx->scan_part.p != before)  // the scans' ticket and status words start at zero
-            HIP_OR_FAIL(ctx, hipMemsetAsync(ctx->scan_part.p, 0, ctx->scan_part.cap, ctx->stream));
-    }
+    RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_scratch_bytes(scan_n)));  // cleared by the first stage
     RC_OR_RETURN(ensure(ctx, ctx->radix_hist, radix_hist_elems(pair_cap) * 4));
 
     // tables through pinned staging
@@ -405,29 +451,35 @@ static int flush(mlv_context *ctx) {
     p.heavy = (uint32_t *)ctx->heavy.p;
     p.ctr = ctx->ctr;
     p.stats = ctx->dstats;
+    // The first stage clears the scan status words, so epochs only need to differ between the
+    // scans of one flush: they count from 1 in every flush, and a replayed graph reuses them.
+    uint32_t flush_epoch = 0;
+    p.scan_status = (uint32_t *)ctx->scan_part.p;
+    p.scan_status_words = (uint32_t)(scan_scratch_bytes(scan_n) / 4);
     ScanScratch sc{ctx->scan_part.p, ctx->scan_part.cap, (uint32_t *)ctx->radix_hist.p, ctx->radix_hist.cap / 4,
-                   &ctx->scan_epoch, &ctx->dstats->scan_errors};
+                   &flush_epoch, &ctx->dstats->scan_errors};
 
     // keys are global tile indices < total_tiles: one 8-bit pass per byte they use
     uint32_t bits = 0;
     while (bits < 32 && (1ull << bits) < total_tiles) ++bits;
     const uint32_t passes = std::max<uint32_t>(1u, (bits + 7u) / 8u);
 
+    const bool timing = ctx->profiling && !ctx->capturing;
     for (int st = 0; st < kStages; ++st) {
         hipEvent_t a = nullptr, b = nullptr;
-        if (ctx->profiling) {
+        if (timing) {
             a = take_event(ctx);
             b = take_event(ctx);
             if (!a || !b) return fail(ctx, MLV_ERROR_DEVICE, "hipEventCreate failed");
             HIP_OR_FAIL(ctx, hipEventRecord(a, ctx->stream));
         }
         HIP_OR_FAIL(ctx, launch_stage(st, p, sc, passes, ctx->stream));
-        if (ctx->profiling) {
+        if (timing) {
             HIP_OR_FAIL(ctx, hipEventRecord(b, ctx->stream));
             ctx->ev_pending.push_back({st, {a, b}});
         }
     }
-    if (ctx->profiling) {
+    if (timing) {
         ctx->prof_draws += n_draws;
         ctx->prof_flushes++;
         if (ctx->ev_pending.size() > 4096) {
@@ -515,6 +567,7 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
                     &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart};
     for (Arena *a : all) release(*a);
+    free_retired(ctx);
     resolve_events(ctx);
     for (hipEvent_t e : ctx->ev_pool) (void)hipEventDestroy(e);
     for (int i = 0; i < kStagingSlots; ++i) {
@@ -836,32 +889,61 @@ static uint32_t trunc_u32(float f) {
     return (uint32_t)(int64_t)f;
 }
 
+static int exec_clear_color(mlv_context *ctx, const Cmd &c) {
+    RC_OR_RETURN(flush_if_targets(ctx, c.rt, nullptr));
+    const size_t n = (size_t)c.rt->width * c.rt->height;
+    HIP_OR_FAIL(ctx, launch_fill_u32(c.rt->ptr, c.value, n, ctx->stream));
+    if (c.vis_tri) {
+        HIP_OR_FAIL(ctx, launch_fill_u32(c.vis_tri, 0xFFFFFFFFu, n, ctx->stream));
+        HIP_OR_FAIL(ctx, launch_fill_u32(c.vis_draw, 0xFFFFFFFFu, n, ctx->stream));
+    }
+    if (c.vis_prim) HIP_OR_FAIL(ctx, launch_fill_u32(c.vis_prim, 0xFFFFFFFFu, n, ctx->stream));
+    return MLV_OK;
+}
+
+static int exec_clear_depth(mlv_context *ctx, const Cmd &c) {
+    RC_OR_RETURN(flush_if_targets(ctx, nullptr, c.ds));
+    HIP_OR_FAIL(ctx, launch_fill_f32(c.ds->ptr, c.depth, (size_t)c.ds->width * c.ds->height, ctx->stream));
+    // main.c:1212-1214: every tile minimum back to 0.0 (1.0 for the LESS_EQUAL mirror)
+    const float h = c.depth_func == MLV_DEPTH_LESS_EQUAL ? 1.0f : 0.0f;
+    HIP_OR_FAIL(ctx, launch_fill_f32(c.ds->hiz, h, (size_t)c.ds->bins_x * c.ds->bins_y, ctx->stream));
+    c.ds->hiz_mode = c.depth_func;
+    return MLV_OK;
+}
+
 MLV_API int mlv_clear_render_target_view(mlv_context *ctx, mlv_render_target *rt, const float rgba[4]) {
     if (!ctx || !rt || !rgba) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "clear_render_target_view: bad argument");
     RC_OR_RETURN(set_device(ctx));
-    RC_OR_RETURN(flush_if_targets(ctx, rt, nullptr));
-    uint32_t v = (trunc_u32(rgba[3] * 255.f) << 24) + (trunc_u32(rgba[2] * 255.f) << 16) +
-                 (trunc_u32(rgba[1] * 255.f) << 8) + trunc_u32(rgba[0] * 255.f);
-    HIP_OR_FAIL(ctx, launch_fill_u32(rt->ptr, v, (size_t)rt->width * rt->height, ctx->stream));
+    Cmd c;
+    c.kind = kCmdClearColor;
+    c.rt = rt;
+    c.value = (trunc_u32(rgba[3] * 255.f) << 24) + (trunc_u32(rgba[2] * 255.f) << 16) +
+              (trunc_u32(rgba[1] * 255.f) << 8) + trunc_u32(rgba[0] * 255.f);
     if (ctx->vis_tri && ctx->rt == rt) {
-        HIP_OR_FAIL(ctx, launch_fill_u32(ctx->vis_tri, 0xFFFFFFFFu, (size_t)rt->width * rt->height, ctx->stream));
-        HIP_OR_FAIL(ctx, launch_fill_u32(ctx->vis_draw, 0xFFFFFFFFu, (size_t)rt->width * rt->height, ctx->stream));
+        c.vis_tri = ctx->vis_tri;
+        c.vis_draw = ctx->vis_draw;
     }
-    if (ctx->vis_prim && ctx->rt == rt)
-        HIP_OR_FAIL(ctx, launch_fill_u32(ctx->vis_prim, 0xFFFFFFFFu, (size_t)rt->width * rt->height, ctx->stream));
-    return MLV_OK;
+    if (ctx->vis_prim && ctx->rt == rt) c.vis_prim = ctx->vis_prim;
+    if (ctx->recording) {
+        ctx->rec.push_back(c);
+        return MLV_OK;
+    }
+    return exec_clear_color(ctx, c);
 }
 
 MLV_API int mlv_clear_depth_stencil_view(mlv_context *ctx, mlv_depth_stencil *ds, float depth) {
     if (!ctx || !ds) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "clear_depth_stencil_view: bad argument");
     RC_OR_RETURN(set_device(ctx));
-    RC_OR_RETURN(flush_if_targets(ctx, nullptr, ds));
-    HIP_OR_FAIL(ctx, launch_fill_f32(ds->ptr, depth, (size_t)ds->width * ds->height, ctx->stream));
-    // main.c:1212-1214: every tile minimum back to 0.0 (1.0 for the LESS_EQUAL mirror)
-    const float h = ctx->depth_func == MLV_DEPTH_LESS_EQUAL ? 1.0f : 0.0f;
-    HIP_OR_FAIL(ctx, launch_fill_f32(ds->hiz, h, (size_t)ds->bins_x * ds->bins_y, ctx->stream));
-    ds->hiz_mode = ctx->depth_func;
-    return MLV_OK;
+    Cmd c;
+    c.kind = kCmdClearDepth;
+    c.ds = ds;
+    c.depth = depth;
+    c.depth_func = ctx->depth_func;
+    if (ctx->recording) {
+        ctx->rec.push_back(c);
+        return MLV_OK;
+    }
+    return exec_clear_depth(ctx, c);
 }
 
 static bool is_float_tex(const mlv_texture2d *t) {
@@ -878,6 +960,31 @@ static TexDesc tex_desc(const mlv_texture2d *t) {
         d.mip_levels = t->mip_levels;
     }
     return d;
+}
+
+// Adds a draw to the pending batch, flushing first where the batch cannot take it.
+static int submit_draw(mlv_context *ctx, const Cmd &c) {
+    const PendingDraw &pd = c.pd;
+    mlv_depth_stencil *ds = pd.key.ds;
+    // one depth function per target within a flush, and hierarchical-Z built for it
+    for (auto &q : ctx->pending)
+        if ((q.key.rt == pd.key.rt || q.key.ds == pd.key.ds) && !(q.key == pd.key)) {
+            RC_OR_RETURN(flush(ctx));
+            break;
+        }
+    if (ds->hiz_mode != pd.key.depth_func) {
+        RC_OR_RETURN(flush_if_targets(ctx, nullptr, ds));
+        const float h = pd.key.depth_func == MLV_DEPTH_LESS_EQUAL ? HUGE_VALF : -HUGE_VALF;  // rejects nothing
+        HIP_OR_FAIL(ctx, launch_fill_f32(ds->hiz, h, (size_t)ds->bins_x * ds->bins_y, ctx->stream));
+        ds->hiz_mode = pd.key.depth_func;
+    }
+    if (ctx->pending.size() >= kMaxBatchDraws || ctx->pending_tris + align256(c.T) > kMaxBatchTriangles ||
+        ctx->pending_verts + align256(c.V) > kMaxBatchVertices)
+        RC_OR_RETURN(flush(ctx));
+    ctx->pending.push_back(pd);
+    ctx->pending_tris += align256(c.T);
+    ctx->pending_verts += align256(c.V);
+    return MLV_OK;
 }
 
 MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t start_index, int32_t base_vertex) {
@@ -921,24 +1028,12 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
     if (T == 0) return MLV_OK;
 
     mlv_depth_stencil *ds = ctx->ds;
-    ViewKey key{ctx->rt, ds, ctx->vis_tri, ctx->vis_draw, ctx->vis_prim, ctx->depth_func};
-    // one depth function per target within a flush, and hierarchical-Z built for it
-    for (auto &pd : ctx->pending)
-        if ((pd.key.rt == key.rt || pd.key.ds == key.ds) && !(pd.key == key)) {
-            RC_OR_RETURN(flush(ctx));
-            break;
-        }
-    if (ds->hiz_mode != ctx->depth_func) {
-        RC_OR_RETURN(flush_if_targets(ctx, nullptr, ds));
-        const float h = ctx->depth_func == MLV_DEPTH_LESS_EQUAL ? HUGE_VALF : -HUGE_VALF;  // rejects nothing
-        HIP_OR_FAIL(ctx, launch_fill_f32(ds->hiz, h, (size_t)ds->bins_x * ds->bins_y, ctx->stream));
-        ds->hiz_mode = ctx->depth_func;
-    }
-    if (ctx->pending.size() >= kMaxBatchDraws || ctx->pending_tris + align256(T) > kMaxBatchTriangles ||
-        ctx->pending_verts + align256(V) > kMaxBatchVertices)
-        RC_OR_RETURN(flush(ctx));
-
-    PendingDraw pd{};
+    Cmd c;
+    c.kind = kCmdDraw;
+    c.T = T;
+    c.V = V;
+    PendingDraw &pd = c.pd;
+    pd.key = ViewKey{ctx->rt, ds, ctx->vis_tri, ctx->vis_draw, ctx->vis_prim, ctx->depth_func};
     DrawRec &r = pd.rec;
     r.ib = (const uint32_t *)((const uint8_t *)ctx->ib->ptr + ctx->ib_off) + start_index;
     r.vb = (const uint8_t *)ctx->vb->ptr + ctx->vb_off;
@@ -961,10 +1056,121 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
                        ? 1
                        : 0;
     r.draw_tag = draw_tag;
-    pd.key = key;
-    ctx->pending.push_back(pd);
-    ctx->pending_tris += align256(T);
-    ctx->pending_verts += align256(V);
+    if (ctx->recording) {
+        ctx->rec.push_back(c);
+        return MLV_OK;
+    }
+    return submit_draw(ctx, c);
+}
+
+// ------------------------------------------------------------------ graphs
+
+static int run_recorded(mlv_context *ctx, const std::vector<Cmd> &rec) {
+    for (const Cmd &c : rec) {
+        switch (c.kind) {
+        case kCmdClearColor: RC_OR_RETURN(exec_clear_color(ctx, c)); break;
+        case kCmdClearDepth: RC_OR_RETURN(exec_clear_depth(ctx, c)); break;
+        default: RC_OR_RETURN(submit_draw(ctx, c)); break;
+        }
+    }
+    return flush(ctx);
+}
+
+MLV_API int mlv_graph_capture_begin(mlv_context *ctx) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (ctx->recording) return fail(ctx, MLV_ERROR_INVALID_STATE, "graph_capture_begin: already recording");
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // earlier draws stay outside the graph
+    ctx->rec.clear();
+    ctx->recording = true;
+    return MLV_OK;
+}
+
+MLV_API int mlv_graph_capture_end(mlv_context *ctx, mlv_graph **out) {
+    if (!ctx || !out) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "graph_capture_end: bad argument");
+    *out = nullptr;
+    if (!ctx->recording) return fail(ctx, MLV_ERROR_INVALID_STATE, "graph_capture_end without graph_capture_begin");
+    ctx->recording = false;
+    std::vector<Cmd> rec;
+    rec.swap(ctx->rec);
+    RC_OR_RETURN(set_device(ctx));
+    // 1. run the commands once: sizes every arena and yields this frame's results
+    int rc = MLV_OK;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        ctx->sizing = true;
+        ctx->cap_sizes.clear();
+        rc = run_recorded(ctx, rec);
+        ctx->sizing = false;
+        if (rc == MLV_OK) {
+            HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+            rc = check_device_stats(ctx);
+        }
+        if (rc != MLV_ERROR_CAPACITY) break;  // a capacity miss raised the capacity: size again
+    }
+    if (rc != MLV_OK) return rc;
+    // 2. run them again under stream capture, uploading tables from buffers the graph owns
+    mlv_graph *g = new (std::nothrow) mlv_graph();
+    if (!g) return MLV_ERROR_OUT_OF_MEMORY;
+    auto drop = [&](int code) {
+        for (void *h : g->pinned) (void)hipHostFree(h);
+        if (g->exec) (void)hipGraphExecDestroy(g->exec);
+        if (g->graph) (void)hipGraphDestroy(g->graph);
+        delete g;
+        return code;
+    };
+    for (size_t b : ctx->cap_sizes) {
+        void *h = nullptr;
+        if (hipHostMalloc(&h, std::max<size_t>(b, 64), hipHostMallocDefault) != hipSuccess)
+            return drop(fail(ctx, MLV_ERROR_OUT_OF_MEMORY, "graph_capture_end: pinned table buffer"));
+        g->pinned.push_back(h);
+    }
+    if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) != hipSuccess)
+        return drop(fail(ctx, MLV_ERROR_DEVICE, "hipStreamBeginCapture failed"));
+    ctx->capturing = true;
+    ctx->cap_bufs = &g->pinned;
+    ctx->cap_next = 0;
+    rc = run_recorded(ctx, rec);
+    ctx->capturing = false;
+    ctx->cap_bufs = nullptr;
+    const hipError_t e = hipStreamEndCapture(ctx->stream, &g->graph);
+    if (rc != MLV_OK) return drop(rc);
+    if (e != hipSuccess) return drop(fail(ctx, MLV_ERROR_DEVICE, "hipStreamEndCapture: %s", hipGetErrorString(e)));
+    const hipError_t ei = hipGraphInstantiate(&g->exec, g->graph, nullptr, nullptr, 0);
+    if (ei != hipSuccess) return drop(fail(ctx, MLV_ERROR_DEVICE, "hipGraphInstantiate: %s", hipGetErrorString(ei)));
+    for (const Cmd &c : rec)
+        if (c.kind == kCmdDraw) {
+            g->draws++;
+            g->input_tris += c.T;
+            g->vertex_count += 3ull * c.T;
+        }
+    ctx->graphs_alive++;
+    *out = g;
+    return MLV_OK;
+}
+
+MLV_API int mlv_graph_launch(mlv_context *ctx, mlv_graph *g) {
+    if (!ctx || !g) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "graph_launch: bad argument");
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    HIP_OR_FAIL(ctx, hipGraphLaunch(g->exec, ctx->stream));
+    ctx->vertex_count += g->vertex_count;
+    ctx->input_tris += g->input_tris;
+    ctx->draws += g->draws;
+    return MLV_OK;
+}
+
+MLV_API int mlv_graph_destroy(mlv_context *ctx, mlv_graph *g) {
+    if (!ctx || !g) return MLV_ERROR_INVALID_ARGUMENT;
+    (void)hipSetDevice(ctx->device);
+    (void)hipStreamSynchronize(ctx->stream);
+    (void)hipGraphExecDestroy(g->exec);
+    (void)hipGraphDestroy(g->graph);
+    for (void *h : g->pinned) (void)hipHostFree(h);
+    delete g;
+    if (--ctx->graphs_alive <= 0) {
+        ctx->graphs_alive = 0;
+        free_retired(ctx);
+    }
     return MLV_OK;
 }
 

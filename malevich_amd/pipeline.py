@@ -103,6 +103,21 @@ class DepthStencil:
             self.handle = None
 
 
+class Graph:
+    """A recorded frame captured as a HIP graph (mlv_graph_*)."""
+
+    def __init__(self, dev: "Device", handle: int):
+        self.dev, self.handle = dev, handle
+
+    def launch(self) -> None:
+        _check(self.dev.ctx, L.load().mlv_graph_launch(self.dev.ctx, self.handle))
+
+    def release(self) -> None:
+        if self.handle:
+            L.load().mlv_graph_destroy(self.dev.ctx, self.handle)
+            self.handle = None
+
+
 class Device:
     """One mlv_context: the pipeline state plus one HIP stream on one GPU."""
 
@@ -257,3 +272,14 @@ class Device:
 
     def reset_stats(self) -> None:
         _check(self.ctx, self.lib.mlv_reset_stats(self.ctx))
+
+    # ---------------------------------------------------------- graphs
+    def graph_capture_begin(self) -> None:
+        """Record the following clears and draws instead of running them."""
+        _check(self.ctx, self.lib.mlv_graph_capture_begin(self.ctx))
+
+    def graph_capture_end(self) -> Graph:
+        """Run the recorded frame once and capture it as a graph to replay with Graph.launch()."""
+        h = C.c_void_p()
+        _check(self.ctx, self.lib.mlv_graph_capture_end(self.ctx, C.byref(h)))
+        return Graph(self, h.value)
