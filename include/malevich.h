@@ -224,6 +224,26 @@ MLV_API int mlv_graph_capture_end(mlv_context *ctx, mlv_graph **out);
 MLV_API int mlv_graph_launch(mlv_context *ctx, mlv_graph *graph);
 MLV_API int mlv_graph_destroy(mlv_context *ctx, mlv_graph *graph);
 
+/* ---- launch tuning (extension; no reference counterpart)
+ *
+ * The per-context launch parameters of the tile and binning stages. mlv_autotune_begin /
+ * mlv_autotune_end record a frame exactly as a graph capture does (only state, clears and
+ * draws in between), then run it repeatedly under candidate parameters (coordinate descent:
+ * heavy-tile threshold, then heavy-tile blocks, then pair blocks), timing each with HIP events
+ * on the context stream, and keep the fastest. Results are the same under every candidate;
+ * the targets hold the recorded frame afterwards. The bin-list capacity follows the largest
+ * pair count measured (with a quarter of headroom) and is reported in pair_capacity. */
+typedef struct mlv_tuning {
+    uint32_t heavy_tile_pairs; /* a 16x16 tile with a longer (triangle, tile) list is walked by 16 waves */
+    uint32_t heavy_blocks;     /* blocks (1024 threads) of the persistent heavy-tile kernel */
+    uint32_t pair_blocks;      /* blocks (256 threads) of the pair count and emit kernels */
+    uint32_t pair_capacity;    /* out: bin-list capacity (pairs) of the last flush; ignored by set */
+} mlv_tuning;
+MLV_API int mlv_get_tuning(mlv_context *ctx, mlv_tuning *out);
+MLV_API int mlv_set_tuning(mlv_context *ctx, const mlv_tuning *tuning);
+MLV_API int mlv_autotune_begin(mlv_context *ctx);
+MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *best);
+
 /* ---- differentiable shading (extension; the reference renders forward only)
  *
  * Shading of a mesh with per-vertex colours, UVs and normals, an RGBA32F mipmapped texture and

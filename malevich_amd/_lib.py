@@ -65,6 +65,11 @@ class DiffGrads(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("positions", "colors", "uvs", "normals", "texture", "sh")]
 
 
+class Tuning(C.Structure):
+    """mlv_tuning"""
+    _fields_ = [(n, C.c_uint32) for n in ("heavy_tile_pairs", "heavy_blocks", "pair_blocks", "pair_capacity")]
+
+
 class DiffMesh(C.Structure):
     """mlv_diff_mesh"""
     _fields_ = [
@@ -130,6 +135,10 @@ SIGNATURES = {
     "mlv_context_set_profiling": (_I, [_P, _I]),
     "mlv_flush": (_I, [_P]),
     "mlv_get_profile": (_I, [_P, C.POINTER(Profile)]),
+    "mlv_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
+    "mlv_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
+    "mlv_autotune_begin": (_I, [_P]),
+    "mlv_autotune_end": (_I, [_P, _U32, C.POINTER(Tuning)]),
     "mlv_graph_capture_begin": (_I, [_P]),
     "mlv_graph_capture_end": (_I, [_P, _PP]),
     "mlv_graph_launch": (_I, [_P, _P]),

@@ -30,7 +30,10 @@ constexpr int kBlock = 256;        // threads per block of the vertex and setup 
 constexpr int kMaxClip = 16;       // MAX_NUM_CLIP_VERTICES (main.c:38)
 constexpr int kBigRect = 32;       // a triangle spanning more tiles is binned by the whole block
 constexpr int kStages = 4;         // vertex, setup, binning, tile
-constexpr uint32_t kHeavyTile = 384;  // a tile with a longer list has its bins walked by 16 waves each
+// Launch parameters the autotuner picks (defaults below): a tile whose list is longer than
+// heavy_min has its bins walked by 16 waves each (4 per bin); the persistent heavy-tile kernel
+// runs heavy_blocks blocks of 1024 threads; the pair count/emit kernels run pair_blocks blocks.
+constexpr uint32_t kHeavyMinDefault = 384, kHeavyBlocksDefault = 1024, kPairBlocksDefault = 2048;
 
 enum VsKind { VS_BASIC = 1, VS_VERTEX_LIGHTING = 2, VS_PASSTHROUGH = 3, VS_FULLSCREEN = 4 };
 enum PsKind { PS_BASIC = 1, PS_ENV_LIGHTING = 2, PS_PASSTHROUGH = 3 };
@@ -113,7 +116,7 @@ struct BatchCounters {
     uint32_t overflow;
     uint32_t radix_blocks;                 // blocks of the radix passes over n_pairs
     uint32_t hist_n;                       // radix_blocks * 256 digit counters
-    uint32_t n_heavy;                      // tiles whose list is longer than kHeavyTile
+    uint32_t n_heavy;                      // tiles whose list is longer than heavy_min
 };
 
 struct DevStats {
@@ -146,7 +149,8 @@ struct BatchParams {
     uint32_t *keys0, *vals0, *keys1, *vals1;
     uint32_t pair_cap;
     uint32_t *tile_cnt, *tile_start;      // per global tile: end and start of its range in the sorted pairs
-    uint32_t *heavy;                      // tiles with more than kHeavyTile pairs (n_heavy of them)
+    uint32_t *heavy;                      // tiles with more than heavy_min pairs (n_heavy of them)
+    uint32_t heavy_min, heavy_blocks, pair_blocks;
     uint32_t *scan_status;                // the scans' status words, cleared by the first stage
     uint32_t scan_status_words;
     BatchCounters *ctr;

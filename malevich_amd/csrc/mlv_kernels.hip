@@ -965,7 +965,7 @@ constexpr uint32_t kHeavyDraws = 1;
 
 __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g) {
     const uint32_t a = p.tile_start[g], e = p.tile_cnt[g];
-    if (e - a <= kHeavyTile) return false;
+    if (e - a <= p.heavy_min) return false;
     const uint32_t d0 = upper_index(p.n_draws, sorted_vals[a], [&](uint32_t i) { return p.asm_base[i]; });
     const uint32_t d1 = upper_index(p.n_draws, sorted_vals[e - 1u], [&](uint32_t i) { return p.asm_base[i]; });
     return d1 - d0 < kHeavyDraws;
@@ -1085,7 +1085,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
     }
 }
 
-// A heavy tile: a sorted list longer than kHeavyTile whose triangles all belong to one draw
+// A heavy tile: a sorted list longer than heavy_min whose triangles all belong to one draw
 // (kHeavyDraws), e.g. where many small triangles of one mesh converge or a stress draw covers
 // the screen. Lists that span draws stay with kb_tile, whose single wave per bin keeps the
 // Hi-Z refresh between draws in order.
@@ -1320,7 +1320,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         break;
     case 2: {
         if (!p.n_tblocks) break;
-        const uint32_t grid = std::min<uint32_t>(p.n_tblocks, 2048u);
+        const uint32_t grid = std::max(1u, std::min<uint32_t>(p.n_tblocks, p.pair_blocks));
         hipLaunchKernelGGL(kb_pairs<false>, dim3(grid), dim3(256), 0, s, p);
         if ((e = launch_scan<uint32_t>(p.pair_cnt, p.pair_off, p.tri_cap, &p.ctr->n_assembled, sc,
                                        &p.ctr->n_pairs, s)) != hipSuccess)
@@ -1349,8 +1349,8 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         const uint32_t *sorted = (radix_passes & 1u) ? p.vals1 : p.vals0;
         hipLaunchKernelGGL(kb_heavy_tiles, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 256u)), dim3(256), 0,
                            s, p, sorted);
-        hipLaunchKernelGGL(kb_tile_heavy, dim3(std::min<uint32_t>(p.total_tiles, 1024u)), dim3(1024), 0, s, p,
-                           sorted);
+        hipLaunchKernelGGL(kb_tile_heavy, dim3(std::max(1u, std::min<uint32_t>(p.total_tiles, p.heavy_blocks))),
+                           dim3(1024), 0, s, p, sorted);
         hipLaunchKernelGGL(kb_tile, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
         hipLaunchKernelGGL(kb_active_sum, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 64u)), dim3(256), 0,
                            s, p);

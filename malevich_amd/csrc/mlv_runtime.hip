@@ -162,6 +162,9 @@ struct mlv_context {
     size_t cap_next = 0;
     int graphs_alive = 0;
     std::vector<void *> retired;     // arenas replaced while a graph may still read them
+    // launch parameters (mlv_tuning)
+    uint32_t heavy_min = kHeavyMinDefault, heavy_blocks = kHeavyBlocksDefault, pair_blocks = kPairBlocksDefault;
+    size_t last_pair_cap = 0;
 };
 
 struct mlv_graph {
@@ -449,6 +452,10 @@ static int flush(mlv_context *ctx) {
     p.tile_cnt = (uint32_t *)ctx->tile_cnt.p;
     p.tile_start = (uint32_t *)ctx->tile_start.p;
     p.heavy = (uint32_t *)ctx->heavy.p;
+    p.heavy_min = ctx->heavy_min;
+    p.heavy_blocks = ctx->heavy_blocks;
+    p.pair_blocks = ctx->pair_blocks;
+    ctx->last_pair_cap = pair_cap;
     p.ctr = ctx->ctr;
     p.stats = ctx->dstats;
     // The first stage clears the scan status words, so epochs only need to differ between the
@@ -1172,6 +1179,100 @@ MLV_API int mlv_graph_destroy(mlv_context *ctx, mlv_graph *g) {
         free_retired(ctx);
     }
     return MLV_OK;
+}
+
+// ------------------------------------------------------------------ tuning
+
+MLV_API int mlv_get_tuning(mlv_context *ctx, mlv_tuning *out) {
+    if (!ctx || !out) return MLV_ERROR_INVALID_ARGUMENT;
+    out->heavy_tile_pairs = ctx->heavy_min;
+    out->heavy_blocks = ctx->heavy_blocks;
+    out->pair_blocks = ctx->pair_blocks;
+    out->pair_capacity = (uint32_t)std::min<size_t>(ctx->last_pair_cap, 0xFFFFFFFFu);
+    return MLV_OK;
+}
+
+MLV_API int mlv_set_tuning(mlv_context *ctx, const mlv_tuning *t) {
+    if (!ctx || !t) return MLV_ERROR_INVALID_ARGUMENT;
+    if (!t->heavy_blocks || t->heavy_blocks > 65536u || !t->pair_blocks || t->pair_blocks > 65536u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "set_tuning: block counts must be 1..65536");
+    RC_OR_RETURN(flush(ctx));  // recorded draws keep the parameters they were recorded under
+    ctx->heavy_min = t->heavy_tile_pairs;
+    ctx->heavy_blocks = t->heavy_blocks;
+    ctx->pair_blocks = t->pair_blocks;
+    return MLV_OK;
+}
+
+MLV_API int mlv_autotune_begin(mlv_context *ctx) { return mlv_graph_capture_begin(ctx); }
+
+// Milliseconds per run of the recorded commands under the context's current parameters.
+static int time_recorded(mlv_context *ctx, const std::vector<Cmd> &rec, uint32_t repeats, double *ms) {
+    RC_OR_RETURN(run_recorded(ctx, rec));  // warm-up under these parameters
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    hipEvent_t a = nullptr, b = nullptr;
+    HIP_OR_FAIL(ctx, hipEventCreate(&a));
+    if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return fail(ctx, MLV_ERROR_DEVICE, "hipEventCreate failed");
+    }
+    int rc = hipEventRecord(a, ctx->stream) == hipSuccess ? MLV_OK : MLV_ERROR_DEVICE;
+    for (uint32_t r = 0; r < repeats && rc == MLV_OK; ++r) rc = run_recorded(ctx, rec);
+    float t = 0.0f;
+    if (rc == MLV_OK && (hipEventRecord(b, ctx->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                         hipEventElapsedTime(&t, a, b) != hipSuccess))
+        rc = fail(ctx, MLV_ERROR_DEVICE, "autotune timing failed");
+    (void)hipEventDestroy(a);
+    (void)hipEventDestroy(b);
+    RC_OR_RETURN(rc);
+    RC_OR_RETURN(check_device_stats(ctx));
+    *ms = (double)t / repeats;
+    return MLV_OK;
+}
+
+MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *best) {
+    if (!ctx || !best || !repeats) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "autotune_end: bad argument");
+    if (!ctx->recording) return fail(ctx, MLV_ERROR_INVALID_STATE, "autotune_end without autotune_begin");
+    ctx->recording = false;
+    std::vector<Cmd> rec;
+    rec.swap(ctx->rec);
+    RC_OR_RETURN(set_device(ctx));
+    // size the arenas (a capacity miss raises the capacity; run again)
+    int rc = MLV_OK;
+    for (int attempt = 0; attempt < 3; ++attempt) {
+        rc = run_recorded(ctx, rec);
+        if (rc == MLV_OK) {
+            HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+            rc = check_device_stats(ctx);
+        }
+        if (rc != MLV_ERROR_CAPACITY) break;
+    }
+    RC_OR_RETURN(rc);
+    static const uint32_t heavy_min[] = {192u, 384u, 768u, 1536u, 0xFFFFFFFFu};
+    static const uint32_t heavy_blocks[] = {256u, 512u, 1024u, 2048u};
+    static const uint32_t pair_blocks[] = {512u, 1024u, 2048u, 4096u};
+    double best_ms = 0.0;
+    RC_OR_RETURN(time_recorded(ctx, rec, repeats, &best_ms));
+    uint32_t *knob[3] = {&ctx->heavy_min, &ctx->heavy_blocks, &ctx->pair_blocks};
+    const uint32_t *cand[3] = {heavy_min, heavy_blocks, pair_blocks};
+    const size_t ncand[3] = {5, 4, 4};
+    for (int k = 0; k < 3; ++k) {
+        uint32_t keep = *knob[k];
+        for (size_t i = 0; i < ncand[k]; ++i) {
+            if (cand[k][i] == keep) continue;
+            *knob[k] = cand[k][i];
+            double ms = 0.0;
+            RC_OR_RETURN(time_recorded(ctx, rec, repeats, &ms));
+            if (ms < best_ms) {
+                best_ms = ms;
+                keep = cand[k][i];
+            }
+        }
+        *knob[k] = keep;
+    }
+    RC_OR_RETURN(run_recorded(ctx, rec));  // leave the frame in its targets under the chosen parameters
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    RC_OR_RETURN(check_device_stats(ctx));
+    return mlv_get_tuning(ctx, best);
 }
 
 MLV_API int mlv_context_set_profiling(mlv_context *ctx, int enable) {

@@ -273,6 +273,26 @@ class Device:
     def reset_stats(self) -> None:
         _check(self.ctx, self.lib.mlv_reset_stats(self.ctx))
 
+    # ---------------------------------------------------------- tuning
+    def tuning(self) -> dict:
+        t = L.Tuning()
+        _check(self.ctx, self.lib.mlv_get_tuning(self.ctx, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in L.Tuning._fields_}
+
+    def set_tuning(self, heavy_tile_pairs: int, heavy_blocks: int, pair_blocks: int) -> None:
+        t = L.Tuning(heavy_tile_pairs, heavy_blocks, pair_blocks, 0)
+        _check(self.ctx, self.lib.mlv_set_tuning(self.ctx, C.byref(t)))
+
+    def autotune_begin(self) -> None:
+        """Record the following clears and draws as the frame to tune on."""
+        _check(self.ctx, self.lib.mlv_autotune_begin(self.ctx))
+
+    def autotune_end(self, repeats: int = 2) -> dict:
+        """Time the recorded frame under candidate launch parameters and keep the fastest."""
+        t = L.Tuning()
+        _check(self.ctx, self.lib.mlv_autotune_end(self.ctx, repeats, C.byref(t)))
+        return {k: getattr(t, k) for k, _ in L.Tuning._fields_}
+
     # ---------------------------------------------------------- graphs
     def graph_capture_begin(self) -> None:
         """Record the following clears and draws instead of running them."""
