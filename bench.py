@@ -329,6 +329,10 @@ def bench_c5(dev, args):
         return ms
 
     warm(dev, lambda: frame(False), 1)
+    dev.autotune_begin()
+    frame(False)
+    tuning = dev.autotune_end(repeats=1)
+    print(f"[bench] c5 autotuned: {tuning}", file=sys.stderr, flush=True)
     ms_b = timed(False, args.c5_steps)
     warm(dev, lambda: frame(True), 1)
     ms_p = timed(True, 1)
@@ -347,7 +351,7 @@ def bench_c5(dev, args):
            "fwd_batched_views_per_s": round(n / (ms_b * 1e-3), 2),
            "fwd_per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
            "fwd_batched_over_per_draw": round(ms_p / ms_b, 2), "draws": n * len(sc.meshes),
-           "triangle_draws": n * sc.triangles_per_view, "scene_gen_s": round(gen_s, 1)}
+           "triangle_draws": n * sc.triangles_per_view, "scene_gen_s": round(gen_s, 1), "tuning": tuning}
     if not args.no_c5_diff:
         res.update(bench_c5_diff(dev, sc, args, ev))
     return res
@@ -516,6 +520,10 @@ def main(argv=None):
     bind_state(dev, res, sc)
     ev = HipEvents()
     warm(dev, lambda: frame(dev, res, sc), args.warmup)
+    dev.autotune_begin()
+    frame(dev, res, sc)
+    tuning = dev.autotune_end(repeats=3)
+    print(f"[bench] autotuned: {tuning}", file=sys.stderr, flush=True)
     dev.reset_stats()
     dev.profile()  # drop anything accumulated
     dev.set_profiling(True)
@@ -610,6 +618,7 @@ def main(argv=None):
         "stage_ms_per_frame": {k: round(v / args.steps, 4) for k, v in stage_ms.items()},
         "flushes_per_frame": flushes_per_frame,
         "graph_replay_ms_per_frame": round(ms_graph / args.steps, 4),
+        "tuning": tuning,
         "assembled_triangles_per_frame": int(stats["assembled_triangle_count"] / args.steps),
         "fwd_bwd_ms": None,
         "batched_views_per_s": None,

@@ -632,38 +632,39 @@ __global__ __launch_bounds__(256) void kb_pair_check(BatchParams p, uint32_t n_p
 // The grids are sized for the arena; blocks past the device's pair count return at once.
 
 size_t radix_hist_elems(size_t n_max) { return 256 * ((n_max + kRadixItems - 1) / kRadixItems + 1); }
+constexpr uint32_t kRadixGrid = 4096;  // blocks of a flush's radix passes (each walks chunks)
 
 // Digit counts of one block's kRadixItems keys, stored digit-major (hist[digit * nb + block])
 // so that one exclusive scan over hist gives every (digit, block) its output offset.
-__device__ __forceinline__ void radix_hist_block(const uint32_t *keys, uint32_t n, uint32_t nb, uint32_t shift,
-                                                 uint32_t *hist) {
+__device__ __forceinline__ void radix_hist_block(const uint32_t *keys, uint32_t n, uint32_t nb, uint32_t b,
+                                                 uint32_t shift, uint32_t *hist) {
     __shared__ uint32_t h[256];
     h[threadIdx.x] = 0u;
     __syncthreads();
-    const uint32_t base = blockIdx.x * kRadixItems;
+    const uint32_t base = b * kRadixItems;
     for (uint32_t r = 0; r < kRadixRounds; ++r) {
         const uint32_t i = base + r * 256u + threadIdx.x;
         if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
     }
     __syncthreads();
-    hist[threadIdx.x * nb + blockIdx.x] = h[threadIdx.x];
+    hist[threadIdx.x * nb + b] = h[threadIdx.x];
 }
 
 // Stable scatter of one block's keys and values to the scanned offsets.
 __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
-                                                    uint32_t *vout, uint32_t n, uint32_t nb, uint32_t shift,
-                                                    const uint32_t *hist) {
+                                                    uint32_t *vout, uint32_t n, uint32_t nb, uint32_t b,
+                                                    uint32_t shift, const uint32_t *hist) {
     __shared__ uint32_t wc[4][256];
     __shared__ uint32_t loff[256];  // per digit: next free slot of the block's items in LDS
     __shared__ uint32_t lbeg[256];  // per digit: first slot of the block's items in LDS
     __shared__ uint32_t goff[256];  // per digit: first slot of the block's items in the output
     __shared__ uint32_t s_w[5];
     __shared__ uint32_t s_key[kRadixItems], s_val[kRadixItems];
-    const uint32_t base = blockIdx.x * kRadixItems;
+    const uint32_t base = b * kRadixItems;
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     // the block's count per digit is the step of the scanned digit-major histogram
     {
-        const uint32_t idx = tid * nb + blockIdx.x;
+        const uint32_t idx = tid * nb + b;
         const uint32_t here = hist[idx];
         const uint32_t next = (idx + 1u < 256u * nb) ? hist[idx + 1u] : n;
         uint32_t total;
@@ -723,12 +724,17 @@ __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const u
 }
 
 // Binning passes: the pair count and the block count are read from the device counters.
+// The flush's passes: the grid is capped (kRadixGrid) and each block walks chunks b, b + grid,
+// ... up to the pair count the device measured, so a flush with few pairs launches few idle
+// blocks whatever the arena's capacity.
 __global__ __launch_bounds__(256) void kb_radix_hist(BatchParams p, const uint32_t *keys, uint32_t shift,
                                                      uint32_t *hist) {
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
     const uint32_t nb = p.ctr->radix_blocks;
-    if (blockIdx.x >= nb) return;
-    radix_hist_block(keys, p.ctr->n_pairs, nb, shift, hist);
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        radix_hist_block(keys, p.ctr->n_pairs, nb, b, shift, hist);
+        __syncthreads();  // the block's LDS counts are reused by its next chunk
+    }
 }
 
 __global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uint32_t *kin, const uint32_t *vin,
@@ -736,19 +742,21 @@ __global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uin
                                                         const uint32_t *hist) {
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
     const uint32_t nb = p.ctr->radix_blocks;
-    if (blockIdx.x >= nb) return;
-    radix_scatter_block(kin, vin, kout, vout, p.ctr->n_pairs, nb, shift, hist);
+    for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
+        radix_scatter_block(kin, vin, kout, vout, p.ctr->n_pairs, nb, b, shift, hist);
+        __syncthreads();  // the block's LDS staging is reused by its next chunk
+    }
 }
 
 // Host-sized passes (n known when launching): one block per kRadixItems keys.
 __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t *keys, uint32_t n, uint32_t shift, uint32_t *hist) {
-    radix_hist_block(keys, n, gridDim.x, shift, hist);
+    radix_hist_block(keys, n, gridDim.x, blockIdx.x, shift, hist);
 }
 
 __global__ __launch_bounds__(256) void k_sort_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                       uint32_t *vout, uint32_t n, uint32_t shift,
                                                       const uint32_t *hist) {
-    radix_scatter_block(kin, vin, kout, vout, n, gridDim.x, shift, hist);
+    radix_scatter_block(kin, vin, kout, vout, n, gridDim.x, blockIdx.x, shift, hist);
 }
 
 hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, uint32_t bits,
@@ -1327,15 +1335,17 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
             return e;
         hipLaunchKernelGGL(kb_pair_check, dim3(1), dim3(256), 0, s, p, grid);
         hipLaunchKernelGGL(kb_pairs<true>, dim3(grid), dim3(256), 0, s, p);
-        const uint32_t nb = (p.pair_cap + kRadixItems - 1) / kRadixItems;
+        const uint32_t nb = (p.pair_cap + kRadixItems - 1) / kRadixItems;  // chunks the arena can hold
+        const uint32_t rgrid = std::max(1u, std::min(nb, kRadixGrid));
         uint32_t *kin = p.keys0, *vin = p.vals0, *kout = p.keys1, *vout = p.vals1;
         for (uint32_t pass = 0; pass < radix_passes; ++pass) {
             const uint32_t shift = 8u * pass;
-            hipLaunchKernelGGL(kb_radix_hist, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin, shift, sc.hist);
+            hipLaunchKernelGGL(kb_radix_hist, dim3(rgrid), dim3(256), 0, s, p, (const uint32_t *)kin, shift,
+                               sc.hist);
             if ((e = launch_scan<uint32_t>(sc.hist, sc.hist, (size_t)nb * 256u, &p.ctr->hist_n, sc, nullptr,
                                            s)) != hipSuccess)
                 return e;
-            hipLaunchKernelGGL(kb_radix_scatter, dim3(nb), dim3(256), 0, s, p, (const uint32_t *)kin,
+            hipLaunchKernelGGL(kb_radix_scatter, dim3(rgrid), dim3(256), 0, s, p, (const uint32_t *)kin,
                                (const uint32_t *)vin, kout, vout, shift, (const uint32_t *)sc.hist);
             std::swap(kin, kout);
             std::swap(vin, vout);

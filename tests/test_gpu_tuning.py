@@ -16,7 +16,7 @@ from tests.render_utils import DrawSpec, assert_frames_match, render_gpu, render
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(heavy_tile_pairs=384, heavy_blocks=1024, pair_blocks=2048)
+DEFAULT = dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048)
 
 
 def _draws(W, H):
