@@ -150,7 +150,7 @@ def measured_traffic(kernel: str, sc):
         return None
     if (rec.get("triangles"), rec.get("width"), rec.get("height")) != (sc.triangle_count, sc.width, sc.height):
         return None
-    k = rec["kernels"].get(kernel)
+    k = next((v for name, v in rec["kernels"].items() if name.split("<")[0] == kernel), None)
     return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
 
 
@@ -574,7 +574,9 @@ def main(argv=None):
     stage_ms = {k: prof[k] for k in ("vertex", "setup", "binning", "tile")}
     launches = max(1, int(prof["flushes"]))  # one launch of each stage kernel per flush
     dom = max(stage_ms, key=stage_ms.get)
-    per_launch_ms = stage_ms[dom] / launches
+    # the tile stage's own kernel is timed alone (events around the kb_tile launch); other
+    # stages are several launches and are timed as a whole
+    per_launch_ms = (prof["tile_kernel"] if dom == "tile" else stage_ms[dom]) / launches
     flushes_per_frame = launches / args.steps
     # Algorithmic (compulsory) bytes per launch, texels excluded:
     #  tile:    TriVis 96 B + TriShade 32 B + three 32 B attribute rows per assembled triangle,

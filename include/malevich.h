@@ -195,6 +195,7 @@ typedef struct mlv_profile {
     double stage_ms[4];
     uint64_t draws;
     uint64_t flushes;
+    double tile_kernel_ms; /* the raster/shade kernel of stage 3 alone (kb_tile) */
 } mlv_profile;
 MLV_API int mlv_context_set_profiling(mlv_context *ctx, int enable);
 MLV_API int mlv_get_profile(mlv_context *ctx, mlv_profile *out);
@@ -229,7 +230,7 @@ MLV_API int mlv_graph_destroy(mlv_context *ctx, mlv_graph *graph);
  * The per-context launch parameters of the tile and binning stages. mlv_autotune_begin /
  * mlv_autotune_end record a frame exactly as a graph capture does (only state, clears and
  * draws in between), then run it repeatedly under candidate parameters (coordinate descent:
- * heavy-tile threshold, then heavy-tile blocks, then pair blocks), timing each with HIP events
+ * tile staging, heavy-tile threshold, heavy-tile blocks, then pair blocks), timing each with HIP events
  * on the context stream, and keep the fastest. Results are the same under every candidate;
  * the targets hold the recorded frame afterwards. The bin-list capacity follows the largest
  * pair count measured (with a quarter of headroom) and is reported in pair_capacity. */
@@ -237,6 +238,7 @@ typedef struct mlv_tuning {
     uint32_t heavy_tile_pairs; /* a 16x16 tile with a longer (triangle, tile) list is walked by 16 waves */
     uint32_t heavy_blocks;     /* blocks (1024 threads) of the persistent heavy-tile kernel */
     uint32_t pair_blocks;      /* blocks (256 threads) of the pair count and emit kernels */
+    uint32_t tile_stage;       /* records of a tile's list the tile kernel stages in LDS at a time: 128 or 256 */
     uint32_t pair_capacity;    /* out: bin-list capacity (pairs) of the last flush; ignored by set */
 } mlv_tuning;
 MLV_API int mlv_get_tuning(mlv_context *ctx, mlv_tuning *out);

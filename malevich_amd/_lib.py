@@ -38,7 +38,8 @@ class Stats(C.Structure):
 
 
 class Profile(C.Structure):
-    _fields_ = [("stage_ms", C.c_double * 4), ("draws", C.c_uint64), ("flushes", C.c_uint64)]
+    _fields_ = [("stage_ms", C.c_double * 4), ("draws", C.c_uint64), ("flushes", C.c_uint64),
+                ("tile_kernel_ms", C.c_double)]
 
 
 STAGES = ("vertex", "setup", "binning", "tile")
@@ -67,7 +68,8 @@ class DiffGrads(C.Structure):
 
 class Tuning(C.Structure):
     """mlv_tuning"""
-    _fields_ = [(n, C.c_uint32) for n in ("heavy_tile_pairs", "heavy_blocks", "pair_blocks", "pair_capacity")]
+    _fields_ = [(n, C.c_uint32) for n in ("heavy_tile_pairs", "heavy_blocks", "pair_blocks", "tile_stage",
+                                          "pair_capacity")]
 
 
 class DiffMesh(C.Structure):

@@ -36,6 +36,9 @@ constexpr int kStages = 4;         // vertex, setup, binning, tile
 // The 16-wave path is off by default: with the per-bin screening, one wave per bin was faster
 // on every measured workload (1080p frames, the binning stress, 512^2 views of dense meshes).
 constexpr uint32_t kHeavyMinDefault = 0xFFFFFFFFu, kHeavyBlocksDefault = 1024, kPairBlocksDefault = 2048;
+// The tile kernel stages its list tile_stage records at a time: 256 (22.5 KB of LDS, 7 blocks per
+// CU) or 128 (half the LDS: 8 blocks per CU, more barriers for long lists).
+constexpr uint32_t kTileStageDefault = 256;
 
 enum VsKind { VS_BASIC = 1, VS_VERTEX_LIGHTING = 2, VS_PASSTHROUGH = 3, VS_FULLSCREEN = 4 };
 enum PsKind { PS_BASIC = 1, PS_ENV_LIGHTING = 2, PS_PASSTHROUGH = 3 };
@@ -153,6 +156,7 @@ struct BatchParams {
     uint32_t *tile_cnt, *tile_start;      // per global tile: end and start of its range in the sorted pairs
     uint32_t *heavy;                      // tiles with more than heavy_min pairs (n_heavy of them)
     uint32_t heavy_min, heavy_blocks, pair_blocks;
+    uint32_t tile_stage;                  // records the tile kernel stages in LDS at a time (128 or 256)
     uint32_t *scan_status;                // the scans' status words, cleared by the first stage
     uint32_t scan_status_words;
     BatchCounters *ctr;
@@ -262,8 +266,9 @@ hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t 
 hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s);
 // The flush in four stages, in this order: vertex, setup (cull/clip/ordered compaction),
 // binning (ordered tile lists), tile (raster + early-Z + shading + output merger).
+// tile_ev, if given, gets two events recorded around the kb_tile launch of stage 3.
 hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, uint32_t radix_passes,
-                        hipStream_t s);
+                        hipStream_t s, hipEvent_t *tile_ev = nullptr);
 // Bytes of scan scratch for scans over up to n_max elements; elements of the radix histograms.
 size_t scan_scratch_bytes(size_t n_max);
 size_t radix_hist_elems(size_t n_max);

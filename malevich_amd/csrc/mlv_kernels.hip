@@ -852,13 +852,19 @@ struct BinState {
 // triangle-parallel screening reads a different record per lane, and with one array per field
 // consecutive records sit in consecutive banks (a 96-byte record stride would put 64 lanes on
 // a handful of banks). The rasteriser reads one record at a time as broadcasts.
-struct TileLds {
-    int32_t a16[3][256], b16[3][256], c[3][256];
-    float ooa[256], z0[256], dz1[256], dz2[256], hiz[256], cull[256];
-    uint32_t lat_x[256], lat_y[256], id[256], draw[256];
+// Up to N staged records of a tile's list, one array per field (a wave reading field f of one
+// record reads one address: a broadcast).
+template <uint32_t N>
+struct TileLdsN {
+    static constexpr uint32_t kN = N;
+    int32_t a16[3][N], b16[3][N], c[3][N];
+    float ooa[N], z0[N], dz1[N], dz2[N], hiz[N], cull[N];
+    uint32_t lat_x[N], lat_y[N], id[N], draw[N];
 };
+using TileLds = TileLdsN<256>;
 
-__device__ __forceinline__ void stage_record(TileLds &L, uint32_t k, const TriVis &t) {
+template <class Lds>
+__device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t) {
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         L.a16[e][k] = t.a16[e];
@@ -889,8 +895,8 @@ __device__ __forceinline__ void bin_draw_change(const BatchParams &p, int32_t dr
 
 // Corner-sampled coverage of one triangle over the bin's 64 pixels and the early-Z key update
 // (main.c:1015-1028, 1083-1168); the triangle has passed hierarchical-Z.
-template <bool GE>
-__device__ __forceinline__ void bin_cover(const TileLds &L, uint32_t k, BinState &st, bool inside, int32_t px,
+template <bool GE, class Lds>
+__device__ __forceinline__ void bin_cover(const Lds &L, uint32_t k, BinState &st, bool inside, int32_t px,
                                           int32_t py) {
     const uint32_t e0 = (uint32_t)L.c[0][k] + (uint32_t)__mul24(L.a16[0][k], px) + (uint32_t)__mul24(L.b16[0][k], py);
     const uint32_t e1 = (uint32_t)L.c[1][k] + (uint32_t)__mul24(L.a16[1][k], px) + (uint32_t)__mul24(L.b16[1][k], py);
@@ -925,8 +931,8 @@ __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c,
 // cull and an edge that is negative over the whole bin (then no pixel of the bin is covered).
 // The survivors are then rasterised pixel-parallel, in order. Every test only removes triangles
 // that could not have written a pixel, so the result equals the one-by-one walk.
-template <bool GE>
-__device__ __forceinline__ void walk_bin(const BatchParams &p, const TileLds &L, const uint16_t *list, uint32_t first,
+template <bool GE, class Lds>
+__device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, const uint16_t *list, uint32_t first,
                                          uint32_t stride, uint32_t n, BinState &st, bool inside, int32_t px,
                                          int32_t py, int32_t x0, int32_t y0, uint32_t lane) {
     for (uint32_t i0 = 0; i0 < n;) {
@@ -979,9 +985,9 @@ __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *s
     return d1 - d0 < kHeavyDraws;
 }
 
-template <bool GE>
+template <bool GE, uint32_t N>
 __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t vi,
-                                          TileLds &s_tri, uint32_t *s_mask, uint16_t (*s_list)[256],
+                                          TileLdsN<N> &s_tri, uint32_t *s_mask, uint16_t (*s_list)[N],
                                           uint32_t *s_active) {
     // the wave index is uniform per wave; saying so keeps everything derived from it (the bin,
     // its Hi-Z value, the walk's branches) in scalar registers
@@ -1015,8 +1021,8 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
     st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
     st.stale = 0u;
 
-    for (uint32_t base = 0; base < lcount; base += 256u) {
-        const uint32_t n = min(256u, lcount - base);
+    for (uint32_t base = 0; base < lcount; base += N) {
+        const uint32_t n = min(N, lcount - base);
         if (tid < n) {
             const uint32_t t = ovf ? base + tid : sorted_vals[lstart + base + tid];
             const TriVis tv = p.tvis[t];
@@ -1036,7 +1042,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
         __syncthreads();
         uint32_t cnt = 0u;
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
+        for (int c = 0; c < (int)(N / 64u); ++c) {
             const uint32_t i = (uint32_t)c * 64u + lane;
             const bool b = i < n && ((s_mask[i] >> w) & 1u);
             const uint64_t bal = __ballot(b);
@@ -1256,10 +1262,11 @@ __global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint3
 // records of the tile's sorted list in LDS; each wave compacts the ones touching its bin and
 // walks them in submission order with its pixels' depths in registers; afterwards each pixel's
 // winner is shaded once and colour, depth, visibility and the bin's Hi-Z are written.
+template <uint32_t N>
 __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
-    __shared__ TileLds s_tri;
-    __shared__ uint32_t s_mask[256];
-    __shared__ __attribute__((aligned(8))) uint16_t s_list[4][256];
+    __shared__ TileLdsN<N> s_tri;
+    __shared__ uint32_t s_mask[N];
+    __shared__ __attribute__((aligned(8))) uint16_t s_list[4][N];
     __shared__ uint32_t s_active[4];
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
@@ -1269,9 +1276,9 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
     const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
     if (threadIdx.x < 4u) s_active[threadIdx.x] = 0u;
     if (p.views[vi].depth_func == DEPTH_GE)
-        tile_body<true>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
+        tile_body<true, N>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
     else
-        tile_body<false>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
+        tile_body<false, N>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
     __syncthreads();
     // (bin, draw) activity of this tile for the stats; this tile's start is no longer needed
     if (threadIdx.x == 0u) p.tile_start[g] = s_active[0] + s_active[1] + s_active[2] + s_active[3];
@@ -1308,7 +1315,7 @@ hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s) {
 }
 
 hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, uint32_t radix_passes,
-                        hipStream_t s) {
+                        hipStream_t s, hipEvent_t *tile_ev) {
     hipError_t e;
     switch (stage) {
     case 0:
@@ -1361,7 +1368,12 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
                            s, p, sorted);
         hipLaunchKernelGGL(kb_tile_heavy, dim3(std::max(1u, std::min<uint32_t>(p.total_tiles, p.heavy_blocks))),
                            dim3(1024), 0, s, p, sorted);
-        hipLaunchKernelGGL(kb_tile, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
+        if (tile_ev && (e = hipEventRecord(tile_ev[0], s)) != hipSuccess) return e;
+        if (p.tile_stage == 128u)
+            hipLaunchKernelGGL(kb_tile<128>, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
+        else
+            hipLaunchKernelGGL(kb_tile<256>, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
+        if (tile_ev && (e = hipEventRecord(tile_ev[1], s)) != hipSuccess) return e;
         hipLaunchKernelGGL(kb_active_sum, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 64u)), dim3(256), 0,
                            s, p);
         break;

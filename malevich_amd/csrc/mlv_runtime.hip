@@ -150,6 +150,7 @@ struct mlv_context {
     std::vector<hipEvent_t> ev_pool;
     std::vector<std::pair<int, std::pair<hipEvent_t, hipEvent_t>>> ev_pending;
     double stage_ms[kStages] = {};
+    double tile_kernel_ms = 0.0;  // the kb_tile launches alone (bucket kStages of ev_pending)
     uint64_t prof_draws = 0, prof_flushes = 0;
     // graphs: commands recorded between capture_begin and capture_end, replayed once to size
     // every arena and then under stream capture
@@ -164,6 +165,7 @@ struct mlv_context {
     std::vector<void *> retired;     // arenas replaced while a graph may still read them
     // launch parameters (mlv_tuning)
     uint32_t heavy_min = kHeavyMinDefault, heavy_blocks = kHeavyBlocksDefault, pair_blocks = kPairBlocksDefault;
+    uint32_t tile_stage = kTileStageDefault;
     size_t last_pair_cap = 0;
 };
 
@@ -249,7 +251,10 @@ static hipEvent_t take_event(mlv_context *ctx) {
 static void resolve_events(mlv_context *ctx) {
     for (auto &pe : ctx->ev_pending) {
         float ms = 0.0f;
-        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) ctx->stage_ms[pe.first] += ms;
+        if (hipEventElapsedTime(&ms, pe.second.first, pe.second.second) == hipSuccess) {
+            if (pe.first < kStages) ctx->stage_ms[pe.first] += ms;
+            else ctx->tile_kernel_ms += ms;
+        }
         ctx->ev_pool.push_back(pe.second.first);
         ctx->ev_pool.push_back(pe.second.second);
     }
@@ -455,6 +460,7 @@ static int flush(mlv_context *ctx) {
     p.heavy_min = ctx->heavy_min;
     p.heavy_blocks = ctx->heavy_blocks;
     p.pair_blocks = ctx->pair_blocks;
+    p.tile_stage = ctx->tile_stage;
     ctx->last_pair_cap = pair_cap;
     p.ctr = ctx->ctr;
     p.stats = ctx->dstats;
@@ -480,10 +486,17 @@ static int flush(mlv_context *ctx) {
             if (!a || !b) return fail(ctx, MLV_ERROR_DEVICE, "hipEventCreate failed");
             HIP_OR_FAIL(ctx, hipEventRecord(a, ctx->stream));
         }
-        HIP_OR_FAIL(ctx, launch_stage(st, p, sc, passes, ctx->stream));
+        hipEvent_t tk[2] = {nullptr, nullptr};
+        if (timing && st == kStages - 1) {
+            tk[0] = take_event(ctx);
+            tk[1] = take_event(ctx);
+            if (!tk[0] || !tk[1]) return fail(ctx, MLV_ERROR_DEVICE, "hipEventCreate failed");
+        }
+        HIP_OR_FAIL(ctx, launch_stage(st, p, sc, passes, ctx->stream, tk[0] ? tk : nullptr));
         if (timing) {
             HIP_OR_FAIL(ctx, hipEventRecord(b, ctx->stream));
             ctx->ev_pending.push_back({st, {a, b}});
+            if (tk[0]) ctx->ev_pending.push_back({kStages, {tk[0], tk[1]}});
         }
     }
     if (timing) {
@@ -1188,6 +1201,7 @@ MLV_API int mlv_get_tuning(mlv_context *ctx, mlv_tuning *out) {
     out->heavy_tile_pairs = ctx->heavy_min;
     out->heavy_blocks = ctx->heavy_blocks;
     out->pair_blocks = ctx->pair_blocks;
+    out->tile_stage = ctx->tile_stage;
     out->pair_capacity = (uint32_t)std::min<size_t>(ctx->last_pair_cap, 0xFFFFFFFFu);
     return MLV_OK;
 }
@@ -1196,10 +1210,13 @@ MLV_API int mlv_set_tuning(mlv_context *ctx, const mlv_tuning *t) {
     if (!ctx || !t) return MLV_ERROR_INVALID_ARGUMENT;
     if (!t->heavy_blocks || t->heavy_blocks > 65536u || !t->pair_blocks || t->pair_blocks > 65536u)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "set_tuning: block counts must be 1..65536");
+    if (t->tile_stage != 128u && t->tile_stage != 256u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "set_tuning: tile_stage must be 128 or 256");
     RC_OR_RETURN(flush(ctx));  // recorded draws keep the parameters they were recorded under
     ctx->heavy_min = t->heavy_tile_pairs;
     ctx->heavy_blocks = t->heavy_blocks;
     ctx->pair_blocks = t->pair_blocks;
+    ctx->tile_stage = t->tile_stage;
     return MLV_OK;
 }
 
@@ -1250,12 +1267,13 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
     static const uint32_t heavy_min[] = {192u, 384u, 768u, 1536u, 0xFFFFFFFFu};
     static const uint32_t heavy_blocks[] = {256u, 512u, 1024u, 2048u};
     static const uint32_t pair_blocks[] = {512u, 1024u, 2048u, 4096u};
+    static const uint32_t tile_stage[] = {128u, 256u};
     double best_ms = 0.0;
     RC_OR_RETURN(time_recorded(ctx, rec, repeats, &best_ms));
-    uint32_t *knob[3] = {&ctx->heavy_min, &ctx->heavy_blocks, &ctx->pair_blocks};
-    const uint32_t *cand[3] = {heavy_min, heavy_blocks, pair_blocks};
-    const size_t ncand[3] = {5, 4, 4};
-    for (int k = 0; k < 3; ++k) {
+    uint32_t *knob[4] = {&ctx->tile_stage, &ctx->heavy_min, &ctx->heavy_blocks, &ctx->pair_blocks};
+    const uint32_t *cand[4] = {tile_stage, heavy_min, heavy_blocks, pair_blocks};
+    const size_t ncand[4] = {2, 5, 4, 4};
+    for (int k = 0; k < 4; ++k) {
         uint32_t keep = *knob[k];
         for (size_t i = 0; i < ncand[k]; ++i) {
             if (cand[k][i] == keep) continue;
@@ -1292,6 +1310,8 @@ MLV_API int mlv_get_profile(mlv_context *ctx, mlv_profile *out) {
         out->stage_ms[i] = ctx->stage_ms[i];
         ctx->stage_ms[i] = 0.0;
     }
+    out->tile_kernel_ms = ctx->tile_kernel_ms;
+    ctx->tile_kernel_ms = 0.0;
     out->draws = ctx->prof_draws;
     out->flushes = ctx->prof_flushes;
     ctx->prof_draws = ctx->prof_flushes = 0;

@@ -264,6 +264,7 @@ class Device:
         out = {name: pr.stage_ms[i] for i, name in enumerate(L.STAGES)}
         out["draws"] = pr.draws
         out["flushes"] = pr.flushes
+        out["tile_kernel"] = pr.tile_kernel_ms
         return out
 
     def flush(self) -> None:
@@ -279,8 +280,8 @@ class Device:
         _check(self.ctx, self.lib.mlv_get_tuning(self.ctx, C.byref(t)))
         return {k: getattr(t, k) for k, _ in L.Tuning._fields_}
 
-    def set_tuning(self, heavy_tile_pairs: int, heavy_blocks: int, pair_blocks: int) -> None:
-        t = L.Tuning(heavy_tile_pairs, heavy_blocks, pair_blocks, 0)
+    def set_tuning(self, heavy_tile_pairs: int, heavy_blocks: int, pair_blocks: int, tile_stage: int = 256) -> None:
+        t = L.Tuning(heavy_tile_pairs, heavy_blocks, pair_blocks, tile_stage, 0)
         _check(self.ctx, self.lib.mlv_set_tuning(self.ctx, C.byref(t)))
 
     def autotune_begin(self) -> None:

@@ -204,10 +204,10 @@ def test_heavy_tiles_multi_draw(device, oracle, depth_func):
         draws.append(DrawSpec(st.indices, st.vertices, "passthrough", "passthrough"))
     clear_depth = 0.0 if depth_func == 0 else 1.0
     saved = device.tuning()
-    device.set_tuning(384, saved["heavy_blocks"], saved["pair_blocks"])  # lists past 384 take the 16-wave path
+    device.set_tuning(384, saved["heavy_blocks"], saved["pair_blocks"], saved["tile_stage"])  # past 384: 16 waves
     try:
         g, o = both(device, oracle, W, H, draws, depth=clear_depth, depth_func=depth_func)
     finally:
-        device.set_tuning(saved["heavy_tile_pairs"], saved["heavy_blocks"], saved["pair_blocks"])
+        device.set_tuning(saved["heavy_tile_pairs"], saved["heavy_blocks"], saved["pair_blocks"], saved["tile_stage"])
     assert_frames_match(g, o, color_tol=0)
     assert len(np.unique(g.vis_draw)) >= 6

@@ -16,7 +16,7 @@ from tests.render_utils import DrawSpec, assert_frames_match, render_gpu, render
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048)
+DEFAULT = dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048, tile_stage=256)
 
 
 def _draws(W, H):
@@ -27,9 +27,9 @@ def _draws(W, H):
     return out
 
 
-@pytest.mark.parametrize("tuning", [dict(heavy_tile_pairs=0, heavy_blocks=1, pair_blocks=3),
-                                    dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=7, pair_blocks=1),
-                                    dict(heavy_tile_pairs=100, heavy_blocks=4096, pair_blocks=4096)])
+@pytest.mark.parametrize("tuning", [dict(heavy_tile_pairs=0, heavy_blocks=1, pair_blocks=3, tile_stage=256),
+                                    dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=7, pair_blocks=1, tile_stage=128),
+                                    dict(heavy_tile_pairs=100, heavy_blocks=4096, pair_blocks=4096, tile_stage=128)])
 def test_tunings_give_the_oracle_frame(device, oracle, tuning):
     W, H = 96, 64
     draws = _draws(W, H)
@@ -81,6 +81,7 @@ def test_autotune_keeps_results_and_choice(device):
         assert best["heavy_tile_pairs"] in (192, 384, 768, 1536, 0xFFFFFFFF)
         assert best["heavy_blocks"] in (256, 512, 1024, 2048)
         assert best["pair_blocks"] in (512, 1024, 2048, 4096)
+        assert best["tile_stage"] in (128, 256)
         assert best["pair_capacity"] > 0
         np.testing.assert_array_equal(rt.read(), ref[0])
         np.testing.assert_array_equal(ds.read().view(np.uint32), ref[1].view(np.uint32))
