@@ -129,6 +129,11 @@ MLV_API void *mlv_buffer_device_ptr(const mlv_buffer *buf);
 MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t height, int format, uint32_t mip_levels,
                                  const void *host_data, mlv_texture2d **out);
 MLV_API int mlv_texture2d_destroy(mlv_context *ctx, mlv_texture2d *tex);
+/* load_texture(..., is_in_srgb = true) (main.c:538-559): the in-place sRGB -> linear pre-pass of
+ * an RGBA8 texture (all four channels, truncating re-encode), run on the device. */
+MLV_API int mlv_texture2d_srgb_to_linear(mlv_context *ctx, mlv_texture2d *tex);
+/* Copies the texture's level-0 texels (width*height texels of its format) to the host. */
+MLV_API int mlv_texture2d_readback(mlv_context *ctx, const mlv_texture2d *tex, void *host_dst);
 MLV_API void *mlv_texture2d_device_ptr(const mlv_texture2d *tex);
 
 MLV_API int mlv_render_target_create(mlv_context *ctx, uint32_t width, uint32_t height, mlv_render_target **out);
@@ -176,8 +181,15 @@ MLV_API int mlv_clear_depth_stencil_view(mlv_context *ctx, mlv_depth_stencil *ds
 /* main.c:1219 draw_indexed(UINT index_count); start_index/base_vertex are the D3D11 arguments
  * the reference leaves as a TODO (main.c:1219). (n, 0, 0) is the reference call. */
 MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t start_index, int32_t base_vertex);
-/* main.c:1301 present(): headless form, copies the render target (u32 per pixel, row-major) to the host. */
+/* Copies the render target's u32 values (row-major) to the host: the values the reference's
+ * frame_buffer holds (main.c:35), shaded pixels R<<16|G<<8|B (main.c:1176-1178), clears
+ * A<<24|B<<16|G<<8|R (math.h:322-324). */
 MLV_API int mlv_render_target_readback(mlv_context *ctx, mlv_render_target *rt, void *host_dst);
+/* main.c:1301 present() / paint_window (main.c:286-311), headless: the image the window shows.
+ * The frame buffer is read as a top-down 32-bit DIB with masks R 0x00FF0000, G 0x0000FF00,
+ * B 0x000000FF and written to host_rgb as width*height*3 bytes, row-major, top row first (a
+ * PPM/PNG-ready RGB8 image). The conversion runs on the device, so a quarter fewer bytes cross PCIe. */
+MLV_API int mlv_present(mlv_context *ctx, mlv_render_target *rt, uint8_t *host_rgb);
 MLV_API int mlv_depth_stencil_readback(mlv_context *ctx, mlv_depth_stencil *ds, float *host_dst);
 
 /* Recorded draws run together when their results are needed (readback, synchronize,

@@ -107,6 +107,8 @@ SIGNATURES = {
     "mlv_buffer_device_ptr": (_P, [_P]),
     "mlv_texture2d_create": (_I, [_P, _U32, _U32, _I, _U32, _P, _PP]),
     "mlv_texture2d_destroy": (_I, [_P, _P]),
+    "mlv_texture2d_srgb_to_linear": (_I, [_P, _P]),
+    "mlv_texture2d_readback": (_I, [_P, _P, _P]),
     "mlv_texture2d_device_ptr": (_P, [_P]),
     "mlv_render_target_create": (_I, [_P, _U32, _U32, _PP]),
     "mlv_render_target_destroy": (_I, [_P, _P]),
@@ -131,6 +133,7 @@ SIGNATURES = {
     "mlv_clear_depth_stencil_view": (_I, [_P, _P, C.c_float]),
     "mlv_draw_indexed": (_I, [_P, _U32, _U32, C.c_int32]),
     "mlv_render_target_readback": (_I, [_P, _P, _P]),
+    "mlv_present": (_I, [_P, _P, _P]),
     "mlv_depth_stencil_readback": (_I, [_P, _P, _P]),
     "mlv_get_stats": (_I, [_P, C.POINTER(Stats)]),
     "mlv_reset_stats": (_I, [_P]),

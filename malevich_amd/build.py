@@ -65,7 +65,8 @@ def build_library(verbose: bool = False) -> Path:
                 print(" ".join(cmd))
         list(ex.map(_run, cmds))
     tmp = LIB.with_suffix(".so.tmp")
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-o", str(tmp)])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", *map(str, objs), "-lrocprofiler-sdk-roctx", "-o",
+          str(tmp)])
     os.replace(tmp, LIB)
     return LIB
 

@@ -263,6 +263,10 @@ hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, ui
 
 // Launchers (mlv_kernels.hip). All enqueue on `s`.
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
+// n_pixels u32 frame-buffer values to 3 * n_pixels bytes of RGB; rgb holds 12 * ceil(n_pixels / 4)
+// bytes and is 4-byte aligned
+hipError_t launch_present_rgb(const uint32_t *fb, uint8_t *rgb, size_t n_pixels, hipStream_t s);
+hipError_t launch_srgb_to_linear_rgba8(uint32_t *texels, size_t n, hipStream_t s);
 hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s);
 // The flush in four stages, in this order: vertex, setup (cull/clip/ordered compaction),
 // binning (ordered tile lists), tile (raster + early-Z + shading + output merger).

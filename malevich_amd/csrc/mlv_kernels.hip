@@ -147,6 +147,47 @@ __device__ __forceinline__ uint32_t draw_of_slot(const BatchParams &p, uint32_t 
 
 // ------------------------------------------------------------ fills
 
+// The image paint_window shows (main.c:286-311): the frame buffer as a top-down 32-bit DIB with
+// red mask 0x00FF0000, green 0x0000FF00 and blue 0x000000FF, as packed 8-bit RGB. Each thread
+// converts 4 pixels into 12 bytes (three words of an output padded to whole words).
+__global__ void k_present_rgb(const uint32_t *fb, uint32_t *rgb_words, size_t n_pixels) {
+    const size_t n_quads = (n_pixels + 3u) / 4u;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n_quads; q += (size_t)gridDim.x * blockDim.x) {
+        uint8_t b[12];
+#pragma unroll
+        for (int k = 0; k < 4; ++k) {
+            const size_t i = 4u * q + (size_t)k;
+            const uint32_t v = i < n_pixels ? fb[i] : 0u;
+            b[3 * k + 0] = (uint8_t)(v >> 16);
+            b[3 * k + 1] = (uint8_t)(v >> 8);
+            b[3 * k + 2] = (uint8_t)v;
+        }
+#pragma unroll
+        for (int w = 0; w < 3; ++w)
+            rgb_words[3 * q + w] = (uint32_t)b[4 * w] | ((uint32_t)b[4 * w + 1] << 8) | ((uint32_t)b[4 * w + 2] << 16) |
+                                   ((uint32_t)b[4 * w + 3] << 24);
+    }
+}
+
+// load_texture's pre-pass for sRGB RGBA8 textures (main.c:546-558), in place. A texel's four bytes
+// each map through the same 256-entry table; every block first builds it in LDS, one byte per
+// thread, with the reference's arithmetic: decode byte * f32(1/255) (math.h:326-334),
+// srgb_to_linear in double (math.h:386-394) stored as f32, truncating re-encode (math.h:322-324).
+__global__ __launch_bounds__(256) void k_srgb_to_linear_rgba8(uint32_t *texels, size_t n) {
+    __shared__ uint32_t lut[256];
+    {
+        const float v = (float)threadIdx.x * (float)(1.0 / 255.0);
+        const float lin = (double)v <= 0.04045 ? (float)((double)v / 12.92) : (float)pow(((double)v + 0.055) / 1.055, 2.4);
+        const float x = lin * 255.f;
+        lut[threadIdx.x] = (x > 0.0f && x < 4294967296.0f) ? (uint32_t)x : 0u;
+    }
+    __syncthreads();
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t t = texels[i];
+        texels[i] = lut[t & 255u] | (lut[(t >> 8) & 255u] << 8) | (lut[(t >> 16) & 255u] << 16) | (lut[t >> 24] << 24);
+    }
+}
+
 __global__ void k_fill_u32(uint32_t *d, uint32_t v, size_t n) {
     for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         d[i] = v;
@@ -1299,6 +1340,19 @@ __global__ __launch_bounds__(256) void kb_active_sum(BatchParams p) {
 }
 
 // ------------------------------------------------------------ launchers
+
+hipError_t launch_present_rgb(const uint32_t *fb, uint8_t *rgb, size_t n_pixels, hipStream_t s) {
+    const size_t quads = (n_pixels + 3u) / 4u;
+    const unsigned grid = (unsigned)std::min<size_t>((quads + 255) / 256, 4096);
+    if (grid) hipLaunchKernelGGL(k_present_rgb, dim3(grid), dim3(256), 0, s, fb, (uint32_t *)rgb, n_pixels);
+    return hipGetLastError();
+}
+
+hipError_t launch_srgb_to_linear_rgba8(uint32_t *texels, size_t n, hipStream_t s) {
+    const unsigned grid = (unsigned)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 2048));
+    hipLaunchKernelGGL(k_srgb_to_linear_rgba8, dim3(grid), dim3(256), 0, s, texels, n);
+    return hipGetLastError();
+}
 
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s) {
     if (!n) return hipSuccess;

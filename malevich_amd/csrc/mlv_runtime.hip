@@ -27,6 +27,8 @@
 #include "malevich.h"
 #include "mlv_internal.h"
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 using namespace mlv;
 
 struct mlv_buffer {
@@ -131,6 +133,7 @@ struct mlv_context {
     Arena d_vposd, d_posacc;
     // batched differentiable shading: device tables, FP64 position sums, SH partials
     Arena b_tables, b_posacc, b_shpart;
+    Arena present_rgb;  // packed RGB of the last present
     BatchCounters *ctr = nullptr;
     DevStats *dstats = nullptr;
     uint32_t scan_epoch = 0;  // advanced by every scan launch (tags the scans' status words)
@@ -202,6 +205,15 @@ static int fail(mlv_context *ctx, int code, const char *fmt, ...) {
         int rc_ = (expr);              \
         if (rc_ != MLV_OK) return rc_; \
     } while (0)
+
+// A roctx range for the life of a scope, named like the reference's Remotery samples
+// (rmt_BeginCPUSample, main.c:663-1305) so that profiler timelines carry the same stage names.
+struct Marker {
+    explicit Marker(const char *name) { roctxRangePushA(name); }
+    ~Marker() { roctxRangePop(); }
+    Marker(const Marker &) = delete;
+    Marker &operator=(const Marker &) = delete;
+};
 
 static int ensure(mlv_context *ctx, Arena &a, size_t bytes) {
     if (bytes <= a.cap) return MLV_OK;
@@ -478,7 +490,13 @@ static int flush(mlv_context *ctx) {
     const uint32_t passes = std::max<uint32_t>(1u, (bits + 7u) / 8u);
 
     const bool timing = ctx->profiling && !ctx->capturing;
+    // the reference's sample names for the work each stage covers (main.c:663, 699, 737, 916, 984, 1047)
+    static const char *const kStageMarker[kStages] = {"input_assambler_stage+vertex_shader_stage",
+                                                       "primitive_assembly_stage", "binner",
+                                                       "rasterizer_stage+pixel_shader_stage"};
+    Marker flush_marker("flush");
     for (int st = 0; st < kStages; ++st) {
+        Marker stage_marker(kStageMarker[st]);
         hipEvent_t a = nullptr, b = nullptr;
         if (timing) {
             a = take_event(ctx);
@@ -585,7 +603,7 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->scan_part, &ctx->radix_hist, &ctx->heavy, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
                     &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
-                    &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart};
+                    &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart, &ctx->present_rgb};
     for (Arena *a : all) release(*a);
     free_retired(ctx);
     resolve_events(ctx);
@@ -716,6 +734,25 @@ MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t heig
         return MLV_ERROR_OUT_OF_MEMORY;
     }
     *out = t;
+    return MLV_OK;
+}
+
+MLV_API int mlv_texture2d_srgb_to_linear(mlv_context *ctx, mlv_texture2d *tex) {
+    if (!ctx || !tex) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "texture2d_srgb_to_linear: bad argument");
+    if (tex->format != MLV_FORMAT_RGBA8_UNORM)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "texture2d_srgb_to_linear: RGBA8 textures only (main.c:546)");
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // recorded draws read the texture as it was
+    HIP_OR_FAIL(ctx, launch_srgb_to_linear_rgba8((uint32_t *)tex->ptr, (size_t)tex->width * tex->height, ctx->stream));
+    return MLV_OK;
+}
+
+MLV_API int mlv_texture2d_readback(mlv_context *ctx, const mlv_texture2d *tex, void *host_dst) {
+    if (!ctx || !tex || !host_dst) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "texture2d_readback: bad argument");
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, tex->ptr, tex->bytes, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     return MLV_OK;
 }
 
@@ -933,6 +970,7 @@ static int exec_clear_depth(mlv_context *ctx, const Cmd &c) {
 
 MLV_API int mlv_clear_render_target_view(mlv_context *ctx, mlv_render_target *rt, const float rgba[4]) {
     if (!ctx || !rt || !rgba) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "clear_render_target_view: bad argument");
+    Marker marker("clear_render_target_view");
     RC_OR_RETURN(set_device(ctx));
     Cmd c;
     c.kind = kCmdClearColor;
@@ -953,6 +991,7 @@ MLV_API int mlv_clear_render_target_view(mlv_context *ctx, mlv_render_target *rt
 
 MLV_API int mlv_clear_depth_stencil_view(mlv_context *ctx, mlv_depth_stencil *ds, float depth) {
     if (!ctx || !ds) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "clear_depth_stencil_view: bad argument");
+    Marker marker("clear_depth_stencil_view");
     RC_OR_RETURN(set_device(ctx));
     Cmd c;
     c.kind = kCmdClearDepth;
@@ -1009,6 +1048,7 @@ static int submit_draw(mlv_context *ctx, const Cmd &c) {
 
 MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t start_index, int32_t base_vertex) {
     if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    Marker marker("draw_indexed");
     if (ctx->topology != MLV_TOPOLOGY_TRIANGLELIST)
         return fail(ctx, MLV_ERROR_INVALID_STATE, "draw_indexed: topology must be TRIANGLELIST (main.c:666)");
     if (index_count % 3u)
@@ -1170,6 +1210,7 @@ MLV_API int mlv_graph_capture_end(mlv_context *ctx, mlv_graph **out) {
 
 MLV_API int mlv_graph_launch(mlv_context *ctx, mlv_graph *g) {
     if (!ctx || !g) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "graph_launch: bad argument");
+    Marker marker("render (graph)");
     RC_OR_RETURN(set_device(ctx));
     RC_OR_RETURN(flush(ctx));
     HIP_OR_FAIL(ctx, hipGraphLaunch(g->exec, ctx->stream));
@@ -1325,10 +1366,24 @@ MLV_API int mlv_flush(mlv_context *ctx) {
 
 MLV_API int mlv_render_target_readback(mlv_context *ctx, mlv_render_target *rt, void *host_dst) {
     if (!ctx || !rt || !host_dst) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "readback: bad argument");
+    Marker marker("present");
     RC_OR_RETURN(set_device(ctx));
     RC_OR_RETURN(flush(ctx));
     HIP_OR_FAIL(ctx, hipMemcpyAsync(host_dst, rt->ptr, (size_t)rt->width * rt->height * 4, hipMemcpyDeviceToHost,
                                     ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    return check_device_stats(ctx);
+}
+
+MLV_API int mlv_present(mlv_context *ctx, mlv_render_target *rt, uint8_t *host_rgb) {
+    if (!ctx || !rt || !host_rgb) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "present: bad argument");
+    Marker marker("present");
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    const size_t n = (size_t)rt->width * rt->height;
+    RC_OR_RETURN(ensure(ctx, ctx->present_rgb, 12 * ((n + 3) / 4)));
+    HIP_OR_FAIL(ctx, launch_present_rgb(rt->ptr, (uint8_t *)ctx->present_rgb.p, n, ctx->stream));
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(host_rgb, ctx->present_rgb.p, 3 * n, hipMemcpyDeviceToHost, ctx->stream));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     return check_device_stats(ctx);
 }

@@ -59,6 +59,21 @@ class Texture2D:
     def __init__(self, dev, handle, width, height, fmt):
         self.dev, self.handle, self.width, self.height, self.format = dev, handle, width, height, fmt
 
+    def srgb_to_linear(self) -> None:
+        """load_texture's sRGB pre-pass (main.c:546-558) on the device, in place (RGBA8 only)."""
+        _check(self.dev.ctx, L.load().mlv_texture2d_srgb_to_linear(self.dev.ctx, self.handle))
+
+    def read(self) -> np.ndarray:
+        """Level-0 texels: (h, w) uint32 for RGBA8, (h, w, 4) float32 / float16 for the float formats."""
+        if self.format == L.FORMAT_RGBA8_UNORM:
+            out = np.empty((self.height, self.width), np.uint32)
+        elif self.format == L.FORMAT_RGBA32F:
+            out = np.empty((self.height, self.width, 4), np.float32)
+        else:
+            out = np.empty((self.height, self.width, 4), np.float16)
+        _check(self.dev.ctx, L.load().mlv_texture2d_readback(self.dev.ctx, self.handle, out.ctypes.data))
+        return out
+
     def release(self) -> None:
         if self.handle:
             L.load().mlv_texture2d_destroy(self.dev.ctx, self.handle)
@@ -76,6 +91,12 @@ class RenderTarget:
     def read(self) -> np.ndarray:
         out = np.empty((self.height, self.width), np.uint32)
         _check(self.dev.ctx, L.load().mlv_render_target_readback(self.dev.ctx, self.handle, out.ctypes.data))
+        return out
+
+    def present(self) -> np.ndarray:
+        """(H, W, 3) uint8: the image the reference's window shows for this frame (mlv_present)."""
+        out = np.empty((self.height, self.width, 3), np.uint8)
+        _check(self.dev.ctx, L.load().mlv_present(self.dev.ctx, self.handle, out.ctypes.data))
         return out
 
     def release(self) -> None:

@@ -128,6 +128,7 @@ def lib() -> C.CDLL:
         L.orc_clip_polygon.argtypes = [P(C.c_float), C.c_int, P(C.c_float)]
         L.orc_clip_polygon.restype = C.c_int
         L.orc_threads.restype = C.c_int
+        L.orc_srgb_to_linear_rgba8.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
         L.orc_diff_forward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p]
         L.orc_diff_backward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p, P(DiffGrads)]
         _lib = L
@@ -333,6 +334,14 @@ def diff_backward(d: DiffInputs, prim: np.ndarray, grad_rgba: np.ndarray) -> dic
     if rc != 0:
         raise RuntimeError(f"orc_diff_backward failed: {rc}")
     return grads
+
+
+def srgb_to_linear_rgba8(texels: np.ndarray) -> np.ndarray:
+    """load_texture's sRGB pre-pass (main.c:546-558) over u32 RGBA8 texels."""
+    src = np.ascontiguousarray(texels, np.uint32)
+    out = np.empty_like(src)
+    lib().orc_srgb_to_linear_rgba8(src.ctypes.data, out.ctypes.data, src.size)
+    return out
 
 
 def threads() -> int:

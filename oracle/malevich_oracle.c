@@ -126,6 +126,24 @@ uint32_t orc_encode_clear(const float rgba[4]) {
     return (a << 24) + (b << 16) + (g << 8) + r;
 }
 
+/* load_texture's pre-pass for sRGB textures (main.c:546-558): each RGBA8 texel is decoded as
+ * byte * f32(1/255) (math.h:326-334), every channel including alpha goes through srgb_to_linear
+ * (math.h:386-394, evaluated in double as written and stored as f32), and the result is
+ * re-encoded with the truncating encode_color_as_u32 (math.h:322-324). */
+static float srgb_to_linear_ref(float v) {
+    if ((double)v <= 0.04045) return (float)((double)v / 12.92);
+    return (float)pow(((double)v + 0.055) / 1.055, 2.4);
+}
+
+void orc_srgb_to_linear_rgba8(const uint32_t *in, uint32_t *out, size_t n) {
+    const float normalizer = (float)(1.0 / 255.0);
+    for (size_t i = 0; i < n; ++i) {
+        float c[4];
+        for (int k = 0; k < 4; ++k) c[k] = srgb_to_linear_ref((float)((in[i] >> (8 * k)) & 255u) * normalizer);
+        out[i] = orc_encode_clear(c);
+    }
+}
+
 /* main.c:1176-1178 output merger: (rne(r*255)<<16) + (rne(g*255)<<8) + rne(b*255), no clamp. */
 uint32_t orc_encode_output(const float rgb[3]) {
     uint32_t r = (uint32_t)cvt_rne(rgb[0] * 255.0f);

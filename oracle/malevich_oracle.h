@@ -23,6 +23,7 @@
  */
 #ifndef MALEVICH_ORACLE_H
 #define MALEVICH_ORACLE_H
+#include <stddef.h>
 #include <stdint.h>
 
 #ifdef __cplusplus
@@ -103,6 +104,7 @@ void orc_bilinear(const orc_texture *t, float u, float v, float out[4]);
 void orc_latlon(const orc_texture *t, const float dir[3], float out[4]);
 uint32_t orc_encode_output(const float rgb[3]);
 uint32_t orc_encode_clear(const float rgba[4]);
+void orc_srgb_to_linear_rgba8(const uint32_t *in, uint32_t *out, size_t n); /* main.c:546-558 */
 float orc_srgb_from_linear_approx(float c);
 float orc_tonemap(float c);
 void orc_normalize3(const float in[3], float out[3]);

@@ -211,3 +211,39 @@ def test_heavy_tiles_multi_draw(device, oracle, depth_func):
         device.set_tuning(saved["heavy_tile_pairs"], saved["heavy_blocks"], saved["pair_blocks"], saved["tile_stage"])
     assert_frames_match(g, o, color_tol=0)
     assert len(np.unique(g.vis_draw)) >= 6
+
+
+@pytest.mark.parametrize("size", [(1200, 720), (37, 23)])
+def test_present_is_the_dib_view_of_the_frame(device, oracle, size):
+    """mlv_present gives the image paint_window shows: each u32 read through the DIB masks
+    R 0x00FF0000, G 0x0000FF00, B 0x000000FF (main.c:305-307), top row first."""
+    W, H = size
+    print(f"case: present {W}x{H}", flush=True)
+    d = device
+    rt, ds = d.create_render_target(W, H), d.create_depth_stencil(W, H)
+    ib = d.create_buffer(L.BUFFER_INDEX_U32, S.SUPREMATISM_INDICES)
+    vb = d.create_buffer(L.BUFFER_VERTEX, S.SUPREMATISM_VERTICES)
+    try:
+        d.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
+        d.ia_set_index_buffer(ib)
+        d.ia_set_vertex_buffer(vb, 32)
+        d.vs_set_shader(L.VS_PASSTHROUGH)
+        d.ps_set_shader(L.PS_PASSTHROUGH)
+        d.rs_set_viewport(0, 0, W, H, 0, 1)
+        d.om_set_render_targets(rt, ds)
+        d.om_set_depth_func(L.DEPTH_GREATER_EQUAL)
+        d.clear_render_target_view(rt, S.CLEAR_COLOR)
+        d.clear_depth_stencil_view(ds, 0.0)
+        d.draw_indexed(S.SUPREMATISM_INDICES.size)
+        img = rt.present()
+        fb = rt.read()
+    finally:
+        for h in (ib, vb, rt, ds):
+            h.release()
+    if W % 8 == 0 and H % 8 == 0:  # the oracle's targets are whole 8x8 bins
+        o = render_oracle(oracle, W, H, [DrawSpec(S.SUPREMATISM_INDICES, S.SUPREMATISM_VERTICES, "passthrough",
+                                                   "passthrough")], S.CLEAR_COLOR)
+        np.testing.assert_array_equal(fb, o.colors)
+    want = np.stack([(fb >> 16) & 255, (fb >> 8) & 255, fb & 255], -1).astype(np.uint8)
+    np.testing.assert_array_equal(img, want)
+    assert len(np.unique(img.reshape(-1, 3), axis=0)) >= 3
