@@ -217,6 +217,24 @@ MLV_API int mlv_get_profile(mlv_context *ctx, mlv_profile *out);
 MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out);
 MLV_API int mlv_reset_stats(mlv_context *ctx);
 
+/* ---- per-view constant buffers (extension of the reference's update(), main.c:1480-1562)
+ *
+ * The camera state the reference keeps (main.c:1422-1430) for each view; width/height give the
+ * aspect as (float)width / height. mlv_camera_constant_buffers writes one PerFrameCB per camera
+ * (48 floats: clip_from_world, view_from_clip, world_from_view; main.c:169-173, 1557-1559) to
+ * device memory, computed on the device as update() does with no input: yaw wrapped into
+ * (-pi, pi], pitch clamped to [-pi/2, pi/2], the yaw and pitch rotations, the view-to-world change
+ * of basis, the position, and the reference's 4x4 inverse (math.h:295-320) and products
+ * (math.h:188-197) in their operation order. Stream-ordered before later draws. */
+typedef struct mlv_camera {
+    float position[3];
+    float yaw_rad, pitch_rad;
+    float fov_y_deg, near_plane;
+    uint32_t width, height;
+} mlv_camera;
+MLV_API int mlv_camera_constant_buffers(mlv_context *ctx, const mlv_camera *cameras, uint32_t count,
+                                        float *device_cbs);
+
 /* ---- graphs (extension; no reference counterpart: the reference runs every draw_indexed to
  * completion, main.c:1219-1261)
  *

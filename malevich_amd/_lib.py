@@ -66,6 +66,12 @@ class DiffGrads(C.Structure):
     _fields_ = [(n, C.c_void_p) for n in ("positions", "colors", "uvs", "normals", "texture", "sh")]
 
 
+class Camera(C.Structure):
+    """mlv_camera"""
+    _fields_ = [("position", C.c_float * 3), ("yaw_rad", C.c_float), ("pitch_rad", C.c_float),
+                ("fov_y_deg", C.c_float), ("near_plane", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
 class Tuning(C.Structure):
     """mlv_tuning"""
     _fields_ = [(n, C.c_uint32) for n in ("heavy_tile_pairs", "heavy_blocks", "pair_blocks", "tile_stage",
@@ -140,6 +146,7 @@ SIGNATURES = {
     "mlv_context_set_profiling": (_I, [_P, _I]),
     "mlv_flush": (_I, [_P]),
     "mlv_get_profile": (_I, [_P, C.POINTER(Profile)]),
+    "mlv_camera_constant_buffers": (_I, [_P, C.POINTER(Camera), _U32, _P]),
     "mlv_get_tuning": (_I, [_P, C.POINTER(Tuning)]),
     "mlv_set_tuning": (_I, [_P, C.POINTER(Tuning)]),
     "mlv_autotune_begin": (_I, [_P]),

@@ -36,7 +36,7 @@ HIP_FLAGS = [
     f"-I{CSRC}",
 ]
 
-SOURCES = ["mlv_kernels.hip", "mlv_runtime.hip", "mlv_diff.hip"]
+SOURCES = ["mlv_kernels.hip", "mlv_runtime.hip", "mlv_diff.hip", "mlv_camera.hip"]
 
 
 def _run(cmd: list[str]) -> None:

@@ -261,6 +261,16 @@ struct ScanScratch {
 hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, uint32_t bits,
                       const ScanScratch &sc, bool *in_alt, hipStream_t s);
 
+// A camera state as the reference keeps it (main.c:1422-1430), for per-view constant buffers
+// (mlv_camera.hip); the layout of mlv_camera.
+struct CameraRec {
+    float position[3];
+    float yaw_rad, pitch_rad, fov_y_deg, near_plane;
+    uint32_t width, height;
+};
+// cbs: n PerFrameCBs of 48 floats (clip_from_world, view_from_clip, world_from_view)
+hipError_t launch_camera_cbs(const CameraRec *cams, uint32_t n, float *cbs, hipStream_t s);
+
 // Launchers (mlv_kernels.hip). All enqueue on `s`.
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
 // n_pixels u32 frame-buffer values to 3 * n_pixels bytes of RGB; rgb holds 12 * ceil(n_pixels / 4)

@@ -134,6 +134,7 @@ struct mlv_context {
     // batched differentiable shading: device tables, FP64 position sums, SH partials
     Arena b_tables, b_posacc, b_shpart;
     Arena present_rgb;  // packed RGB of the last present
+    Arena cameras;      // camera table of the last mlv_camera_constant_buffers
     BatchCounters *ctr = nullptr;
     DevStats *dstats = nullptr;
     uint32_t scan_epoch = 0;  // advanced by every scan launch (tags the scans' status words)
@@ -603,7 +604,7 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->scan_part, &ctx->radix_hist, &ctx->heavy, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
                     &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
-                    &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart, &ctx->present_rgb};
+                    &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart, &ctx->present_rgb, &ctx->cameras};
     for (Arena *a : all) release(*a);
     free_retired(ctx);
     resolve_events(ctx);
@@ -1232,6 +1233,32 @@ MLV_API int mlv_graph_destroy(mlv_context *ctx, mlv_graph *g) {
         ctx->graphs_alive = 0;
         free_retired(ctx);
     }
+    return MLV_OK;
+}
+
+// ------------------------------------------------------------------ cameras
+
+static_assert(sizeof(mlv_camera) == sizeof(CameraRec), "mlv_camera layout");
+
+MLV_API int mlv_camera_constant_buffers(mlv_context *ctx, const mlv_camera *cameras, uint32_t count,
+                                        float *device_cbs) {
+    if (!ctx || (count && (!cameras || !device_cbs)))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "camera_constant_buffers: bad argument");
+    for (uint32_t i = 0; i < count; ++i)
+        if (!cameras[i].width || !cameras[i].height)
+            return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "camera_constant_buffers: camera %u has an empty view", i);
+    if (!count) return MLV_OK;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // recorded draws read the buffers as they were
+    const size_t bytes = sizeof(CameraRec) * count;
+    RC_OR_RETURN(ensure(ctx, ctx->cameras, bytes));
+    void *stage = nullptr;
+    int slot = 0;
+    RC_OR_RETURN(staging_acquire(ctx, bytes, &stage, &slot));
+    memcpy(stage, cameras, bytes);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->cameras.p, stage, bytes, hipMemcpyHostToDevice, ctx->stream));
+    RC_OR_RETURN(staging_commit(ctx, slot));
+    HIP_OR_FAIL(ctx, launch_camera_cbs((const CameraRec *)ctx->cameras.p, count, device_cbs, ctx->stream));
     return MLV_OK;
 }
 

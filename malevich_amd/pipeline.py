@@ -295,6 +295,15 @@ class Device:
     def reset_stats(self) -> None:
         _check(self.ctx, self.lib.mlv_reset_stats(self.ctx))
 
+    # ---------------------------------------------------------- cameras
+    def camera_constant_buffers(self, cameras, device_ptr: int) -> None:
+        """cameras: [(position xyz, yaw_rad, pitch_rad, fov_y_deg, near_plane, width, height)];
+        writes 48 floats per camera (the reference's PerFrameCB) at device_ptr on the device."""
+        arr = (L.Camera * len(cameras))()
+        for k, (pos, yaw, pitch, fov, near, w, h) in enumerate(cameras):
+            arr[k] = L.Camera((C.c_float * 3)(*pos), yaw, pitch, fov, near, w, h)
+        _check(self.ctx, self.lib.mlv_camera_constant_buffers(self.ctx, arr, len(cameras), device_ptr))
+
     # ---------------------------------------------------------- tuning
     def tuning(self) -> dict:
         t = L.Tuning()

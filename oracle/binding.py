@@ -97,6 +97,31 @@ class DiffGrads(C.Structure):
 _lib = None
 
 
+class Camera(C.Structure):
+    _fields_ = [("position", C.c_float * 3), ("yaw_rad", C.c_float), ("pitch_rad", C.c_float),
+                ("fov_y_deg", C.c_float), ("near_plane", C.c_float), ("width", C.c_uint32), ("height", C.c_uint32)]
+
+
+def camera_cb(position, yaw_rad, pitch_rad, fov_y_deg, near_plane, width, height) -> np.ndarray:
+    """The reference's PerFrameCB (48 float32) for a camera state (main.c:1422-1562)."""
+    c = Camera((C.c_float * 3)(*position), yaw_rad, pitch_rad, fov_y_deg, near_plane, width, height)
+    out = np.empty(48, np.float32)
+    lib().orc_camera_cb(C.byref(c), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out
+
+
+def inv4(m: np.ndarray) -> np.ndarray:
+    a = np.ascontiguousarray(m, np.float32).reshape(16)
+    out = np.empty(16, np.float32)
+    lib().orc_inv4(a.ctypes.data_as(C.POINTER(C.c_float)), out.ctypes.data_as(C.POINTER(C.c_float)))
+    return out.reshape(4, 4)
+
+
+def det4(m: np.ndarray) -> float:
+    a = np.ascontiguousarray(m, np.float32).reshape(16)
+    return float(lib().orc_det4(a.ctypes.data_as(C.POINTER(C.c_float))))
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
@@ -129,6 +154,10 @@ def lib() -> C.CDLL:
         L.orc_clip_polygon.restype = C.c_int
         L.orc_threads.restype = C.c_int
         L.orc_srgb_to_linear_rgba8.argtypes = [C.c_void_p, C.c_void_p, C.c_size_t]
+        L.orc_camera_cb.argtypes = [P(Camera), P(C.c_float)]
+        L.orc_det4.argtypes = [P(C.c_float)]
+        L.orc_det4.restype = C.c_float
+        L.orc_inv4.argtypes = [P(C.c_float), P(C.c_float)]
         L.orc_diff_forward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p]
         L.orc_diff_backward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p, P(DiffGrads)]
         _lib = L

@@ -105,6 +105,16 @@ void orc_latlon(const orc_texture *t, const float dir[3], float out[4]);
 uint32_t orc_encode_output(const float rgb[3]);
 uint32_t orc_encode_clear(const float rgba[4]);
 void orc_srgb_to_linear_rgba8(const uint32_t *in, uint32_t *out, size_t n); /* main.c:546-558 */
+
+/* camera state (main.c:1422-1430) -> PerFrameCB (main.c:1480-1562); malevich_camera.c */
+typedef struct orc_camera {
+    float position[3];
+    float yaw_rad, pitch_rad, fov_y_deg, near_plane;
+    uint32_t width, height;
+} orc_camera;
+void orc_camera_cb(const orc_camera *c, float out[48]);
+float orc_det4(const float m[16]);
+void orc_inv4(const float m[16], float r[16]);
 float orc_srgb_from_linear_approx(float c);
 float orc_tonemap(float c);
 void orc_normalize3(const float in[3], float out[3]);
