@@ -889,12 +889,10 @@ struct BinState {
     uint32_t stale;     // updates since wext was last recomputed (uniform)
 };
 
-// A chunk of up to 256 records of a tile's list in LDS, one array per field: the
-// triangle-parallel screening reads a different record per lane, and with one array per field
-// consecutive records sit in consecutive banks (a 96-byte record stride would put 64 lanes on
-// a handful of banks). The rasteriser reads one record at a time as broadcasts.
-// Up to N staged records of a tile's list, one array per field (a wave reading field f of one
-// record reads one address: a broadcast).
+// Up to N staged records of a tile's list in LDS, one array per field: the triangle-parallel
+// screening reads a different record per lane, and with one array per field consecutive records
+// sit in consecutive banks (a 96-byte record stride would put 64 lanes on a handful of banks).
+// The rasteriser reads one record at a time as broadcasts.
 template <uint32_t N>
 struct TileLdsN {
     static constexpr uint32_t kN = N;
