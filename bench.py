@@ -22,6 +22,8 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
+# the library's launch parameters before any autotuning (mlv_tuning)
+DEFAULT_TUNING = dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048, tile_stage=256)
 
 
 class HipEvents:
@@ -199,7 +201,12 @@ def bench_c3(dev, args):
         dev.clear_depth_stencil_view(ds, 0.0)
         dev.draw_indexed(n_idx)
 
+    dev.set_tuning(**DEFAULT_TUNING)  # tuned for this frame, not the previous leg's
     warm(dev, one, args.warmup)
+    dev.autotune_begin()
+    one()
+    tuning = dev.autotune_end(repeats=2)
+    print(f"[bench] c3 autotuned: {tuning}", file=sys.stderr, flush=True)
     dev.reset_stats()
     ev = HipEvents()
     a, b = ev.create(), ev.create()
@@ -219,7 +226,7 @@ def bench_c3(dev, args):
             "gpix_per_s": round(W * H / (ms * 1e-3) / 1e9, 4), "steps": args.c3_steps,
             "assembled_per_frame": int(stats["assembled_triangle_count"] / args.c3_steps),
             "pairs_per_frame": int(stats["total_triangle_count_in_bins"] / args.c3_steps),
-            "scene_gen_s": round(gen_s, 1)}
+            "scene_gen_s": round(gen_s, 1), "tuning": tuning}
 
 
 def bench_c4(dev, args):
@@ -239,6 +246,7 @@ def bench_c4(dev, args):
     idx = cuda(m.indices.astype(np.int32))
     M = torch.from_numpy(sc.clip_from_world.copy())
     rng = np.random.default_rng(args.seed + 4)
+    dev.set_tuning(**DEFAULT_TUNING)  # not the previous leg's tuning
     out = {}
     for det in (False, True):
         r = D.DiffRenderer(dev, sc.width, sc.height, deterministic=det)
@@ -328,6 +336,7 @@ def bench_c5(dev, args):
         dev.synchronize()
         return ms
 
+    dev.set_tuning(**DEFAULT_TUNING)
     warm(dev, lambda: frame(False), 1)
     dev.autotune_begin()
     frame(False)
