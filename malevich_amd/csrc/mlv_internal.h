@@ -122,6 +122,7 @@ struct BatchCounters {
     uint32_t radix_blocks;                 // blocks of the radix passes over n_pairs
     uint32_t hist_n;                       // radix_blocks * 256 digit counters
     uint32_t n_heavy;                      // tiles whose list is longer than heavy_min
+    uint32_t n_long, n_short;              // tiles placed at the front / back of tile_order
 };
 
 struct DevStats {
@@ -155,6 +156,7 @@ struct BatchParams {
     uint32_t pair_cap;
     uint32_t *tile_cnt, *tile_start;      // per global tile: end and start of its range in the sorted pairs
     uint32_t *heavy;                      // tiles with more than heavy_min pairs (n_heavy of them)
+    uint32_t *tile_order;                 // the order the tile kernel takes tiles in: long lists first
     uint32_t heavy_min, heavy_blocks, pair_blocks;
     uint32_t tile_stage;                  // records the tile kernel stages in LDS at a time (128 or 256)
     uint32_t *scan_status;                // the scans' status words, cleared by the first stage

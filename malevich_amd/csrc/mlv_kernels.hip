@@ -1297,6 +1297,33 @@ __global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint3
     }
 }
 
+// The order kb_tile takes tiles in: tiles whose list is more than four times the frame's average
+// (and over 64 entries) first, the rest after them. Per-tile work varies tenfold where meshes
+// crowd; started last, the longest tiles would decide when the launch ends. Each wave claims its
+// slots with one atomic per class.
+__global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
+    const uint32_t n = p.total_tiles;
+    const bool ovf = (p.ctr->overflow & OVF_PAIRS) != 0u;
+    const uint32_t avg = n ? p.ctr->n_pairs / n : 0u;
+    const uint32_t thr = max(64u, 4u * avg);
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += gridDim.x * 256u) {
+        const uint32_t g = base + lane;
+        const bool valid = g < n;
+        const bool is_long = valid && !ovf && p.tile_cnt[g] - p.tile_start[g] > thr;
+        const uint64_t bl = __ballot(is_long), bs = __ballot(valid && !is_long);
+        uint32_t first_long = 0u, first_short = 0u;
+        if (lane == 0u) {
+            if (bl) first_long = atomicAdd(&p.ctr->n_long, (uint32_t)__popcll(bl));
+            if (bs) first_short = atomicAdd(&p.ctr->n_short, (uint32_t)__popcll(bs));
+        }
+        first_long = __shfl(first_long, 0, 64);
+        first_short = __shfl(first_short, 0, 64);
+        if (is_long) p.tile_order[first_long + mbcnt(bl)] = g;
+        else if (valid) p.tile_order[n - 1u - (first_short + mbcnt(bs))] = g;
+    }
+}
+
 // One block per 16x16 tile of every view, one wave per 8x8 bin. The block stages up to 256
 // records of the tile's sorted list in LDS; each wave compacts the ones touching its bin and
 // walks them in submission order with its pixels' depths in registers; afterwards each pixel's
@@ -1309,9 +1336,9 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
     __shared__ uint32_t s_active[4];
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
-    // The dispatcher deals consecutive blocks to the 8 XCDs in turn, so neighbouring tiles
-    // (whose triangle counts are alike) spread over all XCDs and their load evens out.
-    const uint32_t g = blockIdx.x;
+    // Tiles in kb_tile_order's order; the dispatcher deals consecutive blocks to the 8 XCDs in
+    // turn, so the long tiles at the front spread over all XCDs.
+    const uint32_t g = p.tile_order[blockIdx.x];
     const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
     if (threadIdx.x < 4u) s_active[threadIdx.x] = 0u;
     if (p.views[vi].depth_func == DEPTH_GE)
@@ -1420,6 +1447,8 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
                            s, p, sorted);
         hipLaunchKernelGGL(kb_tile_heavy, dim3(std::max(1u, std::min<uint32_t>(p.total_tiles, p.heavy_blocks))),
                            dim3(1024), 0, s, p, sorted);
+        hipLaunchKernelGGL(kb_tile_order, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 1024u)), dim3(256),
+                           0, s, p);
         if (tile_ev && (e = hipEventRecord(tile_ev[0], s)) != hipSuccess) return e;
         if (p.tile_stage == 128u)
             hipLaunchKernelGGL(kb_tile<128>, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);

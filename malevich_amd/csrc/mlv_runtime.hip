@@ -127,7 +127,7 @@ struct mlv_context {
     uint64_t pending_tris = 0, pending_verts = 0;
     // arenas
     Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, tvis, tshade, tbin;
-    Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist, heavy;
+    Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist, heavy, tile_order;
     // differentiable shading scratch
     Arena d_shpart, d_tkey, d_tval, d_tkey2, d_tval2, d_tpay, d_vkey, d_vval, d_vkey2, d_vval2, d_vpay, d_part, d_hist;
     Arena d_vposd, d_posacc;
@@ -411,6 +411,7 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->tile_cnt, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->tile_start, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->heavy, (size_t)(total_tiles + 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->tile_order, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_scratch_bytes(scan_n)));  // cleared by the first stage
     RC_OR_RETURN(ensure(ctx, ctx->radix_hist, radix_hist_elems(pair_cap) * 4));
 
@@ -470,6 +471,7 @@ static int flush(mlv_context *ctx) {
     p.tile_cnt = (uint32_t *)ctx->tile_cnt.p;
     p.tile_start = (uint32_t *)ctx->tile_start.p;
     p.heavy = (uint32_t *)ctx->heavy.p;
+    p.tile_order = (uint32_t *)ctx->tile_order.p;
     p.heavy_min = ctx->heavy_min;
     p.heavy_blocks = ctx->heavy_blocks;
     p.pair_blocks = ctx->pair_blocks;
@@ -601,7 +603,7 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->blocksum,  &ctx->blockoff,  &ctx->asm_base, &ctx->clip_list, &ctx->clip_off,
                     &ctx->tvis,      &ctx->tshade,    &ctx->tbin,    &ctx->pair_cnt, &ctx->pair_off,  &ctx->keys0,
                     &ctx->vals0,     &ctx->keys1,     &ctx->vals1,    &ctx->tile_cnt,  &ctx->tile_start,
-                    &ctx->scan_part, &ctx->radix_hist, &ctx->heavy, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
+                    &ctx->scan_part, &ctx->radix_hist, &ctx->heavy, &ctx->tile_order, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
                     &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
                     &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart, &ctx->present_rgb, &ctx->cameras};
