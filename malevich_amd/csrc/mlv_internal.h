@@ -122,7 +122,7 @@ struct BatchCounters {
     uint32_t radix_blocks;                 // blocks of the radix passes over n_pairs
     uint32_t hist_n;                       // radix_blocks * 256 digit counters
     uint32_t n_heavy;                      // tiles whose list is longer than heavy_min
-    uint32_t n_long, n_short;              // tiles placed at the front / back of tile_order
+    uint32_t class_cnt[8], class_cur[8];   // tiles per list-length class; slots taken so far (tile_order)
 };
 
 struct DevStats {

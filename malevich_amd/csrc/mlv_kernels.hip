@@ -1297,30 +1297,54 @@ __global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint3
     }
 }
 
-// The order kb_tile takes tiles in: tiles whose list is more than four times the frame's average
-// (and over 64 entries) first, the rest after them. Per-tile work varies tenfold where meshes
-// crowd; started last, the longest tiles would decide when the launch ends. Each wave claims its
-// slots with one atomic per class.
+// The order kb_tile takes tiles in: longest lists first (largest-first scheduling). Per-tile work
+// varies tenfold where meshes crowd; started last, the longest tiles would decide when the launch
+// ends. Tiles fall into eight classes by list length against the frame's average (>= 32x, 16x,
+// 8x, 4x, 2x, 1x, shorter, empty); one pass counts the classes, the next places every tile after
+// the longer classes, each wave claiming its slots with one atomic per class.
+__device__ __forceinline__ uint32_t tile_class(const BatchParams &p, uint32_t g, bool ovf, uint32_t avg) {
+    if (ovf) return 0u;  // the slow path walks every triangle in every tile: one class
+    const uint32_t len = p.tile_cnt[g] - p.tile_start[g];
+    if (len == 0u) return 7u;
+    uint32_t c = 6u;
+    for (uint32_t k = 0; k < 6u; ++k)
+        if (len >= (avg << (5u - k))) {
+            c = k;
+            break;
+        }
+    return c;
+}
+
+template <bool PLACE>
 __global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
     const uint32_t n = p.total_tiles;
     const bool ovf = (p.ctr->overflow & OVF_PAIRS) != 0u;
-    const uint32_t avg = n ? p.ctr->n_pairs / n : 0u;
-    const uint32_t thr = max(64u, 4u * avg);
+    const uint32_t avg = max(1u, n ? p.ctr->n_pairs / n : 1u);
     const uint32_t lane = threadIdx.x & 63u;
+    uint32_t start[8];
+    if (PLACE) {
+        uint32_t acc = 0u;
+        for (int c = 0; c < 8; ++c) {
+            start[c] = acc;
+            acc += p.ctr->class_cnt[c];
+        }
+    }
     for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += gridDim.x * 256u) {
         const uint32_t g = base + lane;
-        const bool valid = g < n;
-        const bool is_long = valid && !ovf && p.tile_cnt[g] - p.tile_start[g] > thr;
-        const uint64_t bl = __ballot(is_long), bs = __ballot(valid && !is_long);
-        uint32_t first_long = 0u, first_short = 0u;
-        if (lane == 0u) {
-            if (bl) first_long = atomicAdd(&p.ctr->n_long, (uint32_t)__popcll(bl));
-            if (bs) first_short = atomicAdd(&p.ctr->n_short, (uint32_t)__popcll(bs));
+        const uint32_t cls = g < n ? tile_class(p, g, ovf, avg) : 8u;
+#pragma unroll
+        for (uint32_t c = 0; c < 8u; ++c) {
+            const uint64_t b = __ballot(cls == c);
+            if (!b) continue;
+            if (!PLACE) {
+                if (lane == 0u) atomicAdd(&p.ctr->class_cnt[c], (uint32_t)__popcll(b));
+            } else {
+                uint32_t first = 0u;
+                if (lane == 0u) first = atomicAdd(&p.ctr->class_cur[c], (uint32_t)__popcll(b));
+                first = __shfl(first, 0, 64);
+                if (cls == c) p.tile_order[start[c] + first + mbcnt(b)] = g;
+            }
         }
-        first_long = __shfl(first_long, 0, 64);
-        first_short = __shfl(first_short, 0, 64);
-        if (is_long) p.tile_order[first_long + mbcnt(bl)] = g;
-        else if (valid) p.tile_order[n - 1u - (first_short + mbcnt(bs))] = g;
     }
 }
 
@@ -1443,12 +1467,15 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
     case 3: {
         if (!p.n_tblocks || !p.total_tiles) break;
         const uint32_t *sorted = (radix_passes & 1u) ? p.vals1 : p.vals0;
-        hipLaunchKernelGGL(kb_heavy_tiles, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 256u)), dim3(256), 0,
-                           s, p, sorted);
-        hipLaunchKernelGGL(kb_tile_heavy, dim3(std::max(1u, std::min<uint32_t>(p.total_tiles, p.heavy_blocks))),
-                           dim3(1024), 0, s, p, sorted);
-        hipLaunchKernelGGL(kb_tile_order, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 1024u)), dim3(256),
-                           0, s, p);
+        if (p.heavy_min != 0xFFFFFFFFu) {  // the 16-wave path is on
+            hipLaunchKernelGGL(kb_heavy_tiles, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 256u)), dim3(256),
+                               0, s, p, sorted);
+            hipLaunchKernelGGL(kb_tile_heavy, dim3(std::max(1u, std::min<uint32_t>(p.total_tiles, p.heavy_blocks))),
+                               dim3(1024), 0, s, p, sorted);
+        }
+        const uint32_t ogrid = std::min<uint32_t>((p.total_tiles + 255) / 256, 1024u);
+        hipLaunchKernelGGL(kb_tile_order<false>, dim3(ogrid), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(kb_tile_order<true>, dim3(ogrid), dim3(256), 0, s, p);
         if (tile_ev && (e = hipEventRecord(tile_ev[0], s)) != hipSuccess) return e;
         if (p.tile_stage == 128u)
             hipLaunchKernelGGL(kb_tile<128>, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
