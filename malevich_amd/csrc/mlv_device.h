@@ -327,80 +327,4 @@ __device__ __forceinline__ void rect_of(const TriVis &tv, int32_t &bx0, int32_t 
     by1 = (int32_t)(int16_t)(tv.rect1 >> 16);
 }
 
-// A polygon of up to kMaxClip vertices x 12 floats for one lane, kept in LDS with the
-// lane as the fastest index (a wave's accesses hit 64 consecutive banks).
-struct LdsPoly {
-    float *base;  // &buf[0][0][lane], stride 64 floats between elements
-    __device__ __forceinline__ float &at(int v, int c) const { return base[(v * 12 + c) * 64]; }
-};
-
-// clip_by_plane, main.c:609-647 (plane_d = 0), on LDS polygons, for the first NC floats of each
-// vertex: the clip-space position is components 0..3, so NC = 4 gives the exact polygon shape
-// (what counting needs) and NC = 12 the attributes as well.
-template <int NC>
-__device__ inline int clip_by_plane_lds(const LdsPoly &v, int n, const float *pl, const LdsPoly &out) {
-    int k = 0;
-    if (n <= 0) return 0;
-    float cur = dot4_seq(pl[0], pl[1], pl[2], pl[3], v.at(0, 0), v.at(0, 1), v.at(0, 2), v.at(0, 3));
-    const float plane_d = 0.0f;
-    bool cur_in = cur > -plane_d;
-    for (int i = 0; i < n; ++i) {
-        if (k >= kMaxClip) return -1;
-        int nx = (i + 1) % n;
-        if (cur_in) {
-            for (int c = 0; c < NC; ++c) out.at(k, c) = v.at(i, c);
-            k++;
-        }
-        float nd = dot4_seq(pl[0], pl[1], pl[2], pl[3], v.at(nx, 0), v.at(nx, 1), v.at(nx, 2), v.at(nx, 3));
-        bool nx_in = nd > -plane_d;
-        if (cur_in != nx_in) {
-            if (k >= kMaxClip) return -1;
-            float t = (plane_d + cur) / (cur - nd);
-            float omt = 1.0f - t;
-            for (int c = 0; c < NC; ++c) {
-                float a = v.at(i, c) * omt;
-                float b = v.at(nx, c) * t;
-                out.at(k, c) = a + b;
-            }
-            k++;
-        }
-        cur = nd;
-        cur_in = nx_in;
-    }
-    return k;
-}
-
-// clip_by_plane, main.c:609-647 (plane_d = 0). v holds n vertices of 12 floats.
-__device__ inline int clip_by_plane(float (*v)[12], int n, const float *pl, float (*out)[12]) {
-    int k = 0;
-    if (n <= 0) return 0;
-    float cur = dot4_seq(pl[0], pl[1], pl[2], pl[3], v[0][0], v[0][1], v[0][2], v[0][3]);
-    const float plane_d = 0.0f;
-    bool cur_in = cur > -plane_d;
-    for (int i = 0; i < n; ++i) {
-        if (k >= kMaxClip) return -1;
-        int nx = (i + 1) % n;
-        if (cur_in) {
-            for (int c = 0; c < 12; ++c) out[k][c] = v[i][c];
-            k++;
-        }
-        float nd = dot4_seq(pl[0], pl[1], pl[2], pl[3], v[nx][0], v[nx][1], v[nx][2], v[nx][3]);
-        bool nx_in = nd > -plane_d;
-        if (cur_in != nx_in) {
-            if (k >= kMaxClip) return -1;
-            float t = (plane_d + cur) / (cur - nd);
-            float omt = 1.0f - t;
-            for (int c = 0; c < 12; ++c) {
-                float a = v[i][c] * omt;
-                float b = v[nx][c] * t;
-                out[k][c] = a + b;
-            }
-            k++;
-        }
-        cur = nd;
-        cur_in = nx_in;
-    }
-    return k;
-}
-
 }  // namespace mlv

@@ -145,7 +145,7 @@ struct BatchParams {
     uint32_t *cnt;
     unsigned long long *blocksum, *blockoff;  // per setup block: assembled | clipverts << 32
     uint32_t *asm_base;                   // per draw: first global assembled index
-    uint32_t *clip_list;
+    uint2 *clip_list;       // (triangle slot, draw) of each triangle the clipper takes
     uint2 *clip_off;
     TriVis *tvis;
     TriShade *tshade;

@@ -99,50 +99,119 @@ __device__ __forceinline__ bool load_tri_indices(const DrawRec &d, uint32_t t, u
     return true;
 }
 
-// The clipper's polygon: the three vertices' registers, then the six planes (main.c:649-660).
-// Two LDS polygons per lane (ping-pong); the result is left in `a` and its size returned.
-constexpr int kClipLanes = 64;
-constexpr size_t kClipLdsFloats = (size_t)2 * kMaxClip * 12 * kClipLanes;
+// ------------------------------------------------------------ the clipper
+//
+// Sutherland-Hodgman against the six planes (main.c:609-660), one triangle per group of 16
+// lanes: lane i of the group holds polygon vertex i (NC floats in registers; NC = 4 is the
+// clip-space position, which decides the polygon's shape, NC = 12 adds the attributes). A plane
+// is one parallel step: each vertex's signed distance, the next vertex's by a shuffle, the
+// outputs' slots by a prefix count over two ballots, and one pass through LDS to put output k in
+// lane k. The outputs, their order and their arithmetic are those of the serial loop, so the
+// polygons are bit-identical to it.
+constexpr uint32_t kClipGroup = 16;  // lanes per triangle: a clipped polygon has at most kMaxClip vertices
+static_assert(kMaxClip <= (int)kClipGroup, "one lane per polygon vertex");
 
+struct ClipLanes {
+    uint32_t lane;   // lane in the wave
+    uint32_t li;     // lane in the group = the vertex it holds
+    uint32_t g0;     // the group's first lane
+    uint64_t gmask;  // the group's lanes
+    __device__ __forceinline__ ClipLanes() {
+        lane = threadIdx.x & 63u;
+        li = lane & (kClipGroup - 1u);
+        g0 = lane & ~(kClipGroup - 1u);
+        gmask = 0xFFFFull << g0;
+    }
+    __device__ __forceinline__ uint64_t below() const { return (1ull << lane) - 1ull; }
+};
+
+// One clip_by_plane (main.c:609-647, plane_d = 0) of the group's n-vertex polygon. `s` is the
+// group's LDS, NC x 16 floats. Returns the new vertex count, or -1 where the serial loop gives up
+// (an output slot beyond MAX_NUM_CLIP_VERTICES: its check at the top of each vertex and before
+// each intersection fails exactly when more than kMaxClip vertices come out or kMaxClip are out
+// before the last input vertex).
 template <int NC>
-__device__ __noinline__ int clip_triangle(const BatchParams &p, const DrawRec &d, uint32_t t, LdsPoly &a,
-                                          LdsPoly &b) {
-    uint32_t vi[3];
-    if (!load_tri_indices(d, t, vi)) return 0;
-    for (int k = 0; k < 3; ++k) {
-        const uint32_t slot = d.vbase + vi[k];
-        const float4 x = p.vpos[slot];
-        a.at(k, 0) = x.x; a.at(k, 1) = x.y; a.at(k, 2) = x.z; a.at(k, 3) = x.w;
-        if (NC > 4) {
-            const float4 y = p.vattr[2u * slot], z = p.vattr[2u * slot + 1u];
-            a.at(k, 4) = y.x; a.at(k, 5) = y.y; a.at(k, 6) = y.z; a.at(k, 7) = y.w;
-            a.at(k, 8) = z.x; a.at(k, 9) = z.y; a.at(k, 10) = z.z; a.at(k, 11) = z.w;
+__device__ __forceinline__ int clip_plane_group(const ClipLanes &g, float (&x)[NC], int n, const float *pl, float *s) {
+    const bool act = g.li < (uint32_t)n;
+    const float cur = dot4_seq(pl[0], pl[1], pl[2], pl[3], x[0], x[1], x[2], x[3]);
+    const bool cin = cur > -0.0f;
+    const uint64_t b_in = __ballot(act && cin) & g.gmask;
+    const uint64_t b_act = (((1ull << (uint32_t)n) - 1ull) << g.g0);  // n <= 16
+    if (b_in == b_act) return n;  // every vertex inside: the plane returns the polygon unchanged
+    const uint32_t nx = g.g0 + (g.li + 1u < (uint32_t)n ? g.li + 1u : 0u);
+    const float nd = __shfl(cur, (int)nx, 64);
+    float xn[NC];
+#pragma unroll
+    for (int c = 0; c < NC; ++c) xn[c] = __shfl(x[c], (int)nx, 64);
+    const bool nin = nd > -0.0f;
+    const bool keep = act && cin, cross = act && (cin != nin);
+    const uint64_t b1 = __ballot(keep) & g.gmask, b2 = __ballot(cross) & g.gmask;
+    const uint32_t total = (uint32_t)(__popcll(b1) + __popcll(b2));
+    const uint64_t before_last = (1ull << (g.g0 + (uint32_t)n - 1u)) - 1ull;
+    const uint32_t k_last = (uint32_t)(__popcll(b1 & before_last) + __popcll(b2 & before_last));
+    if (total > (uint32_t)kMaxClip || k_last >= (uint32_t)kMaxClip) return -1;
+    const uint32_t k = (uint32_t)(__popcll(b1 & g.below()) + __popcll(b2 & g.below()));
+    if (keep) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) s[c * kClipGroup + k] = x[c];
+    }
+    if (cross) {
+        const float t = (0.0f + cur) / (cur - nd);
+        const float omt = 1.0f - t;
+        const uint32_t kc = k + (keep ? 1u : 0u);
+#pragma unroll
+        for (int c = 0; c < NC; ++c) {
+            const float a = x[c] * omt;
+            const float b = xn[c] * t;
+            s[c * kClipGroup + kc] = a + b;
         }
+    }
+    // a wave's LDS operations complete in issue order; the barrier keeps the compiler from
+    // moving the reads above the writes (or the next plane's writes above these reads)
+    __builtin_amdgcn_wave_barrier();
+    if (g.li < total) {
+#pragma unroll
+        for (int c = 0; c < NC; ++c) x[c] = s[c * kClipGroup + g.li];
+    }
+    __builtin_amdgcn_wave_barrier();
+    return (int)total;
+}
+
+// The clipper (main.c:649-660) on triangle t of draw d: loads the three vertices into the
+// group's first three lanes and runs the six planes. Returns the polygon's vertex count (0 when
+// it is empty or the clipper gave up); vertex i is left in lane i of the group.
+template <int NC>
+__device__ __forceinline__ int clip_group(const BatchParams &p, const DrawRec &d, uint32_t t, const ClipLanes &g,
+                                          float (&x)[NC], float *s) {
+    uint32_t vi[3];
+    if (!load_tri_indices(d, t, vi)) return 0;  // the same for the whole group
+    const uint32_t slot = d.vbase + (g.li == 1u ? vi[1] : g.li == 2u ? vi[2] : vi[0]);
+    const float4 q = p.vpos[slot];
+    x[0] = q.x; x[1] = q.y; x[2] = q.z; x[3] = q.w;
+    if constexpr (NC > 4) {
+        const float4 y = p.vattr[2u * slot], z = p.vattr[2u * slot + 1u];
+        x[4] = y.x; x[5] = y.y; x[6] = y.z; x[7] = y.w;
+        x[8] = z.x; x[9] = z.y; x[10] = z.z; x[11] = z.w;
     }
     int n = 3;
     for (int pl = 0; pl < 6; ++pl) {
-        // Sutherland-Hodgman returns the polygon unchanged, in order, when every vertex is
-        // inside; skipping that plane is exact and saves the copy
-        bool all_in = true;
-        for (int k = 0; k < n; ++k)
-            all_in = all_in && dot4_seq(p.planes[pl][0], p.planes[pl][1], p.planes[pl][2], p.planes[pl][3], a.at(k, 0),
-                                        a.at(k, 1), a.at(k, 2), a.at(k, 3)) > -0.0f;
-        if (all_in) continue;
-        n = clip_by_plane_lds<NC>(a, n, p.planes[pl], b);
+        n = clip_plane_group<NC>(g, x, n, p.planes[pl], s);
         if (n < 0) return 0;
-        LdsPoly t2 = a;
-        a = b;
-        b = t2;
     }
     return n;
 }
 
-__device__ __forceinline__ float4 poly_pos(const LdsPoly &q, int v) {
-    return make_float4(q.at(v, 0), q.at(v, 1), q.at(v, 2), q.at(v, 3));
-}
-
-__device__ __forceinline__ uint32_t draw_of_slot(const BatchParams &p, uint32_t slot) {
-    return upper_index(p.n_draws, slot, [&](uint32_t i) { return p.draws[i].tbase; });
+// Fan triangle (0, li, li + 1) of the group's polygon (main.c:795-798): the first vertex's
+// and the next vertex's positions, read from their lanes.
+template <int NC>
+__device__ __forceinline__ void fan_positions(const ClipLanes &g, const float (&x)[NC], float4 &p0, float4 &p1,
+                                              float4 &p2) {
+    const uint32_t nx = g.g0 + (g.li + 1u < kClipGroup ? g.li + 1u : g.li);
+    p0 = make_float4(__shfl(x[0], (int)g.g0, 64), __shfl(x[1], (int)g.g0, 64), __shfl(x[2], (int)g.g0, 64),
+                     __shfl(x[3], (int)g.g0, 64));
+    p1 = make_float4(x[0], x[1], x[2], x[3]);
+    p2 = make_float4(__shfl(x[0], (int)nx, 64), __shfl(x[1], (int)nx, 64), __shfl(x[2], (int)nx, 64),
+                     __shfl(x[3], (int)nx, 64));
 }
 
 // ------------------------------------------------------------ fills
@@ -424,7 +493,7 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
                 packed = setup_triangle(d, v, a, bb, c, tv, rw) ? 1u : 0u;
             } else if (cls == 2) {
                 uint32_t k = atomicAdd(&p.ctr->n_clip_list, 1u);
-                p.clip_list[k] = slot;
+                p.clip_list[k] = make_uint2(slot, di);
             }
         }
     }
@@ -435,24 +504,34 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
     if (threadIdx.x == 0) p.blocksum[b] = (unsigned long long)(s_w[0] + s_w[1] + s_w[2] + s_w[3]);
 }
 
-__global__ __launch_bounds__(64) void kb_clip_count(BatchParams p) {
-    __shared__ float s_poly[kClipLdsFloats];
+// Triangles that need the clipper, one per 16-lane group: the polygon's fan triangles that
+// survive setup are counted (packed with the polygon's vertex count, like kb_setup_count's).
+__global__ __launch_bounds__(256) void kb_clip_count(BatchParams p) {
+    __shared__ float s_poly[256 / kClipGroup][4 * kClipGroup];
+    const ClipLanes g;
+    const uint32_t grp = threadIdx.x / kClipGroup;
     const uint32_t n = p.ctr->n_clip_list;
-    for (uint32_t s = blockIdx.x * 64u + threadIdx.x; s < n; s += gridDim.x * 64u) {
-        const uint32_t slot = p.clip_list[s];
-        const DrawRec &d = p.draws[draw_of_slot(p, slot)];
+    for (uint32_t s = blockIdx.x * (256u / kClipGroup) + grp; s < n; s += gridDim.x * (256u / kClipGroup)) {
+        const uint2 e = p.clip_list[s];  // (slot, draw)
+        const DrawRec &d = p.draws[e.y];
         const ViewRec &v = p.views[d.view];
-        LdsPoly pa{s_poly + threadIdx.x}, pb{s_poly + (size_t)kMaxClip * 12 * kClipLanes + threadIdx.x};
-        int nv = clip_triangle<4>(p, d, slot - d.tbase, pa, pb);  // positions decide count and cull
-        uint32_t ntri = 0u;
-        for (int ci = 1; ci < nv - 1; ++ci) {
+        float x[4];
+        const int nv = clip_group<4>(p, d, e.x - d.tbase, g, x, s_poly[grp]);  // positions decide count and cull
+        float4 q0, q1, q2;
+        fan_positions<4>(g, x, q0, q1, q2);
+        bool live = false;
+        if (g.li >= 1u && (int)g.li < nv - 1) {
             TriVis tv;
             float rw[3];
-            if (setup_triangle(d, v, poly_pos(pa, 0), poly_pos(pa, ci), poly_pos(pa, ci + 1), tv, rw)) ntri++;
+            live = setup_triangle(d, v, q0, q1, q2, tv, rw);
         }
-        uint32_t packed = ntri ? (ntri | ((uint32_t)nv << 16)) : 0u;
-        p.cnt[slot] = packed;
-        if (packed) atomicAdd(&p.blocksum[slot >> 8], (unsigned long long)ntri | ((unsigned long long)nv << 32));
+        const uint32_t ntri = (uint32_t)__popcll(__ballot(live) & g.gmask);
+        if (g.li == 0u) {
+            const uint32_t packed = ntri ? (ntri | ((uint32_t)nv << 16)) : 0u;
+            p.cnt[e.x] = packed;
+            if (packed)
+                atomicAdd(&p.blocksum[e.x >> 8], (unsigned long long)ntri | ((unsigned long long)nv << 32));
+        }
     }
 }
 
@@ -522,36 +601,38 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     }
 }
 
-__global__ __launch_bounds__(64) void kb_clip_emit(BatchParams p) {
-    __shared__ float s_poly[kClipLdsFloats];
+// The clipped triangles' output, one per 16-lane group: lane k writes polygon vertex k's
+// attribute row, and lane ci the fan triangle (0, ci, ci + 1) if it survives setup, at the
+// offsets kb_setup_emit scanned.
+__global__ __launch_bounds__(256) void kb_clip_emit(BatchParams p) {
+    __shared__ float s_poly[256 / kClipGroup][12 * kClipGroup];
+    const ClipLanes g;
+    const uint32_t grp = threadIdx.x / kClipGroup;
     const uint32_t n = p.ctr->n_clip_list;
-    for (uint32_t s = blockIdx.x * 64u + threadIdx.x; s < n; s += gridDim.x * 64u) {
-        const uint32_t slot = p.clip_list[s];
-        if (p.cnt[slot] == 0u) continue;
-        const uint32_t di = draw_of_slot(p, slot);
+    for (uint32_t s = blockIdx.x * (256u / kClipGroup) + grp; s < n; s += gridDim.x * (256u / kClipGroup)) {
+        const uint2 e = p.clip_list[s];  // (slot, draw)
+        if (p.cnt[e.x] == 0u) continue;
+        const uint32_t di = e.y;
         const DrawRec &d = p.draws[di];
         const ViewRec &v = p.views[d.view];
-        const uint2 off = p.clip_off[slot];
-        const uint32_t t = slot - d.tbase;
-        LdsPoly pa{s_poly + threadIdx.x}, pb{s_poly + (size_t)kMaxClip * 12 * kClipLanes + threadIdx.x};
-        int nv = clip_triangle<12>(p, d, t, pa, pb);
+        const uint2 off = p.clip_off[e.x];
+        const uint32_t t = e.x - d.tbase;
+        float x[12];
+        const int nv = clip_group<12>(p, d, t, g, x, s_poly[grp]);
         const uint32_t row0 = p.vslots + off.y;
-        for (int k = 0; k < nv; ++k) {
-            uint32_t row = row0 + (uint32_t)k;
-            if (row >= p.vattr_rows) break;
-            p.vattr[2u * row] = make_float4(pa.at(k, 4), pa.at(k, 5), pa.at(k, 6), pa.at(k, 7));
-            p.vattr[2u * row + 1u] = make_float4(pa.at(k, 8), pa.at(k, 9), pa.at(k, 10), pa.at(k, 11));
+        const uint32_t row = row0 + g.li;
+        if ((int)g.li < nv && row < p.vattr_rows) {
+            p.vattr[2u * row] = make_float4(x[4], x[5], x[6], x[7]);
+            p.vattr[2u * row + 1u] = make_float4(x[8], x[9], x[10], x[11]);
         }
-        uint32_t j = 0u;
-        for (int ci = 1; ci < nv - 1; ++ci) {
-            TriVis tv;
-            float rw[3];
-            if (setup_triangle(d, v, poly_pos(pa, 0), poly_pos(pa, ci), poly_pos(pa, ci + 1), tv, rw)) {
-                write_triangle(p, off.x + j, di, tv, rw, row0, row0 + (uint32_t)ci, row0 + (uint32_t)ci + 1u, t,
-                               (uint32_t)(ci - 1));
-                j++;
-            }
-        }
+        float4 q0, q1, q2;
+        fan_positions<12>(g, x, q0, q1, q2);
+        bool live = false;
+        TriVis tv;
+        float rw[3];
+        if (g.li >= 1u && (int)g.li < nv - 1) live = setup_triangle(d, v, q0, q1, q2, tv, rw);
+        const uint32_t j = (uint32_t)__popcll(__ballot(live) & g.gmask & g.below());
+        if (live) write_triangle(p, off.x + j, di, tv, rw, row0, row, row + 1u, t, g.li - 1u);
     }
 }
 
@@ -1428,13 +1509,13 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
     case 1:
         if (!p.n_tblocks) break;
         hipLaunchKernelGGL(kb_setup_count, dim3(p.n_tblocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(kb_clip_count, dim3(1024), dim3(64), 0, s, p);
+        hipLaunchKernelGGL(kb_clip_count, dim3(1024), dim3(256), 0, s, p);
         if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc,
                                                  &p.ctr->n_assembled_clip, s)) != hipSuccess)
             return e;
         hipLaunchKernelGGL(kb_setup_totals, dim3((p.n_draws + 255) / 256), dim3(256), 0, s, p);
         hipLaunchKernelGGL(kb_setup_emit, dim3(p.n_tblocks), dim3(256), 0, s, p);
-        hipLaunchKernelGGL(kb_clip_emit, dim3(1024), dim3(64), 0, s, p);
+        hipLaunchKernelGGL(kb_clip_emit, dim3(1024), dim3(256), 0, s, p);
         break;
     case 2: {
         if (!p.n_tblocks) break;

@@ -397,7 +397,7 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->blocksum, (size_t)(n_tblocks + 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->blockoff, (size_t)(n_tblocks + 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->asm_base, (size_t)n_draws * 4));
-    RC_OR_RETURN(ensure(ctx, ctx->clip_list, std::max<size_t>(T, 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->clip_list, std::max<size_t>(T, 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->clip_off, std::max<size_t>(T, 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->tvis, (size_t)tri_cap * sizeof(TriVis)));
     RC_OR_RETURN(ensure(ctx, ctx->tshade, (size_t)tri_cap * sizeof(TriShade)));
@@ -455,7 +455,7 @@ static int flush(mlv_context *ctx) {
     p.blocksum = (unsigned long long *)ctx->blocksum.p;
     p.blockoff = (unsigned long long *)ctx->blockoff.p;
     p.asm_base = (uint32_t *)ctx->asm_base.p;
-    p.clip_list = (uint32_t *)ctx->clip_list.p;
+    p.clip_list = (uint2 *)ctx->clip_list.p;
     p.clip_off = (uint2 *)ctx->clip_off.p;
     p.tvis = (TriVis *)ctx->tvis.p;
     p.tshade = (TriShade *)ctx->tshade.p;

@@ -77,6 +77,31 @@ def test_clipping_near_plane_and_screen_edges(device, oracle):
     assert_frames_match(g, o, color_tol=1)
 
 
+def test_clipping_against_several_planes(device, oracle):
+    """Large clip-space triangles, many crossing two or more planes and some with vertices
+    behind the eye (w < 0): the clipped polygons' fan triangles and interpolated colours are
+    bit-identical to the oracle's serial clipper."""
+    W, H = 160, 96
+    rng = np.random.default_rng(17)
+    n = 3000
+    w = rng.uniform(0.1, 2.0, (n, 3)) * np.where(rng.random((n, 3)) < 0.15, -1.0, 1.0)
+    v = np.zeros((n, 3, 8), np.float32)
+    v[..., 0] = rng.uniform(-3.0, 3.0, (n, 3)) * np.abs(w)
+    v[..., 1] = rng.uniform(-3.0, 3.0, (n, 3)) * np.abs(w)
+    v[..., 2] = rng.uniform(-0.5, 1.5, (n, 3)) * np.abs(w)
+    v[..., 3] = w
+    v[..., 4:7] = rng.uniform(0.0, 1.0, (n, 1, 3))
+    v[..., 7] = 1.0
+    idx = np.arange(3 * n, dtype=np.uint32).reshape(n, 3)
+    flip = rng.random(n) < 0.5  # both windings, so both cull outcomes occur
+    idx[flip] = idx[flip][:, [0, 2, 1]]
+    d = DrawSpec(idx.reshape(-1), v.reshape(-1, 8), "passthrough", "passthrough")
+    for depth_func in (0, 1):
+        g, o = both(device, oracle, W, H, [d], depth=0.0 if depth_func == 0 else 1.0, depth_func=depth_func)
+        assert (g.depth != (0.0 if depth_func == 0 else 1.0)).mean() > 0.5
+        assert_frames_match(g, o, color_tol=0)
+
+
 def test_binning_stress_lite(device, oracle):
     W, H = 256, 192
     st = S.binning_stress(20000, 20, W, H, seed=7, big_every=1000)
