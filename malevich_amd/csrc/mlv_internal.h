@@ -256,6 +256,7 @@ struct ScanScratch {
     size_t hist_elems;
     uint32_t *epoch;     // host-side counter, advanced by every scan launch
     uint32_t *err;       // device counter of scans whose look-back gave up waiting
+    size_t wide_off;     // byte offset in `part` of the 64-bit scans' statuses (0: none run)
 };
 
 // Stable LSD sort of n (key, value) pairs by the low `bits` of the key (8 bits per pass).
@@ -287,6 +288,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
                         hipStream_t s, hipEvent_t *tile_ev = nullptr);
 // Bytes of scan scratch for scans over up to n_max elements; elements of the radix histograms.
 size_t scan_scratch_bytes(size_t n_max);
+size_t scan_wide_offset(size_t n_max);
 size_t radix_hist_elems(size_t n_max);
 
 }  // namespace mlv

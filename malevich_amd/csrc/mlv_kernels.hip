@@ -333,10 +333,9 @@ struct ScanPub<unsigned long long> {
 
 template <class T>
 __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, T *total,
-                                              uint8_t *scratch, uint32_t epoch, uint32_t nparts, uint32_t *err) {
+                                              uint8_t *st, uint32_t epoch, uint32_t nparts, uint32_t *err) {
     __shared__ T s_prefix;
     __shared__ T s_w[5];
-    uint8_t *st = scratch + kScanHeader;
     const uint32_t part = blockIdx.x;
     const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
     const uint32_t nd = (uint32_t)((n + kScanChunk - 1) / kScanChunk);  // partitions holding data
@@ -410,15 +409,24 @@ static hipError_t launch_scan(const T *in, T *out, size_t n_host, const uint32_t
                               T *total, hipStream_t s) {
     if (n_host == 0) return hipSuccess;
     const uint32_t nparts = (uint32_t)((n_host + kScanChunk - 1) / kScanChunk);
+    // 64-bit statuses keep their own region: a 32-bit scan reading one of their value words
+    // could take its upper half for a status of its own epoch
+    const size_t off = sizeof(T) == 4 ? kScanHeader : sc.wide_off;
+    if (off == 0) return hipErrorInvalidValue;
     const uint32_t epoch = (++*sc.epoch) & 0x3FFFFFFFu;
-    hipLaunchKernelGGL(k_scan<T>, dim3(nparts), dim3(256), 0, s, in, out, n_host, n_dev, total, (uint8_t *)sc.part,
-                       epoch, nparts, sc.err);
+    hipLaunchKernelGGL(k_scan<T>, dim3(nparts), dim3(256), 0, s, in, out, n_host, n_dev, total,
+                       (uint8_t *)sc.part + off, epoch, nparts, sc.err);
     return hipGetLastError();
 }
 
-// Bytes of scan scratch for scans over up to n_max elements (header + one status per partition).
+// Bytes of scan scratch for scans over up to n_max elements: the header, one 32-bit scan status
+// per partition, then one 64-bit scan status per partition (from scan_wide_offset).
 size_t scan_scratch_bytes(size_t n_max) {
-    return kScanHeader + ((n_max + kScanChunk - 1) / kScanChunk + 1) * ScanPub<unsigned long long>::kBytes;
+    return scan_wide_offset(n_max) + ((n_max + kScanChunk - 1) / kScanChunk + 1) * ScanPub<unsigned long long>::kBytes;
+}
+
+size_t scan_wide_offset(size_t n_max) {
+    return kScanHeader + ((n_max + kScanChunk - 1) / kScanChunk + 1) * ScanPub<uint32_t>::kBytes;
 }
 
 
@@ -764,10 +772,16 @@ __device__ __forceinline__ void radix_hist_block(const uint32_t *keys, uint32_t 
     h[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t base = b * kRadixItems;
+    // all of the thread's loads are issued before the first count, so their latencies overlap
+    uint32_t k[kRadixRounds];
+#pragma unroll
     for (uint32_t r = 0; r < kRadixRounds; ++r) {
         const uint32_t i = base + r * 256u + threadIdx.x;
-        if (i < n) atomicAdd(&h[(keys[i] >> shift) & 255u], 1u);
+        k[r] = i < n ? keys[i] : 0xFFFFFFFFu;
     }
+#pragma unroll
+    for (uint32_t r = 0; r < kRadixRounds; ++r)
+        if (base + r * 256u + threadIdx.x < n) atomicAdd(&h[(k[r] >> shift) & 255u], 1u);
     __syncthreads();
     hist[threadIdx.x * nb + b] = h[threadIdx.x];
 }
@@ -795,7 +809,16 @@ __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const u
         loff[tid] = ex;
         lbeg[tid] = ex;
     }
+    // the thread's items, loaded before the first round so the loads' latencies overlap
+    uint32_t keys[kRadixRounds], vals[kRadixRounds];
+#pragma unroll
+    for (uint32_t r = 0; r < kRadixRounds; ++r) {
+        const uint32_t i = base + r * 256u + tid;
+        keys[r] = i < n ? kin[i] : 0u;
+        vals[r] = i < n ? vin[i] : 0u;
+    }
     // rank every item among the block's items of its digit (stable) and place it in LDS
+#pragma unroll
     for (uint32_t r = 0; r < kRadixRounds; ++r) {
         wc[0][tid] = 0u;
         wc[1][tid] = 0u;
@@ -804,8 +827,8 @@ __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const u
         __syncthreads();
         const uint32_t i = base + r * 256u + tid;
         const bool valid = i < n;
-        const uint32_t key = valid ? kin[i] : 0u;
-        const uint32_t val = valid ? vin[i] : 0u;
+        const uint32_t key = keys[r];
+        const uint32_t val = vals[r];
         const uint32_t dg = (key >> shift) & 255u;
         uint64_t m = __ballot(valid);
 #pragma unroll

@@ -485,7 +485,7 @@ static int flush(mlv_context *ctx) {
     p.scan_status = (uint32_t *)ctx->scan_part.p;
     p.scan_status_words = (uint32_t)(scan_scratch_bytes(scan_n) / 4);
     ScanScratch sc{ctx->scan_part.p, ctx->scan_part.cap, (uint32_t *)ctx->radix_hist.p, ctx->radix_hist.cap / 4,
-                   &flush_epoch, &ctx->dstats->scan_errors};
+                   &flush_epoch, &ctx->dstats->scan_errors, scan_wide_offset(scan_n)};
 
     // keys are global tile indices < total_tiles: one 8-bit pass per byte they use
     uint32_t bits = 0;
