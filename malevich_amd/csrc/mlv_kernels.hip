@@ -74,6 +74,14 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T *s_wave, T &total) {
     return r;
 }
 
+// The dispatcher deals workgroups to the 8 XCDs in turn (workgroup b to XCD b % 8), and each
+// XCD has its own L2. Numbering a grid of n blocks so that XCD x runs the x-th contiguous share
+// keeps neighbouring blocks, which gather the same vertices, under one L2.
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
+    const uint32_t x = b & 7u, q = b >> 3, per = n >> 3, rem = n & 7u;
+    return x * per + min(x, rem) + q;
+}
+
 // Largest index lo in [0, n) with key(lo) <= x, for ascending keys with key(0) <= x.
 template <class F>
 __device__ __forceinline__ uint32_t upper_index(uint32_t n, uint32_t x, F key) {
@@ -481,7 +489,7 @@ __global__ __launch_bounds__(256) void kb_reset(BatchParams p) {
 
 __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
     __shared__ uint32_t s_w[4];
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
     const DrawRec &d = p.draws[di];
     const ViewRec &v = p.views[d.view];
@@ -567,6 +575,9 @@ __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t id
                                                const float rw[3], uint32_t r0, uint32_t r1, uint32_t r2,
                                                uint32_t prim, uint32_t sub) {
     if (idx >= p.tri_cap) return;
+    // every load before the first store, which the compiler cannot move loads across
+    const ViewRec &v = p.views[p.draws[draw].view];
+    const uint32_t tile_base = v.tile_base, tiles_x = (uint32_t)v.tiles_x;
     tv.id = idx;
     tv.draw = draw;
     tv.local = idx - p.asm_base[draw];
@@ -577,36 +588,39 @@ __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t id
     sh.prim = prim;
     sh.sub = sub;
     p.tshade[idx] = sh;
-    const ViewRec &v = p.views[p.draws[draw].view];
-    p.tbin[idx] = make_uint4(tv.rect0, tv.rect1, v.tile_base, (uint32_t)v.tiles_x);
+    p.tbin[idx] = make_uint4(tv.rect0, tv.rect1, tile_base, tiles_x);
 }
 
 __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     __shared__ uint32_t s_w[5];
-    const uint32_t b = blockIdx.x;
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t slot = b * 256u + threadIdx.x;
+    const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
+    const DrawRec &d = p.draws[di];
+    const uint32_t t = slot - d.tbase;
+    const unsigned long long bo = p.blockoff[b];
     const uint32_t packed = p.cnt[slot];
+    // the triangle is set up before its place is known, so its loads and arithmetic overlap the
+    // block scan
+    TriVis tv;
+    float rw[3];
+    uint32_t s0 = 0u, s1 = 0u, s2 = 0u;
+    if (packed == 1u) {
+        uint32_t vi[3];
+        load_tri_indices(d, t, vi);
+        s0 = d.vbase + vi[0];
+        s1 = d.vbase + vi[1];
+        s2 = d.vbase + vi[2];
+        setup_triangle(d, p.views[d.view], p.vpos[s0], p.vpos[s1], p.vpos[s2], tv, rw);
+    }
     uint32_t total;
     // within one block the packed counts (<= 14 triangles, <= 16 vertices each) cannot carry
     const uint32_t ex = block_exclusive_scan<256, uint32_t>(packed, s_w, total);
     if (packed == 0u) return;
-    const unsigned long long bo = p.blockoff[b];
     const uint32_t off_tri = (uint32_t)(bo & 0xFFFFFFFFull) + (ex & 0xFFFFu);
     const uint32_t off_vtx = (uint32_t)(bo >> 32) + (ex >> 16);
-    if (packed == 1u) {
-        const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
-        const DrawRec &d = p.draws[di];
-        const uint32_t t = slot - d.tbase;
-        uint32_t vi[3];
-        load_tri_indices(d, t, vi);
-        const uint32_t s0 = d.vbase + vi[0], s1 = d.vbase + vi[1], s2 = d.vbase + vi[2];
-        TriVis tv;
-        float rw[3];
-        setup_triangle(d, p.views[d.view], p.vpos[s0], p.vpos[s1], p.vpos[s2], tv, rw);
-        write_triangle(p, off_tri, di, tv, rw, s0, s1, s2, t, 0u);
-    } else {
-        p.clip_off[slot] = make_uint2(off_tri, off_vtx);
-    }
+    if (packed == 1u) write_triangle(p, off_tri, di, tv, rw, s0, s1, s2, t, 0u);
+    else p.clip_off[slot] = make_uint2(off_tri, off_vtx);
 }
 
 // The clipped triangles' output, one per 16-lane group: lane k writes polygon vertex k's
