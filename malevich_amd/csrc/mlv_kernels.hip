@@ -1050,28 +1050,50 @@ __device__ __forceinline__ void bin_draw_change(const BatchParams &p, int32_t dr
     st.depth_cull = (int32_t)uniform_u((uint32_t)p.draws[draw].depth_cull);
 }
 
+// What the rasteriser reads of one staged record: the edge functions, the depth plane and the
+// order key.
+struct CoverRec {
+    int32_t a16[3], b16[3], c[3];
+    float ooa, z0, dz1, dz2;
+    int32_t id;
+};
+
+template <class Lds>
+__device__ __forceinline__ CoverRec load_cover(const Lds &L, uint32_t k) {
+    CoverRec r;
+#pragma unroll
+    for (int e = 0; e < 3; ++e) {
+        r.a16[e] = L.a16[e][k];
+        r.b16[e] = L.b16[e][k];
+        r.c[e] = L.c[e][k];
+    }
+    r.ooa = L.ooa[k];
+    r.z0 = L.z0[k];
+    r.dz1 = L.dz1[k];
+    r.dz2 = L.dz2[k];
+    r.id = (int32_t)L.id[k];
+    return r;
+}
+
 // Corner-sampled coverage of one triangle over the bin's 64 pixels and the early-Z key update
 // (main.c:1015-1028, 1083-1168); the triangle has passed hierarchical-Z.
-template <bool GE, class Lds>
-__device__ __forceinline__ void bin_cover(const Lds &L, uint32_t k, BinState &st, bool inside, int32_t px,
-                                          int32_t py) {
-    const uint32_t e0 = (uint32_t)L.c[0][k] + (uint32_t)__mul24(L.a16[0][k], px) + (uint32_t)__mul24(L.b16[0][k], py);
-    const uint32_t e1 = (uint32_t)L.c[1][k] + (uint32_t)__mul24(L.a16[1][k], px) + (uint32_t)__mul24(L.b16[1][k], py);
-    const uint32_t e2 = (uint32_t)L.c[2][k] + (uint32_t)__mul24(L.a16[2][k], px) + (uint32_t)__mul24(L.b16[2][k], py);
+template <bool GE>
+__device__ __forceinline__ void bin_cover(const CoverRec &r, BinState &st, bool inside, int32_t px, int32_t py) {
+    const uint32_t e0 = (uint32_t)r.c[0] + (uint32_t)__mul24(r.a16[0], px) + (uint32_t)__mul24(r.b16[0], py);
+    const uint32_t e1 = (uint32_t)r.c[1] + (uint32_t)__mul24(r.a16[1], px) + (uint32_t)__mul24(r.b16[1], py);
+    const uint32_t e2 = (uint32_t)r.c[2] + (uint32_t)__mul24(r.a16[2], px) + (uint32_t)__mul24(r.b16[2], py);
     const bool cov = inside && (int32_t)(e0 | e1 | e2) > 0;  // main.c:1028
     if (__ballot(cov) == 0ull) return;
-    const float ooa = L.ooa[k];
-    const float b1 = (float)((int32_t)e1 >> 8) * ooa;  // main.c:1089-1102
-    const float b2 = (float)((int32_t)e2 >> 8) * ooa;
-    float z = L.z0[k] + L.dz1[k] * b1;  // register 0 is affine (main.c:1118-1154)
-    z = z + L.dz2[k] * b2;
-    const int32_t id = (int32_t)L.id[k];
-    const bool pass = GE ? (z > st.d || (z == st.d && id > st.best)) : (z < st.d || (z == st.d && id > st.best));
-    const bool upd = cov && pass;
-    if (upd) {
-        st.d = z;
-        st.best = id;
-    }
+    const float b1 = (float)((int32_t)e1 >> 8) * r.ooa;  // main.c:1089-1102
+    const float b2 = (float)((int32_t)e2 >> 8) * r.ooa;
+    float z = r.z0 + r.dz1 * b1;  // register 0 is affine (main.c:1118-1154)
+    z = z + r.dz2 * b2;
+    // the greater (depth, order) key wins (GE; the lesser depth under LESS_EQUAL), without branches
+    const bool nearer = GE ? z > st.d : z < st.d;
+    const bool tie = z == st.d;
+    const bool upd = cov & (nearer | (tie & (r.id > st.best)));
+    st.d = upd ? z : st.d;
+    st.best = upd ? r.id : st.best;
     if (__ballot(upd)) st.stale = 1u;
 }
 
@@ -1122,11 +1144,24 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, con
                    edge_max(L.a16[1][k], L.b16[1][k], L.c[1][k], x0, y0) >= 0 &&
                    edge_max(L.a16[2][k], L.b16[2][k], L.c[2][k], x0, y0) >= 0;
         }
+        // survivors in order; lane jj already holds entry i0 + jj's record index, and the next
+        // survivor's record is read from LDS while the current one is rasterised
         uint64_t S = __ballot(live);
-        while (S) {
-            const uint32_t jj = (uint32_t)__builtin_ctzll(S);
+        if (S) {
+            uint32_t jj = (uint32_t)__builtin_ctzll(S);
             S &= S - 1ull;
-            bin_cover<GE>(L, uniform_u(list[first + (i0 + jj) * stride]), st, inside, px, py);
+            CoverRec r = load_cover(L, (uint32_t)__builtin_amdgcn_readlane((int)k, (int)jj));
+            for (;;) {
+                const CoverRec cur = r;
+                const bool more = S != 0ull;
+                if (more) {
+                    jj = (uint32_t)__builtin_ctzll(S);
+                    S &= S - 1ull;
+                    r = load_cover(L, (uint32_t)__builtin_amdgcn_readlane((int)k, (int)jj));
+                }
+                bin_cover<GE>(cur, st, inside, px, py);
+                if (!more) break;
+            }
         }
         i0 += run;
     }
