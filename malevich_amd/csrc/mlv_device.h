@@ -60,9 +60,19 @@ __device__ __forceinline__ float lerp_ref(float a, float b, float f) {  // math.
     return p + q;
 }
 
+// The shaders' power and exponential on the hardware's base-2 logarithm and exponential
+// (v_log_f32, v_exp_f32, 1 ulp each): x^y = 2^(y log2 x) and e^x = 2^(x log2 e). The reference
+// calls SVML's pow and exp (the oracle, libm's); for the tone and sRGB curves (|y log2 x| of a few
+// units) the results differ from a correctly rounded pow by a few parts in 10^7, and the 8-bit
+// encodes by at most one step. Zero, infinities, negative bases and NaN map as pow maps them.
+__device__ __forceinline__ float shader_pow(float x, float y) {
+    return __builtin_amdgcn_exp2f(y * __builtin_amdgcn_logf(x));
+}
+__device__ __forceinline__ float shader_exp(float x) { return __builtin_amdgcn_exp2f(x * 1.44269504088896341f); }
+
 // f256_srgb_from_linear_approx, math.h:419-422
 __device__ __forceinline__ float srgb_approx(float c) {
-    float p = powf(c, 0.416666667f);
+    float p = shader_pow(c, 0.416666667f);
     float s = 1.055f * p;
     s = s + -0.055f;
     return (s > 0.0f) ? s : 0.0f;
@@ -70,8 +80,8 @@ __device__ __forceinline__ float srgb_approx(float c) {
 
 // env_lighting_ps.c:20-21, vertex_lighting_vs.c:34-35
 __device__ __forceinline__ float tonemap(float c) {
-    float e = expf(c * -1.0f);
-    return powf(1.0f - e, (float)(1.0 / 2.2));
+    float e = shader_exp(c * -1.0f);
+    return shader_pow(1.0f - e, (float)(1.0 / 2.2));
 }
 
 // Output merger encode, main.c:1176-1178
