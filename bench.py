@@ -588,17 +588,17 @@ def main(argv=None):
     per_launch_ms = (prof["tile_kernel"] if dom == "tile" else stage_ms[dom]) / launches
     flushes_per_frame = launches / args.steps
     # Algorithmic (compulsory) bytes per launch, texels excluded:
-    #  tile:    TriVis 96 B + TriShade 32 B + three 32 B attribute rows per assembled triangle,
+    #  tile:    TriVis 80 B + TriShade 32 B + three 32 B attribute rows per assembled triangle,
     #           plus one depth read and one colour + one depth write per frame pixel (12 B)
-    #  setup:   12 B of indices and three 16 B positions per input triangle, 144 B written per
+    #  setup:   12 B of indices and three 16 B positions per input triangle, 128 B written per
     #           assembled triangle (TriVis, TriShade, bin record)
     #  vertex:  32 B read and 48 B written per vertex
     #  binning: a 4 B key + 4 B value written and re-read per (triangle, tile) pair per radix pass
     asm_frame = stats["assembled_triangle_count"] / args.steps
     verts_frame = sum(m.vertices.shape[0] for m in sc.instances)
     algo_frame = {
-        "tile": asm_frame * (96 + 32 + 3 * 32) + 12.0 * P_px,
-        "setup": tris * (12 + 3 * 16) + asm_frame * (96 + 32 + 16),
+        "tile": asm_frame * (80 + 32 + 3 * 32) + 12.0 * P_px,
+        "setup": tris * (12 + 3 * 16) + asm_frame * (80 + 32 + 16),
         "vertex": verts_frame * (32 + 48),
         "binning": asm_frame * 16 + (stats["total_triangle_count_in_bins"] / args.steps) * 8 * 2,
     }

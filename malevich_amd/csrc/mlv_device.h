@@ -326,7 +326,6 @@ __device__ __forceinline__ bool setup_triangle(const DrawRec &p, const ViewRec &
     const int32_t ly1 = imin(imax(y[0], imax(y[1], y[2])) >> 4, v.H - 1);
     tv.lat_x = (uint32_t)(uint16_t)(int16_t)imin(lx0, 32767) | ((uint32_t)(uint16_t)(int16_t)imax(lx1, -1) << 16);
     tv.lat_y = (uint32_t)(uint16_t)(int16_t)imin(ly0, 32767) | ((uint32_t)(uint16_t)(int16_t)imax(ly1, -1) << 16);
-    tv.pad[0] = tv.pad[1] = 0u;
     return true;
 }
 

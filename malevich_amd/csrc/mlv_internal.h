@@ -9,7 +9,7 @@
 //   vpos[Vt]         float4    clip-space position per vertex slot (VS register 0)
 //   vattr[2*(Vt+C)]  float4    VS registers 1..2 per vertex slot; rows >= Vt hold clipper vertices
 //   cnt[Tt]          u32       per input-triangle slot: (#assembled) | (#clip vertices << 16)
-//   tvis[N]          TriVis    96 B per assembled triangle (edge functions, depth plane, bin rect)
+//   tvis[N]          TriVis    80 B per assembled triangle (edge functions, depth plane, bin rect)
 //   tshade[N]        TriShade  32 B per assembled triangle (1/w, attribute rows)
 //   tbin[N]          uint4     16 B per assembled triangle (bin rect and its view's tile grid)
 //   pair keys/vals   u32       (global tile, triangle) pairs, stably sorted by tile
@@ -83,7 +83,9 @@ struct ViewRec {
     int32_t depth_func;
 };
 
-// One assembled triangle as the tile kernel reads it (96 B).
+// One assembled triangle as the tile kernel reads it (80 B). Its index in the assembled array
+// is its order key (draw order, then submission order); the reference's per-draw number is that
+// index minus the draw's first index (BatchParams::asm_base).
 // a16/b16 are the edge coefficients pre-multiplied by 16 (mod 2^32), so that
 // E = a16*x + b16*y + c equals ((a*x)<<4) + ((b*y)<<4) + c of main.c:1015-1025.
 struct alignas(16) TriVis {
@@ -93,16 +95,13 @@ struct alignas(16) TriVis {
     float hiz_depth;         // max_depth (main.c:871), or the minimum under LESS_EQUAL
     float cull_bound;        // a bound no interpolated depth of this triangle can pass
     uint32_t rect0, rect1;   // 8x8 bin rect: bx0 | by0<<16, bx1 | by1<<16 (main.c:930-931)
-    uint32_t id;             // global assembled index (draw order, then submission order)
     uint32_t draw;           // index of the draw in the flush
-    uint32_t local;          // assembled index within its draw (what the reference numbers)
     // The pixel corners inside the snapped bounding box, clamped to the view: x in
     // [lat_x & 0xFFFF, lat_x >> 16], y likewise (as int16). Coverage is sampled only at pixel
     // corners (main.c:1015-1025), so a triangle with no corner in a bin covers nothing there.
     uint32_t lat_x, lat_y;
-    uint32_t pad[2];
 };
-static_assert(sizeof(TriVis) == 96, "TriVis layout");
+static_assert(sizeof(TriVis) == 80, "TriVis layout");
 
 struct alignas(16) TriShade {
     float rw[3];             // a_reciprocal_ws (main.c:868-870)

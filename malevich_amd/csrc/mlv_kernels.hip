@@ -265,14 +265,18 @@ __global__ __launch_bounds__(256) void k_srgb_to_linear_rgba8(uint32_t *texels, 
     }
 }
 
-__global__ void k_fill_u32(uint32_t *d, uint32_t v, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        d[i] = v;
-}
-
-__global__ void k_fill_f32(float *d, float v, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
-        d[i] = v;
+// Fill of n 32-bit words: words up to the first 16-byte boundary, then 16-byte stores, then the
+// remaining words.
+__global__ __launch_bounds__(256) void k_fill32(uint32_t *d, uint32_t v, size_t n) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    const size_t head = min(n, (size_t)(((16u - ((uintptr_t)d & 15u)) & 15u) / 4u));
+    if (tid < head) d[tid] = v;
+    uint4 *body = (uint4 *)(d + head);
+    const size_t nb = (n - head) / 4u;
+    const uint4 v4 = make_uint4(v, v, v, v);
+    for (size_t i = tid; i < nb; i += stride) body[i] = v4;
+    const size_t tail = head + nb * 4u;
+    if (tid < n - tail) d[tail + tid] = v;
 }
 
 // ------------------------------------------------------------ scans
@@ -578,9 +582,7 @@ __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t id
     // every load before the first store, which the compiler cannot move loads across
     const ViewRec &v = p.views[p.draws[draw].view];
     const uint32_t tile_base = v.tile_base, tiles_x = (uint32_t)v.tiles_x;
-    tv.id = idx;
     tv.draw = draw;
-    tv.local = idx - p.asm_base[draw];
     p.tvis[idx] = tv;
     TriShade sh;
     sh.rw[0] = rw[0]; sh.rw[1] = rw[1]; sh.rw[2] = rw[2];
@@ -1009,7 +1011,7 @@ struct BinState {
 
 // Up to N staged records of a tile's list in LDS, one array per field: the triangle-parallel
 // screening reads a different record per lane, and with one array per field consecutive records
-// sit in consecutive banks (a 96-byte record stride would put 64 lanes on a handful of banks).
+// sit in consecutive banks (an 80-byte record stride would put 64 lanes on a handful of banks).
 // The rasteriser reads one record at a time as broadcasts.
 template <uint32_t N>
 struct TileLdsN {
@@ -1021,7 +1023,7 @@ struct TileLdsN {
 using TileLds = TileLdsN<256>;
 
 template <class Lds>
-__device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t) {
+__device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t, uint32_t id) {
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         L.a16[e][k] = t.a16[e];
@@ -1036,7 +1038,7 @@ __device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t
     L.cull[k] = t.cull_bound;
     L.lat_x[k] = t.lat_x;
     L.lat_y[k] = t.lat_y;
-    L.id[k] = t.id;
+    L.id[k] = id;
     L.draw[k] = t.draw;
 }
 
@@ -1218,7 +1220,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
         if (tid < n) {
             const uint32_t t = ovf ? base + tid : sorted_vals[lstart + base + tid];
             const TriVis tv = p.tvis[t];
-            stage_record(s_tri, tid, tv);
+            stage_record(s_tri, tid, tv, t);
             int32_t bx0, by0, bx1, by1;
             rect_of(tv, bx0, by0, bx1, by1);
             uint32_t m = 0u;
@@ -1275,7 +1277,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
         V.color[pix] = encode_output(cr, cg, cb);
         V.depth[pix] = st.d;
         if (V.vis_tri) {
-            V.vis_tri[pix] = T.local;
+            V.vis_tri[pix] = (uint32_t)st.best - p.asm_base[T.draw];  // numbered within the draw
             V.vis_draw[pix] = D.draw_tag;
         }
         if (V.vis_prim) V.vis_prim[pix] = S.prim;
@@ -1340,8 +1342,9 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
     for (uint32_t base = 0; base < lcount; base += 256u) {
         const uint32_t n = min(256u, lcount - base);
         if (tid < n) {
-            const TriVis tv = p.tvis[sorted_vals[lstart + base + tid]];
-            stage_record(s_tri, tid, tv);
+            const uint32_t t = sorted_vals[lstart + base + tid];
+            const TriVis tv = p.tvis[t];
+            stage_record(s_tri, tid, tv, t);
             int32_t bx0, by0, bx1, by1;
             rect_of(tv, bx0, by0, bx1, by1);
             uint32_t m = 0u;
@@ -1416,7 +1419,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
         V.color[pix] = encode_output(cr, cg, cb);
         V.depth[pix] = st.d;
         if (V.vis_tri) {
-            V.vis_tri[pix] = T.local;
+            V.vis_tri[pix] = (uint32_t)st.best - p.asm_base[T.draw];  // numbered within the draw
             V.vis_draw[pix] = D.draw_tag;
         }
         if (V.vis_prim) V.vis_prim[pix] = S.prim;
@@ -1558,16 +1561,13 @@ hipError_t launch_srgb_to_linear_rgba8(uint32_t *texels, size_t n, hipStream_t s
 
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s) {
     if (!n) return hipSuccess;
-    unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 65535u * 8u);
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid), dim3(256), 0, s, dst, value, n);
+    const unsigned grid = (unsigned)std::min<size_t>((n / 4u + 255u) / 256u + 1u, 2048u);
+    hipLaunchKernelGGL(k_fill32, dim3(grid), dim3(256), 0, s, dst, value, n);
     return hipGetLastError();
 }
 
 hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s) {
-    if (!n) return hipSuccess;
-    unsigned grid = (unsigned)std::min<size_t>((n + 255) / 256, 65535u * 8u);
-    hipLaunchKernelGGL(k_fill_f32, dim3(grid), dim3(256), 0, s, dst, value, n);
-    return hipGetLastError();
+    return launch_fill_u32((uint32_t *)dst, __builtin_bit_cast(uint32_t, value), n, s);
 }
 
 hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, uint32_t radix_passes,
