@@ -1,11 +1,12 @@
 // mlv_kernels.hip -- the draw pipeline as gfx950 kernels, one pass per stage per flush.
 //
-// Stage 0, vertex:   kb_vertex           one lane per vertex slot of every draw (main.c:662-734)
+// Stage 0, vertex:   kb_vertex           one lane per vertex slot of every draw (main.c:662-734),
+//                                        and the flush's counters cleared
 // Stage 1, setup:    kb_setup_count      one lane per input triangle: cull/classify, output count
 //                    kb_clip_count       triangles that need the clipper (main.c:609-660)
 //                    scan (u64)          ordered compaction offsets over all setup blocks
-//                    kb_setup_totals     totals, capacity checks, first assembled index per draw
-//                    kb_setup_emit       TriVis/TriShade at the ordered slot (main.c:751-908)
+//                    kb_setup_emit       TriVis/TriShade at the ordered slot (main.c:751-908); totals,
+//                                        capacity checks, first assembled index per draw
 //                    kb_clip_emit        clipped fan triangles + clipper vertices
 // Stage 2, binning:  kb_pairs<false>     (triangle, 16x16 tile) pair count per triangle
 //                    scan                pair offsets (triangle order)
@@ -462,8 +463,26 @@ __device__ __forceinline__ void run_vs(const BatchParams &p, const DrawRec &d, u
     p.vattr[2u * slot + 1u] = r2;
 }
 
+// A flush's counters, tile ranges and scan status words start at zero.
+__device__ __forceinline__ void reset_flush(const BatchParams &p) {
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        BatchCounters z = {};
+        *p.ctr = z;
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.total_tiles; i += gridDim.x * blockDim.x) {
+        p.tile_start[i] = 0u;
+        p.tile_cnt[i] = 0u;
+    }
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.scan_status_words; i += gridDim.x * blockDim.x)
+        p.scan_status[i] = 0u;
+}
+
+// Stage 0 in one launch: every block takes a share of the flush's reset, and blocks below
+// n_vblocks one lane per vertex slot of every draw.
 __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
+    reset_flush(p);
     const uint32_t b = blockIdx.x;
+    if (b >= p.n_vblocks) return;
     const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_v0; });
     const DrawRec &d = p.draws[di];
     const uint32_t i = (b - d.blk_v0) * 256u + threadIdx.x;
@@ -478,18 +497,6 @@ __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
 
 // ------------------------------------------------------------ stage 1: setup
 
-__global__ __launch_bounds__(256) void kb_reset(BatchParams p) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        BatchCounters z = {};
-        *p.ctr = z;
-    }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.total_tiles; i += gridDim.x * blockDim.x) {
-        p.tile_start[i] = 0u;
-        p.tile_cnt[i] = 0u;
-    }
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < p.scan_status_words; i += gridDim.x * blockDim.x)
-        p.scan_status[i] = 0u;
-}
 
 __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
     __shared__ uint32_t s_w[4];
@@ -555,25 +562,6 @@ __global__ __launch_bounds__(256) void kb_clip_count(BatchParams p) {
     }
 }
 
-__global__ void kb_setup_totals(BatchParams p) {
-    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
-    const unsigned long long tot = p.ctr->n_assembled_clip;
-    const uint32_t n_asm = (uint32_t)(tot & 0xFFFFFFFFull), n_cv = (uint32_t)(tot >> 32);
-    if (i < p.n_draws) p.asm_base[i] = (uint32_t)(p.blockoff[p.draws[i].blk_t0] & 0xFFFFFFFFull);
-    if (i == 0) {
-        p.ctr->n_assembled = n_asm;
-        p.ctr->n_clipverts = n_cv;
-        uint32_t ovf = 0u;
-        if (n_asm > p.tri_cap) ovf |= OVF_TRIANGLES;
-        if ((unsigned long long)p.vslots + n_cv > (unsigned long long)p.vattr_rows) ovf |= OVF_CLIPVERTS;
-        if (ovf) {
-            atomicOr(&p.ctr->overflow, ovf);
-            atomicAdd(&p.stats->capacity_errors, 1ull);
-        }
-        atomicAdd(&p.stats->assembled, (unsigned long long)n_asm);
-        atomicMax(&p.stats->max_clip_list, (unsigned long long)p.ctr->n_clip_list);
-    }
-}
 
 __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t idx, uint32_t draw, TriVis &tv,
                                                const float rw[3], uint32_t r0, uint32_t r1, uint32_t r2,
@@ -602,6 +590,28 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     const uint32_t t = slot - d.tbase;
     const unsigned long long bo = p.blockoff[b];
     const uint32_t packed = p.cnt[slot];
+    if (threadIdx.x == 0u) {
+        // a draw's first assembled index; empty draws start where the next one does
+        if (b == d.blk_t0)
+            for (int32_t k = (int32_t)di; k >= 0 && p.draws[k].blk_t0 == b; --k) p.asm_base[k] = (uint32_t)bo;
+        if (b == gridDim.x - 1u) {
+            // the flush's totals and capacity checks, and the empty draws after the last triangle
+            const unsigned long long tot = p.ctr->n_assembled_clip;
+            const uint32_t n_asm = (uint32_t)(tot & 0xFFFFFFFFull), n_cv = (uint32_t)(tot >> 32);
+            for (uint32_t k = di + 1u; k < p.n_draws; ++k) p.asm_base[k] = n_asm;
+            p.ctr->n_assembled = n_asm;
+            p.ctr->n_clipverts = n_cv;
+            uint32_t ovf = 0u;
+            if (n_asm > p.tri_cap) ovf |= OVF_TRIANGLES;
+            if ((unsigned long long)p.vslots + n_cv > (unsigned long long)p.vattr_rows) ovf |= OVF_CLIPVERTS;
+            if (ovf) {
+                atomicOr(&p.ctr->overflow, ovf);
+                atomicAdd(&p.stats->capacity_errors, 1ull);
+            }
+            atomicAdd(&p.stats->assembled, (unsigned long long)n_asm);
+            atomicMax(&p.stats->max_clip_list, (unsigned long long)p.ctr->n_clip_list);
+        }
+    }
     // the triangle is set up before its place is known, so its loads and arithmetic overlap the
     // block scan
     TriVis tv;
@@ -686,11 +696,40 @@ constexpr uint32_t kRadixItems = 256 * kRadixRounds;
 // EMIT = false: per-triangle pair counts. EMIT = true: the pairs in triangle order at the
 // scanned offsets. Triangles spanning more than kBigRect tiles are
 // expanded by the whole block.
+// The pair count's totals, by block 0 of the emitting pass: the (triangle, 8x8 bin) count for the
+// stats from the counting blocks' partials, the capacity check and the radix passes' sizes.
+__device__ __forceinline__ void pair_totals(const BatchParams &p, uint32_t total, bool over) {
+    __shared__ unsigned long long s_w[4];
+    unsigned long long acc = 0ull;
+    for (uint32_t i = threadIdx.x; i < gridDim.x; i += 256u) acc += p.blocksum[i];
+    acc = wave_sum(acc);
+    if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = acc;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        p.stats->pairs8 += s_w[0] + s_w[1] + s_w[2] + s_w[3];
+        atomicMax(&p.stats->max_pairs, (unsigned long long)total);
+        uint32_t nb = (total + kRadixItems - 1u) / kRadixItems;
+        if (over) {
+            atomicOr(&p.ctr->overflow, OVF_PAIRS);
+            atomicAdd(&p.stats->overflow_draws, 1ull);
+            nb = 0u;
+        }
+        p.ctr->radix_blocks = nb;
+        p.ctr->hist_n = nb * 256u;
+    }
+}
+
+// Both passes run the same grid: the counting pass leaves one partial per block in blocksum.
 template <bool EMIT>
 __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
     __shared__ uint32_t s_big[256];
     __shared__ uint32_t s_nbig;
-    if (EMIT && (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES))) return;
+    if (EMIT) {
+        const uint32_t total = p.ctr->n_pairs;
+        const bool over = total > p.pair_cap;  // past the arena: the tile stage walks every triangle
+        if (blockIdx.x == 0) pair_totals(p, total, over);
+        if (over || (p.ctr->overflow & OVF_TRIANGLES)) return;
+    }
     const uint32_t n = min(p.ctr->n_assembled, p.tri_cap);
     unsigned long long pairs8_acc = 0ull;
     for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
@@ -749,27 +788,6 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
     }
 }
 
-__global__ __launch_bounds__(256) void kb_pair_check(BatchParams p, uint32_t n_partials) {
-    __shared__ unsigned long long s_w[4];
-    unsigned long long acc = 0ull;
-    for (uint32_t i = threadIdx.x; i < n_partials; i += 256u) acc += p.blocksum[i];
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        p.stats->pairs8 += s_w[0] + s_w[1] + s_w[2] + s_w[3];
-        const uint32_t total = p.ctr->n_pairs;
-        atomicMax(&p.stats->max_pairs, (unsigned long long)total);
-        uint32_t nb = (total + kRadixItems - 1u) / kRadixItems;
-        if (total > p.pair_cap) {
-            atomicOr(&p.ctr->overflow, OVF_PAIRS);
-            atomicAdd(&p.stats->overflow_draws, 1ull);
-            nb = 0u;
-        }
-        p.ctr->radix_blocks = nb;
-        p.ctr->hist_n = nb * 256u;
-    }
-}
 
 // Stable LSD radix sort of (tile, triangle) pairs by tile, 8 bits per pass. A block owns
 // kRadixItems consecutive pairs; within a round of 256, a pair's rank among equal digits
@@ -1575,8 +1593,8 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
     hipError_t e;
     switch (stage) {
     case 0:
-        hipLaunchKernelGGL(kb_reset, dim3((p.total_tiles + 255) / 256 + 1), dim3(256), 0, s, p);
-        if (p.n_vblocks) hipLaunchKernelGGL(kb_vertex, dim3(p.n_vblocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(kb_vertex, dim3(std::max(p.n_vblocks, std::min((p.total_tiles + 255u) / 256u + 1u, 1024u))),
+                           dim3(256), 0, s, p);
         break;
     case 1:
         if (!p.n_tblocks) break;
@@ -1585,7 +1603,6 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc,
                                                  &p.ctr->n_assembled_clip, s)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(kb_setup_totals, dim3((p.n_draws + 255) / 256), dim3(256), 0, s, p);
         hipLaunchKernelGGL(kb_setup_emit, dim3(p.n_tblocks), dim3(256), 0, s, p);
         hipLaunchKernelGGL(kb_clip_emit, dim3(1024), dim3(256), 0, s, p);
         break;
@@ -1596,7 +1613,6 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         if ((e = launch_scan<uint32_t>(p.pair_cnt, p.pair_off, p.tri_cap, &p.ctr->n_assembled, sc,
                                        &p.ctr->n_pairs, s)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(kb_pair_check, dim3(1), dim3(256), 0, s, p, grid);
         hipLaunchKernelGGL(kb_pairs<true>, dim3(grid), dim3(256), 0, s, p);
         const uint32_t nb = (p.pair_cap + kRadixItems - 1) / kRadixItems;  // chunks the arena can hold
         const uint32_t rgrid = std::max(1u, std::min(nb, kRadixGrid));
