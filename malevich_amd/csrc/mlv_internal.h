@@ -111,6 +111,8 @@ struct alignas(16) TriShade {
 };
 static_assert(sizeof(TriShade) == 32, "TriShade layout");
 
+constexpr size_t kMaxBatchDraws = 1u << 16;  // draws per flush; the tile kernels keep draw indices in 16 bits
+
 struct BatchCounters {
     unsigned long long n_assembled_clip;   // packed scan total: assembled | clip vertices << 32
     uint32_t n_assembled;

@@ -1036,7 +1036,9 @@ struct TileLdsN {
     static constexpr uint32_t kN = N;
     int32_t a16[3][N], b16[3][N], c[3][N];
     float ooa[N], z0[N], dz1[N], dz2[N], hiz[N], cull[N];
-    uint32_t lat_x[N], lat_y[N], id[N], draw[N];
+    uint32_t lat_x[N], lat_y[N], id[N];
+    uint16_t draw[N];
+    static_assert(kMaxBatchDraws <= 65536, "draw indices of a flush fit 16 bits");
 };
 using TileLds = TileLdsN<256>;
 
@@ -1131,7 +1133,7 @@ __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c,
 // The survivors are then rasterised pixel-parallel, in order. Every test only removes triangles
 // that could not have written a pixel, so the result equals the one-by-one walk.
 template <bool GE, class Lds>
-__device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, const uint16_t *list, uint32_t first,
+__device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, const uint8_t *list, uint32_t first,
                                          uint32_t stride, uint32_t n, BinState &st, bool inside, int32_t px,
                                          int32_t py, int32_t x0, int32_t y0, uint32_t lane) {
     for (uint32_t i0 = 0; i0 < n;) {
@@ -1199,7 +1201,7 @@ __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *s
 
 template <bool GE, uint32_t N>
 __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t vi,
-                                          TileLdsN<N> &s_tri, uint32_t *s_mask, uint16_t (*s_list)[N],
+                                          TileLdsN<N> &s_tri, uint8_t *s_mask, uint8_t (*s_list)[N],
                                           uint32_t *s_active) {
     // the wave index is uniform per wave; saying so keeps everything derived from it (the bin,
     // its Hi-Z value, the walk's branches) in scalar registers
@@ -1258,7 +1260,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
             const uint32_t i = (uint32_t)c * 64u + lane;
             const bool b = i < n && ((s_mask[i] >> w) & 1u);
             const uint64_t bal = __ballot(b);
-            if (b) s_list[w][cnt + mbcnt(bal)] = (uint16_t)i;
+            if (b) s_list[w][cnt + mbcnt(bal)] = (uint8_t)i;
             cnt += (uint32_t)__popcll(bal);
         }
         if (sub_valid && cnt) walk_bin<GE>(p, s_tri, s_list[w], 0u, 1u, cnt, st, inside, px, py, sbx * kBin, sby * kBin, lane);
@@ -1332,7 +1334,7 @@ constexpr uint32_t kWavesPerBin = kHeavyWaves / 4;
 
 template <bool GE>
 __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g,
-                                           uint32_t vi, TileLds &s_tri, uint32_t *s_mask, uint16_t (*s_list)[256],
+                                           uint32_t vi, TileLds &s_tri, uint8_t *s_mask, uint8_t (*s_list)[256],
                                            uint32_t *s_cnt, float (*s_md)[64], int32_t (*s_mb)[64]) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
     const uint32_t bin = w / kWavesPerBin, q = w % kWavesPerBin;
@@ -1381,7 +1383,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
                 const uint32_t i = (uint32_t)c * 64u + lane;
                 const bool b = i < n && ((s_mask[i] >> w) & 1u);
                 const uint64_t bal = __ballot(b);
-                if (b) s_list[w][cnt + mbcnt(bal)] = (uint16_t)i;
+                if (b) s_list[w][cnt + mbcnt(bal)] = (uint8_t)i;
                 cnt += (uint32_t)__popcll(bal);
             }
             if (lane == 0u) s_cnt[w] = cnt;
@@ -1453,8 +1455,8 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
 
 __global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint32_t *sorted_vals) {
     __shared__ TileLds s_tri;
-    __shared__ uint32_t s_mask[256];
-    __shared__ __attribute__((aligned(8))) uint16_t s_list[4][256];
+    __shared__ uint8_t s_mask[256];
+    __shared__ __attribute__((aligned(8))) uint8_t s_list[4][256];
     __shared__ uint32_t s_cnt[4];
     __shared__ float s_md[kHeavyWaves][64];
     __shared__ int32_t s_mb[kHeavyWaves][64];
@@ -1529,8 +1531,8 @@ __global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
 template <uint32_t N>
 __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
     __shared__ TileLdsN<N> s_tri;
-    __shared__ uint32_t s_mask[N];
-    __shared__ __attribute__((aligned(8))) uint16_t s_list[4][N];
+    __shared__ uint8_t s_mask[N];
+    __shared__ __attribute__((aligned(8))) uint8_t s_list[4][N];
     __shared__ uint32_t s_active[4];
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host

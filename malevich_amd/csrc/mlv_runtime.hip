@@ -97,7 +97,6 @@ struct Cmd {
 // One flush never exceeds these (indices are 32-bit; the arenas stay bounded).
 constexpr uint64_t kMaxBatchTriangles = 48ull << 20;
 constexpr uint64_t kMaxBatchVertices = 48ull << 20;
-constexpr size_t kMaxBatchDraws = 1u << 16;
 constexpr int kStagingSlots = 16;
 
 }  // namespace
