@@ -246,9 +246,9 @@ def bench_c4(dev, args):
     idx = cuda(m.indices.astype(np.int32))
     M = torch.from_numpy(sc.clip_from_world.copy())
     rng = np.random.default_rng(args.seed + 4)
-    dev.set_tuning(**DEFAULT_TUNING)  # not the previous leg's tuning
     out = {}
     for det in (False, True):
+        dev.set_tuning(**DEFAULT_TUNING)  # not the previous leg's tuning; the raster is autotuned below
         r = D.DiffRenderer(dev, sc.width, sc.height, deterministic=det)
         with torch.no_grad():
             tgt = D.render(r, base["positions"] + cuda(rng.uniform(-1e-3, 1e-3, m.positions.shape).astype(np.float32)),
@@ -256,6 +256,8 @@ def bench_c4(dev, args):
                            base["texture"] + cuda(rng.uniform(-0.1, 0.1, sc.texture.shape).astype(np.float32)),
                            base["sh"], M)
         params = {k: v.clone().requires_grad_(True) for k, v in base.items()}
+        verts, cb = D.raster_inputs(base["positions"], base["normals"], base["uvs"], M)
+        out["tuning"] = r.autotune(verts, idx, cb)
 
         def step():
             for v in params.values():
@@ -282,7 +284,7 @@ def bench_c4(dev, args):
     return {"workload": f"c4-512x512-differentiable: {m.triangle_count} triangles, 1024x1024 RGBA32F texture with "
                         f"{sc.levels} mips, SH9 light, L2 loss, gradients to positions/colours/uvs/normals/texels/SH",
             "fwd_bwd_ms": out["atomic"], "fwd_bwd_ms_deterministic": out["deterministic"],
-            "covered_pixels": covered, "steps": args.c4_steps}
+            "covered_pixels": covered, "steps": args.c4_steps, "tuning": out["tuning"]}
 
 
 def bench_c5(dev, args):
@@ -373,12 +375,13 @@ def bench_c5_diff(dev, sc, args, ev):
     shading and one adjoint launch per (view, mesh) pair."""
     import torch
     from malevich_amd import batched as B
-    from malevich_amd import diff as D
+    from malevich_amd import scenes as S
     cuda = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
     meshes = []
     for k in range(len(sc.meshes)):
         dm = sc.diff_mesh(k)
-        chain, levels = D.mip_chain(cuda(sc.textures[k]))
+        chain, levels = S.mip_chain(sc.textures[k])
+        chain = cuda(chain)
         meshes.append(B.MeshTensors(cuda(dm.positions), cuda(dm.colors), cuda(dm.uvs), cuda(dm.normals),
                                     cuda(dm.indices.astype(np.int32)), chain.contiguous(), sc.textures[k].shape[1],
                                     sc.textures[k].shape[0], levels))

@@ -313,6 +313,19 @@ typedef struct mlv_diff_grads {
     float *sh;                                 /* device, [9][3] */
 } mlv_diff_grads;
 
+/* The mip chain the differentiable pass samples, built on the device: level 0 is the texture
+ * ((height, width) RGBA32F), each next level its 2x2 box filter (2x1 or 1x2 once a side is 1; an
+ * odd side drops its last row or column), all levels concatenated. mlv_mip_chain_adjoint maps a
+ * gradient on the whole chain to the level-0 texture. Extension: no counterpart in the reference,
+ * whose textures are single-level (load_texture, main.c:538-560); they replace the per-level
+ * framework calls a host would otherwise make twice per training step. Both run on the context's
+ * stream. */
+MLV_API int mlv_mip_chain_size(uint32_t width, uint32_t height, uint32_t *texels, uint32_t *levels);
+MLV_API int mlv_mip_chain(mlv_context *ctx, const float *device_texture, uint32_t width, uint32_t height,
+                          float *device_chain);
+MLV_API int mlv_mip_chain_adjoint(mlv_context *ctx, const float *device_grad_chain, uint32_t width, uint32_t height,
+                                  float *device_grad_texture);
+
 /* device_rgba_out: width*height float4. */
 MLV_API int mlv_diff_forward(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
                              float *device_rgba_out);

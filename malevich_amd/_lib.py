@@ -159,6 +159,9 @@ SIGNATURES = {
     "mlv_diff_backward": (_I, [_P, C.POINTER(DiffScene), _P, _P, C.POINTER(DiffGrads), _I]),
     "mlv_diff_batch_forward": (_I, [_P, C.POINTER(DiffMesh), _U32, C.POINTER(DiffView), _U32, _P]),
     "mlv_diff_batch_backward": (_I, [_P, C.POINTER(DiffMesh), _U32, C.POINTER(DiffView), _U32, _P, _P, _I]),
+    "mlv_mip_chain_size": (_I, [_U32, _U32, C.POINTER(_U32), C.POINTER(_U32)]),
+    "mlv_mip_chain": (_I, [_P, _P, _U32, _U32, _P]),
+    "mlv_mip_chain_adjoint": (_I, [_P, _P, _U32, _U32, _P]),
 }
 
 _lib: C.CDLL | None = None
@@ -192,3 +195,12 @@ class MalevichError(RuntimeError):
     def __init__(self, code: int, message: str):
         super().__init__(f"[{code}] {message}")
         self.code = code
+
+
+def mip_chain_size(width: int, height: int) -> tuple[int, int]:
+    """(texels, levels) of the mip chain mlv_mip_chain builds for a width x height texture."""
+    n, levels = C.c_uint32(0), C.c_uint32(0)
+    rc = load().mlv_mip_chain_size(width, height, C.byref(n), C.byref(levels))
+    if rc != 0:
+        raise MalevichError(rc, f"mip_chain_size: bad size {width}x{height}")
+    return int(n.value), int(levels.value)

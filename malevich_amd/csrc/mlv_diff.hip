@@ -756,6 +756,110 @@ uint32_t diff_tile_blocks(uint32_t W, uint32_t H) {
     return ((W + kTileSide - 1u) / kTileSide) * ((H + kTileSide - 1u) / kTileSide);
 }
 
+// ------------------------------------------------------------ mip chain and its adjoint
+//
+// The texture's mip chain by 2x2 box filtering (a 2x1 or 1x2 box once a side is 1; an odd side
+// drops its last row or column), every level concatenated after level 0. Each texel of level l + 1
+// is the sum of its window in level l, in row order, divided by the window's area. The adjoint
+// maps a gradient on the whole chain to level 0: a level-0 texel receives its own entry plus, for
+// each coarser level whose window holds its ancestor, that ancestor's entry divided by the
+// product of the window areas in between.
+
+struct MipDims {
+    uint32_t w[kMaxMipLevels], h[kMaxMipLevels], off[kMaxMipLevels];
+    uint32_t levels;
+};
+
+__host__ __device__ inline MipDims mip_dims(uint32_t w, uint32_t h) {
+    MipDims d{};
+    uint32_t off = 0u, l = 0u;
+    for (;;) {
+        d.w[l] = w;
+        d.h[l] = h;
+        d.off[l] = off;
+        off += w * h;
+        ++l;
+        if ((w == 1u && h == 1u) || l == kMaxMipLevels) break;
+        w = w > 1u ? w / 2u : 1u;
+        h = h > 1u ? h / 2u : 1u;
+    }
+    d.levels = l;
+    return d;
+}
+
+__global__ __launch_bounds__(256) void kd_mip_level(float4 *chain, MipDims d, uint32_t l) {
+    const uint32_t W = d.w[l + 1u], H = d.h[l + 1u], sw = d.w[l];
+    const uint32_t kw = d.w[l] > 1u ? 2u : 1u, kh = d.h[l] > 1u ? 2u : 1u;
+    const float4 *src = chain + d.off[l];
+    float4 *dst = chain + d.off[l + 1u];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < W * H; i += gridDim.x * 256u) {
+        const uint32_t x = i % W, y = i / W;
+        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+        for (uint32_t dy = 0; dy < kh; ++dy)
+            for (uint32_t dx = 0; dx < kw; ++dx) {
+                const float4 t = src[(size_t)(kh * y + dy) * sw + kw * x + dx];
+                s.x += t.x;
+                s.y += t.y;
+                s.z += t.z;
+                s.w += t.w;
+            }
+        const float a = (float)(kw * kh);
+        dst[i] = make_float4(s.x / a, s.y / a, s.z / a, s.w / a);
+    }
+}
+
+__global__ __launch_bounds__(256) void kd_mip_copy(const float4 *tex, float4 *chain, size_t n) {
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u) chain[i] = tex[i];
+}
+
+__global__ __launch_bounds__(256) void kd_mip_adjoint(const float4 *gchain, MipDims d, float4 *gtex) {
+    const uint32_t n = d.w[0] * d.h[0];
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < n; i += gridDim.x * 256u) {
+        // walk up the ancestors while each is inside the next level's windows; each level's entry
+        // counts with the product of the window areas below it
+        uint32_t x = i % d.w[0], y = i / d.w[0];
+        float4 acc = gchain[i];
+        float scale = 1.0f;
+        for (uint32_t l = 0; l + 1u < d.levels; ++l) {
+            const uint32_t kw = d.w[l] > 1u ? 2u : 1u, kh = d.h[l] > 1u ? 2u : 1u;
+            x /= kw;
+            y /= kh;
+            if (x >= d.w[l + 1u] || y >= d.h[l + 1u]) break;  // a dropped odd row or column
+            scale = scale / (float)(kw * kh);
+            const float4 g = gchain[d.off[l + 1u] + y * d.w[l + 1u] + x];
+            acc = make_float4(acc.x + g.x * scale, acc.y + g.y * scale, acc.z + g.z * scale, acc.w + g.w * scale);
+        }
+        gtex[i] = acc;
+    }
+}
+
+uint32_t mip_chain_texels(uint32_t w, uint32_t h, uint32_t *levels) {
+    const MipDims d = mip_dims(w, h);
+    if (levels) *levels = d.levels;
+    return d.off[d.levels - 1u] + d.w[d.levels - 1u] * d.h[d.levels - 1u];
+}
+
+hipError_t launch_mip_chain(const float4 *tex, uint32_t w, uint32_t h, float4 *chain, hipStream_t s) {
+    const MipDims d = mip_dims(w, h);
+    const size_t n0 = (size_t)w * h;
+    hipLaunchKernelGGL(kd_mip_copy, dim3((unsigned)std::min<size_t>((n0 + 255) / 256, 4096)), dim3(256), 0, s, tex,
+                       chain, n0);
+    for (uint32_t l = 0; l + 1u < d.levels; ++l) {
+        const uint32_t n = d.w[l + 1u] * d.h[l + 1u];
+        hipLaunchKernelGGL(kd_mip_level, dim3(std::min<uint32_t>((n + 255u) / 256u, 4096u)), dim3(256), 0, s, chain,
+                           d, l);
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_mip_adjoint(const float4 *gchain, uint32_t w, uint32_t h, float4 *gtex, hipStream_t s) {
+    const MipDims d = mip_dims(w, h);
+    const uint32_t n = w * h;
+    hipLaunchKernelGGL(kd_mip_adjoint, dim3(std::min<uint32_t>((n + 255u) / 256u, 4096u)), dim3(256), 0, s, gchain, d,
+                       gtex);
+    return hipGetLastError();
+}
+
 hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s) {
     const uint32_t nb = diff_blocks(p.v.W, p.v.H);
     if (nb == 0) return hipSuccess;

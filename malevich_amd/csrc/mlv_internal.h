@@ -238,6 +238,10 @@ struct DiffScratch {
     uint32_t *err;                           // device: scan error counter
 };
 hipError_t launch_diff_forward(const DiffParams &p, float4 *out, hipStream_t s);
+constexpr uint32_t kMaxMipLevels = 16;  // textures up to 32768 texels on a side
+uint32_t mip_chain_texels(uint32_t w, uint32_t h, uint32_t *levels);
+hipError_t launch_mip_chain(const float4 *tex, uint32_t w, uint32_t h, float4 *chain, hipStream_t s);
+hipError_t launch_mip_adjoint(const float4 *gchain, uint32_t w, uint32_t h, float4 *gtex, hipStream_t s);
 hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const DiffGrads &g, const DiffScratch &sc,
                                 bool deterministic, hipStream_t s);
 // Blocks of the per-pixel launches: 256 pixels in row order, or 16x16 tiles (the atomic

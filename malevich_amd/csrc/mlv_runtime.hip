@@ -1523,6 +1523,34 @@ MLV_API int mlv_diff_forward(mlv_context *ctx, const mlv_diff_scene *scene, cons
     return MLV_OK;
 }
 
+MLV_API int mlv_mip_chain_size(uint32_t width, uint32_t height, uint32_t *texels, uint32_t *levels) {
+    if (!width || !height || width > 32768u || height > 32768u || (uint64_t)width * height > 0x40000000ull || !texels)
+        return MLV_ERROR_INVALID_ARGUMENT;
+    *texels = mip_chain_texels(width, height, levels);
+    return MLV_OK;
+}
+
+MLV_API int mlv_mip_chain(mlv_context *ctx, const float *device_texture, uint32_t width, uint32_t height,
+                          float *device_chain) {
+    uint32_t n = 0;
+    if (!ctx || !device_texture || !device_chain || mlv_mip_chain_size(width, height, &n, nullptr) != MLV_OK)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mip_chain: bad argument");
+    RC_OR_RETURN(set_device(ctx));
+    HIP_OR_FAIL(ctx, launch_mip_chain((const float4 *)device_texture, width, height, (float4 *)device_chain, ctx->stream));
+    return MLV_OK;
+}
+
+MLV_API int mlv_mip_chain_adjoint(mlv_context *ctx, const float *device_grad_chain, uint32_t width, uint32_t height,
+                                  float *device_grad_texture) {
+    uint32_t n = 0;
+    if (!ctx || !device_grad_chain || !device_grad_texture || mlv_mip_chain_size(width, height, &n, nullptr) != MLV_OK)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mip_chain_adjoint: bad argument");
+    RC_OR_RETURN(set_device(ctx));
+    HIP_OR_FAIL(ctx, launch_mip_adjoint((const float4 *)device_grad_chain, width, height, (float4 *)device_grad_texture,
+                                        ctx->stream));
+    return MLV_OK;
+}
+
 MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
                               const float *device_grad_rgba, const mlv_diff_grads *grads, int deterministic) {
     if (!ctx || !device_grad_rgba || !grads) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff_backward: bad argument");

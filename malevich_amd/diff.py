@@ -16,7 +16,6 @@ from __future__ import annotations
 import ctypes as C
 
 import torch
-import torch.nn.functional as F
 
 from . import _lib as L
 from . import pipeline as P
@@ -44,17 +43,39 @@ class _StreamOrder:
             raise RuntimeError("stream ordering failed")
 
 
-def mip_chain(tex: torch.Tensor) -> tuple[torch.Tensor, int]:
-    """(h, w, 4) texture -> ((N, 4) chain of all levels by 2x2 box filtering, level count).
-    Differentiable: gradients on coarse levels flow back to level 0."""
-    levels = [tex]
-    t = tex.permute(2, 0, 1).unsqueeze(0)
-    while t.shape[2] > 1 or t.shape[3] > 1:
-        kh = 2 if t.shape[2] > 1 else 1
-        kw = 2 if t.shape[3] > 1 else 1
-        t = F.avg_pool2d(t, (kh, kw))
-        levels.append(t[0].permute(1, 2, 0))
-    return torch.cat([l.reshape(-1, 4) for l in levels]).contiguous(), len(levels)
+def mip_chain(tex: torch.Tensor, renderer: "DiffRenderer") -> tuple[torch.Tensor, int]:
+    """(h, w, 4) texture -> ((N, 4) chain of all levels by 2x2 box filtering, level count), built
+    by the library on the device (mlv_mip_chain). Differentiable: gradients on coarse levels flow
+    back to level 0 (mlv_mip_chain_adjoint)."""
+    h, w = int(tex.shape[0]), int(tex.shape[1])
+    n, levels = L.mip_chain_size(w, h)
+    return MipChainFunction.apply(tex, renderer, n), levels
+
+
+class MipChainFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, tex, renderer, n):
+        tex = tex.detach().contiguous().float()
+        h, w = int(tex.shape[0]), int(tex.shape[1])
+        chain = torch.empty((n, 4), dtype=torch.float32, device=tex.device)
+        d = renderer.dev
+        renderer._enter()
+        P._check(d.ctx, d.lib.mlv_mip_chain(d.ctx, tex.data_ptr(), w, h, chain.data_ptr()))
+        renderer._leave()
+        ctx.renderer, ctx.size = renderer, (w, h)
+        return chain
+
+    @staticmethod
+    def backward(ctx, grad_chain):
+        w, h = ctx.size
+        g = grad_chain.contiguous().float()
+        out = torch.empty((h, w, 4), dtype=torch.float32, device=g.device)
+        r = ctx.renderer
+        d = r.dev
+        r._enter()
+        P._check(d.ctx, d.lib.mlv_mip_chain_adjoint(d.ctx, g.data_ptr(), w, h, out.data_ptr()))
+        r._leave()
+        return out, None, None
 
 
 class DiffRenderer:
@@ -90,6 +111,26 @@ class DiffRenderer:
         """vertices (V, 8) float32 in basic_vs's layout, indices (T, 3) int32, cb (48,) float32."""
         d = self.dev
         self._enter()
+        self._record_raster(vertices, indices, cb)
+        d.flush()
+        d.om_set_primitive_id_buffer(None)
+        self._leave()
+        return self.prim
+
+    def autotune(self, vertices: torch.Tensor, indices: torch.Tensor, cb: torch.Tensor, repeats: int = 3) -> dict:
+        """Chooses the context's launch parameters for this raster (mlv_autotune_begin/end): the
+        tile stage's form for few tiles with long lists differs from a full-screen frame's."""
+        d = self.dev
+        self._enter()
+        d.autotune_begin()
+        self._record_raster(vertices, indices, cb)
+        best = d.autotune_end(repeats=repeats)
+        d.om_set_primitive_id_buffer(None)
+        self._leave()
+        return best
+
+    def _record_raster(self, vertices: torch.Tensor, indices: torch.Tensor, cb: torch.Tensor) -> None:
+        d = self.dev
         vb = self._wrap(L.BUFFER_VERTEX, vertices)
         ib = self._wrap(L.BUFFER_INDEX_U32, indices)
         cbuf = self._wrap(L.BUFFER_CONSTANT, cb)
@@ -106,10 +147,6 @@ class DiffRenderer:
         d.clear_render_target_view(self.rt, (0.0, 0.0, 0.0, 0.0))
         d.clear_depth_stencil_view(self.ds, 0.0)
         d.draw_indexed(indices.numel())
-        d.flush()
-        d.om_set_primitive_id_buffer(None)
-        self._leave()
-        return self.prim
 
     def scene_struct(self, positions, colors, uvs, normals, indices, chain, tex_w, tex_h, levels, sh,
                      clip_from_world) -> L.DiffScene:
@@ -146,6 +183,16 @@ class DiffRenderer:
         self.ds.release()
 
 
+def raster_inputs(positions, normals, uvs, clip_from_world):
+    """The raster's vertex buffer (basic_vs's layout: position, normal, uv) and constant buffer
+    (clip_from_world, then two identity matrices)."""
+    M = clip_from_world.detach().to(torch.float32).reshape(4, 4).cpu()
+    eye = torch.eye(4, dtype=torch.float32)
+    cb = torch.cat([M.reshape(-1), eye.reshape(-1), eye.reshape(-1)]).cuda()
+    verts = torch.cat([positions.float(), normals.float(), uvs.float()], 1).detach().contiguous()
+    return verts, cb
+
+
 class RenderFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, positions, colors, uvs, normals, chain, sh, renderer, indices, clip_from_world, tex_w, tex_h,
@@ -154,9 +201,7 @@ class RenderFunction(torch.autograd.Function):
         chain, sh = chain.contiguous().float(), sh.contiguous().float()
         idx = indices.contiguous().to(torch.int32)
         M = clip_from_world.detach().to(torch.float32).reshape(4, 4).cpu()
-        eye = torch.eye(4, dtype=torch.float32)
-        cb = torch.cat([M.reshape(-1), eye.reshape(-1), eye.reshape(-1)]).cuda()
-        verts = torch.cat([positions, normals, uvs], 1).detach().contiguous()
+        verts, cb = raster_inputs(positions, normals, uvs, M)
         prim = renderer.rasterize(verts, idx, cb).clone()
         sc = renderer.scene_struct(positions, colors, uvs, normals, idx, chain, tex_w, tex_h, levels, sh, M)
         out = renderer.shade(sc, prim)
@@ -184,6 +229,6 @@ class RenderFunction(torch.autograd.Function):
 def render(renderer: DiffRenderer, positions, colors, uvs, normals, indices, texture, sh, clip_from_world):
     """Image (H, W, 4) of the mesh; differentiable in positions, colours, uvs, normals, the
     level-0 texture (through its box-filtered mip chain) and the SH coefficients."""
-    chain, levels = mip_chain(texture)
+    chain, levels = mip_chain(texture, renderer)
     return RenderFunction.apply(positions, colors, uvs, normals, chain, sh, renderer, indices, clip_from_world,
                                 texture.shape[1], texture.shape[0], levels)

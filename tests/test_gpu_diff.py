@@ -149,3 +149,56 @@ def test_autograd_function(device, oracle, small_scene):
         assert e <= 1e-4, (k, e)
     tex0 = _box_adjoint_to_level0(go["texture"], sc.texture.shape[1], sc.texture.shape[0], sc.levels)
     assert rel_err(params["texture"].grad.cpu().numpy(), tex0) <= 1e-4
+
+
+def test_raster_autotune_keeps_visibility(device, oracle, small_scene):
+    """DiffRenderer.autotune picks launch parameters from the candidate set for its raster; the
+    visibility afterwards is the oracle's, as before."""
+    sc = small_scene
+    prim_o = oracle_prim_ids(oracle, sc)
+    saved = device.tuning()
+    r = D.DiffRenderer(device, sc.width, sc.height)
+    try:
+        t = _tensors(sc)
+        verts, cb = D.raster_inputs(t["positions"], t["normals"], t["uvs"], t["M"])
+        best = r.autotune(verts, t["indices"], cb, repeats=1)
+        assert best["heavy_tile_pairs"] in (192, 384, 768, 1536, 0xFFFFFFFF) and best["tile_stage"] in (128, 256)
+        prim = r.rasterize(verts, t["indices"], cb).clone()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(prim.cpu().numpy().view(np.uint32), prim_o)
+    finally:
+        r.release()
+        device.set_tuning(saved["heavy_tile_pairs"], saved["heavy_blocks"], saved["pair_blocks"], saved["tile_stage"])
+
+
+@pytest.mark.parametrize("size", [(64, 64), (37, 20), (1, 9), (128, 32)])
+def test_mip_chain_and_adjoint(device, size):
+    """mlv_mip_chain against the float64 box-filter chain, and mlv_mip_chain_adjoint against the
+    adjoint PyTorch's autograd takes through the same chain of 2x2 average pools (odd edges
+    dropped), both to 1e-6 relative; sizes with odd and unit sides."""
+    import torch.nn.functional as F
+
+    w, h = size
+    rng = np.random.default_rng(w * 100 + h)
+    tex_np = rng.uniform(0.0, 1.0, (h, w, 4)).astype(np.float32)
+    ref_chain, ref_levels = S.mip_chain(tex_np)
+    r = D.DiffRenderer(device, 8, 8)
+    try:
+        tex = torch.from_numpy(tex_np).cuda().requires_grad_(True)
+        chain, levels = D.mip_chain(tex, r)
+        torch.cuda.synchronize()
+        assert levels == ref_levels and chain.shape == ref_chain.shape
+        assert rel_err(chain.detach().cpu().numpy(), ref_chain) <= 1e-6
+        g = torch.from_numpy(rng.normal(0.0, 1.0, ref_chain.shape).astype(np.float32)).cuda()
+        (chain * g).sum().backward()
+        torch.cuda.synchronize()
+        # the same chain in PyTorch ops, differentiated by autograd
+        t2 = torch.from_numpy(tex_np).cuda().requires_grad_(True)
+        lv, t = [t2], t2.permute(2, 0, 1).unsqueeze(0)
+        while t.shape[2] > 1 or t.shape[3] > 1:
+            t = F.avg_pool2d(t, (2 if t.shape[2] > 1 else 1, 2 if t.shape[3] > 1 else 1))
+            lv.append(t[0].permute(1, 2, 0))
+        (torch.cat([x.reshape(-1, 4) for x in lv]) * g).sum().backward()
+        assert rel_err(tex.grad.cpu().numpy(), t2.grad.cpu().numpy()) <= 1e-6
+    finally:
+        r.release()
