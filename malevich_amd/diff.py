@@ -90,6 +90,7 @@ class DiffRenderer:
         self.prim = torch.empty((height, width), dtype=torch.int32, device="cuda")
         self.order = _StreamOrder()
         self._wrapped: dict = {}
+        self.cb_cache: dict = {}  # the raster's constant buffer per camera matrix
 
     def _wrap(self, kind: int, t: torch.Tensor) -> P.Buffer:
         key = (kind, t.data_ptr(), t.numel() * t.element_size())
@@ -179,16 +180,24 @@ class DiffRenderer:
         for b in self._wrapped.values():
             b.release()
         self._wrapped.clear()
+        self.cb_cache.clear()
         self.rt.release()
         self.ds.release()
 
 
-def raster_inputs(positions, normals, uvs, clip_from_world):
+def raster_inputs(positions, normals, uvs, clip_from_world, cache: dict | None = None):
     """The raster's vertex buffer (basic_vs's layout: position, normal, uv) and constant buffer
-    (clip_from_world, then two identity matrices)."""
+    (clip_from_world, then two identity matrices). With a cache, the constant buffer is uploaded
+    once per matrix: a host-to-device copy every step would make the host wait for the GPU."""
     M = clip_from_world.detach().to(torch.float32).reshape(4, 4).cpu()
-    eye = torch.eye(4, dtype=torch.float32)
-    cb = torch.cat([M.reshape(-1), eye.reshape(-1), eye.reshape(-1)]).cuda()
+    key = M.numpy().tobytes()
+    cb = cache.get(key) if cache is not None else None
+    if cb is None:
+        eye = torch.eye(4, dtype=torch.float32)
+        cb = torch.cat([M.reshape(-1), eye.reshape(-1), eye.reshape(-1)]).to(positions.device)
+        if cache is not None:
+            cache.clear()
+            cache[key] = cb
     verts = torch.cat([positions.float(), normals.float(), uvs.float()], 1).detach().contiguous()
     return verts, cb
 
@@ -201,7 +210,7 @@ class RenderFunction(torch.autograd.Function):
         chain, sh = chain.contiguous().float(), sh.contiguous().float()
         idx = indices.contiguous().to(torch.int32)
         M = clip_from_world.detach().to(torch.float32).reshape(4, 4).cpu()
-        verts, cb = raster_inputs(positions, normals, uvs, M)
+        verts, cb = raster_inputs(positions, normals, uvs, M, renderer.cb_cache)
         prim = renderer.rasterize(verts, idx, cb).clone()
         sc = renderer.scene_struct(positions, colors, uvs, normals, idx, chain, tex_w, tex_h, levels, sh, M)
         out = renderer.shade(sc, prim)
