@@ -55,8 +55,12 @@ class HipEvents:
 def build_c2(args):
     from malevich_amd import scenes as S
     sc = S.c2_scene(args.width, args.height, n_instances=args.instances, seed=args.seed, texture_size=0)
-    env = S.smooth_env_map(2048, 1024, seed=args.seed + 17, lo=0.05, hi=2.0)
-    return sc, env
+    # the configuration's maps: irradiance 256x128 and radiance 2048x1024, RGBA16F (the reference's
+    # shaders sample level 0 of one map each, so the configuration's mip levels and per-mesh
+    # RGBA8 textures have no consumer and are not uploaded)
+    irr = S.smooth_env_map(256, 128, seed=args.seed + 16, lo=0.05, hi=2.0).astype(np.float16)
+    rad = S.smooth_env_map(2048, 1024, seed=args.seed + 17, lo=0.05, hi=2.0).astype(np.float16)
+    return sc, (irr, rad)
 
 
 def upload(dev, sc, env):
@@ -67,7 +71,8 @@ def upload(dev, sc, env):
         res["ib"].append(dev.create_buffer(L.BUFFER_INDEX_U32, m.indices))
         res["counts"].append(m.indices.size)
     res["cb"] = dev.create_buffer(L.BUFFER_CONSTANT, sc.cb)
-    res["env"] = dev.create_texture2d(env)
+    res["irr"] = dev.create_texture2d(env[0])
+    res["env"] = dev.create_texture2d(env[1])
     res["rt"] = dev.create_render_target(sc.width, sc.height)
     res["ds"] = dev.create_depth_stencil(sc.width, sc.height)
     return res
@@ -87,9 +92,10 @@ def frame(dev, res, sc):
 def bind_state(dev, res, sc):
     from malevich_amd import _lib as L
     dev.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
-    dev.vs_set_shader(L.VS_BASIC)
+    dev.vs_set_shader(L.VS_VERTEX_LIGHTING)
     dev.ps_set_shader(L.PS_ENV_LIGHTING)
     dev.vs_set_constant_buffer(0, res["cb"])
+    dev.vs_set_shader_resource(0, res["irr"])
     dev.ps_set_shader_resource(0, res["env"])
     dev.rs_set_viewport(0, 0, sc.width, sc.height, 0.0, 1.0)
     dev.om_set_render_targets(res["rt"], res["ds"])
@@ -103,6 +109,8 @@ def cpu_baseline(sc, env, budget_s: float):
     figure over the first instance draws is reported beside it."""
     import ctypes as C
     from oracle import binding as ob
+    # the nearest-reference workload: the same maps as float32 (RGBA16F texels are exact in float32)
+    irr, rad = (t.astype(np.float32) for t in env)
 
     def run(n_draws=None):
         rt = ob.RenderTarget(sc.width, sc.height, with_vis=False)
@@ -110,7 +118,8 @@ def cpu_baseline(sc, env, budget_s: float):
         rt.clear_depth_stencil_view(0.0)
         tris = 0
         for m in sc.instances[:n_draws]:
-            d = ob.Draw(m.indices, m.vertices, ob.VS_BASIC, ob.PS_ENV_LIGHTING, cb=sc.cb, ps_tex=env,
+            d = ob.Draw(m.indices, m.vertices, ob.VS_VERTEX_LIGHTING, ob.PS_ENV_LIGHTING, cb=sc.cb, vs_tex=irr,
+                        ps_tex=rad,
                         viewport=(0, 0, sc.width, sc.height, 0, 1))
             ob.draw_indexed(rt, d)
             tris += m.triangle_count
@@ -133,8 +142,8 @@ def cpu_baseline(sc, env, budget_s: float):
     dt1 = time.perf_counter() - t1
     gomp.omp_set_num_threads(cores)
     return {"value": tris / dt / 1e6, "unit": "Mtri/s", "cores": cores, "kind": "port",
-            "sample": f"{frames} full frame(s) of the same 1080p workload ({tris} triangles, {dt:.1f} s); oracle "
-                      f"(C, gcc -O2 -fopenmp) on {cores} threads",
+            "sample": f"{frames} full frame(s) of the same 1080p workload ({tris} triangles, {dt:.1f} s) with the "
+                      f"maps as float32 (the reference's texel formats); oracle (C, gcc -O2 -fopenmp) on {cores} threads",
             "single_thread_value": round(tris1 / dt1 / 1e6, 3),
             "single_thread_sample": f"first 8 of {len(sc.instances)} draws ({tris1} triangles, {dt1:.1f} s)"}
 
@@ -626,8 +635,9 @@ def main(argv=None):
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": f"c2-1080p-forward: {len(sc.instances)} displaced geodesic spheres, {tris} "
-                               f"triangles, depth complexity {sc.depth_complexity:.1f}, basic_vs + env_lighting_ps "
-                               f"(2048x1024 RGBA32F equirect), reversed-Z GE, {sc.width}x{sc.height}",
+                               f"triangles, depth complexity {sc.depth_complexity:.1f}, vertex_lighting_vs (256x128 "
+                               f"RGBA16F irradiance) + env_lighting_ps (2048x1024 RGBA16F radiance), reversed-Z GE, "
+                               f"{sc.width}x{sc.height}",
                    "width": sc.width, "height": sc.height, "triangles": tris, "draws_per_frame": len(sc.instances)},
         "gpix_per_s": round(gpix_s, 4),
         "stage_ms_per_frame": {k: round(v / args.steps, 4) for k, v in stage_ms.items()},

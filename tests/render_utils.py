@@ -95,8 +95,10 @@ def render_oracle(oracle, W: int, H: int, draws: list[DrawSpec], clear_color, cl
     rt.clear_render_target_view(clear_color)
     rt.clear_depth_stencil_view(clear_depth, depth_func)
     for d in draws:
-        od = oracle.Draw(d.indices, d.vertices, VS[d.vs][1], PS[d.ps][1], cb=d.cb, vs_tex=d.vs_tex,
-                         ps_tex=d.ps_tex, viewport=(0, 0, W, H, 0, 1), start_index=d.start_index,
+        # RGBA16F texels are exact in float32, so the oracle samples their float32 values
+        up = lambda t: t.astype(np.float32) if t is not None and t.dtype == np.float16 else t
+        od = oracle.Draw(d.indices, d.vertices, VS[d.vs][1], PS[d.ps][1], cb=d.cb, vs_tex=up(d.vs_tex),
+                         ps_tex=up(d.ps_tex), viewport=(0, 0, W, H, 0, 1), start_index=d.start_index,
                          base_vertex=d.base_vertex, index_count=d.index_count, depth_func=depth_func)
         oracle.draw_indexed(rt, od)
     st = {k: getattr(rt.stats, k) for k, _ in oracle.Stats._fields_}

@@ -67,6 +67,20 @@ def test_vertex_lighting_and_env_pixel_lighting(device, oracle):
     assert_frames_match(g, o, color_tol=1)
 
 
+def test_fp16_env_maps(device, oracle):
+    """vertex_lighting_vs and env_lighting_ps sampling RGBA16F maps (an extension format): the
+    frame equals the oracle's on the maps' exact float32 values."""
+    W, H = 160, 96
+    irr = S.smooth_env_map(256, 128, seed=12, lo=0.05, hi=2.0).astype(np.float16)
+    rad = S.smooth_env_map(512, 256, seed=13, lo=0.05, hi=2.0).astype(np.float16)
+    mesh = S.sphere_mesh(12, displace=0.1, seed=4)
+    cb = S.camera_cb((2.8, 0.7, 0.6), (0, 0, 0), 60.0, W, H)
+    d = DrawSpec(mesh.indices, mesh.vertices, "vertex_lighting", "env_lighting", cb=cb, vs_tex=irr, ps_tex=rad)
+    g, o = both(device, oracle, W, H, [d])
+    assert (g.vis_tri != 0xFFFFFFFF).sum() > 1000
+    assert_frames_match(g, o, color_tol=1)
+
+
 def test_clipping_near_plane_and_screen_edges(device, oracle):
     W, H = 192, 128
     tex = S.random_rgba8(64, seed=9)
