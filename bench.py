@@ -454,7 +454,18 @@ def bench_c5_diff(dev, sc, args, ev):
             "fwd_bwd_per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
             "fwd_bwd_batched_over_per_draw": round(ms_p / ms_b, 2),
             "fwd_bwd_batched_phases": ph_b, "fwd_bwd_per_draw_phases": ph_p,
-            "shaded_pixels": int(r.offsets[-1])}
+            "shaded_pixels": int(r.offsets[-1]),
+            "shading_pass": shading_roofline(int(r.offsets[-1]), ph_b["shade_ms"])}
+
+
+def shading_roofline(pixels: int, ms: float) -> dict:
+    """The batched shading launch (kd_batch_forward) against HBM: compulsory bytes are the
+    primitive id and draw tag read and the RGBA32F colour written per pixel (24 B). The visible
+    triangles' attributes and the sampled texels come on top, so this is a lower bound."""
+    b = 24 * pixels
+    gbs = b / (ms * 1e-3) / 1e9
+    return {"kernel": "kd_batch_forward", "ms": ms, "compulsory_bytes": b, "achieved": round(gbs, 1),
+            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
 
 
 def copy_calibration(mib: int):
