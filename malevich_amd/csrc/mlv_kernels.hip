@@ -800,66 +800,75 @@ constexpr uint32_t kRadixGrid = 4096;  // blocks of a flush's radix passes (each
 
 // Digit counts of one block's kRadixItems keys, stored digit-major (hist[digit * nb + block])
 // so that one exclusive scan over hist gives every (digit, block) its output offset.
+template <uint32_t NT>
 __device__ __forceinline__ void radix_hist_block(const uint32_t *keys, uint32_t n, uint32_t nb, uint32_t b,
                                                  uint32_t shift, uint32_t *hist) {
+    constexpr uint32_t R = kRadixItems / NT;
     __shared__ uint32_t h[256];
-    h[threadIdx.x] = 0u;
+    if (threadIdx.x < 256u) h[threadIdx.x] = 0u;
     __syncthreads();
     const uint32_t base = b * kRadixItems;
     // all of the thread's loads are issued before the first count, so their latencies overlap
-    uint32_t k[kRadixRounds];
+    uint32_t k[R];
 #pragma unroll
-    for (uint32_t r = 0; r < kRadixRounds; ++r) {
-        const uint32_t i = base + r * 256u + threadIdx.x;
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t i = base + r * NT + threadIdx.x;
         k[r] = i < n ? keys[i] : 0xFFFFFFFFu;
     }
 #pragma unroll
-    for (uint32_t r = 0; r < kRadixRounds; ++r)
-        if (base + r * 256u + threadIdx.x < n) atomicAdd(&h[(k[r] >> shift) & 255u], 1u);
+    for (uint32_t r = 0; r < R; ++r)
+        if (base + r * NT + threadIdx.x < n) atomicAdd(&h[(k[r] >> shift) & 255u], 1u);
     __syncthreads();
-    hist[threadIdx.x * nb + b] = h[threadIdx.x];
+    if (threadIdx.x < 256u) hist[threadIdx.x * nb + b] = h[threadIdx.x];
 }
 
-// Stable scatter of one block's keys and values to the scanned offsets.
+// Stable scatter of one block's keys and values to the scanned offsets, by NT threads: rounds of
+// NT items in index order; within a round each wave ranks its items by digit with ballots, and
+// the waves' counts per digit are prefixed in wave order.
+template <uint32_t NT>
 __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                     uint32_t *vout, uint32_t n, uint32_t nb, uint32_t b,
                                                     uint32_t shift, const uint32_t *hist) {
-    __shared__ uint32_t wc[4][256];
+    constexpr uint32_t NW = NT / 64u, R = kRadixItems / NT;
+    static_assert(NT >= 256 && NT % 64 == 0 && kRadixItems % NT == 0, "radix block shape");
+    __shared__ uint32_t wc[NW][256];
     __shared__ uint32_t loff[256];  // per digit: next free slot of the block's items in LDS
     __shared__ uint32_t lbeg[256];  // per digit: first slot of the block's items in LDS
     __shared__ uint32_t goff[256];  // per digit: first slot of the block's items in the output
-    __shared__ uint32_t s_w[5];
+    __shared__ uint32_t s_w[NW + 1];
     __shared__ uint32_t s_key[kRadixItems], s_val[kRadixItems];
     const uint32_t base = b * kRadixItems;
     const uint32_t tid = threadIdx.x, w = tid >> 6;
     // the block's count per digit is the step of the scanned digit-major histogram
     {
-        const uint32_t idx = tid * nb + b;
-        const uint32_t here = hist[idx];
-        const uint32_t next = (idx + 1u < 256u * nb) ? hist[idx + 1u] : n;
+        uint32_t here = 0u, next = 0u;
+        if (tid < 256u) {
+            const uint32_t idx = tid * nb + b;
+            here = hist[idx];
+            next = (idx + 1u < 256u * nb) ? hist[idx + 1u] : n;
+        }
         uint32_t total;
-        const uint32_t ex = block_exclusive_scan<256, uint32_t>(next - here, s_w, total);
-        goff[tid] = here;
-        loff[tid] = ex;
-        lbeg[tid] = ex;
+        const uint32_t ex = block_exclusive_scan<NT, uint32_t>(next - here, s_w, total);
+        if (tid < 256u) {
+            goff[tid] = here;
+            loff[tid] = ex;
+            lbeg[tid] = ex;
+        }
     }
     // the thread's items, loaded before the first round so the loads' latencies overlap
-    uint32_t keys[kRadixRounds], vals[kRadixRounds];
+    uint32_t keys[R], vals[R];
 #pragma unroll
-    for (uint32_t r = 0; r < kRadixRounds; ++r) {
-        const uint32_t i = base + r * 256u + tid;
+    for (uint32_t r = 0; r < R; ++r) {
+        const uint32_t i = base + r * NT + tid;
         keys[r] = i < n ? kin[i] : 0u;
         vals[r] = i < n ? vin[i] : 0u;
     }
     // rank every item among the block's items of its digit (stable) and place it in LDS
 #pragma unroll
-    for (uint32_t r = 0; r < kRadixRounds; ++r) {
-        wc[0][tid] = 0u;
-        wc[1][tid] = 0u;
-        wc[2][tid] = 0u;
-        wc[3][tid] = 0u;
+    for (uint32_t r = 0; r < R; ++r) {
+        for (uint32_t e = tid; e < NW * 256u; e += NT) (&wc[0][0])[e] = 0u;
         __syncthreads();
-        const uint32_t i = base + r * 256u + tid;
+        const uint32_t i = base + r * NT + tid;
         const bool valid = i < n;
         const uint32_t key = keys[r];
         const uint32_t val = vals[r];
@@ -874,14 +883,15 @@ __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const u
         const uint32_t rank = mbcnt(m);
         if (valid && rank == 0u) wc[w][dg] = (uint32_t)__popcll(m);
         __syncthreads();
-        {
-            const uint32_t c0 = wc[0][tid], c1 = wc[1][tid], c2 = wc[2][tid], c3 = wc[3][tid];
-            const uint32_t e0 = loff[tid];
-            wc[0][tid] = e0;
-            wc[1][tid] = e0 + c0;
-            wc[2][tid] = e0 + c0 + c1;
-            wc[3][tid] = e0 + c0 + c1 + c2;
-            loff[tid] = e0 + c0 + c1 + c2 + c3;
+        if (tid < 256u) {
+            uint32_t e = loff[tid];
+#pragma unroll
+            for (uint32_t q = 0; q < NW; ++q) {
+                const uint32_t c = wc[q][tid];
+                wc[q][tid] = e;
+                e += c;
+            }
+            loff[tid] = e;
         }
         __syncthreads();
         if (valid) {
@@ -893,7 +903,7 @@ __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const u
     }
     // write the block's items digit run by digit run, so consecutive lanes write consecutive slots
     const uint32_t cnt = n > base ? min(kRadixItems, n - base) : 0u;
-    for (uint32_t j = tid; j < cnt; j += 256u) {
+    for (uint32_t j = tid; j < cnt; j += NT) {
         const uint32_t key = s_key[j];
         const uint32_t dg = (key >> shift) & 255u;
         const uint32_t pos = goff[dg] + (j - lbeg[dg]);
@@ -906,36 +916,39 @@ __device__ __forceinline__ void radix_scatter_block(const uint32_t *kin, const u
 // The flush's passes: the grid is capped (kRadixGrid) and each block walks chunks b, b + grid,
 // ... up to the pair count the device measured, so a flush with few pairs launches few idle
 // blocks whatever the arena's capacity.
-__global__ __launch_bounds__(256) void kb_radix_hist(BatchParams p, const uint32_t *keys, uint32_t shift,
-                                                     uint32_t *hist) {
+constexpr uint32_t kRadixThreads = 1024;  // the flush's radix blocks: 16 waves per 4096 pairs
+
+__global__ __launch_bounds__(kRadixThreads) void kb_radix_hist(BatchParams p, const uint32_t *keys, uint32_t shift,
+                                                               uint32_t *hist) {
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
     const uint32_t nb = p.ctr->radix_blocks;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
-        radix_hist_block(keys, p.ctr->n_pairs, nb, b, shift, hist);
+        radix_hist_block<kRadixThreads>(keys, p.ctr->n_pairs, nb, b, shift, hist);
         __syncthreads();  // the block's LDS counts are reused by its next chunk
     }
 }
 
-__global__ __launch_bounds__(256) void kb_radix_scatter(BatchParams p, const uint32_t *kin, const uint32_t *vin,
-                                                        uint32_t *kout, uint32_t *vout, uint32_t shift,
-                                                        const uint32_t *hist) {
+__global__ __launch_bounds__(kRadixThreads) void kb_radix_scatter(BatchParams p, const uint32_t *kin,
+                                                                  const uint32_t *vin, uint32_t *kout,
+                                                                  uint32_t *vout, uint32_t shift,
+                                                                  const uint32_t *hist) {
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES)) return;
     const uint32_t nb = p.ctr->radix_blocks;
     for (uint32_t b = blockIdx.x; b < nb; b += gridDim.x) {
-        radix_scatter_block(kin, vin, kout, vout, p.ctr->n_pairs, nb, b, shift, hist);
+        radix_scatter_block<kRadixThreads>(kin, vin, kout, vout, p.ctr->n_pairs, nb, b, shift, hist);
         __syncthreads();  // the block's LDS staging is reused by its next chunk
     }
 }
 
 // Host-sized passes (n known when launching): one block per kRadixItems keys.
 __global__ __launch_bounds__(256) void k_sort_hist(const uint32_t *keys, uint32_t n, uint32_t shift, uint32_t *hist) {
-    radix_hist_block(keys, n, gridDim.x, blockIdx.x, shift, hist);
+    radix_hist_block<256>(keys, n, gridDim.x, blockIdx.x, shift, hist);
 }
 
 __global__ __launch_bounds__(256) void k_sort_scatter(const uint32_t *kin, const uint32_t *vin, uint32_t *kout,
                                                       uint32_t *vout, uint32_t n, uint32_t shift,
                                                       const uint32_t *hist) {
-    radix_scatter_block(kin, vin, kout, vout, n, gridDim.x, blockIdx.x, shift, hist);
+    radix_scatter_block<256>(kin, vin, kout, vout, n, gridDim.x, blockIdx.x, shift, hist);
 }
 
 hipError_t sort_pairs(uint32_t *k0, uint32_t *v0, uint32_t *k1, uint32_t *v1, uint32_t n, uint32_t bits,
@@ -1621,12 +1634,12 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         uint32_t *kin = p.keys0, *vin = p.vals0, *kout = p.keys1, *vout = p.vals1;
         for (uint32_t pass = 0; pass < radix_passes; ++pass) {
             const uint32_t shift = 8u * pass;
-            hipLaunchKernelGGL(kb_radix_hist, dim3(rgrid), dim3(256), 0, s, p, (const uint32_t *)kin, shift,
+            hipLaunchKernelGGL(kb_radix_hist, dim3(rgrid), dim3(kRadixThreads), 0, s, p, (const uint32_t *)kin, shift,
                                sc.hist);
             if ((e = launch_scan<uint32_t>(sc.hist, sc.hist, (size_t)nb * 256u, &p.ctr->hist_n, sc, nullptr,
                                            s)) != hipSuccess)
                 return e;
-            hipLaunchKernelGGL(kb_radix_scatter, dim3(rgrid), dim3(256), 0, s, p, (const uint32_t *)kin,
+            hipLaunchKernelGGL(kb_radix_scatter, dim3(rgrid), dim3(kRadixThreads), 0, s, p, (const uint32_t *)kin,
                                (const uint32_t *)vin, kout, vout, shift, (const uint32_t *)sc.hist);
             std::swap(kin, kout);
             std::swap(vin, vout);
