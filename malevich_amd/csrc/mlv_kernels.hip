@@ -360,11 +360,26 @@ __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host
     const size_t i0 = (size_t)part * kScanChunk + (size_t)threadIdx.x * kScanItems;
     T v[kScanItems];
     T sum = T(0);
+    // a thread's 16 items are contiguous: 32-bit items go as four 16-byte loads where all are in range
+    const bool whole = sizeof(T) == 4 && i0 + kScanItems <= n;
+    if constexpr (sizeof(T) == 4) {
+        if (whole) {
 #pragma unroll
-    for (uint32_t k = 0; k < kScanItems; ++k) {
-        v[k] = (i0 + k < n) ? in[i0 + k] : T(0);
-        sum += v[k];
+            for (uint32_t q = 0; q < kScanItems / 4u; ++q) {
+                const uint4 u = ((const uint4 *)(in + i0))[q];
+                v[4 * q] = u.x;
+                v[4 * q + 1] = u.y;
+                v[4 * q + 2] = u.z;
+                v[4 * q + 3] = u.w;
+            }
+        }
     }
+    if (!whole) {
+#pragma unroll
+        for (uint32_t k = 0; k < kScanItems; ++k) v[k] = (i0 + k < n) ? in[i0 + k] : T(0);
+    }
+#pragma unroll
+    for (uint32_t k = 0; k < kScanItems; ++k) sum += v[k];
     T agg;
     const T ex = block_exclusive_scan<256, T>(sum, s_w, agg);
     if (threadIdx.x < 64u) {  // wave 0 publishes and looks back, 64 predecessors at a time
@@ -410,10 +425,29 @@ __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host
     }
     __syncthreads();
     T e = s_prefix + ex;
+    if constexpr (sizeof(T) == 4) {
+        if (whole) {
 #pragma unroll
-    for (uint32_t k = 0; k < kScanItems; ++k) {
-        if (i0 + k < n) out[i0 + k] = e;
-        e += v[k];
+            for (uint32_t q = 0; q < kScanItems / 4u; ++q) {
+                uint4 u;
+                u.x = e;
+                e += v[4 * q];
+                u.y = e;
+                e += v[4 * q + 1];
+                u.z = e;
+                e += v[4 * q + 2];
+                u.w = e;
+                e += v[4 * q + 3];
+                ((uint4 *)(out + i0))[q] = u;
+            }
+        }
+    }
+    if (!whole) {
+#pragma unroll
+        for (uint32_t k = 0; k < kScanItems; ++k) {
+            if (i0 + k < n) out[i0 + k] = e;
+            e += v[k];
+        }
     }
 }
 
