@@ -360,9 +360,18 @@ __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host
     const size_t i0 = (size_t)part * kScanChunk + (size_t)threadIdx.x * kScanItems;
     T v[kScanItems];
     T sum = T(0);
-    // a thread's 16 items are contiguous: 32-bit items go as four 16-byte loads where all are in range
-    const bool whole = sizeof(T) == 4 && i0 + kScanItems <= n;
-    if constexpr (sizeof(T) == 4) {
+    // a thread's 16 items are contiguous: they go as 16-byte loads where all are in range
+    const bool whole = i0 + kScanItems <= n;
+    if constexpr (sizeof(T) == 8) {
+        if (whole) {
+#pragma unroll
+            for (uint32_t q = 0; q < kScanItems / 2u; ++q) {
+                const ulonglong2 u = ((const ulonglong2 *)(in + i0))[q];
+                v[2 * q] = u.x;
+                v[2 * q + 1] = u.y;
+            }
+        }
+    } else {
         if (whole) {
 #pragma unroll
             for (uint32_t q = 0; q < kScanItems / 4u; ++q) {
@@ -425,7 +434,19 @@ __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host
     }
     __syncthreads();
     T e = s_prefix + ex;
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 8) {
+        if (whole) {
+#pragma unroll
+            for (uint32_t q = 0; q < kScanItems / 2u; ++q) {
+                ulonglong2 u;
+                u.x = e;
+                e += v[2 * q];
+                u.y = e;
+                e += v[2 * q + 1];
+                ((ulonglong2 *)(out + i0))[q] = u;
+            }
+        }
+    } else {
         if (whole) {
 #pragma unroll
             for (uint32_t q = 0; q < kScanItems / 4u; ++q) {
