@@ -555,7 +555,7 @@ def main(argv=None):
     warm(dev, lambda: frame(dev, res, sc), args.warmup)
     dev.autotune_begin()
     frame(dev, res, sc)
-    tuning = dev.autotune_end(repeats=3)
+    tuning = dev.autotune_end(repeats=5)
     print(f"[bench] autotuned: {tuning}", file=sys.stderr, flush=True)
     dev.reset_stats()
     dev.profile()  # drop anything accumulated

@@ -260,8 +260,9 @@ MLV_API int mlv_graph_destroy(mlv_context *ctx, mlv_graph *graph);
  * The per-context launch parameters of the tile and binning stages. mlv_autotune_begin /
  * mlv_autotune_end record a frame exactly as a graph capture does (only state, clears and
  * draws in between), then run it repeatedly under candidate parameters (coordinate descent:
- * tile staging, heavy-tile threshold, heavy-tile blocks, then pair blocks), timing each with HIP events
- * on the context stream, and keep the fastest. Results are the same under every candidate;
+ * tile staging, heavy-tile threshold, heavy-tile blocks, then pair blocks). Each candidate is
+ * timed as the fastest of `repeats` runs (HIP events on the context stream, after one warm-up
+ * run) and replaces the incumbent only if it is more than 1% faster. Results are the same under every candidate;
  * the targets hold the recorded frame afterwards. The bin-list capacity follows the largest
  * pair count measured (with a quarter of headroom) and is reported in pair_capacity. */
 typedef struct mlv_tuning {

@@ -1292,6 +1292,8 @@ MLV_API int mlv_set_tuning(mlv_context *ctx, const mlv_tuning *t) {
 MLV_API int mlv_autotune_begin(mlv_context *ctx) { return mlv_graph_capture_begin(ctx); }
 
 // Milliseconds per run of the recorded commands under the context's current parameters.
+// The fastest of `repeats` timed runs of a recorded frame (after a warm-up run): the minimum is
+// robust to the occasional slow run that an average would carry.
 static int time_recorded(mlv_context *ctx, const std::vector<Cmd> &rec, uint32_t repeats, double *ms) {
     RC_OR_RETURN(run_recorded(ctx, rec));  // warm-up under these parameters
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
@@ -1301,17 +1303,22 @@ static int time_recorded(mlv_context *ctx, const std::vector<Cmd> &rec, uint32_t
         (void)hipEventDestroy(a);
         return fail(ctx, MLV_ERROR_DEVICE, "hipEventCreate failed");
     }
-    int rc = hipEventRecord(a, ctx->stream) == hipSuccess ? MLV_OK : MLV_ERROR_DEVICE;
-    for (uint32_t r = 0; r < repeats && rc == MLV_OK; ++r) rc = run_recorded(ctx, rec);
-    float t = 0.0f;
-    if (rc == MLV_OK && (hipEventRecord(b, ctx->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
-                         hipEventElapsedTime(&t, a, b) != hipSuccess))
-        rc = fail(ctx, MLV_ERROR_DEVICE, "autotune timing failed");
+    int rc = MLV_OK;
+    float best = 0.0f;
+    for (uint32_t r = 0; r < repeats && rc == MLV_OK; ++r) {
+        float t = 0.0f;
+        if (hipEventRecord(a, ctx->stream) != hipSuccess) rc = MLV_ERROR_DEVICE;
+        if (rc == MLV_OK) rc = run_recorded(ctx, rec);
+        if (rc == MLV_OK && (hipEventRecord(b, ctx->stream) != hipSuccess || hipEventSynchronize(b) != hipSuccess ||
+                             hipEventElapsedTime(&t, a, b) != hipSuccess))
+            rc = fail(ctx, MLV_ERROR_DEVICE, "autotune timing failed");
+        if (rc == MLV_OK && (r == 0 || t < best)) best = t;
+    }
     (void)hipEventDestroy(a);
     (void)hipEventDestroy(b);
     RC_OR_RETURN(rc);
     RC_OR_RETURN(check_device_stats(ctx));
-    *ms = (double)t / repeats;
+    *ms = (double)best;
     return MLV_OK;
 }
 
@@ -1349,7 +1356,7 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
             *knob[k] = cand[k][i];
             double ms = 0.0;
             RC_OR_RETURN(time_recorded(ctx, rec, repeats, &ms));
-            if (ms < best_ms) {
+            if (ms < best_ms * 0.99) {  // a candidate must win by more than the runs' spread
                 best_ms = ms;
                 keep = cand[k][i];
             }
