@@ -27,6 +27,21 @@ __device__ __forceinline__ float dot4_seq(float ax, float ay, float az, float aw
     return r;
 }
 
+// A load through a pointer that the kernel read from memory, as a global rather than a flat
+// access: a flat load also counts against the scalar-memory counter, so waits for it and for
+// scalar loads become one wait. A uniform address with no store of the kernel before it becomes
+// a scalar load.
+template <class T>
+__device__ __forceinline__ T load_global(const T *q) {
+    static_assert(sizeof(T) == 4 || sizeof(T) == 8 || sizeof(T) == 16, "a load of 4, 8 or 16 bytes");
+    if constexpr (sizeof(T) == 4) {
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) uint32_t *)q);
+    } else {
+        typedef uint32_t W __attribute__((ext_vector_type(sizeof(T) / 4)));
+        return __builtin_bit_cast(T, *(const __attribute__((address_space(1))) W *)q);
+    }
+}
+
 // v4f256_dot against a matrix row (m4x4f32_mul_v4f256, math.h:142-146,174-177)
 __device__ __forceinline__ float dot4_pair(const float *row, float x, float y, float z, float w) {
     float xy = row[0] * x + row[1] * y;
@@ -103,20 +118,46 @@ __device__ __forceinline__ uint32_t texel_index(const TexDesc &t, int32_t s, int
     return (uint32_t)tt * t.width + (uint32_t)s;
 }
 
-// get_texel_u_x8 / get_texel_f_x8 (csc.h:30-53) + decode_u32_as_color_x8 (math.h:336-344)
-__device__ __forceinline__ float4 fetch(const TexDesc &t, int32_t s, int32_t tt) {
-    uint32_t i = texel_index(t, s, tt);
+// decode_u32_as_color_x8 (math.h:336-344)
+__device__ __forceinline__ float4 decode_rgba8(uint32_t c) {
+    const float k = (float)(1.0 / 255.0);
+    return make_float4((float)(c & 0xFFu) * k, (float)((c >> 8) & 0xFFu) * k, (float)((c >> 16) & 0xFFu) * k,
+                       (float)((c >> 24) & 0xFFu) * k);
+}
+
+__device__ __forceinline__ float4 decode_rgba16f(ushort4 h) {
+    return make_float4(__half2float(__ushort_as_half(h.x)), __half2float(__ushort_as_half(h.y)),
+                       __half2float(__ushort_as_half(h.z)), __half2float(__ushort_as_half(h.w)));
+}
+
+// get_texel_u_x8 / get_texel_f_x8 (csc.h:30-53) for the four texels of a bilinear footprint:
+// one branch on the format, then the four loads in flight together.
+__device__ __forceinline__ void fetch4(const TexDesc &t, int32_t s0, int32_t t0, int32_t s1, int32_t t1,
+                                       float4 &f00, float4 &f10, float4 &f01, float4 &f11) {
+    const uint32_t i00 = texel_index(t, s0, t0), i10 = texel_index(t, s1, t0);
+    const uint32_t i01 = texel_index(t, s0, t1), i11 = texel_index(t, s1, t1);
     if (t.format == FMT_RGBA8) {
-        uint32_t c = ((const uint32_t *)t.data)[i];
-        const float k = (float)(1.0 / 255.0);
-        return make_float4((float)(c & 0xFFu) * k, (float)((c >> 8) & 0xFFu) * k, (float)((c >> 16) & 0xFFu) * k,
-                           (float)((c >> 24) & 0xFFu) * k);
+        const uint32_t *q = (const uint32_t *)t.data;
+        const uint32_t c00 = load_global(q + i00), c10 = load_global(q + i10);
+        const uint32_t c01 = load_global(q + i01), c11 = load_global(q + i11);
+        f00 = decode_rgba8(c00);
+        f10 = decode_rgba8(c10);
+        f01 = decode_rgba8(c01);
+        f11 = decode_rgba8(c11);
     } else if (t.format == FMT_RGBA32F) {
-        return ((const float4 *)t.data)[i];
+        const float4 *q = (const float4 *)t.data;
+        f00 = load_global(q + i00);
+        f10 = load_global(q + i10);
+        f01 = load_global(q + i01);
+        f11 = load_global(q + i11);
     } else {
-        const ushort4 h = ((const ushort4 *)t.data)[i];
-        return make_float4(__half2float(__ushort_as_half(h.x)), __half2float(__ushort_as_half(h.y)),
-                           __half2float(__ushort_as_half(h.z)), __half2float(__ushort_as_half(h.w)));
+        const ushort4 *q = (const ushort4 *)t.data;
+        const ushort4 h00 = load_global(q + i00), h10 = load_global(q + i10);
+        const ushort4 h01 = load_global(q + i01), h11 = load_global(q + i11);
+        f00 = decode_rgba16f(h00);
+        f10 = decode_rgba16f(h10);
+        f01 = decode_rgba16f(h01);
+        f11 = decode_rgba16f(h11);
     }
 }
 
@@ -131,8 +172,8 @@ __device__ __forceinline__ float4 bilinear(const TexDesc &t, float u, float v) {
     float fs = sf - (float)s;
     float ft = tf - (float)tt;
     int32_t s1 = (int32_t)((uint32_t)s + 1u), t1 = (int32_t)((uint32_t)tt + 1u);
-    float4 t00 = fetch(t, s, tt), t10 = fetch(t, s1, tt);
-    float4 t01 = fetch(t, s, t1), t11 = fetch(t, s1, t1);
+    float4 t00, t10, t01, t11;
+    fetch4(t, s, tt, s1, t1, t00, t10, t01, t11);
     float4 a = make_float4(lerp_ref(t00.x, t10.x, fs), lerp_ref(t00.y, t10.y, fs), lerp_ref(t00.z, t10.z, fs),
                            lerp_ref(t00.w, t10.w, fs));
     float4 b = make_float4(lerp_ref(t01.x, t11.x, fs), lerp_ref(t01.y, t11.y, fs), lerp_ref(t01.z, t11.z, fs),
@@ -165,13 +206,22 @@ __device__ __forceinline__ float4 latlon(const TexDesc &t, float dx, float dy, f
 
 // ------------------------------------------------------------- shaders
 
+// Floats of constant buffer slot 0 that each vertex shader reads: the world-view-projection
+// rows, or (fullscreen) the view-from-clip and world-from-view rows at floats 16..43.
+template <int VS>
+struct VsConstants {
+    static constexpr int kFloats = VS == VS_FULLSCREEN ? 44 : (VS == VS_PASSTHROUGH ? 0 : 16);
+    float cb[kFloats > 0 ? kFloats : 1];
+    TexDesc tex;
+};
+
 // Vertex shaders (basic_vs.c:22-33, vertex_lighting_vs.c:22-39, passthrough_vs.c:16-25,
 // fullscreen_vs.c:22-39). Writes the three output registers.
 template <int VS>
-__device__ __forceinline__ void vertex_shader(const DrawRec &p, const float *in, float4 &r0, float4 &r1,
+__device__ __forceinline__ void vertex_shader(const VsConstants<VS> &k, const float *in, float4 &r0, float4 &r1,
                                               float4 &r2) {
     if constexpr (VS == VS_BASIC || VS == VS_VERTEX_LIGHTING) {
-        const float *cfw = p.cb;
+        const float *cfw = k.cb;
         float x = in[0], y = in[1], z = in[2];
         r0 = make_float4(dot4_pair(cfw, x, y, z, 1.0f), dot4_pair(cfw + 4, x, y, z, 1.0f),
                          dot4_pair(cfw + 8, x, y, z, 1.0f), dot4_pair(cfw + 12, x, y, z, 1.0f));
@@ -180,7 +230,7 @@ __device__ __forceinline__ void vertex_shader(const DrawRec &p, const float *in,
         if constexpr (VS == VS_BASIC) {
             r1 = make_float4(nx, ny, nz, in[6]);
         } else {
-            float4 c = latlon(p.vs_tex, nx, ny, nz);
+            float4 c = latlon(k.tex, nx, ny, nz);
             r1 = make_float4(tonemap(c.x), tonemap(c.y), tonemap(c.z), in[6]);
         }
         r2 = make_float4(in[7], 0.0f, 0.0f, 0.0f);
@@ -190,7 +240,7 @@ __device__ __forceinline__ void vertex_shader(const DrawRec &p, const float *in,
         r2 = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
     } else {  // VS_FULLSCREEN
         float px = in[0] * 2.0f - 1.0f, py = in[1] * 2.0f - 1.0f, pz = in[2], pw = in[3];
-        const float *vfc = p.cb + 16, *wfv = p.cb + 32;
+        const float *vfc = k.cb + 16, *wfv = k.cb + 32;
         float d0 = dot4_pair(vfc, px, py, pz, pw), d1 = dot4_pair(vfc + 4, px, py, pz, pw);
         float d2 = dot4_pair(vfc + 8, px, py, pz, pw);
         r0 = make_float4(px, py, pz, pw);

@@ -97,15 +97,17 @@ __device__ __forceinline__ uint32_t upper_index(uint32_t n, uint32_t x, F key) {
     return lo;
 }
 
+// The three indices are loaded together and checked afterwards, so one memory round trip
+// brings all three.
 __device__ __forceinline__ bool load_tri_indices(const DrawRec &d, uint32_t t, uint32_t vi[3]) {
     const uint32_t *ib = d.ib + 3u * t;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        long long v = (long long)ib[k] + (long long)d.base_vertex;
-        if (v < 0 || v >= (long long)d.vertex_count) return false;
-        vi[k] = (uint32_t)v;
-    }
-    return true;
+    const uint32_t r0 = load_global(ib), r1 = load_global(ib + 1), r2 = load_global(ib + 2);
+    const long long bv = (long long)d.base_vertex, nv = (long long)d.vertex_count;
+    const long long v0 = (long long)r0 + bv, v1 = (long long)r1 + bv, v2 = (long long)r2 + bv;
+    vi[0] = (uint32_t)v0;
+    vi[1] = (uint32_t)v1;
+    vi[2] = (uint32_t)v2;
+    return v0 >= 0 && v0 < nv && v1 >= 0 && v1 < nv && v2 >= 0 && v2 < nv;
 }
 
 // ------------------------------------------------------------ the clipper
@@ -500,19 +502,33 @@ size_t scan_wide_offset(size_t n_max) {
 
 // ------------------------------------------------------------ stage 0: vertex
 
+// A draw's inputs to the vertex stage, read by kb_vertex before its lanes diverge and before its
+// first store, so that every field and constant is a scalar load.
+struct VsDraw {
+    const uint8_t *vb;
+    uint32_t stride, vbase, vertex_count;
+    float cb[44];  // constant buffer slot 0, floats 0..43 (those the draw's shader reads)
+    TexDesc tex;
+};
+
 template <int VS>
-__device__ __forceinline__ void run_vs(const BatchParams &p, const DrawRec &d, uint32_t i) {
+__device__ __forceinline__ void run_vs(const BatchParams &p, const VsDraw &u, uint32_t i) {
+    if (i >= u.vertex_count) return;
+    VsConstants<VS> k;
+#pragma unroll
+    for (int j = 0; j < VsConstants<VS>::kFloats; ++j) k.cb[j] = u.cb[j];
+    k.tex = u.tex;
     float in[8];
-    const uint8_t *src = d.vb + (size_t)i * d.stride;
-    if ((((uintptr_t)d.vb) & 15u) == 0u && (d.stride & 15u) == 0u) {
-        float4 a = ((const float4 *)src)[0], b = ((const float4 *)src)[1];
+    const uint8_t *src = u.vb + (size_t)i * u.stride;
+    if ((((uintptr_t)u.vb) & 15u) == 0u && (u.stride & 15u) == 0u) {
+        float4 a = load_global((const float4 *)src), b = load_global((const float4 *)src + 1);
         in[0] = a.x; in[1] = a.y; in[2] = a.z; in[3] = a.w; in[4] = b.x; in[5] = b.y; in[6] = b.z; in[7] = b.w;
     } else {
-        for (int k = 0; k < 8; ++k) in[k] = ((const float *)src)[k];
+        for (int k2 = 0; k2 < 8; ++k2) in[k2] = load_global((const float *)src + k2);
     }
     float4 r0, r1, r2;
-    vertex_shader<VS>(d, in, r0, r1, r2);
-    const uint32_t slot = d.vbase + i;
+    vertex_shader<VS>(k, in, r0, r1, r2);
+    const uint32_t slot = u.vbase + i;
     p.vpos[slot] = r0;
     p.vattr[2u * slot] = r1;
     p.vattr[2u * slot + 1u] = r2;
@@ -532,22 +548,40 @@ __device__ __forceinline__ void reset_flush(const BatchParams &p) {
         p.scan_status[i] = 0u;
 }
 
-// Stage 0 in one launch: every block takes a share of the flush's reset, and blocks below
-// n_vblocks one lane per vertex slot of every draw.
+// Stage 0 in one launch: blocks below n_vblocks take one lane per vertex slot of every draw,
+// and every block then takes a share of the flush's reset. The reset's stores come last: a
+// load that no store of the kernel can precede is read through the scalar cache (the draw
+// search, the draw record and the constant buffer), and after a store it would be a vector
+// load on the wave's critical path.
 __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
-    reset_flush(p);
     const uint32_t b = blockIdx.x;
-    if (b >= p.n_vblocks) return;
-    const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_v0; });
-    const DrawRec &d = p.draws[di];
-    const uint32_t i = (b - d.blk_v0) * 256u + threadIdx.x;
-    if (i >= d.vertex_count) return;
-    switch (d.vs) {
-    case VS_BASIC: run_vs<VS_BASIC>(p, d, i); break;
-    case VS_VERTEX_LIGHTING: run_vs<VS_VERTEX_LIGHTING>(p, d, i); break;
-    case VS_PASSTHROUGH: run_vs<VS_PASSTHROUGH>(p, d, i); break;
-    default: run_vs<VS_FULLSCREEN>(p, d, i); break;
+    if (b < p.n_vblocks) {
+        const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_v0; });
+        const DrawRec &d = p.draws[di];
+        const uint32_t i = (b - d.blk_v0) * 256u + threadIdx.x;
+        const int vs = d.vs;
+        VsDraw u;
+        u.vb = d.vb;
+        u.stride = d.stride;
+        u.vbase = d.vbase;
+        u.vertex_count = d.vertex_count;
+        u.tex = d.vs_tex;
+        const float *cb = d.cb;  // unbound (null) for the passthrough shader
+        if (vs == VS_FULLSCREEN) {
+#pragma unroll
+            for (int j = 0; j < 44; ++j) u.cb[j] = load_global(cb + j);
+        } else if (vs != VS_PASSTHROUGH) {
+#pragma unroll
+            for (int j = 0; j < 16; ++j) u.cb[j] = load_global(cb + j);
+        }
+        switch (vs) {
+        case VS_BASIC: run_vs<VS_BASIC>(p, u, i); break;
+        case VS_VERTEX_LIGHTING: run_vs<VS_VERTEX_LIGHTING>(p, u, i); break;
+        case VS_PASSTHROUGH: run_vs<VS_PASSTHROUGH>(p, u, i); break;
+        default: run_vs<VS_FULLSCREEN>(p, u, i); break;
+        }
     }
+    reset_flush(p);
 }
 
 // ------------------------------------------------------------ stage 1: setup
@@ -561,23 +595,28 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
     const ViewRec &v = p.views[d.view];
     const uint32_t t = (b - d.blk_t0) * 256u + threadIdx.x;
     const uint32_t slot = b * 256u + threadIdx.x;
+    // every load comes before the first atomic or store (see kb_vertex)
     uint32_t packed = 0u;
+    bool oob = false;
+    int cls = 0;
     if (t < d.tri_count) {
         uint32_t vi[3];
         if (!load_tri_indices(d, t, vi)) {
-            atomicAdd(&p.stats->oob, 1ull);
+            oob = true;
         } else {
             float4 a = p.vpos[d.vbase + vi[0]], bb = p.vpos[d.vbase + vi[1]], c = p.vpos[d.vbase + vi[2]];
-            int cls = classify(a, bb, c);
+            cls = classify(a, bb, c);
             if (cls == 1) {
                 TriVis tv;
                 float rw[3];
                 packed = setup_triangle(d, v, a, bb, c, tv, rw) ? 1u : 0u;
-            } else if (cls == 2) {
-                uint32_t k = atomicAdd(&p.ctr->n_clip_list, 1u);
-                p.clip_list[k] = make_uint2(slot, di);
             }
         }
+    }
+    if (oob) atomicAdd(&p.stats->oob, 1ull);
+    if (cls == 2) {
+        uint32_t k = atomicAdd(&p.ctr->n_clip_list, 1u);
+        p.clip_list[k] = make_uint2(slot, di);
     }
     p.cnt[slot] = packed;
     uint32_t s = wave_sum(packed);
@@ -618,12 +657,11 @@ __global__ __launch_bounds__(256) void kb_clip_count(BatchParams p) {
 }
 
 
-__device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t idx, uint32_t draw, TriVis &tv,
-                                               const float rw[3], uint32_t r0, uint32_t r1, uint32_t r2,
-                                               uint32_t prim, uint32_t sub) {
+__device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t idx, uint32_t draw, const ViewRec &v,
+                                               TriVis &tv, const float rw[3], uint32_t r0, uint32_t r1,
+                                               uint32_t r2, uint32_t prim, uint32_t sub) {
     if (idx >= p.tri_cap) return;
     // every load before the first store, which the compiler cannot move loads across
-    const ViewRec &v = p.views[p.draws[draw].view];
     const uint32_t tile_base = v.tile_base, tiles_x = (uint32_t)v.tiles_x;
     tv.draw = draw;
     p.tvis[idx] = tv;
@@ -642,9 +680,23 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     const uint32_t slot = b * 256u + threadIdx.x;
     const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
     const DrawRec &d = p.draws[di];
+    const ViewRec v = p.views[d.view];  // a copy: read here, before the first store
     const uint32_t t = slot - d.tbase;
     const unsigned long long bo = p.blockoff[b];
     const uint32_t packed = p.cnt[slot];
+    // the triangle is set up before its place is known, so its loads and arithmetic overlap the
+    // block scan; every load comes before the first store (see kb_vertex)
+    TriVis tv;
+    float rw[3];
+    uint32_t s0 = 0u, s1 = 0u, s2 = 0u;
+    uint32_t vi[3] = {0u, 0u, 0u};
+    if (t < d.tri_count) load_tri_indices(d, t, vi);  // in flight with the count's load
+    if (packed == 1u) {
+        s0 = d.vbase + vi[0];
+        s1 = d.vbase + vi[1];
+        s2 = d.vbase + vi[2];
+        setup_triangle(d, v, p.vpos[s0], p.vpos[s1], p.vpos[s2], tv, rw);
+    }
     if (threadIdx.x == 0u) {
         // a draw's first assembled index; empty draws start where the next one does
         if (b == d.blk_t0)
@@ -667,26 +719,13 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
             atomicMax(&p.stats->max_clip_list, (unsigned long long)p.ctr->n_clip_list);
         }
     }
-    // the triangle is set up before its place is known, so its loads and arithmetic overlap the
-    // block scan
-    TriVis tv;
-    float rw[3];
-    uint32_t s0 = 0u, s1 = 0u, s2 = 0u;
-    if (packed == 1u) {
-        uint32_t vi[3];
-        load_tri_indices(d, t, vi);
-        s0 = d.vbase + vi[0];
-        s1 = d.vbase + vi[1];
-        s2 = d.vbase + vi[2];
-        setup_triangle(d, p.views[d.view], p.vpos[s0], p.vpos[s1], p.vpos[s2], tv, rw);
-    }
     uint32_t total;
     // within one block the packed counts (<= 14 triangles, <= 16 vertices each) cannot carry
     const uint32_t ex = block_exclusive_scan<256, uint32_t>(packed, s_w, total);
     if (packed == 0u) return;
     const uint32_t off_tri = (uint32_t)(bo & 0xFFFFFFFFull) + (ex & 0xFFFFu);
     const uint32_t off_vtx = (uint32_t)(bo >> 32) + (ex >> 16);
-    if (packed == 1u) write_triangle(p, off_tri, di, tv, rw, s0, s1, s2, t, 0u);
+    if (packed == 1u) write_triangle(p, off_tri, di, v, tv, rw, s0, s1, s2, t, 0u);
     else p.clip_off[slot] = make_uint2(off_tri, off_vtx);
 }
 
@@ -721,7 +760,7 @@ __global__ __launch_bounds__(256) void kb_clip_emit(BatchParams p) {
         float rw[3];
         if (g.li >= 1u && (int)g.li < nv - 1) live = setup_triangle(d, v, q0, q1, q2, tv, rw);
         const uint32_t j = (uint32_t)__popcll(__ballot(live) & g.gmask & g.below());
-        if (live) write_triangle(p, off.x + j, di, tv, rw, row0, row, row + 1u, t, g.li - 1u);
+        if (live) write_triangle(p, off.x + j, di, v, tv, rw, row0, row, row + 1u, t, g.li - 1u);
     }
 }
 
