@@ -818,13 +818,15 @@ template <bool EMIT>
 __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
     __shared__ uint32_t s_big[256];
     __shared__ uint32_t s_nbig;
+    // the counters are read before block 0's stores (scalar loads)
+    const uint32_t n = min(p.ctr->n_assembled, p.tri_cap);
     if (EMIT) {
         const uint32_t total = p.ctr->n_pairs;
+        const bool tri_over = (p.ctr->overflow & OVF_TRIANGLES) != 0u;
         const bool over = total > p.pair_cap;  // past the arena: the tile stage walks every triangle
         if (blockIdx.x == 0) pair_totals(p, total, over);
-        if (over || (p.ctr->overflow & OVF_TRIANGLES)) return;
+        if (over || tri_over) return;
     }
-    const uint32_t n = min(p.ctr->n_assembled, p.tri_cap);
     unsigned long long pairs8_acc = 0ull;
     for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
         if (threadIdx.x == 0) s_nbig = 0u;
@@ -832,6 +834,7 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
         const uint32_t i = base + threadIdx.x;
         if (i < n) {
             const uint4 tb = p.tbin[i];
+            const uint32_t off = EMIT ? p.pair_off[i] : 0u;  // in flight with the rect
             int32_t tx0, ty0, tw;
             uint32_t p8;
             const uint32_t nt = tile_rect(tb.x, tb.y, tx0, ty0, tw, p8);
@@ -842,7 +845,6 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
             if (!EMIT) {
                 // counting pass: nothing more per triangle
             } else if (nt <= (uint32_t)kBigRect) {
-                const uint32_t off = p.pair_off[i];
                 for (uint32_t k = 0; k < nt; ++k) {
                     const uint32_t tile = tb.z + (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * tb.w +
                                           (uint32_t)(tx0 + (int32_t)(k % (uint32_t)tw));
