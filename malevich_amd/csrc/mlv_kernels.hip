@@ -346,20 +346,23 @@ struct ScanPub<unsigned long long> {
     }
 };
 
-template <class T>
-__global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, T *total,
-                                              uint8_t *st, uint32_t epoch, uint32_t nparts, uint32_t *err) {
+// NT threads per partition of NT * 16 elements: 256 in general, 1024 when one partition holds
+// the whole input (then nothing looks back).
+template <class T, uint32_t NT>
+__global__ __launch_bounds__(NT) void k_scan(const T *in, T *out, size_t n_host, const uint32_t *n_dev, T *total,
+                                             uint8_t *st, uint32_t epoch, uint32_t nparts, uint32_t *err) {
+    constexpr uint32_t kChunk = NT * kScanItems;
     __shared__ T s_prefix;
-    __shared__ T s_w[5];
+    __shared__ T s_w[NT / 64u + 1u];
     const uint32_t part = blockIdx.x;
     const size_t n = n_dev ? min(n_host, (size_t)*n_dev) : n_host;
-    const uint32_t nd = (uint32_t)((n + kScanChunk - 1) / kScanChunk);  // partitions holding data
+    const uint32_t nd = (uint32_t)((n + kChunk - 1) / kChunk);  // partitions holding data
     if (part >= nd) {
         // nothing to scan here, and no partition with data looks back this far
         if (n == 0 && part == 0u && threadIdx.x == 0 && total) *total = T(0);
         return;
     }
-    const size_t i0 = (size_t)part * kScanChunk + (size_t)threadIdx.x * kScanItems;
+    const size_t i0 = (size_t)part * kChunk + (size_t)threadIdx.x * kScanItems;
     T v[kScanItems];
     T sum = T(0);
     // a thread's 16 items are contiguous: they go as 16-byte loads where all are in range
@@ -392,13 +395,15 @@ __global__ __launch_bounds__(256) void k_scan(const T *in, T *out, size_t n_host
 #pragma unroll
     for (uint32_t k = 0; k < kScanItems; ++k) sum += v[k];
     T agg;
-    const T ex = block_exclusive_scan<256, T>(sum, s_w, agg);
+    const T ex = block_exclusive_scan<NT, T>(sum, s_w, agg);
     if (threadIdx.x < 64u) {  // wave 0 publishes and looks back, 64 predecessors at a time
         const uint32_t lane = threadIdx.x;
         const uint32_t tag = epoch << 2;
         T prefix = T(0);
         if (part == 0u) {
-            if (lane == 0u) ScanPub<T>::publish(st, 0u, tag | kFlagPrefix, agg);
+            // with one partition nobody looks back (and a 64-bit publish is a release, which
+            // writes the L2 back)
+            if (lane == 0u && nd > 1u) ScanPub<T>::publish(st, 0u, tag | kFlagPrefix, agg);
         } else {
             if (lane == 0u) ScanPub<T>::publish(st, part, tag | kFlagAgg, agg);
             int64_t j = (int64_t)part - 1;
@@ -479,13 +484,18 @@ static hipError_t launch_scan(const T *in, T *out, size_t n_host, const uint32_t
                               T *total, hipStream_t s) {
     if (n_host == 0) return hipSuccess;
     const uint32_t nparts = (uint32_t)((n_host + kScanChunk - 1) / kScanChunk);
+    constexpr uint32_t kOneBlock = 1024u * kScanItems;  // one 1024-thread partition: no look-back
     // 64-bit statuses keep their own region: a 32-bit scan reading one of their value words
     // could take its upper half for a status of its own epoch
     const size_t off = sizeof(T) == 4 ? kScanHeader : sc.wide_off;
     if (off == 0) return hipErrorInvalidValue;
     const uint32_t epoch = (++*sc.epoch) & 0x3FFFFFFFu;
-    hipLaunchKernelGGL(k_scan<T>, dim3(nparts), dim3(256), 0, s, in, out, n_host, n_dev, total,
-                       (uint8_t *)sc.part + off, epoch, nparts, sc.err);
+    if (n_host <= kOneBlock)
+        hipLaunchKernelGGL((k_scan<T, 1024>), dim3(1), dim3(1024), 0, s, in, out, n_host, n_dev, total,
+                           (uint8_t *)sc.part + off, epoch, 1u, sc.err);
+    else
+        hipLaunchKernelGGL((k_scan<T, 256>), dim3(nparts), dim3(256), 0, s, in, out, n_host, n_dev, total,
+                           (uint8_t *)sc.part + off, epoch, nparts, sc.err);
     return hipGetLastError();
 }
 
