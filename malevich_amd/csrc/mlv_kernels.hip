@@ -1238,6 +1238,14 @@ __device__ __forceinline__ void bin_cover(const CoverRec &r, BinState &st, bool 
     if (__ballot(upd)) st.stale = 1u;
 }
 
+// The record index of the next survivor (the lowest set bit of S, which it clears); lane j of
+// k holds entry j's record index.
+__device__ __forceinline__ uint32_t next_survivor(uint32_t k, uint64_t &S) {
+    const uint32_t jj = (uint32_t)__builtin_ctzll(S);
+    S &= S - 1ull;
+    return (uint32_t)__builtin_amdgcn_readlane((int)k, (int)jj);
+}
+
 // Largest value an edge function takes over the bin's pixels (x0..x0+7, y0..y0+7), exact when
 // the draw guarantees no i32 wrap at any pixel (DrawRec::depth_cull).
 __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c, int32_t x0, int32_t y0) {
@@ -1286,21 +1294,19 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, con
                    edge_max(L.a16[2][k], L.b16[2][k], L.c[2][k], x0, y0) >= 0;
         }
         // survivors in order; lane jj already holds entry i0 + jj's record index, and the next
-        // survivor's record is read from LDS while the current one is rasterised
+        // survivor's record is read from LDS while the current one is rasterised. Two records
+        // alternate (the loop is unrolled by two), so the prefetch needs no register copies.
         uint64_t S = __ballot(live);
         if (S) {
-            uint32_t jj = (uint32_t)__builtin_ctzll(S);
-            S &= S - 1ull;
-            CoverRec r = load_cover(L, (uint32_t)__builtin_amdgcn_readlane((int)k, (int)jj));
+            CoverRec ra = load_cover(L, next_survivor(k, S)), rb = ra;
             for (;;) {
-                const CoverRec cur = r;
-                const bool more = S != 0ull;
-                if (more) {
-                    jj = (uint32_t)__builtin_ctzll(S);
-                    S &= S - 1ull;
-                    r = load_cover(L, (uint32_t)__builtin_amdgcn_readlane((int)k, (int)jj));
-                }
-                bin_cover<GE>(cur, st, inside, px, py);
+                bool more = S != 0ull;
+                if (more) rb = load_cover(L, next_survivor(k, S));
+                bin_cover<GE>(ra, st, inside, px, py);
+                if (!more) break;
+                more = S != 0ull;
+                if (more) ra = load_cover(L, next_survivor(k, S));
+                bin_cover<GE>(rb, st, inside, px, py);
                 if (!more) break;
             }
         }
