@@ -1143,7 +1143,8 @@ struct BinState {
     int32_t cur_draw;   // draw of the last triangle seen (uniform)
     uint32_t n_draws;   // draws seen (uniform)
     int32_t depth_cull; // the current draw allows the conservative cull (uniform)
-    uint32_t stale;     // updates since wext was last recomputed (uniform)
+    uint64_t stale;     // lanes whose depth changed since wext was last recomputed (uniform)
+    uint64_t inside_m;  // lanes whose pixel is inside the view (uniform)
 };
 
 // Up to N staged records of a tile's list in LDS, one array per field: the triangle-parallel
@@ -1216,15 +1217,27 @@ __device__ __forceinline__ CoverRec load_cover(const Lds &L, uint32_t k) {
     return r;
 }
 
+// c + a * x (mod 2^32), a and x taken as signed 24-bit values (__mul24): one v_mad_i32_i24, where
+// the compiler would emit a multiply per term and a three-input add (three instructions per edge
+// function instead of two).
+__device__ __forceinline__ uint32_t mad24(int32_t a, int32_t x, uint32_t c) {
+    uint32_t r;
+    asm("v_mad_i32_i24 %0, %1, %2, %3" : "=v"(r) : "v"(a), "v"(x), "v"(c));
+    return r;
+}
+
 // Corner-sampled coverage of one triangle over the bin's 64 pixels and the early-Z key update
 // (main.c:1015-1028, 1083-1168); the triangle has passed hierarchical-Z.
 template <bool GE>
 __device__ __forceinline__ void bin_cover(const CoverRec &r, BinState &st, bool inside, int32_t px, int32_t py) {
-    const uint32_t e0 = (uint32_t)r.c[0] + (uint32_t)__mul24(r.a16[0], px) + (uint32_t)__mul24(r.b16[0], py);
-    const uint32_t e1 = (uint32_t)r.c[1] + (uint32_t)__mul24(r.a16[1], px) + (uint32_t)__mul24(r.b16[1], py);
-    const uint32_t e2 = (uint32_t)r.c[2] + (uint32_t)__mul24(r.a16[2], px) + (uint32_t)__mul24(r.b16[2], py);
-    const bool cov = inside && (int32_t)(e0 | e1 | e2) > 0;  // main.c:1028
-    if (__ballot(cov) == 0ull) return;
+    const uint32_t e0 = mad24(r.a16[0], px, mad24(r.b16[0], py, (uint32_t)r.c[0]));
+    const uint32_t e1 = mad24(r.a16[1], px, mad24(r.b16[1], py, (uint32_t)r.c[1]));
+    const uint32_t e2 = mad24(r.a16[2], px, mad24(r.b16[2], py, (uint32_t)r.c[2]));
+    const int32_t eo = (int32_t)(e0 | e1 | e2);
+    const bool cov = inside && eo > 0;  // main.c:1028
+    // the covered lanes straight from the compares' lane masks (38: signed greater, 33: not
+    // equal); a ballot of cov would first turn the mask into a 0/1 register and compare again
+    if ((__builtin_amdgcn_sicmp(eo, 0, 38) & st.inside_m) == 0ull) return;
     const float b1 = (float)((int32_t)e1 >> 8) * r.ooa;  // main.c:1089-1102
     const float b2 = (float)((int32_t)e2 >> 8) * r.ooa;
     float z = r.z0 + r.dz1 * b1;  // register 0 is affine (main.c:1118-1154)
@@ -1233,9 +1246,12 @@ __device__ __forceinline__ void bin_cover(const CoverRec &r, BinState &st, bool 
     const bool nearer = GE ? z > st.d : z < st.d;
     const bool tie = z == st.d;
     const bool upd = cov & (nearer | (tie & (r.id > st.best)));
+    const float d_old = st.d;
     st.d = upd ? z : st.d;
     st.best = upd ? r.id : st.best;
-    if (__ballot(upd)) st.stale = 1u;
+    // the cull bound follows the depths, so only a changed depth makes it stale (14: unordered
+    // or not equal, a compare's lane mask)
+    st.stale |= __builtin_amdgcn_fcmpf(st.d, d_old, 14);
 }
 
 // The record index of the next survivor (the lowest set bit of S, which it clears); lane j of
@@ -1250,7 +1266,7 @@ __device__ __forceinline__ uint32_t next_survivor(uint32_t k, uint64_t &S) {
 // the draw guarantees no i32 wrap at any pixel (DrawRec::depth_cull).
 __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c, int32_t x0, int32_t y0) {
     const int32_t x = a16 > 0 ? x0 + 7 : x0, y = b16 > 0 ? y0 + 7 : y0;
-    return (int32_t)((uint32_t)c + (uint32_t)__mul24(a16, x) + (uint32_t)__mul24(b16, y));
+    return (int32_t)mad24(a16, x, mad24(b16, y, (uint32_t)c));
 }
 
 // Walk entries list[first + i * stride], i < n, of this wave's bin in order. Triangle-parallel
@@ -1274,7 +1290,7 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, con
         const uint32_t run = (~same == 0ull) ? 64u : (uint32_t)__builtin_ctzll(~same);
         if (st.depth_cull && st.stale) {
             st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
-            st.stale = 0u;
+            st.stale = 0ull;
         }
         bool live = lane < run;
         // a pixel corner of the bin inside the triangle's bounding box
@@ -1358,7 +1374,8 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
     st.n_draws = 0u;
     st.depth_cull = 0;
     st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
-    st.stale = 0u;
+    st.stale = 0ull;
+    st.inside_m = __ballot(inside);
 
     for (uint32_t base = 0; base < lcount; base += N) {
         const uint32_t n = min(N, lcount - base);
@@ -1481,7 +1498,8 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
     st.n_draws = 0u;
     st.depth_cull = 0;
     st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
-    st.stale = 0u;
+    st.stale = 0ull;
+    st.inside_m = __ballot(inside);
     bool seen = false;
 
     for (uint32_t base = 0; base < lcount; base += 256u) {
