@@ -597,42 +597,88 @@ __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
 // ------------------------------------------------------------ stage 1: setup
 
 
+// Two 256-slot blocks per workgroup: each lane takes one triangle of each, and both triangles'
+// loads are issued before either is set up, so the two dependent round trips (indices, then
+// positions) are paid once for two triangles.
+constexpr uint32_t kCountPer = 2;
+
 __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
-    __shared__ uint32_t s_w[4];
-    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
-    const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
-    const DrawRec &d = p.draws[di];
-    const ViewRec &v = p.views[d.view];
-    const uint32_t t = (b - d.blk_t0) * 256u + threadIdx.x;
-    const uint32_t slot = b * 256u + threadIdx.x;
+    __shared__ uint32_t s_w[kCountPer][4];
+    const uint32_t hb = xcd_contiguous(blockIdx.x, gridDim.x);
+    uint32_t di[kCountPer], slot[kCountPer], vi[kCountPer][3];
+    bool inb[kCountPer], live[kCountPer], ok[kCountPer];
     // every load comes before the first atomic or store (see kb_vertex)
-    uint32_t packed = 0u;
-    bool oob = false;
-    int cls = 0;
-    if (t < d.tri_count) {
-        uint32_t vi[3];
-        if (!load_tri_indices(d, t, vi)) {
-            oob = true;
-        } else {
-            float4 a = p.vpos[d.vbase + vi[0]], bb = p.vpos[d.vbase + vi[1]], c = p.vpos[d.vbase + vi[2]];
-            cls = classify(a, bb, c);
-            if (cls == 1) {
+#pragma unroll
+    for (uint32_t h = 0; h < kCountPer; ++h) {
+        const uint32_t b = hb * kCountPer + h;
+        inb[h] = b < p.n_tblocks;  // the grid's last workgroup may hold one block
+        const uint32_t bc = inb[h] ? b : p.n_tblocks - 1u;
+        di[h] = upper_index(p.n_draws, bc, [&](uint32_t i) { return p.draws[i].blk_t0; });
+        const DrawRec &d = p.draws[di[h]];
+        const uint32_t t = (bc - d.blk_t0) * 256u + threadIdx.x;
+        slot[h] = bc * 256u + threadIdx.x;
+        live[h] = inb[h] && t < d.tri_count;
+        // the raw indices only, and from every lane (a lane past the draw reads its last
+        // triangle, which a block of the draw always has, and ignores it): a check here, or a
+        // load under a branch, would wait for them before the next block's loads go out
+        const uint32_t *ib = d.ib + 3u * min(t, d.tri_count - 1u);
+        vi[h][0] = load_global(ib);
+        vi[h][1] = load_global(ib + 1);
+        vi[h][2] = load_global(ib + 2);
+    }
+#pragma unroll
+    for (uint32_t h = 0; h < kCountPer; ++h) {
+        // the range check of load_tri_indices
+        const DrawRec &d = p.draws[di[h]];
+        const long long bv = (long long)d.base_vertex, nv = (long long)d.vertex_count;
+        bool in_range = true;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const long long v = (long long)vi[h][k] + bv;
+            in_range = in_range && v >= 0 && v < nv;
+            vi[h][k] = (uint32_t)v;
+        }
+        ok[h] = live[h] && in_range;
+    }
+    float4 q[kCountPer][3];
+#pragma unroll
+    for (uint32_t h = 0; h < kCountPer; ++h) {
+        // from every lane as well (slot 0 where the triangle is not set up)
+        const uint32_t vb = p.draws[di[h]].vbase;
+#pragma unroll
+        for (int k = 0; k < 3; ++k) q[h][k] = p.vpos[ok[h] ? vb + vi[h][k] : 0u];
+    }
+    uint32_t packed[kCountPer];
+    int cls[kCountPer];
+#pragma unroll
+    for (uint32_t h = 0; h < kCountPer; ++h) {
+        packed[h] = 0u;
+        cls[h] = 0;
+        if (ok[h]) {
+            const DrawRec &d = p.draws[di[h]];
+            cls[h] = classify(q[h][0], q[h][1], q[h][2]);
+            if (cls[h] == 1) {
                 TriVis tv;
                 float rw[3];
-                packed = setup_triangle(d, v, a, bb, c, tv, rw) ? 1u : 0u;
+                packed[h] = setup_triangle(d, p.views[d.view], q[h][0], q[h][1], q[h][2], tv, rw) ? 1u : 0u;
             }
         }
     }
-    if (oob) atomicAdd(&p.stats->oob, 1ull);
-    if (cls == 2) {
-        uint32_t k = atomicAdd(&p.ctr->n_clip_list, 1u);
-        p.clip_list[k] = make_uint2(slot, di);
+#pragma unroll
+    for (uint32_t h = 0; h < kCountPer; ++h) {
+        if (live[h] && !ok[h]) atomicAdd(&p.stats->oob, 1ull);
+        if (cls[h] == 2) {
+            uint32_t k = atomicAdd(&p.ctr->n_clip_list, 1u);
+            p.clip_list[k] = make_uint2(slot[h], di[h]);
+        }
+        if (inb[h]) p.cnt[slot[h]] = packed[h];
+        uint32_t sum = wave_sum(packed[h]);
+        if ((threadIdx.x & 63u) == 0u) s_w[h][threadIdx.x >> 6] = sum;
     }
-    p.cnt[slot] = packed;
-    uint32_t s = wave_sum(packed);
-    if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = s;
     __syncthreads();
-    if (threadIdx.x == 0) p.blocksum[b] = (unsigned long long)(s_w[0] + s_w[1] + s_w[2] + s_w[3]);
+    if (threadIdx.x < kCountPer && hb * kCountPer + threadIdx.x < p.n_tblocks)
+        p.blocksum[hb * kCountPer + threadIdx.x] =
+            (unsigned long long)(s_w[threadIdx.x][0] + s_w[threadIdx.x][1] + s_w[threadIdx.x][2] + s_w[threadIdx.x][3]);
 }
 
 // Triangles that need the clipper, one per 16-lane group: the polygon's fan triangles that
@@ -1743,7 +1789,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         break;
     case 1:
         if (!p.n_tblocks) break;
-        hipLaunchKernelGGL(kb_setup_count, dim3(p.n_tblocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(kb_setup_count, dim3((p.n_tblocks + kCountPer - 1u) / kCountPer), dim3(256), 0, s, p);
         hipLaunchKernelGGL(kb_clip_count, dim3(1024), dim3(256), 0, s, p);
         if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc,
                                                  &p.ctr->n_assembled_clip, s)) != hipSuccess)
