@@ -130,6 +130,9 @@ struct DevStats {
     unsigned long long assembled, active8, pairs8, oob, overflow_draws, capacity_errors, max_pairs, max_clip_list;
     uint32_t scan_errors;  // scans whose look-back gave up waiting (never expected)
     uint32_t pad;
+    // kb_tile's active-bin counts, one atomic per tile spread over 64 counters (so that no
+    // single address takes thousands of atomics); the host adds them to active8
+    unsigned long long active8_part[64];
 };
 
 struct BatchParams {

@@ -1736,22 +1736,13 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
         tile_body<false, N>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
     __syncthreads();
     // (bin, draw) activity of this tile for the stats; this tile's start is no longer needed
-    if (threadIdx.x == 0u) p.tile_start[g] = s_active[0] + s_active[1] + s_active[2] + s_active[3];
+    if (threadIdx.x == 0u) {
+        const uint32_t a = s_active[0] + s_active[1] + s_active[2] + s_active[3];
+        if (a) atomicAdd(&p.stats->active8_part[blockIdx.x & 63u], (unsigned long long)a);
+    }
 }
 
 // Sum of the per-tile activity counts into the stats (one atomic per block).
-__global__ __launch_bounds__(256) void kb_active_sum(BatchParams p) {
-    __shared__ unsigned long long s_w[4];
-    unsigned long long acc = 0ull;
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < p.total_tiles; i += gridDim.x * 256u) {
-        // tiles without pairs kept start = 0 and returned early, which counts nothing
-        acc += p.tile_start[i];
-    }
-    acc = wave_sum(acc);
-    if ((threadIdx.x & 63u) == 0u) s_w[threadIdx.x >> 6] = acc;
-    __syncthreads();
-    if (threadIdx.x == 0) atomicAdd(&p.stats->active8, s_w[0] + s_w[1] + s_w[2] + s_w[3]);
-}
 
 // ------------------------------------------------------------ launchers
 
@@ -1842,8 +1833,6 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         else
             hipLaunchKernelGGL(kb_tile<256>, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
         if (tile_ev && (e = hipEventRecord(tile_ev[1], s)) != hipSuccess) return e;
-        hipLaunchKernelGGL(kb_active_sum, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 64u)), dim3(256), 0,
-                           s, p);
         break;
     }
     default:

@@ -1444,6 +1444,7 @@ MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out) {
     out->input_triangle_count = ctx->input_tris;
     out->assembled_triangle_count = s.assembled;
     out->active_bin_count = s.active8;
+    for (unsigned long long a : s.active8_part) out->active_bin_count += a;
     out->total_triangle_count_in_bins = s.pairs8;
     out->oob_triangle_count = s.oob;
     out->draw_count = ctx->draws;
