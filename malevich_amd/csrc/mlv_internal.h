@@ -283,6 +283,22 @@ struct CameraRec {
 hipError_t launch_camera_cbs(const CameraRec *cams, uint32_t n, float *cbs, hipStream_t s);
 
 // Launchers (mlv_kernels.hip). All enqueue on `s`.
+// Buffers of 32-bit words to fill with one value each, in one launch.
+constexpr uint32_t kMaxFills = 4;
+struct FillJobs {
+    uint32_t *d[kMaxFills];
+    uint32_t v[kMaxFills];
+    size_t n[kMaxFills];
+    uint32_t count;
+    void add(void *dst, uint32_t value, size_t words) {
+        if (!words || count >= kMaxFills) return;
+        d[count] = (uint32_t *)dst;
+        v[count] = value;
+        n[count] = words;
+        ++count;
+    }
+};
+hipError_t launch_fill(const FillJobs &f, hipStream_t s);
 hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s);
 // n_pixels u32 frame-buffer values to 3 * n_pixels bytes of RGB; rgb holds 12 * ceil(n_pixels / 4)
 // bytes and is 4-byte aligned

@@ -270,8 +270,7 @@ __global__ __launch_bounds__(256) void k_srgb_to_linear_rgba8(uint32_t *texels, 
 
 // Fill of n 32-bit words: words up to the first 16-byte boundary, then 16-byte stores, then the
 // remaining words.
-__global__ __launch_bounds__(256) void k_fill32(uint32_t *d, uint32_t v, size_t n) {
-    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+__device__ __forceinline__ void fill_one(uint32_t *d, uint32_t v, size_t n, size_t tid, size_t stride) {
     const size_t head = min(n, (size_t)(((16u - ((uintptr_t)d & 15u)) & 15u) / 4u));
     if (tid < head) d[tid] = v;
     uint4 *body = (uint4 *)(d + head);
@@ -280,6 +279,13 @@ __global__ __launch_bounds__(256) void k_fill32(uint32_t *d, uint32_t v, size_t 
     for (size_t i = tid; i < nb; i += stride) body[i] = v4;
     const size_t tail = head + nb * 4u;
     if (tid < n - tail) d[tail + tid] = v;
+}
+
+// Up to kMaxFills buffers filled by one launch (a clear's colour and visibility buffers, or its
+// depth and hierarchical-Z values).
+__global__ __launch_bounds__(256) void k_fill32(FillJobs f) {
+    const size_t tid = (size_t)blockIdx.x * blockDim.x + threadIdx.x, stride = (size_t)gridDim.x * blockDim.x;
+    for (uint32_t k = 0; k < f.count; ++k) fill_one(f.d[k], f.v[k], f.n[k], tid, stride);
 }
 
 // ------------------------------------------------------------ scans
@@ -1759,11 +1765,19 @@ hipError_t launch_srgb_to_linear_rgba8(uint32_t *texels, size_t n, hipStream_t s
     return hipGetLastError();
 }
 
-hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s) {
+hipError_t launch_fill(const FillJobs &f, hipStream_t s) {
+    size_t n = 0;
+    for (uint32_t k = 0; k < f.count; ++k) n = std::max(n, f.n[k]);
     if (!n) return hipSuccess;
     const unsigned grid = (unsigned)std::min<size_t>((n / 4u + 255u) / 256u + 1u, 2048u);
-    hipLaunchKernelGGL(k_fill32, dim3(grid), dim3(256), 0, s, dst, value, n);
+    hipLaunchKernelGGL(k_fill32, dim3(grid), dim3(256), 0, s, f);
     return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t s) {
+    FillJobs f{};
+    f.add(dst, value, n);
+    return launch_fill(f, s);
 }
 
 hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s) {

@@ -951,21 +951,26 @@ static uint32_t trunc_u32(float f) {
 static int exec_clear_color(mlv_context *ctx, const Cmd &c) {
     RC_OR_RETURN(flush_if_targets(ctx, c.rt, nullptr));
     const size_t n = (size_t)c.rt->width * c.rt->height;
-    HIP_OR_FAIL(ctx, launch_fill_u32(c.rt->ptr, c.value, n, ctx->stream));
+    FillJobs f{};  // colour and the visibility buffers in one launch
+    f.add(c.rt->ptr, c.value, n);
     if (c.vis_tri) {
-        HIP_OR_FAIL(ctx, launch_fill_u32(c.vis_tri, 0xFFFFFFFFu, n, ctx->stream));
-        HIP_OR_FAIL(ctx, launch_fill_u32(c.vis_draw, 0xFFFFFFFFu, n, ctx->stream));
+        f.add(c.vis_tri, 0xFFFFFFFFu, n);
+        f.add(c.vis_draw, 0xFFFFFFFFu, n);
     }
-    if (c.vis_prim) HIP_OR_FAIL(ctx, launch_fill_u32(c.vis_prim, 0xFFFFFFFFu, n, ctx->stream));
+    if (c.vis_prim) f.add(c.vis_prim, 0xFFFFFFFFu, n);
+    HIP_OR_FAIL(ctx, launch_fill(f, ctx->stream));
     return MLV_OK;
 }
 
 static int exec_clear_depth(mlv_context *ctx, const Cmd &c) {
     RC_OR_RETURN(flush_if_targets(ctx, nullptr, c.ds));
-    HIP_OR_FAIL(ctx, launch_fill_f32(c.ds->ptr, c.depth, (size_t)c.ds->width * c.ds->height, ctx->stream));
-    // main.c:1212-1214: every tile minimum back to 0.0 (1.0 for the LESS_EQUAL mirror)
+    // main.c:1212-1214: every tile minimum back to 0.0 (1.0 for the LESS_EQUAL mirror); depth and
+    // hierarchical-Z in one launch
     const float h = c.depth_func == MLV_DEPTH_LESS_EQUAL ? 1.0f : 0.0f;
-    HIP_OR_FAIL(ctx, launch_fill_f32(c.ds->hiz, h, (size_t)c.ds->bins_x * c.ds->bins_y, ctx->stream));
+    FillJobs f{};
+    f.add(c.ds->ptr, __builtin_bit_cast(uint32_t, c.depth), (size_t)c.ds->width * c.ds->height);
+    f.add(c.ds->hiz, __builtin_bit_cast(uint32_t, h), (size_t)c.ds->bins_x * c.ds->bins_y);
+    HIP_OR_FAIL(ctx, launch_fill(f, ctx->stream));
     c.ds->hiz_mode = c.depth_func;
     return MLV_OK;
 }
