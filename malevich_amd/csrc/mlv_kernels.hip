@@ -1688,10 +1688,14 @@ __device__ __forceinline__ uint32_t tile_class(const BatchParams &p, uint32_t g,
 
 template <bool PLACE>
 __global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
+    // a block's four waves combine their class counts in LDS, so each block makes one global
+    // atomic per class (one per wave and class contended for the eight counters)
+    __shared__ uint32_t s_wc[4][8];  // per wave and class: its count, then its offset in the block
+    __shared__ uint32_t s_base[8];   // per class: the block's first slot
     const uint32_t n = p.total_tiles;
     const bool ovf = (p.ctr->overflow & OVF_PAIRS) != 0u;
     const uint32_t avg = max(1u, n ? p.ctr->n_pairs / n : 1u);
-    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     uint32_t start[8];
     if (PLACE) {
         uint32_t acc = 0u;
@@ -1700,22 +1704,38 @@ __global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
             acc += p.ctr->class_cnt[c];
         }
     }
-    for (uint32_t base = blockIdx.x * 256u + (threadIdx.x & ~63u); base < n; base += gridDim.x * 256u) {
-        const uint32_t g = base + lane;
+    for (uint32_t base = blockIdx.x * 256u; base < n; base += gridDim.x * 256u) {
+        const uint32_t g = base + threadIdx.x;
         const uint32_t cls = g < n ? tile_class(p, g, ovf, avg) : 8u;
 #pragma unroll
         for (uint32_t c = 0; c < 8u; ++c) {
             const uint64_t b = __ballot(cls == c);
-            if (!b) continue;
-            if (!PLACE) {
-                if (lane == 0u) atomicAdd(&p.ctr->class_cnt[c], (uint32_t)__popcll(b));
-            } else {
-                uint32_t first = 0u;
-                if (lane == 0u) first = atomicAdd(&p.ctr->class_cur[c], (uint32_t)__popcll(b));
-                first = __shfl(first, 0, 64);
-                if (cls == c) p.tile_order[start[c] + first + mbcnt(b)] = g;
+            if (lane == 0u) s_wc[w][c] = (uint32_t)__popcll(b);
+        }
+        __syncthreads();
+        if (threadIdx.x < 8u) {
+            const uint32_t c = threadIdx.x;
+            uint32_t e = 0u;
+#pragma unroll
+            for (uint32_t q = 0; q < 4u; ++q) {
+                const uint32_t t = s_wc[q][c];
+                s_wc[q][c] = e;
+                e += t;
+            }
+            if (e) {
+                if (!PLACE) atomicAdd(&p.ctr->class_cnt[c], e);
+                else s_base[c] = start[c] + atomicAdd(&p.ctr->class_cur[c], e);
             }
         }
+        __syncthreads();
+        if (PLACE) {
+#pragma unroll
+            for (uint32_t c = 0; c < 8u; ++c) {
+                const uint64_t b = __ballot(cls == c);
+                if (cls == c) p.tile_order[s_base[c] + s_wc[w][c] + mbcnt(b)] = g;
+            }
+        }
+        __syncthreads();  // s_wc and s_base are reused by the next round
     }
 }
 
