@@ -222,6 +222,7 @@ def bench_c3(dev, args):
     ev.record(a, dev.stream)
     for _ in range(args.c3_steps):
         one()
+    dev.flush()  # the last frame's draws are recorded, not yet run: run them inside the timed region
     ev.record(b, dev.stream)
     ms = ev.elapsed_ms(a, b) / args.c3_steps
     dev.synchronize()
@@ -565,6 +566,7 @@ def main(argv=None):
     ev.record(e0, dev.stream)
     for _ in range(args.steps):
         frame(dev, res, sc)
+    dev.flush()  # the last frame's draws run before the end event, not at the next clear
     ev.record(e1, dev.stream)
     ms_total = ev.elapsed_ms(e0, e1)
     dev.synchronize()
@@ -576,6 +578,7 @@ def main(argv=None):
     ev.record(e2, dev.stream)
     for _ in range(args.steps):
         frame(dev, res, sc)
+    dev.flush()
     ev.record(e3, dev.stream)
     ms_plain = ev.elapsed_ms(e2, e3)
     dev.synchronize()
@@ -605,6 +608,11 @@ def main(argv=None):
     gpix_s = P_px / (ms_step * 1e-3) / 1e9
     # dominant kernel: the stage with the most time
     stage_ms = {k: prof[k] for k in ("vertex", "setup", "binning", "tile")}
+    stage_sum = sum(stage_ms.values()) / args.steps
+    if ms_step < stage_sum:
+        # the stages of every timed frame must lie inside the timed region
+        raise SystemExit(f"bench: {ms_step:.4f} ms/frame is below the sum of its stages ({stage_sum:.4f} ms): "
+                         "the timed region leaves work out")
     launches = max(1, int(prof["flushes"]))  # one launch of each stage kernel per flush
     dom = max(stage_ms, key=stage_ms.get)
     # the tile stage's own kernel is timed alone (events around the kb_tile launch); other

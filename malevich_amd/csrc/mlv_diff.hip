@@ -175,23 +175,37 @@ __device__ __forceinline__ bool evaluate(const DiffMesh &m, const DiffView &v, c
     Clip<float> cf;
     clip_of(m, v, e.vi, cf);
     e.lvl = mip_level(m, v, e.vi, cf, (float)x, (float)y);
-    Clip<R> c;
+    // The pixel corner's NDC, the barycentrics and the interpolated attributes in FP64 on both
+    // paths: the homogeneous form is ill-conditioned for slivers seen edge-on (the sum of the
+    // three cross products nearly cancels), where f32 barycentrics drift far from the
+    // mathematical value. Only the texture filter and the curves below run in R.
+    Clip<double> c;
     clip_of(m, v, e.vi, c);
-    e.u = (R)ndc_u((float)x, v.W);
-    e.v = (R)ndc_v((float)y, v.H);
-    bary(c, e.u, e.v, e.B);
-    const R *L = e.B.lam;
+    const double ud = (2.0 * (double)x) / (double)v.W - 1.0, vd = 1.0 - (2.0 * (double)y) / (double)v.H;
+    Bary<double> B;
+    bary(c, ud, vd, B);
+    e.u = (R)ud;
+    e.v = (R)vd;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e.B.a[k] = (R)B.a[k];
+        e.B.b[k] = (R)B.b[k];
+        e.B.l[k] = (R)B.l[k];
+        e.B.lam[k] = (R)B.lam[k];
+    }
+    e.B.S = (R)B.S;
+    const double *L = B.lam;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        e.col[j] = (L[0] * (R)m.col[3u * e.vi[0] + j] + L[1] * (R)m.col[3u * e.vi[1] + j]) +
-                   L[2] * (R)m.col[3u * e.vi[2] + j];
-        e.n[j] = (L[0] * (R)m.nrm[3u * e.vi[0] + j] + L[1] * (R)m.nrm[3u * e.vi[1] + j]) +
-                 L[2] * (R)m.nrm[3u * e.vi[2] + j];
+        e.col[j] = (R)((L[0] * (double)m.col[3u * e.vi[0] + j] + L[1] * (double)m.col[3u * e.vi[1] + j]) +
+                       L[2] * (double)m.col[3u * e.vi[2] + j]);
+        e.n[j] = (R)((L[0] * (double)m.nrm[3u * e.vi[0] + j] + L[1] * (double)m.nrm[3u * e.vi[1] + j]) +
+                     L[2] * (double)m.nrm[3u * e.vi[2] + j]);
     }
 #pragma unroll
     for (int j = 0; j < 2; ++j)
-        e.uvs[j] = (L[0] * (R)m.uv[2u * e.vi[0] + j] + L[1] * (R)m.uv[2u * e.vi[1] + j]) +
-                   L[2] * (R)m.uv[2u * e.vi[2] + j];
+        e.uvs[j] = (R)((L[0] * (double)m.uv[2u * e.vi[0] + j] + L[1] * (double)m.uv[2u * e.vi[1] + j]) +
+                       L[2] * (double)m.uv[2u * e.vi[2] + j]);
     // bilinear_f_x8 (csc.h:143-161) on the chosen level
     const uint32_t lw = m.lw[e.lvl], lh = m.lh[e.lvl];
     const R s = (R)lw * e.uvs[0] - (R)0.5;

@@ -148,6 +148,9 @@ struct mlv_context {
     uint64_t capacity_errors = 0;
     size_t clip_hint = 4096, pair_hint = 0;
     size_t pair_limit = 0;  // MLV_PAIR_CAPACITY: caps the bin-list arena (exercises the overflow path)
+    // a flush takes recorded draws until one more would pass either limit (MLV_MAX_BATCH_TRIANGLES
+    // and MLV_MAX_BATCH_DRAWS lower them; a single larger draw still runs, alone)
+    uint64_t batch_tris = kMaxBatchTriangles, batch_draws = kMaxBatchDraws;
     // stage profiling: event pairs recorded per stage, resolved at mlv_get_profile
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -577,6 +580,15 @@ MLV_API int mlv_context_create(int device, mlv_context **out) {
     if (!ctx) return MLV_ERROR_OUT_OF_MEMORY;
     ctx->device = device;
     if (const char *lim = getenv("MLV_PAIR_CAPACITY")) ctx->pair_limit = (size_t)strtoull(lim, nullptr, 10);
+    // smaller flushes than the defaults (a batch split exercised on small scenes); never larger
+    if (const char *lim = getenv("MLV_MAX_BATCH_TRIANGLES")) {
+        const uint64_t v = strtoull(lim, nullptr, 10);
+        if (v) ctx->batch_tris = std::min<uint64_t>(v, kMaxBatchTriangles);
+    }
+    if (const char *lim = getenv("MLV_MAX_BATCH_DRAWS")) {
+        const uint64_t v = strtoull(lim, nullptr, 10);
+        if (v) ctx->batch_draws = std::min<uint64_t>(v, kMaxBatchDraws);
+    }
     if (hipSetDevice(device) != hipSuccess ||
         hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess ||
         hipMalloc(&ctx->ctr, sizeof(BatchCounters)) != hipSuccess ||
@@ -1044,7 +1056,7 @@ static int submit_draw(mlv_context *ctx, const Cmd &c) {
         HIP_OR_FAIL(ctx, launch_fill_f32(ds->hiz, h, (size_t)ds->bins_x * ds->bins_y, ctx->stream));
         ds->hiz_mode = pd.key.depth_func;
     }
-    if (ctx->pending.size() >= kMaxBatchDraws || ctx->pending_tris + align256(c.T) > kMaxBatchTriangles ||
+    if (ctx->pending.size() >= ctx->batch_draws || ctx->pending_tris + align256(c.T) > ctx->batch_tris ||
         ctx->pending_verts + align256(c.V) > kMaxBatchVertices)
         RC_OR_RETURN(flush(ctx));
     ctx->pending.push_back(pd);
@@ -1132,6 +1144,33 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
 
 // ------------------------------------------------------------------ graphs
 
+// The device's additive stat counters (what mlv_get_stats reports from the device) around runs
+// of a recorded frame that are not frames of the caller's: the sizing retries of a capture and
+// the timing runs of the autotuner. The frame counts once, like an eager frame.
+static int read_dstats(mlv_context *ctx, DevStats &s) {
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(&s, ctx->dstats, sizeof s, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    return MLV_OK;
+}
+
+// Sets the additive counters to base + (after - before): the counts from before the runs plus
+// those of the one run between `before` and `after`. High-water marks and error counters are
+// left as the runs left them.
+static int keep_one_run_stats(mlv_context *ctx, const DevStats &base, const DevStats &before, const DevStats &after) {
+    DevStats s{};
+    RC_OR_RETURN(read_dstats(ctx, s));
+    s.assembled = base.assembled + (after.assembled - before.assembled);
+    s.active8 = base.active8 + (after.active8 - before.active8);
+    s.pairs8 = base.pairs8 + (after.pairs8 - before.pairs8);
+    s.oob = base.oob + (after.oob - before.oob);
+    s.overflow_draws = base.overflow_draws + (after.overflow_draws - before.overflow_draws);
+    for (int i = 0; i < 64; ++i)
+        s.active8_part[i] = base.active8_part[i] + (after.active8_part[i] - before.active8_part[i]);
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->dstats, &s, sizeof s, hipMemcpyHostToDevice, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    return MLV_OK;
+}
+
 static int run_recorded(mlv_context *ctx, const std::vector<Cmd> &rec) {
     for (const Cmd &c : rec) {
         switch (c.kind) {
@@ -1163,7 +1202,10 @@ MLV_API int mlv_graph_capture_end(mlv_context *ctx, mlv_graph **out) {
     RC_OR_RETURN(set_device(ctx));
     // 1. run the commands once: sizes every arena and yields this frame's results
     int rc = MLV_OK;
+    DevStats base{}, before{}, after{};
+    RC_OR_RETURN(read_dstats(ctx, base));
     for (int attempt = 0; attempt < 3; ++attempt) {
+        RC_OR_RETURN(read_dstats(ctx, before));
         ctx->sizing = true;
         ctx->cap_sizes.clear();
         rc = run_recorded(ctx, rec);
@@ -1175,6 +1217,8 @@ MLV_API int mlv_graph_capture_end(mlv_context *ctx, mlv_graph **out) {
         if (rc != MLV_ERROR_CAPACITY) break;  // a capacity miss raised the capacity: size again
     }
     if (rc != MLV_OK) return rc;
+    RC_OR_RETURN(read_dstats(ctx, after));
+    RC_OR_RETURN(keep_one_run_stats(ctx, base, before, after));  // the frame's first execution counts once
     // 2. run them again under stream capture, uploading tables from buffers the graph owns
     mlv_graph *g = new (std::nothrow) mlv_graph();
     if (!g) return MLV_ERROR_OUT_OF_MEMORY;
@@ -1334,6 +1378,8 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
     std::vector<Cmd> rec;
     rec.swap(ctx->rec);
     RC_OR_RETURN(set_device(ctx));
+    DevStats base{}, before{}, after{};
+    RC_OR_RETURN(read_dstats(ctx, base));
     // size the arenas (a capacity miss raises the capacity; run again)
     int rc = MLV_OK;
     for (int attempt = 0; attempt < 3; ++attempt) {
@@ -1368,9 +1414,12 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
         }
         *knob[k] = keep;
     }
+    RC_OR_RETURN(read_dstats(ctx, before));
     RC_OR_RETURN(run_recorded(ctx, rec));  // leave the frame in its targets under the chosen parameters
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     RC_OR_RETURN(check_device_stats(ctx));
+    RC_OR_RETURN(read_dstats(ctx, after));
+    RC_OR_RETURN(keep_one_run_stats(ctx, base, before, after));  // the tuned frame counts once
     return mlv_get_tuning(ctx, best);
 }
 

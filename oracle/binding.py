@@ -160,6 +160,7 @@ def lib() -> C.CDLL:
         L.orc_inv4.argtypes = [P(C.c_float), P(C.c_float)]
         L.orc_diff_forward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p]
         L.orc_diff_backward.argtypes = [P(DiffScene), C.c_void_p, C.c_void_p, P(DiffGrads)]
+        L.orc_sh9.argtypes = [P(C.c_double), P(C.c_double)]
         _lib = L
     return _lib
 
@@ -262,6 +263,23 @@ def primitive_assembly_array(draw: Draw, capacity: int | None = None) -> np.ndar
     if rc != 0:
         raise RuntimeError(f"orc_primitive_assembly failed: {rc}")
     return out[: n.value].copy()
+
+
+def vertex_shader(draw: Draw, vertex_index: int) -> np.ndarray:
+    """The 12-float VS output record of one vertex (registers 0..2, main.c:714-727)."""
+    out = np.zeros(12, np.float32)
+    rc = lib().orc_vertex_shader(C.byref(draw.desc), vertex_index, out.ctypes.data_as(C.POINTER(C.c_float)))
+    if rc != 0:
+        raise RuntimeError(f"orc_vertex_shader failed: {rc}")
+    return out
+
+
+def sh9(n) -> np.ndarray:
+    """The oracle's 9 real SH basis values at the unit direction n."""
+    d = (C.c_double * 3)(*[float(x) for x in n])
+    out = (C.c_double * 9)()
+    lib().orc_sh9(d, out)
+    return np.array(out[:], np.float64)
 
 
 def coverage_mask(edges, ox: int, oy: int) -> int:

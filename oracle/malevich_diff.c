@@ -48,6 +48,8 @@ static void sh_basis(double x, double y, double z, double Y[9]) {
     Y[8] = SHC[4] * (x * x - y * y);
 }
 
+void orc_sh9(const double n[3], double out[9]) { sh_basis(n[0], n[1], n[2], out); }
+
 /* ---- the f32 mip choice (same operation sequence as the GPU kernel) */
 
 static float dot4_pair_f(const float *row, float x, float y, float z, float w) {

@@ -142,6 +142,8 @@ typedef struct orc_diff_grads {
     double *positions, *colors, *uvs, *normals, *texture, *sh;
 } orc_diff_grads;
 
+/* The real SH basis of bands 0..2 the shading uses, index l(l+1)+m, at a unit direction n. */
+void orc_sh9(const double n[3], double out[9]);
 int orc_diff_forward(const orc_diff_scene *s, const uint32_t *prim, double *rgba);
 int orc_diff_backward(const orc_diff_scene *s, const uint32_t *prim, const double *grad_rgba, orc_diff_grads *g);
 

@@ -76,9 +76,12 @@ def test_graph_replays_match_eager_frames(device):
         device.reset_stats()
         f.record()
         ref_a = f.read()
+        keys = ("assembled_triangle_count", "active_bin_count", "total_triangle_count_in_bins")
+        st_a = {k: device.stats()[k] for k in keys}
         f.cb.update(cam_b)
         f.record()
         ref_b = f.read()
+        st_ab = {k: device.stats()[k] for k in keys}
         assert (ref_a[2] != 0xFFFFFFFF).sum() > 1000 and (ref_b[2] != 0xFFFFFFFF).sum() > 1000
         assert not np.array_equal(ref_a[0], ref_b[0])
 
@@ -89,6 +92,10 @@ def test_graph_replays_match_eager_frames(device):
             device.flush()  # anything that would run work is refused while recording
         assert e.value.code == L.ERROR_INVALID_STATE
         g = device.graph_capture_end()  # runs the frame once, then captures it
+        # the capture's first run counts once, like an eager frame (its sizing retries do not)
+        st = device.stats()
+        for k in keys:
+            assert st[k] == st_ab[k] + st_a[k], (k, st[k], st_ab[k], st_a[k])
         # draw tags count draws since the last reset: the recorded draws got theirs after the
         # two eager frames, and every replay writes those
         tag0 = 2 * len(sc.instances)

@@ -43,7 +43,7 @@ def test_tunings_give_the_oracle_frame(device, oracle, tuning):
     assert_frames_match(g, o, color_tol=0)
 
 
-def test_autotune_keeps_results_and_choice(device):
+def test_autotune_keeps_results_and_choice(device, oracle):
     W, H = 320, 184
     sc = S.c2_scene(W, H, n_instances=8, seed=21, texture_size=0)
     env = S.smooth_env_map(256, 128, seed=6, lo=0.05, hi=2.0)
@@ -74,10 +74,20 @@ def test_autotune_keeps_results_and_choice(device):
     try:
         frame()
         ref = (rt.read().copy(), ds.read().copy())
+        d.reset_stats()
         d.autotune_begin()
         frame()
         best = d.autotune_end(repeats=2)
         print(f"  chosen: {best}", flush=True)
+        # the tuned frame counts once in the stats, however many times the tuner ran it
+        st = d.stats()
+        draws = [DrawSpec(m.indices, m.vertices, "basic", "env_lighting", cb=sc.cb, ps_tex=env)
+                 for m in sc.instances]
+        o = render_oracle(oracle, W, H, draws, S.CLEAR_COLOR)
+        for k in ("vertex_count", "input_triangle_count", "assembled_triangle_count", "active_bin_count",
+                  "total_triangle_count_in_bins", "oob_triangle_count"):
+            assert st[k] == o.stats[k], (k, st[k], o.stats[k])
+        assert st["draw_count"] == len(sc.instances)
         assert best["heavy_tile_pairs"] in (192, 384, 768, 1536, 0xFFFFFFFF)
         assert best["heavy_blocks"] in (256, 512, 1024, 2048)
         assert best["pair_blocks"] in (512, 1024, 2048, 4096)
