@@ -335,6 +335,16 @@ MLV_API int mlv_diff_forward(mlv_context *ctx, const mlv_diff_scene *scene, cons
  * in pixel order, so repeated calls give bit-identical gradients. */
 MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
                               const float *device_grad_rgba, const mlv_diff_grads *grads, int deterministic);
+/* No counterpart in the reference. For callers that cannot allocate device memory (code under
+ * HIP graph capture, tight serving loops, harnesses that forbid hipMalloc): mlv_diff_backward
+ * with deterministic = 0, whose scratch (the FP64 position sums and the per-block SH partials) is
+ * a fixed 32 MiB device array of the library, one per GPU, held until the process ends. It calls
+ * no hipMalloc or hipFree and runs no flush: recorded draws must have been flushed
+ * (MLV_ERROR_INVALID_STATE otherwise). A scene needing more than 32 MiB of scratch (24 B per
+ * vertex + 216 B per 16x16 pixel tile of the image) returns MLV_ERROR_CAPACITY. Every call on a
+ * GPU uses that GPU's array: calls that overlap on one GPU (other streams or contexts) race. */
+MLV_API int mlv_diff_backward_noalloc(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
+                                      const float *device_grad_rgba, const mlv_diff_grads *grads);
 
 /* ---- batched differentiable shading (extension): many views of many meshes in one launch
  *

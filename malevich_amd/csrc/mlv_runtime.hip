@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <utility>
@@ -560,6 +561,31 @@ static int check_device_stats(mlv_context *ctx) {
     return MLV_OK;
 }
 
+// ------------------------------------------------------------------ fixed scratch
+//
+// The scratch of mlv_diff_backward_noalloc: one fixed device array, which every GPU the library
+// runs on holds (each its own copy at its own address) until the process ends. In host code the
+// array's name is not a device address: each device's address is read once with
+// hipGetSymbolAddress when the first context on that device is created, and only that address
+// is passed to kernels.
+constexpr size_t kFixedScratchBytes = 32ull << 20;
+static __device__ __align__(256) uint8_t scratch[kFixedScratchBytes];
+constexpr int kMaxDevices = 64;
+static std::mutex g_scratch_mu;
+static void *g_scratch_ptr[kMaxDevices];
+static hipError_t g_scratch_err[kMaxDevices];
+
+// With ctx->device current: the device's address of `scratch`, looked up once per device.
+static void bind_fixed_scratch(mlv_context *ctx) {
+    if (ctx->device < 0 || ctx->device >= kMaxDevices) return;
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (g_scratch_ptr[ctx->device]) return;
+    void *ptr = nullptr;
+    const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    g_scratch_err[ctx->device] = e;
+    if (e == hipSuccess) g_scratch_ptr[ctx->device] = ptr;
+}
+
 extern "C" {
 
 MLV_API int mlv_device_count(int *out) {
@@ -601,6 +627,7 @@ MLV_API int mlv_context_create(int device, mlv_context **out) {
         delete ctx;
         return MLV_ERROR_DEVICE;
     }
+    bind_fixed_scratch(ctx);  // the device is current
     *out = ctx;
     return MLV_OK;
 }
@@ -1668,6 +1695,37 @@ MLV_API int mlv_diff_backward(mlv_context *ctx, const mlv_diff_scene *scene, con
     return MLV_OK;
 }
 
+MLV_API int mlv_diff_backward_noalloc(mlv_context *ctx, const mlv_diff_scene *scene, const uint32_t *device_prim_ids,
+                                      const float *device_grad_rgba, const mlv_diff_grads *grads) {
+    if (!ctx || !device_grad_rgba || !grads)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff_backward_noalloc: bad argument");
+    // nothing here may allocate: recorded draws would need a flush, whose arenas may grow
+    if (ctx->recording || !ctx->pending.empty())
+        return fail(ctx, MLV_ERROR_INVALID_STATE,
+                    "diff_backward_noalloc: recorded draws are pending; call mlv_flush first");
+    DiffParams p;
+    RC_OR_RETURN(diff_params(ctx, scene, device_prim_ids, p));
+    const uint32_t nb = diff_tile_blocks(p.v.W, p.v.H);
+    const size_t pos_bytes = ((size_t)p.m.V * 3 * 8 + 255) & ~(size_t)255;
+    const size_t need = pos_bytes + (size_t)std::max<uint32_t>(nb, 1u) * 27 * 8;
+    if (need > kFixedScratchBytes)
+        return fail(ctx, MLV_ERROR_CAPACITY,
+                    "diff_backward_noalloc: %zu bytes of scratch needed (24 B per vertex + 216 B per 16x16 pixel "
+                    "tile), the fixed array holds %zu",
+                    need, kFixedScratchBytes);
+    if (ctx->device < 0 || ctx->device >= kMaxDevices || !g_scratch_ptr[ctx->device])
+        return fail(ctx, MLV_ERROR_DEVICE, "diff_backward_noalloc: no scratch address for device %d (%s)", ctx->device,
+                    (ctx->device >= 0 && ctx->device < kMaxDevices) ? hipGetErrorString(g_scratch_err[ctx->device])
+                                                                    : "device index out of range");
+    RC_OR_RETURN(set_device(ctx));
+    uint8_t *base = (uint8_t *)g_scratch_ptr[ctx->device];
+    DiffScratch sc{};
+    sc.posacc = (double *)base;  // cleared by the launch before use
+    sc.sh_part = (double *)(base + pos_bytes);  // every block writes its 27 partials
+    DiffGrads g{grads->positions, grads->colors, grads->uvs, grads->normals, grads->texture, grads->sh};
+    HIP_OR_FAIL(ctx, launch_diff_backward(p, (const float4 *)device_grad_rgba, g, sc, false, ctx->stream));
+    return MLV_OK;
+}
 
 // Batched shading: validates the meshes and views, lays out the device tables (meshes, views,
 // per-view first blocks, draw -> mesh map, per-mesh first position blocks, the ranges the
