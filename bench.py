@@ -53,39 +53,41 @@ class HipEvents:
 
 
 def build_c2(args):
+    """The 1080p forward workload as configured: 64 displaced spheres (16k-64k triangles each),
+    vertex_lighting_vs sampling a 256x128 RGBA16F irradiance map, the textured environment pixel
+    shader sampling a 2048x1024 RGBA16F radiance map with its full mip chain (12 levels) times a
+    1024x1024 RGBA8 albedo texture per instance with its mip chain (11 levels)."""
     from malevich_amd import scenes as S
     sc = S.c2_scene(args.width, args.height, n_instances=args.instances, seed=args.seed, texture_size=0)
-    # the configuration's maps: irradiance 256x128 and radiance 2048x1024, RGBA16F (the reference's
-    # shaders sample level 0 of one map each, so the configuration's mip levels and per-mesh
-    # RGBA8 textures have no consumer and are not uploaded)
-    irr = S.smooth_env_map(256, 128, seed=args.seed + 16, lo=0.05, hi=2.0).astype(np.float16)
-    rad = S.smooth_env_map(2048, 1024, seed=args.seed + 17, lo=0.05, hi=2.0).astype(np.float16)
-    return sc, (irr, rad)
+    return sc, S.c2_textures(args.instances, seed=args.seed, albedo_size=1024)
 
 
 def upload(dev, sc, env):
     from malevich_amd import _lib as L
+    irr, rad, albedo = env
     res = {"vb": [], "ib": [], "counts": []}
     for m in sc.instances:
         res["vb"].append(dev.create_buffer(L.BUFFER_VERTEX, m.vertices))
         res["ib"].append(dev.create_buffer(L.BUFFER_INDEX_U32, m.indices))
         res["counts"].append(m.indices.size)
     res["cb"] = dev.create_buffer(L.BUFFER_CONSTANT, sc.cb)
-    res["irr"] = dev.create_texture2d(env[0])
-    res["env"] = dev.create_texture2d(env[1])
+    res["irr"] = dev.create_texture2d(irr)
+    # the mip chains are built on the device from level 0 (mlv_texture2d_generate_mips)
+    res["env"] = dev.create_texture2d(rad, mip_levels=12, generate_mips=True)
+    res["albedo"] = [dev.create_texture2d(a, mip_levels=11, generate_mips=True) for a in albedo]
     res["rt"] = dev.create_render_target(sc.width, sc.height)
     res["ds"] = dev.create_depth_stencil(sc.width, sc.height)
     return res
 
 
 def frame(dev, res, sc):
-    from malevich_amd import _lib as L
     from malevich_amd import scenes as S
     dev.clear_render_target_view(res["rt"], S.CLEAR_COLOR)
     dev.clear_depth_stencil_view(res["ds"], 0.0)
-    for vb, ib, n in zip(res["vb"], res["ib"], res["counts"]):
+    for vb, ib, n, alb in zip(res["vb"], res["ib"], res["counts"], res["albedo"]):
         dev.ia_set_vertex_buffer(vb, 32)
         dev.ia_set_index_buffer(ib)
+        dev.ps_set_shader_resource(1, alb)
         dev.draw_indexed(n)
 
 
@@ -93,7 +95,7 @@ def bind_state(dev, res, sc):
     from malevich_amd import _lib as L
     dev.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
     dev.vs_set_shader(L.VS_VERTEX_LIGHTING)
-    dev.ps_set_shader(L.PS_ENV_LIGHTING)
+    dev.ps_set_shader(L.PS_ENV_TEXTURED)
     dev.vs_set_constant_buffer(0, res["cb"])
     dev.vs_set_shader_resource(0, res["irr"])
     dev.ps_set_shader_resource(0, res["env"])
@@ -102,25 +104,27 @@ def bind_state(dev, res, sc):
     dev.om_set_depth_func(L.DEPTH_GREATER_EQUAL)
 
 
-def cpu_baseline(sc, env, budget_s: float):
+def cpu_baseline(sc, env, res, budget_s: float):
     """The oracle (scalar C restatement of the reference's raster path, OpenMP over triangle
     chunks and bins like the reference's own loops) rendering the same frame on this host's
     cores, repeated for about `budget_s`; reported as input triangles per second. A one-thread
-    figure over the first instance draws is reported beside it."""
+    figure over the first instance draws is reported beside it. The oracle samples the same mip
+    chains (read back from the device, whose generation the tests pin to oracle/mips.py) as
+    float32 (RGBA16F texels are exact in float32)."""
     import ctypes as C
     from oracle import binding as ob
-    # the nearest-reference workload: the same maps as float32 (RGBA16F texels are exact in float32)
-    irr, rad = (t.astype(np.float32) for t in env)
+    irr = env[0].astype(np.float32)
+    rad = ob.MipTexture(res["env"].read_levels())
+    albedo = [ob.MipTexture(t.read_levels()) for t in res["albedo"]]
 
     def run(n_draws=None):
         rt = ob.RenderTarget(sc.width, sc.height, with_vis=False)
         rt.clear_render_target_view((227 / 255, 223 / 255, 216 / 255, 0.0))
         rt.clear_depth_stencil_view(0.0)
         tris = 0
-        for m in sc.instances[:n_draws]:
-            d = ob.Draw(m.indices, m.vertices, ob.VS_VERTEX_LIGHTING, ob.PS_ENV_LIGHTING, cb=sc.cb, vs_tex=irr,
-                        ps_tex=rad,
-                        viewport=(0, 0, sc.width, sc.height, 0, 1))
+        for m, alb in list(zip(sc.instances, albedo))[:n_draws]:
+            d = ob.Draw(m.indices, m.vertices, ob.VS_VERTEX_LIGHTING, ob.PS_ENV_TEXTURED, cb=sc.cb, vs_tex=irr,
+                        ps_tex=rad, ps_tex1=alb, viewport=(0, 0, sc.width, sc.height, 0, 1))
             ob.draw_indexed(rt, d)
             tris += m.triangle_count
         return tris
@@ -142,8 +146,8 @@ def cpu_baseline(sc, env, budget_s: float):
     dt1 = time.perf_counter() - t1
     gomp.omp_set_num_threads(cores)
     return {"value": tris / dt / 1e6, "unit": "Mtri/s", "cores": cores, "kind": "port",
-            "sample": f"{frames} full frame(s) of the same 1080p workload ({tris} triangles, {dt:.1f} s) with the "
-                      f"maps as float32 (the reference's texel formats); oracle (C, gcc -O2 -fopenmp) on {cores} threads",
+            "sample": f"{frames} full frame(s) of the same 1080p workload ({tris} triangles, {dt:.1f} s), the "
+                      f"same mip chains (RGBA16F texels as float32); oracle (C, gcc -O2 -fopenmp) on {cores} threads",
             "single_thread_value": round(tris1 / dt1 / 1e6, 3),
             "single_thread_sample": f"first 8 of {len(sc.instances)} draws ({tris1} triangles, {dt1:.1f} s)"}
 
@@ -156,7 +160,7 @@ def measured_traffic(kernel: str, sc):
     (rocprofv3 FETCH_SIZE + WRITE_SIZE, scaled by a copy of known size), or None."""
     path = ROOT / "profiles" / "traffic.json"
     try:
-        rec = json.loads(path.read_text())["c2-1080p-forward"]
+        rec = json.loads(path.read_text())["c2-1080p-forward-textured"]
     except (OSError, KeyError, ValueError):
         return None
     if (rec.get("triangles"), rec.get("width"), rec.get("height")) != (sc.triangle_count, sc.width, sc.height):
@@ -655,8 +659,9 @@ def main(argv=None):
         "data": "synthetic",
         "config": {"workload": f"c2-1080p-forward: {len(sc.instances)} displaced geodesic spheres, {tris} "
                                f"triangles, depth complexity {sc.depth_complexity:.1f}, vertex_lighting_vs (256x128 "
-                               f"RGBA16F irradiance) + env_lighting_ps (2048x1024 RGBA16F radiance), reversed-Z GE, "
-                               f"{sc.width}x{sc.height}",
+                               f"RGBA16F irradiance) + textured env lighting (2048x1024 RGBA16F radiance, 12 mips, "
+                               f"times a 1024x1024 RGBA8 albedo with 11 mips per instance; level nearest the "
+                               f"footprint), reversed-Z GE, {sc.width}x{sc.height}",
                    "width": sc.width, "height": sc.height, "triangles": tris, "draws_per_frame": len(sc.instances)},
         "gpix_per_s": round(gpix_s, 4),
         "stage_ms_per_frame": {k: round(v / args.steps, 4) for k, v in stage_ms.items()},
@@ -685,7 +690,7 @@ def main(argv=None):
         out["per_draw_views_per_s"] = c5[f"{key}_per_draw_views_per_s"]
         print(f"[bench] c5: {c5}", file=sys.stderr, flush=True)
     if not args.no_cpu:
-        out["cpu_baseline"] = cpu_baseline(sc, env, args.cpu_budget)
+        out["cpu_baseline"] = cpu_baseline(sc, env, res, args.cpu_budget)
         out["cpu_baseline"]["gpu_over_cpu"] = round(mtri_s / out["cpu_baseline"]["value"], 1)
     dev.close()
     print(json.dumps(out))

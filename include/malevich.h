@@ -74,11 +74,26 @@ typedef enum mlv_vertex_shader {
     MLV_VS_FULLSCREEN = 4
 } mlv_vertex_shader;
 
-/* The reference's pixel shaders: basic_ps.c, env_lighting_ps.c, passthrough_ps.c. */
+/* The reference's pixel shaders: basic_ps.c, env_lighting_ps.c, passthrough_ps.c; and two
+ * extensions that sample mip chains on the level nearest the pixel's footprint (the reference's
+ * Texture2D has one level, csc.h:20-24):
+ *   MLV_PS_BASIC_LAMBERT  basic_ps (basic_ps.c:16-25) times a directional light: texture (slot 0)
+ *                         at UV * (light_colour * max(dot(normalize(NORMAL), L), 0) + ambient),
+ *                         then basic_ps's sRGB curve. PS constant buffer slot 0, 12 floats:
+ *                         L.xyz _, light_colour.rgb _, ambient.rgb _ (L points at the light).
+ *   MLV_PS_ENV_TEXTURED   env_lighting_ps (env_lighting_ps.c:13-22) times an albedo texture: the
+ *                         float equirect map (slot 0) at normalize(NORMAL) times the texture
+ *                         (slot 1) at UV, then env_lighting_ps's tone curve.
+ * The level: rho^2 = the larger squared distance, in level-0 texels, between the sampled
+ * coordinate at the pixel corner and at the next corners right and below (for the equirect map:
+ * the chord between the unit directions times width / 2 pi); level k is taken when
+ * rho^2 >= 2^(2k-1). */
 typedef enum mlv_pixel_shader {
     MLV_PS_BASIC = 1,
     MLV_PS_ENV_LIGHTING = 2,
-    MLV_PS_PASSTHROUGH = 3
+    MLV_PS_PASSTHROUGH = 3,
+    MLV_PS_BASIC_LAMBERT = 4, /* extension */
+    MLV_PS_ENV_TEXTURED = 5   /* extension */
 } mlv_pixel_shader;
 
 typedef enum mlv_depth_func {
@@ -126,13 +141,20 @@ MLV_API int mlv_buffer_readback(mlv_context *ctx, const mlv_buffer *buf, size_t 
 MLV_API int mlv_buffer_destroy(mlv_context *ctx, mlv_buffer *buf);
 MLV_API void *mlv_buffer_device_ptr(const mlv_buffer *buf);
 
+/* mip_levels (0 means 1) levels are allocated, level 0 first, level l of max(width >> l, 1) x
+ * max(height >> l, 1) texels, at most down to 1x1; host_data, if given, holds all of them. */
 MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t height, int format, uint32_t mip_levels,
                                  const void *host_data, mlv_texture2d **out);
+/* Extension: fills levels 1.. of the texture's chain from level 0 on the device, each texel the
+ * average of its 2x2 window of the level above (2x1 or 1x2 once a side is 1; an odd side drops its
+ * last row or column): RGBA8 per byte in integers rounding halves up, the float formats summed in
+ * row order in f32 and divided by the window's area (RGBA16F rounded to half, nearest-even). */
+MLV_API int mlv_texture2d_generate_mips(mlv_context *ctx, mlv_texture2d *tex);
 MLV_API int mlv_texture2d_destroy(mlv_context *ctx, mlv_texture2d *tex);
 /* load_texture(..., is_in_srgb = true) (main.c:538-559): the in-place sRGB -> linear pre-pass of
- * an RGBA8 texture (all four channels, truncating re-encode), run on the device. */
+ * an RGBA8 texture (all four channels, truncating re-encode; every level), run on the device. */
 MLV_API int mlv_texture2d_srgb_to_linear(mlv_context *ctx, mlv_texture2d *tex);
-/* Copies the texture's level-0 texels (width*height texels of its format) to the host. */
+/* Copies the texture's texels, every level (level 0 first), to the host. */
 MLV_API int mlv_texture2d_readback(mlv_context *ctx, const mlv_texture2d *tex, void *host_dst);
 MLV_API void *mlv_texture2d_device_ptr(const mlv_texture2d *tex);
 
@@ -160,6 +182,9 @@ MLV_API int mlv_rs_set_viewport(mlv_context *ctx, const mlv_viewport *vp);
 /* ---- PS (main.c:98-101) */
 MLV_API int mlv_ps_set_shader(mlv_context *ctx, int pixel_shader);
 MLV_API int mlv_ps_set_shader_resource(mlv_context *ctx, uint32_t slot, mlv_texture2d *tex);
+/* Extension: the reference's PS stage has shader resources only (main.c:98-101); the extension
+ * shaders read constant buffer slot 0. */
+MLV_API int mlv_ps_set_constant_buffer(mlv_context *ctx, uint32_t slot, mlv_buffer *cb);
 
 /* ---- OM (main.c:103-107) */
 MLV_API int mlv_om_set_render_targets(mlv_context *ctx, mlv_render_target *rt, mlv_depth_stencil *ds);

@@ -23,7 +23,7 @@ BUFFER_VERTEX, BUFFER_INDEX_U32, BUFFER_CONSTANT = 1, 2, 3
 FORMAT_RGBA8_UNORM, FORMAT_RGBA32F, FORMAT_RGBA16F = 1, 2, 3
 TOPOLOGY_UNDEFINED, TOPOLOGY_TRIANGLELIST = 0, 1
 VS_BASIC, VS_VERTEX_LIGHTING, VS_PASSTHROUGH, VS_FULLSCREEN = 1, 2, 3, 4
-PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH = 1, 2, 3
+PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH, PS_BASIC_LAMBERT, PS_ENV_TEXTURED = 1, 2, 3, 4, 5
 DEPTH_GREATER_EQUAL, DEPTH_LESS_EQUAL = 0, 1
 
 
@@ -113,6 +113,7 @@ SIGNATURES = {
     "mlv_buffer_device_ptr": (_P, [_P]),
     "mlv_texture2d_create": (_I, [_P, _U32, _U32, _I, _U32, _P, _PP]),
     "mlv_texture2d_destroy": (_I, [_P, _P]),
+    "mlv_texture2d_generate_mips": (_I, [_P, _P]),
     "mlv_texture2d_srgb_to_linear": (_I, [_P, _P]),
     "mlv_texture2d_readback": (_I, [_P, _P, _P]),
     "mlv_texture2d_device_ptr": (_P, [_P]),
@@ -131,6 +132,7 @@ SIGNATURES = {
     "mlv_rs_set_viewport": (_I, [_P, C.POINTER(Viewport)]),
     "mlv_ps_set_shader": (_I, [_P, _I]),
     "mlv_ps_set_shader_resource": (_I, [_P, _U32, _P]),
+    "mlv_ps_set_constant_buffer": (_I, [_P, _U32, _P]),
     "mlv_om_set_render_targets": (_I, [_P, _P, _P]),
     "mlv_om_set_depth_func": (_I, [_P, _I]),
     "mlv_om_set_visibility_buffer": (_I, [_P, _P, _P]),

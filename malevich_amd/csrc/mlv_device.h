@@ -204,6 +204,68 @@ __device__ __forceinline__ float4 latlon(const TexDesc &t, float dx, float dy, f
     return bilinear(t, uv_x, uv_y);
 }
 
+// ------------------------------------------------------------ mip levels
+//
+// Extension: the reference's Texture2D has one level (csc.h:20-24). A texture here may hold a
+// chain of levels (level 0 first, level l of max(w >> l, 1) x max(h >> l, 1) texels), and the
+// extension shaders pick one level per pixel, nearest to the pixel's footprint, then filter it
+// bilinearly as csc.h:103-161 does. The footprint compares the sampled coordinate at the pixel
+// corner with those at the next corners right and below (the registers interpolated there with
+// the same triangle's planes), in level-0 texels; level k is taken when rho^2 >= 2^(2k-1), i.e.
+// rho >= 2^(k - 1/2), which needs no logarithm, so every step is an IEEE f32 operation.
+
+__device__ __forceinline__ uint32_t texel_bytes(uint32_t format) {
+    return format == FMT_RGBA8 ? 4u : (format == FMT_RGBA16F ? 8u : 16u);
+}
+
+// Level l of a texture's chain as a one-level texture.
+__device__ __forceinline__ TexDesc tex_level(const TexDesc &t, uint32_t l) {
+    uint32_t w = t.width, h = t.height;
+    size_t off = 0;
+    for (uint32_t k = 0; k < l; ++k) {
+        off += (size_t)w * h;
+        w = w > 1u ? w >> 1 : 1u;
+        h = h > 1u ? h >> 1 : 1u;
+    }
+    TexDesc r = t;
+    r.data = (const uint8_t *)t.data + off * texel_bytes(t.format);
+    r.width = w;
+    r.height = h;
+    r.mip_levels = 1u;
+    return r;
+}
+
+__device__ __forceinline__ uint32_t nearest_level(float rho2, uint32_t levels) {
+    uint32_t lvl = 0u;
+    float thr = 2.0f;  // 2^(2k-1) for k = 1
+    for (uint32_t k = 1; k < levels; ++k) {
+        if (rho2 >= thr) lvl = k;
+        thr *= 4.0f;
+    }
+    return lvl;
+}
+
+// rho^2 of a uv footprint: uv at the pixel corner (u, v) and at the next corners right (ux, vx)
+// and below (uy, vy), in level-0 texels.
+__device__ __forceinline__ float uv_footprint(const TexDesc &t, float u, float v, float ux, float vx, float uy,
+                                              float vy) {
+    const float tw = (float)t.width, th = (float)t.height;
+    const float dux = (ux - u) * tw, dvx = (vx - v) * th, duy = (uy - u) * tw, dvy = (vy - v) * th;
+    const float rx = dux * dux + dvx * dvx, ry = duy * duy + dvy * dvy;
+    return rx > ry ? rx : ry;
+}
+
+// rho^2 of a direction footprint on an equirect map (2 pi radians across its width): the chord
+// between the unit directions at the pixel corner and at the next corners, in level-0 texels.
+__device__ __forceinline__ float dir_footprint(const TexDesc &t, float dx, float dy, float dz, float xx, float xy,
+                                               float xz, float yx, float yy, float yz) {
+    const float ax = xx - dx, ay = xy - dy, az = xz - dz, bx = yx - dx, by = yy - dy, bz = yz - dz;
+    const float k = (float)t.width * 0.159154943f;  // texels per radian
+    const float k2 = k * k;
+    const float rx = dot3(ax, ay, az, ax, ay, az) * k2, ry = dot3(bx, by, bz, bx, by, bz) * k2;
+    return rx > ry ? rx : ry;
+}
+
 // ------------------------------------------------------------- shaders
 
 // Floats of constant buffer slot 0 that each vertex shader reads: the world-view-projection
@@ -270,6 +332,57 @@ __device__ __forceinline__ void pixel_shader(int ps, const TexDesc &tex, float4 
         r = a1.x;
         g = a1.y;
         b = a1.z;
+    }
+}
+
+// Extension pixel shaders. a1/a2 are registers 1 and 2 at the pixel corner, x1/x2 and y1/y2 the
+// same registers at the next corners right and below (for the footprints).
+//  PS_BASIC_LAMBERT (basic_ps.c:16-25 with a directional light): the texture (slot 0) on the level
+//   nearest the UV footprint; c = albedo * (light_colour * max(dot(normalize(NORMAL), L), 0) +
+//   ambient); basic_ps's sRGB curve. Constant buffer: L.xyz _, light_colour.rgb _, ambient.rgb _.
+//   With light_colour 0 and ambient 1 it is basic_ps exactly.
+//  PS_ENV_TEXTURED (env_lighting_ps.c:13-22 with an albedo texture): the equirect map (slot 0) at
+//   normalize(NORMAL) on the level nearest the direction footprint, times the albedo texture
+//   (slot 1) at UV on the level nearest the UV footprint; env_lighting_ps's tone curve. With a
+//   one-level map and an albedo of 1 it is env_lighting_ps exactly.
+__device__ __forceinline__ void ext_pixel_shader(const DrawRec &D, float4 a1, float4 a2, float4 x1, float4 x2,
+                                                 float4 y1, float4 y2, float &r, float &g, float &b) {
+    const float u = a1.w, v = a2.x;
+    if (D.ps == PS_BASIC_LAMBERT) {
+        const TexDesc &t = D.ps_tex;
+        const uint32_t lvl =
+            t.mip_levels > 1u ? nearest_level(uv_footprint(t, u, v, x1.w, x2.x, y1.w, y2.x), t.mip_levels) : 0u;
+        const float4 c = bilinear(lvl ? tex_level(t, lvl) : t, u, v);
+        float nx, ny, nz;
+        normalize3(a1.x, a1.y, a1.z, nx, ny, nz);
+        const float *k = D.ps_cb;
+        float ndl = dot3(nx, ny, nz, k[0], k[1], k[2]);
+        ndl = ndl > 0.0f ? ndl : 0.0f;
+        float lr = k[4] * ndl, lg = k[5] * ndl, lb = k[6] * ndl;
+        lr = lr + k[8];
+        lg = lg + k[9];
+        lb = lb + k[10];
+        r = srgb_approx(c.x * lr);
+        g = srgb_approx(c.y * lg);
+        b = srgb_approx(c.z * lb);
+    } else {  // PS_ENV_TEXTURED
+        const TexDesc &m = D.ps_tex, &t = D.ps_tex1;
+        float dx, dy, dz;
+        normalize3(a1.x, a1.y, a1.z, dx, dy, dz);
+        uint32_t lm = 0u;
+        if (m.mip_levels > 1u) {
+            float ax, ay, az, bx, by, bz;
+            normalize3(x1.x, x1.y, x1.z, ax, ay, az);
+            normalize3(y1.x, y1.y, y1.z, bx, by, bz);
+            lm = nearest_level(dir_footprint(m, dx, dy, dz, ax, ay, az, bx, by, bz), m.mip_levels);
+        }
+        const float4 e = latlon(lm ? tex_level(m, lm) : m, dx, dy, dz);
+        const uint32_t lt =
+            t.mip_levels > 1u ? nearest_level(uv_footprint(t, u, v, x1.w, x2.x, y1.w, y2.x), t.mip_levels) : 0u;
+        const float4 c = bilinear(lt ? tex_level(t, lt) : t, u, v);
+        r = tonemap(c.x * e.x);
+        g = tonemap(c.y * e.y);
+        b = tonemap(c.z * e.z);
     }
 }
 

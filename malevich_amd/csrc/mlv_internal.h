@@ -41,7 +41,9 @@ constexpr uint32_t kHeavyMinDefault = 0xFFFFFFFFu, kHeavyBlocksDefault = 1024, k
 constexpr uint32_t kTileStageDefault = 256;
 
 enum VsKind { VS_BASIC = 1, VS_VERTEX_LIGHTING = 2, VS_PASSTHROUGH = 3, VS_FULLSCREEN = 4 };
-enum PsKind { PS_BASIC = 1, PS_ENV_LIGHTING = 2, PS_PASSTHROUGH = 3 };
+// PS_BASIC_LAMBERT and PS_ENV_TEXTURED are extensions (mip-mapped fetch, a light, a second
+// texture): they read the registers at the next pixel corners too (footprints for the mip level).
+enum PsKind { PS_BASIC = 1, PS_ENV_LIGHTING = 2, PS_PASSTHROUGH = 3, PS_BASIC_LAMBERT = 4, PS_ENV_TEXTURED = 5 };
 enum TexFormat { FMT_RGBA8 = 1, FMT_RGBA32F = 2, FMT_RGBA16F = 3 };
 enum DepthFunc { DEPTH_GE = 0, DEPTH_LE = 1 };
 
@@ -51,16 +53,21 @@ enum OverflowBits : uint32_t {
     OVF_PAIRS = 4u,       // more (triangle, tile) pairs than the bin-list arena holds (slow path ran)
 };
 
+// A texture and its mip chain: mip_levels levels concatenated, level 0 first, level l of
+// max(width >> l, 1) x max(height >> l, 1) texels.
 struct TexDesc {
     const void *data;
     uint32_t width, height, format, mip_levels;
 };
+constexpr uint32_t kPsConstants = 12;  // floats of the extension pixel shaders' constant buffer
 
 struct DrawRec {
     const uint32_t *ib;           // index buffer + IA offset + start_index
     const uint8_t *vb;            // vertex buffer + IA offset
     const float *cb;              // constant buffer slot 0 (48 floats)
     TexDesc vs_tex, ps_tex;
+    TexDesc ps_tex1;              // pixel shader resource slot 1 (extension shaders)
+    const float *ps_cb;           // pixel shader constant buffer slot 0 (extension shaders, kPsConstants floats)
     float screen[4][4];           // screen_from_ndc of the draw's viewport (main.c:826-831)
     uint32_t stride, vertex_count, tri_count;
     int32_t base_vertex;
@@ -304,6 +311,8 @@ hipError_t launch_fill_u32(uint32_t *dst, uint32_t value, size_t n, hipStream_t 
 // bytes and is 4-byte aligned
 hipError_t launch_present_rgb(const uint32_t *fb, uint8_t *rgb, size_t n_pixels, hipStream_t s);
 hipError_t launch_srgb_to_linear_rgba8(uint32_t *texels, size_t n, hipStream_t s);
+// levels 1 .. mip_levels-1 of the texture's chain from level 0, in order
+hipError_t launch_generate_mips(const TexDesc &t, hipStream_t s);
 hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s);
 // The flush in four stages, in this order: vertex, setup (cull/clip/ordered compaction),
 // binning (ordered tile lists), tile (raster + early-Z + shading + output merger).

@@ -1384,6 +1384,66 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, con
 
 constexpr uint32_t kHeavyDraws = 1;
 
+// Perspective-correct barycentrics of pixel corner (x, y) in an assembled triangle: the quantised
+// affine barycentrics (E >> 8) * one_over_area (main.c:1083-1102), then main.c:1106-1115.
+__device__ __forceinline__ void persp_bary(const TriVis &T, const TriShade &S, int32_t x, int32_t y, float &pbx,
+                                           float &pby) {
+    const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], x) + (uint32_t)__mul24(T.b16[1], y);
+    const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], x) + (uint32_t)__mul24(T.b16[2], y);
+    const float bx = (float)((int32_t)e1 >> 8) * T.ooa;
+    const float by = (float)((int32_t)e2 >> 8) * T.ooa;
+    float den = 1.0f - (bx + by);
+    den = den * S.rw[0];
+    den = den + bx * S.rw[1];
+    den = den + by * S.rw[2];
+    den = 1.0f / den;
+    pbx = (bx * S.rw[1]) * den;
+    pby = (by * S.rw[2]) * den;
+}
+
+// v0 + (v1 - v0) * pbx + (v2 - v0) * pby per component, main.c:1118-1160
+__device__ __forceinline__ float4 interp_reg(float4 c0, float4 c1, float4 c2, float pbx, float pby) {
+#define MLV_INTERP(q) (((c0.q) + ((c1.q) - (c0.q)) * pbx) + ((c2.q) - (c0.q)) * pby)
+    return make_float4(MLV_INTERP(x), MLV_INTERP(y), MLV_INTERP(z), MLV_INTERP(w));
+#undef MLV_INTERP
+}
+
+// The pixel shader stage for one pixel's winner (main.c:1106-1181): interpolated registers 1-2,
+// the draw's pixel shader, the output encode, and the depth and visibility writes. The extension
+// shaders also get the registers at the next corners right and below (x + 1, y + 1 on the same
+// triangle's planes, like the neighbouring lanes of a 2x2 quad) for their mip footprints; the
+// corners of an integer pixel grid need no edge or bounds test to be evaluated there.
+__device__ __forceinline__ void shade_winner(const BatchParams &p, const ViewRec &V, uint32_t best, float depth,
+                                             int32_t px, int32_t py, size_t pix) {
+    const TriVis T = p.tvis[best];
+    const TriShade S = p.tshade[best];
+    const DrawRec &D = p.draws[T.draw];
+    float pbx, pby;
+    persp_bary(T, S, px, py, pbx, pby);
+    const float4 a0 = p.vattr[2u * S.row[0]], b0 = p.vattr[2u * S.row[0] + 1u];
+    const float4 a1 = p.vattr[2u * S.row[1]], b1 = p.vattr[2u * S.row[1] + 1u];
+    const float4 a2 = p.vattr[2u * S.row[2]], b2 = p.vattr[2u * S.row[2] + 1u];
+    const float4 r1 = interp_reg(a0, a1, a2, pbx, pby);
+    const float4 r2 = interp_reg(b0, b1, b2, pbx, pby);
+    float cr, cg, cb;
+    if (D.ps >= PS_BASIC_LAMBERT) {
+        float qx, qy, sx, sy;
+        persp_bary(T, S, px + 1, py, qx, qy);
+        persp_bary(T, S, px, py + 1, sx, sy);
+        ext_pixel_shader(D, r1, r2, interp_reg(a0, a1, a2, qx, qy), interp_reg(b0, b1, b2, qx, qy),
+                         interp_reg(a0, a1, a2, sx, sy), interp_reg(b0, b1, b2, sx, sy), cr, cg, cb);
+    } else {
+        pixel_shader(D.ps, D.ps_tex, r1, r2, cr, cg, cb);
+    }
+    V.color[pix] = encode_output(cr, cg, cb);
+    V.depth[pix] = depth;
+    if (V.vis_tri) {
+        V.vis_tri[pix] = best - p.asm_base[T.draw];  // numbered within the draw
+        V.vis_draw[pix] = D.draw_tag;
+    }
+    if (V.vis_prim) V.vis_prim[pix] = S.prim;
+}
+
 __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g) {
     const uint32_t a = p.tile_start[g], e = p.tile_cnt[g];
     if (e - a <= p.heavy_min) return false;
@@ -1461,41 +1521,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
         __syncthreads();
     }
 
-    if (st.best >= 0) {
-        const TriVis T = p.tvis[st.best];
-        const TriShade S = p.tshade[st.best];
-        const DrawRec &D = p.draws[T.draw];
-        const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
-        const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
-        const float bx = (float)((int32_t)e1 >> 8) * T.ooa;
-        const float by = (float)((int32_t)e2 >> 8) * T.ooa;
-        // perspective correction, main.c:1106-1115
-        float den = 1.0f - (bx + by);
-        den = den * S.rw[0];
-        den = den + bx * S.rw[1];
-        den = den + by * S.rw[2];
-        den = 1.0f / den;
-        const float pbx = (bx * S.rw[1]) * den;
-        const float pby = (by * S.rw[2]) * den;
-        const float4 a0 = p.vattr[2u * S.row[0]], b0 = p.vattr[2u * S.row[0] + 1u];
-        const float4 a1 = p.vattr[2u * S.row[1]], b1 = p.vattr[2u * S.row[1] + 1u];
-        const float4 a2 = p.vattr[2u * S.row[2]], b2 = p.vattr[2u * S.row[2] + 1u];
-#define MLV_INTERP(c0, c1, c2) (((c0) + ((c1) - (c0)) * pbx) + ((c2) - (c0)) * pby)
-        const float4 r1 = make_float4(MLV_INTERP(a0.x, a1.x, a2.x), MLV_INTERP(a0.y, a1.y, a2.y),
-                                      MLV_INTERP(a0.z, a1.z, a2.z), MLV_INTERP(a0.w, a1.w, a2.w));
-        const float4 r2 = make_float4(MLV_INTERP(b0.x, b1.x, b2.x), MLV_INTERP(b0.y, b1.y, b2.y),
-                                      MLV_INTERP(b0.z, b1.z, b2.z), MLV_INTERP(b0.w, b1.w, b2.w));
-#undef MLV_INTERP
-        float cr, cg, cb;
-        pixel_shader(D.ps, D.ps_tex, r1, r2, cr, cg, cb);
-        V.color[pix] = encode_output(cr, cg, cb);
-        V.depth[pix] = st.d;
-        if (V.vis_tri) {
-            V.vis_tri[pix] = (uint32_t)st.best - p.asm_base[T.draw];  // numbered within the draw
-            V.vis_draw[pix] = D.draw_tag;
-        }
-        if (V.vis_prim) V.vis_prim[pix] = S.prim;
-    }
+    if (st.best >= 0) shade_winner(p, V, (uint32_t)st.best, st.d, px, py, pix);
 
     if (lane == 0u) s_active[w] = 0u;
     if (sub_valid && st.cur_draw >= 0) {
@@ -1605,40 +1631,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
             st.best = bq;
         }
     }
-    if (st.best >= 0) {
-        const TriVis T = p.tvis[st.best];
-        const TriShade S = p.tshade[st.best];
-        const DrawRec &D = p.draws[T.draw];
-        const uint32_t e1 = (uint32_t)T.c[1] + (uint32_t)__mul24(T.a16[1], px) + (uint32_t)__mul24(T.b16[1], py);
-        const uint32_t e2 = (uint32_t)T.c[2] + (uint32_t)__mul24(T.a16[2], px) + (uint32_t)__mul24(T.b16[2], py);
-        const float bx = (float)((int32_t)e1 >> 8) * T.ooa;
-        const float by = (float)((int32_t)e2 >> 8) * T.ooa;
-        float den = 1.0f - (bx + by);  // main.c:1106-1115
-        den = den * S.rw[0];
-        den = den + bx * S.rw[1];
-        den = den + by * S.rw[2];
-        den = 1.0f / den;
-        const float pbx = (bx * S.rw[1]) * den;
-        const float pby = (by * S.rw[2]) * den;
-        const float4 a0 = p.vattr[2u * S.row[0]], b0 = p.vattr[2u * S.row[0] + 1u];
-        const float4 a1 = p.vattr[2u * S.row[1]], b1 = p.vattr[2u * S.row[1] + 1u];
-        const float4 a2 = p.vattr[2u * S.row[2]], b2 = p.vattr[2u * S.row[2] + 1u];
-#define MLV_INTERP(c0, c1, c2) (((c0) + ((c1) - (c0)) * pbx) + ((c2) - (c0)) * pby)
-        const float4 r1 = make_float4(MLV_INTERP(a0.x, a1.x, a2.x), MLV_INTERP(a0.y, a1.y, a2.y),
-                                      MLV_INTERP(a0.z, a1.z, a2.z), MLV_INTERP(a0.w, a1.w, a2.w));
-        const float4 r2 = make_float4(MLV_INTERP(b0.x, b1.x, b2.x), MLV_INTERP(b0.y, b1.y, b2.y),
-                                      MLV_INTERP(b0.z, b1.z, b2.z), MLV_INTERP(b0.w, b1.w, b2.w));
-#undef MLV_INTERP
-        float cr, cg, cb;
-        pixel_shader(D.ps, D.ps_tex, r1, r2, cr, cg, cb);
-        V.color[pix] = encode_output(cr, cg, cb);
-        V.depth[pix] = st.d;
-        if (V.vis_tri) {
-            V.vis_tri[pix] = (uint32_t)st.best - p.asm_base[T.draw];  // numbered within the draw
-            V.vis_draw[pix] = D.draw_tag;
-        }
-        if (V.vis_prim) V.vis_prim[pix] = S.prim;
-    }
+    if (st.best >= 0) shade_winner(p, V, (uint32_t)st.best, st.d, px, py, pix);
     if (seen) {
         const float m = tile_hiz<GE>(inside, st.d);
         if (lane == 0u) {
@@ -1771,6 +1764,66 @@ __global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *so
 // Sum of the per-tile activity counts into the stats (one atomic per block).
 
 // ------------------------------------------------------------ launchers
+
+// Level l + 1 of a texture's chain from level l: each texel the average of its 2x2 window (2x1
+// or 1x2 once a side of level l is 1; an odd side drops its last row or column). RGBA8 averages
+// each byte in integers, rounding halves up; the float formats sum the window in row order in
+// f32 and divide by its area (RGBA16F then rounds to half, nearest-even).
+__global__ __launch_bounds__(256) void k_mip_level(TexDesc t, uint32_t l) {
+    const TexDesc src = tex_level(t, l), dst = tex_level(t, l + 1u);
+    const uint32_t W = dst.width, H = dst.height, sw = src.width;
+    const uint32_t kw = src.width > 1u ? 2u : 1u, kh = src.height > 1u ? 2u : 1u, n = kw * kh;
+    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < W * H; i += gridDim.x * 256u) {
+        const uint32_t x = i % W, y = i / W;
+        if (t.format == FMT_RGBA8) {
+            const uint32_t *q = (const uint32_t *)src.data;
+            uint32_t acc[4] = {0u, 0u, 0u, 0u};
+            for (uint32_t dy = 0; dy < kh; ++dy)
+                for (uint32_t dx = 0; dx < kw; ++dx) {
+                    const uint32_t c = q[(size_t)(kh * y + dy) * sw + kw * x + dx];
+#pragma unroll
+                    for (int k = 0; k < 4; ++k) acc[k] += (c >> (8 * k)) & 255u;
+                }
+            uint32_t o = 0u;
+#pragma unroll
+            for (int k = 0; k < 4; ++k) o |= ((acc[k] + n / 2u) / n) << (8 * k);
+            ((uint32_t *)dst.data)[i] = o;
+        } else {
+            float4 acc = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (uint32_t dy = 0; dy < kh; ++dy)
+                for (uint32_t dx = 0; dx < kw; ++dx) {
+                    const size_t j = (size_t)(kh * y + dy) * sw + kw * x + dx;
+                    const float4 c = t.format == FMT_RGBA32F ? ((const float4 *)src.data)[j]
+                                                             : decode_rgba16f(((const ushort4 *)src.data)[j]);
+                    acc.x += c.x;
+                    acc.y += c.y;
+                    acc.z += c.z;
+                    acc.w += c.w;
+                }
+            const float a = (float)n;
+            const float4 o = make_float4(acc.x / a, acc.y / a, acc.z / a, acc.w / a);
+            if (t.format == FMT_RGBA32F) {
+                ((float4 *)dst.data)[i] = o;
+            } else {
+                ((ushort4 *)dst.data)[i] = make_ushort4(__half_as_ushort(__float2half_rn(o.x)),
+                                                        __half_as_ushort(__float2half_rn(o.y)),
+                                                        __half_as_ushort(__float2half_rn(o.z)),
+                                                        __half_as_ushort(__float2half_rn(o.w)));
+            }
+        }
+    }
+}
+
+hipError_t launch_generate_mips(const TexDesc &t, hipStream_t s) {
+    uint32_t w = t.width, h = t.height;
+    for (uint32_t l = 0; l + 1u < t.mip_levels; ++l) {
+        w = w > 1u ? w >> 1 : 1u;
+        h = h > 1u ? h >> 1 : 1u;
+        const uint32_t n = w * h;
+        hipLaunchKernelGGL(k_mip_level, dim3(std::min<uint32_t>((n + 255u) / 256u, 4096u)), dim3(256), 0, s, t, l);
+    }
+    return hipGetLastError();
+}
 
 hipError_t launch_present_rgb(const uint32_t *fb, uint8_t *rgb, size_t n_pixels, hipStream_t s) {
     const size_t quads = (n_pixels + 3u) / 4u;

@@ -116,6 +116,7 @@ struct mlv_context {
     mlv_buffer *vs_cb[16] = {};
     mlv_texture2d *vs_srv[16] = {};
     mlv_texture2d *ps_srv[16] = {};
+    mlv_buffer *ps_cb[16] = {};  // extension: the reference's PS stage has no constant buffers (main.c:98-101)
     mlv_viewport vp{};
     bool vp_set = false;
     mlv_render_target *rt = nullptr;
@@ -734,6 +735,8 @@ MLV_API int mlv_buffer_destroy(mlv_context *ctx, mlv_buffer *buf) {
     if (buf->owned) (void)hipFree(buf->ptr);
     for (auto &c : ctx->vs_cb)
         if (c == buf) c = nullptr;
+    for (auto &c : ctx->ps_cb)
+        if (c == buf) c = nullptr;
     if (ctx->ib == buf) ctx->ib = nullptr;
     if (ctx->vb == buf) ctx->vb = nullptr;
     delete buf;
@@ -751,14 +754,48 @@ static size_t texel_bytes(int fmt) {
     }
 }
 
+static bool is_float_tex(const mlv_texture2d *t) {
+    return t && (t->format == MLV_FORMAT_RGBA32F || t->format == MLV_FORMAT_RGBA16F);
+}
+
+static TexDesc tex_desc(const mlv_texture2d *t) {
+    TexDesc d{};
+    if (t) {
+        d.data = t->ptr;
+        d.width = t->width;
+        d.height = t->height;
+        d.format = t->format;
+        d.mip_levels = t->mip_levels;
+    }
+    return d;
+}
+
+// Texels of the first `levels` levels of a width x height chain (level l is max(w >> l, 1) x
+// max(h >> l, 1)); 0 when levels exceeds the full chain's count.
+static size_t chain_texels(uint32_t width, uint32_t height, uint32_t levels) {
+    size_t n = 0;
+    uint32_t w = width, h = height;
+    for (uint32_t l = 0; l < levels; ++l) {
+        n += (size_t)w * h;
+        if (w == 1u && h == 1u && l + 1u < levels) return 0;  // no level after 1x1
+        w = w > 1u ? w >> 1 : 1u;
+        h = h > 1u ? h >> 1 : 1u;
+    }
+    return n;
+}
+
 MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t height, int format, uint32_t mip_levels,
                                  const void *host_data, mlv_texture2d **out) {
     const size_t tb = texel_bytes(format);
     if (!ctx || !out || !width || !height || !tb || width > 65536u || height > 65536u)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_texture2d_create: bad argument");
     if (mip_levels == 0) mip_levels = 1;
+    const size_t texels = mip_levels <= kMaxMipLevels ? chain_texels(width, height, mip_levels) : 0;
+    if (!texels)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "mlv_texture2d_create: %u levels do not fit a %ux%u texture",
+                    mip_levels, width, height);
     RC_OR_RETURN(set_device(ctx));
-    const size_t bytes = (size_t)width * height * tb;
+    const size_t bytes = texels * tb;
     void *p = nullptr;
     HIP_OR_FAIL(ctx, hipMalloc(&p, bytes));
     if (host_data) {
@@ -778,13 +815,22 @@ MLV_API int mlv_texture2d_create(mlv_context *ctx, uint32_t width, uint32_t heig
     return MLV_OK;
 }
 
+MLV_API int mlv_texture2d_generate_mips(mlv_context *ctx, mlv_texture2d *tex) {
+    if (!ctx || !tex) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "texture2d_generate_mips: bad argument");
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // recorded draws read the texture as it was
+    TexDesc d = tex_desc(tex);
+    HIP_OR_FAIL(ctx, launch_generate_mips(d, ctx->stream));
+    return MLV_OK;
+}
+
 MLV_API int mlv_texture2d_srgb_to_linear(mlv_context *ctx, mlv_texture2d *tex) {
     if (!ctx || !tex) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "texture2d_srgb_to_linear: bad argument");
     if (tex->format != MLV_FORMAT_RGBA8_UNORM)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "texture2d_srgb_to_linear: RGBA8 textures only (main.c:546)");
     RC_OR_RETURN(set_device(ctx));
     RC_OR_RETURN(flush(ctx));  // recorded draws read the texture as it was
-    HIP_OR_FAIL(ctx, launch_srgb_to_linear_rgba8((uint32_t *)tex->ptr, (size_t)tex->width * tex->height, ctx->stream));
+    HIP_OR_FAIL(ctx, launch_srgb_to_linear_rgba8((uint32_t *)tex->ptr, tex->bytes / 4, ctx->stream));  // every level
     return MLV_OK;
 }
 
@@ -937,7 +983,7 @@ MLV_API int mlv_rs_set_viewport(mlv_context *ctx, const mlv_viewport *vp) {
 }
 
 MLV_API int mlv_ps_set_shader(mlv_context *ctx, int pixel_shader) {
-    if (!ctx || pixel_shader < MLV_PS_BASIC || pixel_shader > MLV_PS_PASSTHROUGH)
+    if (!ctx || pixel_shader < MLV_PS_BASIC || pixel_shader > MLV_PS_ENV_TEXTURED)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown pixel shader");
     ctx->ps = pixel_shader;
     return MLV_OK;
@@ -946,6 +992,12 @@ MLV_API int mlv_ps_set_shader(mlv_context *ctx, int pixel_shader) {
 MLV_API int mlv_ps_set_shader_resource(mlv_context *ctx, uint32_t slot, mlv_texture2d *tex) {
     if (!ctx || slot >= 16) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "bad shader resource slot");
     ctx->ps_srv[slot] = tex;
+    return MLV_OK;
+}
+
+MLV_API int mlv_ps_set_constant_buffer(mlv_context *ctx, uint32_t slot, mlv_buffer *cb) {
+    if (!ctx || slot >= 16) return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "bad constant buffer slot");
+    ctx->ps_cb[slot] = cb;
     return MLV_OK;
 }
 
@@ -1051,22 +1103,6 @@ MLV_API int mlv_clear_depth_stencil_view(mlv_context *ctx, mlv_depth_stencil *ds
     return exec_clear_depth(ctx, c);
 }
 
-static bool is_float_tex(const mlv_texture2d *t) {
-    return t && (t->format == MLV_FORMAT_RGBA32F || t->format == MLV_FORMAT_RGBA16F);
-}
-
-static TexDesc tex_desc(const mlv_texture2d *t) {
-    TexDesc d{};
-    if (t) {
-        d.data = t->ptr;
-        d.width = t->width;
-        d.height = t->height;
-        d.format = t->format;
-        d.mip_levels = t->mip_levels;
-    }
-    return d;
-}
-
 // Adds a draw to the pending batch, flushing first where the batch cannot take it.
 static int submit_draw(mlv_context *ctx, const Cmd &c) {
     const PendingDraw &pd = c.pd;
@@ -1119,6 +1155,13 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
         return fail(ctx, MLV_ERROR_INVALID_STATE, "env_lighting_ps needs a float RGBA texture in slot 0");
     if (ctx->ps == MLV_PS_BASIC && !ctx->ps_srv[0])
         return fail(ctx, MLV_ERROR_INVALID_STATE, "basic_ps needs a texture in slot 0");
+    if (ctx->ps == MLV_PS_BASIC_LAMBERT &&
+        (!ctx->ps_srv[0] || !ctx->ps_cb[0] || ctx->ps_cb[0]->bytes < kPsConstants * sizeof(float)))
+        return fail(ctx, MLV_ERROR_INVALID_STATE,
+                    "the Lambert shader needs a texture in slot 0 and a 12-float constant buffer in slot 0");
+    if (ctx->ps == MLV_PS_ENV_TEXTURED && (!is_float_tex(ctx->ps_srv[0]) || !ctx->ps_srv[1]))
+        return fail(ctx, MLV_ERROR_INVALID_STATE,
+                    "the textured environment shader needs a float RGBA map in slot 0 and a texture in slot 1");
     RC_OR_RETURN(set_device(ctx));
 
     const uint32_t T = index_count / 3u;
@@ -1146,6 +1189,8 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
     r.cb = ctx->vs_cb[0] ? (const float *)ctx->vs_cb[0]->ptr : nullptr;
     r.vs_tex = tex_desc(ctx->vs_srv[0]);
     r.ps_tex = tex_desc(ctx->ps_srv[0]);
+    r.ps_tex1 = tex_desc(ctx->ps_srv[1]);
+    r.ps_cb = ctx->ps_cb[0] ? (const float *)ctx->ps_cb[0]->ptr : nullptr;
     screen_matrix(ctx->vp, r.screen);
     r.stride = ctx->vb_stride;
     r.vertex_count = V;

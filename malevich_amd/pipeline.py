@@ -55,9 +55,40 @@ class Buffer:
             self.handle = None
 
 
+def _level_sizes(width: int, height: int, levels: int) -> list:
+    out = [(width, height)]
+    while len(out) < levels:
+        w, h = out[-1]
+        out.append((max(w >> 1, 1), max(h >> 1, 1)))
+    return out
+
+
+def _texel_shape(fmt: int, w: int, h: int):
+    return ((h, w), np.uint32) if fmt == L.FORMAT_RGBA8_UNORM else \
+        ((h, w, 4), np.float32 if fmt == L.FORMAT_RGBA32F else np.float16)
+
+
 class Texture2D:
-    def __init__(self, dev, handle, width, height, fmt):
+    def __init__(self, dev, handle, width, height, fmt, mip_levels: int = 1):
         self.dev, self.handle, self.width, self.height, self.format = dev, handle, width, height, fmt
+        self.mip_levels = mip_levels
+
+    def generate_mips(self) -> None:
+        """Levels 1.. of the chain from level 0 on the device (2x2 box filter)."""
+        _check(self.dev.ctx, L.load().mlv_texture2d_generate_mips(self.dev.ctx, self.handle))
+
+    def read_levels(self) -> list:
+        """Every level of the chain, level 0 first, shaped as read() shapes level 0."""
+        sizes = _level_sizes(self.width, self.height, self.mip_levels)
+        shapes = [_texel_shape(self.format, w, h) for w, h in sizes]
+        flat = np.empty(sum(int(np.prod(s)) for s, _ in shapes), shapes[0][1])
+        _check(self.dev.ctx, L.load().mlv_texture2d_readback(self.dev.ctx, self.handle, flat.ctypes.data))
+        out, o = [], 0
+        for shape, _ in shapes:
+            n = int(np.prod(shape))
+            out.append(flat[o:o + n].reshape(shape))
+            o += n
+        return out
 
     def srgb_to_linear(self) -> None:
         """load_texture's sRGB pre-pass (main.c:546-558) on the device, in place (RGBA8 only)."""
@@ -65,6 +96,8 @@ class Texture2D:
 
     def read(self) -> np.ndarray:
         """Level-0 texels: (h, w) uint32 for RGBA8, (h, w, 4) float32 / float16 for the float formats."""
+        if self.mip_levels > 1:
+            return self.read_levels()[0]
         if self.format == L.FORMAT_RGBA8_UNORM:
             out = np.empty((self.height, self.width), np.uint32)
         elif self.format == L.FORMAT_RGBA32F:
@@ -181,17 +214,36 @@ class Device:
         _check(self.ctx, self.lib.mlv_buffer_wrap(self.ctx, kind, device_ptr, nbytes, C.byref(h)))
         return Buffer(self, h.value, nbytes, kind)
 
-    def create_texture2d(self, data: np.ndarray, fmt: int | None = None, mip_levels: int = 1) -> Texture2D:
-        """data: (h, w) uint32 RGBA8 (R in the low byte), (h, w, 4) float32 or float16."""
-        a = np.ascontiguousarray(data)
+    def create_texture2d(self, data, fmt: int | None = None, mip_levels: int = 1,
+                         generate_mips: bool = False) -> Texture2D:
+        """data: (h, w) uint32 RGBA8 (R in the low byte), (h, w, 4) float32 or float16: level 0, or a
+        list of every level (level 0 first). With mip_levels > 1 and level 0 alone, generate_mips
+        fills the other levels on the device."""
+        levels = [np.ascontiguousarray(x) for x in data] if isinstance(data, (list, tuple)) else \
+            [np.ascontiguousarray(data)]
+        a = levels[0]
         if fmt is None:
             fmt = {np.dtype(np.uint32): L.FORMAT_RGBA8_UNORM, np.dtype(np.float32): L.FORMAT_RGBA32F,
                    np.dtype(np.float16): L.FORMAT_RGBA16F}[a.dtype]
         h_, w_ = a.shape[:2]
+        if len(levels) > 1:
+            mip_levels = len(levels)
+        sizes = _level_sizes(w_, h_, mip_levels)
+        if len(levels) == 1 and mip_levels > 1:
+            if not generate_mips:
+                raise ValueError("mip_levels > 1 needs every level or generate_mips=True")
+            rest = [np.zeros(_texel_shape(fmt, w, h)[0], a.dtype) for w, h in sizes[1:]]
+            levels = levels + rest
+        for lv, (w, h) in zip(levels, sizes):
+            assert lv.shape[:2] == (h, w), (lv.shape, (h, w))
+        flat = np.concatenate([lv.reshape(-1) for lv in levels]) if len(levels) > 1 else a
         h = C.c_void_p()
-        _check(self.ctx, self.lib.mlv_texture2d_create(self.ctx, w_, h_, fmt, mip_levels, a.ctypes.data,
+        _check(self.ctx, self.lib.mlv_texture2d_create(self.ctx, w_, h_, fmt, mip_levels, flat.ctypes.data,
                                                        C.byref(h)))
-        return Texture2D(self, h.value, w_, h_, fmt)
+        t = Texture2D(self, h.value, w_, h_, fmt, mip_levels)
+        if generate_mips and mip_levels > 1:
+            t.generate_mips()
+        return t
 
     def create_render_target(self, width: int, height: int) -> RenderTarget:
         h = C.c_void_p()
@@ -235,6 +287,10 @@ class Device:
 
     def ps_set_shader_resource(self, slot: int, tex: Texture2D | None) -> None:
         _check(self.ctx, self.lib.mlv_ps_set_shader_resource(self.ctx, slot, tex.handle if tex else None))
+
+    def ps_set_constant_buffer(self, slot: int, buf: Buffer | None) -> None:
+        """Extension: the Lambert shader's light (12 floats: L.xyz _, colour.rgb _, ambient.rgb _)."""
+        _check(self.ctx, self.lib.mlv_ps_set_constant_buffer(self.ctx, slot, buf.handle if buf else None))
 
     # ---------------------------------------------------------------- OM
     def om_set_render_targets(self, rt: RenderTarget | None, ds: DepthStencil | None) -> None:

@@ -339,6 +339,26 @@ def c1_scene(width: int = 640, height: int = 360, seed: int = 1):
     return mesh, cb, tex
 
 
+def c1_light() -> np.ndarray:
+    """The 640x360 workload's directional light for the Lambert pixel shader (12 floats:
+    direction towards the light, colour, ambient, each padded to four): white light from the
+    camera's side and above, a dim ambient term."""
+    d = np.array([1.0, 0.5, 0.8])
+    d /= np.linalg.norm(d)
+    return np.array([d[0], d[1], d[2], 0.0, 1.0, 1.0, 1.0, 0.0, 0.15, 0.15, 0.15, 0.0], np.float32)
+
+
+def c2_textures(n_instances: int = 64, seed: int = 2, albedo_size: int = 1024):
+    """The 1080p workload's maps: a 256x128 RGBA16F irradiance map (vertex_lighting_vs, one level),
+    a 2048x1024 RGBA16F radiance map and one albedo_size^2 RGBA8 texture of uniform random bytes per
+    instance (level 0 of each; their mip chains are built by the device)."""
+    irr = smooth_env_map(256, 128, seed=seed + 16, lo=0.05, hi=2.0).astype(np.float16)
+    rad = smooth_env_map(2048, 1024, seed=seed + 17, lo=0.05, hi=2.0).astype(np.float16)
+    rng = np.random.default_rng(seed + 18)
+    albedo = [rng.integers(0, 2 ** 32, (albedo_size, albedo_size), dtype=np.uint32) for _ in range(n_instances)]
+    return irr, rad, albedo
+
+
 def random_rotation(rng) -> np.ndarray:
     q = rng.normal(size=4)
     q /= np.linalg.norm(q)

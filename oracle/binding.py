@@ -15,13 +15,14 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 VS_BASIC, VS_VERTEX_LIGHTING, VS_PASSTHROUGH, VS_FULLSCREEN = 0, 1, 2, 3
-PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH = 0, 1, 2
+PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH, PS_BASIC_LAMBERT, PS_ENV_TEXTURED = 0, 1, 2, 3, 4
 FMT_RGBA8, FMT_RGBA32F = 0, 1
 DEPTH_GE, DEPTH_LE = 0, 1
 
 
 class Texture(C.Structure):
-    _fields_ = [("data", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32)]
+    _fields_ = [("data", C.c_void_p), ("width", C.c_uint32), ("height", C.c_uint32), ("format", C.c_uint32),
+                ("levels", C.c_uint32)]
 
 
 class Viewport(C.Structure):
@@ -44,6 +45,8 @@ class DrawDesc(C.Structure):
         ("ps_srv0", Texture),
         ("viewport", Viewport),
         ("depth_func", C.c_int32),
+        ("ps_srv1", Texture),
+        ("ps_cb", C.c_void_p),
     ]
 
 
@@ -137,6 +140,9 @@ def lib() -> C.CDLL:
         L.orc_coverage_mask.argtypes = [P(C.c_int32), C.c_int32, C.c_int32]
         L.orc_coverage_mask.restype = C.c_uint64
         L.orc_bilinear.argtypes = [P(Texture), C.c_float, C.c_float, P(C.c_float)]
+        L.orc_nearest_level.argtypes = [C.c_float, C.c_uint32]
+        L.orc_nearest_level.restype = C.c_uint32
+        L.orc_bilinear_level.argtypes = [P(Texture), C.c_uint32, C.c_float, C.c_float, P(C.c_float)]
         L.orc_latlon.argtypes = [P(Texture), P(C.c_float), P(C.c_float)]
         L.orc_encode_output.argtypes = [P(C.c_float)]
         L.orc_encode_output.restype = C.c_uint32
@@ -169,17 +175,35 @@ def _ptr(a: np.ndarray | None) -> int | None:
     return None if a is None else a.ctypes.data
 
 
-def make_texture(arr: np.ndarray | None) -> Texture:
-    """arr: (h, w) uint32 RGBA8 or (h, w, 4) float32."""
+class MipTexture:
+    """A texture with a chain of levels for the oracle: the levels (level 0 first) concatenated,
+    RGBA8 as uint32 or float32 (RGBA16F texels are exact in float32)."""
+
+    def __init__(self, levels: list):
+        l0 = levels[0]
+        self.width, self.height = l0.shape[1], l0.shape[0]
+        self.levels = len(levels)
+        if l0.dtype == np.uint32:
+            self.flat = np.ascontiguousarray(np.concatenate([x.reshape(-1) for x in levels]))
+            self.format = FMT_RGBA8
+        else:
+            self.flat = np.ascontiguousarray(np.concatenate([x.reshape(-1, 4) for x in levels]).astype(np.float32))
+            self.format = FMT_RGBA32F
+
+
+def make_texture(arr) -> Texture:
+    """arr: (h, w) uint32 RGBA8, (h, w, 4) float32, or a MipTexture."""
     if arr is None:
-        return Texture(None, 0, 0, 0)
+        return Texture(None, 0, 0, 0, 1)
+    if isinstance(arr, MipTexture):
+        return Texture(arr.flat.ctypes.data, arr.width, arr.height, arr.format, arr.levels)
     arr = np.ascontiguousarray(arr)
     if arr.dtype == np.uint32:
         h, w = arr.shape
-        return Texture(arr.ctypes.data, w, h, FMT_RGBA8)
+        return Texture(arr.ctypes.data, w, h, FMT_RGBA8, 1)
     assert arr.dtype == np.float32 and arr.ndim == 3 and arr.shape[2] == 4
     h, w, _ = arr.shape
-    return Texture(arr.ctypes.data, w, h, FMT_RGBA32F)
+    return Texture(arr.ctypes.data, w, h, FMT_RGBA32F, 1)
 
 
 class RenderTarget:
@@ -210,20 +234,23 @@ class Draw:
     """Keeps the numpy inputs alive for the lifetime of the descriptor."""
 
     def __init__(self, indices, vertices, vs, ps, cb=None, vs_tex=None, ps_tex=None, viewport=None,
-                 start_index=0, base_vertex=0, index_count=None, depth_func=DEPTH_GE):
+                 start_index=0, base_vertex=0, index_count=None, depth_func=DEPTH_GE, ps_tex1=None, ps_cb=None):
         self.indices = np.ascontiguousarray(indices, np.uint32).ravel()
         self.vertices = np.ascontiguousarray(vertices, np.float32)
         assert self.vertices.ndim == 2
         self.cb = np.ascontiguousarray(cb if cb is not None else np.zeros(48), np.float32).ravel()
         assert self.cb.size == 48
-        self.vs_tex_arr = None if vs_tex is None else np.ascontiguousarray(vs_tex)
-        self.ps_tex_arr = None if ps_tex is None else np.ascontiguousarray(ps_tex)
+        keep = lambda t: t if t is None or isinstance(t, MipTexture) else np.ascontiguousarray(t)
+        self.vs_tex_arr, self.ps_tex_arr, self.ps_tex1_arr = keep(vs_tex), keep(ps_tex), keep(ps_tex1)
+        self.ps_cb = None if ps_cb is None else np.ascontiguousarray(ps_cb, np.float32).ravel()
+        assert self.ps_cb is None or self.ps_cb.size >= 12
         n = self.indices.size if index_count is None else index_count
         self.desc = DrawDesc(
             self.indices.ctypes.data, n, start_index, base_vertex,
             self.vertices.ctypes.data, self.vertices.shape[1] * 4, self.vertices.shape[0],
             vs, ps, self.cb.ctypes.data, make_texture(self.vs_tex_arr), make_texture(self.ps_tex_arr),
-            Viewport(*(viewport if viewport is not None else (0, 0, 0, 0, 0, 1))), depth_func)
+            Viewport(*(viewport if viewport is not None else (0, 0, 0, 0, 0, 1))), depth_func,
+            make_texture(self.ps_tex1_arr), None if self.ps_cb is None else self.ps_cb.ctypes.data)
 
 
 def draw_indexed(rt: RenderTarget, draw: Draw) -> None:
@@ -285,6 +312,17 @@ def sh9(n) -> np.ndarray:
 def coverage_mask(edges, ox: int, oy: int) -> int:
     e = (C.c_int32 * 9)(*np.asarray(edges, np.int64).astype(np.int32).ravel().tolist())
     return int(lib().orc_coverage_mask(e, ox, oy))
+
+
+def bilinear_level(tex, level: int, u: float, v: float) -> np.ndarray:
+    t = make_texture(tex)
+    out = (C.c_float * 4)()
+    lib().orc_bilinear_level(C.byref(t), level, C.c_float(u), C.c_float(v), out)
+    return np.array(out[:], np.float32)
+
+
+def nearest_level(rho2: float, levels: int) -> int:
+    return int(lib().orc_nearest_level(C.c_float(rho2), levels))
 
 
 def bilinear(tex: np.ndarray, u: float, v: float) -> np.ndarray:

@@ -218,6 +218,63 @@ void orc_latlon(const orc_texture *t, const float dir[3], float out[4]) {
     orc_bilinear(t, uv_x, uv_y, out);
 }
 
+/* ------------------------------------------------ mip levels (extension)
+ * The library's extension shaders pick one level of a texture's chain per pixel, nearest to the
+ * footprint of the sampled coordinate between the pixel corner and the next corners right and
+ * below; level k when rho^2 >= 2^(2k-1). Every step is an IEEE f32 operation in this order. */
+
+static orc_texture tex_level(const orc_texture *t, uint32_t l) {
+    uint32_t w = t->width, h = t->height;
+    size_t off = 0;
+    for (uint32_t k = 0; k < l; ++k) {
+        off += (size_t)w * h;
+        w = w > 1u ? w >> 1 : 1u;
+        h = h > 1u ? h >> 1 : 1u;
+    }
+    const size_t tb = t->format == ORC_FMT_RGBA8 ? 4 : 16;
+    orc_texture r = *t;
+    r.data = (const uint8_t *)t->data + off * tb;
+    r.width = w;
+    r.height = h;
+    r.levels = 1;
+    return r;
+}
+
+static uint32_t nearest_level(float rho2, uint32_t levels) {
+    uint32_t lvl = 0;
+    float thr = 2.0f;
+    for (uint32_t k = 1; k < levels; ++k) {
+        if (rho2 >= thr) lvl = k;
+        thr *= 4.0f;
+    }
+    return lvl;
+}
+
+static float uv_footprint(const orc_texture *t, float u, float v, float ux, float vx, float uy, float vy) {
+    const float tw = (float)t->width, th = (float)t->height;
+    const float dux = (ux - u) * tw, dvx = (vx - v) * th, duy = (uy - u) * tw, dvy = (vy - v) * th;
+    const float rx = dux * dux + dvx * dvx, ry = duy * duy + dvy * dvy;
+    return rx > ry ? rx : ry;
+}
+
+/* chord between unit directions, in level-0 texels of an equirect map (2 pi across its width) */
+static float dir_footprint(const orc_texture *t, const float d[3], const float x[3], const float y[3]) {
+    const float ax = x[0] - d[0], ay = x[1] - d[1], az = x[2] - d[2];
+    const float bx = y[0] - d[0], by = y[1] - d[1], bz = y[2] - d[2];
+    const float k = (float)t->width * 0.159154943f;
+    const float k2 = k * k;
+    const float rx = dot3(ax, ay, az, ax, ay, az) * k2, ry = dot3(bx, by, bz, bx, by, bz) * k2;
+    return rx > ry ? rx : ry;
+}
+
+uint32_t orc_nearest_level(float rho2, uint32_t levels) { return nearest_level(rho2, levels); }
+
+/* bilinear_u_x8 / bilinear_f_x8 on level `level` of a texture's chain. */
+void orc_bilinear_level(const orc_texture *t, uint32_t level, float u, float v, float out[4]) {
+    const orc_texture tl = tex_level(t, level);
+    orc_bilinear(&tl, u, v, out);
+}
+
 /* ------------------------------------------------------------- shaders */
 
 static const float *vertex_ptr(const orc_draw_desc *d, uint32_t vi) {
@@ -267,9 +324,51 @@ int orc_vertex_shader(const orc_draw_desc *d, uint32_t vi, float o[12]) {
     }
 }
 
-/* basic_ps.c:16-25, env_lighting_ps.c:13-22, passthrough_ps.c:13-19. in = 12 interpolated floats. */
-static void pixel_shader(const orc_draw_desc *d, const float in[12], float rgb[3]) {
+/* basic_ps.c:16-25, env_lighting_ps.c:13-22, passthrough_ps.c:13-19. in = 12 interpolated floats;
+ * inx / iny: the same at the next pixel corners right and below (extension shaders only). */
+static void pixel_shader(const orc_draw_desc *d, const float in[12], const float inx[12], const float iny[12],
+                         float rgb[3]) {
     switch (d->ps) {
+    case ORC_PS_BASIC_LAMBERT: {
+        /* basic_ps with a directional light: albedo * (light_colour * max(N.L, 0) + ambient) */
+        const orc_texture *t = &d->ps_srv0;
+        const uint32_t lvl =
+            t->levels > 1 ? nearest_level(uv_footprint(t, in[7], in[8], inx[7], inx[8], iny[7], iny[8]), t->levels) : 0;
+        const orc_texture tl = tex_level(t, lvl);
+        float c[4], n[3];
+        orc_bilinear(&tl, in[7], in[8], c);
+        normalize3(in[4], in[5], in[6], n);
+        const float *k = d->ps_cb;
+        float ndl = dot3(n[0], n[1], n[2], k[0], k[1], k[2]);
+        ndl = ndl > 0.0f ? ndl : 0.0f;
+        for (int j = 0; j < 3; ++j) {
+            float l = k[4 + j] * ndl;
+            l = l + k[8 + j];
+            rgb[j] = orc_srgb_from_linear_approx(c[j] * l);
+        }
+        break;
+    }
+    case ORC_PS_ENV_TEXTURED: {
+        /* env_lighting_ps times an albedo texture, both on the level nearest their footprints */
+        const orc_texture *m = &d->ps_srv0, *t = &d->ps_srv1;
+        float dir[3], e[4], c[4];
+        normalize3(in[4], in[5], in[6], dir);
+        uint32_t lm = 0;
+        if (m->levels > 1) {
+            float dx[3], dy[3];
+            normalize3(inx[4], inx[5], inx[6], dx);
+            normalize3(iny[4], iny[5], iny[6], dy);
+            lm = nearest_level(dir_footprint(m, dir, dx, dy), m->levels);
+        }
+        const orc_texture ml = tex_level(m, lm);
+        orc_latlon(&ml, dir, e);
+        const uint32_t lt =
+            t->levels > 1 ? nearest_level(uv_footprint(t, in[7], in[8], inx[7], inx[8], iny[7], iny[8]), t->levels) : 0;
+        const orc_texture tl = tex_level(t, lt);
+        orc_bilinear(&tl, in[7], in[8], c);
+        for (int j = 0; j < 3; ++j) rgb[j] = orc_tonemap(c[j] * e[j]);
+        break;
+    }
     case ORC_PS_BASIC: {
         float c[4];
         orc_bilinear(&d->ps_srv0, in[7], in[8], c);
@@ -368,8 +467,12 @@ static void set_edge(int32_t e[3], int32_t area, int32_t x0, int32_t y0, int32_t
 }
 
 static int is_valid_vs_ps(const orc_draw_desc *d) {
-    return d->vs >= ORC_VS_BASIC && d->vs <= ORC_VS_FULLSCREEN && d->ps >= ORC_PS_BASIC &&
-           d->ps <= ORC_PS_PASSTHROUGH;
+    if (!(d->vs >= ORC_VS_BASIC && d->vs <= ORC_VS_FULLSCREEN && d->ps >= ORC_PS_BASIC &&
+          d->ps <= ORC_PS_ENV_TEXTURED))
+        return 0;
+    if (d->ps == ORC_PS_BASIC_LAMBERT && (!d->ps_cb || !d->ps_srv0.data)) return 0;
+    if (d->ps == ORC_PS_ENV_TEXTURED && (!d->ps_srv0.data || !d->ps_srv1.data)) return 0;
+    return 1;
 }
 
 /* main.c:662-734 (IA + VS, de-indexed) and main.c:736-913 (primitive assembly) in input order. */
@@ -547,6 +650,32 @@ int orc_threads(void) {
 
 /* ------------------------------------------------------- rasterisation */
 
+static inline int32_t edge_at(const int32_t e[3], int32_t x, int32_t y);
+
+/* main.c:1083-1160 at pixel corner (x, y): quantised barycentrics, perspective correction,
+ * register 0 affine and the others perspective-correct. */
+static void interp_fragment(const orc_triangle *tr, int32_t x, int32_t y, float frag[12]) {
+    int32_t e1 = edge_at(tr->edge[1], x, y) >> (SUBPIX_BITS * 2);
+    int32_t e2 = edge_at(tr->edge[2], x, y) >> (SUBPIX_BITS * 2);
+    float bx = (float)e1 * tr->one_over_area;
+    float by = (float)e2 * tr->one_over_area;
+    float den = 1.0f - (bx + by);
+    den = den * tr->rw[0];
+    den = den + bx * tr->rw[1];
+    den = den + by * tr->rw[2];
+    den = 1.0f / den;
+    float pbx = (bx * tr->rw[1]) * den;
+    float pby = (by * tr->rw[2]) * den;
+    for (int reg = 0; reg < NREG; ++reg) {
+        float u = reg == 0 ? bx : pbx, w = reg == 0 ? by : pby;
+        for (int c = 0; c < 4; ++c) {
+            float v0 = tr->attr[0][4 * reg + c], v1 = tr->attr[1][4 * reg + c], v2 = tr->attr[2][4 * reg + c];
+            float tv0 = v0 + (v1 - v0) * u;
+            frag[4 * reg + c] = tv0 + (v2 - v0) * w;
+        }
+    }
+}
+
 static inline int32_t edge_at(const int32_t e[3], int32_t x, int32_t y) {
     /* main.c:1015-1025: ((a*x)<<4) + ((b*y)<<4) + c in i32 wrap */
     uint32_t v = ((uint32_t)e[0] * (uint32_t)x) << SUBPIX_BITS;
@@ -681,29 +810,7 @@ int orc_draw_indexed(orc_target *rt, const orc_draw_desc *d, orc_stats *st) {
                 int any = 0;
                 for (int l = 0; l < 8; ++l) {
                     const int32_t x = ox + l;
-                    /* main.c:1083-1102 quantised barycentrics */
-                    int32_t e1 = edge_at(tr->edge[1], x, y) >> (SUBPIX_BITS * 2);
-                    int32_t e2 = edge_at(tr->edge[2], x, y) >> (SUBPIX_BITS * 2);
-                    float bx = (float)e1 * tr->one_over_area;
-                    float by = (float)e2 * tr->one_over_area;
-                    /* main.c:1106-1115 perspective correction */
-                    float den = 1.0f - (bx + by);
-                    den = den * tr->rw[0];
-                    den = den + bx * tr->rw[1];
-                    den = den + by * tr->rw[2];
-                    den = 1.0f / den;
-                    float pbx = (bx * tr->rw[1]) * den;
-                    float pby = (by * tr->rw[2]) * den;
-                    /* main.c:1118-1160: reg0 affine, reg1.. perspective */
-                    for (int reg = 0; reg < NREG; ++reg) {
-                        float u = reg == 0 ? bx : pbx, w = reg == 0 ? by : pby;
-                        for (int c = 0; c < 4; ++c) {
-                            float v0 = tr->attr[0][4 * reg + c], v1 = tr->attr[1][4 * reg + c],
-                                  v2 = tr->attr[2][4 * reg + c];
-                            float tv0 = v0 + (v1 - v0) * u;
-                            frag[l][4 * reg + c] = tv0 + (v2 - v0) * w;
-                        }
-                    }
+                    interp_fragment(tr, x, y, frag[l]);
                     /* main.c:1164-1168 early-Z, GE ordered (LE mirror) */
                     float z = frag[l][2];
                     int ok = le ? (z <= tz[r * 8 + l]) : (z >= tz[r * 8 + l]);
@@ -713,8 +820,12 @@ int orc_draw_indexed(orc_target *rt, const orc_draw_desc *d, orc_stats *st) {
                 if (!any) continue;
                 for (int l = 0; l < 8; ++l) {
                     if (!pass[l]) continue;
-                    float rgb[3];
-                    pixel_shader(d, frag[l], rgb); /* main.c:1172 */
+                    float rgb[3], fx[12], fy[12];
+                    if (d->ps == ORC_PS_BASIC_LAMBERT || d->ps == ORC_PS_ENV_TEXTURED) {
+                        interp_fragment(tr, ox + l + 1, y, fx);
+                        interp_fragment(tr, ox + l, y + 1, fy);
+                    }
+                    pixel_shader(d, frag[l], fx, fy, rgb); /* main.c:1172 */
                     tc[r * 8 + l] = orc_encode_output(rgb);
                     tz[r * 8 + l] = frag[l][2];
                     tv[r * 8 + l] = tid;

@@ -31,7 +31,11 @@ extern "C" {
 #endif
 
 enum { ORC_VS_BASIC = 0, ORC_VS_VERTEX_LIGHTING = 1, ORC_VS_PASSTHROUGH = 2, ORC_VS_FULLSCREEN = 3 };
-enum { ORC_PS_BASIC = 0, ORC_PS_ENV_LIGHTING = 1, ORC_PS_PASSTHROUGH = 2 };
+/* ORC_PS_BASIC_LAMBERT and ORC_PS_ENV_TEXTURED restate the library's extension shaders (the
+ * reference has no mips, lights or second texture): mip level by footprint, Lambert light,
+ * albedo texture. */
+enum { ORC_PS_BASIC = 0, ORC_PS_ENV_LIGHTING = 1, ORC_PS_PASSTHROUGH = 2, ORC_PS_BASIC_LAMBERT = 3,
+       ORC_PS_ENV_TEXTURED = 4 };
 enum { ORC_FMT_RGBA8 = 0, ORC_FMT_RGBA32F = 1 };
 enum { ORC_DEPTH_GREATER_EQUAL = 0, ORC_DEPTH_LESS_EQUAL = 1 };
 
@@ -43,8 +47,9 @@ enum {
 };
 
 typedef struct orc_texture {
-    const void *data; /* RGBA8 as u32 (R in the low byte) or float4 */
+    const void *data; /* RGBA8 as u32 (R in the low byte) or float4; `levels` levels, level 0 first */
     uint32_t width, height, format;
+    uint32_t levels;  /* 0 or 1: one level; level l is max(w >> l, 1) x max(h >> l, 1) */
 } orc_texture;
 
 typedef struct orc_viewport {
@@ -63,6 +68,8 @@ typedef struct orc_draw_desc {
     orc_texture vs_srv0, ps_srv0;
     orc_viewport viewport;
     int32_t depth_func;
+    orc_texture ps_srv1; /* extension shaders: slot 1 */
+    const float *ps_cb;  /* extension shaders: 12 floats */
 } orc_draw_desc;
 
 typedef struct orc_target {
@@ -101,6 +108,8 @@ int orc_primitive_assembly(const orc_draw_desc *d, orc_triangle *out, uint32_t c
                            uint32_t *out_count, uint64_t *oob_count);
 uint64_t orc_coverage_mask(const int32_t edge[3][3], int32_t origin_x, int32_t origin_y);
 void orc_bilinear(const orc_texture *t, float u, float v, float out[4]);
+uint32_t orc_nearest_level(float rho2, uint32_t levels); /* extension: level k when rho2 >= 2^(2k-1) */
+void orc_bilinear_level(const orc_texture *t, uint32_t level, float u, float v, float out[4]);
 void orc_latlon(const orc_texture *t, const float dir[3], float out[4]);
 uint32_t orc_encode_output(const float rgb[3]);
 uint32_t orc_encode_clear(const float rgba[4]);

@@ -10,7 +10,8 @@ from malevich_amd import pipeline as P
 
 VS = {"basic": (L.VS_BASIC, 0), "vertex_lighting": (L.VS_VERTEX_LIGHTING, 1), "passthrough": (L.VS_PASSTHROUGH, 2),
       "fullscreen": (L.VS_FULLSCREEN, 3)}
-PS = {"basic": (L.PS_BASIC, 0), "env_lighting": (L.PS_ENV_LIGHTING, 1), "passthrough": (L.PS_PASSTHROUGH, 2)}
+PS = {"basic": (L.PS_BASIC, 0), "env_lighting": (L.PS_ENV_LIGHTING, 1), "passthrough": (L.PS_PASSTHROUGH, 2),
+      "basic_lambert": (L.PS_BASIC_LAMBERT, 3), "env_textured": (L.PS_ENV_TEXTURED, 4)}
 
 
 @dataclass
@@ -25,6 +26,9 @@ class DrawSpec:
     start_index: int = 0
     base_vertex: int = 0
     index_count: int | None = None
+    ps_tex1: np.ndarray | list | None = None  # extension shaders: texture slot 1
+    ps_cb: np.ndarray | None = None           # extension shaders: 12 floats
+    # a texture given as a list is a mip chain (level 0 first)
 
 
 @dataclass
@@ -64,13 +68,21 @@ def render_gpu(dev: P.Device, W: int, H: int, draws: list[DrawSpec], clear_color
                 cb = dev.create_buffer(L.BUFFER_CONSTANT, np.ascontiguousarray(d.cb, np.float32))
                 keep.append(cb)
                 dev.vs_set_constant_buffer(0, cb)
-            for tex, setter in ((d.vs_tex, dev.vs_set_shader_resource), (d.ps_tex, dev.ps_set_shader_resource)):
+            for tex, setter, slot in ((d.vs_tex, dev.vs_set_shader_resource, 0),
+                                      (d.ps_tex, dev.ps_set_shader_resource, 0),
+                                      (d.ps_tex1, dev.ps_set_shader_resource, 1)):
                 if tex is not None:
                     t = dev.create_texture2d(tex)
                     keep.append(t)
-                    setter(0, t)
+                    setter(slot, t)
                 else:
-                    setter(0, None)
+                    setter(slot, None)
+            if d.ps_cb is not None:
+                pcb = dev.create_buffer(L.BUFFER_CONSTANT, np.ascontiguousarray(d.ps_cb, np.float32))
+                keep.append(pcb)
+                dev.ps_set_constant_buffer(0, pcb)
+            else:
+                dev.ps_set_constant_buffer(0, None)
             n = d.indices.size if d.index_count is None else d.index_count
             dev.draw_indexed(n, d.start_index, d.base_vertex)
         colors = rt.read()
@@ -95,14 +107,23 @@ def render_oracle(oracle, W: int, H: int, draws: list[DrawSpec], clear_color, cl
     rt.clear_render_target_view(clear_color)
     rt.clear_depth_stencil_view(clear_depth, depth_func)
     for d in draws:
-        # RGBA16F texels are exact in float32, so the oracle samples their float32 values
-        up = lambda t: t.astype(np.float32) if t is not None and t.dtype == np.float16 else t
-        od = oracle.Draw(d.indices, d.vertices, VS[d.vs][1], PS[d.ps][1], cb=d.cb, vs_tex=up(d.vs_tex),
-                         ps_tex=up(d.ps_tex), viewport=(0, 0, W, H, 0, 1), start_index=d.start_index,
-                         base_vertex=d.base_vertex, index_count=d.index_count, depth_func=depth_func)
+        od = oracle.Draw(d.indices, d.vertices, VS[d.vs][1], PS[d.ps][1], cb=d.cb, vs_tex=oracle_texture(oracle, d.vs_tex),
+                         ps_tex=oracle_texture(oracle, d.ps_tex), viewport=(0, 0, W, H, 0, 1),
+                         start_index=d.start_index, base_vertex=d.base_vertex, index_count=d.index_count,
+                         depth_func=depth_func, ps_tex1=oracle_texture(oracle, d.ps_tex1), ps_cb=d.ps_cb)
         oracle.draw_indexed(rt, od)
     st = {k: getattr(rt.stats, k) for k, _ in oracle.Stats._fields_}
     return Frame(rt.colors.copy(), rt.depth.copy(), rt.vis_tri.copy(), rt.vis_draw.copy(), st)
+
+
+def oracle_texture(oracle, t):
+    """A texture for the oracle: RGBA16F texels are exact in float32, so the oracle samples their
+    float32 values; a list is a mip chain."""
+    if t is None:
+        return None
+    if isinstance(t, (list, tuple)):
+        return oracle.MipTexture(list(t))
+    return t.astype(np.float32) if t.dtype == np.float16 else t
 
 
 def channel_diff(a: np.ndarray, b: np.ndarray) -> int:

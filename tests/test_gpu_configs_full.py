@@ -1,8 +1,10 @@
 """The benchmark's workloads at their full shape against the oracle.
 
 - The 1080p forward frame: 64 displaced spheres (2.49 M triangles), vertex_lighting_vs with a
-  256x128 RGBA16F irradiance map and env_lighting_ps with a 2048x1024 RGBA16F radiance map,
-  exactly as bench.py draws it. Visibility, depth and stats bit-exact; colour within 1/255.
+  256x128 RGBA16F irradiance map and env_lighting_ps with a 2048x1024 RGBA16F radiance map; and
+  the same frame as bench.py draws it, with the textured environment shader on mip chains (the
+  radiance map's and a 1024x1024 RGBA8 albedo per instance). Visibility, depth and stats
+  bit-exact; colour within 1/255.
 - Batched views at the batched workload's shape: the first 8 views (256^2 to 1024^2) of the
   benchmark's 256-view scene, each drawing all 64 meshes (1k-500k triangles, 5.0 M per view),
   with the flush limits lowered (MLV_MAX_BATCH_TRIANGLES, MLV_MAX_BATCH_DRAWS) so the 512 draws
@@ -82,6 +84,63 @@ def test_c2_full_frame_against_oracle(device, oracle):
     print(f"  covered {covered} px, assembled {g.stats['assembled_triangle_count']}, colour max diff "
           f"{channel_diff(g.colors, o.colors)}", flush=True)
     assert covered > 0.5 * W * H
+    assert_frames_match(g, o, color_tol=1)
+
+
+def test_c2_full_textured_frame_against_oracle(device, oracle):
+    """The benchmark's headline frame as configured: the same 64 spheres with the textured
+    environment shader, a 2048x1024 RGBA16F radiance chain (12 levels) and a 1024x1024 RGBA8 albedo
+    chain (11 levels) per instance, the chains built on the device as bench.py builds them."""
+    W, H = 1920, 1080
+    sc = S.c2_scene(W, H, n_instances=64, seed=C2_SEED, texture_size=0)
+    irr, rad, albedo = S.c2_textures(64, seed=C2_SEED, albedo_size=1024)
+    print(f"case: c2 full textured frame {W}x{H}, {len(sc.instances)} instances", flush=True)
+    dev = device
+    keep = [dev.create_buffer(L.BUFFER_VERTEX, m.vertices) for m in sc.instances]
+    keep += [dev.create_buffer(L.BUFFER_INDEX_U32, m.indices) for m in sc.instances]
+    cbuf = dev.create_buffer(L.BUFFER_CONSTANT, sc.cb)
+    t_irr = dev.create_texture2d(irr)
+    t_rad = dev.create_texture2d(rad, mip_levels=12, generate_mips=True)
+    t_alb = [dev.create_texture2d(a, mip_levels=11, generate_mips=True) for a in albedo]
+    rt, ds = dev.create_render_target(W, H), dev.create_depth_stencil(W, H)
+    vt = dev.create_buffer(L.BUFFER_VERTEX, nbytes=W * H * 4)
+    vd = dev.create_buffer(L.BUFFER_VERTEX, nbytes=W * H * 4)
+    try:
+        rad_chain = t_rad.read_levels()
+        alb_chains = [t.read_levels() for t in t_alb]
+        dev.reset_stats()
+        dev.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
+        dev.vs_set_shader(L.VS_VERTEX_LIGHTING)
+        dev.ps_set_shader(L.PS_ENV_TEXTURED)
+        dev.vs_set_constant_buffer(0, cbuf)
+        dev.vs_set_shader_resource(0, t_irr)
+        dev.ps_set_shader_resource(0, t_rad)
+        dev.rs_set_viewport(0, 0, W, H, 0.0, 1.0)
+        dev.om_set_render_targets(rt, ds)
+        dev.om_set_depth_func(L.DEPTH_GREATER_EQUAL)
+        dev.om_set_visibility_buffer(vt, vd)
+        dev.clear_render_target_view(rt, S.CLEAR_COLOR)
+        dev.clear_depth_stencil_view(ds, 0.0)
+        n = len(sc.instances)
+        for vb, ib, m, ta in zip(keep[:n], keep[n:], sc.instances, t_alb):
+            dev.ia_set_vertex_buffer(vb, 32)
+            dev.ia_set_index_buffer(ib)
+            dev.ps_set_shader_resource(1, ta)
+            dev.draw_indexed(m.indices.size)
+        g = Frame(rt.read(), ds.read(), vt.read(np.uint32).reshape(H, W), vd.read(np.uint32).reshape(H, W),
+                  dev.stats())
+    finally:
+        dev.om_set_visibility_buffer(None, None)
+        dev.vs_set_shader_resource(0, None)
+        dev.ps_set_shader_resource(0, None)
+        dev.ps_set_shader_resource(1, None)
+        for h in keep + t_alb + [cbuf, t_irr, t_rad, rt, ds, vt, vd]:
+            h.release()
+    draws = [DrawSpec(m.indices, m.vertices, "vertex_lighting", "env_textured", cb=sc.cb, vs_tex=irr,
+                      ps_tex=rad_chain, ps_tex1=ch) for m, ch in zip(sc.instances, alb_chains)]
+    o = render_oracle(oracle, W, H, draws, S.CLEAR_COLOR)
+    print(f"  covered {int((g.vis_tri != 0xFFFFFFFF).sum())} px, colour max diff {channel_diff(g.colors, o.colors)}",
+          flush=True)
     assert_frames_match(g, o, color_tol=1)
 
 
