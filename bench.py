@@ -232,12 +232,16 @@ def bench_c3(dev, args):
     dev.synchronize()
     stats = dev.stats()
     tris = n_idx // 3
+    fp = footprint_of(dev, one)
     for h in (vb, ib, rt, ds):
         h.release()
     return {"workload": f"c3-1080p-binning-stress: {args.c3_tiny} tiny (1-4 px^2) + {args.c3_big} screen-covering "
                         f"triangles, passthrough_vs/ps, one draw, reversed-Z GE, {W}x{H}",
             "ms_per_frame": round(ms, 4), "mtri_per_s": round(tris / (ms * 1e-3) / 1e6, 2),
             "gpix_per_s": round(W * H / (ms * 1e-3) / 1e9, 4), "steps": args.c3_steps,
+            "frame_roofline": dict(hbm_frac(bytes_fwd(n_idx, st.vertices.shape[0], 28, W * H, 0), ms),
+                                   bytes_formula="B_fwd = 4 I + 28 V + 12 P"),
+            "footprint_per_frame": fp,
             "assembled_per_frame": int(stats["assembled_triangle_count"] / args.c3_steps),
             "pairs_per_frame": int(stats["total_triangle_count_in_bins"] / args.c3_steps),
             "scene_gen_s": round(gen_s, 1), "tuning": tuning}
@@ -293,12 +297,23 @@ def bench_c4(dev, args):
         torch.cuda.synchronize()
         ms = e0.elapsed_time(e1) / args.c4_steps
         out["deterministic" if det else "atomic"] = round(ms, 4)
+        if not det:  # one more step under footprint counting (untimed)
+            out["footprint"] = footprint_of(dev, lambda: (step(), torch.cuda.synchronize()))
         r.release()
     covered = int((tgt[..., 3] > 0).sum().item())
+    fp = out["footprint"]
+    P, V, I = sc.width * sc.height, m.positions.shape[0], 3 * m.triangle_count
+    # forward: the raster's frame bytes, the shading's texels and its RGBA32F image; backward: B_bwd
+    b_step = bytes_fwd(I, V, 32, P, fp["texel_bytes"]) + 16 * P + \
+        bytes_bwd(P, fp["visible_triangles"], fp["texel_bytes"], fp["texels"], V)
+    step_roofline = dict(hbm_frac(b_step, out["atomic"]),
+                         bytes_formula="B_fwd (4 I + 32 V + 12 P + B_tex) + 16 P + B_bwd (16 P + 4 P + 144 T_vis + "
+                                       "B_tex + 16 texels + 44 V + 36)")
     return {"workload": f"c4-512x512-differentiable: {m.triangle_count} triangles, 1024x1024 RGBA32F texture with "
                         f"{sc.levels} mips, SH9 light, L2 loss, gradients to positions/colours/uvs/normals/texels/SH",
             "fwd_bwd_ms": out["atomic"], "fwd_bwd_ms_deterministic": out["deterministic"],
-            "covered_pixels": covered, "steps": args.c4_steps, "tuning": out["tuning"],
+            "covered_pixels": covered, "steps": args.c4_steps, "fwd_bwd_roofline": step_roofline,
+            "footprint_per_step": fp, "tuning": out["tuning"],
             "tuning_deterministic": out["tuning_deterministic"]}
 
 
@@ -360,8 +375,11 @@ def bench_c5(dev, args):
     tuning = dev.autotune_end(repeats=1)
     print(f"[bench] c5 autotuned: {tuning}", file=sys.stderr, flush=True)
     ms_b = timed(False, args.c5_steps)
-    warm(dev, lambda: frame(True), 1)
-    ms_p = timed(True, 1)
+    if args.no_per_draw:  # profiling runs of the batched kernels
+        ms_p = float("nan")
+    else:
+        warm(dev, lambda: frame(True), 1)
+        ms_p = timed(True, 1)
     n = len(sc.views)
     for h in vbs + ibs + texs + cbs:
         h.release()
@@ -448,8 +466,14 @@ def bench_c5_diff(dev, sc, args, ev):
     step(False)
     ms_b, _ = timed(False, args.c5_steps)
     _, ph_b = timed(False, 1)
-    step(True)
-    ms_p, ph_p = timed(True, 1)
+    if args.no_per_draw:
+        ms_p, ph_p = float("nan"), {}
+    # the batched step's footprint (untimed): visible triangles over all views from the raster,
+    # the texels the shading's footprints read
+    fp = footprint_of(dev, lambda: (r.rasterize(), fwd(), torch.cuda.synchronize()))
+    if not args.no_per_draw:
+        step(True)
+        ms_p, ph_p = timed(True, 1)
     n = len(sc.views)
     r.release()
     del grads, accum, out, target, gout
@@ -459,18 +483,66 @@ def bench_c5_diff(dev, sc, args, ev):
             "fwd_bwd_per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
             "fwd_bwd_batched_over_per_draw": round(ms_p / ms_b, 2),
             "fwd_bwd_batched_phases": ph_b, "fwd_bwd_per_draw_phases": ph_p,
-            "shaded_pixels": int(r.offsets[-1]),
-            "shading_pass": shading_roofline(int(r.offsets[-1]), ph_b["shade_ms"])}
+            "shaded_pixels": int(r.offsets[-1]), "footprint_per_step": fp,
+            "shading_pass": shading_roofline(int(r.offsets[-1]), fp, ph_b["shade_ms"]),
+            "adjoint_pass": adjoint_roofline(int(r.offsets[-1]), fp, sum(mm.vertex_count for mm in meshes),
+                                             ph_b["loss_and_adjoint_ms"])}
 
 
-def shading_roofline(pixels: int, ms: float) -> dict:
-    """The batched shading launch (kd_batch_forward) against HBM: compulsory bytes are the
-    primitive id and draw tag read and the RGBA32F colour written per pixel (24 B). The visible
-    triangles' attributes and the sampled texels come on top, so this is a lower bound."""
-    b = 24 * pixels
-    gbs = b / (ms * 1e-3) / 1e9
-    return {"kernel": "kd_batch_forward", "ms": ms, "compulsory_bytes": b, "achieved": round(gbs, 1),
-            "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 4)}
+def shading_roofline(pixels: int, fp: dict, ms: float) -> dict:
+    """The batched shading launch (kd_batch_forward) against B_shade: the primitive id and draw tag
+    in (8 B a pixel), the visible triangles' attributes, the texels read and the RGBA32F colour out."""
+    r = hbm_frac(bytes_shade(pixels, fp["visible_triangles"], fp["texel_bytes"], id_bytes=8, out_bytes=16), ms)
+    r.update(kernel="kd_batch_forward", bytes_formula="B_shade = 8 P + 144 T_vis + B_tex + 16 P")
+    return r
+
+
+def adjoint_roofline(pixels: int, fp: dict, vertices: int, ms: float) -> dict:
+    """The loss gradient and the batched adjoint launch (kd_batch_backward) against B_bwd, with the
+    draw tag read too (8 B of ids a pixel)."""
+    r = hbm_frac(bytes_bwd(pixels, fp["visible_triangles"], fp["texel_bytes"], fp["texels"], vertices, id_bytes=8),
+                 ms)
+    r.update(kernel="kd_batch_backward", bytes_formula="B_bwd = 16 P + 8 P + 144 T_vis + B_tex + 16 texels + 44 V + 36")
+    return r
+
+
+def footprint_of(dev, run) -> dict:
+    """Visible triangles, pixels written, distinct texels and texel bytes of one run of `run`, from
+    the library's footprint counting (a measurement mode with atomics, never timed)."""
+    dev.set_footprint_counting(True)
+    try:
+        dev.footprint()  # drop anything counted before
+        run()
+        dev.flush()
+        return dev.footprint()
+    finally:
+        dev.set_footprint_counting(False)
+
+
+def hbm_frac(bytes_: float, ms: float) -> dict:
+    gbs = bytes_ / (ms * 1e-3) / 1e9
+    return {"algorithmic_bytes": int(bytes_), "ms": round(ms, 4), "achieved": round(gbs, 2), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(gbs / HBM_PEAK_GBS, 5)}
+
+
+# Algorithmic (compulsory) bytes, each input read once and each output written once, no
+# intermediates: I indices, V vertices of S_in bytes, P pixels, T_vis visible triangles (three
+# vertices of three 16-byte registers each), B_tex the distinct texels the sampling footprints read.
+def bytes_fwd(I, V, s_in, P, b_tex):
+    """A forward frame: indices, vertices, the depth read and the colour, depth and id writes."""
+    return 4 * I + V * s_in + 12 * P + b_tex
+
+
+def bytes_shade(P, t_vis, b_tex, id_bytes=4, out_bytes=4):
+    """The shading pass: the visibility id in, the visible triangles' attributes, the texels, the colour out."""
+    return id_bytes * P + 144 * t_vis + b_tex + out_bytes * P
+
+
+def bytes_bwd(P, t_vis, b_tex, texels, V, id_bytes=4):
+    """The adjoint pass: dL/dRGBA (float4) and the ids in, the visible triangles' attributes and the
+    texels read, each touched texel's float4 gradient and each vertex's 44 gradient bytes (position,
+    colour, uv, normal) written, the 9x3 SH gradient."""
+    return 16 * P + id_bytes * P + 144 * t_vis + b_tex + 16 * texels + 44 * V + 36
 
 
 def copy_calibration(mib: int):
@@ -516,6 +588,8 @@ def main(argv=None):
     ap.add_argument("--no-c5", action="store_true")
     ap.add_argument("--no-c5-diff", action="store_true", help="C5 forward legs only (no batched fwd+bwd)")
     ap.add_argument("--c5-only", action="store_true", help="run only the batched-views leg")
+    ap.add_argument("--no-per-draw", action="store_true",
+                    help="batched-views leg without its per-draw comparison (for profiling the batched kernels)")
     ap.add_argument("--c4-steps", type=int, default=20)
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-only", action="store_true", help="run only the differentiable-render leg")
@@ -639,11 +713,21 @@ def main(argv=None):
         "binning": asm_frame * 16 + (stats["total_triangle_count_in_bins"] / args.steps) * 8 * 2,
     }
     algo = {k: v / flushes_per_frame for k, v in algo_frame.items()}
+    # the frame's footprint (counting mode, untimed): the tile kernel is the shading pass, whose
+    # compulsory bytes are B_shade; the frame's are B_fwd
+    fp = footprint_of(dev, lambda: frame(dev, res, sc))
+    print(f"[bench] footprint: {fp}", file=sys.stderr, flush=True)
+    idx_frame = sum(m.indices.size for m in sc.instances)
+    if dom == "tile":
+        algo["tile"] = bytes_shade(P_px, fp["visible_triangles"], fp["texel_bytes"]) / flushes_per_frame
     achieved = algo[dom] / (per_launch_ms * 1e-3) / 1e9
     traffic = measured_traffic(STAGE_KERNEL[dom], sc)
     roofline = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": STAGE_KERNEL[dom],
-                "avg_launch_us": round(per_launch_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(algo[dom])}
+                "avg_launch_us": round(per_launch_ms * 1e3, 2), "algorithmic_bytes_per_launch": int(algo[dom]),
+                "bytes_formula": "B_shade = 4 P + 144 T_vis + B_tex + 4 P" if dom == "tile" else "stage formula"}
+    frame_roofline = hbm_frac(bytes_fwd(idx_frame, verts_frame, 32, P_px, fp["texel_bytes"]), ms_step)
+    frame_roofline["bytes_formula"] = "B_fwd = 4 I + 32 V + 12 P + B_tex"
     out = {
         "metric": "fwd Mtri/s + Gpix/s @1080p; fwd+bwd ms/frame; batched vs per-draw views/s",
         "value": round(mtri_s, 2),
@@ -673,6 +757,9 @@ def main(argv=None):
         "batched_views_per_s": None,
         "per_draw_views_per_s": None,
         "roofline": roofline,
+        "frame_roofline": frame_roofline,
+        "footprint_per_frame": fp,
+        "shaded_fragments_per_s": round(fp["shaded_pixels"] / (ms_step * 1e-3), 1),
     }
     if not args.no_c3:
         out["c3_binning_stress"] = bench_c3(dev, args)

@@ -237,6 +237,22 @@ typedef struct mlv_profile {
 MLV_API int mlv_context_set_profiling(mlv_context *ctx, int enable);
 MLV_API int mlv_get_profile(mlv_context *ctx, mlv_profile *out);
 
+/* Extension: footprint counting, for measurement (no reference counterpart). While enabled, the
+ * tile stage marks every assembled triangle that wins a pixel and counts the pixels written, and
+ * the shaders (and the differentiable passes) mark every texel their bilinear footprints read, in
+ * one bitmap per texture (allocated when a counted draw first samples it). mlv_get_footprint
+ * flushes, returns the totals since its last call and clears them: visible triangles (summed over
+ * flushes), pixels written, distinct texels per mip level and their bytes. Counting adds atomics
+ * to the shading: leave it off when timing. */
+typedef struct mlv_footprint {
+    uint64_t visible_triangles;
+    uint64_t shaded_pixels;
+    uint64_t texels;
+    uint64_t texel_bytes;
+} mlv_footprint;
+MLV_API int mlv_context_set_footprint_counting(mlv_context *ctx, int enable);
+MLV_API int mlv_get_footprint(mlv_context *ctx, mlv_footprint *out);
+
 /* Stats (main.c:199-206), reset like render() does at main.c:1268-1269. The reset also
  * restarts the draw tags written to the visibility buffer at 0. */
 MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out);

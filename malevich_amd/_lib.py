@@ -42,6 +42,10 @@ class Profile(C.Structure):
                 ("tile_kernel_ms", C.c_double)]
 
 
+class Footprint(C.Structure):
+    _fields_ = [(n, C.c_uint64) for n in ("visible_triangles", "shaded_pixels", "texels", "texel_bytes")]
+
+
 STAGES = ("vertex", "setup", "binning", "tile")
 
 _P = C.c_void_p
@@ -146,6 +150,8 @@ SIGNATURES = {
     "mlv_get_stats": (_I, [_P, C.POINTER(Stats)]),
     "mlv_reset_stats": (_I, [_P]),
     "mlv_context_set_profiling": (_I, [_P, _I]),
+    "mlv_context_set_footprint_counting": (_I, [_P, _I]),
+    "mlv_get_footprint": (_I, [_P, C.POINTER(Footprint)]),
     "mlv_flush": (_I, [_P]),
     "mlv_get_profile": (_I, [_P, C.POINTER(Profile)]),
     "mlv_camera_constant_buffers": (_I, [_P, C.POINTER(Camera), _U32, _P]),

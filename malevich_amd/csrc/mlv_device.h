@@ -130,12 +130,24 @@ __device__ __forceinline__ float4 decode_rgba16f(ushort4 h) {
                        __half2float(__ushort_as_half(h.z)), __half2float(__ushort_as_half(h.w)));
 }
 
+// Footprint counting: sets the texel's bit in the texture's bitmap.
+__device__ __forceinline__ void mark_texel(const TexDesc &t, uint32_t i) {
+    const uint64_t b = t.mark_base + i;
+    atomicOr(t.mark + (b >> 5), 1u << (uint32_t)(b & 31u));
+}
+
 // get_texel_u_x8 / get_texel_f_x8 (csc.h:30-53) for the four texels of a bilinear footprint:
 // one branch on the format, then the four loads in flight together.
 __device__ __forceinline__ void fetch4(const TexDesc &t, int32_t s0, int32_t t0, int32_t s1, int32_t t1,
                                        float4 &f00, float4 &f10, float4 &f01, float4 &f11) {
     const uint32_t i00 = texel_index(t, s0, t0), i10 = texel_index(t, s1, t0);
     const uint32_t i01 = texel_index(t, s0, t1), i11 = texel_index(t, s1, t1);
+    if (t.mark) {  // footprint counting only
+        mark_texel(t, i00);
+        mark_texel(t, i10);
+        mark_texel(t, i01);
+        mark_texel(t, i11);
+    }
     if (t.format == FMT_RGBA8) {
         const uint32_t *q = (const uint32_t *)t.data;
         const uint32_t c00 = load_global(q + i00), c10 = load_global(q + i10);
@@ -229,6 +241,7 @@ __device__ __forceinline__ TexDesc tex_level(const TexDesc &t, uint32_t l) {
     }
     TexDesc r = t;
     r.data = (const uint8_t *)t.data + off * texel_bytes(t.format);
+    r.mark_base = t.mark_base + off;
     r.width = w;
     r.height = h;
     r.mip_levels = 1u;

@@ -221,6 +221,15 @@ __device__ __forceinline__ bool evaluate(const DiffMesh &m, const DiffView &v, c
     e.y0 = min(max(iy, 0), hm);
     e.y1 = min(max(iy + 1, 0), hm);
     const float4 *T = m.tex + m.loff[e.lvl];
+    if constexpr (sizeof(R) == 4) {
+        if (m.tex_mark) {  // footprint counting (forward passes only): the four taps' texels
+            const uint32_t b0 = m.loff[e.lvl];
+            const uint32_t taps[4] = {b0 + (uint32_t)e.y0 * lw + (uint32_t)e.x0, b0 + (uint32_t)e.y0 * lw + (uint32_t)e.x1,
+                                      b0 + (uint32_t)e.y1 * lw + (uint32_t)e.x0, b0 + (uint32_t)e.y1 * lw + (uint32_t)e.x1};
+#pragma unroll
+            for (int k = 0; k < 4; ++k) atomicOr(m.tex_mark + (taps[k] >> 5), 1u << (taps[k] & 31u));
+        }
+    }
     const float4 q00 = T[(uint32_t)e.y0 * lw + (uint32_t)e.x0], q10 = T[(uint32_t)e.y0 * lw + (uint32_t)e.x1],
                  q01 = T[(uint32_t)e.y1 * lw + (uint32_t)e.x0], q11 = T[(uint32_t)e.y1 * lw + (uint32_t)e.x1];
     e.t00[0] = q00.x; e.t00[1] = q00.y; e.t00[2] = q00.z;

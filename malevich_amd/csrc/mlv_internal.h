@@ -57,6 +57,8 @@ enum OverflowBits : uint32_t {
 // max(width >> l, 1) x max(height >> l, 1) texels.
 struct TexDesc {
     const void *data;
+    uint32_t *mark;               // footprint counting: one bit per texel of the chain (NULL: off)
+    uint64_t mark_base;           // bit of this level's first texel in `mark`
     uint32_t width, height, format, mip_levels;
 };
 constexpr uint32_t kPsConstants = 12;  // floats of the extension pixel shaders' constant buffer
@@ -133,6 +135,14 @@ struct BatchCounters {
     uint32_t class_cnt[8], class_cur[8];   // tiles per list-length class; slots taken so far (tile_order)
 };
 
+// Footprint counting (mlv_context_set_footprint_counting): totals since the last read.
+struct FootCounts {
+    unsigned long long shaded_px;    // pixels a draw wrote (winners shaded)
+    unsigned long long vis_tris;     // assembled triangles that won a pixel, summed over flushes
+    unsigned long long tex_bytes;    // bytes of the distinct texels the counted bitmaps hold
+    unsigned long long texels;       // their count
+};
+
 struct DevStats {
     unsigned long long assembled, active8, pairs8, oob, overflow_draws, capacity_errors, max_pairs, max_clip_list;
     uint32_t scan_errors;  // scans whose look-back gave up waiting (never expected)
@@ -174,6 +184,8 @@ struct BatchParams {
     uint32_t scan_status_words;
     BatchCounters *ctr;
     DevStats *stats;
+    uint32_t *tri_mark;                   // footprint counting: one bit per assembled triangle (NULL: off)
+    FootCounts *foot;
 };
 
 // Differentiable shading over the raster's primitive ids (mlv_diff.hip).
@@ -187,6 +199,7 @@ struct DiffMesh {
     uint32_t tex_total;                   // texels of the whole chain
     uint32_t levels;
     uint32_t lw[kMaxMips], lh[kMaxMips], loff[kMaxMips];
+    uint32_t *tex_mark;                   // footprint counting: one bit per texel of the chain (NULL: off)
 };
 // One view: the clip_from_world matrix and the image size (the viewport covers the image).
 struct DiffView {
@@ -313,6 +326,9 @@ hipError_t launch_present_rgb(const uint32_t *fb, uint8_t *rgb, size_t n_pixels,
 hipError_t launch_srgb_to_linear_rgba8(uint32_t *texels, size_t n, hipStream_t s);
 // levels 1 .. mip_levels-1 of the texture's chain from level 0, in order
 hipError_t launch_generate_mips(const TexDesc &t, hipStream_t s);
+// adds mul * (set bits of words[0..n)) to *out
+hipError_t launch_popcount(const uint32_t *words, size_t n, uint32_t mul, unsigned long long *out_bits,
+                           unsigned long long *out_bytes, hipStream_t s);
 hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s);
 // The flush in four stages, in this order: vertex, setup (cull/clip/ordered compaction),
 // binning (ordered tile lists), tile (raster + early-Z + shading + output merger).

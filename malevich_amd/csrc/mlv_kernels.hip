@@ -1437,6 +1437,7 @@ __device__ __forceinline__ void shade_winner(const BatchParams &p, const ViewRec
     }
     V.color[pix] = encode_output(cr, cg, cb);
     V.depth[pix] = depth;
+    if (p.tri_mark) atomicOr(p.tri_mark + (best >> 5), 1u << (best & 31u));  // footprint counting
     if (V.vis_tri) {
         V.vis_tri[pix] = best - p.asm_base[T.draw];  // numbered within the draw
         V.vis_draw[pix] = D.draw_tag;
@@ -1522,6 +1523,10 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
     }
 
     if (st.best >= 0) shade_winner(p, V, (uint32_t)st.best, st.d, px, py, pix);
+    if (p.foot) {  // footprint counting: pixels written
+        const uint64_t m = __ballot(st.best >= 0);
+        if (lane == 0u && m) atomicAdd(&p.foot->shaded_px, (unsigned long long)__popcll(m));
+    }
 
     if (lane == 0u) s_active[w] = 0u;
     if (sub_valid && st.cur_draw >= 0) {
@@ -1632,6 +1637,10 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
         }
     }
     if (st.best >= 0) shade_winner(p, V, (uint32_t)st.best, st.d, px, py, pix);
+    if (p.foot) {  // footprint counting: pixels written
+        const uint64_t m = __ballot(st.best >= 0);
+        if (lane == 0u && m) atomicAdd(&p.foot->shaded_px, (unsigned long long)__popcll(m));
+    }
     if (seen) {
         const float m = tile_hiz<GE>(inside, st.d);
         if (lane == 0u) {
@@ -1812,6 +1821,27 @@ __global__ __launch_bounds__(256) void k_mip_level(TexDesc t, uint32_t l) {
             }
         }
     }
+}
+
+// Footprint counting: adds the set bits of words[0..n) to *out_bits and mul times them to *out_bytes.
+__global__ __launch_bounds__(256) void k_popcount(const uint32_t *words, size_t n, uint32_t mul,
+                                                  unsigned long long *out_bits, unsigned long long *out_bytes) {
+    unsigned long long c = 0;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u)
+        c += (unsigned long long)__popc(words[i]);
+    c = wave_sum(c);
+    if ((threadIdx.x & 63u) == 0u && c) {
+        if (out_bits) atomicAdd(out_bits, c);
+        if (out_bytes) atomicAdd(out_bytes, c * mul);
+    }
+}
+
+hipError_t launch_popcount(const uint32_t *words, size_t n, uint32_t mul, unsigned long long *out_bits,
+                           unsigned long long *out_bytes, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_popcount, dim3((unsigned)std::min<size_t>((n + 255) / 256, 2048)), dim3(256), 0, s, words,
+                       n, mul, out_bits, out_bytes);
+    return hipGetLastError();
 }
 
 hipError_t launch_generate_mips(const TexDesc &t, hipStream_t s) {

@@ -43,6 +43,8 @@ struct mlv_texture2d {
     void *ptr;
     uint32_t width, height, format, mip_levels;
     size_t bytes;
+    uint32_t *mark = nullptr;  // footprint counting: one bit per texel of the chain
+    size_t mark_words = 0;
 };
 
 struct mlv_render_target {
@@ -135,6 +137,12 @@ struct mlv_context {
     // batched differentiable shading: device tables, FP64 position sums, SH partials
     Arena b_tables, b_posacc, b_shpart;
     Arena present_rgb;  // packed RGB of the last present
+    // footprint counting (mlv_context_set_footprint_counting)
+    bool counting = false;
+    FootCounts *foot = nullptr;
+    Arena tri_mark, diff_mark;
+    size_t diff_mark_words = 0;
+    std::vector<mlv_texture2d *> marked;  // textures holding a footprint bitmap
     Arena cameras;      // camera table of the last mlv_camera_constant_buffers
     BatchCounters *ctr = nullptr;
     DevStats *dstats = nullptr;
@@ -483,6 +491,13 @@ static int flush(mlv_context *ctx) {
     ctx->last_pair_cap = pair_cap;
     p.ctr = ctx->ctr;
     p.stats = ctx->dstats;
+    if (ctx->counting) {
+        const size_t words = ((size_t)tri_cap + 31) / 32;
+        RC_OR_RETURN(ensure(ctx, ctx->tri_mark, words * 4));
+        HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->tri_mark.p, 0u, words, ctx->stream));
+        p.tri_mark = (uint32_t *)ctx->tri_mark.p;
+        p.foot = ctx->foot;
+    }
     // The first stage clears the scan status words, so epochs only need to differ between the
     // scans of one flush: they count from 1 in every flush, and a replayed graph reuses them.
     uint32_t flush_epoch = 0;
@@ -524,6 +539,9 @@ static int flush(mlv_context *ctx) {
             if (tk[0]) ctx->ev_pending.push_back({kStages, {tk[0], tk[1]}});
         }
     }
+    if (p.tri_mark)
+        HIP_OR_FAIL(ctx, launch_popcount(p.tri_mark, ((size_t)tri_cap + 31) / 32, 0u, &ctx->foot->vis_tris, nullptr,
+                                         ctx->stream));
     if (timing) {
         ctx->prof_draws += n_draws;
         ctx->prof_flushes++;
@@ -645,7 +663,8 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->scan_part, &ctx->radix_hist, &ctx->heavy, &ctx->tile_order, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
                     &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
-                    &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart, &ctx->present_rgb, &ctx->cameras};
+                    &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart, &ctx->present_rgb, &ctx->cameras,
+                    &ctx->tri_mark, &ctx->diff_mark};
     for (Arena *a : all) release(*a);
     free_retired(ctx);
     resolve_events(ctx);
@@ -656,6 +675,12 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
     }
     (void)hipFree(ctx->ctr);
     (void)hipFree(ctx->dstats);
+    if (ctx->foot) (void)hipFree(ctx->foot);
+    for (mlv_texture2d *t : ctx->marked) {  // textures outliving their context keep no bitmap
+        (void)hipFree(t->mark);
+        t->mark = nullptr;
+        t->mark_words = 0;
+    }
     (void)hipStreamDestroy(ctx->stream);
     delete ctx;
     return MLV_OK;
@@ -766,8 +791,21 @@ static TexDesc tex_desc(const mlv_texture2d *t) {
         d.height = t->height;
         d.format = t->format;
         d.mip_levels = t->mip_levels;
+        d.mark = t->mark;
     }
     return d;
+}
+
+// Footprint counting: the texture's bitmap (one bit per texel of the chain), allocated cleared
+// the first time a counted draw samples it.
+static int ensure_mark(mlv_context *ctx, mlv_texture2d *t) {
+    if (!t || t->mark) return MLV_OK;
+    const size_t words = (t->bytes / texel_bytes((int)t->format) + 31) / 32;
+    HIP_OR_FAIL(ctx, hipMalloc(&t->mark, words * 4));
+    t->mark_words = words;
+    HIP_OR_FAIL(ctx, launch_fill_u32(t->mark, 0u, words, ctx->stream));
+    ctx->marked.push_back(t);
+    return MLV_OK;
 }
 
 // Texels of the first `levels` levels of a width x height chain (level l is max(w >> l, 1) x
@@ -820,6 +858,7 @@ MLV_API int mlv_texture2d_generate_mips(mlv_context *ctx, mlv_texture2d *tex) {
     RC_OR_RETURN(set_device(ctx));
     RC_OR_RETURN(flush(ctx));  // recorded draws read the texture as it was
     TexDesc d = tex_desc(tex);
+    d.mark = nullptr;
     HIP_OR_FAIL(ctx, launch_generate_mips(d, ctx->stream));
     return MLV_OK;
 }
@@ -847,7 +886,14 @@ MLV_API int mlv_texture2d_destroy(mlv_context *ctx, mlv_texture2d *tex) {
     if (!ctx || !tex) return MLV_ERROR_INVALID_ARGUMENT;
     (void)flush(ctx);
     (void)hipStreamSynchronize(ctx->stream);
+    if (tex->mark && ctx->foot) {  // a counted texture's distinct texels join the totals before it goes
+        (void)launch_popcount(tex->mark, tex->mark_words, (uint32_t)texel_bytes((int)tex->format), &ctx->foot->texels,
+                              &ctx->foot->tex_bytes, ctx->stream);
+        (void)hipStreamSynchronize(ctx->stream);
+    }
     (void)hipFree(tex->ptr);
+    if (tex->mark) (void)hipFree(tex->mark);
+    ctx->marked.erase(std::remove(ctx->marked.begin(), ctx->marked.end(), tex), ctx->marked.end());
     for (auto &s : ctx->vs_srv)
         if (s == tex) s = nullptr;
     for (auto &s : ctx->ps_srv)
@@ -1187,9 +1233,15 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
     r.ib = (const uint32_t *)((const uint8_t *)ctx->ib->ptr + ctx->ib_off) + start_index;
     r.vb = (const uint8_t *)ctx->vb->ptr + ctx->vb_off;
     r.cb = ctx->vs_cb[0] ? (const float *)ctx->vs_cb[0]->ptr : nullptr;
+    if (ctx->counting) {
+        RC_OR_RETURN(ensure_mark(ctx, ctx->vs_srv[0]));
+        RC_OR_RETURN(ensure_mark(ctx, ctx->ps_srv[0]));
+        RC_OR_RETURN(ensure_mark(ctx, ctx->ps_srv[1]));
+    }
     r.vs_tex = tex_desc(ctx->vs_srv[0]);
     r.ps_tex = tex_desc(ctx->ps_srv[0]);
     r.ps_tex1 = tex_desc(ctx->ps_srv[1]);
+    if (!ctx->counting) r.vs_tex.mark = r.ps_tex.mark = r.ps_tex1.mark = nullptr;
     r.ps_cb = ctx->ps_cb[0] ? (const float *)ctx->ps_cb[0]->ptr : nullptr;
     screen_matrix(ctx->vp, r.screen);
     r.stride = ctx->vb_stride;
@@ -1495,6 +1547,42 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
     return mlv_get_tuning(ctx, best);
 }
 
+MLV_API int mlv_context_set_footprint_counting(mlv_context *ctx, int enable) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (ctx->recording)
+        return fail(ctx, MLV_ERROR_INVALID_STATE, "footprint counting cannot change while recording");
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // recorded draws keep the mode they were recorded under
+    if (enable && !ctx->foot) {
+        HIP_OR_FAIL(ctx, hipMalloc(&ctx->foot, sizeof(FootCounts)));
+        HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->foot, 0u, sizeof(FootCounts) / 4, ctx->stream));
+    }
+    ctx->counting = enable != 0;
+    return MLV_OK;
+}
+
+MLV_API int mlv_get_footprint(mlv_context *ctx, mlv_footprint *out) {
+    if (!ctx || !out) return MLV_ERROR_INVALID_ARGUMENT;
+    *out = mlv_footprint{};
+    if (!ctx->foot) return MLV_OK;
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    for (mlv_texture2d *t : ctx->marked) {
+        HIP_OR_FAIL(ctx, launch_popcount(t->mark, t->mark_words, (uint32_t)texel_bytes((int)t->format),
+                                         &ctx->foot->texels, &ctx->foot->tex_bytes, ctx->stream));
+        HIP_OR_FAIL(ctx, launch_fill_u32(t->mark, 0u, t->mark_words, ctx->stream));
+    }
+    FootCounts f{};
+    HIP_OR_FAIL(ctx, hipMemcpyAsync(&f, ctx->foot, sizeof f, hipMemcpyDeviceToHost, ctx->stream));
+    HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->foot, 0u, sizeof(FootCounts) / 4, ctx->stream));
+    HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
+    out->shaded_pixels = f.shaded_px;
+    out->visible_triangles = f.vis_tris;
+    out->texels = f.texels;
+    out->texel_bytes = f.tex_bytes;
+    return MLV_OK;
+}
+
 MLV_API int mlv_context_set_profiling(mlv_context *ctx, int enable) {
     if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
     RC_OR_RETURN(flush(ctx));
@@ -1653,7 +1741,17 @@ MLV_API int mlv_diff_forward(mlv_context *ctx, const mlv_diff_scene *scene, cons
     RC_OR_RETURN(diff_params(ctx, scene, device_prim_ids, p));
     RC_OR_RETURN(set_device(ctx));
     RC_OR_RETURN(flush(ctx));  // the primitive ids come from recorded draws
+    size_t words = 0;
+    if (ctx->counting) {  // footprint counting: a bitmap over the chain
+        words = ((size_t)p.m.tex_total + 31) / 32;
+        RC_OR_RETURN(ensure(ctx, ctx->diff_mark, words * 4));
+        HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->diff_mark.p, 0u, words, ctx->stream));
+        p.m.tex_mark = (uint32_t *)ctx->diff_mark.p;
+    }
     HIP_OR_FAIL(ctx, launch_diff_forward(p, (float4 *)device_rgba_out, ctx->stream));
+    if (words)
+        HIP_OR_FAIL(ctx, launch_popcount(p.m.tex_mark, words, 16u, &ctx->foot->texels, &ctx->foot->tex_bytes,
+                                         ctx->stream));
     return MLV_OK;
 }
 
@@ -1819,6 +1917,18 @@ static int diff_batch_setup(mlv_context *ctx, const mlv_diff_mesh *meshes, uint3
             return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: too many pixels or draws");
     }
     if (draw_mesh.empty()) draw_mesh.push_back(0xFFFFFFFFu);
+    if (!backward && ctx->counting) {  // footprint counting: one bitmap per mesh chain
+        size_t words = 0;
+        for (auto &b : bm) words += ((size_t)b.m.tex_total + 31) / 32;
+        RC_OR_RETURN(ensure(ctx, ctx->diff_mark, std::max<size_t>(words, 1) * 4));
+        HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->diff_mark.p, 0u, words, ctx->stream));
+        size_t off = 0;
+        for (auto &b : bm) {
+            b.m.tex_mark = (uint32_t *)ctx->diff_mark.p + off;
+            off += ((size_t)b.m.tex_total + 31) / 32;
+        }
+        ctx->diff_mark_words = words;
+    }
 
     // FP64 position sums: the caller's pos_accum, else one [V][3] range of library scratch
     if (backward) RC_OR_RETURN(ensure(ctx, ctx->b_posacc, std::max<uint64_t>(pos_total, 1) * 8));
@@ -1915,6 +2025,9 @@ MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes
     DiffBatchParams P;
     RC_OR_RETURN(diff_batch_setup(ctx, meshes, mesh_count, views, view_count, device_sh, false, false, P));
     HIP_OR_FAIL(ctx, launch_diff_batch_forward(P, ctx->stream));
+    if (ctx->counting)  // RGBA32F chains: 16 B a texel
+        HIP_OR_FAIL(ctx, launch_popcount((const uint32_t *)ctx->diff_mark.p, ctx->diff_mark_words, 16u,
+                                         &ctx->foot->texels, &ctx->foot->tex_bytes, ctx->stream));
     return MLV_OK;
 }
 

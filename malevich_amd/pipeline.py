@@ -344,6 +344,16 @@ class Device:
         out["tile_kernel"] = pr.tile_kernel_ms
         return out
 
+    def set_footprint_counting(self, enable: bool) -> None:
+        """Measurement mode: mark visible triangles and sampled texels (adds atomics; not for timing)."""
+        _check(self.ctx, self.lib.mlv_context_set_footprint_counting(self.ctx, int(bool(enable))))
+
+    def footprint(self) -> dict:
+        """Visible triangles, pixels written, distinct texels and their bytes since the last call."""
+        f = L.Footprint()
+        _check(self.ctx, self.lib.mlv_get_footprint(self.ctx, C.byref(f)))
+        return {k: getattr(f, k) for k, _ in L.Footprint._fields_}
+
     def flush(self) -> None:
         """Enqueue the recorded draws now (they otherwise run when their results are needed)."""
         _check(self.ctx, self.lib.mlv_flush(self.ctx))
