@@ -606,6 +606,8 @@ def main(argv=None):
 
     if args.c5_only:
         dev = P.Device(0)
+        if args.copy_calibration_mib > 0:
+            copy_calibration(args.copy_calibration_mib)
         print(json.dumps({"c5_batched_views": bench_c5(dev, args)}))
         dev.close()
         return

@@ -66,20 +66,49 @@ struct Clip {
     R X[3], Y[3], W[3];
 };
 
-__device__ __forceinline__ void clip_of(const DiffMesh &m, const DiffView &v, const uint32_t vi[3], Clip<float> &c) {
+// A triangle's three corners as the shading reads them: loaded once, each vertex's position,
+// colour and normal as one 12-byte load and its uv as one 8-byte load (the attribute arrays are
+// tightly packed float triples and pairs, 4-byte aligned), so a pixel issues 13 vector loads for
+// its triangle instead of 36 single-float gathers.
+struct Corners {
+    float p[3][3], c[3][3], uv[3][2], n[3][3];
+};
+typedef const __attribute__((address_space(1))) float *gfloat;
+typedef const __attribute__((address_space(1))) uint32_t *guint;
+
+// n consecutive floats through a global pointer: adjacent 4-byte loads, merged by the compiler
+// into one 8- or 12-byte load
+template <int N>
+__device__ __forceinline__ void ld_floats(const float *q, float (&o)[N]) {
+    const gfloat g = (gfloat)q;
+#pragma unroll
+    for (int j = 0; j < N; ++j) o[j] = g[j];
+}
+
+__device__ __forceinline__ void load_corners(const DiffMesh &m, const uint32_t vi[3], Corners &q) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const float px = m.pos[3u * vi[k]], py = m.pos[3u * vi[k] + 1u], pz = m.pos[3u * vi[k] + 2u];
+        ld_floats<3>(m.pos + 3u * vi[k], q.p[k]);
+        ld_floats<3>(m.col + 3u * vi[k], q.c[k]);
+        ld_floats<3>(m.nrm + 3u * vi[k], q.n[k]);
+        ld_floats<2>(m.uv + 2u * vi[k], q.uv[k]);
+    }
+}
+
+__device__ __forceinline__ void clip_of(const Corners &q, const DiffView &v, Clip<float> &c) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        const float px = q.p[k][0], py = q.p[k][1], pz = q.p[k][2];
         c.X[k] = dot4_pair(v.M + 0, px, py, pz, 1.0f);
         c.Y[k] = dot4_pair(v.M + 4, px, py, pz, 1.0f);
         c.W[k] = dot4_pair(v.M + 12, px, py, pz, 1.0f);
     }
 }
 
-__device__ __forceinline__ void clip_of(const DiffMesh &m, const DiffView &v, const uint32_t vi[3], Clip<double> &c) {
+__device__ __forceinline__ void clip_of(const Corners &q, const DiffView &v, Clip<double> &c) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
-        const double px = m.pos[3u * vi[k]], py = m.pos[3u * vi[k] + 1u], pz = m.pos[3u * vi[k] + 2u];
+        const double px = q.p[k][0], py = q.p[k][1], pz = q.p[k][2];
         const float *M = v.M;
         c.X[k] = (double)M[0] * px + (double)M[1] * py + (double)M[2] * pz + (double)M[3];
         c.Y[k] = (double)M[4] * px + (double)M[5] * py + (double)M[6] * pz + (double)M[7];
@@ -107,30 +136,41 @@ __device__ __forceinline__ void bary(const Clip<R> &c, R u, R v, Bary<R> &B) {
     B.l[1] = B.a[2] * B.b[0] - B.a[0] * B.b[2];
     B.l[2] = B.a[0] * B.b[1] - B.a[1] * B.b[0];
     B.S = (B.l[0] + B.l[1]) + B.l[2];
+    const R inv = (R)1 / B.S;  // one division, three products
 #pragma unroll
-    for (int k = 0; k < 3; ++k) B.lam[k] = B.l[k] / B.S;
+    for (int k = 0; k < 3; ++k) B.lam[k] = B.l[k] * inv;
 }
 
-__device__ __forceinline__ void uv_at(const DiffMesh &m, const uint32_t vi[3], const float lam[3], float &u,
-                                      float &v) {
-    u = (lam[0] * m.uv[2u * vi[0]] + lam[1] * m.uv[2u * vi[1]]) + lam[2] * m.uv[2u * vi[2]];
-    v = (lam[0] * m.uv[2u * vi[0] + 1u] + lam[1] * m.uv[2u * vi[1] + 1u]) + lam[2] * m.uv[2u * vi[2] + 1u];
+// Whether the f32 barycentrics of B are accurate to a few ulp of 1: the cross products' terms,
+// summed in magnitude, stay within 64 times |S| (inside a triangle seen at a normal angle they sum
+// to about |S|). Slivers seen edge-on and corners far outside the float triangle fail it (the
+// terms nearly cancel), and so do NaN and infinities.
+__device__ __forceinline__ bool well_conditioned(const Bary<float> &B) {
+    const float mag = (fabsf(B.a[1] * B.b[2]) + fabsf(B.a[2] * B.b[1])) +
+                      (fabsf(B.a[2] * B.b[0]) + fabsf(B.a[0] * B.b[2])) +
+                      (fabsf(B.a[0] * B.b[1]) + fabsf(B.a[1] * B.b[0]));
+    return mag < 64.0f * fabsf(B.S);
+}
+
+__device__ __forceinline__ void uv_at(const Corners &q, const float lam[3], float &u, float &v) {
+    u = (lam[0] * q.uv[0][0] + lam[1] * q.uv[1][0]) + lam[2] * q.uv[2][0];
+    v = (lam[0] * q.uv[0][1] + lam[1] * q.uv[1][1]) + lam[2] * q.uv[2][1];
 }
 
 // Mip level nearest the pixel's footprint, in f32: the uv differences to the next pixel right
 // and below, in level-0 texels; level k is taken when rho >= 2^(k - 1/2), i.e.
 // rho^2 >= 2^(2k-1), which needs no logarithm.
-__device__ __forceinline__ uint32_t mip_level(const DiffMesh &m, const DiffView &v, const uint32_t vi[3], const Clip<float> &c,
-                                              float x, float y) {
+// b0: the barycentrics at the pixel corner itself.
+__device__ __forceinline__ uint32_t mip_level(const DiffMesh &m, const DiffView &v, const Corners &q, const Clip<float> &c,
+                                              const Bary<float> &b0, float x, float y) {
     if (m.levels <= 1u) return 0u;
-    Bary<float> b0, bx, by;
-    bary(c, ndc_u(x, v.W), ndc_v(y, v.H), b0);
+    Bary<float> bx, by;
     bary(c, ndc_u(x + 1.0f, v.W), ndc_v(y, v.H), bx);
     bary(c, ndc_u(x, v.W), ndc_v(y + 1.0f, v.H), by);
     float u0, v0, ux, vx, uy, vy;
-    uv_at(m, vi, b0.lam, u0, v0);
-    uv_at(m, vi, bx.lam, ux, vx);
-    uv_at(m, vi, by.lam, uy, vy);
+    uv_at(q, b0.lam, u0, v0);
+    uv_at(q, bx.lam, ux, vx);
+    uv_at(q, by.lam, uy, vy);
     const float tw = (float)m.lw[0], th = (float)m.lh[0];
     const float dux = (ux - u0) * tw, dvx = (vx - v0) * th, duy = (uy - u0) * tw, dvy = (vy - v0) * th;
     const float rx = dux * dux + dvx * dvx, ry = duy * duy + dvy * dvy;
@@ -165,47 +205,9 @@ __device__ __forceinline__ R lerp_r(R a, R b, R f) {
     return a * ((R)1 - f) + b * f;
 }
 
+// The texture filter, the SH light and the curve, from the interpolated attributes in e.
 template <class R>
-__device__ __forceinline__ bool evaluate(const DiffMesh &m, const DiffView &v, const float *sh, uint32_t t, uint32_t x,
-                                         uint32_t y, Eval<R> &e) {
-    if (t >= m.T) return false;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) e.vi[k] = m.idx[3u * t + (uint32_t)k];
-    if (e.vi[0] >= m.V || e.vi[1] >= m.V || e.vi[2] >= m.V) return false;
-    Clip<float> cf;
-    clip_of(m, v, e.vi, cf);
-    e.lvl = mip_level(m, v, e.vi, cf, (float)x, (float)y);
-    // The pixel corner's NDC, the barycentrics and the interpolated attributes in FP64 on both
-    // paths: the homogeneous form is ill-conditioned for slivers seen edge-on (the sum of the
-    // three cross products nearly cancels), where f32 barycentrics drift far from the
-    // mathematical value. Only the texture filter and the curves below run in R.
-    Clip<double> c;
-    clip_of(m, v, e.vi, c);
-    const double ud = (2.0 * (double)x) / (double)v.W - 1.0, vd = 1.0 - (2.0 * (double)y) / (double)v.H;
-    Bary<double> B;
-    bary(c, ud, vd, B);
-    e.u = (R)ud;
-    e.v = (R)vd;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        e.B.a[k] = (R)B.a[k];
-        e.B.b[k] = (R)B.b[k];
-        e.B.l[k] = (R)B.l[k];
-        e.B.lam[k] = (R)B.lam[k];
-    }
-    e.B.S = (R)B.S;
-    const double *L = B.lam;
-#pragma unroll
-    for (int j = 0; j < 3; ++j) {
-        e.col[j] = (R)((L[0] * (double)m.col[3u * e.vi[0] + j] + L[1] * (double)m.col[3u * e.vi[1] + j]) +
-                       L[2] * (double)m.col[3u * e.vi[2] + j]);
-        e.n[j] = (R)((L[0] * (double)m.nrm[3u * e.vi[0] + j] + L[1] * (double)m.nrm[3u * e.vi[1] + j]) +
-                     L[2] * (double)m.nrm[3u * e.vi[2] + j]);
-    }
-#pragma unroll
-    for (int j = 0; j < 2; ++j)
-        e.uvs[j] = (R)((L[0] * (double)m.uv[2u * e.vi[0] + j] + L[1] * (double)m.uv[2u * e.vi[1] + j]) +
-                       L[2] * (double)m.uv[2u * e.vi[2] + j]);
+__device__ __forceinline__ bool shade_tail(const DiffMesh &m, const float *sh, Eval<R> &e) {
     // bilinear_f_x8 (csc.h:143-161) on the chosen level
     const uint32_t lw = m.lw[e.lvl], lh = m.lh[e.lvl];
     const R s = (R)lw * e.uvs[0] - (R)0.5;
@@ -230,8 +232,10 @@ __device__ __forceinline__ bool evaluate(const DiffMesh &m, const DiffView &v, c
             for (int k = 0; k < 4; ++k) atomicOr(m.tex_mark + (taps[k] >> 5), 1u << (taps[k] & 31u));
         }
     }
-    const float4 q00 = T[(uint32_t)e.y0 * lw + (uint32_t)e.x0], q10 = T[(uint32_t)e.y0 * lw + (uint32_t)e.x1],
-                 q01 = T[(uint32_t)e.y1 * lw + (uint32_t)e.x0], q11 = T[(uint32_t)e.y1 * lw + (uint32_t)e.x1];
+    const float4 q00 = load_global(T + (uint32_t)e.y0 * lw + (uint32_t)e.x0),
+                 q10 = load_global(T + (uint32_t)e.y0 * lw + (uint32_t)e.x1),
+                 q01 = load_global(T + (uint32_t)e.y1 * lw + (uint32_t)e.x0),
+                 q11 = load_global(T + (uint32_t)e.y1 * lw + (uint32_t)e.x1);
     e.t00[0] = q00.x; e.t00[1] = q00.y; e.t00[2] = q00.z;
     e.t10[0] = q10.x; e.t10[1] = q10.y; e.t10[2] = q10.z;
     e.t01[0] = q01.x; e.t01[1] = q01.y; e.t01[2] = q01.z;
@@ -255,6 +259,74 @@ __device__ __forceinline__ bool evaluate(const DiffMesh &m, const DiffView &v, c
         e.out[j] = srgb_of(e.lin[j]);
     }
     return true;
+}
+
+// 0: nothing to shade (invalid triangle), 1: shaded into e. With kDefer (the f32 forward), a pixel
+// whose f32 barycentrics are ill-conditioned returns kDeferred instead of running the FP64 path.
+constexpr int kDeferred = 2;
+template <class R, bool kDefer = false>
+__device__ __forceinline__ int evaluate(const DiffMesh &m, const DiffView &v, const float *sh, uint32_t t, uint32_t x,
+                                        uint32_t y, Eval<R> &e) {
+    if (t >= m.T) return 0;
+    const guint tri = (guint)(m.idx + 3u * t);
+    e.vi[0] = tri[0];
+    e.vi[1] = tri[1];
+    e.vi[2] = tri[2];
+    if (e.vi[0] >= m.V || e.vi[1] >= m.V || e.vi[2] >= m.V) return 0;
+    Corners q;
+    load_corners(m, e.vi, q);
+    Clip<float> cf;
+    clip_of(q, v, cf);
+    Bary<float> bf;
+    bary(cf, ndc_u((float)x, v.W), ndc_v((float)y, v.H), bf);
+    e.lvl = mip_level(m, v, q, cf, bf, (float)x, (float)y);
+    if constexpr (sizeof(R) == 4) {
+        // forward: the f32 barycentrics where they are accurate (nearly every pixel)
+        if (well_conditioned(bf)) {
+            e.u = ndc_u((float)x, v.W);
+            e.v = ndc_v((float)y, v.H);
+            e.B = bf;
+            const float *L = bf.lam;
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                e.col[j] = (L[0] * q.c[0][j] + L[1] * q.c[1][j]) + L[2] * q.c[2][j];
+                e.n[j] = (L[0] * q.n[0][j] + L[1] * q.n[1][j]) + L[2] * q.n[2][j];
+            }
+#pragma unroll
+            for (int j = 0; j < 2; ++j) e.uvs[j] = (L[0] * q.uv[0][j] + L[1] * q.uv[1][j]) + L[2] * q.uv[2][j];
+            return shade_tail(m, sh, e) ? 1 : 0;
+        }
+        if constexpr (kDefer) return kDeferred;
+    }
+    // Otherwise (and always for the adjoint's recompute) the pixel corner's NDC, the barycentrics
+    // and the interpolated attributes in FP64: the homogeneous form is ill-conditioned for slivers
+    // seen edge-on (the sum of the three cross products nearly cancels), where f32 barycentrics
+    // drift far from the mathematical value. Only the texture filter and the curves run in R.
+    Clip<double> c;
+    clip_of(q, v, c);
+    const double ud = (2.0 * (double)x) / (double)v.W - 1.0, vd = 1.0 - (2.0 * (double)y) / (double)v.H;
+    Bary<double> B;
+    bary(c, ud, vd, B);
+    e.u = (R)ud;
+    e.v = (R)vd;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        e.B.a[k] = (R)B.a[k];
+        e.B.b[k] = (R)B.b[k];
+        e.B.l[k] = (R)B.l[k];
+        e.B.lam[k] = (R)B.lam[k];
+    }
+    e.B.S = (R)B.S;
+    const double *L = B.lam;
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        e.col[j] = (R)((L[0] * (double)q.c[0][j] + L[1] * (double)q.c[1][j]) + L[2] * (double)q.c[2][j]);
+        e.n[j] = (R)((L[0] * (double)q.n[0][j] + L[1] * (double)q.n[1][j]) + L[2] * (double)q.n[2][j]);
+    }
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+        e.uvs[j] = (R)((L[0] * (double)q.uv[0][j] + L[1] * (double)q.uv[1][j]) + L[2] * (double)q.uv[2][j]);
+    return shade_tail(m, sh, e) ? 1 : 0;
 }
 
 __global__ __launch_bounds__(256) void kd_forward(DiffParams p, float4 *out) {
@@ -679,41 +751,71 @@ uint32_t bits_for(uint32_t n) {
 
 // Largest i < n with start[i] <= b (start[0] == 0, ascending): the owner of block b. The block
 // index is wave-uniform, so the search runs on scalar loads.
+// The last i with start[i] <= b (start ascending, start[0] = 0), by the whole wave at once: each
+// lane compares one entry of each 64-entry chunk and the ballots are counted, so the loads are
+// independent (a binary search would chain log2(n) dependent loads at the start of every block).
+// Every lane of the wave must call it with the same b.
 __device__ __forceinline__ uint32_t block_owner(const uint32_t *start, uint32_t n, uint32_t b) {
-    uint32_t lo = 0u, hi = n;
-    while (hi - lo > 1u) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (start[mid] <= b) lo = mid;
-        else hi = mid;
+    const uint32_t lane = threadIdx.x & 63u;
+    uint32_t cnt = 0u;
+    for (uint32_t c = 0; c < n; c += 64u) {
+        const uint32_t j = c + lane;
+        cnt += (uint32_t)__popcll(__ballot(j < n && start[j] <= b));
     }
-    return lo;
+    return cnt - 1u;
 }
 
 // Mesh shading pixel i of view V, or kNone when its draw is not one this call shades.
+// A view whose draws map to meshes 0, 1, ... in order (identity_map) needs no table load.
 __device__ __forceinline__ uint32_t pixel_mesh(const DiffBatchParams &P, const DiffBatchView &V, uint32_t i) {
     const uint32_t d = V.tags[i] - V.first_tag;
-    return d < V.n_draws ? P.draw_mesh[V.draw_off + d] : kNone;
+    if (d >= V.n_draws) return kNone;
+    return V.identity_map ? d : P.draw_mesh[V.draw_off + d];
 }
 
 // Pixels with no primitive get (0, 0, 0, 0); pixels of draws mapped to no mesh are left as they are.
-// Each block covers one 16x16 tile of one view.
-__global__ __launch_bounds__(256) void kd_batch_forward(DiffBatchParams P) {
+// Each block covers one 16x16 tile of one view. kDefer: the main pass, which leaves pixels whose
+// f32 barycentrics are ill-conditioned (slivers seen edge-on; rare) to the fix-up pass and flags
+// their block; without the FP64 path in it the kernel holds 70 VGPRs instead of 88 (7 waves per
+// SIMD instead of 5). The fix-up pass re-shades the flagged blocks in full (the f32 pixels come out
+// bit-identical) and clears their flags, so the flags are all zero between calls.
+template <bool kDefer>
+__device__ __forceinline__ void batch_forward_pixel(const DiffBatchParams &P) {
     const uint32_t vi = block_owner(P.view_block0, P.n_views, blockIdx.x);
     const DiffBatchView &V = P.views[vi];
     uint32_t x, y;
     tile_pixel(blockIdx.x - P.view_block0[vi], V.v.W, x, y);
     if (x >= V.v.W || y >= V.v.H) return;
     const uint32_t i = y * V.v.W + x;
-    const uint32_t t = V.prim[i];
+    const uint32_t t = V.prim[i], tag = V.tags[i];  // both loads in flight together
     if (t == kNone) {
         V.out[i] = make_float4(0.f, 0.f, 0.f, 0.f);
         return;
     }
-    const uint32_t mi = pixel_mesh(P, V, i);
+    const uint32_t d = tag - V.first_tag;
+    const uint32_t mi = d >= V.n_draws ? kNone : (V.identity_map ? d : P.draw_mesh[V.draw_off + d]);
     if (mi >= P.n_meshes) return;
     Eval<float> e;
-    V.out[i] = evaluate(P.meshes[mi].m, V.v, P.sh, t, x, y, e) ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f)
-                                                                : make_float4(0.f, 0.f, 0.f, 0.f);
+    // A wave's pixels mostly show one mesh: its record (pointers, sizes, level tables) is then
+    // read with scalar loads through a wave-uniform index, off the vector-memory path.
+    const uint32_t m0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)mi);
+    const int r = __ballot(mi != m0) == 0ull ? evaluate<float, kDefer>(P.meshes[m0].m, V.v, P.sh, t, x, y, e)
+                                             : evaluate<float, kDefer>(P.meshes[mi].m, V.v, P.sh, t, x, y, e);
+    if (kDefer && r == kDeferred) {
+        P.defer[blockIdx.x] = 1u;
+        return;
+    }
+    V.out[i] = r ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+__global__ __launch_bounds__(256) void kd_batch_forward(DiffBatchParams P) { batch_forward_pixel<true>(P); }
+
+__global__ __launch_bounds__(256) void kd_batch_forward_fix(DiffBatchParams P) {
+    const uint32_t flagged = P.defer[blockIdx.x];
+    __syncthreads();  // every wave has read the flag before it is cleared
+    if (!flagged) return;
+    if (threadIdx.x == 0u) P.defer[blockIdx.x] = 0u;
+    batch_forward_pixel<false>(P);
 }
 
 __device__ __forceinline__ GradSink mesh_sink(const DiffBatchMesh &B) {
@@ -935,6 +1037,7 @@ hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const D
 hipError_t launch_diff_batch_forward(const DiffBatchParams &p, hipStream_t s) {
     if (p.n_blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(kd_batch_forward, dim3(p.n_blocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(kd_batch_forward_fix, dim3(p.n_blocks), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

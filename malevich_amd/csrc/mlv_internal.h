@@ -226,7 +226,8 @@ struct DiffBatchMesh {
 struct DiffBatchView {
     DiffView v;
     const uint32_t *prim, *tags;
-    uint32_t first_tag, n_draws, draw_off, pad;
+    uint32_t first_tag, n_draws, draw_off;
+    uint32_t identity_map;    // draw d of the view shades mesh d (d < n_meshes): no draw_mesh lookup
     float4 *out;              // forward output
     const float4 *gout;       // backward input
 };
@@ -245,6 +246,7 @@ struct DiffBatchParams {
     const uint32_t *zero_block0;          // [n_zero] first block of each segment (1024 words a block)
     uint32_t n_meshes, n_views, n_blocks, n_pos_blocks, n_zero, n_zero_blocks;
     const float *sh;
+    uint32_t *defer;                      // forward: per block, pixels left to the fix-up pass (zero between calls)
 };
 // Scratch of the deterministic backward: per-pixel contributions, their sort keys and order.
 struct DiffScratch {

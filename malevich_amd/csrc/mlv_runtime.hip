@@ -136,6 +136,7 @@ struct mlv_context {
     Arena d_vposd, d_posacc;
     // batched differentiable shading: device tables, FP64 position sums, SH partials
     Arena b_tables, b_posacc, b_shpart;
+    Arena b_defer;  // batched forward: per-block flags for the fix-up pass, all zero between calls
     Arena present_rgb;  // packed RGB of the last present
     // footprint counting (mlv_context_set_footprint_counting)
     bool counting = false;
@@ -664,7 +665,7 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
                     &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
                     &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart, &ctx->present_rgb, &ctx->cameras,
-                    &ctx->tri_mark, &ctx->diff_mark};
+                    &ctx->tri_mark, &ctx->diff_mark, &ctx->b_defer};
     for (Arena *a : all) release(*a);
     free_retired(ctx);
     resolve_events(ctx);
@@ -1908,7 +1909,11 @@ static int diff_batch_setup(mlv_context *ctx, const mlv_diff_mesh *meshes, uint3
         o.first_tag = w.first_tag;
         o.n_draws = w.draw_count;
         o.draw_off = (uint32_t)draw_mesh.size();
-        for (uint32_t d = 0; d < w.draw_count; ++d) draw_mesh.push_back(w.draw_mesh[d]);
+        o.identity_map = 1u;
+        for (uint32_t d = 0; d < w.draw_count; ++d) {
+            draw_mesh.push_back(w.draw_mesh[d]);
+            if (w.draw_mesh[d] != d || d >= n_meshes) o.identity_map = 0u;
+        }
         o.out = (float4 *)w.rgba;
         o.gout = (const float4 *)w.grad_rgba;
         view_block0[v] = (uint32_t)blocks;
@@ -2024,6 +2029,11 @@ MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes
     RC_OR_RETURN(flush(ctx));  // the id buffers come from recorded draws
     DiffBatchParams P;
     RC_OR_RETURN(diff_batch_setup(ctx, meshes, mesh_count, views, view_count, device_sh, false, false, P));
+    const void *old = ctx->b_defer.p;
+    RC_OR_RETURN(ensure(ctx, ctx->b_defer, (size_t)std::max(P.n_blocks, 1u) * 4u));
+    if (ctx->b_defer.p != old)  // new memory; afterwards the fix-up pass leaves every flag cleared
+        HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->b_defer.p, 0u, ctx->b_defer.cap / 4u, ctx->stream));
+    P.defer = (uint32_t *)ctx->b_defer.p;
     HIP_OR_FAIL(ctx, launch_diff_batch_forward(P, ctx->stream));
     if (ctx->counting)  // RGBA32F chains: 16 B a texel
         HIP_OR_FAIL(ctx, launch_popcount((const uint32_t *)ctx->diff_mark.p, ctx->diff_mark_words, 16u,

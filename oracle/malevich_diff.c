@@ -71,7 +71,8 @@ static void bary_f(const float X[3], const float Y[3], const float Wc[3], float 
     l[1] = a[2] * b[0] - a[0] * b[2];
     l[2] = a[0] * b[1] - a[1] * b[0];
     float S = (l[0] + l[1]) + l[2];
-    for (int k = 0; k < 3; ++k) lam[k] = l[k] / S;
+    const float inv = 1.0f / S; /* one division, three products */
+    for (int k = 0; k < 3; ++k) lam[k] = l[k] * inv;
 }
 
 static void uv_at_f(const float *uv, const uint32_t vi[3], const float lam[3], float *u, float *v) {
