@@ -15,6 +15,15 @@
 
 namespace mlv {
 
+// The dispatcher deals workgroups to the 8 XCDs in turn (workgroup b to XCD b % 8), and each
+// XCD has its own L2. Numbering a grid of n blocks so that XCD x runs the x-th contiguous share
+// keeps neighbouring blocks, which gather the same vertices, under one L2.
+__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
+    const uint32_t x = b & 7u, q = b >> 3, per = n >> 3, rem = n & 7u;
+    return x * per + min(x, rem) + q;
+}
+
+
 // ---------------------------------------------------------------- math.h
 
 // v4f32_dot, math.h:137-140: ((x*x' + y*y') + z*z') + w*w'

@@ -68,8 +68,10 @@ struct Clip {
 
 // A triangle's three corners as the shading reads them: loaded once, each vertex's position,
 // colour and normal as one 12-byte load and its uv as one 8-byte load (the attribute arrays are
-// tightly packed float triples and pairs, 4-byte aligned), so a pixel issues 13 vector loads for
-// its triangle instead of 36 single-float gathers.
+// tightly packed float triples and pairs, 4-byte aligned), so a pixel issues 12 vector loads for
+// its triangle instead of 33 single-float gathers. Positions and uvs come first (they place the
+// texel taps); colours and normals are loaded with the taps, so they hold no registers while the
+// barycentrics and the mip level are computed.
 struct Corners {
     float p[3][3], c[3][3], uv[3][2], n[3][3];
 };
@@ -85,13 +87,18 @@ __device__ __forceinline__ void ld_floats(const float *q, float (&o)[N]) {
     for (int j = 0; j < N; ++j) o[j] = g[j];
 }
 
-__device__ __forceinline__ void load_corners(const DiffMesh &m, const uint32_t vi[3], Corners &q) {
+__device__ __forceinline__ void load_placement(const DiffMesh &m, const uint32_t vi[3], Corners &q) {
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         ld_floats<3>(m.pos + 3u * vi[k], q.p[k]);
+        ld_floats<2>(m.uv + 2u * vi[k], q.uv[k]);
+    }
+}
+__device__ __forceinline__ void load_surface(const DiffMesh &m, const uint32_t vi[3], Corners &q) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
         ld_floats<3>(m.col + 3u * vi[k], q.c[k]);
         ld_floats<3>(m.nrm + 3u * vi[k], q.n[k]);
-        ld_floats<2>(m.uv + 2u * vi[k], q.uv[k]);
     }
 }
 
@@ -274,7 +281,7 @@ __device__ __forceinline__ int evaluate(const DiffMesh &m, const DiffView &v, co
     e.vi[2] = tri[2];
     if (e.vi[0] >= m.V || e.vi[1] >= m.V || e.vi[2] >= m.V) return 0;
     Corners q;
-    load_corners(m, e.vi, q);
+    load_placement(m, e.vi, q);
     Clip<float> cf;
     clip_of(q, v, cf);
     Bary<float> bf;
@@ -283,6 +290,7 @@ __device__ __forceinline__ int evaluate(const DiffMesh &m, const DiffView &v, co
     if constexpr (sizeof(R) == 4) {
         // forward: the f32 barycentrics where they are accurate (nearly every pixel)
         if (well_conditioned(bf)) {
+            load_surface(m, e.vi, q);
             e.u = ndc_u((float)x, v.W);
             e.v = ndc_v((float)y, v.H);
             e.B = bf;
@@ -302,6 +310,7 @@ __device__ __forceinline__ int evaluate(const DiffMesh &m, const DiffView &v, co
     // and the interpolated attributes in FP64: the homogeneous form is ill-conditioned for slivers
     // seen edge-on (the sum of the three cross products nearly cancels), where f32 barycentrics
     // drift far from the mathematical value. Only the texture filter and the curves run in R.
+    load_surface(m, e.vi, q);
     Clip<double> c;
     clip_of(q, v, c);
     const double ud = (2.0 * (double)x) / (double)v.W - 1.0, vd = 1.0 - (2.0 * (double)y) / (double)v.H;
@@ -592,10 +601,13 @@ __device__ __forceinline__ void tables_flush(const VtxTable &vt, const TexTable 
 }
 
 // Pixel of thread t in tile `tile` of a W-wide image (16x16 tiles, row-major).
-__device__ __forceinline__ void tile_pixel(uint32_t tile, uint32_t W, uint32_t &x, uint32_t &y) {
+__device__ __forceinline__ void tile_pixel(uint32_t tile, uint32_t W, uint32_t tid, uint32_t &x, uint32_t &y) {
     const uint32_t tx = (W + kTileSide - 1u) / kTileSide;
-    x = (tile % tx) * kTileSide + (threadIdx.x & (kTileSide - 1u));
-    y = (tile / tx) * kTileSide + threadIdx.x / kTileSide;
+    x = (tile % tx) * kTileSide + (tid & (kTileSide - 1u));
+    y = (tile / tx) * kTileSide + tid / kTileSide;
+}
+__device__ __forceinline__ void tile_pixel(uint32_t tile, uint32_t W, uint32_t &x, uint32_t &y) {
+    tile_pixel(tile, W, threadIdx.x, x, y);
 }
 
 __global__ __launch_bounds__(256) void kd_backward_atomic(DiffParams p, const float4 *gout, DiffGrads g,
@@ -774,17 +786,16 @@ __device__ __forceinline__ uint32_t pixel_mesh(const DiffBatchParams &P, const D
 }
 
 // Pixels with no primitive get (0, 0, 0, 0); pixels of draws mapped to no mesh are left as they are.
-// Each block covers one 16x16 tile of one view. kDefer: the main pass, which leaves pixels whose
-// f32 barycentrics are ill-conditioned (slivers seen edge-on; rare) to the fix-up pass and flags
-// their block; without the FP64 path in it the kernel holds 70 VGPRs instead of 88 (7 waves per
-// SIMD instead of 5). The fix-up pass re-shades the flagged blocks in full (the f32 pixels come out
-// bit-identical) and clears their flags, so the flags are all zero between calls.
+// Each block covers one 16x16 tile of one view; pixel `tid` of block `blk`. kDefer: the main pass,
+// which leaves the pixels whose f32 barycentrics are ill-conditioned (slivers seen edge-on; rare)
+// to the fix-up pass: each wave with such pixels appends its (block, wave) to a list. Without the
+// FP64 path in it the kernel holds 70 VGPRs instead of 88 (7 waves per SIMD instead of 5).
 template <bool kDefer>
-__device__ __forceinline__ void batch_forward_pixel(const DiffBatchParams &P) {
-    const uint32_t vi = block_owner(P.view_block0, P.n_views, blockIdx.x);
+__device__ __forceinline__ void batch_forward_pixel(const DiffBatchParams &P, uint32_t blk, uint32_t tid) {
+    const uint32_t vi = block_owner(P.view_block0, P.n_views, blk);
     const DiffBatchView &V = P.views[vi];
     uint32_t x, y;
-    tile_pixel(blockIdx.x - P.view_block0[vi], V.v.W, x, y);
+    tile_pixel(blk - P.view_block0[vi], V.v.W, tid, x, y);
     if (x >= V.v.W || y >= V.v.H) return;
     const uint32_t i = y * V.v.W + x;
     const uint32_t t = V.prim[i], tag = V.tags[i];  // both loads in flight together
@@ -802,20 +813,39 @@ __device__ __forceinline__ void batch_forward_pixel(const DiffBatchParams &P) {
     const int r = __ballot(mi != m0) == 0ull ? evaluate<float, kDefer>(P.meshes[m0].m, V.v, P.sh, t, x, y, e)
                                              : evaluate<float, kDefer>(P.meshes[mi].m, V.v, P.sh, t, x, y, e);
     if (kDefer && r == kDeferred) {
-        P.defer[blockIdx.x] = 1u;
+        // the active lanes here are the deferred ones: the lowest appends the wave
+        if ((tid & 63u) == (uint32_t)__ffsll((unsigned long long)__ballot(1)) - 1u)
+            P.defer_list[atomicAdd(&P.defer[0], 1u)] = blk * 4u + (tid >> 6);
         return;
     }
     V.out[i] = r ? make_float4(e.out[0], e.out[1], e.out[2], 1.0f) : make_float4(0.f, 0.f, 0.f, 0.f);
 }
 
-__global__ __launch_bounds__(256) void kd_batch_forward(DiffBatchParams P) { batch_forward_pixel<true>(P); }
+// One block per tile; XCD x takes the x-th contiguous share of the tiles, so neighbouring tiles,
+// which gather the same corners, share an L2. (A resident grid looping over tiles measured slower:
+// the loop costs VGPRs, 97 against 72.)
+__global__ __launch_bounds__(256) void kd_batch_forward(DiffBatchParams P) {
+    batch_forward_pixel<true>(P, xcd_contiguous(blockIdx.x, gridDim.x), threadIdx.x);
+}
 
+// The fix-up pass: each wave takes listed waves of the main pass and shades them in full (their
+// f32 pixels come out as before). The last block to finish zeroes the list's count and its own
+// finish count, so both are zero between calls.
 __global__ __launch_bounds__(256) void kd_batch_forward_fix(DiffBatchParams P) {
-    const uint32_t flagged = P.defer[blockIdx.x];
-    __syncthreads();  // every wave has read the flag before it is cleared
-    if (!flagged) return;
-    if (threadIdx.x == 0u) P.defer[blockIdx.x] = 0u;
-    batch_forward_pixel<false>(P);
+    const uint32_t n = P.defer[0];
+    const uint32_t lane = threadIdx.x & 63u;
+    for (uint32_t k = blockIdx.x * 4u + (threadIdx.x >> 6); k < n; k += gridDim.x * 4u) {
+        const uint32_t w = P.defer_list[k];
+        batch_forward_pixel<false>(P, w >> 2, (w & 3u) * 64u + lane);
+    }
+    __syncthreads();  // every wave of the block has read the count
+    if (threadIdx.x == 0u) {
+        __threadfence();
+        if (atomicAdd(&P.defer[1], 1u) == gridDim.x - 1u) {
+            P.defer[0] = 0u;
+            P.defer[1] = 0u;
+        }
+    }
 }
 
 __device__ __forceinline__ GradSink mesh_sink(const DiffBatchMesh &B) {
@@ -827,10 +857,11 @@ __global__ __launch_bounds__(256) void kd_batch_backward(DiffBatchParams P, doub
     __shared__ TexTable tt;
     tables_clear(vt, tt);
     __syncthreads();
-    const uint32_t vi = block_owner(P.view_block0, P.n_views, blockIdx.x);
+    const uint32_t blk = xcd_contiguous(blockIdx.x, gridDim.x);  // neighbouring tiles under one L2
+    const uint32_t vi = block_owner(P.view_block0, P.n_views, blk);
     const DiffBatchView &V = P.views[vi];
     uint32_t x, y;
-    tile_pixel(blockIdx.x - P.view_block0[vi], V.v.W, x, y);
+    tile_pixel(blk - P.view_block0[vi], V.v.W, x, y);
     Eval<double> e;
     Adj a;
     bool live = false;
@@ -1034,10 +1065,12 @@ hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const D
     return hipGetLastError();
 }
 
+constexpr uint32_t kFixBlocks = 512;  // the fix-up pass's grid (its list is short)
+
 hipError_t launch_diff_batch_forward(const DiffBatchParams &p, hipStream_t s) {
     if (p.n_blocks == 0) return hipSuccess;
     hipLaunchKernelGGL(kd_batch_forward, dim3(p.n_blocks), dim3(256), 0, s, p);
-    hipLaunchKernelGGL(kd_batch_forward_fix, dim3(p.n_blocks), dim3(256), 0, s, p);
+    hipLaunchKernelGGL(kd_batch_forward_fix, dim3(std::min(p.n_blocks, kFixBlocks)), dim3(256), 0, s, p);
     return hipGetLastError();
 }
 

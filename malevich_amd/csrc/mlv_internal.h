@@ -246,7 +246,8 @@ struct DiffBatchParams {
     const uint32_t *zero_block0;          // [n_zero] first block of each segment (1024 words a block)
     uint32_t n_meshes, n_views, n_blocks, n_pos_blocks, n_zero, n_zero_blocks;
     const float *sh;
-    uint32_t *defer;                      // forward: per block, pixels left to the fix-up pass (zero between calls)
+    uint32_t *defer;                      // forward: [0] waves listed for the fix-up pass, [1] its finished blocks
+    uint32_t *defer_list;                 // forward: (block << 2 | wave) of each listed wave
 };
 // Scratch of the deterministic backward: per-pixel contributions, their sort keys and order.
 struct DiffScratch {

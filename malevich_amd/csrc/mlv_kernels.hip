@@ -75,14 +75,6 @@ __device__ __forceinline__ T block_exclusive_scan(T v, T *s_wave, T &total) {
     return r;
 }
 
-// The dispatcher deals workgroups to the 8 XCDs in turn (workgroup b to XCD b % 8), and each
-// XCD has its own L2. Numbering a grid of n blocks so that XCD x runs the x-th contiguous share
-// keeps neighbouring blocks, which gather the same vertices, under one L2.
-__device__ __forceinline__ uint32_t xcd_contiguous(uint32_t b, uint32_t n) {
-    const uint32_t x = b & 7u, q = b >> 3, per = n >> 3, rem = n & 7u;
-    return x * per + min(x, rem) + q;
-}
-
 // Largest index lo in [0, n) with key(lo) <= x, for ascending keys with key(0) <= x.
 template <class F>
 __device__ __forceinline__ uint32_t upper_index(uint32_t n, uint32_t x, F key) {

@@ -136,7 +136,7 @@ struct mlv_context {
     Arena d_vposd, d_posacc;
     // batched differentiable shading: device tables, FP64 position sums, SH partials
     Arena b_tables, b_posacc, b_shpart;
-    Arena b_defer;  // batched forward: per-block flags for the fix-up pass, all zero between calls
+    Arena b_defer;  // batched forward: the fix-up pass's counts (zero between calls) and wave list
     Arena present_rgb;  // packed RGB of the last present
     // footprint counting (mlv_context_set_footprint_counting)
     bool counting = false;
@@ -2030,10 +2030,11 @@ MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes
     DiffBatchParams P;
     RC_OR_RETURN(diff_batch_setup(ctx, meshes, mesh_count, views, view_count, device_sh, false, false, P));
     const void *old = ctx->b_defer.p;
-    RC_OR_RETURN(ensure(ctx, ctx->b_defer, (size_t)std::max(P.n_blocks, 1u) * 4u));
-    if (ctx->b_defer.p != old)  // new memory; afterwards the fix-up pass leaves every flag cleared
-        HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->b_defer.p, 0u, ctx->b_defer.cap / 4u, ctx->stream));
+    RC_OR_RETURN(ensure(ctx, ctx->b_defer, 64u + (size_t)P.n_blocks * 16u));
+    if (ctx->b_defer.p != old)  // new memory; afterwards the fix-up pass leaves its counts at zero
+        HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->b_defer.p, 0u, 16u, ctx->stream));
     P.defer = (uint32_t *)ctx->b_defer.p;
+    P.defer_list = P.defer + 16;
     HIP_OR_FAIL(ctx, launch_diff_batch_forward(P, ctx->stream));
     if (ctx->counting)  // RGBA32F chains: 16 B a texel
         HIP_OR_FAIL(ctx, launch_popcount((const uint32_t *)ctx->diff_mark.p, ctx->diff_mark_words, 16u,
