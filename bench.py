@@ -23,7 +23,8 @@ sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBS = 8000.0  # MI355X HBM3E, 8 TB/s
 # the library's launch parameters before any autotuning (mlv_tuning)
-DEFAULT_TUNING = dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048, tile_stage=256)
+DEFAULT_TUNING = dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048, tile_stage=256, tile_edge=16,
+                      tile_waves_per_cu=0, setup_waves_per_cu=0, skip_cornerless=1)
 
 
 class HipEvents:

@@ -296,22 +296,30 @@ MLV_API int mlv_graph_capture_end(mlv_context *ctx, mlv_graph **out);
 MLV_API int mlv_graph_launch(mlv_context *ctx, mlv_graph *graph);
 MLV_API int mlv_graph_destroy(mlv_context *ctx, mlv_graph *graph);
 
-/* ---- launch tuning (extension; no reference counterpart)
+/* ---- launch tuning (extension; no reference counterpart; the reference's tile is fixed at
+ * 8x8 bins, main.c:24-26)
  *
- * The per-context launch parameters of the tile and binning stages. mlv_autotune_begin /
+ * The per-context launch parameters of the setup, binning and tile stages. mlv_autotune_begin /
  * mlv_autotune_end record a frame exactly as a graph capture does (only state, clears and
- * draws in between), then run it repeatedly under candidate parameters (coordinate descent:
- * tile staging, heavy-tile threshold, heavy-tile blocks, then pair blocks). Each candidate is
- * timed as the fastest of `repeats` runs (HIP events on the context stream, after one warm-up
- * run) and replaces the incumbent only if it is more than 1% faster. Results are the same under every candidate;
- * the targets hold the recorded frame afterwards. The bin-list capacity follows the largest
- * pair count measured (with a quarter of headroom) and is reported in pair_capacity. */
+ * draws in between), then run it repeatedly under candidate parameters (coordinate descent, in
+ * the order of the fields below: skip_cornerless, tile_edge, tile_stage, tile_waves_per_cu,
+ * setup_waves_per_cu, heavy_tile_pairs, heavy_blocks, pair_blocks). Each candidate is timed as
+ * the fastest of `repeats` runs (HIP events on the context stream, after one warm-up run) and
+ * replaces the incumbent only if it is more than 1% faster. Results (frames and stats) are the
+ * same under every candidate; the targets hold the recorded frame afterwards. The bin-list
+ * capacity follows the largest pair count measured (with a quarter of headroom) and is reported
+ * in pair_capacity. */
 typedef struct mlv_tuning {
     uint32_t heavy_tile_pairs; /* a 16x16 tile with a longer (triangle, tile) list is walked by 16 waves */
     uint32_t heavy_blocks;     /* blocks (1024 threads) of the persistent heavy-tile kernel */
     uint32_t pair_blocks;      /* blocks (256 threads) of the pair count and emit kernels */
-    uint32_t tile_stage;       /* records of a tile's list the tile kernel stages in LDS at a time: 128 or 256 */
+    uint32_t tile_stage;       /* bin capacity: records of a tile's list held in LDS at a time, 128 or 256 */
     uint32_t pair_capacity;    /* out: bin-list capacity (pairs) of the last flush; ignored by set */
+    uint32_t tile_edge;        /* screen tile edge of the binning, 16 or 32 pixels (one wave per 8x8 bin) */
+    uint32_t tile_waves_per_cu;  /* 0, or about this many waves of the tile kernel per CU at most (an LDS cap) */
+    uint32_t setup_waves_per_cu; /* likewise for the setup kernels */
+    uint32_t skip_cornerless;  /* 1: an assembled triangle whose snapped bounding box holds no pixel corner
+                                  covers nothing; it is counted in the stats but not binned */
 } mlv_tuning;
 MLV_API int mlv_get_tuning(mlv_context *ctx, mlv_tuning *out);
 MLV_API int mlv_set_tuning(mlv_context *ctx, const mlv_tuning *tuning);

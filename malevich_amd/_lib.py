@@ -79,7 +79,8 @@ class Camera(C.Structure):
 class Tuning(C.Structure):
     """mlv_tuning"""
     _fields_ = [(n, C.c_uint32) for n in ("heavy_tile_pairs", "heavy_blocks", "pair_blocks", "tile_stage",
-                                          "pair_capacity")]
+                                          "pair_capacity", "tile_edge", "tile_waves_per_cu", "setup_waves_per_cu",
+                                          "skip_cornerless")]
 
 
 class DiffMesh(C.Structure):

@@ -521,4 +521,11 @@ __device__ __forceinline__ void rect_of(const TriVis &tv, int32_t &bx0, int32_t 
     by1 = (int32_t)(int16_t)(tv.rect1 >> 16);
 }
 
+// Whether the snapped bounding box of a set-up triangle holds a pixel corner of the view. Coverage
+// is sampled at pixel corners only (main.c:1015-1025), so a triangle without one covers nothing.
+__device__ __forceinline__ bool has_corner(const TriVis &tv) {
+    return (int32_t)(int16_t)(tv.lat_x & 0xFFFFu) <= (int32_t)(int16_t)(tv.lat_x >> 16) &&
+           (int32_t)(int16_t)(tv.lat_y & 0xFFFFu) <= (int32_t)(int16_t)(tv.lat_y >> 16);
+}
+
 }  // namespace mlv

@@ -8,10 +8,16 @@
 //   views[NV]        ViewRec   per render target (colour/depth/hi-Z/visibility pointers, tile grid)
 //   vpos[Vt]         float4    clip-space position per vertex slot (VS register 0)
 //   vattr[2*(Vt+C)]  float4    VS registers 1..2 per vertex slot; rows >= Vt hold clipper vertices
-//   cnt[Tt]          u32       per input-triangle slot: (#assembled) | (#clip vertices << 16)
-//   tvis[N]          TriVis    80 B per assembled triangle (edge functions, depth plane, bin rect)
-//   tshade[N]        TriShade  32 B per assembled triangle (1/w, attribute rows)
-//   tbin[N]          uint4     16 B per assembled triangle (bin rect and its view's tile grid)
+//   cnt[Tt]          u32       per input-triangle slot: #binned | #assembled << 8 | #clip vertices << 16
+//   tvis[N]          TriVis    80 B per binned triangle (edge functions, depth plane, bin rect)
+//   tshade[N]        TriShade  32 B per binned triangle (1/w, attribute rows, numbers)
+//   tbin[N]          uint4     16 B per binned triangle (bin rect and its view's tile grid)
+//   cmark            u32       bitmap, one bit per (draw, 8x8 bin of its view): bins met by a
+//                              triangle that was assembled but not binned (zero between flushes)
+// An assembled triangle whose snapped bounding box holds no pixel corner covers no pixel
+// (coverage is sampled at pixel corners only), so it is not binned: it counts in the stats, its
+// bins in the bitmap. Binned triangles are numbered in draw order, then submission order; each
+// also carries its number among the draw's assembled triangles (the reference's triangle id).
 //   pair keys/vals   u32       (global tile, triangle) pairs, stably sorted by tile
 //   tile_start/end   u32       per global tile (tiles of all views, view-major): its pair range
 // Slots of draw d: vertices [vbase, vbase+V_d), input triangles [tbase, tbase+T_d); both
@@ -24,7 +30,7 @@
 
 namespace mlv {
 
-constexpr int kTile = 16;          // screen tile edge for binning (pixels)
+constexpr int kTile = 16;          // default screen tile edge for binning (pixels); 32 is the other choice
 constexpr int kBin = 8;            // reference bin edge (main.c:24-25): hierarchical-Z granularity
 constexpr int kBlock = 256;        // threads per block of the vertex and setup stages
 constexpr int kMaxClip = 16;       // MAX_NUM_CLIP_VERTICES (main.c:38)
@@ -39,6 +45,8 @@ constexpr uint32_t kHeavyMinDefault = 0xFFFFFFFFu, kHeavyBlocksDefault = 1024, k
 // The tile kernel stages its list tile_stage records at a time: 256 (22.5 KB of LDS, 7 blocks per
 // CU) or 128 (half the LDS: 8 blocks per CU, more barriers for long lists).
 constexpr uint32_t kTileStageDefault = 256;
+constexpr uint32_t kLdsPerCu = 160u * 1024u;   // gfx950
+constexpr uint32_t kLdsPerBlockMax = 64u * 1024u;
 
 enum VsKind { VS_BASIC = 1, VS_VERTEX_LIGHTING = 2, VS_PASSTHROUGH = 3, VS_FULLSCREEN = 4 };
 // PS_BASIC_LAMBERT and PS_ENV_TEXTURED are extensions (mip-mapped fetch, a light, a second
@@ -78,7 +86,7 @@ struct DrawRec {
     uint32_t vbase, tbase;        // first vertex slot / first input-triangle slot
     uint32_t blk_v0, blk_t0;      // first block of the vertex / setup stage
     uint32_t draw_tag;            // written to the visibility buffer
-    uint32_t pad;
+    uint32_t cbase;               // first bit of this draw's bins in BatchParams::cmark
 };
 
 struct ViewRec {
@@ -92,9 +100,9 @@ struct ViewRec {
     int32_t depth_func;
 };
 
-// One assembled triangle as the tile kernel reads it (80 B). Its index in the assembled array
-// is its order key (draw order, then submission order); the reference's per-draw number is that
-// index minus the draw's first index (BatchParams::asm_base).
+// One binned triangle as the tile kernel reads it (80 B). Its index in the binned array is its
+// order key (draw order, then submission order); the reference's per-draw number is
+// TriShade::num.
 // a16/b16 are the edge coefficients pre-multiplied by 16 (mod 2^32), so that
 // E = a16*x + b16*y + c equals ((a*x)<<4) + ((b*y)<<4) + c of main.c:1015-1025.
 struct alignas(16) TriVis {
@@ -116,16 +124,16 @@ struct alignas(16) TriShade {
     float rw[3];             // a_reciprocal_ws (main.c:868-870)
     uint32_t row[3];         // attribute rows of the three vertices in vattr
     uint32_t prim;           // input triangle index within the draw
-    uint32_t sub;            // fan index within a clipped polygon
+    uint32_t num;            // number among the draw's assembled triangles (the visibility id)
 };
 static_assert(sizeof(TriShade) == 32, "TriShade layout");
 
 constexpr size_t kMaxBatchDraws = 1u << 16;  // draws per flush; the tile kernels keep draw indices in 16 bits
 
 struct BatchCounters {
-    unsigned long long n_assembled_clip;   // packed scan total: assembled | clip vertices << 32
-    uint32_t n_assembled;
-    uint32_t n_clipverts;
+    unsigned long long scan_total;         // setup scan total: binned | assembled << 32
+    uint32_t n_assembled;                  // binned triangles (the tile stage's triangles)
+    uint32_t n_clipverts;                  // clipper vertex rows taken
     uint32_t n_clip_list;
     uint32_t n_pairs;
     uint32_t overflow;
@@ -150,6 +158,7 @@ struct DevStats {
     // kb_tile's active-bin counts, one atomic per tile spread over 64 counters (so that no
     // single address takes thousands of atomics); the host adds them to active8
     unsigned long long active8_part[64];
+    unsigned long long pairs8_part[64];    // (triangle, 8x8 bin) pairs of unbinned triangles, spread likewise
 };
 
 struct BatchParams {
@@ -165,9 +174,12 @@ struct BatchParams {
     uint32_t vattr_rows;                  // capacity in rows
     uint32_t *cnt;
     unsigned long long *blocksum, *blockoff;  // per setup block: assembled | clipverts << 32
-    uint32_t *asm_base;                   // per draw: first global assembled index
+    uint32_t *asm_base;                   // per draw: its first binned triangle's index
+    uint32_t *cmark;                      // (draw, bin) bits of unbinned triangles (NULL: all are binned)
+    uint32_t cmark_words;
     uint2 *clip_list;       // (triangle slot, draw) of each triangle the clipper takes
-    uint2 *clip_off;
+    uint2 *clip_off;                      // per clipped slot: first binned index, first number in its draw
+    uint32_t *clip_rows;                  // per clipped slot: its first clipper vertex row (minus vslots)
     TriVis *tvis;
     TriShade *tshade;
     uint4 *tbin;                          // per assembled triangle: rect0, rect1, tile_base, tiles_x of its view
@@ -180,6 +192,8 @@ struct BatchParams {
     uint32_t *tile_order;                 // the order the tile kernel takes tiles in: long lists first
     uint32_t heavy_min, heavy_blocks, pair_blocks;
     uint32_t tile_stage;                  // records the tile kernel stages in LDS at a time (128 or 256)
+    uint32_t tile_shift;                  // tiles of 2^tile_shift x 2^tile_shift bins: 1 (16x16 px) or 2 (32x32)
+    uint32_t tile_lds_pad, setup_lds_pad; // dynamic LDS of the tile / setup kernels (caps their blocks per CU)
     uint32_t *scan_status;                // the scans' status words, cleared by the first stage
     uint32_t scan_status_words;
     BatchCounters *ctr;
@@ -330,6 +344,9 @@ hipError_t launch_srgb_to_linear_rgba8(uint32_t *texels, size_t n, hipStream_t s
 // levels 1 .. mip_levels-1 of the texture's chain from level 0, in order
 hipError_t launch_generate_mips(const TexDesc &t, hipStream_t s);
 // adds mul * (set bits of words[0..n)) to *out
+hipError_t launch_count_clear(uint32_t *words, size_t n, unsigned long long *out, hipStream_t s);
+uint32_t tile_kernel_lds(uint32_t tile_stage, uint32_t shift);
+uint32_t setup_kernel_lds();
 hipError_t launch_popcount(const uint32_t *words, size_t n, uint32_t mul, unsigned long long *out_bits,
                            unsigned long long *out_bytes, hipStream_t s);
 hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s);

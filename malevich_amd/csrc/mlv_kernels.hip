@@ -55,6 +55,33 @@ __device__ __forceinline__ T wave_sum(T x) {
     return x;
 }
 
+// Whole-wave reductions (every lane of the wave active).
+__device__ __forceinline__ uint32_t wave_or(uint32_t x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x |= (uint32_t)__shfl_xor((int)x, d, 64);
+    return x;
+}
+__device__ __forceinline__ uint32_t wave_max(uint32_t x) {
+#pragma unroll
+    for (int d = 32; d >= 1; d >>= 1) x = max(x, (uint32_t)__shfl_xor((int)x, d, 64));
+    return x;
+}
+
+// Sets bit `bit` in `words` for every lane with `has`, one atomic per distinct word of the wave
+// (the lanes' bits are mostly neighbours). Every lane of the wave must be active.
+__device__ __forceinline__ void wave_mark(uint32_t *words, bool has, uint32_t bit) {
+    uint64_t pend = __ballot(has);
+    const uint32_t lane = threadIdx.x & 63u;
+    while (pend) {
+        const uint32_t lead = (uint32_t)__builtin_ctzll(pend);
+        const uint32_t w = (uint32_t)__builtin_amdgcn_readlane((int)(bit >> 5), (int)lead);
+        const bool mine = has && (bit >> 5) == w;
+        const uint32_t v = wave_or(mine ? 1u << (bit & 31u) : 0u);
+        if (lane == lead) atomicOr(words + w, v);
+        pend &= ~__ballot(mine);
+    }
+}
+
 // Block-wide exclusive scan for NT threads (NT a multiple of 64, <= 1024). s_wave holds NT/64+1.
 template <int NT, class T>
 __device__ __forceinline__ T block_exclusive_scan(T v, T *s_wave, T &total) {
@@ -646,7 +673,7 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
 #pragma unroll
         for (int k = 0; k < 3; ++k) q[h][k] = p.vpos[ok[h] ? vb + vi[h][k] : 0u];
     }
-    uint32_t packed[kCountPer];
+    uint32_t packed[kCountPer];  // binned | assembled << 16
     int cls[kCountPer];
 #pragma unroll
     for (uint32_t h = 0; h < kCountPer; ++h) {
@@ -658,7 +685,8 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
             if (cls[h] == 1) {
                 TriVis tv;
                 float rw[3];
-                packed[h] = setup_triangle(d, p.views[d.view], q[h][0], q[h][1], q[h][2], tv, rw) ? 1u : 0u;
+                if (setup_triangle(d, p.views[d.view], q[h][0], q[h][1], q[h][2], tv, rw))
+                    packed[h] = (!p.cmark || has_corner(tv)) ? 0x10001u : 0x10000u;
             }
         }
     }
@@ -669,18 +697,20 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
             uint32_t k = atomicAdd(&p.ctr->n_clip_list, 1u);
             p.clip_list[k] = make_uint2(slot[h], di[h]);
         }
-        if (inb[h]) p.cnt[slot[h]] = packed[h];
+        if (inb[h]) p.cnt[slot[h]] = (packed[h] & 0xFFu) | ((packed[h] >> 16) << 8);
         uint32_t sum = wave_sum(packed[h]);
         if ((threadIdx.x & 63u) == 0u) s_w[h][threadIdx.x >> 6] = sum;
     }
     __syncthreads();
-    if (threadIdx.x < kCountPer && hb * kCountPer + threadIdx.x < p.n_tblocks)
+    if (threadIdx.x < kCountPer && hb * kCountPer + threadIdx.x < p.n_tblocks) {
+        const uint32_t sum = s_w[threadIdx.x][0] + s_w[threadIdx.x][1] + s_w[threadIdx.x][2] + s_w[threadIdx.x][3];
         p.blocksum[hb * kCountPer + threadIdx.x] =
-            (unsigned long long)(s_w[threadIdx.x][0] + s_w[threadIdx.x][1] + s_w[threadIdx.x][2] + s_w[threadIdx.x][3]);
+            (unsigned long long)(sum & 0xFFFFu) | ((unsigned long long)(sum >> 16) << 32);
+    }
 }
 
 // Triangles that need the clipper, one per 16-lane group: the polygon's fan triangles that
-// survive setup are counted (packed with the polygon's vertex count, like kb_setup_count's).
+// survive setup are counted, and those of them to bin (packed with the polygon's vertex count).
 __global__ __launch_bounds__(256) void kb_clip_count(BatchParams p) {
     __shared__ float s_poly[256 / kClipGroup][4 * kClipGroup];
     const ClipLanes g;
@@ -694,18 +724,20 @@ __global__ __launch_bounds__(256) void kb_clip_count(BatchParams p) {
         const int nv = clip_group<4>(p, d, e.x - d.tbase, g, x, s_poly[grp]);  // positions decide count and cull
         float4 q0, q1, q2;
         fan_positions<4>(g, x, q0, q1, q2);
-        bool live = false;
+        bool live = false, binned = false;
         if (g.li >= 1u && (int)g.li < nv - 1) {
             TriVis tv;
             float rw[3];
             live = setup_triangle(d, v, q0, q1, q2, tv, rw);
+            binned = live && (!p.cmark || has_corner(tv));
         }
-        const uint32_t ntri = (uint32_t)__popcll(__ballot(live) & g.gmask);
+        const uint32_t n_asm = (uint32_t)__popcll(__ballot(live) & g.gmask);
+        const uint32_t n_bin = (uint32_t)__popcll(__ballot(binned) & g.gmask);
         if (g.li == 0u) {
-            const uint32_t packed = ntri ? (ntri | ((uint32_t)nv << 16)) : 0u;
+            const uint32_t packed = n_asm ? (n_bin | (n_asm << 8) | ((uint32_t)nv << 16)) : 0u;
             p.cnt[e.x] = packed;
             if (packed)
-                atomicAdd(&p.blocksum[e.x >> 8], (unsigned long long)ntri | ((unsigned long long)nv << 32));
+                atomicAdd(&p.blocksum[e.x >> 8], (unsigned long long)n_bin | ((unsigned long long)n_asm << 32));
         }
     }
 }
@@ -713,7 +745,7 @@ __global__ __launch_bounds__(256) void kb_clip_count(BatchParams p) {
 
 __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t idx, uint32_t draw, const ViewRec &v,
                                                TriVis &tv, const float rw[3], uint32_t r0, uint32_t r1,
-                                               uint32_t r2, uint32_t prim, uint32_t sub) {
+                                               uint32_t r2, uint32_t prim, uint32_t num) {
     if (idx >= p.tri_cap) return;
     // every load before the first store, which the compiler cannot move loads across
     const uint32_t tile_base = v.tile_base, tiles_x = (uint32_t)v.tiles_x;
@@ -723,21 +755,54 @@ __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t id
     sh.rw[0] = rw[0]; sh.rw[1] = rw[1]; sh.rw[2] = rw[2];
     sh.row[0] = r0; sh.row[1] = r1; sh.row[2] = r2;
     sh.prim = prim;
-    sh.sub = sub;
+    sh.num = num;
     p.tshade[idx] = sh;
     p.tbin[idx] = make_uint4(tv.rect0, tv.rect1, tile_base, tiles_x);
 }
 
+// The bins of an assembled triangle that is not binned: counted for the stats and marked in
+// the (draw, bin) bitmap. Every lane of the wave must be active.
+__device__ __forceinline__ uint32_t mark_unbinned(const BatchParams &p, const DrawRec &d, const ViewRec &v,
+                                                  bool unbinned, const TriVis &tv) {
+    int32_t bx0 = 0, by0 = 0, bx1 = -1, by1 = -1;
+    if (unbinned) rect_of(tv, bx0, by0, bx1, by1);
+    const uint32_t rw = bx1 >= bx0 ? (uint32_t)(bx1 - bx0 + 1) : 0u;
+    const uint32_t nb = by1 >= by0 ? rw * (uint32_t)(by1 - by0 + 1) : 0u;
+    const uint32_t kmax = wave_max(nb);
+    for (uint32_t k = 0; k < kmax; ++k) {
+        const uint32_t bx = (uint32_t)bx0 + (rw ? k % rw : 0u), by = (uint32_t)by0 + (rw ? k / rw : 0u);
+        wave_mark(p.cmark, k < nb, d.cbase + by * (uint32_t)v.bins_x + bx);
+    }
+    return nb;
+}
+
+// Adds a block's count of unbinned (triangle, 8x8 bin) pairs to the stats (spread over 64
+// counters: one shared counter hit by every block would serialise in L2).
+__device__ __forceinline__ void add_unbinned_pairs(const BatchParams &p, uint32_t nb) {
+    __shared__ uint32_t s_p8[4];
+    const uint32_t s = wave_sum(nb);
+    if ((threadIdx.x & 63u) == 0u) s_p8[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0u) {
+        const uint32_t t = s_p8[0] + s_p8[1] + s_p8[2] + s_p8[3];
+        if (t) atomicAdd(&p.stats->pairs8_part[blockIdx.x & 63u], (unsigned long long)t);
+    }
+}
+
 __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
-    __shared__ uint32_t s_w[5];
+    __shared__ unsigned long long s_w[5];
     const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t slot = b * 256u + threadIdx.x;
     const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
     const DrawRec &d = p.draws[di];
     const ViewRec v = p.views[d.view];  // a copy: read here, before the first store
     const uint32_t t = slot - d.tbase;
-    const unsigned long long bo = p.blockoff[b];
+    const unsigned long long bo = p.blockoff[b];                       // binned | assembled << 32
+    const uint32_t draw_asm0 = (uint32_t)(p.blockoff[d.blk_t0] >> 32);  // the draw's first assembled
     const uint32_t packed = p.cnt[slot];
+    const uint32_t n_bin = packed & 0xFFu, n_asm = (packed >> 8) & 0xFFu;
+    const bool plain = n_asm == 1u && (packed >> 16) == 0u;  // assembled without the clipper
+    const uint32_t n_rows = (!plain && n_bin) ? packed >> 16 : 0u;  // a clipped polygon's vertex rows
     // the triangle is set up before its place is known, so its loads and arithmetic overlap the
     // block scan; every load comes before the first store (see kb_vertex)
     TriVis tv;
@@ -745,55 +810,62 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     uint32_t s0 = 0u, s1 = 0u, s2 = 0u;
     uint32_t vi[3] = {0u, 0u, 0u};
     if (t < d.tri_count) load_tri_indices(d, t, vi);  // in flight with the count's load
-    if (packed == 1u) {
+    if (plain) {
         s0 = d.vbase + vi[0];
         s1 = d.vbase + vi[1];
         s2 = d.vbase + vi[2];
         setup_triangle(d, v, p.vpos[s0], p.vpos[s1], p.vpos[s2], tv, rw);
     }
     if (threadIdx.x == 0u) {
-        // a draw's first assembled index; empty draws start where the next one does
+        // a draw's first binned index; empty draws start where the next one does
         if (b == d.blk_t0)
             for (int32_t k = (int32_t)di; k >= 0 && p.draws[k].blk_t0 == b; --k) p.asm_base[k] = (uint32_t)bo;
         if (b == gridDim.x - 1u) {
-            // the flush's totals and capacity checks, and the empty draws after the last triangle
-            const unsigned long long tot = p.ctr->n_assembled_clip;
-            const uint32_t n_asm = (uint32_t)(tot & 0xFFFFFFFFull), n_cv = (uint32_t)(tot >> 32);
-            for (uint32_t k = di + 1u; k < p.n_draws; ++k) p.asm_base[k] = n_asm;
-            p.ctr->n_assembled = n_asm;
-            p.ctr->n_clipverts = n_cv;
-            uint32_t ovf = 0u;
-            if (n_asm > p.tri_cap) ovf |= OVF_TRIANGLES;
-            if ((unsigned long long)p.vslots + n_cv > (unsigned long long)p.vattr_rows) ovf |= OVF_CLIPVERTS;
-            if (ovf) {
-                atomicOr(&p.ctr->overflow, ovf);
+            // the flush's totals and capacity check, and the empty draws after the last triangle
+            const unsigned long long tot = p.ctr->scan_total;
+            const uint32_t n_binned = (uint32_t)(tot & 0xFFFFFFFFull), n_assembled = (uint32_t)(tot >> 32);
+            for (uint32_t k = di + 1u; k < p.n_draws; ++k) p.asm_base[k] = n_binned;
+            p.ctr->n_assembled = n_binned;
+            if (n_binned > p.tri_cap) {
+                atomicOr(&p.ctr->overflow, OVF_TRIANGLES);
                 atomicAdd(&p.stats->capacity_errors, 1ull);
             }
-            atomicAdd(&p.stats->assembled, (unsigned long long)n_asm);
+            atomicAdd(&p.stats->assembled, (unsigned long long)n_assembled);
             atomicMax(&p.stats->max_clip_list, (unsigned long long)p.ctr->n_clip_list);
         }
     }
-    uint32_t total;
-    // within one block the packed counts (<= 14 triangles, <= 16 vertices each) cannot carry
-    const uint32_t ex = block_exclusive_scan<256, uint32_t>(packed, s_w, total);
-    if (packed == 0u) return;
-    const uint32_t off_tri = (uint32_t)(bo & 0xFFFFFFFFull) + (ex & 0xFFFFu);
-    const uint32_t off_vtx = (uint32_t)(bo >> 32) + (ex >> 16);
-    if (packed == 1u) write_triangle(p, off_tri, di, v, tv, rw, s0, s1, s2, t, 0u);
-    else p.clip_off[slot] = make_uint2(off_tri, off_vtx);
+    unsigned long long total;
+    // within one block the counts (<= 14 triangles, <= 16 rows each) cannot carry out of 16 bits
+    const unsigned long long ex = block_exclusive_scan<256, unsigned long long>(
+        (unsigned long long)(n_bin | (n_asm << 16)) | ((unsigned long long)n_rows << 32), s_w, total);
+    if (total >> 32) {  // the block's clipped polygons take their vertex rows with one atomic
+        __shared__ uint32_t s_rows;
+        if (threadIdx.x == 0u) s_rows = atomicAdd(&p.ctr->n_clipverts, (uint32_t)(total >> 32));
+        __syncthreads();
+        if (n_rows) p.clip_rows[slot] = s_rows + (uint32_t)(ex >> 32);
+    }
+    if (p.cmark) add_unbinned_pairs(p, mark_unbinned(p, d, v, plain && n_bin == 0u, tv));
+    if (n_asm == 0u) return;
+    const uint32_t off_bin = (uint32_t)(bo & 0xFFFFFFFFull) + (uint32_t)(ex & 0xFFFFu);
+    const uint32_t num = (uint32_t)(bo >> 32) + (uint32_t)((ex >> 16) & 0xFFFFu) - draw_asm0;
+    if (!plain) p.clip_off[slot] = make_uint2(off_bin, num);
+    else if (n_bin) write_triangle(p, off_bin, di, v, tv, rw, s0, s1, s2, t, num);
 }
 
-// The clipped triangles' output, one per 16-lane group: lane k writes polygon vertex k's
-// attribute row, and lane ci the fan triangle (0, ci, ci + 1) if it survives setup, at the
-// offsets kb_setup_emit scanned.
+// The clipped triangles' output, one per 16-lane group: lane k writes polygon vertex k's attribute
+// row (when any fan triangle is binned: the rows kb_setup_emit gave it), and lane ci the fan
+// triangle (0, ci, ci + 1) if it survives setup, at the offsets kb_setup_emit scanned.
+// Unbinned fan triangles go to the stats and the bitmap, one atomic per bin (the clipper is rare).
 __global__ __launch_bounds__(256) void kb_clip_emit(BatchParams p) {
     __shared__ float s_poly[256 / kClipGroup][12 * kClipGroup];
     const ClipLanes g;
     const uint32_t grp = threadIdx.x / kClipGroup;
     const uint32_t n = p.ctr->n_clip_list;
+    uint32_t p8 = 0u;
     for (uint32_t s = blockIdx.x * (256u / kClipGroup) + grp; s < n; s += gridDim.x * (256u / kClipGroup)) {
         const uint2 e = p.clip_list[s];  // (slot, draw)
-        if (p.cnt[e.x] == 0u) continue;
+        const uint32_t c = p.cnt[e.x];
+        if (((c >> 8) & 0xFFu) == 0u) continue;  // nothing assembled
         const uint32_t di = e.y;
         const DrawRec &d = p.draws[di];
         const ViewRec &v = p.views[d.view];
@@ -801,9 +873,19 @@ __global__ __launch_bounds__(256) void kb_clip_emit(BatchParams p) {
         const uint32_t t = e.x - d.tbase;
         float x[12];
         const int nv = clip_group<12>(p, d, t, g, x, s_poly[grp]);
-        const uint32_t row0 = p.vslots + off.y;
+        uint32_t row0 = 0u;
+        bool rows = false;
+        if (c & 0xFFu) {
+            const uint32_t r = p.clip_rows[e.x];
+            rows = (unsigned long long)p.vslots + r + (uint32_t)nv <= (unsigned long long)p.vattr_rows;
+            if (!rows && g.li == 0u) {
+                const uint32_t old = atomicOr(&p.ctr->overflow, OVF_CLIPVERTS);
+                if (!(old & (OVF_TRIANGLES | OVF_CLIPVERTS))) atomicAdd(&p.stats->capacity_errors, 1ull);
+            }
+            row0 = p.vslots + r;
+        }
         const uint32_t row = row0 + g.li;
-        if ((int)g.li < nv && row < p.vattr_rows) {
+        if (rows && (int)g.li < nv) {
             p.vattr[2u * row] = make_float4(x[4], x[5], x[6], x[7]);
             p.vattr[2u * row + 1u] = make_float4(x[8], x[9], x[10], x[11]);
         }
@@ -813,16 +895,30 @@ __global__ __launch_bounds__(256) void kb_clip_emit(BatchParams p) {
         TriVis tv;
         float rw[3];
         if (g.li >= 1u && (int)g.li < nv - 1) live = setup_triangle(d, v, q0, q1, q2, tv, rw);
-        const uint32_t j = (uint32_t)__popcll(__ballot(live) & g.gmask & g.below());
-        if (live) write_triangle(p, off.x + j, di, v, tv, rw, row0, row, row + 1u, t, g.li - 1u);
+        const bool binned = live && (!p.cmark || has_corner(tv));
+        const uint32_t j_asm = (uint32_t)__popcll(__ballot(live) & g.gmask & g.below());
+        const uint32_t j_bin = (uint32_t)__popcll(__ballot(binned) & g.gmask & g.below());
+        if (binned && rows) write_triangle(p, off.x + j_bin, di, v, tv, rw, row0, row, row + 1u, t, off.y + j_asm);
+        if (live && !binned) {
+            int32_t bx0, by0, bx1, by1;
+            rect_of(tv, bx0, by0, bx1, by1);
+            for (int32_t by = by0; by <= by1; ++by)
+                for (int32_t bx = bx0; bx <= bx1; ++bx) {
+                    const uint32_t bit = d.cbase + (uint32_t)by * (uint32_t)v.bins_x + (uint32_t)bx;
+                    atomicOr(p.cmark + (bit >> 5), 1u << (bit & 31u));
+                    ++p8;
+                }
+        }
     }
+    if (p.cmark) add_unbinned_pairs(p, p8);
 }
 
 // ------------------------------------------------------------ stage 2: binning
 
-// Tile rect (16x16 tiles) of a TriVis 8x8-bin rect; returns the tile count (0 if empty).
-__device__ __forceinline__ uint32_t tile_rect(uint32_t rect0, uint32_t rect1, int32_t &tx0, int32_t &ty0, int32_t &tw,
-                                              uint32_t &pairs8) {
+// Tile rect (tiles of 2^shift x 2^shift bins) of a TriVis 8x8-bin rect; returns the tile count
+// (0 if empty).
+__device__ __forceinline__ uint32_t tile_rect(uint32_t rect0, uint32_t rect1, uint32_t shift, int32_t &tx0, int32_t &ty0,
+                                              int32_t &tw, uint32_t &pairs8) {
     int32_t bx0 = (int32_t)(rect0 & 0xFFFFu), by0 = (int32_t)(rect0 >> 16);
     int32_t bx1 = (int32_t)(int16_t)(rect1 & 0xFFFFu), by1 = (int32_t)(int16_t)(rect1 >> 16);
     if (bx1 < bx0 || by1 < by0) {
@@ -832,10 +928,10 @@ __device__ __forceinline__ uint32_t tile_rect(uint32_t rect0, uint32_t rect1, in
         return 0u;
     }
     pairs8 = (uint32_t)(bx1 - bx0 + 1) * (uint32_t)(by1 - by0 + 1);
-    tx0 = bx0 >> 1;
-    ty0 = by0 >> 1;
-    tw = (bx1 >> 1) - tx0 + 1;
-    return (uint32_t)tw * (uint32_t)((by1 >> 1) - ty0 + 1);
+    tx0 = bx0 >> shift;
+    ty0 = by0 >> shift;
+    tw = (bx1 >> shift) - tx0 + 1;
+    return (uint32_t)tw * (uint32_t)((by1 >> shift) - ty0 + 1);
 }
 
 constexpr uint32_t kRadixRounds = 16;
@@ -891,7 +987,7 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
             const uint32_t off = EMIT ? p.pair_off[i] : 0u;  // in flight with the rect
             int32_t tx0, ty0, tw;
             uint32_t p8;
-            const uint32_t nt = tile_rect(tb.x, tb.y, tx0, ty0, tw, p8);
+            const uint32_t nt = tile_rect(tb.x, tb.y, p.tile_shift, tx0, ty0, tw, p8);
             if (!EMIT) {
                 p.pair_cnt[i] = nt;
                 pairs8_acc += p8;
@@ -916,7 +1012,7 @@ __global__ __launch_bounds__(256) void kb_pairs(BatchParams p) {
             const uint4 tb = p.tbin[ti];
             int32_t tx0, ty0, tw;
             uint32_t p8;
-            const uint32_t nt = tile_rect(tb.x, tb.y, tx0, ty0, tw, p8);
+            const uint32_t nt = tile_rect(tb.x, tb.y, p.tile_shift, tx0, ty0, tw, p8);
             const uint32_t off = p.pair_off[ti];
             for (uint32_t k = threadIdx.x; k < nt; k += 256u) {
                 const uint32_t tile = tb.z + (uint32_t)(ty0 + (int32_t)(k / (uint32_t)tw)) * tb.w +
@@ -1189,6 +1285,8 @@ struct BinState {
     int32_t depth_cull; // the current draw allows the conservative cull (uniform)
     uint64_t stale;     // lanes whose depth changed since wext was last recomputed (uniform)
     uint64_t inside_m;  // lanes whose pixel is inside the view (uniform)
+    uint32_t bin;       // the bin's index in its view (uniform)
+    uint32_t seen;      // lane j: the unbinned-triangle bit of the j-th draw met (bitmap on, j < 64)
 };
 
 // Up to N staged records of a tile's list in LDS, one array per field: the triangle-parallel
@@ -1228,12 +1326,29 @@ __device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t
 
 // Entering a new draw: the bin's hierarchical-Z value becomes the tile minimum of its depths
 // after the previous draw (write_tile, main.c:589-603).
+// The (bin, draw) pair is counted here as active, so its bit in the bitmap of unbinned
+// triangles is to be cleared (that bitmap's remaining bits are the pairs met only by unbinned
+// ones): lane j keeps the j-th draw's bit and clear_seen clears them after the walk, off its path.
 template <bool GE>
 __device__ __forceinline__ void bin_draw_change(const BatchParams &p, int32_t draw, BinState &st, bool inside) {
     if (st.cur_draw >= 0) st.hz = uniform_f(tile_hiz<GE>(inside, st.d));
     st.cur_draw = draw;
+    const DrawRec &D = p.draws[draw];
+    st.depth_cull = (int32_t)uniform_u((uint32_t)D.depth_cull);
+    if (p.cmark) {
+        const uint32_t bit = D.cbase + st.bin, lane = threadIdx.x & 63u;
+        if (st.n_draws < 64u) {
+            if (lane == st.n_draws) st.seen = bit;
+        } else if (lane == 0u) {
+            atomicAnd(p.cmark + (bit >> 5), ~(1u << (bit & 31u)));
+        }
+    }
     st.n_draws++;
-    st.depth_cull = (int32_t)uniform_u((uint32_t)p.draws[draw].depth_cull);
+}
+
+__device__ __forceinline__ void clear_seen(const BatchParams &p, const BinState &st) {
+    if (p.cmark && (threadIdx.x & 63u) < min(st.n_draws, 64u))
+        atomicAnd(p.cmark + (st.seen >> 5), ~(1u << (st.seen & 31u)));  // marks precede this kernel
 }
 
 // What the rasteriser reads of one staged record: the edge functions, the depth plane and the
@@ -1431,7 +1546,7 @@ __device__ __forceinline__ void shade_winner(const BatchParams &p, const ViewRec
     V.depth[pix] = depth;
     if (p.tri_mark) atomicOr(p.tri_mark + (best >> 5), 1u << (best & 31u));  // footprint counting
     if (V.vis_tri) {
-        V.vis_tri[pix] = best - p.asm_base[T.draw];  // numbered within the draw
+        V.vis_tri[pix] = S.num;  // the reference's number within the draw
         V.vis_draw[pix] = D.draw_tag;
     }
     if (V.vis_prim) V.vis_prim[pix] = S.prim;
@@ -1445,9 +1560,10 @@ __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *s
     return d1 - d0 < kHeavyDraws;
 }
 
-template <bool GE, uint32_t N>
+// TB x TB bins per tile (TB = 2: 16x16 tiles, TB = 4: 32x32), one wave per bin.
+template <bool GE, uint32_t N, uint32_t TB>
 __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t vi,
-                                          TileLdsN<N> &s_tri, uint8_t *s_mask, uint8_t (*s_list)[N],
+                                          TileLdsN<N> &s_tri, uint16_t *s_mask, uint8_t (*s_list)[N],
                                           uint32_t *s_active) {
     // the wave index is uniform per wave; saying so keeps everything derived from it (the bin,
     // its Hi-Z value, the walk's branches) in scalar registers
@@ -1455,7 +1571,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
     const ViewRec &V = p.views[vi];
     const uint32_t tile = g - V.tile_base;
     const int32_t tx = (int32_t)(tile % (uint32_t)V.tiles_x), ty = (int32_t)(tile / (uint32_t)V.tiles_x);
-    const int32_t sbx = 2 * tx + (int32_t)(w & 1u), sby = 2 * ty + (int32_t)(w >> 1);
+    const int32_t sbx = (int32_t)TB * tx + (int32_t)(w % TB), sby = (int32_t)TB * ty + (int32_t)(w / TB);
     const int32_t px = sbx * kBin + (int32_t)(lane & 7u), py = sby * kBin + (int32_t)(lane >> 3);
     const bool sub_valid = sbx < V.bins_x && sby < V.bins_y;
     const bool inside = px < V.W && py < V.H;
@@ -1481,6 +1597,8 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
     st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
     st.stale = 0ull;
     st.inside_m = __ballot(inside);
+    st.bin = (uint32_t)(sby * V.bins_x + sbx);
+    st.seen = 0u;
 
     for (uint32_t base = 0; base < lcount; base += N) {
         const uint32_t n = min(N, lcount - base);
@@ -1493,12 +1611,12 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
             uint32_t m = 0u;
             if (!ovf || p.draws[tv.draw].view == vi) {
 #pragma unroll
-                for (int k = 0; k < 4; ++k) {
-                    int32_t gx = 2 * tx + (k & 1), gy = 2 * ty + (k >> 1);
+                for (int k = 0; k < (int)(TB * TB); ++k) {
+                    int32_t gx = (int32_t)TB * tx + (k % (int)TB), gy = (int32_t)TB * ty + (k / (int)TB);
                     if (gx >= bx0 && gx <= bx1 && gy >= by0 && gy <= by1) m |= 1u << k;
                 }
             }
-            s_mask[tid] = m;
+            s_mask[tid] = (uint16_t)m;
         }
         __syncthreads();
         uint32_t cnt = 0u;
@@ -1528,6 +1646,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
             s_active[w] = st.n_draws;
         }
     }
+    clear_seen(p, st);  // last: no later load of the wave waits for these atomics
 }
 
 // A heavy tile: a sorted list longer than heavy_min whose triangles all belong to one draw
@@ -1575,6 +1694,8 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
     st.wext = uniform_f(wave_depth_extreme<GE>(inside, st.d));
     st.stale = 0ull;
     st.inside_m = __ballot(inside);
+    st.bin = (uint32_t)(sby * V.bins_x + sbx);
+    st.seen = 0u;
     bool seen = false;
 
     for (uint32_t base = 0; base < lcount; base += 256u) {
@@ -1618,6 +1739,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
     s_md[w][lane] = st.d;
     s_mb[w][lane] = st.best;
     __syncthreads();
+    clear_seen(p, st);
     if (q != 0u || !sub_valid) return;
     for (uint32_t k = 1; k < kWavesPerBin; ++k) {
         const float dq = s_md[w + k][lane];
@@ -1733,31 +1855,34 @@ __global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
     }
 }
 
-// One block per 16x16 tile of every view, one wave per 8x8 bin. The block stages up to 256
-// records of the tile's sorted list in LDS; each wave compacts the ones touching its bin and
-// walks them in submission order with its pixels' depths in registers; afterwards each pixel's
-// winner is shaded once and colour, depth, visibility and the bin's Hi-Z are written.
-template <uint32_t N>
-__global__ __launch_bounds__(256) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
+// One block per tile of every view (TB x TB bins of 8x8: 16x16 or 32x32 pixels), one wave per
+// bin. The block stages up to N records of the tile's sorted list in LDS; each wave compacts the
+// ones touching its bin and walks them in submission order with its pixels' depths in registers;
+// afterwards each pixel's winner is shaded once and colour, depth, visibility and the bin's Hi-Z
+// are written.
+template <uint32_t N, uint32_t TB>
+__global__ __launch_bounds__(64 * TB * TB) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
     __shared__ TileLdsN<N> s_tri;
-    __shared__ uint8_t s_mask[N];
-    __shared__ __attribute__((aligned(8))) uint8_t s_list[4][N];
-    __shared__ uint32_t s_active[4];
+    __shared__ uint16_t s_mask[N];
+    __shared__ __attribute__((aligned(8))) uint8_t s_list[TB * TB][N];
+    __shared__ uint32_t s_active[TB * TB];
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
     // Tiles in kb_tile_order's order; the dispatcher deals consecutive blocks to the 8 XCDs in
     // turn, so the long tiles at the front spread over all XCDs.
     const uint32_t g = p.tile_order[blockIdx.x];
     const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
-    if (threadIdx.x < 4u) s_active[threadIdx.x] = 0u;
+    if (threadIdx.x < TB * TB) s_active[threadIdx.x] = 0u;
     if (p.views[vi].depth_func == DEPTH_GE)
-        tile_body<true, N>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
+        tile_body<true, N, TB>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
     else
-        tile_body<false, N>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
+        tile_body<false, N, TB>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
     __syncthreads();
     // (bin, draw) activity of this tile for the stats; this tile's start is no longer needed
     if (threadIdx.x == 0u) {
-        const uint32_t a = s_active[0] + s_active[1] + s_active[2] + s_active[3];
+        uint32_t a = 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < TB * TB; ++k) a += s_active[k];
         if (a) atomicAdd(&p.stats->active8_part[blockIdx.x & 63u], (unsigned long long)a);
     }
 }
@@ -1828,6 +1953,27 @@ __global__ __launch_bounds__(256) void k_popcount(const uint32_t *words, size_t 
     }
 }
 
+// Adds the set bits of words[0, n) to *out and clears them.
+__global__ __launch_bounds__(256) void k_count_clear(uint32_t *words, size_t n, unsigned long long *out) {
+    unsigned long long c = 0;
+    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u) {
+        const uint32_t w = words[i];
+        if (w) {
+            c += (unsigned long long)__popc(w);
+            words[i] = 0u;
+        }
+    }
+    c = wave_sum(c);
+    if ((threadIdx.x & 63u) == 0u && c) atomicAdd(out, c);
+}
+
+hipError_t launch_count_clear(uint32_t *words, size_t n, unsigned long long *out, hipStream_t s) {
+    if (!n) return hipSuccess;
+    hipLaunchKernelGGL(k_count_clear, dim3((unsigned)std::min<size_t>((n + 255) / 256, 1024)), dim3(256), 0, s, words,
+                       n, out);
+    return hipGetLastError();
+}
+
 hipError_t launch_popcount(const uint32_t *words, size_t n, uint32_t mul, unsigned long long *out_bits,
                            unsigned long long *out_bytes, hipStream_t s) {
     if (!n) return hipSuccess;
@@ -1879,6 +2025,27 @@ hipError_t launch_fill_f32(float *dst, float value, size_t n, hipStream_t s) {
     return launch_fill_u32((uint32_t *)dst, __builtin_bit_cast(uint32_t, value), n, s);
 }
 
+// Static LDS bytes of the tile kernel variant (tile_stage records, 2^shift bins a side) and of the
+// setup kernels (the larger of count and emit), for the occupancy caps.
+uint32_t tile_kernel_lds(uint32_t tile_stage, uint32_t shift) {
+    hipFuncAttributes a{};
+    hipError_t e;
+    if (shift == 2u)
+        e = tile_stage == 128u ? hipFuncGetAttributes(&a, (const void *)kb_tile<128, 4>)
+                               : hipFuncGetAttributes(&a, (const void *)kb_tile<256, 4>);
+    else
+        e = tile_stage == 128u ? hipFuncGetAttributes(&a, (const void *)kb_tile<128, 2>)
+                               : hipFuncGetAttributes(&a, (const void *)kb_tile<256, 2>);
+    return e == hipSuccess ? (uint32_t)a.sharedSizeBytes : 0u;
+}
+
+uint32_t setup_kernel_lds() {
+    hipFuncAttributes a{}, b{};
+    if (hipFuncGetAttributes(&a, (const void *)kb_setup_count) != hipSuccess) return 0u;
+    if (hipFuncGetAttributes(&b, (const void *)kb_setup_emit) != hipSuccess) return 0u;
+    return (uint32_t)std::max(a.sharedSizeBytes, b.sharedSizeBytes);
+}
+
 hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, uint32_t radix_passes,
                         hipStream_t s, hipEvent_t *tile_ev) {
     hipError_t e;
@@ -1889,12 +2056,13 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         break;
     case 1:
         if (!p.n_tblocks) break;
-        hipLaunchKernelGGL(kb_setup_count, dim3((p.n_tblocks + kCountPer - 1u) / kCountPer), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(kb_setup_count, dim3((p.n_tblocks + kCountPer - 1u) / kCountPer), dim3(256), p.setup_lds_pad, s,
+                           p);
         hipLaunchKernelGGL(kb_clip_count, dim3(1024), dim3(256), 0, s, p);
         if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc,
-                                                 &p.ctr->n_assembled_clip, s)) != hipSuccess)
+                                                 &p.ctr->scan_total, s)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(kb_setup_emit, dim3(p.n_tblocks), dim3(256), 0, s, p);
+        hipLaunchKernelGGL(kb_setup_emit, dim3(p.n_tblocks), dim3(256), p.setup_lds_pad, s, p);
         hipLaunchKernelGGL(kb_clip_emit, dim3(1024), dim3(256), 0, s, p);
         break;
     case 2: {
@@ -1927,7 +2095,7 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
     case 3: {
         if (!p.n_tblocks || !p.total_tiles) break;
         const uint32_t *sorted = (radix_passes & 1u) ? p.vals1 : p.vals0;
-        if (p.heavy_min != 0xFFFFFFFFu) {  // the 16-wave path is on
+        if (p.heavy_min != 0xFFFFFFFFu && p.tile_shift == 1u) {  // the 16-wave path is on (16x16 tiles)
             hipLaunchKernelGGL(kb_heavy_tiles, dim3(std::min<uint32_t>((p.total_tiles + 255) / 256, 256u)), dim3(256),
                                0, s, p, sorted);
             hipLaunchKernelGGL(kb_tile_heavy, dim3(std::max(1u, std::min<uint32_t>(p.total_tiles, p.heavy_blocks))),
@@ -1937,11 +2105,20 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         hipLaunchKernelGGL(kb_tile_order<false>, dim3(ogrid), dim3(256), 0, s, p);
         hipLaunchKernelGGL(kb_tile_order<true>, dim3(ogrid), dim3(256), 0, s, p);
         if (tile_ev && (e = hipEventRecord(tile_ev[0], s)) != hipSuccess) return e;
-        if (p.tile_stage == 128u)
-            hipLaunchKernelGGL(kb_tile<128>, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
-        else
-            hipLaunchKernelGGL(kb_tile<256>, dim3(p.total_tiles), dim3(256), 0, s, p, sorted);
+        const uint32_t pad = p.tile_lds_pad;  // dynamic LDS that caps the blocks per CU (0: none)
+        if (p.tile_shift == 2u) {
+            if (p.tile_stage == 128u)
+                hipLaunchKernelGGL((kb_tile<128, 4>), dim3(p.total_tiles), dim3(1024), pad, s, p, sorted);
+            else
+                hipLaunchKernelGGL((kb_tile<256, 4>), dim3(p.total_tiles), dim3(1024), pad, s, p, sorted);
+        } else if (p.tile_stage == 128u) {
+            hipLaunchKernelGGL((kb_tile<128, 2>), dim3(p.total_tiles), dim3(256), pad, s, p, sorted);
+        } else {
+            hipLaunchKernelGGL((kb_tile<256, 2>), dim3(p.total_tiles), dim3(256), pad, s, p, sorted);
+        }
         if (tile_ev && (e = hipEventRecord(tile_ev[1], s)) != hipSuccess) return e;
+        // (bin, draw) pairs met only by unbinned triangles: active bins of the stats
+        if (p.cmark && (e = launch_count_clear(p.cmark, p.cmark_words, &p.stats->active8, s)) != hipSuccess) return e;
         break;
     }
     default:

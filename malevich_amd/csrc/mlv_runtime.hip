@@ -129,7 +129,8 @@ struct mlv_context {
     std::vector<PendingDraw> pending;
     uint64_t pending_tris = 0, pending_verts = 0;
     // arenas
-    Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, tvis, tshade, tbin;
+    Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, clip_rows, tvis,
+        tshade, tbin;
     Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist, heavy, tile_order;
     // differentiable shading scratch
     Arena d_shpart, d_tkey, d_tval, d_tkey2, d_tval2, d_tpay, d_vkey, d_vval, d_vkey2, d_vval2, d_vpay, d_part, d_hist;
@@ -142,6 +143,8 @@ struct mlv_context {
     bool counting = false;
     FootCounts *foot = nullptr;
     Arena tri_mark, diff_mark;
+    Arena cmark;  // (draw, bin) bitmap of assembled triangles that are not binned (zero between flushes)
+    bool skip_cornerless = true;  // assembled triangles whose box holds no pixel corner are not binned
     size_t diff_mark_words = 0;
     std::vector<mlv_texture2d *> marked;  // textures holding a footprint bitmap
     Arena cameras;      // camera table of the last mlv_camera_constant_buffers
@@ -183,6 +186,7 @@ struct mlv_context {
     // launch parameters (mlv_tuning)
     uint32_t heavy_min = kHeavyMinDefault, heavy_blocks = kHeavyBlocksDefault, pair_blocks = kPairBlocksDefault;
     uint32_t tile_stage = kTileStageDefault;
+    uint32_t tile_edge = kTile, tile_waves = 0, setup_waves = 0;  // 0 waves: no occupancy cap
     size_t last_pair_cap = 0;
 };
 
@@ -335,6 +339,16 @@ static int staging_commit(mlv_context *ctx, int slot) {
     return MLV_OK;
 }
 
+// Dynamic LDS that holds a kernel of `wpb` waves a block, `lds` static LDS bytes, to about
+// `waves` waves per CU: LDS per CU over the blocks that many waves make, less the static part
+// (within the per-block limit). 0 waves: no cap.
+static uint32_t lds_pad(uint32_t lds, uint32_t wpb, uint32_t waves) {
+    if (!waves) return 0u;
+    const uint32_t blocks = std::max(1u, waves / wpb);
+    const uint32_t per = std::min(kLdsPerCu / blocks, kLdsPerBlockMax);
+    return per > lds ? (per - lds) & ~255u : 0u;
+}
+
 static int flush(mlv_context *ctx) {
     if (ctx->recording)
         return fail(ctx, MLV_ERROR_INVALID_STATE, "only state, clears and draws may be issued while recording a graph");
@@ -362,8 +376,8 @@ static int flush(mlv_context *ctx) {
             v.vis_prim = pd.key.vis_prim;
             v.W = (int32_t)pd.key.rt->width;
             v.H = (int32_t)pd.key.rt->height;
-            v.tiles_x = (v.W + kTile - 1) / kTile;
-            v.tiles_y = (v.H + kTile - 1) / kTile;
+            v.tiles_x = (v.W + (int32_t)ctx->tile_edge - 1) / (int32_t)ctx->tile_edge;
+            v.tiles_y = (v.H + (int32_t)ctx->tile_edge - 1) / (int32_t)ctx->tile_edge;
             v.bins_x = (int32_t)pd.key.ds->bins_x;
             v.bins_y = (int32_t)pd.key.ds->bins_y;
             v.tile_base = (uint32_t)total_tiles;
@@ -383,6 +397,14 @@ static int flush(mlv_context *ctx) {
     const uint32_t n_draws = (uint32_t)draws.size();
     const uint32_t n_views = (uint32_t)views.size();
     const uint64_t T = tslots, Vs = vslots;
+    // one bit per (draw, 8x8 bin of its view) for the triangles that are assembled but not binned
+    uint64_t cbits = 0;
+    for (auto &pd : draws) {
+        pd.rec.cbase = (uint32_t)cbits;
+        const ViewRec &v = views[pd.rec.view];
+        cbits += (uint64_t)v.bins_x * (uint64_t)v.bins_y;
+    }
+    const bool cull = ctx->skip_cornerless && cbits < (1ull << 31);  // else every assembled triangle is binned
     // Clipped triangles are unknown until setup runs: start from 1/32 of the batch (triangles
     // straddling the screen edges of a dense scene) and grow from the measured maximum.
     const size_t clip_tris = std::max<size_t>({ctx->clip_hint, (size_t)4096, (size_t)(T / 32)});
@@ -412,6 +434,7 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->asm_base, (size_t)n_draws * 4));
     RC_OR_RETURN(ensure(ctx, ctx->clip_list, std::max<size_t>(T, 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->clip_off, std::max<size_t>(T, 1) * 8));
+    RC_OR_RETURN(ensure(ctx, ctx->clip_rows, std::max<size_t>(T, 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->tvis, (size_t)tri_cap * sizeof(TriVis)));
     RC_OR_RETURN(ensure(ctx, ctx->tshade, (size_t)tri_cap * sizeof(TriShade)));
     RC_OR_RETURN(ensure(ctx, ctx->tbin, (size_t)tri_cap * 16));
@@ -427,6 +450,13 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->tile_order, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_scratch_bytes(scan_n)));  // cleared by the first stage
     RC_OR_RETURN(ensure(ctx, ctx->radix_hist, radix_hist_elems(pair_cap) * 4));
+    const size_t cwords = (size_t)((cbits + 31) / 32);
+    if (cull) {
+        const void *old = ctx->cmark.p;
+        RC_OR_RETURN(ensure(ctx, ctx->cmark, std::max<size_t>(cwords, 1) * 4));
+        if (ctx->cmark.p != old)  // new memory starts at zero; each flush clears the words it used
+            HIP_OR_FAIL(ctx, launch_fill_u32((uint32_t *)ctx->cmark.p, 0u, ctx->cmark.cap / 4, ctx->stream));
+    }
 
     // tables through pinned staging
     const size_t bytes_d = sizeof(DrawRec) * n_draws, bytes_v = sizeof(ViewRec) * n_views;
@@ -468,8 +498,11 @@ static int flush(mlv_context *ctx) {
     p.blocksum = (unsigned long long *)ctx->blocksum.p;
     p.blockoff = (unsigned long long *)ctx->blockoff.p;
     p.asm_base = (uint32_t *)ctx->asm_base.p;
+    p.cmark = cull ? (uint32_t *)ctx->cmark.p : nullptr;
+    p.cmark_words = cull ? (uint32_t)cwords : 0u;
     p.clip_list = (uint2 *)ctx->clip_list.p;
     p.clip_off = (uint2 *)ctx->clip_off.p;
+    p.clip_rows = (uint32_t *)ctx->clip_rows.p;
     p.tvis = (TriVis *)ctx->tvis.p;
     p.tshade = (TriShade *)ctx->tshade.p;
     p.tbin = (uint4 *)ctx->tbin.p;
@@ -485,7 +518,11 @@ static int flush(mlv_context *ctx) {
     p.tile_start = (uint32_t *)ctx->tile_start.p;
     p.heavy = (uint32_t *)ctx->heavy.p;
     p.tile_order = (uint32_t *)ctx->tile_order.p;
-    p.heavy_min = ctx->heavy_min;
+    p.tile_shift = ctx->tile_edge == 32u ? 2u : 1u;
+    p.heavy_min = p.tile_shift == 1u ? ctx->heavy_min : 0xFFFFFFFFu;  // the 16-wave path needs 16x16 tiles
+    p.tile_lds_pad = lds_pad(tile_kernel_lds(ctx->tile_stage, p.tile_shift), p.tile_shift == 2u ? 16u : 4u,
+                             ctx->tile_waves);
+    p.setup_lds_pad = lds_pad(setup_kernel_lds(), 4u, ctx->setup_waves);
     p.heavy_blocks = ctx->heavy_blocks;
     p.pair_blocks = ctx->pair_blocks;
     p.tile_stage = ctx->tile_stage;
@@ -658,14 +695,14 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
     (void)flush(ctx);
     (void)hipStreamSynchronize(ctx->stream);
     Arena *all[] = {&ctx->draws_dev, &ctx->views_dev, &ctx->vpos,     &ctx->vattr,     &ctx->cnt,
-                    &ctx->blocksum,  &ctx->blockoff,  &ctx->asm_base, &ctx->clip_list, &ctx->clip_off,
+                    &ctx->blocksum,  &ctx->blockoff,  &ctx->asm_base, &ctx->clip_list, &ctx->clip_off, &ctx->clip_rows,
                     &ctx->tvis,      &ctx->tshade,    &ctx->tbin,    &ctx->pair_cnt, &ctx->pair_off,  &ctx->keys0,
                     &ctx->vals0,     &ctx->keys1,     &ctx->vals1,    &ctx->tile_cnt,  &ctx->tile_start,
                     &ctx->scan_part, &ctx->radix_hist, &ctx->heavy, &ctx->tile_order, &ctx->d_shpart, &ctx->d_tkey, &ctx->d_tval,
                     &ctx->d_tkey2,   &ctx->d_tval2,   &ctx->d_tpay,   &ctx->d_vkey,    &ctx->d_vval,
                     &ctx->d_vkey2,   &ctx->d_vval2,   &ctx->d_vpay,   &ctx->d_part,    &ctx->d_hist,
                     &ctx->d_vposd,   &ctx->d_posacc,  &ctx->b_tables, &ctx->b_posacc, &ctx->b_shpart, &ctx->present_rgb, &ctx->cameras,
-                    &ctx->tri_mark, &ctx->diff_mark, &ctx->b_defer};
+                    &ctx->tri_mark, &ctx->diff_mark, &ctx->b_defer, &ctx->cmark};
     for (Arena *a : all) release(*a);
     free_retired(ctx);
     resolve_events(ctx);
@@ -1289,8 +1326,10 @@ static int keep_one_run_stats(mlv_context *ctx, const DevStats &base, const DevS
     s.pairs8 = base.pairs8 + (after.pairs8 - before.pairs8);
     s.oob = base.oob + (after.oob - before.oob);
     s.overflow_draws = base.overflow_draws + (after.overflow_draws - before.overflow_draws);
-    for (int i = 0; i < 64; ++i)
+    for (int i = 0; i < 64; ++i) {
         s.active8_part[i] = base.active8_part[i] + (after.active8_part[i] - before.active8_part[i]);
+        s.pairs8_part[i] = base.pairs8_part[i] + (after.pairs8_part[i] - before.pairs8_part[i]);
+    }
     HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->dstats, &s, sizeof s, hipMemcpyHostToDevice, ctx->stream));
     HIP_OR_FAIL(ctx, hipStreamSynchronize(ctx->stream));
     return MLV_OK;
@@ -1446,6 +1485,10 @@ MLV_API int mlv_get_tuning(mlv_context *ctx, mlv_tuning *out) {
     out->pair_blocks = ctx->pair_blocks;
     out->tile_stage = ctx->tile_stage;
     out->pair_capacity = (uint32_t)std::min<size_t>(ctx->last_pair_cap, 0xFFFFFFFFu);
+    out->tile_edge = ctx->tile_edge;
+    out->tile_waves_per_cu = ctx->tile_waves;
+    out->setup_waves_per_cu = ctx->setup_waves;
+    out->skip_cornerless = ctx->skip_cornerless ? 1u : 0u;
     return MLV_OK;
 }
 
@@ -1455,11 +1498,20 @@ MLV_API int mlv_set_tuning(mlv_context *ctx, const mlv_tuning *t) {
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "set_tuning: block counts must be 1..65536");
     if (t->tile_stage != 128u && t->tile_stage != 256u)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "set_tuning: tile_stage must be 128 or 256");
+    if (t->tile_edge != 16u && t->tile_edge != 32u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "set_tuning: tile_edge must be 16 or 32");
+    if (t->tile_waves_per_cu > 32u || t->setup_waves_per_cu > 32u || t->skip_cornerless > 1u)
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT,
+                    "set_tuning: waves per CU must be 0..32 and skip_cornerless 0 or 1");
     RC_OR_RETURN(flush(ctx));  // recorded draws keep the parameters they were recorded under
     ctx->heavy_min = t->heavy_tile_pairs;
     ctx->heavy_blocks = t->heavy_blocks;
     ctx->pair_blocks = t->pair_blocks;
     ctx->tile_stage = t->tile_stage;
+    ctx->tile_edge = t->tile_edge;
+    ctx->tile_waves = t->tile_waves_per_cu;
+    ctx->setup_waves = t->setup_waves_per_cu;
+    ctx->skip_cornerless = t->skip_cornerless != 0u;
     return MLV_OK;
 }
 
@@ -1516,20 +1568,30 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
         if (rc != MLV_ERROR_CAPACITY) break;
     }
     RC_OR_RETURN(rc);
+    static const uint32_t skip[] = {1u, 0u};
+    static const uint32_t tile_edge[] = {16u, 32u};
+    static const uint32_t tile_stage[] = {128u, 256u};
+    static const uint32_t tile_waves[] = {0u, 8u, 16u, 24u};
+    static const uint32_t setup_waves[] = {0u, 16u, 24u};
     static const uint32_t heavy_min[] = {192u, 384u, 768u, 1536u, 0xFFFFFFFFu};
     static const uint32_t heavy_blocks[] = {256u, 512u, 1024u, 2048u};
     static const uint32_t pair_blocks[] = {512u, 1024u, 2048u, 4096u};
-    static const uint32_t tile_stage[] = {128u, 256u};
     double best_ms = 0.0;
     RC_OR_RETURN(time_recorded(ctx, rec, repeats, &best_ms));
-    uint32_t *knob[4] = {&ctx->tile_stage, &ctx->heavy_min, &ctx->heavy_blocks, &ctx->pair_blocks};
-    const uint32_t *cand[4] = {tile_stage, heavy_min, heavy_blocks, pair_blocks};
-    const size_t ncand[4] = {2, 5, 4, 4};
-    for (int k = 0; k < 4; ++k) {
+    uint32_t skip_now = ctx->skip_cornerless ? 1u : 0u;
+    constexpr int kKnobs = 8;
+    uint32_t *knob[kKnobs] = {&skip_now,         &ctx->tile_edge, &ctx->tile_stage,   &ctx->tile_waves,
+                              &ctx->setup_waves, &ctx->heavy_min, &ctx->heavy_blocks, &ctx->pair_blocks};
+    const uint32_t *cand[kKnobs] = {skip, tile_edge, tile_stage, tile_waves, setup_waves, heavy_min, heavy_blocks,
+                                    pair_blocks};
+    const size_t ncand[kKnobs] = {2, 2, 2, 4, 3, 5, 4, 4};
+    for (int k = 0; k < kKnobs; ++k) {
+        if ((k == 5 || k == 6) && ctx->tile_edge != 16u) continue;  // the heavy-tile path needs 16x16 tiles
         uint32_t keep = *knob[k];
         for (size_t i = 0; i < ncand[k]; ++i) {
             if (cand[k][i] == keep) continue;
             *knob[k] = cand[k][i];
+            ctx->skip_cornerless = skip_now != 0u;
             double ms = 0.0;
             RC_OR_RETURN(time_recorded(ctx, rec, repeats, &ms));
             if (ms < best_ms * 0.99) {  // a candidate must win by more than the runs' spread
@@ -1538,6 +1600,7 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
             }
         }
         *knob[k] = keep;
+        ctx->skip_cornerless = skip_now != 0u;
     }
     RC_OR_RETURN(read_dstats(ctx, before));
     RC_OR_RETURN(run_recorded(ctx, rec));  // leave the frame in its targets under the chosen parameters
@@ -1661,6 +1724,7 @@ MLV_API int mlv_get_stats(mlv_context *ctx, mlv_stats *out) {
     out->active_bin_count = s.active8;
     for (unsigned long long a : s.active8_part) out->active_bin_count += a;
     out->total_triangle_count_in_bins = s.pairs8;
+    for (unsigned long long a : s.pairs8_part) out->total_triangle_count_in_bins += a;
     out->oob_triangle_count = s.oob;
     out->draw_count = ctx->draws;
     out->overflow_count = s.overflow_draws;

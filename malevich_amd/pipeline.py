@@ -376,8 +376,12 @@ class Device:
         _check(self.ctx, self.lib.mlv_get_tuning(self.ctx, C.byref(t)))
         return {k: getattr(t, k) for k, _ in L.Tuning._fields_}
 
-    def set_tuning(self, heavy_tile_pairs: int, heavy_blocks: int, pair_blocks: int, tile_stage: int = 256) -> None:
-        t = L.Tuning(heavy_tile_pairs, heavy_blocks, pair_blocks, tile_stage, 0)
+    def set_tuning(self, heavy_tile_pairs: int, heavy_blocks: int, pair_blocks: int, tile_stage: int = 256,
+                   tile_edge: int = 16, tile_waves_per_cu: int = 0, setup_waves_per_cu: int = 0,
+                   skip_cornerless: int = 1, pair_capacity: int = 0) -> None:
+        """Launch parameters (mlv_tuning); pair_capacity is reported by tuning() and ignored here."""
+        t = L.Tuning(heavy_tile_pairs, heavy_blocks, pair_blocks, tile_stage, 0, tile_edge, tile_waves_per_cu,
+                     setup_waves_per_cu, skip_cornerless)
         _check(self.ctx, self.lib.mlv_set_tuning(self.ctx, C.byref(t)))
 
     def autotune_begin(self) -> None:

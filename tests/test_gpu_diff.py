@@ -168,7 +168,7 @@ def test_raster_autotune_keeps_visibility(device, oracle, small_scene):
         np.testing.assert_array_equal(prim.cpu().numpy().view(np.uint32), prim_o)
     finally:
         r.release()
-        device.set_tuning(saved["heavy_tile_pairs"], saved["heavy_blocks"], saved["pair_blocks"], saved["tile_stage"])
+        device.set_tuning(**saved)
 
 
 @pytest.mark.parametrize("size", [(64, 64), (37, 20), (1, 9), (128, 32)])

@@ -243,11 +243,11 @@ def test_heavy_tiles_multi_draw(device, oracle, depth_func):
         draws.append(DrawSpec(st.indices, st.vertices, "passthrough", "passthrough"))
     clear_depth = 0.0 if depth_func == 0 else 1.0
     saved = device.tuning()
-    device.set_tuning(384, saved["heavy_blocks"], saved["pair_blocks"], saved["tile_stage"])  # past 384: 16 waves
+    device.set_tuning(**dict(saved, heavy_tile_pairs=384, tile_edge=16))  # past 384: 16 waves
     try:
         g, o = both(device, oracle, W, H, draws, depth=clear_depth, depth_func=depth_func)
     finally:
-        device.set_tuning(saved["heavy_tile_pairs"], saved["heavy_blocks"], saved["pair_blocks"], saved["tile_stage"])
+        device.set_tuning(**saved)
     assert_frames_match(g, o, color_tol=0)
     assert len(np.unique(g.vis_draw)) >= 6
 

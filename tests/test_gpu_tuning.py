@@ -1,9 +1,10 @@
 """Launch parameters change speed, never results.
 
-Bar: frames drawn under extreme tunings (every long tile on the 16-wave path, or none; one
-heavy-tile block; few pair blocks) are bit-identical to the oracle's; the autotuner returns
-parameters from its candidate set, leaves the recorded frame in the targets (bit-identical to
-the eager frame) and the context keeps the chosen parameters.
+Bar: frames and stats drawn under extreme tunings (every long tile on the 16-wave path, or none;
+one heavy-tile block; few pair blocks; 16x16 and 32x32 tiles; occupancy caps; triangles without a
+pixel corner binned or not) are bit-identical to the oracle's; the autotuner returns parameters
+from its candidate set, leaves the recorded frame in the targets (bit-identical to the eager
+frame) and the context keeps the chosen parameters.
 """
 from __future__ import annotations
 
@@ -16,7 +17,8 @@ from tests.render_utils import DrawSpec, assert_frames_match, render_gpu, render
 
 pytestmark = pytest.mark.gpu
 
-DEFAULT = dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048, tile_stage=256)
+DEFAULT = dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048, tile_stage=256, tile_edge=16,
+               tile_waves_per_cu=0, setup_waves_per_cu=0, skip_cornerless=1)
 
 
 def _draws(W, H):
@@ -29,7 +31,13 @@ def _draws(W, H):
 
 @pytest.mark.parametrize("tuning", [dict(heavy_tile_pairs=0, heavy_blocks=1, pair_blocks=3, tile_stage=256),
                                     dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=7, pair_blocks=1, tile_stage=128),
-                                    dict(heavy_tile_pairs=100, heavy_blocks=4096, pair_blocks=4096, tile_stage=128)])
+                                    dict(heavy_tile_pairs=100, heavy_blocks=4096, pair_blocks=4096, tile_stage=128),
+                                    dict(heavy_tile_pairs=0, heavy_blocks=64, pair_blocks=512, tile_stage=256,
+                                         skip_cornerless=0, tile_waves_per_cu=8, setup_waves_per_cu=16),
+                                    dict(heavy_tile_pairs=0, heavy_blocks=64, pair_blocks=5, tile_stage=128,
+                                         tile_edge=32, tile_waves_per_cu=16, setup_waves_per_cu=24),
+                                    dict(heavy_tile_pairs=0xFFFFFFFF, heavy_blocks=1024, pair_blocks=2048,
+                                         tile_stage=256, tile_edge=32, skip_cornerless=0)])
 def test_tunings_give_the_oracle_frame(device, oracle, tuning):
     W, H = 96, 64
     draws = _draws(W, H)
@@ -41,6 +49,29 @@ def test_tunings_give_the_oracle_frame(device, oracle, tuning):
         device.set_tuning(**DEFAULT)
     o = render_oracle(oracle, W, H, draws, S.CLEAR_COLOR)
     assert_frames_match(g, o, color_tol=0)
+
+
+@pytest.mark.parametrize("skip", [1, 0])
+@pytest.mark.parametrize("edge", [16, 32])
+def test_triangles_without_a_pixel_corner(device, oracle, skip, edge):
+    """A dense mesh in a small view: most of its triangles hold no pixel corner. Skipped or binned,
+    the frame and every stat (assembled triangles, active bins, triangle-bin pairs) equal the
+    oracle's."""
+    W, H = 80, 56
+    mesh, _ = S.sphere_of_about(60000, 0.1, 77, np.eye(4))
+    cb = S.camera_cb(eye=(0.0, 0.4, 2.6), target=(0.0, 0.0, 0.0), fov_y_deg=50.0, width=W, height=H)
+    tex = S.random_rgba8(64, seed=5)
+    draws = [DrawSpec(mesh.indices, mesh.vertices, "basic", "basic", cb=cb, ps_tex=tex)]
+    big = S.binning_stress(300, 2, W, H, seed=9, big_every=100)
+    draws.append(DrawSpec(big.indices, big.vertices, "passthrough", "passthrough"))
+    print(f"case: cornerless skip={skip} edge={edge}", flush=True)
+    try:
+        device.set_tuning(**dict(DEFAULT, skip_cornerless=skip, tile_edge=edge))
+        g = render_gpu(device, W, H, draws, S.CLEAR_COLOR)
+    finally:
+        device.set_tuning(**DEFAULT)
+    o = render_oracle(oracle, W, H, draws, S.CLEAR_COLOR)
+    assert_frames_match(g, o, color_tol=1)
 
 
 def test_autotune_keeps_results_and_choice(device, oracle):
@@ -92,6 +123,8 @@ def test_autotune_keeps_results_and_choice(device, oracle):
         assert best["heavy_blocks"] in (256, 512, 1024, 2048)
         assert best["pair_blocks"] in (512, 1024, 2048, 4096)
         assert best["tile_stage"] in (128, 256)
+        assert best["tile_edge"] in (16, 32) and best["skip_cornerless"] in (0, 1)
+        assert best["tile_waves_per_cu"] in (0, 8, 16, 24) and best["setup_waves_per_cu"] in (0, 16, 24)
         assert best["pair_capacity"] > 0
         np.testing.assert_array_equal(rt.read(), ref[0])
         np.testing.assert_array_equal(ds.read().view(np.uint32), ref[1].view(np.uint32))
