@@ -201,7 +201,7 @@ struct Eval {
     uint32_t lvl;
     int32_t x0, y0, x1, y1;    // clamped taps
     R fs, ft;
-    R t00[3], t10[3], t01[3], t11[3];
+    float t00[3], t10[3], t01[3], t11[3];  // the taps' texels (RGBA32F: exact in either precision)
     R alb[3];
     R nh[3], inv_len;
     R Y[9], E[3], lin[3], out[3];
@@ -249,7 +249,7 @@ __device__ __forceinline__ bool shade_tail(const DiffMesh &m, const float *sh, E
     e.t11[0] = q11.x; e.t11[1] = q11.y; e.t11[2] = q11.z;
 #pragma unroll
     for (int j = 0; j < 3; ++j)
-        e.alb[j] = lerp_r(lerp_r(e.t00[j], e.t10[j], e.fs), lerp_r(e.t01[j], e.t11[j], e.fs), e.ft);
+        e.alb[j] = lerp_r(lerp_r((R)e.t00[j], (R)e.t10[j], e.fs), lerp_r((R)e.t01[j], (R)e.t11[j], e.fs), e.ft);
     const R len = r_sqrt((e.n[0] * e.n[0] + e.n[1] * e.n[1]) + e.n[2] * e.n[2]);
     e.inv_len = (R)1 / len;
     e.nh[0] = e.n[0] * e.inv_len;
@@ -348,12 +348,15 @@ __global__ __launch_bounds__(256) void kd_forward(DiffParams p, float4 *out) {
 }
 
 // Per-pixel adjoint (FP64). Outputs: per-corner vertex gradients (dpos, dcol, duv, dn),
-// per-tap texel gradients (rgb) and the pixel's SH gradient.
+// per-tap texel gradients (rgb) and dE, from which the pixel's SH gradient is dE[j] * Y[k] for
+// coefficient k, channel j. The gradients that are summed in f32 (all but positions) are kept
+// rounded to f32 here, as they would be when summed.
 struct Adj {
-    double dpos[3][3], dcol[3][3], duv[3][2], dn[3][3];
-    double dtex[4][3];
+    double dpos[3][3];
+    float dcol[3][3], duv[3][2], dn[3][3];
+    float dtex[4][3];
     uint32_t tap[4];
-    double dsh[27];
+    double dE[3];
 };
 
 __device__ __forceinline__ void adjoint(const DiffMesh &m, const DiffView &v, const float *sh, const Eval<double> &e, float4 g,
@@ -374,13 +377,12 @@ __device__ __forceinline__ void adjoint(const DiffMesh &m, const DiffView &v, co
         dE[j] = dlin[j] * e.alb[j] * e.col[j];
     }
     double dY[9];
+    a.dE[0] = dE[0];
+    a.dE[1] = dE[1];
+    a.dE[2] = dE[2];
 #pragma unroll
-    for (int k = 0; k < 9; ++k) {
-        a.dsh[3 * k + 0] = dE[0] * e.Y[k];
-        a.dsh[3 * k + 1] = dE[1] * e.Y[k];
-        a.dsh[3 * k + 2] = dE[2] * e.Y[k];
+    for (int k = 0; k < 9; ++k)
         dY[k] = dE[0] * (double)sh[3 * k] + dE[1] * (double)sh[3 * k + 1] + dE[2] * (double)sh[3 * k + 2];
-    }
     const double x = e.nh[0], y = e.nh[1], z = e.nh[2];
     const double c1 = kY1, c2 = kY2, c3 = kY3, c4 = kY4;
     double dnh[3];
@@ -401,12 +403,13 @@ __device__ __forceinline__ void adjoint(const DiffMesh &m, const DiffView &v, co
     double ds = 0.0, dt = 0.0;
 #pragma unroll
     for (int j = 0; j < 3; ++j) {
-        a.dtex[0][j] = w00 * dalb[j];
-        a.dtex[1][j] = w10 * dalb[j];
-        a.dtex[2][j] = w01 * dalb[j];
-        a.dtex[3][j] = w11 * dalb[j];
-        const double dAs = (1.0 - e.ft) * (e.t10[j] - e.t00[j]) + e.ft * (e.t11[j] - e.t01[j]);
-        const double dAt = lerp_r(e.t01[j], e.t11[j], e.fs) - lerp_r(e.t00[j], e.t10[j], e.fs);
+        a.dtex[0][j] = (float)(w00 * dalb[j]);
+        a.dtex[1][j] = (float)(w10 * dalb[j]);
+        a.dtex[2][j] = (float)(w01 * dalb[j]);
+        a.dtex[3][j] = (float)(w11 * dalb[j]);
+        const double t00 = e.t00[j], t10 = e.t10[j], t01 = e.t01[j], t11 = e.t11[j];
+        const double dAs = (1.0 - e.ft) * (t10 - t00) + e.ft * (t11 - t01);
+        const double dAt = lerp_r(t01, t11, e.fs) - lerp_r(t00, t10, e.fs);
         ds += dalb[j] * dAs;
         dt += dalb[j] * dAt;
     }
@@ -420,12 +423,12 @@ __device__ __forceinline__ void adjoint(const DiffMesh &m, const DiffView &v, co
         double acc = 0.0;
 #pragma unroll
         for (int j = 0; j < 3; ++j) {
-            a.dcol[k][j] = L[k] * dci[j];
-            a.dn[k][j] = L[k] * dni[j];
+            a.dcol[k][j] = (float)(L[k] * dci[j]);
+            a.dn[k][j] = (float)(L[k] * dni[j]);
             acc += dci[j] * (double)m.col[3u * v + j] + dni[j] * (double)m.nrm[3u * v + j];
         }
-        a.duv[k][0] = L[k] * duvi[0];
-        a.duv[k][1] = L[k] * duvi[1];
+        a.duv[k][0] = (float)(L[k] * duvi[0]);
+        a.duv[k][1] = (float)(L[k] * duvi[1]);
         acc += duvi[0] * (double)m.uv[2u * v] + duvi[1] * (double)m.uv[2u * v + 1u];
         dlam[k] = acc;
     }
@@ -467,14 +470,18 @@ __device__ __forceinline__ double wave_sum_d(double v) {
     return v;
 }
 
-// Block sum of the 27 SH gradients into sh_part[coefficient][block] (fixed order: deterministic;
-// coefficient-major so that the final sum reads each coefficient's partials contiguously).
-__device__ __forceinline__ void sh_block_sum(const double dsh[27], double *sh_part) {
+// Block sum of the 27 SH gradients (dE[j] * Y[k] of the live pixels, Y the basis at the unit
+// normal nh, evaluated again here rather than kept through the adjoint) into
+// sh_part[coefficient][block] (fixed order: deterministic; coefficient-major so that the final
+// sum reads each coefficient's partials contiguously).
+__device__ __forceinline__ void sh_block_sum(bool live, const double dE[3], const double nh[3], double *sh_part) {
     __shared__ double s_w[4][27];
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    double Y[9];
+    sh_basis(nh[0], nh[1], nh[2], Y);
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
-        const double v = wave_sum_d(dsh[k]);
+        const double v = wave_sum_d(live ? dE[k % 3] * Y[k / 3] : 0.0);
         if (lane == 0u) s_w[w][k] = v;
     }
     __syncthreads();
@@ -491,7 +498,8 @@ __device__ __forceinline__ void sh_block_sum(const double dsh[27], double *sh_pa
 // entry to global memory once. Device-wide atomics cost far more than LDS ones; a contribution
 // that finds no free slot within kProbe probes goes straight to global memory.
 constexpr uint32_t kTileSide = 16;
-constexpr uint32_t kVtxSlots = 512, kTexSlots = 1024, kProbe = 16;
+// 256 vertex and 512 texel slots: 27 KB of LDS, four blocks a CU.
+constexpr uint32_t kVtxSlots = 256, kTexSlots = 512, kProbe = 16;
 constexpr unsigned long long kEmptyKey = ~0ull;
 
 struct VtxTable {
@@ -556,8 +564,8 @@ __device__ __forceinline__ void gather_pixel(VtxTable &vt, TexTable &tt, const G
 #pragma unroll
     for (int k = 0; k < 3; ++k) {
         const uint32_t v = e.vi[k];
-        const float att[8] = {(float)a.dcol[k][0], (float)a.dcol[k][1], (float)a.dcol[k][2], (float)a.dn[k][0],
-                              (float)a.dn[k][1],   (float)a.dn[k][2],   (float)a.duv[k][0], (float)a.duv[k][1]};
+        const float att[8] = {a.dcol[k][0], a.dcol[k][1], a.dcol[k][2], a.dn[k][0],
+                              a.dn[k][1],   a.dn[k][2],   a.duv[k][0], a.duv[k][1]};
         const int s = lds_slot<kVtxSlots>(vt.key, ((unsigned long long)m << 32) | v);
         if (s < 0) {
             vertex_to_global(g, v, a.dpos[k], att);
@@ -573,9 +581,9 @@ __device__ __forceinline__ void gather_pixel(VtxTable &vt, TexTable &tt, const G
     for (int t = 0; t < 4; ++t) {
         const int s = lds_slot<kTexSlots>(tt.key, ((unsigned long long)m << 32) | a.tap[t]);
         for (int j = 0; j < 3; ++j) {
-            if (a.dtex[t][j] == 0.0) continue;
-            if (s < 0) atomic_add(&g.tex[4u * a.tap[t] + j], a.dtex[t][j]);
-            else unsafeAtomicAdd(&tt.rgb[j][s], (float)a.dtex[t][j]);
+            if (a.dtex[t][j] == 0.0f) continue;
+            if (s < 0) unsafeAtomicAdd(&g.tex[4u * a.tap[t] + j], a.dtex[t][j]);
+            else unsafeAtomicAdd(&tt.rgb[j][s], a.dtex[t][j]);
         }
     }
 }
@@ -627,12 +635,10 @@ __global__ __launch_bounds__(256) void kd_backward_atomic(DiffParams p, const fl
         live = evaluate(p.m, p.v, p.sh, p.prim[i], x, y, e);
         if (live) adjoint(p.m, p.v, p.sh, e, gout[i], a);
     }
-    if (!live)
-        for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
+    sh_block_sum(live, a.dE, e.nh, sh_part);  // first: dE and nh hold no registers past it
     if (live) gather_pixel(vt, tt, sink, 0u, e, a);
     __syncthreads();
     tables_flush(vt, tt, [&](uint32_t) { return sink; });
-    sh_block_sum(a.dsh, sh_part);
 }
 
 __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const float4 *gout, DiffScratch sc) {
@@ -645,15 +651,14 @@ __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const f
         live = evaluate(p.m, p.v, p.sh, p.prim[i], i % p.v.W, i / p.v.W, e);
         if (live) adjoint(p.m, p.v, p.sh, e, gout[i], a);
     }
-    if (!live)
-        for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
+    sh_block_sum(live, a.dE, e.nh, sc.sh_part);
     if (i < P) {
 #pragma unroll
         for (int t = 0; t < 4; ++t) {
             const uint32_t r = 4u * i + (uint32_t)t;
             sc.tkey[r] = live ? a.tap[t] : p.m.tex_total;
             sc.tval[r] = r;
-            sc.tpay[r] = live ? make_float4((float)a.dtex[t][0], (float)a.dtex[t][1], (float)a.dtex[t][2], 0.0f)
+            sc.tpay[r] = live ? make_float4(a.dtex[t][0], a.dtex[t][1], a.dtex[t][2], 0.0f)
                               : make_float4(0.f, 0.f, 0.f, 0.f);
         }
 #pragma unroll
@@ -665,14 +670,11 @@ __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const f
                 sc.vposd[4u * r] = a.dpos[k][0];
                 sc.vposd[4u * r + 1u] = a.dpos[k][1];
                 sc.vposd[4u * r + 2u] = a.dpos[k][2];
-                sc.vpay[2u * r] = make_float4((float)a.dcol[k][0], (float)a.dcol[k][1], (float)a.dcol[k][2],
-                                              (float)a.duv[k][0]);
-                sc.vpay[2u * r + 1u] = make_float4((float)a.duv[k][1], (float)a.dn[k][0], (float)a.dn[k][1],
-                                                   (float)a.dn[k][2]);
+                sc.vpay[2u * r] = make_float4(a.dcol[k][0], a.dcol[k][1], a.dcol[k][2], a.duv[k][0]);
+                sc.vpay[2u * r + 1u] = make_float4(a.duv[k][1], a.dn[k][0], a.dn[k][1], a.dn[k][2]);
             }
         }
     }
-    sh_block_sum(a.dsh, sc.sh_part);
 }
 
 // Sum of the per-block SH partials, one block per coefficient: thread t sums blocks t, t+256, ...
@@ -877,12 +879,10 @@ __global__ __launch_bounds__(256) void kd_batch_backward(DiffBatchParams P, doub
             }
         }
     }
-    if (!live)
-        for (int k = 0; k < 27; ++k) a.dsh[k] = 0.0;
+    sh_block_sum(live, a.dE, e.nh, sh_part);  // first: dE and nh hold no registers past it
     if (live) gather_pixel(vt, tt, mesh_sink(P.meshes[mi]), mi, e, a);
     __syncthreads();
     tables_flush(vt, tt, [&](uint32_t m) { return mesh_sink(P.meshes[m]); });
-    sh_block_sum(a.dsh, sh_part);
 }
 
 // Clears the listed word ranges, 1024 words a block.
