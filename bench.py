@@ -469,6 +469,21 @@ def bench_c5_diff(dev, sc, args, ev):
     _, ph_b = timed(False, 1)
     if args.no_per_draw:
         ms_p, ph_p = float("nan"), {}
+    # the same step as one HIP graph (raster frame, shading, loss gradient, adjoint), replayed
+    graph = r.capture_step(sh, out, target, scale, gout, grads, grad_sh)
+    try:
+        r.run_graph(graph)
+        a, b = ev.create(), ev.create()
+        torch.cuda.synchronize()
+        dev.synchronize()
+        ev.record(a, dev.stream)
+        for _ in range(args.c5_steps):
+            r.run_graph(graph)
+        ev.record(b, dev.stream)
+        ms_graph = ev.elapsed_ms(a, b) / args.c5_steps
+        dev.synchronize()
+    finally:
+        graph.release()
     # the batched step's footprint (untimed): visible triangles over all views from the raster,
     # the texels the shading's footprints read
     fp = footprint_of(dev, lambda: (r.rasterize(), fwd(), torch.cuda.synchronize()))
@@ -479,7 +494,8 @@ def bench_c5_diff(dev, sc, args, ev):
     r.release()
     del grads, accum, out, target, gout
     torch.cuda.empty_cache()
-    return {"fwd_bwd_batched_ms": round(ms_b, 3), "fwd_bwd_per_draw_ms": round(ms_p, 3),
+    return {"fwd_bwd_batched_ms": round(ms_b, 3), "fwd_bwd_graph_ms": round(ms_graph, 3),
+            "fwd_bwd_per_draw_ms": round(ms_p, 3),
             "fwd_bwd_batched_views_per_s": round(n / (ms_b * 1e-3), 2),
             "fwd_bwd_per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
             "fwd_bwd_batched_over_per_draw": round(ms_p / ms_b, 2),

@@ -281,8 +281,9 @@ MLV_API int mlv_camera_constant_buffers(mlv_context *ctx, const mlv_camera *came
  *
  * Between mlv_graph_capture_begin and mlv_graph_capture_end, clears and draws are recorded
  * instead of run, with the state they read resolved as they are issued; only state setters,
- * clears and draws may be called meanwhile (anything that flushes returns
- * MLV_ERROR_INVALID_STATE). capture_end runs the recorded commands once (sizing every scratch
+ * clears, draws, mlv_diff_batch_forward / mlv_diff_batch_backward and mlv_diff_l2_grad may be
+ * called meanwhile (anything that flushes returns MLV_ERROR_INVALID_STATE; a recorded batched
+ * pass runs after the draws recorded before it). capture_end runs the recorded commands once (sizing every scratch
  * arena; this is the frame's first execution) and then captures the same commands into a HIP
  * graph. mlv_graph_launch replays the whole frame as one graph launch on the context stream:
  * the same targets, buffers and textures, whose contents the caller may change between
@@ -444,6 +445,15 @@ MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes
 MLV_API int mlv_diff_batch_backward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
                                     const mlv_diff_view *views, uint32_t view_count, const float *device_sh,
                                     float *device_grad_sh, int accumulate);
+/* out[i] = (a[i] - b[i]) * scale for i < count (device arrays): the gradient of
+ * (scale / 2) * sum((a - b)^2) with respect to a, i.e. of the mean squared error for
+ * scale = 2 / count. No reference counterpart: the loss of the differentiable step. Like the two
+ * batched passes, it may be recorded into a graph (mlv_graph_capture_begin/end), so that one
+ * graph holds a whole step: the raster frame, the shading pass, the loss gradient and the adjoint;
+ * a recorded pass keeps copies of its descriptor arrays and draw maps, and the device buffers
+ * they name must outlive the graph. */
+MLV_API int mlv_diff_l2_grad(mlv_context *ctx, const float *device_a, const float *device_b, uint64_t count,
+                             float scale, float *device_out);
 
 #ifdef __cplusplus
 }

@@ -169,6 +169,7 @@ SIGNATURES = {
     "mlv_diff_backward_noalloc": (_I, [_P, C.POINTER(DiffScene), _P, _P, C.POINTER(DiffGrads)]),
     "mlv_diff_batch_forward": (_I, [_P, C.POINTER(DiffMesh), _U32, C.POINTER(DiffView), _U32, _P]),
     "mlv_diff_batch_backward": (_I, [_P, C.POINTER(DiffMesh), _U32, C.POINTER(DiffView), _U32, _P, _P, _I]),
+    "mlv_diff_l2_grad": (_I, [_P, _P, _P, C.c_uint64, C.c_float, _P]),
     "mlv_mip_chain_size": (_I, [_U32, _U32, C.POINTER(_U32), C.POINTER(_U32)]),
     "mlv_mip_chain": (_I, [_P, _P, _U32, _U32, _P]),
     "mlv_mip_chain_adjoint": (_I, [_P, _P, _U32, _U32, _P]),

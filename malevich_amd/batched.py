@@ -105,13 +105,14 @@ class BatchedDiffRenderer:
         return flat[o: o + self.pixels[v]].view(self.views[v].height, self.views[v].width, 4)
 
     # ---------------------------------------------------------------- raster
-    def rasterize(self, per_draw: bool = False) -> None:
+    def rasterize(self, per_draw: bool = False, recording: bool = False) -> None:
         """Draws every mesh into every view (basic_vs, reversed-Z GREATER_EQUAL); primitive ids and
         draw tags land in self.prim / self.tags. Draw tags restart at 0: view v's draw of mesh k
-        has tag v * len(meshes) + k."""
+        has tag v * len(meshes) + k. recording: inside a graph capture (no stats reset, no flush)."""
         d = self.dev
         self._enter()
-        d.reset_stats()
+        if not recording:
+            d.reset_stats()
         d.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
         d.vs_set_shader(L.VS_BASIC)
         d.ps_set_shader(L.PS_PASSTHROUGH)
@@ -130,7 +131,8 @@ class BatchedDiffRenderer:
                 d.draw_indexed(m.indices.numel())
                 if per_draw:
                     d.flush()
-        d.flush()
+        if not recording:
+            d.flush()
         d.om_set_primitive_id_buffer(None)
         d.om_set_visibility_buffer(None, None)
         self._leave()
@@ -238,6 +240,40 @@ class BatchedDiffRenderer:
                 t.zero_()
             self._run(calls)
         return run
+
+    def capture_step(self, sh: torch.Tensor, out: torch.Tensor, target: torch.Tensor, scale: float,
+                     grad_out: torch.Tensor, grads: list, grad_sh: torch.Tensor | None) -> P.Graph:
+        """One HIP graph of a whole batched step: the raster frame, the shading pass into `out`,
+        grad_out = (out - target) * scale (the mean-squared-error gradient for scale = 2 / numel)
+        and the adjoint into grads / grad_sh. Capturing runs the step once; replay it with
+        run_graph."""
+        fwd = self.plan_forward(sh, out)
+        bwd = self.plan_backward(sh, grad_out, grads, grad_sh)
+        d = self.dev
+        d.reset_stats()  # draw tags count from 0, as in rasterize(); the graph keeps them
+        d.graph_capture_begin()
+        try:
+            self.rasterize(recording=True)
+            fwd()
+            self._run([(d.lib.mlv_diff_l2_grad, (out.data_ptr(), target.data_ptr(), out.numel(), float(scale),
+                                                 grad_out.data_ptr()))])
+            bwd()
+        except BaseException:
+            try:  # leave the recording state
+                d.graph_capture_end().release()
+            except Exception:
+                pass
+            raise
+        self._enter()
+        g = d.graph_capture_end()
+        self._leave()
+        return g
+
+    def run_graph(self, g: P.Graph) -> None:
+        """Replays a captured step, ordered with the torch stream."""
+        self._enter()
+        g.launch()
+        self._leave()
 
     def forward(self, sh: torch.Tensor, out: torch.Tensor | None = None) -> torch.Tensor:
         out = self.new_image() if out is None else out

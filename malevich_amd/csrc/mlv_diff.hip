@@ -1065,6 +1065,26 @@ hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const D
     return hipGetLastError();
 }
 
+// out = (a - b) * scale, four floats a lane where the arrays allow.
+__global__ __launch_bounds__(256) void kd_l2_grad(const float *a, const float *b, uint64_t n, float scale, float *out) {
+    const uint64_t stride = (uint64_t)gridDim.x * 256u;
+    const uint64_t n4 = ((((uintptr_t)a | (uintptr_t)b | (uintptr_t)out) & 15u) == 0u) ? n / 4u : 0u;
+    for (uint64_t i = (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n4; i += stride) {
+        const float4 x = ((const float4 *)a)[i], y = ((const float4 *)b)[i];
+        ((float4 *)out)[i] = make_float4((x.x - y.x) * scale, (x.y - y.y) * scale, (x.z - y.z) * scale,
+                                         (x.w - y.w) * scale);
+    }
+    for (uint64_t i = 4u * n4 + (uint64_t)blockIdx.x * 256u + threadIdx.x; i < n; i += stride)
+        out[i] = (a[i] - b[i]) * scale;
+}
+
+hipError_t launch_l2_grad(const float *a, const float *b, uint64_t n, float scale, float *out, hipStream_t s) {
+    if (!n) return hipSuccess;
+    const unsigned grid = (unsigned)std::min<uint64_t>((n / 4u + 255u) / 256u + 1u, 8192u);
+    hipLaunchKernelGGL(kd_l2_grad, dim3(grid), dim3(256), 0, s, a, b, n, scale, out);
+    return hipGetLastError();
+}
+
 constexpr uint32_t kFixBlocks = 512;  // the fix-up pass's grid (its list is short)
 
 hipError_t launch_diff_batch_forward(const DiffBatchParams &p, hipStream_t s) {

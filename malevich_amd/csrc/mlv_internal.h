@@ -289,6 +289,7 @@ hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const D
 uint32_t diff_blocks(uint32_t W, uint32_t H);
 uint32_t diff_tile_blocks(uint32_t W, uint32_t H);
 hipError_t launch_diff_batch_forward(const DiffBatchParams &p, hipStream_t s);
+hipError_t launch_l2_grad(const float *a, const float *b, uint64_t n, float scale, float *out, hipStream_t s);
 // accumulate: keep the gradients and position sums of the previous call and add to them.
 hipError_t launch_diff_batch_backward(const DiffBatchParams &p, double *sh_part, float *grad_sh, bool accumulate,
                                       hipStream_t s);

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <memory>
 #include <mutex>
 #include <new>
 #include <string>
@@ -82,9 +83,26 @@ struct PendingDraw {
     ViewKey key;
 };
 
-// A command recorded for a graph: a colour clear, a depth clear or a draw, with every piece of
-// state it reads resolved when it was issued.
-enum CmdKind { kCmdClearColor = 0, kCmdClearDepth = 1, kCmdDraw = 2 };
+// A batched differentiable pass or a loss gradient recorded for a graph: its arguments, with the
+// host arrays they point to (the views' draw maps) copied.
+struct DiffCmd {
+    std::vector<mlv_diff_mesh> meshes;
+    std::vector<mlv_diff_view> views;
+    std::vector<uint32_t> draw_mesh;  // the views' draw maps, concatenated
+    const float *sh = nullptr;
+    float *grad_sh = nullptr;
+    int accumulate = 0;
+    const float *a = nullptr, *b = nullptr;  // loss gradient: out = (a - b) * scale
+    float *out = nullptr;
+    uint64_t n = 0;
+    float scale = 0.0f;
+};
+
+// A command recorded for a graph: a colour clear, a depth clear, a draw, a batched
+// differentiable pass or a loss gradient, with every piece of state it reads resolved when it
+// was issued.
+enum CmdKind { kCmdClearColor = 0, kCmdClearDepth = 1, kCmdDraw = 2, kCmdDiffForward = 3, kCmdDiffBackward = 4,
+               kCmdL2Grad = 5 };
 struct Cmd {
     int kind = 0;
     mlv_render_target *rt = nullptr;
@@ -95,11 +113,15 @@ struct Cmd {
     uint32_t *vis_tri = nullptr, *vis_draw = nullptr, *vis_prim = nullptr;  // cleared with the colour
     PendingDraw pd{};
     uint32_t T = 0, V = 0;
+    std::shared_ptr<DiffCmd> diff;
 };
 
 // One flush never exceeds these (indices are 32-bit; the arenas stay bounded).
-constexpr uint64_t kMaxBatchTriangles = 48ull << 20;
-constexpr uint64_t kMaxBatchVertices = 48ull << 20;
+// Default flush size, and a draw's limit: 96M input triangles (256 views of 64 meshes, 1.28G
+// triangle draws, measured 85.2 / 81.6 / 83.3 ms of forward raster with 48M / 96M / 192M flushes).
+constexpr uint64_t kMaxBatchTriangles = 96ull << 20;
+constexpr uint64_t kMaxBatchVertices = 96ull << 20;
+constexpr uint64_t kBatchCeiling = 256ull << 20;      // MLV_MAX_BATCH_TRIANGLES may raise flushes to this
 constexpr int kStagingSlots = 16;
 
 }  // namespace
@@ -164,7 +186,7 @@ struct mlv_context {
     size_t pair_limit = 0;  // MLV_PAIR_CAPACITY: caps the bin-list arena (exercises the overflow path)
     // a flush takes recorded draws until one more would pass either limit (MLV_MAX_BATCH_TRIANGLES
     // and MLV_MAX_BATCH_DRAWS lower them; a single larger draw still runs, alone)
-    uint64_t batch_tris = kMaxBatchTriangles, batch_draws = kMaxBatchDraws;
+    uint64_t batch_tris = kMaxBatchTriangles, batch_verts = kMaxBatchVertices, batch_draws = kMaxBatchDraws;
     // stage profiling: event pairs recorded per stage, resolved at mlv_get_profile
     bool profiling = false;
     std::vector<hipEvent_t> ev_pool;
@@ -351,7 +373,8 @@ static uint32_t lds_pad(uint32_t lds, uint32_t wpb, uint32_t waves) {
 
 static int flush(mlv_context *ctx) {
     if (ctx->recording)
-        return fail(ctx, MLV_ERROR_INVALID_STATE, "only state, clears and draws may be issued while recording a graph");
+        return fail(ctx, MLV_ERROR_INVALID_STATE,
+                    "only state, clears, draws and batched differentiable passes may be issued while recording a graph");
     if (ctx->pending.empty()) return MLV_OK;
     RC_OR_RETURN(set_device(ctx));
     std::vector<PendingDraw> draws;
@@ -663,10 +686,10 @@ MLV_API int mlv_context_create(int device, mlv_context **out) {
     if (!ctx) return MLV_ERROR_OUT_OF_MEMORY;
     ctx->device = device;
     if (const char *lim = getenv("MLV_PAIR_CAPACITY")) ctx->pair_limit = (size_t)strtoull(lim, nullptr, 10);
-    // smaller flushes than the defaults (a batch split exercised on small scenes); never larger
+    // flushes of another size than the default (smaller: a batch split exercised on small scenes)
     if (const char *lim = getenv("MLV_MAX_BATCH_TRIANGLES")) {
         const uint64_t v = strtoull(lim, nullptr, 10);
-        if (v) ctx->batch_tris = std::min<uint64_t>(v, kMaxBatchTriangles);
+        if (v) ctx->batch_tris = ctx->batch_verts = std::min<uint64_t>(v, kBatchCeiling);
     }
     if (const char *lim = getenv("MLV_MAX_BATCH_DRAWS")) {
         const uint64_t v = strtoull(lim, nullptr, 10);
@@ -1204,7 +1227,7 @@ static int submit_draw(mlv_context *ctx, const Cmd &c) {
         ds->hiz_mode = pd.key.depth_func;
     }
     if (ctx->pending.size() >= ctx->batch_draws || ctx->pending_tris + align256(c.T) > ctx->batch_tris ||
-        ctx->pending_verts + align256(c.V) > kMaxBatchVertices)
+        ctx->pending_verts + align256(c.V) > ctx->batch_verts)
         RC_OR_RETURN(flush(ctx));
     ctx->pending.push_back(pd);
     ctx->pending_tris += align256(c.T);
@@ -1335,12 +1358,18 @@ static int keep_one_run_stats(mlv_context *ctx, const DevStats &base, const DevS
     return MLV_OK;
 }
 
+static int exec_diff_cmd(mlv_context *ctx, const Cmd &c);
+
 static int run_recorded(mlv_context *ctx, const std::vector<Cmd> &rec) {
     for (const Cmd &c : rec) {
         switch (c.kind) {
         case kCmdClearColor: RC_OR_RETURN(exec_clear_color(ctx, c)); break;
         case kCmdClearDepth: RC_OR_RETURN(exec_clear_depth(ctx, c)); break;
-        default: RC_OR_RETURN(submit_draw(ctx, c)); break;
+        case kCmdDraw: RC_OR_RETURN(submit_draw(ctx, c)); break;
+        default:
+            RC_OR_RETURN(flush(ctx));  // the passes read what the recorded draws wrote
+            RC_OR_RETURN(exec_diff_cmd(ctx, c));
+            break;
         }
     }
     return flush(ctx);
@@ -2086,11 +2115,8 @@ static int diff_batch_setup(mlv_context *ctx, const mlv_diff_mesh *meshes, uint3
     return MLV_OK;
 }
 
-MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
-                                   const mlv_diff_view *views, uint32_t view_count, const float *device_sh) {
-    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
-    RC_OR_RETURN(set_device(ctx));
-    RC_OR_RETURN(flush(ctx));  // the id buffers come from recorded draws
+static int diff_batch_forward_now(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                                  const mlv_diff_view *views, uint32_t view_count, const float *device_sh) {
     DiffBatchParams P;
     RC_OR_RETURN(diff_batch_setup(ctx, meshes, mesh_count, views, view_count, device_sh, false, false, P));
     const void *old = ctx->b_defer.p;
@@ -2106,17 +2132,103 @@ MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes
     return MLV_OK;
 }
 
-MLV_API int mlv_diff_batch_backward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
-                                    const mlv_diff_view *views, uint32_t view_count, const float *device_sh,
-                                    float *device_grad_sh, int accumulate) {
-    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
-    RC_OR_RETURN(set_device(ctx));
-    RC_OR_RETURN(flush(ctx));
+static int diff_batch_backward_now(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                                   const mlv_diff_view *views, uint32_t view_count, const float *device_sh,
+                                   float *device_grad_sh, int accumulate) {
     DiffBatchParams P;
     RC_OR_RETURN(diff_batch_setup(ctx, meshes, mesh_count, views, view_count, device_sh, true, accumulate != 0, P));
     RC_OR_RETURN(ensure(ctx, ctx->b_shpart, (size_t)std::max<uint32_t>(P.n_blocks, 1u) * 27 * 8));
     HIP_OR_FAIL(ctx, launch_diff_batch_backward(P, (double *)ctx->b_shpart.p, device_grad_sh, accumulate != 0,
                                                 ctx->stream));
+    return MLV_OK;
+}
+
+// While a graph is recorded, a batched pass or a loss gradient is recorded with copies of its
+// descriptor arrays and draw maps (the device buffers they name must outlive the graph).
+static int record_diff(mlv_context *ctx, int kind, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                       const mlv_diff_view *views, uint32_t view_count, std::shared_ptr<DiffCmd> d) {
+    if ((mesh_count && !meshes) || (view_count && !views))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: meshes and views are required");
+    d->meshes.assign(meshes, meshes + mesh_count);
+    d->views.assign(views, views + view_count);
+    for (const mlv_diff_view &v : d->views) {
+        if (v.draw_count && !v.draw_mesh)
+            return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "diff batch: a view lacks its draw map");
+        if (v.draw_count) d->draw_mesh.insert(d->draw_mesh.end(), v.draw_mesh, v.draw_mesh + v.draw_count);
+    }
+    Cmd c;
+    c.kind = kind;
+    c.diff = std::move(d);
+    ctx->rec.push_back(std::move(c));
+    return MLV_OK;
+}
+
+static int exec_diff_cmd(mlv_context *ctx, const Cmd &c) {
+    const DiffCmd &d = *c.diff;
+    if (c.kind == kCmdL2Grad) {
+        HIP_OR_FAIL(ctx, launch_l2_grad(d.a, d.b, d.n, d.scale, d.out, ctx->stream));
+        return MLV_OK;
+    }
+    std::vector<mlv_diff_view> views = d.views;
+    size_t off = 0;
+    for (mlv_diff_view &v : views) {
+        v.draw_mesh = v.draw_count ? d.draw_mesh.data() + off : nullptr;
+        off += v.draw_count;
+    }
+    if (c.kind == kCmdDiffForward)
+        return diff_batch_forward_now(ctx, d.meshes.data(), (uint32_t)d.meshes.size(), views.data(),
+                                      (uint32_t)views.size(), d.sh);
+    return diff_batch_backward_now(ctx, d.meshes.data(), (uint32_t)d.meshes.size(), views.data(),
+                                   (uint32_t)views.size(), d.sh, d.grad_sh, d.accumulate);
+}
+
+MLV_API int mlv_diff_batch_forward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                                   const mlv_diff_view *views, uint32_t view_count, const float *device_sh) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (ctx->recording) {
+        auto d = std::make_shared<DiffCmd>();
+        d->sh = device_sh;
+        return record_diff(ctx, kCmdDiffForward, meshes, mesh_count, views, view_count, std::move(d));
+    }
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));  // the id buffers come from recorded draws
+    return diff_batch_forward_now(ctx, meshes, mesh_count, views, view_count, device_sh);
+}
+
+MLV_API int mlv_diff_batch_backward(mlv_context *ctx, const mlv_diff_mesh *meshes, uint32_t mesh_count,
+                                    const mlv_diff_view *views, uint32_t view_count, const float *device_sh,
+                                    float *device_grad_sh, int accumulate) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (ctx->recording) {
+        auto d = std::make_shared<DiffCmd>();
+        d->sh = device_sh;
+        d->grad_sh = device_grad_sh;
+        d->accumulate = accumulate;
+        return record_diff(ctx, kCmdDiffBackward, meshes, mesh_count, views, view_count, std::move(d));
+    }
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    return diff_batch_backward_now(ctx, meshes, mesh_count, views, view_count, device_sh, device_grad_sh,
+                                   accumulate);
+}
+
+MLV_API int mlv_diff_l2_grad(mlv_context *ctx, const float *device_a, const float *device_b, uint64_t count,
+                             float scale, float *device_out) {
+    if (!ctx) return MLV_ERROR_INVALID_ARGUMENT;
+    if (count && (!device_a || !device_b || !device_out))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "l2_grad: a, b and out are required");
+    if (ctx->recording) {
+        auto d = std::make_shared<DiffCmd>();
+        d->a = device_a;
+        d->b = device_b;
+        d->out = device_out;
+        d->n = count;
+        d->scale = scale;
+        return record_diff(ctx, kCmdL2Grad, nullptr, 0, nullptr, 0, std::move(d));
+    }
+    RC_OR_RETURN(set_device(ctx));
+    RC_OR_RETURN(flush(ctx));
+    HIP_OR_FAIL(ctx, launch_l2_grad(device_a, device_b, count, scale, device_out, ctx->stream));
     return MLV_OK;
 }
 

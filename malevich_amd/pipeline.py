@@ -396,7 +396,8 @@ class Device:
 
     # ---------------------------------------------------------- graphs
     def graph_capture_begin(self) -> None:
-        """Record the following clears and draws instead of running them."""
+        """Record the following clears, draws and batched differentiable passes instead of running
+        them."""
         _check(self.ctx, self.lib.mlv_graph_capture_begin(self.ctx))
 
     def graph_capture_end(self) -> Graph:
