@@ -251,8 +251,10 @@ def bench_c3(dev, args):
 def bench_c4(dev, args):
     """Differentiable render at 512x512: a 200k-triangle displaced sphere with per-vertex colours,
     UVs and normals, a 1024x1024 RGBA32F texture with its mip chain and a 9-coefficient SH light.
-    One step = raster (visibility) + shading forward + L2 loss against a target rendered with
-    perturbed parameters (texels +-0.1, vertices +-1e-3) + the backward pass to every parameter."""
+    One step = raster (visibility) + shading forward + L2 loss gradient against a target rendered
+    with perturbed parameters (texels +-0.1, vertices +-1e-3) + the backward pass to every parameter
+    and through the mip chain to the texture. Timed as DiffStep issues it (fwd_bwd_ms) and through
+    torch.autograd (autograd_fwd_bwd_ms); host_issue_ms is the host's time to issue a step."""
     import torch
 
     from malevich_amd import diff as D
@@ -287,17 +289,33 @@ def bench_c4(dev, args):
             loss.backward()
             return loss
 
-        for _ in range(max(1, args.warmup)):
-            step()
-        torch.cuda.synchronize()
-        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-        e0.record()
-        for _ in range(args.c4_steps):
-            step()
-        e1.record()
-        torch.cuda.synchronize()
-        ms = e0.elapsed_time(e1) / args.c4_steps
+        def timed(run):
+            for _ in range(max(1, args.warmup)):
+                run()
+            torch.cuda.synchronize()
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            h0 = time.perf_counter()
+            for _ in range(args.c4_steps):
+                run()
+            host_ms = (time.perf_counter() - h0) * 1e3 / args.c4_steps  # issuing only (no synchronisation)
+            e1.record()
+            torch.cuda.synchronize()
+            return e0.elapsed_time(e1) / args.c4_steps, host_ms
+
+        # the step as a training loop's inner iteration issues it (DiffStep: a few library calls, the
+        # raster one graph launch), then the same kernels through torch.autograd, whose Python work
+        # per step can exceed the GPU's
+        fast = D.DiffStep(r, params["positions"], params["colors"], params["uvs"], params["normals"], idx,
+                          params["texture"], params["sh"], M, tgt)
+        ms, host_ms = timed(fast)
+        fast.release()
+        ms_ag, host_ag = timed(step)
+        sfx = "_deterministic" if det else ""
         out["deterministic" if det else "atomic"] = round(ms, 4)
+        out["host_issue_ms" + sfx] = round(host_ms, 4)
+        out["autograd_ms" + sfx] = round(ms_ag, 4)
+        out["autograd_host_issue_ms" + sfx] = round(host_ag, 4)
         if not det:  # one more step under footprint counting (untimed)
             out["footprint"] = footprint_of(dev, lambda: (step(), torch.cuda.synchronize()))
         r.release()
@@ -314,6 +332,11 @@ def bench_c4(dev, args):
                         f"{sc.levels} mips, SH9 light, L2 loss, gradients to positions/colours/uvs/normals/texels/SH",
             "fwd_bwd_ms": out["atomic"], "fwd_bwd_ms_deterministic": out["deterministic"],
             "covered_pixels": covered, "steps": args.c4_steps, "fwd_bwd_roofline": step_roofline,
+            "host_issue_ms_per_step": out["host_issue_ms"],
+            "host_issue_ms_per_step_deterministic": out["host_issue_ms_deterministic"],
+            "autograd_fwd_bwd_ms": out["autograd_ms"], "autograd_fwd_bwd_ms_deterministic": out["autograd_ms_deterministic"],
+            "autograd_host_issue_ms_per_step": out["autograd_host_issue_ms"],
+            "autograd_host_issue_ms_per_step_deterministic": out["autograd_host_issue_ms_deterministic"],
             "footprint_per_step": fp, "tuning": out["tuning"],
             "tuning_deterministic": out["tuning_deterministic"]}
 

@@ -241,3 +241,61 @@ def render(renderer: DiffRenderer, positions, colors, uvs, normals, indices, tex
     chain, levels = mip_chain(texture, renderer)
     return RenderFunction.apply(positions, colors, uvs, normals, chain, sh, renderer, indices, clip_from_world,
                                 texture.shape[1], texture.shape[0], levels)
+
+
+class DiffStep:
+    """One differentiable step (raster, shading, the L2 loss gradient, the adjoint) of a fixed
+    mesh, camera and target, with the host work of a training loop's inner iteration cut to a few
+    library calls: the vertex buffer packed from the current parameters (one torch op), the
+    texture's mip chain, the raster frame as one graph launch (captured once), mlv_diff_forward,
+    mlv_diff_l2_grad (the gradient of sum((image - target)^2): scale 2), mlv_diff_backward and the
+    mip chain's adjoint. The same kernels as render() + backward; the parameters' gradients land
+    in self.grads (positions, colors, uvs, normals, texture, sh)."""
+
+    def __init__(self, renderer: DiffRenderer, positions, colors, uvs, normals, indices, texture, sh,
+                 clip_from_world, target):
+        self.r = r = renderer
+        d = r.dev
+        self.params = dict(positions=positions, colors=colors, uvs=uvs, normals=normals, texture=texture, sh=sh)
+        self.idx = indices.contiguous().to(torch.int32)
+        self.target = target.contiguous().float()
+        h, w = int(texture.shape[0]), int(texture.shape[1])
+        self.tex_w, self.tex_h = w, h
+        n, self.levels = L.mip_chain_size(w, h)
+        self.chain = torch.empty((n, 4), dtype=torch.float32, device="cuda")
+        self.verts, self.cb = raster_inputs(positions, normals, uvs,
+                                            clip_from_world.detach().to(torch.float32).reshape(4, 4).cpu(), r.cb_cache)
+        self.image = torch.empty((r.height, r.width, 4), dtype=torch.float32, device="cuda")
+        self.grad_image = torch.empty_like(self.image)
+        self.grad_chain = torch.empty_like(self.chain)
+        self.grads = {k: torch.empty_like(v) for k, v in self.params.items()}
+        g = self.grads
+        self.gstruct = L.DiffGrads(g["positions"].data_ptr(), g["colors"].data_ptr(), g["uvs"].data_ptr(),
+                                   g["normals"].data_ptr(), self.grad_chain.data_ptr(), g["sh"].data_ptr())
+        self.scene = r.scene_struct(positions, colors, uvs, normals, self.idx, self.chain, w, h, self.levels, sh,
+                                    clip_from_world.detach().to(torch.float32).reshape(4, 4).cpu())
+        r._enter()
+        d.graph_capture_begin()
+        r._record_raster(self.verts, self.idx, self.cb)
+        self.raster = d.graph_capture_end()
+        d.om_set_primitive_id_buffer(None)
+        r._leave()
+
+    def __call__(self) -> None:
+        r, d, p = self.r, self.r.dev, self.params
+        torch.cat([p["positions"].detach(), p["normals"].detach(), p["uvs"].detach()], 1, out=self.verts)
+        r._enter()
+        lib, ctx = d.lib, d.ctx
+        P._check(ctx, lib.mlv_mip_chain(ctx, p["texture"].data_ptr(), self.tex_w, self.tex_h, self.chain.data_ptr()))
+        self.raster.launch()
+        P._check(ctx, lib.mlv_diff_forward(ctx, C.byref(self.scene), r.prim.data_ptr(), self.image.data_ptr()))
+        P._check(ctx, lib.mlv_diff_l2_grad(ctx, self.image.data_ptr(), self.target.data_ptr(), self.image.numel(), 2.0,
+                                           self.grad_image.data_ptr()))
+        P._check(ctx, lib.mlv_diff_backward(ctx, C.byref(self.scene), r.prim.data_ptr(), self.grad_image.data_ptr(),
+                                            C.byref(self.gstruct), int(r.deterministic)))
+        P._check(ctx, lib.mlv_mip_chain_adjoint(ctx, self.grad_chain.data_ptr(), self.tex_w, self.tex_h,
+                                                self.grads["texture"].data_ptr()))
+        r._leave()
+
+    def release(self) -> None:
+        self.raster.release()

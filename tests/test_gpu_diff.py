@@ -111,6 +111,49 @@ def test_c4_full_scene_gradients_deterministic(device, oracle, full_scene):
     _check(device, oracle, full_scene, deterministic=True)
 
 
+
+@pytest.mark.parametrize("deterministic", [False, True])
+def test_diff_step_equals_the_autograd_step(device, small_scene, deterministic):
+    """DiffStep runs the kernels of render() + backward() with a few host calls (the raster as a
+    graph launch). Bar: its gradients equal the autograd path's, bit for bit in deterministic mode
+    and to 1e-6 relative with atomics (they add in another order), on the first call and after the
+    parameters change in place."""
+    sc = small_scene
+    print(f"case: diff step det={deterministic}", flush=True)
+    r = D.DiffRenderer(device, sc.width, sc.height, deterministic=deterministic)
+    step = None
+    try:
+        t = _tensors(sc)
+        params = {k: t[k].clone().requires_grad_(True) for k in ("positions", "colors", "uvs", "normals", "texture",
+                                                                  "sh")}
+        gen = torch.Generator(device="cuda").manual_seed(3)
+        target = torch.rand((sc.height, sc.width, 4), device="cuda", generator=gen)
+        step = D.DiffStep(r, params["positions"], params["colors"], params["uvs"], params["normals"], t["indices"],
+                          params["texture"], params["sh"], t["M"], target)
+        for rnd in range(2):
+            if rnd == 1:
+                with torch.no_grad():
+                    params["positions"].mul_(1.001)
+                    params["texture"].mul_(0.9)
+            for v in params.values():
+                v.grad = None
+            img = D.render(r, params["positions"], params["colors"], params["uvs"], params["normals"], t["indices"],
+                           params["texture"], params["sh"], t["M"])
+            ((img - target) ** 2).sum().backward()
+            step()
+            torch.cuda.synchronize()
+            for k, v in params.items():
+                a, b = step.grads[k], v.grad
+                if deterministic:
+                    assert torch.equal(a.view(torch.int32), b.view(torch.int32)), (rnd, k)
+                else:
+                    e = rel_err(a.cpu().numpy(), b.cpu().numpy())
+                    assert e <= 1e-6, (rnd, k, e)
+    finally:
+        if step is not None:
+            step.release()
+        r.release()
+
 def _box_adjoint_to_level0(chain_grad: np.ndarray, w: int, h: int, levels: int) -> np.ndarray:
     """Adjoint of the 2x2 box mip chain (power-of-two sizes): every level's gradient spread back
     to level 0, each coarse texel's gradient divided equally over its 4^l source texels."""
