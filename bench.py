@@ -166,7 +166,8 @@ def measured_traffic(kernel: str, sc):
         return None
     if (rec.get("triangles"), rec.get("width"), rec.get("height")) != (sc.triangle_count, sc.width, sc.height):
         return None
-    k = next((v for name, v in rec["kernels"].items() if name.split("<")[0] == kernel), None)
+    ks = [v for name, v in rec["kernels"].items() if name.split("<")[0] == kernel]
+    k = max(ks, key=lambda v: v.get("launches", 0), default=None)  # the variant the frame ran
     return None if k is None else int(k["fetch_bytes"] + k["write_bytes"])
 
 
@@ -546,9 +547,14 @@ def adjoint_roofline(pixels: int, fp: dict, vertices: int, ms: float) -> dict:
     return r
 
 
+NO_FOOTPRINT = False  # --no-footprint: counter runs, whose per-kernel averages a counting run would skew
+
+
 def footprint_of(dev, run) -> dict:
     """Visible triangles, pixels written, distinct texels and texel bytes of one run of `run`, from
     the library's footprint counting (a measurement mode with atomics, never timed)."""
+    if NO_FOOTPRINT:
+        return {"visible_triangles": 0, "shaded_pixels": 0, "texels": 0, "texel_bytes": 0}
     dev.set_footprint_counting(True)
     try:
         dev.footprint()  # drop anything counted before
@@ -633,10 +639,16 @@ def main(argv=None):
     ap.add_argument("--c4-steps", type=int, default=20)
     ap.add_argument("--no-c4", action="store_true")
     ap.add_argument("--c4-only", action="store_true", help="run only the differentiable-render leg")
+    ap.add_argument("--no-footprint", action="store_true",
+                    help="skip the footprint-counting runs (for counter runs; the rooflines then read zero)")
+    ap.add_argument("--tuning", type=str, default="",
+                    help="JSON launch parameters for the 1080p leg instead of autotuning them (for counter runs)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (a known-size kernel "
                          "against which HBM byte counters are scaled)")
     args = ap.parse_args(argv)
+    global NO_FOOTPRINT
+    NO_FOOTPRINT = args.no_footprint
     if args.gpus != 1:
         print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
               "design is described in DESIGN.md and is not launched here.", file=sys.stderr)
@@ -674,10 +686,14 @@ def main(argv=None):
     bind_state(dev, res, sc)
     ev = HipEvents()
     warm(dev, lambda: frame(dev, res, sc), args.warmup)
-    dev.autotune_begin()
-    frame(dev, res, sc)
-    tuning = dev.autotune_end(repeats=5)
-    print(f"[bench] autotuned: {tuning}", file=sys.stderr, flush=True)
+    if args.tuning:
+        dev.set_tuning(**dict(DEFAULT_TUNING, **json.loads(args.tuning)))
+        tuning = dev.tuning()
+    else:
+        dev.autotune_begin()
+        frame(dev, res, sc)
+        tuning = dev.autotune_end(repeats=5)
+    print(f"[bench] {'set' if args.tuning else 'autotuned'}: {tuning}", file=sys.stderr, flush=True)
     dev.reset_stats()
     dev.profile()  # drop anything accumulated
     dev.set_profiling(True)

@@ -1050,8 +1050,12 @@ template <uint32_t NT>
 __device__ __forceinline__ void radix_hist_block(const uint32_t *keys, uint32_t n, uint32_t nb, uint32_t b,
                                                  uint32_t shift, uint32_t *hist) {
     constexpr uint32_t R = kRadixItems / NT;
-    __shared__ uint32_t h[256];
-    if (threadIdx.x < 256u) h[threadIdx.x] = 0u;
+    // Four copies of the counters, lane & 3 choosing one: neighbouring keys (pairs of neighbouring
+    // triangles) often share a digit, and one LDS atomic serialises its lanes per address. The
+    // copies sit kHistPitch words apart, so one digit's four counters fall in four banks.
+    constexpr uint32_t kHistPitch = 257u;
+    __shared__ uint32_t h[4 * kHistPitch];
+    for (uint32_t i = threadIdx.x; i < 4u * kHistPitch; i += NT) h[i] = 0u;
     __syncthreads();
     const uint32_t base = b * kRadixItems;
     // all of the thread's loads are issued before the first count, so their latencies overlap
@@ -1061,11 +1065,14 @@ __device__ __forceinline__ void radix_hist_block(const uint32_t *keys, uint32_t 
         const uint32_t i = base + r * NT + threadIdx.x;
         k[r] = i < n ? keys[i] : 0xFFFFFFFFu;
     }
+    uint32_t *mine = h + (threadIdx.x & 3u) * kHistPitch;
 #pragma unroll
     for (uint32_t r = 0; r < R; ++r)
-        if (base + r * NT + threadIdx.x < n) atomicAdd(&h[(k[r] >> shift) & 255u], 1u);
+        if (base + r * NT + threadIdx.x < n) atomicAdd(&mine[(k[r] >> shift) & 255u], 1u);
     __syncthreads();
-    if (threadIdx.x < 256u) hist[threadIdx.x * nb + b] = h[threadIdx.x];
+    if (threadIdx.x < 256u)
+        hist[threadIdx.x * nb + b] = (h[threadIdx.x] + h[kHistPitch + threadIdx.x]) +
+                                     (h[2u * kHistPitch + threadIdx.x] + h[3u * kHistPitch + threadIdx.x]);
 }
 
 // Stable scatter of one block's keys and values to the scanned offsets, by NT threads: rounds of
