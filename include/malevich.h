@@ -101,6 +101,17 @@ typedef enum mlv_depth_func {
     MLV_DEPTH_LESS_EQUAL = 1     /* extension: mirrored test and hierarchical-Z */
 } mlv_depth_func;
 
+typedef enum mlv_cull_mode {
+    MLV_CULL_BACK = 0,  /* reference: a triangle of signed area > 0 is culled (main.c:856) */
+    MLV_CULL_NONE = 1,  /* extension: both windings are drawn */
+    MLV_CULL_FRONT = 2  /* extension: a triangle of signed area < 0 is culled */
+} mlv_cull_mode;
+
+typedef enum mlv_fill_rule {
+    MLV_FILL_INCLUSIVE = 0, /* reference: a pixel corner on any edge is covered (main.c:1015-1028) */
+    MLV_FILL_TOP_LEFT = 1   /* extension, D3D: a corner on an edge is covered only by a top or left edge */
+} mlv_fill_rule;
+
 /* Viewport, field for field (main.c:85-92). */
 typedef struct mlv_viewport {
     float top_left_x;
@@ -178,6 +189,14 @@ MLV_API int mlv_vs_set_shader_resource(mlv_context *ctx, uint32_t slot, mlv_text
 
 /* ---- RS (main.c:94-96) */
 MLV_API int mlv_rs_set_viewport(mlv_context *ctx, const mlv_viewport *vp);
+/* Extension: the reference hard-codes back-face culling (main.c:856). Kept triangles of either
+ * winding are set up alike (edge functions oriented by the area's sign, main.c:563-575). */
+MLV_API int mlv_rs_set_cull(mlv_context *ctx, int cull_mode);
+/* Extension: the reference's coverage is inclusive and leaves the top-left rule a TODO
+ * (main.c:1027). Under MLV_FILL_TOP_LEFT an edge whose inward normal (a, b) has a > 0 (left) or
+ * a == 0, b > 0 (top; screen y grows downward) keeps the corners on it, any other edge drops
+ * them, so a mesh's shared edges cover each corner once. Depth and barycentrics are unchanged. */
+MLV_API int mlv_rs_set_fill_rule(mlv_context *ctx, int fill_rule);
 
 /* ---- PS (main.c:98-101) */
 MLV_API int mlv_ps_set_shader(mlv_context *ctx, int pixel_shader);

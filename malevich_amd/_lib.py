@@ -25,6 +25,8 @@ TOPOLOGY_UNDEFINED, TOPOLOGY_TRIANGLELIST = 0, 1
 VS_BASIC, VS_VERTEX_LIGHTING, VS_PASSTHROUGH, VS_FULLSCREEN = 1, 2, 3, 4
 PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH, PS_BASIC_LAMBERT, PS_ENV_TEXTURED = 1, 2, 3, 4, 5
 DEPTH_GREATER_EQUAL, DEPTH_LESS_EQUAL = 0, 1
+CULL_BACK, CULL_NONE, CULL_FRONT = 0, 1, 2
+FILL_INCLUSIVE, FILL_TOP_LEFT = 0, 1
 
 
 class Viewport(C.Structure):
@@ -135,6 +137,8 @@ SIGNATURES = {
     "mlv_vs_set_constant_buffer": (_I, [_P, _U32, _P]),
     "mlv_vs_set_shader_resource": (_I, [_P, _U32, _P]),
     "mlv_rs_set_viewport": (_I, [_P, C.POINTER(Viewport)]),
+    "mlv_rs_set_cull": (_I, [_P, _I]),
+    "mlv_rs_set_fill_rule": (_I, [_P, _I]),
     "mlv_ps_set_shader": (_I, [_P, _I]),
     "mlv_ps_set_shader_resource": (_I, [_P, _U32, _P]),
     "mlv_ps_set_constant_buffer": (_I, [_P, _U32, _P]),

@@ -469,14 +469,21 @@ __device__ __forceinline__ bool setup_triangle(const DrawRec &p, const ViewRec &
     }
     int32_t area = (int32_t)(((uint32_t)x[1] - (uint32_t)x[0]) * ((uint32_t)y[2] - (uint32_t)y[0]) -
                              ((uint32_t)x[2] - (uint32_t)x[0]) * ((uint32_t)y[1] - (uint32_t)y[0]));
-    if (area > 0) return false;  // main.c:856
+    // main.c:856 culls signed_area > 0; the other modes are extensions (set_edge orients the edge
+    // functions by the area's sign, so a kept triangle of either winding is set up alike)
+    if (p.cull == CULL_BACK ? area > 0 : (p.cull == CULL_FRONT ? area < 0 : false)) return false;
     int32_t ea, eb, ec;
+    uint32_t excl = 0u;  // top-left rule (extension): edges neither left (a > 0) nor top (a == 0, b > 0)
     set_edge(area, x[1], y[1], x[2], y[2], ea, eb, ec);  // slot 0 = (v1, v2)
     tv.a16[0] = (int32_t)((uint32_t)ea << 4); tv.b16[0] = (int32_t)((uint32_t)eb << 4); tv.c[0] = ec;
+    excl |= (ea > 0 || (ea == 0 && eb > 0)) ? 0u : 1u;
     set_edge(area, x[2], y[2], x[0], y[0], ea, eb, ec);  // slot 1 = (v2, v0)
     tv.a16[1] = (int32_t)((uint32_t)ea << 4); tv.b16[1] = (int32_t)((uint32_t)eb << 4); tv.c[1] = ec;
+    excl |= (ea > 0 || (ea == 0 && eb > 0)) ? 0u : 2u;
     set_edge(area, x[0], y[0], x[1], y[1], ea, eb, ec);  // slot 2 = (v0, v1)
     tv.a16[2] = (int32_t)((uint32_t)ea << 4); tv.b16[2] = (int32_t)((uint32_t)eb << 4); tv.c[2] = ec;
+    excl |= (ea > 0 || (ea == 0 && eb > 0)) ? 0u : 4u;
+    tv.draw = p.fill == FILL_TOP_LEFT ? excl << 16 : 0u;  // the draw index is added when written
     float af = (float)(area >> 8);
     if (af == 0.0f) af = 1.0f;
     tv.ooa = fabsf(1.f / af);

@@ -54,6 +54,8 @@ enum VsKind { VS_BASIC = 1, VS_VERTEX_LIGHTING = 2, VS_PASSTHROUGH = 3, VS_FULLS
 enum PsKind { PS_BASIC = 1, PS_ENV_LIGHTING = 2, PS_PASSTHROUGH = 3, PS_BASIC_LAMBERT = 4, PS_ENV_TEXTURED = 5 };
 enum TexFormat { FMT_RGBA8 = 1, FMT_RGBA32F = 2, FMT_RGBA16F = 3 };
 enum DepthFunc { DEPTH_GE = 0, DEPTH_LE = 1 };
+enum CullMode { CULL_BACK = 0, CULL_NONE = 1, CULL_FRONT = 2 };  // mlv_cull_mode
+enum FillRule { FILL_INCLUSIVE = 0, FILL_TOP_LEFT = 1 };         // mlv_fill_rule
 
 enum OverflowBits : uint32_t {
     OVF_TRIANGLES = 1u,   // more assembled triangles than the triangle arena holds
@@ -82,6 +84,7 @@ struct DrawRec {
     uint32_t stride, vertex_count, tri_count;
     int32_t base_vertex;
     int32_t vs, ps, depth_func, depth_cull;
+    int32_t cull, fill;           // mlv_cull_mode, mlv_fill_rule
     uint32_t view;                // index into the view table
     uint32_t vbase, tbase;        // first vertex slot / first input-triangle slot
     uint32_t blk_v0, blk_t0;      // first block of the vertex / setup stage
@@ -112,7 +115,7 @@ struct alignas(16) TriVis {
     float hiz_depth;         // max_depth (main.c:871), or the minimum under LESS_EQUAL
     float cull_bound;        // a bound no interpolated depth of this triangle can pass
     uint32_t rect0, rect1;   // 8x8 bin rect: bx0 | by0<<16, bx1 | by1<<16 (main.c:930-931)
-    uint32_t draw;           // index of the draw in the flush
+    uint32_t draw;           // index of the draw in the flush; bits 16-18: edge slots the top-left rule excludes
     // The pixel corners inside the snapped bounding box, clamped to the view: x in
     // [lat_x & 0xFFFF, lat_x >> 16], y likewise (as int16). Coverage is sampled only at pixel
     // corners (main.c:1015-1025), so a triangle with no corner in a bin covers nothing there.

@@ -749,7 +749,7 @@ __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t id
     if (idx >= p.tri_cap) return;
     // every load before the first store, which the compiler cannot move loads across
     const uint32_t tile_base = v.tile_base, tiles_x = (uint32_t)v.tiles_x;
-    tv.draw = draw;
+    tv.draw = (tv.draw & 0x70000u) | draw;  // keeps the top-left rule's edge bits
     p.tvis[idx] = tv;
     TriShade sh;
     sh.rw[0] = rw[0]; sh.rw[1] = rw[1]; sh.rw[2] = rw[2];
@@ -1307,6 +1307,7 @@ struct TileLdsN {
     float ooa[N], z0[N], dz1[N], dz2[N], hiz[N], cull[N];
     uint32_t lat_x[N], lat_y[N], id[N];
     uint16_t draw[N];
+    uint8_t excl[N];  // edge slots the top-left rule excludes (0 under the reference's rule)
     static_assert(kMaxBatchDraws <= 65536, "draw indices of a flush fit 16 bits");
 };
 using TileLds = TileLdsN<256>;
@@ -1328,7 +1329,8 @@ __device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t
     L.lat_x[k] = t.lat_x;
     L.lat_y[k] = t.lat_y;
     L.id[k] = id;
-    L.draw[k] = t.draw;
+    L.draw[k] = (uint16_t)t.draw;
+    L.excl[k] = (uint8_t)(t.draw >> 16);
 }
 
 // Entering a new draw: the bin's hierarchical-Z value becomes the tile minimum of its depths
@@ -1364,6 +1366,7 @@ struct CoverRec {
     int32_t a16[3], b16[3], c[3];
     float ooa, z0, dz1, dz2;
     int32_t id;
+    uint32_t excl;
 };
 
 template <class Lds>
@@ -1380,6 +1383,7 @@ __device__ __forceinline__ CoverRec load_cover(const Lds &L, uint32_t k) {
     r.dz1 = L.dz1[k];
     r.dz2 = L.dz2[k];
     r.id = (int32_t)L.id[k];
+    r.excl = L.excl[k];
     return r;
 }
 
@@ -1400,7 +1404,9 @@ __device__ __forceinline__ void bin_cover(const CoverRec &r, BinState &st, bool 
     const uint32_t e1 = mad24(r.a16[1], px, mad24(r.b16[1], py, (uint32_t)r.c[1]));
     const uint32_t e2 = mad24(r.a16[2], px, mad24(r.b16[2], py, (uint32_t)r.c[2]));
     const int32_t eo = (int32_t)(e0 | e1 | e2);
-    const bool cov = inside && eo > 0;  // main.c:1028
+    bool cov = inside && eo > 0;  // main.c:1028
+    if (r.excl)  // the top-left rule (extension): a corner on an excluded edge is not covered
+        cov = cov && !((r.excl & 1u) && e0 == 0u) && !((r.excl & 2u) && e1 == 0u) && !((r.excl & 4u) && e2 == 0u);
     // the covered lanes straight from the compares' lane masks (38: signed greater, 33: not
     // equal); a ballot of cov would first turn the mask into a 0/1 register and compare again
     if ((__builtin_amdgcn_sicmp(eo, 0, 38) & st.inside_m) == 0ull) return;
@@ -1531,7 +1537,7 @@ __device__ __forceinline__ void shade_winner(const BatchParams &p, const ViewRec
                                              int32_t px, int32_t py, size_t pix) {
     const TriVis T = p.tvis[best];
     const TriShade S = p.tshade[best];
-    const DrawRec &D = p.draws[T.draw];
+    const DrawRec &D = p.draws[T.draw & 0xFFFFu];
     float pbx, pby;
     persp_bary(T, S, px, py, pbx, pby);
     const float4 a0 = p.vattr[2u * S.row[0]], b0 = p.vattr[2u * S.row[0] + 1u];
@@ -1616,7 +1622,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
             int32_t bx0, by0, bx1, by1;
             rect_of(tv, bx0, by0, bx1, by1);
             uint32_t m = 0u;
-            if (!ovf || p.draws[tv.draw].view == vi) {
+            if (!ovf || p.draws[tv.draw & 0xFFFFu].view == vi) {
 #pragma unroll
                 for (int k = 0; k < (int)(TB * TB); ++k) {
                     int32_t gx = (int32_t)TB * tx + (k % (int)TB), gy = (int32_t)TB * ty + (k / (int)TB);

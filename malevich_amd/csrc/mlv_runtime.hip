@@ -146,6 +146,7 @@ struct mlv_context {
     mlv_render_target *rt = nullptr;
     mlv_depth_stencil *ds = nullptr;
     int depth_func = MLV_DEPTH_GREATER_EQUAL;
+    int cull = MLV_CULL_BACK, fill = MLV_FILL_INCLUSIVE;
     uint32_t *vis_tri = nullptr, *vis_draw = nullptr, *vis_prim = nullptr;
     // recorded draws
     std::vector<PendingDraw> pending;
@@ -1117,6 +1118,20 @@ MLV_API int mlv_om_set_render_targets(mlv_context *ctx, mlv_render_target *rt, m
     return MLV_OK;
 }
 
+MLV_API int mlv_rs_set_cull(mlv_context *ctx, int cull_mode) {
+    if (!ctx || (cull_mode != MLV_CULL_BACK && cull_mode != MLV_CULL_NONE && cull_mode != MLV_CULL_FRONT))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown cull mode");
+    ctx->cull = cull_mode;
+    return MLV_OK;
+}
+
+MLV_API int mlv_rs_set_fill_rule(mlv_context *ctx, int fill_rule) {
+    if (!ctx || (fill_rule != MLV_FILL_INCLUSIVE && fill_rule != MLV_FILL_TOP_LEFT))
+        return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown fill rule");
+    ctx->fill = fill_rule;
+    return MLV_OK;
+}
+
 MLV_API int mlv_om_set_depth_func(mlv_context *ctx, int depth_func) {
     if (!ctx || (depth_func != MLV_DEPTH_GREATER_EQUAL && depth_func != MLV_DEPTH_LESS_EQUAL))
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown depth func");
@@ -1312,6 +1327,8 @@ MLV_API int mlv_draw_indexed(mlv_context *ctx, uint32_t index_count, uint32_t st
     r.vs = ctx->vs;
     r.ps = ctx->ps;
     r.depth_func = ctx->depth_func;
+    r.cull = ctx->cull;
+    r.fill = ctx->fill;
     // the conservative in-draw depth cull and the per-bin edge screening rely on exact edge
     // values: |E| < 2^31 at every pixel of every bin
     // (bins reach up to 7 pixels past the right and bottom edges)

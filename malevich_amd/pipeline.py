@@ -281,6 +281,14 @@ class Device:
         vp = L.Viewport(top_left_x, top_left_y, width, height, min_depth, max_depth)
         _check(self.ctx, self.lib.mlv_rs_set_viewport(self.ctx, C.byref(vp)))
 
+    def rs_set_cull(self, mode: int) -> None:
+        """L.CULL_BACK (the reference's), L.CULL_NONE or L.CULL_FRONT."""
+        _check(self.ctx, self.lib.mlv_rs_set_cull(self.ctx, mode))
+
+    def rs_set_fill_rule(self, rule: int) -> None:
+        """L.FILL_INCLUSIVE (the reference's) or L.FILL_TOP_LEFT."""
+        _check(self.ctx, self.lib.mlv_rs_set_fill_rule(self.ctx, rule))
+
     # ---------------------------------------------------------------- PS
     def ps_set_shader(self, ps: int) -> None:
         _check(self.ctx, self.lib.mlv_ps_set_shader(self.ctx, ps))

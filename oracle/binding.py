@@ -18,6 +18,8 @@ VS_BASIC, VS_VERTEX_LIGHTING, VS_PASSTHROUGH, VS_FULLSCREEN = 0, 1, 2, 3
 PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH, PS_BASIC_LAMBERT, PS_ENV_TEXTURED = 0, 1, 2, 3, 4
 FMT_RGBA8, FMT_RGBA32F = 0, 1
 DEPTH_GE, DEPTH_LE = 0, 1
+CULL_BACK, CULL_NONE, CULL_FRONT = 0, 1, 2  # extension; CULL_BACK is the reference's (main.c:856)
+FILL_INCLUSIVE, FILL_TOP_LEFT = 0, 1  # extension; FILL_INCLUSIVE is the reference's (main.c:1015-1028)
 
 
 class Texture(C.Structure):
@@ -47,6 +49,8 @@ class DrawDesc(C.Structure):
         ("depth_func", C.c_int32),
         ("ps_srv1", Texture),
         ("ps_cb", C.c_void_p),
+        ("cull_mode", C.c_int32),
+        ("fill_rule", C.c_int32),
     ]
 
 
@@ -79,6 +83,7 @@ class Triangle(C.Structure):
         ("max_bounds", C.c_int32 * 2),
         ("prim", C.c_uint32),
         ("attr", (C.c_float * 12) * 3),
+        ("excl", C.c_uint32),
     ]
 
 
@@ -139,6 +144,8 @@ def lib() -> C.CDLL:
         L.orc_primitive_assembly.argtypes = [P(DrawDesc), P(Triangle), C.c_uint32, P(C.c_uint32), P(C.c_uint64)]
         L.orc_coverage_mask.argtypes = [P(C.c_int32), C.c_int32, C.c_int32]
         L.orc_coverage_mask.restype = C.c_uint64
+        L.orc_coverage_mask_excl.argtypes = [P(C.c_int32), C.c_uint32, C.c_int32, C.c_int32]
+        L.orc_coverage_mask_excl.restype = C.c_uint64
         L.orc_bilinear.argtypes = [P(Texture), C.c_float, C.c_float, P(C.c_float)]
         L.orc_nearest_level.argtypes = [C.c_float, C.c_uint32]
         L.orc_nearest_level.restype = C.c_uint32
@@ -234,7 +241,8 @@ class Draw:
     """Keeps the numpy inputs alive for the lifetime of the descriptor."""
 
     def __init__(self, indices, vertices, vs, ps, cb=None, vs_tex=None, ps_tex=None, viewport=None,
-                 start_index=0, base_vertex=0, index_count=None, depth_func=DEPTH_GE, ps_tex1=None, ps_cb=None):
+                 start_index=0, base_vertex=0, index_count=None, depth_func=DEPTH_GE, ps_tex1=None, ps_cb=None,
+                 cull_mode=0, fill_rule=0):
         self.indices = np.ascontiguousarray(indices, np.uint32).ravel()
         self.vertices = np.ascontiguousarray(vertices, np.float32)
         assert self.vertices.ndim == 2
@@ -250,7 +258,8 @@ class Draw:
             self.vertices.ctypes.data, self.vertices.shape[1] * 4, self.vertices.shape[0],
             vs, ps, self.cb.ctypes.data, make_texture(self.vs_tex_arr), make_texture(self.ps_tex_arr),
             Viewport(*(viewport if viewport is not None else (0, 0, 0, 0, 0, 1))), depth_func,
-            make_texture(self.ps_tex1_arr), None if self.ps_cb is None else self.ps_cb.ctypes.data)
+            make_texture(self.ps_tex1_arr), None if self.ps_cb is None else self.ps_cb.ctypes.data, cull_mode,
+            fill_rule)
 
 
 def draw_indexed(rt: RenderTarget, draw: Draw) -> None:
@@ -274,7 +283,7 @@ def primitive_assembly(draw: Draw, capacity: int | None = None) -> list:
 
 TRIANGLE_DTYPE = np.dtype([("edge", np.int32, (3, 3)), ("rw", np.float32, 3), ("one_over_area", np.float32),
                            ("max_depth", np.float32), ("min_bounds", np.int32, 2), ("max_bounds", np.int32, 2),
-                           ("prim", np.uint32), ("attr", np.float32, (3, 12))])
+                           ("prim", np.uint32), ("attr", np.float32, (3, 12)), ("excl", np.uint32)])
 assert TRIANGLE_DTYPE.itemsize == C.sizeof(Triangle)
 
 
@@ -309,9 +318,9 @@ def sh9(n) -> np.ndarray:
     return np.array(out[:], np.float64)
 
 
-def coverage_mask(edges, ox: int, oy: int) -> int:
+def coverage_mask(edges, ox: int, oy: int, excl: int = 0) -> int:
     e = (C.c_int32 * 9)(*np.asarray(edges, np.int64).astype(np.int32).ravel().tolist())
-    return int(lib().orc_coverage_mask(e, ox, oy))
+    return int(lib().orc_coverage_mask_excl(e, excl, ox, oy))
 
 
 def bilinear_level(tex, level: int, u: float, v: float) -> np.ndarray:

@@ -543,7 +543,9 @@ static int pa_range(const orc_draw_desc *d, const float planes[6][4], const floa
             /* main.c:851 */
             int32_t area = (int32_t)(((uint32_t)x[1] - (uint32_t)x[0]) * ((uint32_t)y[2] - (uint32_t)y[0]) -
                                      ((uint32_t)x[2] - (uint32_t)x[0]) * ((uint32_t)y[1] - (uint32_t)y[0]));
-            if (area > 0) continue; /* main.c:856 back-face cull */
+            /* main.c:856 back-face cull; the other modes are extensions (edge functions are
+             * oriented by area's sign, so a kept triangle of either winding is set up alike) */
+            if (d->cull_mode == ORC_CULL_BACK ? area > 0 : d->cull_mode == ORC_CULL_FRONT ? area < 0 : 0) continue;
             if (*n >= capacity) return ORC_ERR_CAPACITY;
             orc_triangle scratch_tr;
             orc_triangle *tr = out ? &out[*n] : &scratch_tr;
@@ -552,6 +554,15 @@ static int pa_range(const orc_draw_desc *d, const float planes[6][4], const floa
             set_edge(tr->edge[2], area, x[0], y[0], x[1], y[1]);
             set_edge(tr->edge[0], area, x[1], y[1], x[2], y[2]);
             set_edge(tr->edge[1], area, x[2], y[2], x[0], y[0]);
+            /* extension, D3D top-left rule: an edge whose inward normal (a, b) points right (a > 0:
+             * a left edge) or straight down (a == 0, b > 0: a top edge; screen y grows down) keeps
+             * its corners; any other edge excludes the corners that lie on it */
+            tr->excl = 0u;
+            if (d->fill_rule == ORC_FILL_TOP_LEFT)
+                for (int k = 0; k < 3; ++k) {
+                    const int32_t a = tr->edge[k][0], b = tr->edge[k][1];
+                    if (!(a > 0 || (a == 0 && b > 0))) tr->excl |= 1u << k;
+                }
             /* main.c:863-866 */
             float af = (float)(area >> (SUBPIX_BITS * 2));
             if (af == 0.0f) af = 1.0f;
@@ -685,15 +696,21 @@ static inline int32_t edge_at(const int32_t e[3], int32_t x, int32_t y) {
 }
 
 /* main.c:1012-1041: 8x8 corner-sampled coverage, bit row*8+col; covered = (E0|E1|E2) > 0. */
-uint64_t orc_coverage_mask(const int32_t e[3][3], int32_t ox, int32_t oy) {
+uint64_t orc_coverage_mask_excl(const int32_t e[3][3], uint32_t excl, int32_t ox, int32_t oy) {
     uint64_t m = 0;
     for (int r = 0; r < 8; ++r)
         for (int c = 0; c < 8; ++c) {
             int32_t x = ox + c, y = oy + r;
             int32_t a = edge_at(e[0], x, y), b = edge_at(e[1], x, y), g = edge_at(e[2], x, y);
-            if ((int32_t)((uint32_t)a | (uint32_t)b | (uint32_t)g) > 0) m |= 1ull << (r * 8 + c);
+            int cov = (int32_t)((uint32_t)a | (uint32_t)b | (uint32_t)g) > 0;
+            if (((excl & 1u) && a == 0) || ((excl & 2u) && b == 0) || ((excl & 4u) && g == 0)) cov = 0;
+            if (cov) m |= 1ull << (r * 8 + c);
         }
     return m;
+}
+
+uint64_t orc_coverage_mask(const int32_t e[3][3], int32_t ox, int32_t oy) {
+    return orc_coverage_mask_excl(e, 0u, ox, oy);
 }
 
 int orc_clear_render_target_view(orc_target *rt, const float rgba[4]) {
@@ -799,7 +816,7 @@ int orc_draw_indexed(orc_target *rt, const orc_draw_desc *d, orc_stats *st) {
                 float mn = MIN_(tr->attr[0][2], MIN_(tr->attr[1][2], tr->attr[2][2]));
                 if (mn > hz) continue;
             }
-            uint64_t mask = orc_coverage_mask(tr->edge, ox, oy);
+            uint64_t mask = orc_coverage_mask_excl(tr->edge, tr->excl, ox, oy);
             if (!mask) continue;
             for (int r = 0; r < 8; ++r) {
                 uint32_t m8 = (uint32_t)(mask >> (8 * r)) & 0xFFu;

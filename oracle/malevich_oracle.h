@@ -70,7 +70,11 @@ typedef struct orc_draw_desc {
     int32_t depth_func;
     orc_texture ps_srv1; /* extension shaders: slot 1 */
     const float *ps_cb;  /* extension shaders: 12 floats */
+    int32_t cull_mode;   /* extension: ORC_CULL_BACK (main.c:856), ORC_CULL_NONE or ORC_CULL_FRONT */
+    int32_t fill_rule;   /* extension: ORC_FILL_INCLUSIVE (main.c:1015-1028) or ORC_FILL_TOP_LEFT */
 } orc_draw_desc;
+enum { ORC_CULL_BACK = 0, ORC_CULL_NONE = 1, ORC_CULL_FRONT = 2 };
+enum { ORC_FILL_INCLUSIVE = 0, ORC_FILL_TOP_LEFT = 1 };
 
 typedef struct orc_target {
     uint32_t *colors;
@@ -96,6 +100,7 @@ typedef struct orc_triangle {
     int32_t min_bounds[2], max_bounds[2];
     uint32_t prim;        /* input triangle index within the draw */
     float attr[3][12];    /* 3 vertices x 3 registers x 4 floats; reg0 = screen position */
+    uint32_t excl;        /* extension, top-left rule: bit k set = edge slot k is neither top nor left */
 } orc_triangle;
 
 int orc_clear_render_target_view(orc_target *rt, const float rgba[4]);
@@ -107,6 +112,8 @@ int orc_vertex_shader(const orc_draw_desc *d, uint32_t vertex_index, float out12
 int orc_primitive_assembly(const orc_draw_desc *d, orc_triangle *out, uint32_t capacity,
                            uint32_t *out_count, uint64_t *oob_count);
 uint64_t orc_coverage_mask(const int32_t edge[3][3], int32_t origin_x, int32_t origin_y);
+/* extension: the same under the top-left rule (a corner on an edge of slot k in excl is not covered) */
+uint64_t orc_coverage_mask_excl(const int32_t edge[3][3], uint32_t excl, int32_t origin_x, int32_t origin_y);
 void orc_bilinear(const orc_texture *t, float u, float v, float out[4]);
 uint32_t orc_nearest_level(float rho2, uint32_t levels); /* extension: level k when rho2 >= 2^(2k-1) */
 void orc_bilinear_level(const orc_texture *t, uint32_t level, float u, float v, float out[4]);

@@ -28,6 +28,8 @@ class DrawSpec:
     index_count: int | None = None
     ps_tex1: np.ndarray | list | None = None  # extension shaders: texture slot 1
     ps_cb: np.ndarray | None = None           # extension shaders: 12 floats
+    cull: int = 0                             # extension: 0 back (the reference's), 1 none, 2 front
+    fill: int = 0                             # extension: 0 inclusive (the reference's), 1 top-left
     # a texture given as a list is a mip chain (level 0 first)
 
 
@@ -83,8 +85,12 @@ def render_gpu(dev: P.Device, W: int, H: int, draws: list[DrawSpec], clear_color
                 dev.ps_set_constant_buffer(0, pcb)
             else:
                 dev.ps_set_constant_buffer(0, None)
+            dev.rs_set_cull(d.cull)
+            dev.rs_set_fill_rule(d.fill)
             n = d.indices.size if d.index_count is None else d.index_count
             dev.draw_indexed(n, d.start_index, d.base_vertex)
+        dev.rs_set_cull(L.CULL_BACK)
+        dev.rs_set_fill_rule(L.FILL_INCLUSIVE)
         colors = rt.read()
         depth = ds.read()
         vt = vis_t.read(np.uint32).reshape(H, W)
@@ -110,7 +116,8 @@ def render_oracle(oracle, W: int, H: int, draws: list[DrawSpec], clear_color, cl
         od = oracle.Draw(d.indices, d.vertices, VS[d.vs][1], PS[d.ps][1], cb=d.cb, vs_tex=oracle_texture(oracle, d.vs_tex),
                          ps_tex=oracle_texture(oracle, d.ps_tex), viewport=(0, 0, W, H, 0, 1),
                          start_index=d.start_index, base_vertex=d.base_vertex, index_count=d.index_count,
-                         depth_func=depth_func, ps_tex1=oracle_texture(oracle, d.ps_tex1), ps_cb=d.ps_cb)
+                         depth_func=depth_func, ps_tex1=oracle_texture(oracle, d.ps_tex1), ps_cb=d.ps_cb,
+                         cull_mode=d.cull, fill_rule=d.fill)
         oracle.draw_indexed(rt, od)
     st = {k: getattr(rt.stats, k) for k, _ in oracle.Stats._fields_}
     return Frame(rt.colors.copy(), rt.depth.copy(), rt.vis_tri.copy(), rt.vis_draw.copy(), st)

@@ -1,0 +1,67 @@
+"""The extension raster modes on the GPU against the oracle: cull modes (back, the reference's;
+none; front) and the opt-in top-left fill rule.
+
+Bar: visibility, depth and every stat bit-exact, colour within 1/255; a rectangle tiled by
+triangles of both windings with every vertex on a pixel corner is covered exactly once per
+corner of its half-open extent under the top-left rule (the oracle's own pin, checked here on
+the GPU's visibility buffer).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from malevich_amd import _lib as L
+from malevich_amd import scenes as S
+from tests.render_utils import DrawSpec, assert_frames_match, render_gpu, render_oracle
+from tests.test_oracle_pins import passthrough_vertices
+
+pytestmark = pytest.mark.gpu
+
+N = 64
+
+
+def _tiled(seed, x0, y0, x1, y1, step):
+    rng = np.random.default_rng(seed)
+    verts = []
+    for y in range(y0, y1, step):
+        for x in range(x0, x1, step):
+            a, b, c, d = (x, y), (x + step, y), (x + step, y + step), (x, y + step)
+            halves = [(a, b, c), (a, c, d)] if rng.integers(2) else [(a, b, d), (b, c, d)]
+            for h in halves:
+                h = [(16 * p[0], 16 * p[1]) for p in h]
+                h = h if rng.integers(2) else [h[0], h[2], h[1]]
+                v = passthrough_vertices(h, z=tuple(rng.uniform(0.2, 0.8, 3)), size=N)
+                v[:, 4:7] = rng.uniform(0, 1, 3)
+                verts.append(v)
+    v = np.concatenate(verts)
+    return v, np.arange(len(v), dtype=np.uint32)
+
+
+@pytest.mark.parametrize("cull,fill", [(L.CULL_NONE, L.FILL_TOP_LEFT), (L.CULL_NONE, L.FILL_INCLUSIVE),
+                                       (L.CULL_FRONT, L.FILL_TOP_LEFT), (L.CULL_BACK, L.FILL_TOP_LEFT)])
+def test_tiled_rectangle(device, oracle, cull, fill):
+    x0, y0, x1, y1 = 5, 3, 57, 59  # multiples of the step apart
+    v, idx = _tiled(21, x0, y0, x1, y1, 4)
+    d = DrawSpec(idx, v, "passthrough", "passthrough", cull=cull, fill=fill)
+    print(f"case: tiled rectangle cull={cull} fill={fill}, {len(idx) // 3} triangles", flush=True)
+    g = render_gpu(device, N, N, [d], (0, 0, 0, 0))
+    o = render_oracle(oracle, N, N, [d], (0, 0, 0, 0))
+    assert_frames_match(g, o, color_tol=0)
+    if cull == L.CULL_NONE:
+        yy, xx = np.mgrid[0:N, 0:N]
+        want = ((xx >= x0) & (xx < x1) & (yy >= y0) & (yy < y1)) if fill == L.FILL_TOP_LEFT else \
+               ((xx >= x0) & (xx <= x1) & (yy >= y0) & (yy <= y1))
+        np.testing.assert_array_equal(g.vis_tri != 0xFFFFFFFF, want)
+
+
+@pytest.mark.parametrize("cull,fill", [(L.CULL_FRONT, L.FILL_INCLUSIVE), (L.CULL_NONE, L.FILL_TOP_LEFT)])
+def test_sphere_modes(device, oracle, cull, fill):
+    W, H = 160, 96
+    mesh, cb, tex = S.c1_scene(W, H)
+    d = DrawSpec(mesh.indices, mesh.vertices, "basic", "basic", cb=cb, ps_tex=tex, cull=cull, fill=fill)
+    print(f"case: sphere cull={cull} fill={fill}", flush=True)
+    g = render_gpu(device, W, H, [d], S.CLEAR_COLOR)
+    o = render_oracle(oracle, W, H, [d], S.CLEAR_COLOR)
+    assert (o.vis_tri != 0xFFFFFFFF).sum() > 1000
+    assert_frames_match(g, o, color_tol=1)
