@@ -761,7 +761,9 @@ __device__ __forceinline__ void write_triangle(const BatchParams &p, uint32_t id
 }
 
 // The bins of an assembled triangle that is not binned: counted for the stats and marked in
-// the (draw, bin) bitmap. Every lane of the wave must be active.
+// the (draw, bin) bitmap, the lanes' bits merged into one atomic per distinct word of the wave
+// (one atomic per lane and bin measured slower: 256 views of 64 meshes, forward raster 86 ms
+// against more than 147). Every lane of the wave must be active.
 __device__ __forceinline__ uint32_t mark_unbinned(const BatchParams &p, const DrawRec &d, const ViewRec &v,
                                                   bool unbinned, const TriVis &tv) {
     int32_t bx0 = 0, by0 = 0, bx1 = -1, by1 = -1;
