@@ -395,10 +395,14 @@ def bench_c5(dev, args):
 
     dev.set_tuning(**DEFAULT_TUNING)
     warm(dev, lambda: frame(False), 1)
-    dev.autotune_begin()
-    frame(False)
-    tuning = dev.autotune_end(repeats=1)
-    print(f"[bench] c5 autotuned: {tuning}", file=sys.stderr, flush=True)
+    if args.tuning:
+        dev.set_tuning(**dict(DEFAULT_TUNING, **json.loads(args.tuning)))
+        tuning = dev.tuning()
+    else:
+        dev.autotune_begin()
+        frame(False)
+        tuning = dev.autotune_end(repeats=1)
+    print(f"[bench] c5 {'set' if args.tuning else 'autotuned'}: {tuning}", file=sys.stderr, flush=True)
     ms_b = timed(False, args.c5_steps)
     if args.no_per_draw:  # profiling runs of the batched kernels
         ms_p = float("nan")
@@ -642,7 +646,8 @@ def main(argv=None):
     ap.add_argument("--no-footprint", action="store_true",
                     help="skip the footprint-counting runs (for counter runs; the rooflines then read zero)")
     ap.add_argument("--tuning", type=str, default="",
-                    help="JSON launch parameters for the 1080p leg instead of autotuning them (for counter runs)")
+                    help="JSON launch parameters for the 1080p and batched-views legs instead of autotuning "
+                         "them (for counter runs)")
     ap.add_argument("--copy-calibration-mib", type=int, default=0,
                     help="also run one plain device-to-device copy of this many MiB (a known-size kernel "
                          "against which HBM byte counters are scaled)")
