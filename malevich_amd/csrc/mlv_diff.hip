@@ -464,11 +464,40 @@ __device__ __forceinline__ void atomic_add(double *dst, double v) {
     if (v != 0.0) unsafeAtomicAdd(dst, v);
 }
 
-__device__ __forceinline__ double wave_sum_d(double v) {
-#pragma unroll
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
+// Sum over the 64 lanes of the wave (every lane active), returned in every lane: DPP steps within
+// quads, half-rows and rows, then the row broadcasts leave the total in lane 63.
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ int dpp_move(int v) {
+    return __builtin_amdgcn_update_dpp(0, v, Ctrl, RowMask, 0xF, false);
 }
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ void dpp_add(float &v) {
+    v += __int_as_float(dpp_move<Ctrl, RowMask>(__float_as_int(v)));
+}
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ void dpp_add(double &v) {
+    const long long b = __double_as_longlong(v);
+    const int lo = dpp_move<Ctrl, RowMask>((int)(b & 0xFFFFFFFFll)), hi = dpp_move<Ctrl, RowMask>((int)(b >> 32));
+    v += __longlong_as_double((long long)(((unsigned long long)(uint32_t)hi << 32) | (uint32_t)lo));
+}
+template <class T>
+__device__ __forceinline__ T wave_total(T v) {
+    dpp_add<0xB1, 0xF>(v);   // quad_perm [1,0,3,2]
+    dpp_add<0x4E, 0xF>(v);   // quad_perm [2,3,0,1]
+    dpp_add<0x141, 0xF>(v);  // row_half_mirror
+    dpp_add<0x140, 0xF>(v);  // row_mirror
+    dpp_add<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    dpp_add<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
+    if constexpr (sizeof(T) == 8) {
+        const long long b = __double_as_longlong(v);
+        const uint32_t lo = (uint32_t)__builtin_amdgcn_readlane((int)(b & 0xFFFFFFFFll), 63);
+        const uint32_t hi = (uint32_t)__builtin_amdgcn_readlane((int)(b >> 32), 63);
+        return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+    } else {
+        return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), 63));
+    }
+}
+
 
 // Block sum of the 27 SH gradients (dE[j] * Y[k] of the live pixels, Y the basis at the unit
 // normal nh, evaluated again here rather than kept through the adjoint) into
@@ -481,7 +510,7 @@ __device__ __forceinline__ void sh_block_sum(bool live, const double dE[3], cons
     sh_basis(nh[0], nh[1], nh[2], Y);
 #pragma unroll
     for (int k = 0; k < 27; ++k) {
-        const double v = wave_sum_d(live ? dE[k % 3] * Y[k / 3] : 0.0);
+        const double v = wave_total(live ? dE[k % 3] * Y[k / 3] : 0.0);
         if (lane == 0u) s_w[w][k] = v;
     }
     __syncthreads();
@@ -522,15 +551,28 @@ __device__ __forceinline__ uint32_t slot_hash(unsigned long long k) {
     return (uint32_t)((k * 0x9E3779B97F4A7C15ull) >> 40);
 }
 
+// Slot of key k in a table of N slots: its hash slot, or one of the next kProbe - 1 (linear
+// probing); -1 when none is free or holds k. first_old is what the compare-and-swap at the hash
+// slot returned (gather_pixel issues the first probes of all its keys together).
 template <uint32_t N>
-__device__ __forceinline__ int lds_slot(unsigned long long *keys, unsigned long long k) {
+__device__ __forceinline__ int lds_slot_after(unsigned long long *keys, unsigned long long k,
+                                              unsigned long long first_old) {
     uint32_t s = slot_hash(k) & (N - 1u);
-    for (uint32_t i = 0; i < kProbe; ++i) {
+    if (first_old == kEmptyKey || first_old == k) return (int)s;
+    for (uint32_t i = 1; i < kProbe; ++i) {
+        s = (s + 1u) & (N - 1u);
         const unsigned long long old = atomicCAS(&keys[s], kEmptyKey, k);
         if (old == kEmptyKey || old == k) return (int)s;
-        s = (s + 1u) & (N - 1u);
     }
     return -1;
+}
+template <uint32_t N>
+__device__ __forceinline__ unsigned long long first_probe(unsigned long long *keys, unsigned long long k) {
+    return atomicCAS(&keys[slot_hash(k) & (N - 1u)], kEmptyKey, k);
+}
+template <uint32_t N>
+__device__ __forceinline__ int lds_slot(unsigned long long *keys, unsigned long long k) {
+    return lds_slot_after<N>(keys, k, first_probe<N>(keys, k));
 }
 
 __device__ __forceinline__ void tables_clear(VtxTable &vt, TexTable &tt) {
@@ -558,28 +600,90 @@ __device__ __forceinline__ void vertex_to_global(const GradSink &g, uint32_t v, 
     }
 }
 
-// One pixel's contributions into the block's tables (mesh index m in the keys' high word).
-__device__ __forceinline__ void gather_pixel(VtxTable &vt, TexTable &tt, const GradSink &g, uint32_t m,
-                                             const Eval<double> &e, const Adj &a) {
+// One vertex's contribution into the block's table slot s (from lds_slot on the key of mesh m,
+// vertex v), or into global memory when the table had no slot for it (s < 0).
+__device__ __forceinline__ void vertex_to_slot(VtxTable &vt, const GradSink &g, int s, uint32_t v,
+                                               const double dpos[3], const float att[8]) {
+    if (s < 0) {
+        vertex_to_global(g, v, dpos, att);
+        return;
+    }
+    for (int j = 0; j < 3; ++j)
+        if (dpos[j] != 0.0) unsafeAtomicAdd(&vt.pos[j][s], dpos[j]);
+    for (int j = 0; j < 8; ++j)
+        if (att[j] != 0.0f) unsafeAtomicAdd(&vt.att[j][s], att[j]);
+}
+
+// Lanes whose pixels show the same triangle of the same mesh (on large triangles, most of a
+// wave) add their vertex contributions up across the wave first, so that the triangle's three
+// vertices take one set of table atomics instead of one per lane: atomics on one LDS address
+// serialise. Up to kMaxGroups triangles of the wave, each shown by at least kMinGroup lanes, are
+// taken this way; returns whether this lane's vertex contributions were. Every lane of the wave
+// must call it.
+constexpr uint32_t kMinGroup = 12;
+constexpr int kMaxGroups = 4;
+
+__device__ __forceinline__ bool group_vertices(VtxTable &vt, const GradSink &g, bool live, uint32_t m,
+                                               const Eval<double> &e, const Adj &a) {
+    uint64_t pend = __ballot(live);
+    const uint32_t lane = threadIdx.x & 63u;
+    bool done = false;
+    for (int it = 0; it < kMaxGroups && pend; ++it) {
+        const int lead = (int)__builtin_ctzll(pend);
+        const uint32_t km = (uint32_t)__builtin_amdgcn_readlane((int)m, lead);
+        const uint32_t k0 = (uint32_t)__builtin_amdgcn_readlane((int)e.vi[0], lead);
+        const uint32_t k1 = (uint32_t)__builtin_amdgcn_readlane((int)e.vi[1], lead);
+        const uint32_t k2 = (uint32_t)__builtin_amdgcn_readlane((int)e.vi[2], lead);
+        const bool mine = live && m == km && e.vi[0] == k0 && e.vi[1] == k1 && e.vi[2] == k2;
+        const uint64_t grp = __ballot(mine);
+        pend &= ~grp;
+        if ((uint32_t)__popcll(grp) < kMinGroup) continue;
+        done = done || mine;
+        const uint32_t kv[3] = {k0, k1, k2};
 #pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        const uint32_t v = e.vi[k];
-        const float att[8] = {a.dcol[k][0], a.dcol[k][1], a.dcol[k][2], a.dn[k][0],
-                              a.dn[k][1],   a.dn[k][2],   a.duv[k][0], a.duv[k][1]};
-        const int s = lds_slot<kVtxSlots>(vt.key, ((unsigned long long)m << 32) | v);
-        if (s < 0) {
-            vertex_to_global(g, v, a.dpos[k], att);
-            continue;
+        for (int k = 0; k < 3; ++k) {
+            double dpos[3];
+            float att[8];
+#pragma unroll
+            for (int j = 0; j < 3; ++j) dpos[j] = wave_total(mine ? a.dpos[k][j] : 0.0);
+#pragma unroll
+            for (int j = 0; j < 3; ++j) {
+                att[j] = wave_total(mine ? a.dcol[k][j] : 0.0f);
+                att[3 + j] = wave_total(mine ? a.dn[k][j] : 0.0f);
+            }
+            att[6] = wave_total(mine ? a.duv[k][0] : 0.0f);
+            att[7] = wave_total(mine ? a.duv[k][1] : 0.0f);
+            if (lane == (uint32_t)lead)
+                vertex_to_slot(vt, g, lds_slot<kVtxSlots>(vt.key, ((unsigned long long)km << 32) | kv[k]), kv[k],
+                               dpos, att);
         }
-        for (int j = 0; j < 3; ++j)
-            if (a.dpos[k][j] != 0.0) unsafeAtomicAdd(&vt.pos[j][s], a.dpos[k][j]);
-        for (int j = 0; j < 8; ++j)
-            if (att[j] != 0.0f) unsafeAtomicAdd(&vt.att[j][s], att[j]);
+    }
+    return done;
+}
+
+// One pixel's contributions into the block's tables (mesh index m in the keys' high word); the
+// vertex contributions only when group_vertices did not take them.
+__device__ __forceinline__ void gather_pixel(VtxTable &vt, TexTable &tt, const GradSink &g, uint32_t m,
+                                             const Eval<double> &e, const Adj &a, bool vertices_done) {
+    // the first probes of the seven keys go out together, so their round trips overlap
+    const unsigned long long hi = (unsigned long long)m << 32;
+    unsigned long long ov[3], ot[4];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) ov[k] = vertices_done ? 0ull : first_probe<kVtxSlots>(vt.key, hi | e.vi[k]);
+#pragma unroll
+    for (int t = 0; t < 4; ++t) ot[t] = g.tex ? first_probe<kTexSlots>(tt.key, hi | a.tap[t]) : 0ull;
+    if (!vertices_done) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const float att[8] = {a.dcol[k][0], a.dcol[k][1], a.dcol[k][2], a.dn[k][0],
+                                  a.dn[k][1],   a.dn[k][2],   a.duv[k][0], a.duv[k][1]};
+            vertex_to_slot(vt, g, lds_slot_after<kVtxSlots>(vt.key, hi | e.vi[k], ov[k]), e.vi[k], a.dpos[k], att);
+        }
     }
     if (!g.tex) return;
 #pragma unroll
     for (int t = 0; t < 4; ++t) {
-        const int s = lds_slot<kTexSlots>(tt.key, ((unsigned long long)m << 32) | a.tap[t]);
+        const int s = lds_slot_after<kTexSlots>(tt.key, hi | a.tap[t], ot[t]);
         for (int j = 0; j < 3; ++j) {
             if (a.dtex[t][j] == 0.0f) continue;
             if (s < 0) unsafeAtomicAdd(&g.tex[4u * a.tap[t] + j], a.dtex[t][j]);
@@ -636,7 +740,8 @@ __global__ __launch_bounds__(256) void kd_backward_atomic(DiffParams p, const fl
         if (live) adjoint(p.m, p.v, p.sh, e, gout[i], a);
     }
     sh_block_sum(live, a.dE, e.nh, sh_part);  // first: dE and nh hold no registers past it
-    if (live) gather_pixel(vt, tt, sink, 0u, e, a);
+    const bool grouped = group_vertices(vt, sink, live, 0u, e, a);
+    if (live) gather_pixel(vt, tt, sink, 0u, e, a, grouped);
     __syncthreads();
     tables_flush(vt, tt, [&](uint32_t) { return sink; });
 }
@@ -880,7 +985,9 @@ __global__ __launch_bounds__(256) void kd_batch_backward(DiffBatchParams P, doub
         }
     }
     sh_block_sum(live, a.dE, e.nh, sh_part);  // first: dE and nh hold no registers past it
-    if (live) gather_pixel(vt, tt, mesh_sink(P.meshes[mi]), mi, e, a);
+    const GradSink sink = live ? mesh_sink(P.meshes[mi]) : GradSink{};
+    const bool grouped = group_vertices(vt, sink, live, mi, e, a);
+    if (live) gather_pixel(vt, tt, sink, mi, e, a, grouped);
     __syncthreads();
     tables_flush(vt, tt, [&](uint32_t m) { return mesh_sink(P.meshes[m]); });
 }

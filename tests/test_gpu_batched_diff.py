@@ -22,8 +22,11 @@ pytestmark = pytest.mark.gpu
 GROUPS = ("positions", "colors", "uvs", "normals", "texture")
 
 
-@pytest.fixture(scope="module")
-def scene():
+@pytest.fixture(scope="module", params=["dense", "coarse"])
+def scene(request):
+    if request.param == "coarse":  # triangles of tens to hundreds of pixels (the wave-sum path)
+        return S.batched_scene(n_views=3, n_meshes=8, min_tris=20, max_tris=60, resolutions=(192, 256),
+                               tex_sizes=(32,), seed=12)
     return S.batched_scene(n_views=5, n_meshes=4, min_tris=300, max_tris=5000, resolutions=(64, 96, 128),
                            tex_sizes=(32, 64), seed=11)
 

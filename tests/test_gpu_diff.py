@@ -90,6 +90,13 @@ def test_small_scene_atomic(device, oracle, small_scene):
     _check(device, oracle, small_scene, deterministic=False)
 
 
+def test_coarse_mesh_atomic(device, oracle):
+    """96 triangles of about 150 pixels each: most waves see one or two triangles, so the vertex
+    gradients take the wave-sum path (lanes of one triangle added across the wave first)."""
+    sc = S.c4_scene(width=128, height=128, rings=6, segments=8, tex_size=64, seed=7)
+    _check(device, oracle, sc, deterministic=False)
+
+
 def test_small_scene_deterministic(device, oracle, small_scene):
     g1 = _check(device, oracle, small_scene, deterministic=True)
     rng = np.random.default_rng(1)
