@@ -521,11 +521,16 @@ __device__ __forceinline__ bool setup_triangle(const DrawRec &p, const ViewRec &
     return true;
 }
 
+// The 8x8-bin rect packed in TriVis::rect0/rect1 (empty when bx1 < bx0 or by1 < by0).
+__device__ __forceinline__ void rect_decode(uint32_t rect0, uint32_t rect1, int32_t &bx0, int32_t &by0, int32_t &bx1,
+                                            int32_t &by1) {
+    bx0 = (int32_t)(rect0 & 0xFFFFu);
+    by0 = (int32_t)(rect0 >> 16);
+    bx1 = (int32_t)(int16_t)(rect1 & 0xFFFFu);
+    by1 = (int32_t)(int16_t)(rect1 >> 16);
+}
 __device__ __forceinline__ void rect_of(const TriVis &tv, int32_t &bx0, int32_t &by0, int32_t &bx1, int32_t &by1) {
-    bx0 = (int32_t)(tv.rect0 & 0xFFFFu);
-    by0 = (int32_t)(tv.rect0 >> 16);
-    bx1 = (int32_t)(int16_t)(tv.rect1 & 0xFFFFu);
-    by1 = (int32_t)(int16_t)(tv.rect1 >> 16);
+    rect_decode(tv.rect0, tv.rect1, bx0, by0, bx1, by1);
 }
 
 // Whether the snapped bounding box of a set-up triangle holds a pixel corner of the view. Coverage

@@ -48,6 +48,7 @@ constexpr uint32_t kTileStageDefault = 256;
 constexpr uint32_t kLdsPerCu = 160u * 1024u;   // gfx950
 constexpr uint32_t kLdsPerBlockMax = 64u * 1024u;
 
+constexpr uint32_t kFirstOfDraw = 0x80000000u;  // blk_draw flag: the block is its draw's first
 enum VsKind { VS_BASIC = 1, VS_VERTEX_LIGHTING = 2, VS_PASSTHROUGH = 3, VS_FULLSCREEN = 4 };
 // PS_BASIC_LAMBERT and PS_ENV_TEXTURED are extensions (mip-mapped fetch, a light, a second
 // texture): they read the registers at the next pixel corners too (footprints for the mip level).
@@ -178,6 +179,7 @@ struct BatchParams {
     uint32_t *cnt;
     unsigned long long *blocksum, *blockoff;  // per setup block: assembled | clipverts << 32
     uint32_t *asm_base;                   // per draw: its first binned triangle's index
+    uint32_t *blk_draw;                   // per setup block: its draw | kFirstOfDraw (kb_setup_count writes it)
     uint32_t *cmark;                      // (draw, bin) bits of unbinned triangles (NULL: all are binned)
     uint32_t cmark_words;
     uint2 *clip_list;       // (triangle slot, draw) of each triangle the clipper takes

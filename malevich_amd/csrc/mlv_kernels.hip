@@ -607,6 +607,67 @@ __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
 // positions) are paid once for two triangles.
 constexpr uint32_t kCountPer = 2;
 
+// The bins of the workgroup's assembled triangles that are not binned (rect of triangle h of this
+// lane in draw d[h] when unb[h]): counted for the stats and marked in the (draw, bin) bitmap. The
+// workgroup merges its bits in LDS first, in a window of kMarkWindow words from its lowest marked
+// word (bits beyond it, rare for consecutive triangles of one mesh, go to memory directly), so
+// that each word it touches takes one atomic: neighbouring triangles meet the same few words, and
+// atomics on one word serialise (one atomic per wave and word instead, on 32 views of 64 meshes:
+// 2.7 ms per setup launch, against 1.18 ms with the merge and 0.96 ms without any marks; one
+// atomic per lane and bin: 11.5 ms). Every thread of the 256-thread workgroup must call it.
+constexpr uint32_t kMarkWindow = 256;
+static_assert(kMarkWindow == (uint32_t)kBlock, "one window word per thread of a setup workgroup");
+
+__device__ __forceinline__ void mark_unbinned(const BatchParams &p, const uint32_t d[kCountPer],
+                                              const bool unb[kCountPer], const uint32_t r0[kCountPer],
+                                              const uint32_t r1[kCountPer]) {
+    __shared__ uint32_t s_win[kMarkWindow];
+    __shared__ uint32_t s_lo[4], s_p8[4];
+    uint32_t lo_t = 0xFFFFFFFFu, nb_t = 0u, cbase[kCountPer], bins_x[kCountPer];
+#pragma unroll
+    for (uint32_t h = 0; h < kCountPer; ++h) {
+        const DrawRec &D = p.draws[d[h]];
+        cbase[h] = D.cbase;
+        bins_x[h] = (uint32_t)p.views[D.view].bins_x;
+        if (unb[h]) {
+            int32_t bx0, by0, bx1, by1;
+            rect_decode(r0[h], r1[h], bx0, by0, bx1, by1);
+            if (bx1 >= bx0 && by1 >= by0) {
+                nb_t += (uint32_t)(bx1 - bx0 + 1) * (uint32_t)(by1 - by0 + 1);
+                lo_t = min(lo_t, (cbase[h] + (uint32_t)by0 * bins_x[h] + (uint32_t)bx0) >> 5);
+            }
+        }
+    }
+    const uint32_t lo_w = wave_min(lo_t);
+    const uint32_t n_w = wave_sum(nb_t);
+    s_win[threadIdx.x] = 0u;
+    if ((threadIdx.x & 63u) == 0u) {
+        s_lo[threadIdx.x >> 6] = lo_w;
+        s_p8[threadIdx.x >> 6] = n_w;
+    }
+    __syncthreads();
+    const uint32_t lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
+    if (lo == 0xFFFFFFFFu) return;  // nothing to mark in this workgroup (uniform)
+#pragma unroll
+    for (uint32_t h = 0; h < kCountPer; ++h) {
+        if (!unb[h]) continue;
+        int32_t bx0, by0, bx1, by1;
+        rect_decode(r0[h], r1[h], bx0, by0, bx1, by1);
+        for (int32_t by = by0; by <= by1; ++by)
+            for (int32_t bx = bx0; bx <= bx1; ++bx) {
+                const uint32_t bit = cbase[h] + (uint32_t)by * bins_x[h] + (uint32_t)bx;
+                const uint32_t w = (bit >> 5) - lo;
+                if (w < kMarkWindow) atomicOr(&s_win[w], 1u << (bit & 31u));
+                else atomicOr(p.cmark + (bit >> 5), 1u << (bit & 31u));
+            }
+    }
+    __syncthreads();
+    const uint32_t m = s_win[threadIdx.x];
+    if (m) atomicOr(p.cmark + lo + threadIdx.x, m);
+    if (threadIdx.x == 0u)
+        atomicAdd(&p.stats->pairs8_part[blockIdx.x & 63u], (unsigned long long)(s_p8[0] + s_p8[1] + s_p8[2] + s_p8[3]));
+}
+
 __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
     __shared__ uint32_t s_w[kCountPer][4];
     const uint32_t hb = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -655,18 +716,26 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
     }
     uint32_t packed[kCountPer];  // binned | assembled << 16
     int cls[kCountPer];
+    bool unb[kCountPer];         // assembled without the clipper and not binned: its bins are marked here
+    uint32_t r0[kCountPer], r1[kCountPer];
 #pragma unroll
     for (uint32_t h = 0; h < kCountPer; ++h) {
         packed[h] = 0u;
         cls[h] = 0;
+        unb[h] = false;
+        r0[h] = r1[h] = 0u;
         if (ok[h]) {
             const DrawRec &d = p.draws[di[h]];
             cls[h] = classify(q[h][0], q[h][1], q[h][2]);
             if (cls[h] == 1) {
                 TriVis tv;
                 float rw[3];
-                if (setup_triangle(d, p.views[d.view], q[h][0], q[h][1], q[h][2], tv, rw))
-                    packed[h] = (!p.cmark || has_corner(tv)) ? 0x10001u : 0x10000u;
+                if (setup_triangle(d, p.views[d.view], q[h][0], q[h][1], q[h][2], tv, rw)) {
+                    unb[h] = p.cmark && !has_corner(tv);
+                    packed[h] = unb[h] ? 0x10000u : 0x10001u;
+                    r0[h] = tv.rect0;
+                    r1[h] = tv.rect1;
+                }
             }
         }
     }
@@ -686,7 +755,10 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
         const uint32_t sum = s_w[threadIdx.x][0] + s_w[threadIdx.x][1] + s_w[threadIdx.x][2] + s_w[threadIdx.x][3];
         p.blocksum[hb * kCountPer + threadIdx.x] =
             (unsigned long long)(sum & 0xFFFFu) | ((unsigned long long)(sum >> 16) << 32);
+        const uint32_t bk = hb * kCountPer + threadIdx.x, dk = di[threadIdx.x];
+        p.blk_draw[bk] = dk | (p.draws[dk].blk_t0 == bk ? kFirstOfDraw : 0u);
     }
+    if (p.cmark) mark_unbinned(p, di, unb, r0, r1);
 }
 
 // Triangles that need the clipper, one per 16-lane group: the polygon's fan triangles that
@@ -753,55 +825,18 @@ __device__ __forceinline__ void add_unbinned_pairs(const BatchParams &p, uint32_
     }
 }
 
-// The bins of the block's assembled triangles that are not binned: counted for the stats and
-// marked in the (draw, bin) bitmap. The block merges its bits in LDS first, in a window of
-// kMarkWindow words from its lowest marked word (bits beyond it, rare for a block of consecutive
-// triangles of one mesh, go to memory directly), so that each word the block touches takes one
-// atomic: neighbouring triangles meet the same few words, and atomics on one word serialise (one
-// atomic per wave and word instead, on 32 views of 64 meshes: 2.7 ms per setup launch, against
-// 1.18 ms with the merge and 0.96 ms without any marks; one atomic per lane and bin: 11.5 ms).
-// Every thread of the 256-thread block must call it.
-constexpr uint32_t kMarkWindow = 256;
-static_assert(kMarkWindow == (uint32_t)kBlock, "one window word per thread of a setup block");
-
-__device__ __forceinline__ void mark_unbinned(const BatchParams &p, const DrawRec &d, const ViewRec &v, bool unbinned,
-                                              const TriVis &tv) {
-    __shared__ uint32_t s_win[kMarkWindow];
-    __shared__ uint32_t s_lo[4], s_p8[4];
-    int32_t bx0 = 0, by0 = 0, bx1 = -1, by1 = -1;
-    if (unbinned) rect_of(tv, bx0, by0, bx1, by1);
-    const uint32_t rw = bx1 >= bx0 ? (uint32_t)(bx1 - bx0 + 1) : 0u;
-    const uint32_t nb = by1 >= by0 ? rw * (uint32_t)(by1 - by0 + 1) : 0u;
-    const uint32_t first = d.cbase + (uint32_t)by0 * (uint32_t)v.bins_x + (uint32_t)bx0;
-    const uint32_t lo_w = wave_min(nb ? first >> 5 : 0xFFFFFFFFu);
-    const uint32_t n_w = wave_sum(nb);
-    s_win[threadIdx.x] = 0u;
-    if ((threadIdx.x & 63u) == 0u) {
-        s_lo[threadIdx.x >> 6] = lo_w;
-        s_p8[threadIdx.x >> 6] = n_w;
-    }
-    __syncthreads();
-    const uint32_t lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
-    if (lo == 0xFFFFFFFFu) return;  // nothing to mark in this block (uniform)
-    for (int32_t by = by0; by <= by1; ++by)
-        for (int32_t bx = bx0; bx <= bx1; ++bx) {
-            const uint32_t bit = d.cbase + (uint32_t)by * (uint32_t)v.bins_x + (uint32_t)bx;
-            const uint32_t w = (bit >> 5) - lo;
-            if (w < kMarkWindow) atomicOr(&s_win[w], 1u << (bit & 31u));
-            else atomicOr(p.cmark + (bit >> 5), 1u << (bit & 31u));
-        }
-    __syncthreads();
-    const uint32_t m = s_win[threadIdx.x];
-    if (m) atomicOr(p.cmark + lo + threadIdx.x, m);
-    if (threadIdx.x == 0u)
-        atomicAdd(&p.stats->pairs8_part[blockIdx.x & 63u], (unsigned long long)(s_p8[0] + s_p8[1] + s_p8[2] + s_p8[3]));
-}
-
 __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     __shared__ unsigned long long s_w[5];
     const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t slot = b * 256u + threadIdx.x;
-    const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_t0; });
+    const uint32_t bd = p.blk_draw[b];  // the count pass found the block's draw
+    const uint32_t blk_binned = (uint32_t)p.blocksum[b];  // binned triangles of the block (clipped ones too)
+    // a block without binned triangles writes nothing: no records, no clipper rows (a clipped
+    // polygon takes rows only when one of its fan triangles is binned) and no clipper offsets
+    // (kb_clip_emit reads them only for binned fan triangles); only a draw's first block and the
+    // last block have per-draw and per-flush values to write
+    if (blk_binned == 0u && !(bd & kFirstOfDraw) && b != gridDim.x - 1u) return;
+    const uint32_t di = bd & ~kFirstOfDraw;
     const DrawRec &d = p.draws[di];
     const ViewRec v = p.views[d.view];  // a copy: read here, before the first store
     const uint32_t t = slot - d.tbase;
@@ -817,8 +852,10 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     float rw[3];
     uint32_t s0 = 0u, s1 = 0u, s2 = 0u;
     uint32_t vi[3] = {0u, 0u, 0u};
-    if (t < d.tri_count) load_tri_indices(d, t, vi);  // in flight with the count's load
-    if (plain) {
+    // only binned triangles are written (kb_setup_count marked the bins of the others)
+    const bool emit = plain && n_bin != 0u;
+    if (emit) load_tri_indices(d, t, vi);
+    if (emit) {
         s0 = d.vbase + vi[0];
         s1 = d.vbase + vi[1];
         s2 = d.vbase + vi[2];
@@ -842,6 +879,7 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
             atomicMax(&p.stats->max_clip_list, (unsigned long long)p.ctr->n_clip_list);
         }
     }
+    if (blk_binned == 0u) return;
     unsigned long long total;
     // within one block the counts (<= 14 triangles, <= 16 rows each) cannot carry out of 16 bits
     const unsigned long long ex = block_exclusive_scan<256, unsigned long long>(
@@ -852,7 +890,6 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
         __syncthreads();
         if (n_rows) p.clip_rows[slot] = s_rows + (uint32_t)(ex >> 32);
     }
-    if (p.cmark) mark_unbinned(p, d, v, plain && n_bin == 0u, tv);
     if (n_asm == 0u) return;
     const uint32_t off_bin = (uint32_t)(bo & 0xFFFFFFFFull) + (uint32_t)(ex & 0xFFFFu);
     const uint32_t num = (uint32_t)(bo >> 32) + (uint32_t)((ex >> 16) & 0xFFFFu) - draw_asm0;
