@@ -194,7 +194,8 @@ struct BatchParams {
     uint32_t pair_cap;
     uint32_t *tile_cnt, *tile_start;      // per global tile: end and start of its range in the sorted pairs
     uint32_t *heavy;                      // tiles with more than heavy_min pairs (n_heavy of them)
-    uint32_t *tile_order;                 // the order the tile kernel takes tiles in: long lists first
+    uint4 *tile_order;                    // the order the tile kernel takes tiles in, long lists first: per
+                                          // slot (global tile, list start, list length, view)
     uint32_t heavy_min, heavy_blocks, pair_blocks;
     uint32_t tile_stage;                  // records the tile kernel stages in LDS at a time (128 or 256)
     uint32_t tile_shift;                  // tiles of 2^tile_shift x 2^tile_shift bins: 1 (16x16 px) or 2 (32x32)

@@ -1610,9 +1610,9 @@ __device__ __forceinline__ void shade_winner(const BatchParams &p, const ViewRec
     if (V.vis_prim) V.vis_prim[pix] = S.prim;
 }
 
-__device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g) {
-    const uint32_t a = p.tile_start[g], e = p.tile_cnt[g];
-    if (e - a <= p.heavy_min) return false;
+__device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *sorted_vals, uint32_t a, uint32_t len) {
+    if (len <= p.heavy_min) return false;
+    const uint32_t e = a + len;
     const uint32_t d0 = upper_index(p.n_draws, sorted_vals[a], [&](uint32_t i) { return p.asm_base[i]; });
     const uint32_t d1 = upper_index(p.n_draws, sorted_vals[e - 1u], [&](uint32_t i) { return p.asm_base[i]; });
     return d1 - d0 < kHeavyDraws;
@@ -1621,8 +1621,8 @@ __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *s
 // TB x TB bins per tile (TB = 2: 16x16 tiles, TB = 4: 32x32), one wave per bin.
 template <bool GE, uint32_t N, uint32_t TB>
 __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t vi,
-                                          TileLdsN<N> &s_tri, uint16_t *s_mask, uint8_t (*s_list)[N],
-                                          uint32_t *s_active) {
+                                          uint32_t lstart, uint32_t lcount, TileLdsN<N> &s_tri, uint16_t *s_mask,
+                                          uint8_t (*s_list)[N], uint32_t *s_active) {
     // the wave index is uniform per wave; saying so keeps everything derived from it (the bin,
     // its Hi-Z value, the walk's branches) in scalar registers
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
@@ -1635,14 +1635,9 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
     const bool inside = px < V.W && py < V.H;
     const size_t pix = (size_t)py * (size_t)V.W + (size_t)px;
 
-    const uint32_t n_asm = min(p.ctr->n_assembled, p.tri_cap);
+    // the list range comes from kb_tile_order (every triangle of the flush when the pairs overflowed)
     const bool ovf = (p.ctr->overflow & OVF_PAIRS) != 0u;
-    uint32_t lstart = 0u, lcount = n_asm;
-    if (!ovf) {
-        lstart = p.tile_start[g];
-        lcount = p.tile_cnt[g] - lstart;
-        if (is_heavy(p, sorted_vals, g)) return;  // kb_tile_heavy has drawn it
-    }
+    if (!ovf && is_heavy(p, sorted_vals, lstart, lcount)) return;  // kb_tile_heavy has drawn it
     if (lcount == 0u) return;
 
     BinState st;
@@ -1714,7 +1709,8 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
 __global__ __launch_bounds__(256) void kb_heavy_tiles(BatchParams p, const uint32_t *sorted_vals) {
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES | OVF_CLIPVERTS)) return;
     for (uint32_t g = blockIdx.x * 256u + threadIdx.x; g < p.total_tiles; g += gridDim.x * 256u)
-        if (is_heavy(p, sorted_vals, g)) p.heavy[atomicAdd(&p.ctr->n_heavy, 1u)] = g;
+        if (is_heavy(p, sorted_vals, p.tile_start[g], p.tile_cnt[g] - p.tile_start[g]))
+            p.heavy[atomicAdd(&p.ctr->n_heavy, 1u)] = g;
 }
 
 // A heavy tile per block iteration, its four 8x8 bins walked by four waves each. Heavy tiles
@@ -1871,6 +1867,7 @@ __global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
     const uint32_t avg = max(1u, n ? p.ctr->n_pairs / n : 1u);
     const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
     uint32_t start[8];
+    const uint32_t n_asm = min(p.ctr->n_assembled, p.tri_cap);
     if (PLACE) {
         uint32_t acc = 0u;
         for (int c = 0; c < 8; ++c) {
@@ -1906,7 +1903,12 @@ __global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
 #pragma unroll
             for (uint32_t c = 0; c < 8u; ++c) {
                 const uint64_t b = __ballot(cls == c);
-                if (cls == c) p.tile_order[s_base[c] + s_wc[w][c] + mbcnt(b)] = g;
+                if (cls == c) {
+                    // the tile's list range and view go with it: the tile kernel's first load
+                    const uint32_t a = ovf ? 0u : p.tile_start[g], len = ovf ? n_asm : p.tile_cnt[g] - a;
+                    const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
+                    p.tile_order[s_base[c] + s_wc[w][c] + mbcnt(b)] = make_uint4(g, a, len, vi);
+                }
             }
         }
         __syncthreads();  // s_wc and s_base are reused by the next round
@@ -1928,13 +1930,14 @@ __global__ __launch_bounds__(64 * TB * TB) void kb_tile(BatchParams p, const uin
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
     // Tiles in kb_tile_order's order; the dispatcher deals consecutive blocks to the 8 XCDs in
     // turn, so the long tiles at the front spread over all XCDs.
-    const uint32_t g = p.tile_order[blockIdx.x];
-    const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
+    const uint4 e = p.tile_order[blockIdx.x];  // (global tile, list start, list length, view)
+    if (e.z == 0u) return;                      // an empty tile: nothing drawn, nothing counted
+    const uint32_t g = e.x, vi = e.w;
     if (threadIdx.x < TB * TB) s_active[threadIdx.x] = 0u;
     if (p.views[vi].depth_func == DEPTH_GE)
-        tile_body<true, N, TB>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
+        tile_body<true, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list, s_active);
     else
-        tile_body<false, N, TB>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_active);
+        tile_body<false, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list, s_active);
     __syncthreads();
     // (bin, draw) activity of this tile for the stats; this tile's start is no longer needed
     if (threadIdx.x == 0u) {

@@ -473,7 +473,7 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->tile_cnt, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->tile_start, (size_t)(total_tiles + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->heavy, (size_t)(total_tiles + 1) * 4));
-    RC_OR_RETURN(ensure(ctx, ctx->tile_order, (size_t)(total_tiles + 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->tile_order, (size_t)(total_tiles + 1) * 16));
     RC_OR_RETURN(ensure(ctx, ctx->scan_part, scan_scratch_bytes(scan_n)));  // cleared by the first stage
     RC_OR_RETURN(ensure(ctx, ctx->radix_hist, radix_hist_elems(pair_cap) * 4));
     const size_t cwords = (size_t)((cbits + 31) / 32);
@@ -544,7 +544,7 @@ static int flush(mlv_context *ctx) {
     p.tile_cnt = (uint32_t *)ctx->tile_cnt.p;
     p.tile_start = (uint32_t *)ctx->tile_start.p;
     p.heavy = (uint32_t *)ctx->heavy.p;
-    p.tile_order = (uint32_t *)ctx->tile_order.p;
+    p.tile_order = (uint4 *)ctx->tile_order.p;
     p.tile_shift = ctx->tile_edge == 32u ? 2u : 1u;
     p.heavy_min = p.tile_shift == 1u ? ctx->heavy_min : 0xFFFFFFFFu;  // the 16-wave path needs 16x16 tiles
     p.tile_lds_pad = lds_pad(tile_kernel_lds(ctx->tile_stage, p.tile_shift), p.tile_shift == 2u ? 16u : 4u,
