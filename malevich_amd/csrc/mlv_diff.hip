@@ -1050,29 +1050,61 @@ __host__ __device__ inline MipDims mip_dims(uint32_t w, uint32_t h) {
     return d;
 }
 
-__global__ __launch_bounds__(256) void kd_mip_level(float4 *chain, MipDims d, uint32_t l) {
-    const uint32_t W = d.w[l + 1u], H = d.h[l + 1u], sw = d.w[l];
-    const uint32_t kw = d.w[l] > 1u ? 2u : 1u, kh = d.h[l] > 1u ? 2u : 1u;
-    const float4 *src = chain + d.off[l];
-    float4 *dst = chain + d.off[l + 1u];
-    for (uint32_t i = blockIdx.x * 256u + threadIdx.x; i < W * H; i += gridDim.x * 256u) {
-        const uint32_t x = i % W, y = i / W;
-        float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
-        for (uint32_t dy = 0; dy < kh; ++dy)
-            for (uint32_t dx = 0; dx < kw; ++dx) {
-                const float4 t = src[(size_t)(kh * y + dy) * sw + kw * x + dx];
-                s.x += t.x;
-                s.y += t.y;
-                s.z += t.z;
-                s.w += t.w;
-            }
-        const float a = (float)(kw * kh);
-        dst[i] = make_float4(s.x / a, s.y / a, s.z / a, s.w / a);
-    }
-}
+// Levels src + 1 .. src + kMipTileLevels of the chain, one block per 32x32 tile of level src
+// (read from tex when src = 0, the block then also copying its tile to level 0). Each level is
+// made in LDS from the one before, each texel the sum of its window in row order divided by the
+// window's area, so the chain is bit-identical to a level-by-level build (which took one launch
+// per level: 11 for a 1024^2 texture, now 2). A tile's texels of level k + 1 are those whose
+// windows lie in its texels of level k: the tile corners are multiples of 2^5, so every window of
+// an inner tile is whole, and the last tile of an odd side drops its last row or column as the
+// whole level does.
+constexpr uint32_t kMipTile = 32, kMipTileLevels = 5;
 
-__global__ __launch_bounds__(256) void kd_mip_copy(const float4 *tex, float4 *chain, size_t n) {
-    for (size_t i = (size_t)blockIdx.x * 256u + threadIdx.x; i < n; i += (size_t)gridDim.x * 256u) chain[i] = tex[i];
+__global__ __launch_bounds__(256) void kd_mip_tiles(const float4 *tex, float4 *chain, MipDims d, uint32_t src) {
+    __shared__ float4 s_a[kMipTile * kMipTile];              // levels src, src + 2, src + 4 (pitch 32)
+    __shared__ float4 s_b[(kMipTile / 2) * (kMipTile / 2)];  // levels src + 1, src + 3, src + 5 (pitch 16)
+    const uint32_t ntx = (d.w[src] + kMipTile - 1u) / kMipTile;
+    uint32_t xs = (blockIdx.x % ntx) * kMipTile, ys = (blockIdx.x / ntx) * kMipTile;
+    uint32_t rw = min(kMipTile, d.w[src] - xs), rh = min(kMipTile, d.h[src] - ys);
+    const float4 *in = src == 0u ? tex : chain + d.off[src];
+    for (uint32_t i = threadIdx.x; i < rw * rh; i += 256u) {
+        const uint32_t x = i % rw, y = i / rw;
+        const size_t at = (size_t)(ys + y) * d.w[src] + xs + x;
+        const float4 t = in[at];
+        s_a[y * kMipTile + x] = t;
+        if (src == 0u) chain[at] = t;
+    }
+    __syncthreads();
+    for (uint32_t k = 1; k <= kMipTileLevels && src + k < d.levels; ++k) {
+        const uint32_t l = src + k - 1u, nl = src + k;
+        const uint32_t kw = d.w[l] > 1u ? 2u : 1u, kh = d.h[l] > 1u ? 2u : 1u;
+        const float4 *prev = (k & 1u) ? s_a : s_b;
+        float4 *next = (k & 1u) ? s_b : s_a;
+        const uint32_t pp = (k & 1u) ? kMipTile : kMipTile / 2u, np = (k & 1u) ? kMipTile / 2u : kMipTile;
+        const uint32_t nxs = xs / kw, nys = ys / kh;
+        const uint32_t nrw = min(rw / kw, d.w[nl] - nxs), nrh = min(rh / kh, d.h[nl] - nys);
+        const float a = (float)(kw * kh);
+        for (uint32_t i = threadIdx.x; i < nrw * nrh; i += 256u) {
+            const uint32_t x = i % nrw, y = i / nrw;
+            float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+            for (uint32_t dy = 0; dy < kh; ++dy)
+                for (uint32_t dx = 0; dx < kw; ++dx) {
+                    const float4 t = prev[(kh * y + dy) * pp + kw * x + dx];
+                    s.x += t.x;
+                    s.y += t.y;
+                    s.z += t.z;
+                    s.w += t.w;
+                }
+            const float4 o = make_float4(s.x / a, s.y / a, s.z / a, s.w / a);
+            next[y * np + x] = o;
+            chain[d.off[nl] + (size_t)(nys + y) * d.w[nl] + nxs + x] = o;
+        }
+        __syncthreads();
+        xs = nxs;
+        ys = nys;
+        rw = nrw;
+        rh = nrh;
+    }
 }
 
 __global__ __launch_bounds__(256) void kd_mip_adjoint(const float4 *gchain, MipDims d, float4 *gtex) {
@@ -1104,13 +1136,11 @@ uint32_t mip_chain_texels(uint32_t w, uint32_t h, uint32_t *levels) {
 
 hipError_t launch_mip_chain(const float4 *tex, uint32_t w, uint32_t h, float4 *chain, hipStream_t s) {
     const MipDims d = mip_dims(w, h);
-    const size_t n0 = (size_t)w * h;
-    hipLaunchKernelGGL(kd_mip_copy, dim3((unsigned)std::min<size_t>((n0 + 255) / 256, 4096)), dim3(256), 0, s, tex,
-                       chain, n0);
-    for (uint32_t l = 0; l + 1u < d.levels; ++l) {
-        const uint32_t n = d.w[l + 1u] * d.h[l + 1u];
-        hipLaunchKernelGGL(kd_mip_level, dim3(std::min<uint32_t>((n + 255u) / 256u, 4096u)), dim3(256), 0, s, chain,
-                           d, l);
+    if (w == 0u || h == 0u) return hipSuccess;
+    // level 0 is copied by the first pass; a 1x1 texture has no other level
+    for (uint32_t src = 0; src == 0u || src + 1u < d.levels; src += kMipTileLevels) {
+        const uint32_t tiles = ((d.w[src] + kMipTile - 1u) / kMipTile) * ((d.h[src] + kMipTile - 1u) / kMipTile);
+        hipLaunchKernelGGL(kd_mip_tiles, dim3(tiles), dim3(256), 0, s, tex, chain, d, src);
     }
     return hipGetLastError();
 }

@@ -221,11 +221,28 @@ def test_raster_autotune_keeps_visibility(device, oracle, small_scene):
         device.set_tuning(**saved)
 
 
-@pytest.mark.parametrize("size", [(64, 64), (37, 20), (1, 9), (128, 32)])
+def _f32_chain(tex: np.ndarray) -> np.ndarray:
+    """The chain level by level in float32: each texel the sum of its window in row order, then
+    divided by the window's area (numpy's float32 adds and divides round like the device's)."""
+    levels, t = [tex], tex
+    while t.shape[0] > 1 or t.shape[1] > 1:
+        kh, kw = (2 if t.shape[0] > 1 else 1), (2 if t.shape[1] > 1 else 1)
+        H, W = t.shape[0] // kh, t.shape[1] // kw
+        s = np.zeros((H, W, 4), np.float32)
+        for dy in range(kh):
+            for dx in range(kw):
+                s = s + t[dy:kh * H:kh, dx:kw * W:kw]
+        t = (s / np.float32(kw * kh)).astype(np.float32)
+        levels.append(t)
+    return np.concatenate([x.reshape(-1, 4) for x in levels])
+
+
+@pytest.mark.parametrize("size", [(64, 64), (37, 20), (1, 9), (128, 32), (600, 45), (1024, 1024), (33, 70)])
 def test_mip_chain_and_adjoint(device, size):
-    """mlv_mip_chain against the float64 box-filter chain, and mlv_mip_chain_adjoint against the
-    adjoint PyTorch's autograd takes through the same chain of 2x2 average pools (odd edges
-    dropped), both to 1e-6 relative; sizes with odd and unit sides."""
+    """mlv_mip_chain bit for bit against a float32 level-by-level build and to 1e-6 against the
+    float64 box-filter chain, and mlv_mip_chain_adjoint against the adjoint PyTorch's autograd
+    takes through the same chain of 2x2 average pools (odd edges dropped) to 1e-6 relative; sizes
+    with odd and unit sides, several 32x32 tiles and more than five levels."""
     import torch.nn.functional as F
 
     w, h = size
@@ -238,6 +255,8 @@ def test_mip_chain_and_adjoint(device, size):
         chain, levels = D.mip_chain(tex, r)
         torch.cuda.synchronize()
         assert levels == ref_levels and chain.shape == ref_chain.shape
+        got = chain.detach().cpu().numpy().reshape(-1, 4)
+        np.testing.assert_array_equal(got.view(np.uint32), _f32_chain(tex_np).view(np.uint32))
         assert rel_err(chain.detach().cpu().numpy(), ref_chain) <= 1e-6
         g = torch.from_numpy(rng.normal(0.0, 1.0, ref_chain.shape).astype(np.float32)).cuda()
         (chain * g).sum().backward()
