@@ -93,7 +93,11 @@ typedef enum mlv_pixel_shader {
     MLV_PS_ENV_LIGHTING = 2,
     MLV_PS_PASSTHROUGH = 3,
     MLV_PS_BASIC_LAMBERT = 4, /* extension */
-    MLV_PS_ENV_TEXTURED = 5   /* extension */
+    MLV_PS_ENV_TEXTURED = 5,  /* extension */
+    MLV_PS_NONE = 6           /* extension: no pixel shader (a depth-only pass, D3D's null pixel
+                                 shader): depth, hierarchical Z and the visibility buffers are
+                                 written as by any shader, the colour target is left as it is, and
+                                 the vertex stage keeps only the position (register 0) */
 } mlv_pixel_shader;
 
 typedef enum mlv_depth_func {

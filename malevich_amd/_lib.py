@@ -23,7 +23,7 @@ BUFFER_VERTEX, BUFFER_INDEX_U32, BUFFER_CONSTANT = 1, 2, 3
 FORMAT_RGBA8_UNORM, FORMAT_RGBA32F, FORMAT_RGBA16F = 1, 2, 3
 TOPOLOGY_UNDEFINED, TOPOLOGY_TRIANGLELIST = 0, 1
 VS_BASIC, VS_VERTEX_LIGHTING, VS_PASSTHROUGH, VS_FULLSCREEN = 1, 2, 3, 4
-PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH, PS_BASIC_LAMBERT, PS_ENV_TEXTURED = 1, 2, 3, 4, 5
+PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH, PS_BASIC_LAMBERT, PS_ENV_TEXTURED, PS_NONE = 1, 2, 3, 4, 5, 6
 DEPTH_GREATER_EQUAL, DEPTH_LESS_EQUAL = 0, 1
 CULL_BACK, CULL_NONE, CULL_FRONT = 0, 1, 2
 FILL_INCLUSIVE, FILL_TOP_LEFT = 0, 1

@@ -106,8 +106,8 @@ class BatchedDiffRenderer:
 
     # ---------------------------------------------------------------- raster
     def rasterize(self, per_draw: bool = False, recording: bool = False) -> None:
-        """Draws every mesh into every view (basic_vs, reversed-Z GREATER_EQUAL); primitive ids and
-        draw tags land in self.prim / self.tags. Draw tags restart at 0: view v's draw of mesh k
+        """Draws every mesh into every view (basic_vs, no pixel shader: a depth-only pass, reversed-Z
+        GREATER_EQUAL); primitive ids and draw tags land in self.prim / self.tags. Draw tags restart at 0: view v's draw of mesh k
         has tag v * len(meshes) + k. recording: inside a graph capture (no stats reset, no flush)."""
         d = self.dev
         self._enter()
@@ -115,7 +115,7 @@ class BatchedDiffRenderer:
             d.reset_stats()
         d.ia_set_primitive_topology(L.TOPOLOGY_TRIANGLELIST)
         d.vs_set_shader(L.VS_BASIC)
-        d.ps_set_shader(L.PS_PASSTHROUGH)
+        d.ps_set_shader(L.PS_NONE)  # the shading pass recomputes everything from the primitive ids
         d.om_set_depth_func(L.DEPTH_GREATER_EQUAL)
         for v, o, (rt, ds, tri, tag), cb in zip(self.views, self.offsets[:-1], self.targets, self.cbs):
             d.rs_set_viewport(0, 0, v.width, v.height, 0.0, 1.0)

@@ -52,7 +52,7 @@ constexpr uint32_t kFirstOfDraw = 0x80000000u;  // blk_draw flag: the block is i
 enum VsKind { VS_BASIC = 1, VS_VERTEX_LIGHTING = 2, VS_PASSTHROUGH = 3, VS_FULLSCREEN = 4 };
 // PS_BASIC_LAMBERT and PS_ENV_TEXTURED are extensions (mip-mapped fetch, a light, a second
 // texture): they read the registers at the next pixel corners too (footprints for the mip level).
-enum PsKind { PS_BASIC = 1, PS_ENV_LIGHTING = 2, PS_PASSTHROUGH = 3, PS_BASIC_LAMBERT = 4, PS_ENV_TEXTURED = 5 };
+enum PsKind { PS_BASIC = 1, PS_ENV_LIGHTING = 2, PS_PASSTHROUGH = 3, PS_BASIC_LAMBERT = 4, PS_ENV_TEXTURED = 5, PS_NONE = 6 };
 enum TexFormat { FMT_RGBA8 = 1, FMT_RGBA32F = 2, FMT_RGBA16F = 3 };
 enum DepthFunc { DEPTH_GE = 0, DEPTH_LE = 1 };
 enum CullMode { CULL_BACK = 0, CULL_NONE = 1, CULL_FRONT = 2 };  // mlv_cull_mode

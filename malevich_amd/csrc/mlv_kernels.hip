@@ -522,6 +522,7 @@ size_t scan_wide_offset(size_t n_max) {
 struct VsDraw {
     const uint8_t *vb;
     uint32_t stride, vbase, vertex_count;
+    bool attrs;    // registers 1-2 are kept (every pixel shader but MLV_PS_NONE reads them)
     float cb[44];  // constant buffer slot 0, floats 0..43 (those the draw's shader reads)
     TexDesc tex;
 };
@@ -545,8 +546,10 @@ __device__ __forceinline__ void run_vs(const BatchParams &p, const VsDraw &u, ui
     vertex_shader<VS>(k, in, r0, r1, r2);
     const uint32_t slot = u.vbase + i;
     p.vpos[slot] = r0;
-    p.vattr[2u * slot] = r1;
-    p.vattr[2u * slot + 1u] = r2;
+    if (u.attrs) {
+        p.vattr[2u * slot] = r1;
+        p.vattr[2u * slot + 1u] = r2;
+    }
 }
 
 // A flush's counters, tile ranges and scan status words start at zero.
@@ -580,6 +583,7 @@ __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
         u.stride = d.stride;
         u.vbase = d.vbase;
         u.vertex_count = d.vertex_count;
+        u.attrs = d.ps != PS_NONE;
         u.tex = d.vs_tex;
         const float *cb = d.cb;  // unbound (null) for the passthrough shader
         if (vs == VS_FULLSCREEN) {
@@ -1583,6 +1587,16 @@ __device__ __forceinline__ void shade_winner(const BatchParams &p, const ViewRec
     const TriVis T = p.tvis[best];
     const TriShade S = p.tshade[best];
     const DrawRec &D = p.draws[T.draw & 0xFFFFu];
+    if (D.ps == PS_NONE) {  // depth-only: no shader, no colour write (the vertex stage kept no attributes)
+        V.depth[pix] = depth;
+        if (p.tri_mark) atomicOr(p.tri_mark + (best >> 5), 1u << (best & 31u));  // footprint counting
+        if (V.vis_tri) {
+            V.vis_tri[pix] = S.num;
+            V.vis_draw[pix] = D.draw_tag;
+        }
+        if (V.vis_prim) V.vis_prim[pix] = S.prim;
+        return;
+    }
     float pbx, pby;
     persp_bary(T, S, px, py, pbx, pby);
     const float4 a0 = p.vattr[2u * S.row[0]], b0 = p.vattr[2u * S.row[0] + 1u];

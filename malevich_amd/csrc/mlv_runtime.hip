@@ -1094,7 +1094,7 @@ MLV_API int mlv_rs_set_viewport(mlv_context *ctx, const mlv_viewport *vp) {
 }
 
 MLV_API int mlv_ps_set_shader(mlv_context *ctx, int pixel_shader) {
-    if (!ctx || pixel_shader < MLV_PS_BASIC || pixel_shader > MLV_PS_ENV_TEXTURED)
+    if (!ctx || pixel_shader < MLV_PS_BASIC || pixel_shader > MLV_PS_NONE)
         return fail(ctx, MLV_ERROR_INVALID_ARGUMENT, "unknown pixel shader");
     ctx->ps = pixel_shader;
     return MLV_OK;
@@ -1238,6 +1238,16 @@ static int submit_draw(mlv_context *ctx, const Cmd &c) {
             RC_OR_RETURN(flush(ctx));
             break;
         }
+    // A flush shades each pixel's final winner only. A depth-only draw's winner writes no colour,
+    // so the colour a pixel keeps is that of the last shaded fragment before it: a depth-only draw
+    // after a shaded draw on the same target starts a new flush (depth-only draws before shaded
+    // ones need none, nor do flushes of depth-only draws alone).
+    if (pd.rec.ps == PS_NONE)
+        for (auto &q : ctx->pending)
+            if (q.key.rt == pd.key.rt && q.rec.ps != PS_NONE) {
+                RC_OR_RETURN(flush(ctx));
+                break;
+            }
     if (ds->hiz_mode != pd.key.depth_func) {
         RC_OR_RETURN(flush_if_targets(ctx, nullptr, ds));
         const float h = pd.key.depth_func == MLV_DEPTH_LESS_EQUAL ? HUGE_VALF : -HUGE_VALF;  // rejects nothing

@@ -140,7 +140,7 @@ class DiffRenderer:
         d.ia_set_index_buffer(ib)
         d.vs_set_shader(L.VS_BASIC)
         d.vs_set_constant_buffer(0, cbuf)
-        d.ps_set_shader(L.PS_PASSTHROUGH)
+        d.ps_set_shader(L.PS_NONE)  # visibility only: the shading recomputes from the primitive ids
         d.rs_set_viewport(0, 0, self.width, self.height, 0.0, 1.0)
         d.om_set_render_targets(self.rt, self.ds)
         d.om_set_depth_func(L.DEPTH_GREATER_EQUAL)

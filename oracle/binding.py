@@ -15,7 +15,7 @@ HERE = Path(__file__).resolve().parent
 LIB_PATH = HERE / "liboracle.so"
 
 VS_BASIC, VS_VERTEX_LIGHTING, VS_PASSTHROUGH, VS_FULLSCREEN = 0, 1, 2, 3
-PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH, PS_BASIC_LAMBERT, PS_ENV_TEXTURED = 0, 1, 2, 3, 4
+PS_BASIC, PS_ENV_LIGHTING, PS_PASSTHROUGH, PS_BASIC_LAMBERT, PS_ENV_TEXTURED, PS_NONE = 0, 1, 2, 3, 4, 5
 FMT_RGBA8, FMT_RGBA32F = 0, 1
 DEPTH_GE, DEPTH_LE = 0, 1
 CULL_BACK, CULL_NONE, CULL_FRONT = 0, 1, 2  # extension; CULL_BACK is the reference's (main.c:856)

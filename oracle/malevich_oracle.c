@@ -467,8 +467,7 @@ static void set_edge(int32_t e[3], int32_t area, int32_t x0, int32_t y0, int32_t
 }
 
 static int is_valid_vs_ps(const orc_draw_desc *d) {
-    if (!(d->vs >= ORC_VS_BASIC && d->vs <= ORC_VS_FULLSCREEN && d->ps >= ORC_PS_BASIC &&
-          d->ps <= ORC_PS_ENV_TEXTURED))
+    if (!(d->vs >= ORC_VS_BASIC && d->vs <= ORC_VS_FULLSCREEN && d->ps >= ORC_PS_BASIC && d->ps <= ORC_PS_NONE))
         return 0;
     if (d->ps == ORC_PS_BASIC_LAMBERT && (!d->ps_cb || !d->ps_srv0.data)) return 0;
     if (d->ps == ORC_PS_ENV_TEXTURED && (!d->ps_srv0.data || !d->ps_srv1.data)) return 0;
@@ -842,8 +841,10 @@ int orc_draw_indexed(orc_target *rt, const orc_draw_desc *d, orc_stats *st) {
                         interp_fragment(tr, ox + l + 1, y, fx);
                         interp_fragment(tr, ox + l, y + 1, fy);
                     }
-                    pixel_shader(d, frag[l], fx, fy, rgb); /* main.c:1172 */
-                    tc[r * 8 + l] = orc_encode_output(rgb);
+                    if (d->ps != ORC_PS_NONE) { /* the depth-only extension writes no colour */
+                        pixel_shader(d, frag[l], fx, fy, rgb); /* main.c:1172 */
+                        tc[r * 8 + l] = orc_encode_output(rgb);
+                    }
                     tz[r * 8 + l] = frag[l][2];
                     tv[r * 8 + l] = tid;
                     td[r * 8 + l] = rt->draw_counter;

@@ -33,9 +33,10 @@ extern "C" {
 enum { ORC_VS_BASIC = 0, ORC_VS_VERTEX_LIGHTING = 1, ORC_VS_PASSTHROUGH = 2, ORC_VS_FULLSCREEN = 3 };
 /* ORC_PS_BASIC_LAMBERT and ORC_PS_ENV_TEXTURED restate the library's extension shaders (the
  * reference has no mips, lights or second texture): mip level by footprint, Lambert light,
- * albedo texture. */
+ * albedo texture. ORC_PS_NONE restates its depth-only pass: a fragment that passes early-Z
+ * writes depth and visibility, and no colour. */
 enum { ORC_PS_BASIC = 0, ORC_PS_ENV_LIGHTING = 1, ORC_PS_PASSTHROUGH = 2, ORC_PS_BASIC_LAMBERT = 3,
-       ORC_PS_ENV_TEXTURED = 4 };
+       ORC_PS_ENV_TEXTURED = 4, ORC_PS_NONE = 5 };
 enum { ORC_FMT_RGBA8 = 0, ORC_FMT_RGBA32F = 1 };
 enum { ORC_DEPTH_GREATER_EQUAL = 0, ORC_DEPTH_LESS_EQUAL = 1 };
 

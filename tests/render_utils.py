@@ -11,7 +11,7 @@ from malevich_amd import pipeline as P
 VS = {"basic": (L.VS_BASIC, 0), "vertex_lighting": (L.VS_VERTEX_LIGHTING, 1), "passthrough": (L.VS_PASSTHROUGH, 2),
       "fullscreen": (L.VS_FULLSCREEN, 3)}
 PS = {"basic": (L.PS_BASIC, 0), "env_lighting": (L.PS_ENV_LIGHTING, 1), "passthrough": (L.PS_PASSTHROUGH, 2),
-      "basic_lambert": (L.PS_BASIC_LAMBERT, 3), "env_textured": (L.PS_ENV_TEXTURED, 4)}
+      "basic_lambert": (L.PS_BASIC_LAMBERT, 3), "env_textured": (L.PS_ENV_TEXTURED, 4), "none": (L.PS_NONE, 5)}
 
 
 @dataclass

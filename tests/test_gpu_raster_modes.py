@@ -1,5 +1,5 @@
 """The extension raster modes on the GPU against the oracle: cull modes (back, the reference's;
-none; front) and the opt-in top-left fill rule.
+none; front), the opt-in top-left fill rule and the depth-only pass (no pixel shader).
 
 Bar: visibility, depth and every stat bit-exact, colour within 1/255; a rectangle tiled by
 triangles of both windings with every vertex on a pixel corner is covered exactly once per
@@ -65,3 +65,19 @@ def test_sphere_modes(device, oracle, cull, fill):
     o = render_oracle(oracle, W, H, [d], S.CLEAR_COLOR)
     assert (o.vis_tri != 0xFFFFFFFF).sum() > 1000
     assert_frames_match(g, o, color_tol=1)
+
+
+@pytest.mark.parametrize("order", [("passthrough", "none", "passthrough"), ("none", "passthrough", "none"),
+                                   ("none", "none", "none")])
+def test_depth_only_pass(device, oracle, order):
+    """Three overlapping draws, some without a pixel shader (a depth-only draw after a shaded one,
+    and before one): colour, depth, visibility and stats bit-exact against the oracle (whose
+    depth-only pass is pinned on the CPU)."""
+    from tests.test_oracle_raster_modes import _depth_only_draws
+    W, H = 48, 40
+    draws = _depth_only_draws(order[1], (order[0], order[2]))
+    print(f"case: depth-only draws {order}", flush=True)
+    g = render_gpu(device, W, H, draws, S.CLEAR_COLOR)
+    o = render_oracle(oracle, W, H, draws, S.CLEAR_COLOR)
+    assert_frames_match(g, o, color_tol=0)
+    assert (o.vis_draw == 1).sum() > 100  # the depth-only draw wins pixels
