@@ -1164,14 +1164,18 @@ hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const D
                                 bool deterministic, hipStream_t s) {
     const uint32_t nb = deterministic ? diff_blocks(p.v.W, p.v.H) : diff_tile_blocks(p.v.W, p.v.H);
     hipError_t e;
-    if ((e = zero(g.pos, 3ull * p.m.V, s)) != hipSuccess || (e = zero(g.col, 3ull * p.m.V, s)) != hipSuccess ||
-        (e = zero(g.uv, 2ull * p.m.V, s)) != hipSuccess || (e = zero(g.nrm, 3ull * p.m.V, s)) != hipSuccess ||
-        (e = zero(g.tex, 4ull * p.m.tex_total, s)) != hipSuccess)
-        return e;
-    if (nb == 0) return zero(g.sh, 27, s);
+    double *posacc = (!deterministic && g.pos) ? sc.posacc : nullptr;
+    // every gradient array (and the FP64 position sums) cleared by one launch
+    FillJobs z{};
+    z.add(g.pos, 0u, 3ull * p.m.V);
+    z.add(g.col, 0u, 3ull * p.m.V);
+    z.add(g.uv, 0u, 2ull * p.m.V);
+    z.add(g.nrm, 0u, 3ull * p.m.V);
+    z.add(g.tex, 0u, 4ull * p.m.tex_total);
+    z.add(posacc, 0u, 6ull * p.m.V);
+    if (nb == 0) z.add(g.sh, 0u, 27);
+    if ((e = launch_fill(z, s)) != hipSuccess || nb == 0) return e;
     if (!deterministic) {
-        double *posacc = g.pos ? sc.posacc : nullptr;
-        if ((e = zero(posacc, 3ull * p.m.V, s)) != hipSuccess) return e;
         hipLaunchKernelGGL(kd_backward_atomic, dim3(nb), dim3(256), 0, s, p, gout, g, posacc, sc.sh_part);
         if (g.pos)
             hipLaunchKernelGGL(kd_pos_finalize, dim3((3u * p.m.V + 255u) / 256u), dim3(256), 0, s,

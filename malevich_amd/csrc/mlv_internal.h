@@ -328,14 +328,14 @@ hipError_t launch_camera_cbs(const CameraRec *cams, uint32_t n, float *cbs, hipS
 
 // Launchers (mlv_kernels.hip). All enqueue on `s`.
 // Buffers of 32-bit words to fill with one value each, in one launch.
-constexpr uint32_t kMaxFills = 4;
+constexpr uint32_t kMaxFills = 8;
 struct FillJobs {
     uint32_t *d[kMaxFills];
     uint32_t v[kMaxFills];
     size_t n[kMaxFills];
     uint32_t count;
-    void add(void *dst, uint32_t value, size_t words) {
-        if (!words || count >= kMaxFills) return;
+    void add(void *dst, uint32_t value, size_t words) {  // a null dst or no words: nothing to fill
+        if (!dst || !words || count >= kMaxFills) return;
         d[count] = (uint32_t *)dst;
         v[count] = value;
         n[count] = words;
