@@ -784,10 +784,10 @@ __global__ __launch_bounds__(256) void kd_backward_records(DiffParams p, const f
 
 // Sum of the per-block SH partials, one block per coefficient: thread t sums blocks t, t+256, ...
 // in order, then a fixed tree over the 256 partial sums (the same order on every call).
-__global__ __launch_bounds__(256) void kd_sh_sum(const double *sh_part, uint32_t nblocks, float *dsh,
-                                                 uint32_t accumulate) {
+__device__ __forceinline__ void sh_sum_block(const double *sh_part, uint32_t nblocks, float *dsh, uint32_t accumulate,
+                                             uint32_t k) {
     __shared__ double s[256];
-    const uint32_t k = blockIdx.x, t = threadIdx.x;
+    const uint32_t t = threadIdx.x;
     double acc = 0.0;
     for (uint32_t b = t; b < nblocks; b += 256u) acc += sh_part[(size_t)k * nblocks + b];
     s[t] = acc;
@@ -799,8 +799,21 @@ __global__ __launch_bounds__(256) void kd_sh_sum(const double *sh_part, uint32_t
     if (t == 0u) dsh[k] = accumulate ? dsh[k] + (float)s[0] : (float)s[0];
 }
 
-__global__ __launch_bounds__(256) void kd_pos_finalize(const double *acc, float *out, uint32_t n) {
-    const uint32_t i = blockIdx.x * 256u + threadIdx.x;
+// SH coefficient k = blockIdx.x: the sum of its per-block partials in a fixed tree order.
+__global__ __launch_bounds__(256) void kd_sh_sum(const double *sh_part, uint32_t nblocks, float *dsh,
+                                                 uint32_t accumulate) {
+    sh_sum_block(sh_part, nblocks, dsh, accumulate, blockIdx.x);
+}
+
+// The atomic backward's tail in one launch: the first n_sh blocks sum the SH partials (as
+// kd_sh_sum), the others turn the FP64 position sums into the float gradient.
+__global__ __launch_bounds__(256) void kd_backward_tail(const double *sh_part, uint32_t nblocks, float *dsh,
+                                                        uint32_t n_sh, const double *acc, float *out, uint32_t n) {
+    if (blockIdx.x < n_sh) {
+        sh_sum_block(sh_part, nblocks, dsh, 0u, blockIdx.x);
+        return;
+    }
+    const uint32_t i = (blockIdx.x - n_sh) * 256u + threadIdx.x;
     if (i < n) out[i] = (float)acc[i];
 }
 
@@ -1177,9 +1190,11 @@ hipError_t launch_diff_backward(const DiffParams &p, const float4 *gout, const D
     if ((e = launch_fill(z, s)) != hipSuccess || nb == 0) return e;
     if (!deterministic) {
         hipLaunchKernelGGL(kd_backward_atomic, dim3(nb), dim3(256), 0, s, p, gout, g, posacc, sc.sh_part);
-        if (g.pos)
-            hipLaunchKernelGGL(kd_pos_finalize, dim3((3u * p.m.V + 255u) / 256u), dim3(256), 0, s,
-                               (const double *)posacc, g.pos, 3u * p.m.V);
+        const uint32_t n_sh = g.sh ? 27u : 0u, n_pos = g.pos ? 3u * p.m.V : 0u;
+        if (n_sh + n_pos)
+            hipLaunchKernelGGL(kd_backward_tail, dim3(n_sh + (n_pos + 255u) / 256u), dim3(256), 0, s,
+                               (const double *)sc.sh_part, nb, g.sh, n_sh, (const double *)posacc, g.pos, n_pos);
+        return hipGetLastError();
     } else {
         hipLaunchKernelGGL(kd_backward_records, dim3(nb), dim3(256), 0, s, p, gout, sc);
         const uint32_t P = p.v.W * p.v.H;
