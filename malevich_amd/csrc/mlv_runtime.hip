@@ -1632,7 +1632,7 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
     static const uint32_t tile_stage[] = {128u, 256u};
     static const uint32_t tile_waves[] = {0u, 8u, 16u, 24u};
     static const uint32_t setup_waves[] = {0u, 16u, 24u};
-    static const uint32_t heavy_min[] = {192u, 384u, 768u, 1536u, 0xFFFFFFFFu};
+    static const uint32_t heavy_min[] = {0u, 32u, 64u, 192u, 384u, 768u, 1536u, 0xFFFFFFFFu};
     static const uint32_t heavy_blocks[] = {256u, 512u, 1024u, 2048u};
     static const uint32_t pair_blocks[] = {512u, 1024u, 2048u, 4096u};
     double best_ms = 0.0;
@@ -1643,7 +1643,7 @@ MLV_API int mlv_autotune_end(mlv_context *ctx, uint32_t repeats, mlv_tuning *bes
                               &ctx->setup_waves, &ctx->heavy_min, &ctx->heavy_blocks, &ctx->pair_blocks};
     const uint32_t *cand[kKnobs] = {skip, tile_edge, tile_stage, tile_waves, setup_waves, heavy_min, heavy_blocks,
                                     pair_blocks};
-    const size_t ncand[kKnobs] = {2, 2, 2, 4, 3, 5, 4, 4};
+    const size_t ncand[kKnobs] = {2, 2, 2, 4, 3, 8, 4, 4};
     for (int k = 0; k < kKnobs; ++k) {
         if ((k == 5 || k == 6) && ctx->tile_edge != 16u) continue;  // the heavy-tile path needs 16x16 tiles
         uint32_t keep = *knob[k];
