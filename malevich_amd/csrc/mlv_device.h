@@ -301,7 +301,8 @@ struct VsConstants {
 
 // Vertex shaders (basic_vs.c:22-33, vertex_lighting_vs.c:22-39, passthrough_vs.c:16-25,
 // fullscreen_vs.c:22-39). Writes the three output registers.
-template <int VS>
+// kAttrs = false: register 0 only (a draw whose registers 1-2 another draw of the flush writes).
+template <int VS, bool kAttrs = true>
 __device__ __forceinline__ void vertex_shader(const VsConstants<VS> &k, const float *in, float4 &r0, float4 &r1,
                                               float4 &r2) {
     if constexpr (VS == VS_BASIC || VS == VS_VERTEX_LIGHTING) {
@@ -309,6 +310,7 @@ __device__ __forceinline__ void vertex_shader(const VsConstants<VS> &k, const fl
         float x = in[0], y = in[1], z = in[2];
         r0 = make_float4(dot4_pair(cfw, x, y, z, 1.0f), dot4_pair(cfw + 4, x, y, z, 1.0f),
                          dot4_pair(cfw + 8, x, y, z, 1.0f), dot4_pair(cfw + 12, x, y, z, 1.0f));
+        if constexpr (!kAttrs) return;
         float nx, ny, nz;
         normalize3(in[3], in[4], in[5], nx, ny, nz);
         if constexpr (VS == VS_BASIC) {

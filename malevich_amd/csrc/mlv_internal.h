@@ -88,6 +88,12 @@ struct DrawRec {
     int32_t cull, fill;           // mlv_cull_mode, mlv_fill_rule
     uint32_t view;                // index into the view table
     uint32_t vbase, tbase;        // first vertex slot / first input-triangle slot
+    // First attribute row of the draw's vertices (VS registers 1-2). Registers 1-2 of basic_vs,
+    // vertex_lighting_vs and passthrough_vs depend on the vertex alone (and vertex_lighting_vs's
+    // texture), not on the constant buffer, so a draw of the same vertices with the same shader
+    // earlier in the flush (another view of the same mesh) lends its rows: abase is then that
+    // draw's vbase and the vertex stage writes positions only. Otherwise abase = vbase.
+    uint32_t abase;
     uint32_t blk_v0, blk_t0;      // first block of the vertex / setup stage
     uint32_t draw_tag;            // written to the visibility buffer
     uint32_t cbase;               // first bit of this draw's bins in BatchParams::cmark

@@ -195,11 +195,12 @@ __device__ __forceinline__ int clip_group(const BatchParams &p, const DrawRec &d
                                           float (&x)[NC], float *s) {
     uint32_t vi[3];
     if (!load_tri_indices(d, t, vi)) return 0;  // the same for the whole group
-    const uint32_t slot = d.vbase + (g.li == 1u ? vi[1] : g.li == 2u ? vi[2] : vi[0]);
-    const float4 q = p.vpos[slot];
+    const uint32_t v = g.li == 1u ? vi[1] : g.li == 2u ? vi[2] : vi[0];
+    const float4 q = p.vpos[d.vbase + v];
     x[0] = q.x; x[1] = q.y; x[2] = q.z; x[3] = q.w;
     if constexpr (NC > 4) {
-        const float4 y = p.vattr[2u * slot], z = p.vattr[2u * slot + 1u];
+        const uint32_t row = d.abase + v;  // registers 1-2 (DrawRec::abase)
+        const float4 y = p.vattr[2u * row], z = p.vattr[2u * row + 1u];
         x[4] = y.x; x[5] = y.y; x[6] = y.z; x[7] = y.w;
         x[8] = z.x; x[9] = z.y; x[10] = z.z; x[11] = z.w;
     }
@@ -522,7 +523,8 @@ size_t scan_wide_offset(size_t n_max) {
 struct VsDraw {
     const uint8_t *vb;
     uint32_t stride, vbase, vertex_count;
-    bool attrs;    // registers 1-2 are kept (every pixel shader but MLV_PS_NONE reads them)
+    bool attrs;    // registers 1-2 are written: the draw's pixel shader reads them (all but MLV_PS_NONE)
+                   // and no earlier draw of the flush wrote them for the same vertices
     float cb[44];  // constant buffer slot 0, floats 0..43 (those the draw's shader reads)
     TexDesc tex;
 };
@@ -543,13 +545,15 @@ __device__ __forceinline__ void run_vs(const BatchParams &p, const VsDraw &u, ui
         for (int k2 = 0; k2 < 8; ++k2) in[k2] = load_global((const float *)src + k2);
     }
     float4 r0, r1, r2;
-    vertex_shader<VS>(k, in, r0, r1, r2);
     const uint32_t slot = u.vbase + i;
-    p.vpos[slot] = r0;
     if (u.attrs) {
+        vertex_shader<VS, true>(k, in, r0, r1, r2);
         p.vattr[2u * slot] = r1;
         p.vattr[2u * slot + 1u] = r2;
+    } else {
+        vertex_shader<VS, false>(k, in, r0, r1, r2);
     }
+    p.vpos[slot] = r0;
 }
 
 // A flush's counters, tile ranges and scan status words start at zero.
@@ -583,7 +587,7 @@ __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
         u.stride = d.stride;
         u.vbase = d.vbase;
         u.vertex_count = d.vertex_count;
-        u.attrs = d.ps != PS_NONE;
+        u.attrs = d.ps != PS_NONE && d.abase == d.vbase;
         u.tex = d.vs_tex;
         const float *cb = d.cb;  // unbound (null) for the passthrough shader
         if (vs == VS_FULLSCREEN) {
@@ -860,10 +864,10 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     const bool emit = plain && n_bin != 0u;
     if (emit) load_tri_indices(d, t, vi);
     if (emit) {
-        s0 = d.vbase + vi[0];
-        s1 = d.vbase + vi[1];
-        s2 = d.vbase + vi[2];
-        setup_triangle(d, v, p.vpos[s0], p.vpos[s1], p.vpos[s2], tv, rw);
+        setup_triangle(d, v, p.vpos[d.vbase + vi[0]], p.vpos[d.vbase + vi[1]], p.vpos[d.vbase + vi[2]], tv, rw);
+        s0 = d.abase + vi[0];  // the attribute rows the pixel shader reads (DrawRec::abase)
+        s1 = d.abase + vi[1];
+        s2 = d.abase + vi[2];
     }
     if (threadIdx.x == 0u) {
         // a draw's first binned index; empty draws start where the next one does

@@ -19,6 +19,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <map>
 #include <memory>
 #include <mutex>
 #include <new>
@@ -65,6 +66,21 @@ namespace {
 struct Arena {
     void *p = nullptr;
     size_t cap = 0;
+};
+
+// What VS registers 1-2 of a draw's vertices depend on (see DrawRec::abase).
+struct AttrKey {
+    const void *vb;
+    uint32_t stride, count;
+    int32_t vs;
+    const void *tex;
+    bool operator<(const AttrKey &o) const {
+        if (vb != o.vb) return vb < o.vb;
+        if (stride != o.stride) return stride < o.stride;
+        if (count != o.count) return count < o.count;
+        if (vs != o.vs) return vs < o.vs;
+        return tex < o.tex;
+    }
 };
 
 struct ViewKey {
@@ -386,6 +402,7 @@ static int flush(mlv_context *ctx) {
     // view table (first appearance order) and slot offsets
     std::vector<ViewKey> keys;
     std::vector<ViewRec> views;
+    std::map<AttrKey, uint32_t> attr_owner;  // vertices whose registers 1-2 a draw of this flush writes
     uint64_t total_tiles = 0, vslots = 0, tslots = 0;
     for (auto &pd : draws) {
         size_t vi = 0;
@@ -413,6 +430,13 @@ static int flush(mlv_context *ctx) {
         DrawRec &r = pd.rec;
         r.view = (uint32_t)vi;
         r.vbase = (uint32_t)vslots;
+        r.abase = r.vbase;
+        if (r.ps != PS_NONE && r.vs != VS_FULLSCREEN) {  // registers 1-2 from the vertex alone
+            const AttrKey ak{r.vb, r.stride, r.vertex_count, r.vs, r.vs == VS_VERTEX_LIGHTING ? r.vs_tex.data : nullptr};
+            auto it = attr_owner.find(ak);
+            if (it != attr_owner.end()) r.abase = it->second;
+            else attr_owner.emplace(ak, r.vbase);
+        }
         r.blk_v0 = (uint32_t)(vslots / 256);
         vslots += align256(r.vertex_count);
         r.tbase = (uint32_t)tslots;
