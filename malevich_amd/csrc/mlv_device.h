@@ -412,10 +412,16 @@ __device__ __forceinline__ void ext_pixel_shader(const DrawRec &D, float4 a1, fl
 
 // ---------------------------------------------------- primitive assembly
 
-__device__ __forceinline__ int32_t snap28_4(float v) {  // main.c:842-848: floor(v*16 + 0.5) in double
-    double d = floor((double)(v * 16.0f) + 0.5);
-    if (!(d >= -2147483648.0 && d < 2147483648.0)) return INT32_MIN;
-    return (int32_t)d;
+// main.c:842-848: floor(v*16 + 0.5) in double, here in float and exact: with x = v*16 (exact),
+// floor(x + 0.5) = floor(x) + (x - floor(x) >= 0.5). x - floor(x) is exact except for negative x
+// above -0.5, where it rounds monotonically from a value above 0.5 and so stays at or above it;
+// x at or beyond 2^24 is an integer (difference 0). Out of the int32 range (or NaN): INT32_MIN.
+__device__ __forceinline__ int32_t snap28_4(float v) {
+    const float x = v * 16.0f;
+    const float fl = floorf(x);
+    const float r = (x - fl >= 0.5f) ? fl + 1.0f : fl;
+    if (!(r >= -2147483648.0f && r < 2147483648.0f)) return INT32_MIN;
+    return (int32_t)r;
 }
 
 // set_edge_function, main.c:563-575, u32 wrap
