@@ -25,6 +25,7 @@
 // order and can refresh that bin's hierarchical-Z minimum between draws exactly as the
 // reference's write_tile does at the end of every draw (main.c:589-603).
 #include <algorithm>
+#include <type_traits>
 #include <utility>
 
 #include "mlv_device.h"
@@ -1637,10 +1638,10 @@ __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *s
 }
 
 // TB x TB bins per tile (TB = 2: 16x16 tiles, TB = 4: 32x32), one wave per bin.
-template <bool GE, uint32_t N, uint32_t TB>
+template <bool GE, uint32_t N, uint32_t TB, class Mask>
 __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t vi,
-                                          uint32_t lstart, uint32_t lcount, TileLdsN<N> &s_tri, uint16_t *s_mask,
-                                          uint8_t (*s_list)[N], uint32_t *s_active) {
+                                          uint32_t lstart, uint32_t lcount, TileLdsN<N> &s_tri, Mask *s_mask,
+                                          uint8_t (*s_list)[N]) {
     // the wave index is uniform per wave; saying so keeps everything derived from it (the bin,
     // its Hi-Z value, the walk's branches) in scalar registers
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
@@ -1687,7 +1688,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
                     if (gx >= bx0 && gx <= bx1 && gy >= by0 && gy <= by1) m |= 1u << k;
                 }
             }
-            s_mask[tid] = (uint16_t)m;
+            s_mask[tid] = (Mask)m;
         }
         __syncthreads();
         uint32_t cnt = 0u;
@@ -1709,12 +1710,12 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
         if (lane == 0u && m) atomicAdd(&p.foot->shaded_px, (unsigned long long)__popcll(m));
     }
 
-    if (lane == 0u) s_active[w] = 0u;
     if (sub_valid && st.cur_draw >= 0) {
         const float m = tile_hiz<GE>(inside, st.d);
         if (lane == 0u) {
             V.hiz[sby * V.bins_x + sbx] = m;
-            s_active[w] = st.n_draws;
+            // (bin, draw) activity for the stats, spread over 64 counters
+            atomicAdd(&p.stats->active8_part[blockIdx.x & 63u], (unsigned long long)st.n_draws);
         }
     }
     clear_seen(p, st);  // last: no later load of the wave waits for these atomics
@@ -1938,12 +1939,14 @@ __global__ __launch_bounds__(256) void kb_tile_order(BatchParams p) {
 // ones touching its bin and walks them in submission order with its pixels' depths in registers;
 // afterwards each pixel's winner is shaded once and colour, depth, visibility and the bin's Hi-Z
 // are written.
+// 16x16 tiles run at 8 blocks per CU: 64 VGPRs and 20 KiB of LDS a block.
 template <uint32_t N, uint32_t TB>
-__global__ __launch_bounds__(64 * TB * TB) void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
+__global__ __launch_bounds__(64 * TB * TB) __attribute__((amdgpu_waves_per_eu(TB == 2u ? 8u : 1u)))
+void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
+    using Mask = typename std::conditional<TB * TB <= 8u, uint8_t, uint16_t>::type;  // a bit per bin
     __shared__ TileLdsN<N> s_tri;
-    __shared__ uint16_t s_mask[N];
+    __shared__ Mask s_mask[N];
     __shared__ __attribute__((aligned(8))) uint8_t s_list[TB * TB][N];
-    __shared__ uint32_t s_active[TB * TB];
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
     // Tiles in kb_tile_order's order; the dispatcher deals consecutive blocks to the 8 XCDs in
@@ -1951,19 +1954,10 @@ __global__ __launch_bounds__(64 * TB * TB) void kb_tile(BatchParams p, const uin
     const uint4 e = p.tile_order[blockIdx.x];  // (global tile, list start, list length, view)
     if (e.z == 0u) return;                      // an empty tile: nothing drawn, nothing counted
     const uint32_t g = e.x, vi = e.w;
-    if (threadIdx.x < TB * TB) s_active[threadIdx.x] = 0u;
     if (p.views[vi].depth_func == DEPTH_GE)
-        tile_body<true, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list, s_active);
+        tile_body<true, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list);
     else
-        tile_body<false, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list, s_active);
-    __syncthreads();
-    // (bin, draw) activity of this tile for the stats; this tile's start is no longer needed
-    if (threadIdx.x == 0u) {
-        uint32_t a = 0u;
-#pragma unroll
-        for (uint32_t k = 0; k < TB * TB; ++k) a += s_active[k];
-        if (a) atomicAdd(&p.stats->active8_part[blockIdx.x & 63u], (unsigned long long)a);
-    }
+        tile_body<false, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list);
 }
 
 // Sum of the per-tile activity counts into the stats (one atomic per block).
