@@ -1488,6 +1488,91 @@ __device__ __forceinline__ uint32_t next_survivor(uint32_t k, uint64_t &S) {
     return (uint32_t)__builtin_amdgcn_readlane((int)k, (int)jj);
 }
 
+// Triangle-parallel rasterisation of tiny survivors. Within one draw the walk's result does not
+// depend on the order of its triangles: each pixel keeps the greatest (depth, order) key (GE; the
+// least depth under LESS_EQUAL) among the triangles covering it, ties in depth going to the later
+// triangle, and the draw's hierarchical-Z value and the screening do not change while it lasts.
+// So a wave can take up to 64 survivors of one draw at once, a lane each, every lane evaluating
+// only the pixel corners inside its triangle's snapped box within the bin (coverage lies inside
+// that box when edge values cannot wrap: DrawRec::depth_cull), and merge them per pixel with
+// 64-bit LDS maxima of keys: high word the depth in an order-preserving code, low word
+// (lane + 1) << 1 (0 for the pixel's state before the batch, so a triangle wins a tie with it,
+// and among the batch the later entry wins) plus a bit that marks a depth of -0.0, which the key
+// orders as +0.0 (they compare equal) but the depth buffer keeps as written.
+constexpr uint32_t kTpCorners = 16u;  // at most this many corners of the bin in the box
+constexpr uint32_t kTpMin = 8u;      // at least this many such survivors in the batch
+
+template <bool GE>
+__device__ __forceinline__ uint32_t depth_key(float z) {
+    uint32_t u = __float_as_uint(z);
+    if ((u << 1) == 0u) u = 0u;                                   // -0.0 as +0.0
+    const uint32_t o = u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);  // unsigned order = float order
+    return GE ? o : ~o;
+}
+
+template <bool GE>
+__device__ __forceinline__ float depth_of_key(uint32_t hi, bool neg_zero) {
+    const uint32_t o = GE ? hi : ~hi;
+    const uint32_t u = (o & 0x80000000u) ? (o ^ 0x80000000u) : ~o;
+    return neg_zero ? -0.0f : __uint_as_float(u);
+}
+
+// The batch's tiny survivors (lanes of T; lane j holds entry j's record slot k and the corners
+// [cx0, cx1] x [cy0, cy1] of its box, relative to the bin origin x0, y0) into the bin's state.
+// s_key holds this wave's 64 keys.
+template <bool GE, class Lds>
+__device__ __forceinline__ void tp_rasterise(const Lds &L, uint64_t *s_key, uint64_t T, uint32_t k, int32_t cx0,
+                                             int32_t cx1, int32_t cy0, int32_t cy1, BinState &st, int32_t x0,
+                                             int32_t y0, uint32_t lane) {
+    // the pixel's state before the batch; a NaN depth takes no update, so nothing may beat it
+    s_key[lane] = (st.d != st.d) ? ~0ull : ((uint64_t)depth_key<GE>(st.d) << 32);
+    __builtin_amdgcn_wave_barrier();
+    const bool mine = (T >> lane) & 1ull;
+    if (mine) {
+        const CoverRec r = load_cover(L, k);
+        const uint32_t tag = (lane + 1u) << 1;
+        int32_t x = cx0, y = cy0;
+        for (;;) {
+            const int32_t px = x0 + x, py = y0 + y;
+            const uint32_t e0 = mad24(r.a16[0], px, mad24(r.b16[0], py, (uint32_t)r.c[0]));
+            const uint32_t e1 = mad24(r.a16[1], px, mad24(r.b16[1], py, (uint32_t)r.c[1]));
+            const uint32_t e2 = mad24(r.a16[2], px, mad24(r.b16[2], py, (uint32_t)r.c[2]));
+            bool cov = (int32_t)(e0 | e1 | e2) > 0;  // main.c:1028 (the box lies inside the view)
+            if (r.excl)
+                cov = cov && !((r.excl & 1u) && e0 == 0u) && !((r.excl & 2u) && e1 == 0u) &&
+                      !((r.excl & 4u) && e2 == 0u);
+            if (cov) {
+                // the depth exactly as bin_cover computes it
+                const float b1 = (float)((int32_t)e1 >> 8) * r.ooa;
+                const float b2 = (float)((int32_t)e2 >> 8) * r.ooa;
+                float z = r.z0 + r.dz1 * b1;
+                z = z + r.dz2 * b2;
+                if (z == z) {  // a NaN depth never wins (its comparisons are false)
+                    const uint64_t key = ((uint64_t)depth_key<GE>(z) << 32) |
+                                         (uint64_t)(tag | (__float_as_uint(z) == 0x80000000u ? 1u : 0u));
+                    atomicMax((unsigned long long *)&s_key[y * 8 + x], (unsigned long long)key);
+                }
+            }
+            if (++x > cx1) {
+                x = cx0;
+                if (++y > cy1) break;
+            }
+        }
+    }
+    __builtin_amdgcn_wave_barrier();
+    const uint64_t kk = s_key[lane];
+    const uint32_t lo = (uint32_t)kk;
+    const bool won = lo != 0u && lo != 0xFFFFFFFFu;  // a triangle of the batch took this pixel
+    const uint32_t j = won ? (lo >> 1) - 1u : 0u;
+    const uint32_t kw = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(j << 2), (int)k);  // lane j's slot
+    if (won) {
+        st.d = depth_of_key<GE>((uint32_t)(kk >> 32), (lo & 1u) != 0u);
+        st.best = (int32_t)L.id[kw];
+    }
+    st.stale |= __ballot(won);
+    __builtin_amdgcn_wave_barrier();  // s_key is rewritten by the next batch
+}
+
 // Largest value an edge function takes over the bin's pixels (x0..x0+7, y0..y0+7), exact when
 // the draw guarantees no i32 wrap at any pixel (DrawRec::depth_cull).
 __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c, int32_t x0, int32_t y0) {
@@ -1504,7 +1589,7 @@ __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c,
 template <bool GE, class Lds>
 __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, const uint8_t *list, uint32_t first,
                                          uint32_t stride, uint32_t n, BinState &st, bool inside, int32_t px,
-                                         int32_t py, int32_t x0, int32_t y0, uint32_t lane) {
+                                         int32_t py, int32_t x0, int32_t y0, uint32_t lane, uint64_t *s_key) {
     for (uint32_t i0 = 0; i0 < n;) {
         const uint32_t i = i0 + lane;
         const bool in = i < n;
@@ -1519,13 +1604,12 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, con
             st.stale = 0ull;
         }
         bool live = lane < run;
-        // a pixel corner of the bin inside the triangle's bounding box
-        {
-            const uint32_t lx = L.lat_x[k], ly = L.lat_y[k];
-            const int32_t lx0 = (int16_t)(lx & 0xFFFFu), lx1 = (int16_t)(lx >> 16);
-            const int32_t ly0 = (int16_t)(ly & 0xFFFFu), ly1 = (int16_t)(ly >> 16);
-            live = live && imax(lx0, x0) <= imin(lx1, x0 + 7) && imax(ly0, y0) <= imin(ly1, y0 + 7);
-        }
+        // a pixel corner of the bin inside the triangle's bounding box: the bin's corners
+        // [bx0, bx1] x [by0, by1] in it
+        const uint32_t lx = L.lat_x[k], ly = L.lat_y[k];
+        const int32_t bx0 = imax((int16_t)(lx & 0xFFFFu), x0), bx1 = imin((int16_t)(lx >> 16), x0 + 7);
+        const int32_t by0 = imax((int16_t)(ly & 0xFFFFu), y0), by1 = imin((int16_t)(ly >> 16), y0 + 7);
+        live = live && bx0 <= bx1 && by0 <= by1;
         const float hzd = L.hiz[k];
         live = live && !(GE ? (hzd < st.hz) : (hzd > st.hz));
         if (st.depth_cull) {
@@ -1539,6 +1623,14 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, con
         // survivor's record is read from LDS while the current one is rasterised. Two records
         // alternate (the loop is unrolled by two), so the prefetch needs no register copies.
         uint64_t S = __ballot(live);
+        if (s_key && st.depth_cull && (uint32_t)__popcll(S) >= kTpMin) {
+            // tiny survivors triangle-parallel when there are enough of them; the rest below
+            const uint64_t T = __ballot(live && (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1)) <= kTpCorners);
+            if ((uint32_t)__popcll(T) >= kTpMin) {
+                tp_rasterise<GE>(L, s_key, T, k, bx0 - x0, bx1 - x0, by0 - y0, by1 - y0, st, x0, y0, lane);
+                S &= ~T;
+            }
+        }
         if (S) {
             CoverRec ra = load_cover(L, next_survivor(k, S)), rb = ra;
             for (;;) {
@@ -1641,7 +1733,7 @@ __device__ __forceinline__ bool is_heavy(const BatchParams &p, const uint32_t *s
 template <bool GE, uint32_t N, uint32_t TB, class Mask>
 __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g, uint32_t vi,
                                           uint32_t lstart, uint32_t lcount, TileLdsN<N> &s_tri, Mask *s_mask,
-                                          uint8_t (*s_list)[N]) {
+                                          uint8_t (*s_list)[N], uint64_t (*s_key)[64]) {
     // the wave index is uniform per wave; saying so keeps everything derived from it (the bin,
     // its Hi-Z value, the walk's branches) in scalar registers
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
@@ -1700,7 +1792,8 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
             if (b) s_list[w][cnt + mbcnt(bal)] = (uint8_t)i;
             cnt += (uint32_t)__popcll(bal);
         }
-        if (sub_valid && cnt) walk_bin<GE>(p, s_tri, s_list[w], 0u, 1u, cnt, st, inside, px, py, sbx * kBin, sby * kBin, lane);
+        if (sub_valid && cnt)
+            walk_bin<GE>(p, s_tri, s_list[w], 0u, 1u, cnt, st, inside, px, py, sbx * kBin, sby * kBin, lane, s_key[w]);
         __syncthreads();
     }
 
@@ -1744,7 +1837,8 @@ constexpr uint32_t kWavesPerBin = kHeavyWaves / 4;
 template <bool GE>
 __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t *sorted_vals, uint32_t g,
                                            uint32_t vi, TileLds &s_tri, uint8_t *s_mask, uint8_t (*s_list)[256],
-                                           uint32_t *s_cnt, float (*s_md)[64], int32_t (*s_mb)[64]) {
+                                           uint32_t *s_cnt, float (*s_md)[64], int32_t (*s_mb)[64],
+                                           uint64_t (*s_key)[64]) {
     const uint32_t tid = threadIdx.x, lane = tid & 63u, w = uniform_u(tid >> 6);
     const uint32_t bin = w / kWavesPerBin, q = w % kWavesPerBin;
     const ViewRec &V = p.views[vi];
@@ -1805,7 +1899,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
         if (cnt) seen = true;
         if (sub_valid && q < cnt)
             walk_bin<GE>(p, s_tri, s_list[bin], q, kWavesPerBin, (cnt - q + kWavesPerBin - 1u) / kWavesPerBin, st,
-                         inside, px, py, sbx * kBin, sby * kBin, lane);
+                         inside, px, py, sbx * kBin, sby * kBin, lane, s_key[w]);
         __syncthreads();
     }
     // merge the bin's four keys: the greater (depth, order) key wins, as in bin_cover
@@ -1844,15 +1938,16 @@ __global__ __launch_bounds__(1024) void kb_tile_heavy(BatchParams p, const uint3
     __shared__ uint32_t s_cnt[4];
     __shared__ float s_md[kHeavyWaves][64];
     __shared__ int32_t s_mb[kHeavyWaves][64];
+    __shared__ uint64_t s_key[kHeavyWaves][64];
     if (p.ctr->overflow & (OVF_PAIRS | OVF_TRIANGLES | OVF_CLIPVERTS)) return;
     const uint32_t items = p.ctr->n_heavy;
     for (uint32_t h = blockIdx.x; h < items; h += gridDim.x) {
         const uint32_t g = p.heavy[h];
         const uint32_t vi = upper_index(p.n_views, g, [&](uint32_t i) { return p.views[i].tile_base; });
         if (p.views[vi].depth_func == DEPTH_GE)
-            heavy_tile<true>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_cnt, s_md, s_mb);
+            heavy_tile<true>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_cnt, s_md, s_mb, s_key);
         else
-            heavy_tile<false>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_cnt, s_md, s_mb);
+            heavy_tile<false>(p, sorted_vals, g, vi, s_tri, s_mask, s_list, s_cnt, s_md, s_mb, s_key);
         __syncthreads();  // LDS is reused by the next tile
     }
 }
@@ -1947,6 +2042,7 @@ void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
     __shared__ TileLdsN<N> s_tri;
     __shared__ Mask s_mask[N];
     __shared__ __attribute__((aligned(8))) uint8_t s_list[TB * TB][N];
+    __shared__ uint64_t s_key[TB * TB][64];  // per wave: the triangle-parallel keys
 
     if (p.ctr->overflow & (OVF_TRIANGLES | OVF_CLIPVERTS)) return;  // flush failed; reported on the host
     // Tiles in kb_tile_order's order; the dispatcher deals consecutive blocks to the 8 XCDs in
@@ -1955,9 +2051,9 @@ void kb_tile(BatchParams p, const uint32_t *sorted_vals) {
     if (e.z == 0u) return;                      // an empty tile: nothing drawn, nothing counted
     const uint32_t g = e.x, vi = e.w;
     if (p.views[vi].depth_func == DEPTH_GE)
-        tile_body<true, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list);
+        tile_body<true, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list, s_key);
     else
-        tile_body<false, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list);
+        tile_body<false, N, TB>(p, sorted_vals, g, vi, e.y, e.z, s_tri, s_mask, s_list, s_key);
 }
 
 // Sum of the per-tile activity counts into the stats (one atomic per block).
