@@ -1353,21 +1353,32 @@ struct BinState {
 // Up to N staged records of a tile's list in LDS, one array per field: the triangle-parallel
 // screening reads a different record per lane, and with one array per field consecutive records
 // sit in consecutive banks (an 80-byte record stride would put 64 lanes on a handful of banks).
-// The rasteriser reads one record at a time as broadcasts.
+// The rasteriser reads one record at a time as broadcasts. A record's order key is its position
+// in the tile's list (the list is in submission order); the winner's position becomes its
+// triangle index once, after the walk.
 template <uint32_t N>
 struct TileLdsN {
     static constexpr uint32_t kN = N;
     int32_t a16[3][N], b16[3][N], c[3][N];
     float ooa[N], z0[N], dz1[N], dz2[N], hiz[N], cull[N];
-    uint32_t lat_x[N], lat_y[N], id[N];
+    uint32_t box[N];  // the snapped box's pixel corners relative to the tile origin (tile_box)
     uint16_t draw[N];
     uint8_t excl[N];  // edge slots the top-left rule excludes (0 under the reference's rule)
     static_assert(kMaxBatchDraws <= 65536, "draw indices of a flush fit 16 bits");
 };
 using TileLds = TileLdsN<256>;
 
+// The pixel corners of TriVis::lat_x/lat_y relative to the tile origin (ox, oy), each clamped to
+// [-1, 127] (the tile is at most 32 pixels wide, so the clamp keeps every comparison with its
+// bins), as bytes x0, x1, y0, y1.
+__device__ __forceinline__ uint32_t tile_box(const TriVis &t, int32_t ox, int32_t oy) {
+    const auto rel = [](int32_t v, int32_t o) { return (uint32_t)(uint8_t)(int8_t)imin(imax(v - o, -1), 127); };
+    return rel((int16_t)(t.lat_x & 0xFFFFu), ox) | (rel((int16_t)(t.lat_x >> 16), ox) << 8) |
+           (rel((int16_t)(t.lat_y & 0xFFFFu), oy) << 16) | (rel((int16_t)(t.lat_y >> 16), oy) << 24);
+}
+
 template <class Lds>
-__device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t, uint32_t id) {
+__device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t, int32_t ox, int32_t oy) {
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
         L.a16[e][k] = t.a16[e];
@@ -1380,9 +1391,7 @@ __device__ __forceinline__ void stage_record(Lds &L, uint32_t k, const TriVis &t
     L.dz2[k] = t.dz2;
     L.hiz[k] = t.hiz_depth;
     L.cull[k] = t.cull_bound;
-    L.lat_x[k] = t.lat_x;
-    L.lat_y[k] = t.lat_y;
-    L.id[k] = id;
+    L.box[k] = tile_box(t, ox, oy);
     L.draw[k] = (uint16_t)t.draw;
     L.excl[k] = (uint8_t)(t.draw >> 16);
 }
@@ -1419,12 +1428,12 @@ __device__ __forceinline__ void clear_seen(const BatchParams &p, const BinState 
 struct CoverRec {
     int32_t a16[3], b16[3], c[3];
     float ooa, z0, dz1, dz2;
-    int32_t id;
+    int32_t id;  // order key: the position in the tile's list
     uint32_t excl;
 };
 
 template <class Lds>
-__device__ __forceinline__ CoverRec load_cover(const Lds &L, uint32_t k) {
+__device__ __forceinline__ CoverRec load_cover(const Lds &L, uint32_t k, uint32_t base) {
     CoverRec r;
 #pragma unroll
     for (int e = 0; e < 3; ++e) {
@@ -1436,7 +1445,7 @@ __device__ __forceinline__ CoverRec load_cover(const Lds &L, uint32_t k) {
     r.z0 = L.z0[k];
     r.dz1 = L.dz1[k];
     r.dz2 = L.dz2[k];
-    r.id = (int32_t)L.id[k];
+    r.id = (int32_t)(base + k);  // the list position
     r.excl = L.excl[k];
     return r;
 }
@@ -1519,17 +1528,17 @@ __device__ __forceinline__ float depth_of_key(uint32_t hi, bool neg_zero) {
 
 // The batch's tiny survivors (lanes of T; lane j holds entry j's record slot k and the corners
 // [cx0, cx1] x [cy0, cy1] of its box, relative to the bin origin x0, y0) into the bin's state.
-// s_key holds this wave's 64 keys.
+// s_key holds this wave's 64 keys; slot k is list position base + k.
 template <bool GE, class Lds>
-__device__ __forceinline__ void tp_rasterise(const Lds &L, uint64_t *s_key, uint64_t T, uint32_t k, int32_t cx0,
-                                             int32_t cx1, int32_t cy0, int32_t cy1, BinState &st, int32_t x0,
-                                             int32_t y0, uint32_t lane) {
+__device__ __forceinline__ void tp_rasterise(const Lds &L, uint64_t *s_key, uint64_t T, uint32_t k, uint32_t base,
+                                             int32_t cx0, int32_t cx1, int32_t cy0, int32_t cy1, BinState &st,
+                                             int32_t x0, int32_t y0, uint32_t lane) {
     // the pixel's state before the batch; a NaN depth takes no update, so nothing may beat it
     s_key[lane] = (st.d != st.d) ? ~0ull : ((uint64_t)depth_key<GE>(st.d) << 32);
     __builtin_amdgcn_wave_barrier();
     const bool mine = (T >> lane) & 1ull;
     if (mine) {
-        const CoverRec r = load_cover(L, k);
+        const CoverRec r = load_cover(L, k, base);
         const uint32_t tag = (lane + 1u) << 1;
         int32_t x = cx0, y = cy0;
         for (;;) {
@@ -1567,7 +1576,7 @@ __device__ __forceinline__ void tp_rasterise(const Lds &L, uint64_t *s_key, uint
     const uint32_t kw = (uint32_t)__builtin_amdgcn_ds_bpermute((int)(j << 2), (int)k);  // lane j's slot
     if (won) {
         st.d = depth_of_key<GE>((uint32_t)(kk >> 32), (lo & 1u) != 0u);
-        st.best = (int32_t)L.id[kw];
+        st.best = (int32_t)(base + kw);
     }
     st.stale |= __ballot(won);
     __builtin_amdgcn_wave_barrier();  // s_key is rewritten by the next batch
@@ -1580,7 +1589,8 @@ __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c,
     return (int32_t)mad24(a16, x, mad24(b16, y, (uint32_t)c));
 }
 
-// Walk entries list[first + i * stride], i < n, of this wave's bin in order. Triangle-parallel
+// Walk entries list[first + i * stride], i < n, of this wave's bin in order (staged slot k is
+// list position base + k; the bin's origin is (x0, y0), (rx, ry) relative to the tile). Triangle-parallel
 // screening first: lane j takes entry i0 + j and tests, at once for up to 64 entries of the
 // current draw, the reference's hierarchical-Z reject (main.c:1005-1010), the conservative depth
 // cull and an edge that is negative over the whole bin (then no pixel of the bin is covered).
@@ -1588,8 +1598,9 @@ __device__ __forceinline__ int32_t edge_max(int32_t a16, int32_t b16, int32_t c,
 // that could not have written a pixel, so the result equals the one-by-one walk.
 template <bool GE, class Lds>
 __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, const uint8_t *list, uint32_t first,
-                                         uint32_t stride, uint32_t n, BinState &st, bool inside, int32_t px,
-                                         int32_t py, int32_t x0, int32_t y0, uint32_t lane, uint64_t *s_key) {
+                                         uint32_t stride, uint32_t n, uint32_t base, BinState &st, bool inside,
+                                         int32_t px, int32_t py, int32_t x0, int32_t y0, int32_t rx, int32_t ry,
+                                         uint32_t lane, uint64_t *s_key) {
     for (uint32_t i0 = 0; i0 < n;) {
         const uint32_t i = i0 + lane;
         const bool in = i < n;
@@ -1605,10 +1616,10 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, con
         }
         bool live = lane < run;
         // a pixel corner of the bin inside the triangle's bounding box: the bin's corners
-        // [bx0, bx1] x [by0, by1] in it
-        const uint32_t lx = L.lat_x[k], ly = L.lat_y[k];
-        const int32_t bx0 = imax((int16_t)(lx & 0xFFFFu), x0), bx1 = imin((int16_t)(lx >> 16), x0 + 7);
-        const int32_t by0 = imax((int16_t)(ly & 0xFFFFu), y0), by1 = imin((int16_t)(ly >> 16), y0 + 7);
+        // [bx0, bx1] x [by0, by1] in it (relative to the tile)
+        const uint32_t bw = L.box[k];
+        const int32_t bx0 = imax((int8_t)(bw & 0xFFu), rx), bx1 = imin((int8_t)((bw >> 8) & 0xFFu), rx + 7);
+        const int32_t by0 = imax((int8_t)((bw >> 16) & 0xFFu), ry), by1 = imin((int8_t)(bw >> 24), ry + 7);
         live = live && bx0 <= bx1 && by0 <= by1;
         const float hzd = L.hiz[k];
         live = live && !(GE ? (hzd < st.hz) : (hzd > st.hz));
@@ -1627,19 +1638,19 @@ __device__ __forceinline__ void walk_bin(const BatchParams &p, const Lds &L, con
             // tiny survivors triangle-parallel when there are enough of them; the rest below
             const uint64_t T = __ballot(live && (uint32_t)((bx1 - bx0 + 1) * (by1 - by0 + 1)) <= kTpCorners);
             if ((uint32_t)__popcll(T) >= kTpMin) {
-                tp_rasterise<GE>(L, s_key, T, k, bx0 - x0, bx1 - x0, by0 - y0, by1 - y0, st, x0, y0, lane);
+                tp_rasterise<GE>(L, s_key, T, k, base, bx0 - rx, bx1 - rx, by0 - ry, by1 - ry, st, x0, y0, lane);
                 S &= ~T;
             }
         }
         if (S) {
-            CoverRec ra = load_cover(L, next_survivor(k, S)), rb = ra;
+            CoverRec ra = load_cover(L, next_survivor(k, S), base), rb = ra;
             for (;;) {
                 bool more = S != 0ull;
-                if (more) rb = load_cover(L, next_survivor(k, S));
+                if (more) rb = load_cover(L, next_survivor(k, S), base);
                 bin_cover<GE>(ra, st, inside, px, py);
                 if (!more) break;
                 more = S != 0ull;
-                if (more) ra = load_cover(L, next_survivor(k, S));
+                if (more) ra = load_cover(L, next_survivor(k, S), base);
                 bin_cover<GE>(rb, st, inside, px, py);
                 if (!more) break;
             }
@@ -1769,7 +1780,7 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
         if (tid < n) {
             const uint32_t t = ovf ? base + tid : sorted_vals[lstart + base + tid];
             const TriVis tv = p.tvis[t];
-            stage_record(s_tri, tid, tv, t);
+            stage_record(s_tri, tid, tv, (int32_t)(TB * kBin) * tx, (int32_t)(TB * kBin) * ty);
             int32_t bx0, by0, bx1, by1;
             rect_of(tv, bx0, by0, bx1, by1);
             uint32_t m = 0u;
@@ -1793,10 +1804,13 @@ __device__ __forceinline__ void tile_body(const BatchParams &p, const uint32_t *
             cnt += (uint32_t)__popcll(bal);
         }
         if (sub_valid && cnt)
-            walk_bin<GE>(p, s_tri, s_list[w], 0u, 1u, cnt, st, inside, px, py, sbx * kBin, sby * kBin, lane, s_key[w]);
+            walk_bin<GE>(p, s_tri, s_list[w], 0u, 1u, cnt, base, st, inside, px, py, sbx * kBin, sby * kBin,
+                         (int32_t)((w % TB) * kBin), (int32_t)((w / TB) * kBin), lane, s_key[w]);
         __syncthreads();
     }
 
+    // the winner's list position to its assembled triangle
+    if (st.best >= 0) st.best = ovf ? st.best : (int32_t)sorted_vals[lstart + (uint32_t)st.best];
     if (st.best >= 0) shade_winner(p, V, (uint32_t)st.best, st.d, px, py, pix);
     if (p.foot) {  // footprint counting: pixels written
         const uint64_t m = __ballot(st.best >= 0);
@@ -1870,7 +1884,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
         if (tid < n) {
             const uint32_t t = sorted_vals[lstart + base + tid];
             const TriVis tv = p.tvis[t];
-            stage_record(s_tri, tid, tv, t);
+            stage_record(s_tri, tid, tv, 16 * tx, 16 * ty);
             int32_t bx0, by0, bx1, by1;
             rect_of(tv, bx0, by0, bx1, by1);
             uint32_t m = 0u;
@@ -1898,8 +1912,9 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
         const uint32_t cnt = s_cnt[bin];
         if (cnt) seen = true;
         if (sub_valid && q < cnt)
-            walk_bin<GE>(p, s_tri, s_list[bin], q, kWavesPerBin, (cnt - q + kWavesPerBin - 1u) / kWavesPerBin, st,
-                         inside, px, py, sbx * kBin, sby * kBin, lane, s_key[w]);
+            walk_bin<GE>(p, s_tri, s_list[bin], q, kWavesPerBin, (cnt - q + kWavesPerBin - 1u) / kWavesPerBin, base,
+                         st, inside, px, py, sbx * kBin, sby * kBin, (int32_t)((bin & 1u) * kBin),
+                         (int32_t)((bin >> 1) * kBin), lane, s_key[w]);
         __syncthreads();
     }
     // merge the bin's four keys: the greater (depth, order) key wins, as in bin_cover
@@ -1917,6 +1932,7 @@ __device__ __forceinline__ void heavy_tile(const BatchParams &p, const uint32_t 
             st.best = bq;
         }
     }
+    if (st.best >= 0) st.best = (int32_t)sorted_vals[lstart + (uint32_t)st.best];  // list position to triangle
     if (st.best >= 0) shade_winner(p, V, (uint32_t)st.best, st.d, px, py, pix);
     if (p.foot) {  // footprint counting: pixels written
         const uint64_t m = __ballot(st.best >= 0);
