@@ -836,6 +836,9 @@ __device__ __forceinline__ void add_unbinned_pairs(const BatchParams &p, uint32_
 
 __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     __shared__ unsigned long long s_w[5];
+    __shared__ unsigned long long s_ex[256];
+    __shared__ uint32_t s_nw[4];
+    __shared__ uint8_t s_list[256];
     const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x);
     const uint32_t slot = b * 256u + threadIdx.x;
     const uint32_t bd = p.blk_draw[b];  // the count pass found the block's draw
@@ -848,28 +851,14 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     const uint32_t di = bd & ~kFirstOfDraw;
     const DrawRec &d = p.draws[di];
     const ViewRec v = p.views[d.view];  // a copy: read here, before the first store
-    const uint32_t t = slot - d.tbase;
     const unsigned long long bo = p.blockoff[b];                       // binned | assembled << 32
     const uint32_t draw_asm0 = (uint32_t)(p.blockoff[d.blk_t0] >> 32);  // the draw's first assembled
     const uint32_t packed = p.cnt[slot];
     const uint32_t n_bin = packed & 0xFFu, n_asm = (packed >> 8) & 0xFFu;
     const bool plain = n_asm == 1u && (packed >> 16) == 0u;  // assembled without the clipper
     const uint32_t n_rows = (!plain && n_bin) ? packed >> 16 : 0u;  // a clipped polygon's vertex rows
-    // the triangle is set up before its place is known, so its loads and arithmetic overlap the
-    // block scan; every load comes before the first store (see kb_vertex)
-    TriVis tv;
-    float rw[3];
-    uint32_t s0 = 0u, s1 = 0u, s2 = 0u;
-    uint32_t vi[3] = {0u, 0u, 0u};
     // only binned triangles are written (kb_setup_count marked the bins of the others)
     const bool emit = plain && n_bin != 0u;
-    if (emit) load_tri_indices(d, t, vi);
-    if (emit) {
-        setup_triangle(d, v, p.vpos[d.vbase + vi[0]], p.vpos[d.vbase + vi[1]], p.vpos[d.vbase + vi[2]], tv, rw);
-        s0 = d.abase + vi[0];  // the attribute rows the pixel shader reads (DrawRec::abase)
-        s1 = d.abase + vi[1];
-        s2 = d.abase + vi[2];
-    }
     if (threadIdx.x == 0u) {
         // a draw's first binned index; empty draws start where the next one does
         if (b == d.blk_t0)
@@ -899,11 +888,35 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
         __syncthreads();
         if (n_rows) p.clip_rows[slot] = s_rows + (uint32_t)(ex >> 32);
     }
-    if (n_asm == 0u) return;
-    const uint32_t off_bin = (uint32_t)(bo & 0xFFFFFFFFull) + (uint32_t)(ex & 0xFFFFu);
-    const uint32_t num = (uint32_t)(bo >> 32) + (uint32_t)((ex >> 16) & 0xFFFFu) - draw_asm0;
-    if (!plain) p.clip_off[slot] = make_uint2(off_bin, num);
-    else if (n_bin) write_triangle(p, off_bin, di, v, tv, rw, s0, s1, s2, t, num);
+    if (!plain && n_asm)
+        p.clip_off[slot] = make_uint2((uint32_t)(bo & 0xFFFFFFFFull) + (uint32_t)(ex & 0xFFFFu),
+                                      (uint32_t)(bo >> 32) + (uint32_t)((ex >> 16) & 0xFFFFu) - draw_asm0);
+    // The triangles to write, packed into the block's first waves: where few of a block's
+    // triangles are binned (small triangles without a pixel corner, back faces), the set-up work
+    // runs on one or two waves instead of four.
+    const uint32_t lane = threadIdx.x & 63u, w = threadIdx.x >> 6;
+    const uint64_t bal = __ballot(emit);
+    if (lane == 0u) s_nw[w] = (uint32_t)__popcll(bal);
+    s_ex[threadIdx.x] = ex;
+    __syncthreads();
+    uint32_t before = 0u;
+    for (uint32_t k = 0; k < w; ++k) before += s_nw[k];
+    const uint32_t n_emit = s_nw[0] + s_nw[1] + s_nw[2] + s_nw[3];
+    if (emit) s_list[before + mbcnt(bal)] = (uint8_t)threadIdx.x;
+    __syncthreads();
+    if (threadIdx.x >= n_emit) return;
+    const uint32_t tl = s_list[threadIdx.x];
+    const uint32_t tt = b * 256u + tl - d.tbase;  // the triangle within its draw
+    const unsigned long long ext = s_ex[tl];
+    uint32_t vi[3];
+    load_tri_indices(d, tt, vi);
+    TriVis tv;
+    float rw[3];
+    setup_triangle(d, v, p.vpos[d.vbase + vi[0]], p.vpos[d.vbase + vi[1]], p.vpos[d.vbase + vi[2]], tv, rw);
+    const uint32_t off_bin = (uint32_t)(bo & 0xFFFFFFFFull) + (uint32_t)(ext & 0xFFFFu);
+    const uint32_t num = (uint32_t)(bo >> 32) + (uint32_t)((ext >> 16) & 0xFFFFu) - draw_asm0;
+    // the attribute rows the pixel shader reads (DrawRec::abase)
+    write_triangle(p, off_bin, di, v, tv, rw, d.abase + vi[0], d.abase + vi[1], d.abase + vi[2], tt, num);
 }
 
 // The clipped triangles' output, one per 16-lane group: lane k writes polygon vertex k's attribute
