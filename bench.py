@@ -404,6 +404,30 @@ def bench_c5(dev, args):
         tuning = dev.autotune_end(repeats=1)
     print(f"[bench] c5 {'set' if args.tuning else 'autotuned'}: {tuning}", file=sys.stderr, flush=True)
     ms_b = timed(False, args.c5_steps)
+    # the host's side alone: the frame's calls (16 k draws through ctypes) timed on the host, the
+    # GPU running behind them
+    dev.synchronize()
+    t0 = time.perf_counter()
+    frame(False)
+    dev.flush()
+    host_ms = (time.perf_counter() - t0) * 1e3
+    dev.synchronize()
+    # the same frame captured once as a HIP graph and replayed: the GPU's time, no host issue
+    dev.graph_capture_begin()
+    frame(False)
+    graph = dev.graph_capture_end()
+    graph.launch()
+    dev.synchronize()
+    ga, gb = ev.create(), ev.create()
+    ev.record(ga, dev.stream)
+    for _ in range(args.c5_steps):
+        graph.launch()
+    ev.record(gb, dev.stream)
+    ms_g = ev.elapsed_ms(ga, gb) / args.c5_steps
+    dev.synchronize()
+    graph.release()
+    print(f"[bench] c5 forward: {ms_b:.3f} ms eager, host issue {host_ms:.3f} ms, graph replay {ms_g:.3f} ms",
+          file=sys.stderr, flush=True)
     if args.no_per_draw:  # profiling runs of the batched kernels
         ms_p = float("nan")
     else:
@@ -421,6 +445,8 @@ def bench_c5(dev, args):
                        f"shading (texture mip chain x vertex colour x SH9, sRGB) + L2 loss + adjoint to every "
                        f"mesh's positions/colours/uvs/normals/texels and the SH light",
            "fwd_batched_ms": round(ms_b, 3), "fwd_per_draw_ms": round(ms_p, 3),
+           "fwd_host_issue_ms": round(host_ms, 3), "fwd_graph_ms": round(ms_g, 3),
+           "fwd_graph_views_per_s": round(n / (ms_g * 1e-3), 2),
            "fwd_batched_views_per_s": round(n / (ms_b * 1e-3), 2),
            "fwd_per_draw_views_per_s": round(n / (ms_p * 1e-3), 2),
            "fwd_batched_over_per_draw": round(ms_p / ms_b, 2), "draws": n * len(sc.meshes),

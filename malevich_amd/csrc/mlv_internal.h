@@ -138,7 +138,8 @@ struct alignas(16) TriShade {
 };
 static_assert(sizeof(TriShade) == 32, "TriShade layout");
 
-constexpr size_t kMaxBatchDraws = 1u << 16;  // draws per flush; the tile kernels keep draw indices in 16 bits
+constexpr size_t kMaxBatchDraws = 1u << 16;
+constexpr uint32_t kBlockDrawTable = 256;  // draws per flush from which blocks look their draw up in a table  // draws per flush; the tile kernels keep draw indices in 16 bits
 
 struct BatchCounters {
     unsigned long long scan_total;         // setup scan total: binned | assembled << 32
@@ -186,6 +187,9 @@ struct BatchParams {
     unsigned long long *blocksum, *blockoff;  // per setup block: assembled | clipverts << 32
     uint32_t *asm_base;                   // per draw: its first binned triangle's index
     uint32_t *blk_draw;                   // per setup block: its draw | kFirstOfDraw (kb_setup_count writes it)
+    // per vertex / setup block: its draw (kb_block_draws writes them), for flushes of at least
+    // kBlockDrawTable draws; null for fewer, where a binary search over the draws is short
+    uint32_t *vblk_draw, *tblk_draw;
     uint32_t *cmark;                      // (draw, bin) bits of unbinned triangles (NULL: all are binned)
     uint32_t cmark_words;
     uint2 *clip_list;       // (triangle slot, draw) of each triangle the clipper takes

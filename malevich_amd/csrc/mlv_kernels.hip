@@ -557,6 +557,18 @@ __device__ __forceinline__ void run_vs(const BatchParams &p, const VsDraw &u, ui
     p.vpos[slot] = r0;
 }
 
+// Each vertex-stage and setup block's draw, one workgroup per draw: the vertex and setup kernels
+// then find their draw with one load instead of a binary search over the draws' first blocks
+// (a chain of dependent loads at the head of every block: eleven for 2,048 draws). Launched for
+// flushes of kBlockDrawTable draws or more; below, the search is shorter than the extra launch.
+__global__ __launch_bounds__(256) void kb_block_draws(BatchParams p) {
+    const uint32_t d = blockIdx.x;
+    const DrawRec &r = p.draws[d];
+    const uint32_t nv = (r.vertex_count + 255u) / 256u, nt = (r.tri_count + 255u) / 256u;
+    for (uint32_t i = threadIdx.x; i < nv; i += 256u) p.vblk_draw[r.blk_v0 + i] = d;
+    for (uint32_t i = threadIdx.x; i < nt; i += 256u) p.tblk_draw[r.blk_t0 + i] = d;
+}
+
 // A flush's counters, tile ranges and scan status words start at zero.
 __device__ __forceinline__ void reset_flush(const BatchParams &p) {
     if (blockIdx.x == 0 && threadIdx.x == 0) {
@@ -576,10 +588,12 @@ __device__ __forceinline__ void reset_flush(const BatchParams &p) {
 // load that no store of the kernel can precede is read through the scalar cache (the draw
 // search, the draw record and the constant buffer), and after a store it would be a vector
 // load on the wave's critical path.
+template <bool kTable>  // the block's draw from BatchParams::vblk_draw, else by a binary search
 __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
     const uint32_t b = blockIdx.x;
     if (b < p.n_vblocks) {
-        const uint32_t di = upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_v0; });
+        const uint32_t di = kTable ? p.vblk_draw[b]
+                                   : upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_v0; });
         const DrawRec &d = p.draws[di];
         const uint32_t i = (b - d.blk_v0) * 256u + threadIdx.x;
         const int vs = d.vs;
@@ -677,6 +691,7 @@ __device__ __forceinline__ void mark_unbinned(const BatchParams &p, const uint32
         atomicAdd(&p.stats->pairs8_part[blockIdx.x & 63u], (unsigned long long)(s_p8[0] + s_p8[1] + s_p8[2] + s_p8[3]));
 }
 
+template <bool kTable>  // the blocks' draws from BatchParams::tblk_draw, else by a binary search
 __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
     __shared__ uint32_t s_w[kCountPer][4];
     const uint32_t hb = xcd_contiguous(blockIdx.x, gridDim.x);
@@ -688,7 +703,8 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
         const uint32_t b = hb * kCountPer + h;
         inb[h] = b < p.n_tblocks;  // the grid's last workgroup may hold one block
         const uint32_t bc = inb[h] ? b : p.n_tblocks - 1u;
-        di[h] = upper_index(p.n_draws, bc, [&](uint32_t i) { return p.draws[i].blk_t0; });
+        di[h] = kTable ? p.tblk_draw[bc]
+                       : upper_index(p.n_draws, bc, [&](uint32_t i) { return p.draws[i].blk_t0; });
         const DrawRec &d = p.draws[di[h]];
         const uint32_t t = (bc - d.blk_t0) * 256u + threadIdx.x;
         slot[h] = bc * 256u + threadIdx.x;
@@ -2239,7 +2255,7 @@ uint32_t tile_kernel_lds(uint32_t tile_stage, uint32_t shift) {
 
 uint32_t setup_kernel_lds() {
     hipFuncAttributes a{}, b{};
-    if (hipFuncGetAttributes(&a, (const void *)kb_setup_count) != hipSuccess) return 0u;
+    if (hipFuncGetAttributes(&a, (const void *)kb_setup_count<false>) != hipSuccess) return 0u;
     if (hipFuncGetAttributes(&b, (const void *)kb_setup_emit) != hipSuccess) return 0u;
     return (uint32_t)std::max(a.sharedSizeBytes, b.sharedSizeBytes);
 }
@@ -2249,13 +2265,24 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
     hipError_t e;
     switch (stage) {
     case 0:
-        hipLaunchKernelGGL(kb_vertex, dim3(std::max(p.n_vblocks, std::min((p.total_tiles + 255u) / 256u + 1u, 1024u))),
-                           dim3(256), 0, s, p);
+    {
+        const dim3 grid(std::max(p.n_vblocks, std::min((p.total_tiles + 255u) / 256u + 1u, 1024u)));
+        if (p.vblk_draw) {
+            hipLaunchKernelGGL(kb_block_draws, dim3(p.n_draws), dim3(256), 0, s, p);
+            hipLaunchKernelGGL(kb_vertex<true>, grid, dim3(256), 0, s, p);
+        } else {
+            hipLaunchKernelGGL(kb_vertex<false>, grid, dim3(256), 0, s, p);
+        }
         break;
+    }
     case 1:
         if (!p.n_tblocks) break;
-        hipLaunchKernelGGL(kb_setup_count, dim3((p.n_tblocks + kCountPer - 1u) / kCountPer), dim3(256), p.setup_lds_pad, s,
-                           p);
+        if (p.tblk_draw)
+            hipLaunchKernelGGL(kb_setup_count<true>, dim3((p.n_tblocks + kCountPer - 1u) / kCountPer), dim3(256),
+                               p.setup_lds_pad, s, p);
+        else
+            hipLaunchKernelGGL(kb_setup_count<false>, dim3((p.n_tblocks + kCountPer - 1u) / kCountPer), dim3(256),
+                               p.setup_lds_pad, s, p);
         hipLaunchKernelGGL(kb_clip_count, dim3(1024), dim3(256), 0, s, p);
         if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc,
                                                  &p.ctr->scan_total, s)) != hipSuccess)

@@ -169,6 +169,7 @@ struct mlv_context {
     uint64_t pending_tris = 0, pending_verts = 0;
     // arenas
     Arena blk_draw;  // per setup block: its draw (written by the count pass for the emit pass)
+    Arena vblk_draw, tblk_draw;  // per vertex / setup block: its draw (the flush's first kernel)
     Arena draws_dev, views_dev, vpos, vattr, cnt, blocksum, blockoff, asm_base, clip_list, clip_off, clip_rows, tvis,
         tshade, tbin;
     Arena pair_cnt, pair_off, keys0, vals0, keys1, vals1, tile_cnt, tile_start, scan_part, radix_hist, heavy, tile_order;
@@ -482,6 +483,8 @@ static int flush(mlv_context *ctx) {
     RC_OR_RETURN(ensure(ctx, ctx->blockoff, (size_t)(n_tblocks + 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->asm_base, (size_t)n_draws * 4));
     RC_OR_RETURN(ensure(ctx, ctx->blk_draw, (size_t)(n_tblocks + 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->vblk_draw, (size_t)(n_vblocks + 1) * 4));
+    RC_OR_RETURN(ensure(ctx, ctx->tblk_draw, (size_t)(n_tblocks + 1) * 4));
     RC_OR_RETURN(ensure(ctx, ctx->clip_list, std::max<size_t>(T, 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->clip_off, std::max<size_t>(T, 1) * 8));
     RC_OR_RETURN(ensure(ctx, ctx->clip_rows, std::max<size_t>(T, 1) * 4));
@@ -549,6 +552,8 @@ static int flush(mlv_context *ctx) {
     p.blockoff = (unsigned long long *)ctx->blockoff.p;
     p.asm_base = (uint32_t *)ctx->asm_base.p;
     p.blk_draw = (uint32_t *)ctx->blk_draw.p;
+    p.vblk_draw = n_draws >= kBlockDrawTable ? (uint32_t *)ctx->vblk_draw.p : nullptr;
+    p.tblk_draw = n_draws >= kBlockDrawTable ? (uint32_t *)ctx->tblk_draw.p : nullptr;
     p.cmark = cull ? (uint32_t *)ctx->cmark.p : nullptr;
     p.cmark_words = cull ? (uint32_t)cwords : 0u;
     p.clip_list = (uint2 *)ctx->clip_list.p;
@@ -745,7 +750,8 @@ MLV_API int mlv_context_destroy(mlv_context *ctx) {
     (void)hipSetDevice(ctx->device);
     (void)flush(ctx);
     (void)hipStreamSynchronize(ctx->stream);
-    Arena *all[] = {&ctx->blk_draw,  &ctx->draws_dev, &ctx->views_dev, &ctx->vpos,     &ctx->vattr,     &ctx->cnt,
+    Arena *all[] = {&ctx->blk_draw,  &ctx->vblk_draw, &ctx->tblk_draw, &ctx->draws_dev, &ctx->views_dev,
+                    &ctx->vpos,      &ctx->vattr,     &ctx->cnt,
                     &ctx->blocksum,  &ctx->blockoff,  &ctx->asm_base, &ctx->clip_list, &ctx->clip_off, &ctx->clip_rows,
                     &ctx->tvis,      &ctx->tshade,    &ctx->tbin,    &ctx->pair_cnt, &ctx->pair_off,  &ctx->keys0,
                     &ctx->vals0,     &ctx->keys1,     &ctx->vals1,    &ctx->tile_cnt,  &ctx->tile_start,

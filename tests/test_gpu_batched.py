@@ -62,8 +62,15 @@ def render_views(dev, sc, per_draw: bool):
             k.release()
 
 
-def test_batched_views_equal_per_draw_and_oracle(device, oracle):
-    sc = S.batched_scene(n_views=6, n_meshes=5, min_tris=200, max_tris=6000, resolutions=(64, 96, 128),
+# (views, meshes, min/max triangles): 30 draws in one flush, and 320 (a flush of 256 draws or
+# more looks each block's draw up in a table filled on the device instead of searching)
+SCENES = {"few_draws": (6, 5, 200, 6000), "many_draws": (4, 80, 20, 400)}
+
+
+@pytest.mark.parametrize("scene", list(SCENES))
+def test_batched_views_equal_per_draw_and_oracle(device, oracle, scene):
+    nv, nm, lo, hi = SCENES[scene]
+    sc = S.batched_scene(n_views=nv, n_meshes=nm, min_tris=lo, max_tris=hi, resolutions=(64, 96, 128),
                          tex_sizes=(32, 64), seed=8)
     print(f"case: batched {len(sc.views)} views x {len(sc.meshes)} meshes", flush=True)
     fb, sb = render_views(device, sc, per_draw=False)
