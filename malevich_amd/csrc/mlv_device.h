@@ -460,6 +460,14 @@ __device__ __forceinline__ int classify(float4 a, float4 b, float4 c) {
 
 // Projection, viewport transform, snapping, area, cull and setup for one fan triangle
 // (main.c:799-898). Returns false when back-face culled (signed_area > 0).
+// kInside: the triangle passed the frustum's inside test (main.c:775-781), so w > 0 and the
+// normalised x, y, z are finite whenever 1/w is. The viewport matrix has zeros off its diagonal
+// and last column (screen_matrix), and a product with a zero entry is then a signed zero, which
+// leaves a nonzero running sum unchanged; so x and y are formed from their two nonzero terms in
+// the same order, and can differ from the four-term sum only in the sign of a zero result, which
+// snaps to 0 either way (an infinite 1/w gives NaN or infinity both ways, which snap alike). z
+// keeps all four terms: its sign of zero reaches the depth buffer.
+template <bool kInside = false>
 __device__ __forceinline__ bool setup_triangle(const DrawRec &p, const ViewRec &v, float4 q0, float4 q1, float4 q2,
                                                TriVis &tv, float rw[3]) {
     float4 q[3] = {q0, q1, q2};
@@ -469,8 +477,13 @@ __device__ __forceinline__ bool setup_triangle(const DrawRec &p, const ViewRec &
     for (int k = 0; k < 3; ++k) {
         rw[k] = 1.0f / q[k].w;  // (f32)(1.0/(f64)w) equals the f32 quotient
         float X = q[k].x * rw[k], Y = q[k].y * rw[k], Z = q[k].z * rw[k], Wn = q[k].w * rw[k];
-        sx[k] = dot4_seq(p.screen[0][0], p.screen[0][1], p.screen[0][2], p.screen[0][3], X, Y, Z, Wn);
-        sy[k] = dot4_seq(p.screen[1][0], p.screen[1][1], p.screen[1][2], p.screen[1][3], X, Y, Z, Wn);
+        if (kInside) {
+            sx[k] = p.screen[0][0] * X + p.screen[0][3] * Wn;
+            sy[k] = p.screen[1][1] * Y + p.screen[1][3] * Wn;
+        } else {
+            sx[k] = dot4_seq(p.screen[0][0], p.screen[0][1], p.screen[0][2], p.screen[0][3], X, Y, Z, Wn);
+            sy[k] = dot4_seq(p.screen[1][0], p.screen[1][1], p.screen[1][2], p.screen[1][3], X, Y, Z, Wn);
+        }
         sz[k] = dot4_seq(p.screen[2][0], p.screen[2][1], p.screen[2][2], p.screen[2][3], X, Y, Z, Wn);
         x[k] = snap28_4(sx[k]);
         y[k] = snap28_4(sy[k]);

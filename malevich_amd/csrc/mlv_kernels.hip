@@ -755,7 +755,7 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
             if (cls[h] == 1) {
                 TriVis tv;
                 float rw[3];
-                if (setup_triangle(d, p.views[d.view], q[h][0], q[h][1], q[h][2], tv, rw)) {
+                if (setup_triangle<true>(d, p.views[d.view], q[h][0], q[h][1], q[h][2], tv, rw)) {
                     unb[h] = p.cmark && !has_corner(tv);
                     packed[h] = unb[h] ? 0x10000u : 0x10001u;
                     r0[h] = tv.rect0;
@@ -928,7 +928,7 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     load_tri_indices(d, tt, vi);
     TriVis tv;
     float rw[3];
-    setup_triangle(d, v, p.vpos[d.vbase + vi[0]], p.vpos[d.vbase + vi[1]], p.vpos[d.vbase + vi[2]], tv, rw);
+    setup_triangle<true>(d, v, p.vpos[d.vbase + vi[0]], p.vpos[d.vbase + vi[1]], p.vpos[d.vbase + vi[2]], tv, rw);
     const uint32_t off_bin = (uint32_t)(bo & 0xFFFFFFFFull) + (uint32_t)(ext & 0xFFFFu);
     const uint32_t num = (uint32_t)(bo >> 32) + (uint32_t)((ext >> 16) & 0xFFFFu) - draw_asm0;
     // the attribute rows the pixel shader reads (DrawRec::abase)

@@ -21,7 +21,9 @@ pytestmark = pytest.mark.gpu
 N = 64
 
 
-def _tiled(seed, x0, y0, x1, y1, step):
+def _tiled(seed, x0, y0, x1, y1, step, flat_depths=None):
+    """Squares of `step` pixels split into two triangles each (either diagonal, either winding),
+    corners on pixel corners; depths random per vertex, or one of `flat_depths` per triangle."""
     rng = np.random.default_rng(seed)
     verts = []
     for y in range(y0, y1, step):
@@ -31,7 +33,8 @@ def _tiled(seed, x0, y0, x1, y1, step):
             for h in halves:
                 h = [(16 * p[0], 16 * p[1]) for p in h]
                 h = h if rng.integers(2) else [h[0], h[2], h[1]]
-                v = passthrough_vertices(h, z=tuple(rng.uniform(0.2, 0.8, 3)), size=N)
+                z = tuple(rng.uniform(0.2, 0.8, 3)) if flat_depths is None else (float(rng.choice(flat_depths)),) * 3
+                v = passthrough_vertices(h, z=z, size=N)
                 v[:, 4:7] = rng.uniform(0, 1, 3)
                 verts.append(v)
     v = np.concatenate(verts)
@@ -53,6 +56,27 @@ def test_tiled_rectangle(device, oracle, cull, fill):
         want = ((xx >= x0) & (xx < x1) & (yy >= y0) & (yy < y1)) if fill == L.FILL_TOP_LEFT else \
                ((xx >= x0) & (xx <= x1) & (yy >= y0) & (yy <= y1))
         np.testing.assert_array_equal(g.vis_tri != 0xFFFFFFFF, want)
+
+
+@pytest.mark.parametrize("depth_func,fill", [(0, L.FILL_INCLUSIVE), (1, L.FILL_INCLUSIVE), (0, L.FILL_TOP_LEFT),
+                                             (1, L.FILL_TOP_LEFT)])
+def test_tiny_triangle_layers(device, oracle, depth_func, fill):
+    """Three draws tiling the target with 2-pixel triangles, each triangle at one of three flat
+    depths (so equal depths meet and the later triangle must win them): every bin walks long runs
+    of survivors with a handful of pixel corners each, which the tile kernel rasterises one
+    triangle per lane and merges per pixel."""
+    draws = []
+    for layer in range(3):
+        v, idx = _tiled(100 + layer, 0, 0, N, N, 2, flat_depths=(0.3, 0.5, 0.7))
+        draws.append(DrawSpec(idx, v, "passthrough", "passthrough", cull=L.CULL_NONE, fill=fill))
+    clear = 0.0 if depth_func == 0 else 1.0
+    gpu_func = L.DEPTH_GREATER_EQUAL if depth_func == 0 else L.DEPTH_LESS_EQUAL
+    print(f"case: tiny triangle layers depth_func={depth_func} fill={fill}, 3 x {len(draws[0].indices) // 3} "
+          f"triangles", flush=True)
+    g = render_gpu(device, N, N, draws, (0, 0, 0, 0), clear_depth=clear, depth_func=gpu_func)
+    o = render_oracle(oracle, N, N, draws, (0, 0, 0, 0), clear_depth=clear, depth_func=depth_func)
+    assert_frames_match(g, o, color_tol=0)
+    assert (g.vis_tri != 0xFFFFFFFF).all()
 
 
 @pytest.mark.parametrize("cull,fill", [(L.CULL_FRONT, L.FILL_INCLUSIVE), (L.CULL_NONE, L.FILL_TOP_LEFT)])
