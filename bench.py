@@ -681,8 +681,8 @@ def main(argv=None):
     global NO_FOOTPRINT
     NO_FOOTPRINT = args.no_footprint
     if args.gpus != 1:
-        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); the multi-GPU sharding "
-              "design is described in DESIGN.md and is not launched here.", file=sys.stderr)
+        print(f"bench.py runs on one GPU only (asked for --gpus {args.gpus}); sharding the path over "
+              "several GPUs is a design that is not launched here.", file=sys.stderr)
         sys.exit(2)
 
     from malevich_amd import pipeline as P

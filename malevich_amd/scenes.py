@@ -331,7 +331,7 @@ def binning_stress(n_tiny: int, n_big: int, width: int, height: int, seed: int,
 
 
 def c1_scene(width: int = 640, height: int = 360, seed: int = 1):
-    """sphere of 10,242 vertices / 20,480 triangles, camera at distance 3,
+    """The 640x360 sphere workload: 10,242 vertices / 20,480 triangles, camera at distance 3,
     60 deg vertical FOV, 512x512 RGBA8 random texture, basic_vs + basic_ps."""
     mesh = sphere_mesh(32)
     cb = camera_cb(eye=(3.0, 0.0, 0.0), target=(0.0, 0.0, 0.0), fov_y_deg=60.0, width=width, height=height)
@@ -384,7 +384,7 @@ class C2Scene:
 
 def c2_scene(width: int = 1920, height: int = 1080, n_instances: int = 64, seed: int = 2,
              texture_size: int = 1024, target_complexity: float = 4.0) -> C2Scene:
-    """seeded displaced spheres of 16k-64k triangles (geodesic
+    """The 1080p forward workload: seeded displaced spheres of 16k-64k triangles (geodesic
     frequency 29..56, about 2M triangles for 64 instances) at random positions in a
     20-unit cube, random rotations, scale 0.5-2 times a common factor chosen so the
     summed projected sphere area is about 4 screens (mean depth complexity 4)."""
