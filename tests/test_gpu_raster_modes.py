@@ -79,6 +79,29 @@ def test_tiny_triangle_layers(device, oracle, depth_func, fill):
     assert (g.vis_tri != 0xFFFFFFFF).all()
 
 
+@pytest.mark.parametrize("depth_func", [0, 1])
+@pytest.mark.parametrize("path", ["heavy", "wide"])
+def test_tiny_triangles_heavy_and_wide_tiles(device, oracle, depth_func, path):
+    """One draw of 2-pixel triangles at three flat depths (every tile list holds one draw), walked
+    by the heavy-tile path (four waves per bin, their keys merged) or by 32x32 tiles, with the
+    triangle-parallel merge in both."""
+    v, idx = _tiled(200, 0, 0, N, N, 2, flat_depths=(0.3, 0.5, 0.7))
+    d = DrawSpec(idx, v, "passthrough", "passthrough", cull=L.CULL_NONE)
+    clear = 0.0 if depth_func == 0 else 1.0
+    gpu_func = L.DEPTH_GREATER_EQUAL if depth_func == 0 else L.DEPTH_LESS_EQUAL
+    keep = device.tuning()
+    t = dict(keep, heavy_tile_pairs=0, tile_edge=16) if path == "heavy" else \
+        dict(keep, heavy_tile_pairs=0xFFFFFFFF, tile_edge=32)
+    print(f"case: tiny triangles {path} depth_func={depth_func}, {len(idx) // 3} triangles", flush=True)
+    try:
+        device.set_tuning(**t)
+        g = render_gpu(device, N, N, [d], (0, 0, 0, 0), clear_depth=clear, depth_func=gpu_func)
+    finally:
+        device.set_tuning(**keep)
+    o = render_oracle(oracle, N, N, [d], (0, 0, 0, 0), clear_depth=clear, depth_func=depth_func)
+    assert_frames_match(g, o, color_tol=0)
+
+
 @pytest.mark.parametrize("cull,fill", [(L.CULL_FRONT, L.FILL_INCLUSIVE), (L.CULL_NONE, L.FILL_TOP_LEFT)])
 def test_sphere_modes(device, oracle, cull, fill):
     W, H = 160, 96
