@@ -1,6 +1,7 @@
 // mlv_kernels.hip -- the draw pipeline as gfx950 kernels, one pass per stage per flush.
 //
-// Stage 0, vertex:   kb_vertex           one lane per vertex slot of every draw (main.c:662-734),
+// Stage 0, vertex:   kb_block_draws      each vertex / setup block's draw (flushes of many draws)
+//                    kb_vertex           one lane per vertex slot of every draw (main.c:662-734),
 //                                        and the flush's counters cleared
 // Stage 1, setup:    kb_setup_count      one lane per input triangle: cull/classify, output count
 //                    kb_clip_count       triangles that need the clipper (main.c:609-660)
@@ -15,7 +16,8 @@
 //                    kb_tile_ranges      each tile's [start, end) read off the sorted keys
 // Stage 3, tile:     kb_tile             one block per 16x16 tile of every view, one wave per
 //                                        8x8 bin: hierarchical-Z, corner-sampled coverage,
-//                                        early-Z in registers, the pixel shader of the surviving
+//                                        early-Z in registers (tiny triangles one per lane, merged
+//                                        by LDS maxima), the pixel shader of the surviving
 //                                        fragment and the output merger (main.c:983-1189)
 //
 // Visibility per pixel is the maximum of (depth, submission order) under the reference's
