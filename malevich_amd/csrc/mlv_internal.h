@@ -174,6 +174,7 @@ struct DevStats {
 
 struct BatchParams {
     const DrawRec *draws;
+    const uint32_t *draw_v0, *draw_t0;    // DrawRec::blk_v0 / blk_t0 of every draw, dense
     const ViewRec *views;
     uint32_t n_draws, n_views;
     uint32_t n_vblocks, n_tblocks;        // blocks of the vertex / setup stages

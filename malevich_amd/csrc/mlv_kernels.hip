@@ -595,7 +595,7 @@ __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
     const uint32_t b = blockIdx.x;
     if (b < p.n_vblocks) {
         const uint32_t di = kTable ? p.vblk_draw[b]
-                                   : upper_index(p.n_draws, b, [&](uint32_t i) { return p.draws[i].blk_v0; });
+                                   : upper_index(p.n_draws, b, [&](uint32_t i) { return p.draw_v0[i]; });
         const DrawRec &d = p.draws[di];
         const uint32_t i = (b - d.blk_v0) * 256u + threadIdx.x;
         const int vs = d.vs;
@@ -706,7 +706,7 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
         inb[h] = b < p.n_tblocks;  // the grid's last workgroup may hold one block
         const uint32_t bc = inb[h] ? b : p.n_tblocks - 1u;
         di[h] = kTable ? p.tblk_draw[bc]
-                       : upper_index(p.n_draws, bc, [&](uint32_t i) { return p.draws[i].blk_t0; });
+                       : upper_index(p.n_draws, bc, [&](uint32_t i) { return p.draw_t0[i]; });
         const DrawRec &d = p.draws[di[h]];
         const uint32_t t = (bc - d.blk_t0) * 256u + threadIdx.x;
         slot[h] = bc * 256u + threadIdx.x;

@@ -474,7 +474,7 @@ static int flush(mlv_context *ctx) {
     const size_t scan_n =
         std::max<size_t>({(size_t)n_tblocks, (size_t)tri_cap, (size_t)total_tiles, radix_hist_elems(pair_cap)});
 
-    RC_OR_RETURN(ensure(ctx, ctx->draws_dev, sizeof(DrawRec) * n_draws));
+    RC_OR_RETURN(ensure(ctx, ctx->draws_dev, (sizeof(DrawRec) + 8) * n_draws));  // + the first blocks, dense
     RC_OR_RETURN(ensure(ctx, ctx->views_dev, sizeof(ViewRec) * n_views));
     RC_OR_RETURN(ensure(ctx, ctx->vpos, std::max<size_t>(Vs, 1) * 16));
     RC_OR_RETURN(ensure(ctx, ctx->vattr, (size_t)vattr_rows * 32));
@@ -512,12 +512,20 @@ static int flush(mlv_context *ctx) {
     }
 
     // tables through pinned staging
-    const size_t bytes_d = sizeof(DrawRec) * n_draws, bytes_v = sizeof(ViewRec) * n_views;
+    // the draw records, then each draw's first vertex-stage and setup block as two dense arrays
+    // (the blocks' binary searches read those: a few cache lines instead of one per probe)
+    const size_t bytes_r = sizeof(DrawRec) * n_draws, bytes_d = bytes_r + 8u * n_draws;
+    const size_t bytes_v = sizeof(ViewRec) * n_views;
     void *stage = nullptr;
     int slot = 0;
     RC_OR_RETURN(staging_acquire(ctx, bytes_d + bytes_v, &stage, &slot));
     DrawRec *stage_d = (DrawRec *)stage;
-    for (uint32_t i = 0; i < n_draws; ++i) stage_d[i] = draws[i].rec;
+    uint32_t *stage_b = (uint32_t *)((uint8_t *)stage + bytes_r);
+    for (uint32_t i = 0; i < n_draws; ++i) {
+        stage_d[i] = draws[i].rec;
+        stage_b[i] = draws[i].rec.blk_v0;
+        stage_b[n_draws + i] = draws[i].rec.blk_t0;
+    }
     memcpy((uint8_t *)stage + bytes_d, views.data(), bytes_v);
     HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->draws_dev.p, stage, bytes_d, hipMemcpyHostToDevice, ctx->stream));
     HIP_OR_FAIL(ctx, hipMemcpyAsync(ctx->views_dev.p, (uint8_t *)stage + bytes_d, bytes_v, hipMemcpyHostToDevice,
@@ -526,6 +534,8 @@ static int flush(mlv_context *ctx) {
 
     BatchParams p{};
     p.draws = (const DrawRec *)ctx->draws_dev.p;
+    p.draw_v0 = (const uint32_t *)((const uint8_t *)ctx->draws_dev.p + bytes_r);
+    p.draw_t0 = p.draw_v0 + n_draws;
     p.views = (const ViewRec *)ctx->views_dev.p;
     p.n_draws = n_draws;
     p.n_views = n_views;
