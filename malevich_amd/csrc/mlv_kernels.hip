@@ -627,10 +627,14 @@ __global__ __launch_bounds__(256) void kb_vertex(BatchParams p) {
 // ------------------------------------------------------------ stage 1: setup
 
 
-// Two 256-slot blocks per workgroup: each lane takes one triangle of each, and both triangles'
-// loads are issued before either is set up, so the two dependent round trips (indices, then
-// positions) are paid once for two triangles.
-constexpr uint32_t kCountPer = 2;
+// kCountPer 256-slot blocks per workgroup: each lane takes one triangle of each, and all the
+// triangles' loads are issued before any is set up, so the two dependent round trips (indices,
+// then positions) are paid once for all of them. Two on small flushes; four on flushes of many
+// draws (kBlockDrawTable or more), which have millions of triangles: there the count pass is
+// bound by its arithmetic, and fewer, longer workgroups run it 6 % faster (the 1080p frame's
+// count pass is 3 % slower that way).
+template <bool kBig>
+constexpr uint32_t kCountPer = kBig ? 4u : 2u;
 
 // The bins of the workgroup's assembled triangles that are not binned (rect of triangle h of this
 // lane in draw d[h] when unb[h]): counted for the stats and marked in the (draw, bin) bitmap. The
@@ -643,14 +647,14 @@ constexpr uint32_t kCountPer = 2;
 constexpr uint32_t kMarkWindow = 256;
 static_assert(kMarkWindow == (uint32_t)kBlock, "one window word per thread of a setup workgroup");
 
-__device__ __forceinline__ void mark_unbinned(const BatchParams &p, const uint32_t d[kCountPer],
-                                              const bool unb[kCountPer], const uint32_t r0[kCountPer],
-                                              const uint32_t r1[kCountPer]) {
+template <uint32_t N>
+__device__ __forceinline__ void mark_unbinned(const BatchParams &p, const uint32_t d[N], const bool unb[N],
+                                              const uint32_t r0[N], const uint32_t r1[N]) {
     __shared__ uint32_t s_win[kMarkWindow];
     __shared__ uint32_t s_lo[4], s_p8[4];
-    uint32_t lo_t = 0xFFFFFFFFu, nb_t = 0u, cbase[kCountPer], bins_x[kCountPer];
+    uint32_t lo_t = 0xFFFFFFFFu, nb_t = 0u, cbase[N], bins_x[N];
 #pragma unroll
-    for (uint32_t h = 0; h < kCountPer; ++h) {
+    for (uint32_t h = 0; h < N; ++h) {
         const DrawRec &D = p.draws[d[h]];
         cbase[h] = D.cbase;
         bins_x[h] = (uint32_t)p.views[D.view].bins_x;
@@ -674,7 +678,7 @@ __device__ __forceinline__ void mark_unbinned(const BatchParams &p, const uint32
     const uint32_t lo = min(min(s_lo[0], s_lo[1]), min(s_lo[2], s_lo[3]));
     if (lo == 0xFFFFFFFFu) return;  // nothing to mark in this workgroup (uniform)
 #pragma unroll
-    for (uint32_t h = 0; h < kCountPer; ++h) {
+    for (uint32_t h = 0; h < N; ++h) {
         if (!unb[h]) continue;
         int32_t bx0, by0, bx1, by1;
         rect_decode(r0[h], r1[h], bx0, by0, bx1, by1);
@@ -693,16 +697,19 @@ __device__ __forceinline__ void mark_unbinned(const BatchParams &p, const uint32
         atomicAdd(&p.stats->pairs8_part[blockIdx.x & 63u], (unsigned long long)(s_p8[0] + s_p8[1] + s_p8[2] + s_p8[3]));
 }
 
-template <bool kTable>  // the blocks' draws from BatchParams::tblk_draw, else by a binary search
+// kTable: the blocks' draws from BatchParams::tblk_draw (and four blocks a workgroup), else by a
+// binary search (and two)
+template <bool kTable>
 __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
-    __shared__ uint32_t s_w[kCountPer][4];
+    constexpr uint32_t kPer = kCountPer<kTable>;
+    __shared__ uint32_t s_w[kPer][4];
     const uint32_t hb = xcd_contiguous(blockIdx.x, gridDim.x);
-    uint32_t di[kCountPer], slot[kCountPer], vi[kCountPer][3];
-    bool inb[kCountPer], live[kCountPer], ok[kCountPer];
+    uint32_t di[kPer], slot[kPer], vi[kPer][3];
+    bool inb[kPer], live[kPer], ok[kPer];
     // every load comes before the first atomic or store (see kb_vertex)
 #pragma unroll
-    for (uint32_t h = 0; h < kCountPer; ++h) {
-        const uint32_t b = hb * kCountPer + h;
+    for (uint32_t h = 0; h < kPer; ++h) {
+        const uint32_t b = hb * kPer + h;
         inb[h] = b < p.n_tblocks;  // the grid's last workgroup may hold one block
         const uint32_t bc = inb[h] ? b : p.n_tblocks - 1u;
         di[h] = kTable ? p.tblk_draw[bc]
@@ -720,7 +727,7 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
         vi[h][2] = load_global(ib + 2);
     }
 #pragma unroll
-    for (uint32_t h = 0; h < kCountPer; ++h) {
+    for (uint32_t h = 0; h < kPer; ++h) {
         // the range check of load_tri_indices
         const DrawRec &d = p.draws[di[h]];
         const long long bv = (long long)d.base_vertex, nv = (long long)d.vertex_count;
@@ -733,20 +740,20 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
         }
         ok[h] = live[h] && in_range;
     }
-    float4 q[kCountPer][3];
+    float4 q[kPer][3];
 #pragma unroll
-    for (uint32_t h = 0; h < kCountPer; ++h) {
+    for (uint32_t h = 0; h < kPer; ++h) {
         // from every lane as well (slot 0 where the triangle is not set up)
         const uint32_t vb = p.draws[di[h]].vbase;
 #pragma unroll
         for (int k = 0; k < 3; ++k) q[h][k] = p.vpos[ok[h] ? vb + vi[h][k] : 0u];
     }
-    uint32_t packed[kCountPer];  // binned | assembled << 16
-    int cls[kCountPer];
-    bool unb[kCountPer];         // assembled without the clipper and not binned: its bins are marked here
-    uint32_t r0[kCountPer], r1[kCountPer];
+    uint32_t packed[kPer];  // binned | assembled << 16
+    int cls[kPer];
+    bool unb[kPer];         // assembled without the clipper and not binned: its bins are marked here
+    uint32_t r0[kPer], r1[kPer];
 #pragma unroll
-    for (uint32_t h = 0; h < kCountPer; ++h) {
+    for (uint32_t h = 0; h < kPer; ++h) {
         packed[h] = 0u;
         cls[h] = 0;
         unb[h] = false;
@@ -767,7 +774,7 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
         }
     }
 #pragma unroll
-    for (uint32_t h = 0; h < kCountPer; ++h) {
+    for (uint32_t h = 0; h < kPer; ++h) {
         if (live[h] && !ok[h]) atomicAdd(&p.stats->oob, 1ull);
         if (cls[h] == 2) {
             uint32_t k = atomicAdd(&p.ctr->n_clip_list, 1u);
@@ -778,14 +785,14 @@ __global__ __launch_bounds__(256) void kb_setup_count(BatchParams p) {
         if ((threadIdx.x & 63u) == 0u) s_w[h][threadIdx.x >> 6] = sum;
     }
     __syncthreads();
-    if (threadIdx.x < kCountPer && hb * kCountPer + threadIdx.x < p.n_tblocks) {
+    if (threadIdx.x < kPer && hb * kPer + threadIdx.x < p.n_tblocks) {
         const uint32_t sum = s_w[threadIdx.x][0] + s_w[threadIdx.x][1] + s_w[threadIdx.x][2] + s_w[threadIdx.x][3];
-        p.blocksum[hb * kCountPer + threadIdx.x] =
+        p.blocksum[hb * kPer + threadIdx.x] =
             (unsigned long long)(sum & 0xFFFFu) | ((unsigned long long)(sum >> 16) << 32);
-        const uint32_t bk = hb * kCountPer + threadIdx.x, dk = di[threadIdx.x];
+        const uint32_t bk = hb * kPer + threadIdx.x, dk = di[threadIdx.x];
         p.blk_draw[bk] = dk | (p.draws[dk].blk_t0 == bk ? kFirstOfDraw : 0u);
     }
-    if (p.cmark) mark_unbinned(p, di, unb, r0, r1);
+    if (p.cmark) mark_unbinned<kPer>(p, di, unb, r0, r1);
 }
 
 // Triangles that need the clipper, one per 16-lane group: the polygon's fan triangles that
@@ -2280,11 +2287,11 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
     case 1:
         if (!p.n_tblocks) break;
         if (p.tblk_draw)
-            hipLaunchKernelGGL(kb_setup_count<true>, dim3((p.n_tblocks + kCountPer - 1u) / kCountPer), dim3(256),
-                               p.setup_lds_pad, s, p);
+            hipLaunchKernelGGL(kb_setup_count<true>, dim3((p.n_tblocks + kCountPer<true> - 1u) / kCountPer<true>),
+                               dim3(256), p.setup_lds_pad, s, p);
         else
-            hipLaunchKernelGGL(kb_setup_count<false>, dim3((p.n_tblocks + kCountPer - 1u) / kCountPer), dim3(256),
-                               p.setup_lds_pad, s, p);
+            hipLaunchKernelGGL(kb_setup_count<false>, dim3((p.n_tblocks + kCountPer<false> - 1u) / kCountPer<false>),
+                               dim3(256), p.setup_lds_pad, s, p);
         hipLaunchKernelGGL(kb_clip_count, dim3(1024), dim3(256), 0, s, p);
         if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc,
                                                  &p.ctr->scan_total, s)) != hipSuccess)
