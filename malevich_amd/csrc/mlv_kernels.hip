@@ -944,6 +944,121 @@ __global__ __launch_bounds__(256) void kb_setup_emit(BatchParams p) {
     write_triangle(p, off_bin, di, v, tv, rw, d.abase + vi[0], d.abase + vi[1], d.abase + vi[2], tt, num);
 }
 
+// kb_setup_emit for flushes of many draws (kBlockDrawTable or more: millions of small triangles,
+// few of them binned): one wave per 256-slot block, four independent waves a workgroup and no
+// workgroup barrier: lane
+// l holds slots j * 64 + l (j < 4), the block's scan is four wave scans, and the binned triangles
+// are packed into rounds of 64 through a per-wave LDS list. The pass is bound by each block's
+// chain of dependent loads (its draw, the draw's view and first offset, then indices and
+// corners), so what pays is many blocks in flight: a wave ends as soon as its block is done
+// (batched views 663 -> 579 us per flush). Where many of a block's triangles are binned (the
+// 1080p frame) one wave setting them all up in turn is slower than kb_setup_emit's four.
+constexpr uint32_t kEmitWaves = 4;
+
+__global__ __launch_bounds__(256) void kb_setup_emit_waves(BatchParams p) {
+    __shared__ uint8_t s_list[kEmitWaves][256];
+    __shared__ unsigned long long s_ex[kEmitWaves][256];
+    const uint32_t lane = threadIdx.x & 63u, wv = (uint32_t)__builtin_amdgcn_readfirstlane((int)(threadIdx.x >> 6));
+    const uint32_t b = xcd_contiguous(blockIdx.x, gridDim.x) * kEmitWaves + wv;
+    if (b >= p.n_tblocks) return;
+    const uint32_t bd = p.blk_draw[b];  // the count pass found the block's draw
+    const uint32_t blk_binned = (uint32_t)p.blocksum[b];  // binned triangles of the block (clipped ones too)
+    const bool last = b == p.n_tblocks - 1u;
+    // a block without binned triangles writes nothing: no records, no clipper rows (a clipped
+    // polygon takes rows only when one of its fan triangles is binned) and no clipper offsets
+    // (kb_clip_emit reads them only for binned fan triangles); only a draw's first block and the
+    // last block have per-draw and per-flush values to write
+    if (blk_binned == 0u && !(bd & kFirstOfDraw) && !last) return;
+    const uint32_t di = bd & ~kFirstOfDraw;
+    const DrawRec &d = p.draws[di];
+    const ViewRec v = p.views[d.view];  // a copy: read here, before the first store
+    const unsigned long long bo = p.blockoff[b];                       // binned | assembled << 32
+    const uint32_t draw_asm0 = (uint32_t)(p.blockoff[d.blk_t0] >> 32);  // the draw's first assembled
+    uint32_t packed[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) packed[j] = p.cnt[b * 256u + j * 64u + lane];
+    if (lane == 0u) {
+        // a draw's first binned index; empty draws start where the next one does
+        if (b == d.blk_t0)
+            for (int32_t k = (int32_t)di; k >= 0 && p.draws[k].blk_t0 == b; --k) p.asm_base[k] = (uint32_t)bo;
+        if (last) {
+            // the flush's totals and capacity check, and the empty draws after the last triangle
+            const unsigned long long tot = p.ctr->scan_total;
+            const uint32_t n_binned = (uint32_t)(tot & 0xFFFFFFFFull), n_assembled = (uint32_t)(tot >> 32);
+            for (uint32_t k = di + 1u; k < p.n_draws; ++k) p.asm_base[k] = n_binned;
+            p.ctr->n_assembled = n_binned;
+            if (n_binned > p.tri_cap) {
+                atomicOr(&p.ctr->overflow, OVF_TRIANGLES);
+                atomicAdd(&p.stats->capacity_errors, 1ull);
+            }
+            atomicAdd(&p.stats->assembled, (unsigned long long)n_assembled);
+            atomicMax(&p.stats->max_clip_list, (unsigned long long)p.ctr->n_clip_list);
+        }
+    }
+    if (blk_binned == 0u) return;
+    // the block's exclusive scan in slot order (within one block the counts, <= 14 triangles and
+    // <= 16 rows a slot, cannot carry out of 16 bits)
+    unsigned long long ex[4], carry = 0ull;
+    bool emit[4], plain[4];
+    uint32_t n_rows[4], n_asm[4];
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        const uint32_t n_bin = packed[j] & 0xFFu;
+        n_asm[j] = (packed[j] >> 8) & 0xFFu;
+        plain[j] = n_asm[j] == 1u && (packed[j] >> 16) == 0u;  // assembled without the clipper
+        n_rows[j] = (!plain[j] && n_bin) ? packed[j] >> 16 : 0u;  // a clipped polygon's vertex rows
+        emit[j] = plain[j] && n_bin != 0u;  // only binned triangles are written (count marked the others' bins)
+        const unsigned long long x =
+            (unsigned long long)(n_bin | (n_asm[j] << 16)) | ((unsigned long long)n_rows[j] << 32);
+        const unsigned long long inc = wave_inclusive_scan(x);
+        ex[j] = carry + inc - x;
+        carry += (unsigned long long)__builtin_amdgcn_readlane((int)(uint32_t)inc, 63) |
+                 ((unsigned long long)(uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)(inc >> 32), 63) << 32);
+    }
+    if (carry >> 32) {  // the block's clipped polygons take their vertex rows with one atomic
+        uint32_t rows0 = 0u;
+        if (lane == 0u) rows0 = atomicAdd(&p.ctr->n_clipverts, (uint32_t)(carry >> 32));
+        rows0 = (uint32_t)__builtin_amdgcn_readfirstlane((int)rows0);
+#pragma unroll
+        for (uint32_t j = 0; j < 4u; ++j)
+            if (n_rows[j]) p.clip_rows[b * 256u + j * 64u + lane] = rows0 + (uint32_t)(ex[j] >> 32);
+    }
+    // clipped polygons' offsets for kb_clip_emit; the binned triangles packed into s_list
+    uint32_t n_emit = 0u;
+#pragma unroll
+    for (uint32_t j = 0; j < 4u; ++j) {
+        const uint32_t sl = j * 64u + lane;
+        if (!plain[j] && n_asm[j])
+            p.clip_off[b * 256u + sl] = make_uint2((uint32_t)(bo & 0xFFFFFFFFull) + (uint32_t)(ex[j] & 0xFFFFu),
+                                                   (uint32_t)(bo >> 32) + (uint32_t)((ex[j] >> 16) & 0xFFFFu) -
+                                                       draw_asm0);
+        const uint64_t bal = __ballot(emit[j]);
+        if (emit[j]) {
+            const uint32_t r = n_emit + mbcnt(bal);
+            s_list[wv][r] = (uint8_t)sl;
+            s_ex[wv][r] = ex[j];
+        }
+        n_emit += (uint32_t)__popcll(bal);
+    }
+    __builtin_amdgcn_wave_barrier();  // the wave's own LDS writes come before its reads
+    for (uint32_t r0 = 0; r0 < n_emit; r0 += 64u) {
+        const uint32_t i = r0 + lane;
+        if (i >= n_emit) break;
+        const uint32_t tt = b * 256u + s_list[wv][i] - d.tbase;  // the triangle within its draw
+        const unsigned long long ext = s_ex[wv][i];
+        uint32_t vi[3];
+        load_tri_indices(d, tt, vi);
+        TriVis tv;
+        float rw[3];
+        setup_triangle<true>(d, v, p.vpos[d.vbase + vi[0]], p.vpos[d.vbase + vi[1]], p.vpos[d.vbase + vi[2]], tv,
+                             rw);
+        const uint32_t off_bin = (uint32_t)(bo & 0xFFFFFFFFull) + (uint32_t)(ext & 0xFFFFu);
+        const uint32_t num = (uint32_t)(bo >> 32) + (uint32_t)((ext >> 16) & 0xFFFFu) - draw_asm0;
+        // the attribute rows the pixel shader reads (DrawRec::abase)
+        write_triangle(p, off_bin, di, v, tv, rw, d.abase + vi[0], d.abase + vi[1], d.abase + vi[2], tt, num);
+    }
+}
+
 // The clipped triangles' output, one per 16-lane group: lane k writes polygon vertex k's attribute
 // row (when any fan triangle is binned: the rows kb_setup_emit gave it), and lane ci the fan
 // triangle (0, ci, ci + 1) if it survives setup, at the offsets kb_setup_emit scanned.
@@ -2265,8 +2380,10 @@ uint32_t tile_kernel_lds(uint32_t tile_stage, uint32_t shift) {
 uint32_t setup_kernel_lds() {
     hipFuncAttributes a{}, b{};
     if (hipFuncGetAttributes(&a, (const void *)kb_setup_count<false>) != hipSuccess) return 0u;
+    hipFuncAttributes c{};
     if (hipFuncGetAttributes(&b, (const void *)kb_setup_emit) != hipSuccess) return 0u;
-    return (uint32_t)std::max(a.sharedSizeBytes, b.sharedSizeBytes);
+    if (hipFuncGetAttributes(&c, (const void *)kb_setup_emit_waves) != hipSuccess) return 0u;
+    return (uint32_t)std::max({a.sharedSizeBytes, b.sharedSizeBytes, c.sharedSizeBytes});
 }
 
 hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, uint32_t radix_passes,
@@ -2296,7 +2413,11 @@ hipError_t launch_stage(int stage, const BatchParams &p, const ScanScratch &sc, 
         if ((e = launch_scan<unsigned long long>(p.blocksum, p.blockoff, p.n_tblocks, nullptr, sc,
                                                  &p.ctr->scan_total, s)) != hipSuccess)
             return e;
-        hipLaunchKernelGGL(kb_setup_emit, dim3(p.n_tblocks), dim3(256), p.setup_lds_pad, s, p);
+        if (p.tblk_draw)
+            hipLaunchKernelGGL(kb_setup_emit_waves, dim3((p.n_tblocks + kEmitWaves - 1u) / kEmitWaves), dim3(256),
+                               p.setup_lds_pad, s, p);
+        else
+            hipLaunchKernelGGL(kb_setup_emit, dim3(p.n_tblocks), dim3(256), p.setup_lds_pad, s, p);
         hipLaunchKernelGGL(kb_clip_emit, dim3(1024), dim3(256), 0, s, p);
         break;
     case 2: {
