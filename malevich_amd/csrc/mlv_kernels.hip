@@ -542,7 +542,10 @@ __device__ __forceinline__ void run_vs(const BatchParams &p, const VsDraw &u, ui
     float in[8];
     const uint8_t *src = u.vb + (size_t)i * u.stride;
     if ((((uintptr_t)u.vb) & 15u) == 0u && (u.stride & 15u) == 0u) {
-        float4 a = load_global((const float4 *)src), b = load_global((const float4 *)src + 1);
+        // a draw that writes positions only reads the first 16 bytes of each vertex (u.attrs is
+        // uniform over the draw)
+        const float4 a = load_global((const float4 *)src);
+        const float4 b = u.attrs ? load_global((const float4 *)src + 1) : make_float4(0.f, 0.f, 0.f, 0.f);
         in[0] = a.x; in[1] = a.y; in[2] = a.z; in[3] = a.w; in[4] = b.x; in[5] = b.y; in[6] = b.z; in[7] = b.w;
     } else {
         for (int k2 = 0; k2 < 8; ++k2) in[k2] = load_global((const float *)src + k2);
